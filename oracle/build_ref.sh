@@ -1,0 +1,43 @@
+#!/bin/bash
+# Builds the reference Fortran program with flang into oracle/_ref/ (TEST INFRASTRUCTURE).
+#
+# Usage: oracle/build_ref.sh <lord> <strat>
+#   lord  : GCR order written into `parameter (lordx=..,lord=..)` (reference default 9)
+#   strat : stratification N^2/g written over the hard-coded `strat=1.e-05`
+#
+# The source is read where it lies (REF_SRC, default under /root/reference) and a patched
+# copy is written to oracle/_ref/ only (git-ignored, never committed). Patches:
+#   1. program nftincv10 -> subroutine nftmain, so ref_harness.f90 can be the main program
+#   2. neighbour lists widened from 10 to 32 entries (extruded tet meshes have up to ~22
+#      neighbours; 10 overflows the array)
+#   3. GCR order `lord`
+#   4. stratification constant
+#   5. restart-file formats f14.6 -> es26.17e3 so the final state is written at full
+#      precision
+# The build is FP64 (-fdefault-real-8 -fdefault-double-8), as the reference's REAL*4
+# default would otherwise apply.
+set -euo pipefail
+HERE=$(cd "$(dirname "$0")" && pwd)
+REF_SRC=${REF_SRC:-/root/reference/lippsLFroude_implgwopks3D_unstr3.f}
+LORD=${1:-3}
+STRAT=${2:-1.e-05}
+FC=${FC:-flang}
+OUT="$HERE/_ref"
+TAG="l${LORD}_s${STRAT}"
+mkdir -p "$OUT"
+if [ ! -f "$REF_SRC" ]; then
+  echo "reference source not found: $REF_SRC" >&2
+  exit 2
+fi
+sed -e '1s/^      program nftincv10/      subroutine nftmain/' \
+    -e 's/(npoinx,10)/(npoinx,32)/g' \
+    -e "s/parameter (lordx=9,nn=npoinx,lord=9)/parameter (lordx=${LORD},nn=npoinx,lord=${LORD})/" \
+    -e "s/^      strat=1.e-05/      strat=${STRAT}/" \
+    -e 's/format(3f14.6)/format(3es26.17e3)/' \
+    -e 's/format(4f14.6)/format(4es26.17e3)/' \
+    "$REF_SRC" > "$OUT/ref_${TAG}.f"
+FFLAGS="-O3 -fdefault-real-8 -fdefault-double-8"
+$FC $FFLAGS -c "$OUT/ref_${TAG}.f" -o "$OUT/ref_${TAG}.o"
+$FC $FFLAGS -c "$HERE/ref_harness.f90" -o "$OUT/ref_harness.o" -J "$OUT"
+$FC $FFLAGS "$OUT/ref_harness.o" "$OUT/ref_${TAG}.o" -o "$OUT/nft_ref_${TAG}"
+echo "$OUT/nft_ref_${TAG}"

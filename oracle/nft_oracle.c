@@ -1,0 +1,739 @@
+/* CPU oracle for the edge-based NFT step: TEST INFRASTRUCTURE ONLY (see nft_oracle.h).
+ *
+ * Every function below restates one reference subroutine of
+ * lippsLFroude_implgwopks3D_unstr3.f ("F:n" = line n of that file) in its own loop order
+ * and operation order. Compile with -ffp-contract=off so products and sums round as
+ * the reference's scalar code does.
+ */
+#include "nft_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static const double G = 9.80616; /* F:94 */
+
+struct oc_ctx {
+  oc_mesh m;
+  oc_params p;
+  int n;
+  /* profiles and setup fields */
+  double *rh0, *th0, *the, *ue, *ve, *coefw, *alpha, *d1, *dthedz;
+  double beta1;
+  /* line preconditioner factors */
+  double *la, *linv, *lc;
+  /* scratch */
+  double *s[16];
+  /* gcrk work */
+  double *gx, *gax, *gr, *gqr, *gar;
+  /* nft state */
+  double *unk, *frc, *uuu, *unkdash, *prpot, *unk1, *uuu0;
+  oc_stats st;
+};
+
+static double amax1(double a, double b) { return a > b ? a : b; }
+static double amin1(double a, double b) { return a < b ? a : b; }
+
+static double *zalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
+
+static void build_line(oc_ctx *c);
+
+oc_ctx *oc_create(const oc_mesh *m, const oc_params *p) {
+  oc_ctx *c = (oc_ctx *)calloc(1, sizeof(oc_ctx));
+  c->m = *m;
+  c->p = *p;
+  int n = m->npoin;
+  c->n = n;
+  c->rh0 = zalloc(n); c->th0 = zalloc(n); c->the = zalloc(n); c->ue = zalloc(n);
+  c->ve = zalloc(n); c->coefw = zalloc(n); c->alpha = zalloc(n); c->d1 = zalloc(n);
+  c->dthedz = zalloc(n);
+  c->la = zalloc(n); c->linv = zalloc(n); c->lc = zalloc(n);
+  for (int i = 0; i < 16; ++i) c->s[i] = zalloc(n);
+  int lord = p->lord > 0 ? p->lord : 1;
+  c->gx = zalloc((size_t)lord * n); c->gax = zalloc((size_t)lord * n);
+  c->gr = zalloc(n); c->gqr = zalloc(n); c->gar = zalloc(n);
+  c->unk = zalloc(4 * (size_t)n); c->frc = zalloc(4 * (size_t)n); c->uuu = zalloc(3 * (size_t)n);
+  c->unkdash = zalloc(3 * (size_t)n); c->prpot = zalloc(n); c->unk1 = zalloc(4 * (size_t)n);
+  c->uuu0 = zalloc(3 * (size_t)n);
+
+  /* rhprof/thprof with lipps=0 (F:4549-4579, F:4532-4547); metric factors are 1 in the
+   * Cartesian case (F:149-157) */
+  for (int i = 0; i < n; ++i) {
+    c->rh0[i] = 1.0;
+    c->th0[i] = p->theta0;
+    c->ue[i] = p->u0;
+    c->ve[i] = p->v0;
+    /* F:187-190 */
+    c->the[i] = p->theta0 * (1. + p->strat * m->coord[2 * (size_t)n + i]);
+    c->dthedz[i] = p->theta0 * p->strat * 1.0;
+  }
+  /* absorbers (F:2915-3026) */
+  {
+    double *alpha = c->alpha;
+    const double *X = m->coord, *Y = m->coord + n, *Z = m->coord + 2 * (size_t)n;
+    double eps = 1.e-07;
+    if (p->irl == 1) {
+      double abxmax = -1.e+07, abxmin = 1.e+07, abymax = -1.e+07, abymin = 1.e+07;
+      double abzmax = -1.e+07, abzmin = 1.e+07;
+      for (int i = 0; i < n; ++i) {
+        abxmax = amax1(X[i], abxmax); abxmin = amin1(X[i], abxmin);
+        abymax = amax1(Y[i], abymax); abymin = amin1(Y[i], abymin);
+        abzmax = amax1(Z[i], abzmax); abzmin = amin1(Z[i], abzmin);
+      }
+      double r = 1.0;
+      double xabr = abxmax * r - p->dxabr, xabl = abxmin * r + p->dxabl;
+      double yabu = abymax * r - p->dyabu, yabd = abymin * r + p->dyabd;
+      double zabu = abzmax - p->dzabu, zabd = abzmin + p->dzabd;
+      double tolil = p->towxl != 0. ? 1. / p->towxl : 0.;
+      double tolir = p->towxr != 0. ? 1. / p->towxr : 0.;
+      double toliu = p->towyu != 0. ? 1. / p->towyu : 0.;
+      double tolid = p->towyd != 0. ? 1. / p->towyd : 0.;
+      double toliuz = p->towzu != 0. ? 1. / p->towzu : 0.;
+      double tolidz = p->towzd != 0. ? 1. / p->towzd : 0.;
+      for (int i = 0; i < n; ++i) {
+        double relx, rely, relz, r0, r1, r2, relb;
+        r0 = X[i] * r; r1 = 0.; r2 = 0.; relb = 0.;
+        if (p->dxabl != 0.) r1 = amax1(0., xabl - r0) / p->dxabl;
+        if (p->dxabr != 0.) r2 = amax1(0., r0 - xabr) / p->dxabr;
+        if ((tolil + tolir) != 0.)
+          relb = ((tolil * r1) * (tolil * r1) + (tolir * r2) * (tolir * r2)) / (tolil * r1 + tolir * r2 + eps);
+        relx = relb;
+        r1 = 0.; r2 = 0.; relb = 0.; r0 = Y[i] * r;
+        if (p->dyabd != 0.) r1 = amax1(0., -r0 + yabd) / p->dyabd;
+        if (p->dyabu != 0.) r2 = amax1(0., r0 - yabu) / p->dyabu;
+        if ((tolid + toliu) != 0.)
+          relb = ((tolid * r1) * (tolid * r1) + (toliu * r2) * (toliu * r2)) / (tolid * r1 + toliu * r2 + eps);
+        rely = relb;
+        r1 = 0.; r2 = 0.; relb = 0.; r0 = Z[i];
+        if (p->dzabd != 0.) r1 = amax1(0., -r0 + zabd) / p->dzabd;
+        if (p->dzabu != 0.) r2 = amax1(0., r0 - zabu) / p->dzabu;
+        if ((tolidz + toliuz) != 0.)
+          relb = ((tolidz * r1) * (tolidz * r1) + (toliuz * r2) * (toliuz * r2)) / (tolidz * r1 + toliuz * r2 + eps);
+        relz = relb;
+        double alpha1 = (rely * rely + relx * relx) / (rely + relx + eps);
+        alpha[i] = (alpha1 * alpha1 + relz * relz) / (alpha1 + relz + eps);
+      }
+    }
+  }
+  /* beta_timestep_unstr(1) (F:953-1109) */
+  {
+    double ddx = 10.e+06, hill = 1500.0;
+    const double *X = m->coord, *Y = m->coord + n, *Z = m->coord + 2 * (size_t)n;
+    const double *d = p->dom;
+    for (int e = 0; e < m->nedge; ++e) {
+      int i1 = m->iedge[e] - 1, i2 = m->iedge[m->nedge + e] - 1;
+      if (X[i1] != d[0] && X[i1] != d[1] && Y[i1] != d[2] && Y[i1] != d[3] && Z[i1] != d[4] &&
+          Z[i1] != d[5] && X[i2] != d[0] && X[i2] != d[1] && Y[i2] != d[2] && Y[i2] != d[3] &&
+          Z[i2] != d[4] && Z[i2] != d[5] && Z[i1] > hill && Z[i2] > hill) {
+        double hx = fabs(m->sn[e]), hy = fabs(m->sn[m->nedge + e]), hz = fabs(m->sn[2 * (size_t)m->nedge + e]);
+        double h = amax1(amax1(hx, hy), hz);
+        ddx = amin1(amin1(ddx, m->vol[i1] / h), m->vol[i2] / h);
+      }
+    }
+    c->beta1 = 0.5 * ddx * ddx * 1.0;
+  }
+  /* compute_d_unstr(1) (F:1112-1284) */
+  {
+    double *dix = c->s[0], *diy = c->s[1], *diz = c->s[2];
+    for (int i = 0; i < n; ++i) { dix[i] = 0.; diy[i] = 0.; diz[i] = 0.; }
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? m->nedge : m->nbface;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      const double *S = pass == 0 ? m->sn : m->sb;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+        double sx = S[e], sy = S[ne + e], sz = S[2 * (size_t)ne + e];
+        dix[i1] = dix[i1] + sx * sx * c->rh0[i2] / m->vol[i2];
+        dix[i2] = dix[i2] + sx * sx * c->rh0[i1] / m->vol[i1];
+        diy[i1] = diy[i1] + sy * sy * c->rh0[i2] / m->vol[i2];
+        diy[i2] = diy[i2] + sy * sy * c->rh0[i1] / m->vol[i1];
+        diz[i1] = diz[i1] + sz * sz * c->rh0[i2] / m->vol[i2];
+        diz[i2] = diz[i2] + sz * sz * c->rh0[i1] / m->vol[i1];
+      }
+    }
+    for (int i = 0; i < n; ++i)
+      c->d1[i] = -(0.5 * p->dt * (dix[i] + diy[i] + diz[i]) / (4 * m->vol[i] * c->rh0[i]));
+  }
+  /* coefw (F:250-252); set here so laplc/precon can be called stand-alone */
+  for (int i = 0; i < n; ++i) {
+    double gth0 = G / c->th0[i];
+    double cw = 1. + 0.25 * (p->dt * p->dt) * gth0 * c->dthedz[i];
+    c->coefw[i] = 1. / cw;
+  }
+  build_line(c);
+  return c;
+}
+
+void oc_destroy(oc_ctx *c) {
+  if (!c) return;
+  double *all[] = {c->rh0, c->th0, c->the, c->ue, c->ve, c->coefw, c->alpha, c->d1, c->dthedz,
+                   c->la, c->linv, c->lc, c->gx, c->gax, c->gr, c->gqr, c->gar, c->unk, c->frc,
+                   c->uuu, c->unkdash, c->prpot, c->unk1, c->uuu0};
+  for (size_t i = 0; i < sizeof(all) / sizeof(all[0]); ++i) free(all[i]);
+  for (int i = 0; i < 16; ++i) free(c->s[i]);
+  free(c);
+}
+
+double *oc_field(oc_ctx *c, const char *name) {
+  if (!strcmp(name, "rh0")) return c->rh0;
+  if (!strcmp(name, "th0")) return c->th0;
+  if (!strcmp(name, "the")) return c->the;
+  if (!strcmp(name, "ue")) return c->ue;
+  if (!strcmp(name, "ve")) return c->ve;
+  if (!strcmp(name, "coefw")) return c->coefw;
+  if (!strcmp(name, "alpha")) return c->alpha;
+  if (!strcmp(name, "d1")) return c->d1;
+  if (!strcmp(name, "dthedz")) return c->dthedz;
+  if (!strcmp(name, "line_a")) return c->la;
+  if (!strcmp(name, "line_inv")) return c->linv;
+  if (!strcmp(name, "line_c")) return c->lc;
+  return NULL;
+}
+
+double oc_beta(oc_ctx *c) { return c->beta1; }
+void oc_get_stats(oc_ctx *c, oc_stats *s) { *s = c->st; }
+
+/* derivative(unk,dpdxa,idx), F:3943-4022, Cartesian branch */
+void oc_derivative(oc_ctx *c, const double *unk, double *dpdxa, int idx) {
+  const oc_mesh *m = &c->m;
+  int n = c->n, d = idx - 1;
+  for (int i = 0; i < n; ++i) dpdxa[i] = 0.;
+  for (int e = 0; e < m->nedge; ++e) {
+    int i1 = m->iedge[e] - 1, i2 = m->iedge[m->nedge + e] - 1;
+    double sx = m->sn[(size_t)d * m->nedge + e];
+    double u = (unk[i1] + unk[i2]) * 0.5;
+    dpdxa[i1] = dpdxa[i1] + sx * u;
+    dpdxa[i2] = dpdxa[i2] - sx * u;
+  }
+  for (int e = 0; e < m->nbface; ++e) {
+    int i1 = m->kbface[e] - 1, i2 = m->kbface[m->nbface + e] - 1;
+    double sx = m->sb[(size_t)d * m->nbface + e];
+    double u = (unk[i1] + unk[i2]) * 0.5;
+    dpdxa[i1] = dpdxa[i1] + sx * u;
+    dpdxa[i2] = dpdxa[i2] - sx * u;
+  }
+  for (int b = 0; b < m->nboun; ++b) {
+    int i1 = m->ibpoin[b] - 1;
+    double sx = m->sbb[(size_t)d * m->nboun + b];
+    double u = unk[i1];
+    dpdxa[i1] = dpdxa[i1] + sx * u;
+  }
+}
+
+/* bcv(uout,vout,wout,flag), F:1583-1643, Cartesian branch */
+void oc_bcv(oc_ctx *c, double *u, double *v, double *w, double flag) {
+  const oc_mesh *m = &c->m;
+  int nb = m->nboun;
+  for (int b = 0; b < nb; ++b) {
+    int ii = m->ibpoin[b] - 1, f = m->ibpoin[nb + b];
+    double ax = m->anp[b], ay = m->anp[nb + b], az = m->anp[2 * (size_t)nb + b];
+    if (f == 11 || f == 5 || f == 6) {
+      double vn = u[ii] * ax + v[ii] * ay + w[ii] * az;
+      u[ii] = u[ii] - ax * vn;
+      v[ii] = v[ii] - ay * vn;
+      w[ii] = w[ii] - az * vn;
+    }
+    if (f == 7 || f == 8) u[ii] = c->ue[ii] * flag;
+  }
+}
+
+/* laplc(press,alap,impfl), F:1288-1353 */
+void oc_laplc(oc_ctx *c, const double *press, double *alap, int impfl) {
+  int n = c->n;
+  double *gx = c->s[3], *gy = c->s[4], *gz = c->s[5];
+  double dt = c->p.dt;
+  for (int i = 0; i < n; ++i) alap[i] = 0.;
+  oc_derivative(c, press, gx, 1);
+  oc_derivative(c, press, gy, 2);
+  oc_derivative(c, press, gz, 3);
+  for (int i = 0; i < n; ++i) {
+    double v = c->m.vol[i];
+    double fz = impfl ? c->coefw[i] : 1.0; /* coefw*impfl+(1-impfl) */
+    gx[i] = dt * gx[i] / v;
+    gy[i] = dt * gy[i] / v;
+    gz[i] = dt * gz[i] / v;
+    gx[i] = 0.5 * gx[i];
+    gy[i] = 0.5 * gy[i];
+    gz[i] = 0.5 * gz[i] * fz;
+  }
+  oc_bcv(c, gx, gy, gz, 0.);
+  for (int i = 0; i < n; ++i) {
+    gx[i] = gx[i] * c->rh0[i];
+    gy[i] = gy[i] * c->rh0[i];
+    gz[i] = gz[i] * c->rh0[i];
+  }
+  double *ax = c->s[6], *ay = c->s[7], *az = c->s[8];
+  oc_derivative(c, gx, ax, 1);
+  oc_derivative(c, gy, ay, 2);
+  oc_derivative(c, gz, az, 3);
+  for (int i = 0; i < n; ++i) alap[i] = (ax[i] + ay[i] + az[i]) / (c->m.vol[i] * c->rh0[i]);
+}
+
+/* prfr(press,ustr,vstr,wstr,uout,vout,wout,impfl), F:1536-1580 */
+void oc_prfr(oc_ctx *c, const double *press, const double *ustr, const double *vstr,
+             const double *wstr, double *uout, double *vout, double *wout, int impfl) {
+  int n = c->n;
+  double *gx = c->s[9], *gy = c->s[10], *gz = c->s[11];
+  double dt = c->p.dt;
+  oc_derivative(c, press, gx, 1);
+  oc_derivative(c, press, gy, 2);
+  oc_derivative(c, press, gz, 3);
+  for (int i = 0; i < n; ++i) {
+    double v = c->m.vol[i];
+    gx[i] = gx[i] / v;
+    gy[i] = gy[i] / v;
+    gz[i] = gz[i] / v;
+  }
+  for (int i = 0; i < n; ++i) {
+    double fz = impfl ? c->coefw[i] : 1.0;
+    uout[i] = ustr[i] - 0.5 * dt * gx[i];
+    vout[i] = vstr[i] - 0.5 * dt * gy[i];
+    wout[i] = wstr[i] - 0.5 * dt * gz[i] * fz;
+  }
+  oc_bcv(c, uout, vout, wout, 1.);
+}
+
+/* div(ua,va,wa,rhs,iflg), F:2096-2146 (diagnostic print of iflg=1 omitted) */
+void oc_div(oc_ctx *c, const double *ua, const double *va, const double *wa, double *rhs, int iflg) {
+  int n = c->n;
+  double *ub = c->s[12], *vb = c->s[13], *wb = c->s[14];
+  for (int i = 0; i < n; ++i) {
+    if (iflg == -1) { ub[i] = ua[i] * c->rh0[i]; vb[i] = va[i] * c->rh0[i]; wb[i] = wa[i] * c->rh0[i]; }
+    else { ub[i] = ua[i]; vb[i] = va[i]; wb[i] = wa[i]; }
+  }
+  double *dx = c->s[3], *dy = c->s[4], *dz = c->s[5];
+  oc_derivative(c, ub, dx, 1);
+  oc_derivative(c, vb, dy, 2);
+  oc_derivative(c, wb, dz, 3);
+  double fl = (double)iflg;
+  for (int i = 0; i < n; ++i) {
+    double v = c->m.vol[i];
+    rhs[i] = fl * (dx[i] / v + dy[i] / v + dz[i] / v) / c->rh0[i];
+  }
+}
+
+/* ---- vertical line preconditioner (no reference counterpart) ----------------------
+ * Two-point-flux edge Laplacian: for edge (i,j) with dual facet S and d = x_j - x_i,
+ *   c_ij = 0.5*dt * 0.5*(rho_i+rho_j) * (Sx^2 + Sy^2 + cw*Sz^2) / max(S.d, 1e-3|S||d|)
+ *          / (vol_i * rho_i),   cw = 0.5*(coefw_i+coefw_j)
+ * M_ii = -sum_j c_ij over all edges of i; M_{i,i+-1} = c_ij over the vertical edges that
+ * join consecutive nodes of a column. Per column the tridiagonal M is factorised once
+ * (Thomas): a_k = M_{k,k-1}, cp_k = M_{k,k+1} * inv_k, inv_k = 1/(M_kk - a_k cp_{k-1}). */
+static void build_line(oc_ctx *c) {
+  const oc_mesh *m = &c->m;
+  int n = c->n;
+  double *diag = c->s[0], *lo = c->s[1], *up = c->s[2];
+  for (int i = 0; i < n; ++i) { diag[i] = 0.; lo[i] = 0.; up[i] = 0.; }
+  if (m->ncol <= 0) return;
+  const double *X = m->coord, *Y = m->coord + n, *Z = m->coord + 2 * (size_t)n;
+  double hdt = 0.5 * c->p.dt;
+  /* column position of every node: (col, level) */
+  int *colof = (int *)malloc(sizeof(int) * (size_t)n);
+  int *levof = (int *)malloc(sizeof(int) * (size_t)n);
+  for (int i = 0; i < n; ++i) { colof[i] = -1; levof[i] = -1; }
+  for (int k = 0; k < m->ncol; ++k)
+    for (int i = m->col_start[k]; i < m->col_start[k + 1]; ++i) { colof[i] = k; levof[i] = i - m->col_start[k]; }
+  for (int pass = 0; pass < 2; ++pass) {
+    int ne = pass == 0 ? m->nedge : m->nbface;
+    const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+    const double *S = pass == 0 ? m->sn : m->sb;
+    for (int e = 0; e < ne; ++e) {
+      int i = ie[e] - 1, j = ie[ne + e] - 1;
+      double sx = S[e], sy = S[ne + e], sz = S[2 * (size_t)ne + e];
+      double dx = X[j] - X[i], dy = Y[j] - Y[i], dz = Z[j] - Z[i];
+      double sd = sx * dx + sy * dy + sz * dz;
+      double sl = sqrt(sx * sx + sy * sy + sz * sz), dl = sqrt(dx * dx + dy * dy + dz * dz);
+      double den = sd > 1e-3 * sl * dl ? sd : 1e-3 * sl * dl;
+      double cw = 0.5 * (c->coefw[i] + c->coefw[j]);
+      double rb = 0.5 * (c->rh0[i] + c->rh0[j]);
+      double k0 = hdt * rb * (sx * sx + sy * sy + cw * sz * sz) / den;
+      double ci = k0 / (m->vol[i] * c->rh0[i]);
+      double cj = k0 / (m->vol[j] * c->rh0[j]);
+      diag[i] -= ci;
+      diag[j] -= cj;
+      if (colof[i] >= 0 && colof[i] == colof[j]) {
+        if (levof[j] == levof[i] + 1) { up[i] = ci; lo[j] = cj; }
+        else if (levof[i] == levof[j] + 1) { lo[i] = ci; up[j] = cj; }
+      }
+    }
+  }
+  for (int k = 0; k < m->ncol; ++k) {
+    int s0 = m->col_start[k], s1 = m->col_start[k + 1];
+    double cprev = 0.;
+    for (int i = s0; i < s1; ++i) {
+      double a = (i == s0) ? 0. : lo[i];
+      double den = diag[i] - a * cprev;
+      double inv = 1. / den;
+      double cp = (i == s1 - 1) ? 0. : up[i] * inv;
+      c->la[i] = a; c->linv[i] = inv; c->lc[i] = cp;
+      cprev = cp;
+    }
+  }
+  free(colof);
+  free(levof);
+}
+
+/* z = M^-1 rhs, column by column (forward then backward Thomas sweep) */
+void oc_line_solve(oc_ctx *c, const double *rhs, double *z) {
+  const oc_mesh *m = &c->m;
+  for (int k = 0; k < m->ncol; ++k) {
+    int s0 = m->col_start[k], s1 = m->col_start[k + 1];
+    double dprev = 0.;
+    for (int i = s0; i < s1; ++i) {
+      double d = (rhs[i] - c->la[i] * dprev) * c->linv[i];
+      z[i] = d;
+      dprev = d;
+    }
+    for (int i = s1 - 2; i >= s0; --i) z[i] = z[i] - c->lc[i] * z[i + 1];
+  }
+}
+
+/* precon(rhs,p,r,iprc,elas,impfl), F:482-612 fine-grid smoothing path (mg_flag=0).
+ * precon=2 replaces the point step pstep*(r-rhs) by the line solve M^-1 (rhs-r). */
+void oc_precon(oc_ctx *c, const double *rhs, double *p, double *r, int impfl) {
+  int n = c->n;
+  if (c->p.precon == 0) { for (int i = 0; i < n; ++i) p[i] = rhs[i]; return; }
+  for (int i = 0; i < n; ++i) { p[i] = 0.; r[i] = 0.; }
+  double om = c->p.omega;
+  if (c->p.precon == 1) {
+    double *pstep = c->s[15];
+    for (int i = 0; i < n; ++i) {
+      if (c->p.sm_flag == 1) pstep[i] = (-1.) / c->d1[i];
+      else if (c->p.sm_flag == 2) pstep[i] = c->beta1;
+      else pstep[i] = c->beta1 / (1 - c->d1[i] * c->beta1);
+    }
+    for (int it = 1; it <= c->p.itrp1; ++it) {
+      if (it > 1) oc_laplc(c, p, r, impfl);
+      for (int i = 0; i < n; ++i) p[i] = p[i] + om * pstep[i] * (r[i] - rhs[i]);
+    }
+  } else {
+    double *res = c->s[15], *z = c->s[2];
+    for (int it = 1; it <= c->p.itrp1; ++it) {
+      if (it > 1) oc_laplc(c, p, r, impfl);
+      for (int i = 0; i < n; ++i) res[i] = rhs[i] - r[i];
+      oc_line_solve(c, res, z);
+      for (int i = 0; i < n; ++i) p[i] = p[i] + om * z[i];
+    }
+  }
+}
+
+/* gcrk(p,u1,v1,w1,u2,v2,w2,impfl), F:1776-1972 (ielast=0).
+ * exit_mode 0: max|r| <= eps0/dt (reference, F:1805, F:1895);
+ * exit_mode 1: max|r| <= eps0 * max|r0| with r0 the residual on entry. */
+void oc_gcrk(oc_ctx *c, double *p, const double *u1, const double *v1, const double *w1,
+             double *u2, double *v2, double *w2, int impfl) {
+  int n = c->n, lord = c->p.lord;
+  double *r = c->gr, *qr = c->gqr, *ar = c->gar, *x = c->gx, *ax = c->gax;
+  double ax2[64], axar[64], del[64];
+  double epa = 1.e-38;
+  int nlc = 0, itmn = 3;
+  double eps = c->p.eps0 / c->p.dt;
+  for (int k = 0; k < n; ++k) { r[k] = 0.; ar[k] = 0.; qr[k] = 0.; }
+  for (size_t k = 0; k < (size_t)lord * n; ++k) { x[k] = 0.; ax[k] = 0.; }
+  oc_prfr(c, p, u1, v1, w1, u2, v2, w2, impfl);
+  oc_div(c, u2, v2, w2, r, -1);
+  oc_precon(c, r, qr, ar, impfl);
+  double eer0 = 0., eem0 = -1.e15, rl20 = 0., r0mx = 0.;
+  for (int k = 0; k < n; ++k) {
+    eer0 = eer0 + qr[k] * qr[k];
+    eem0 = amax1(eem0, fabs(qr[k]));
+    rl20 = rl20 + r[k] * r[k];
+    r0mx = amax1(r0mx, fabs(r[k]));
+  }
+  eer0 = amax1(eer0, epa);
+  eem0 = amax1(eem0, epa);
+  rl20 = amax1(rl20, epa);
+  if (c->p.exit_mode == 1) eps = c->p.eps0 * r0mx;
+  for (int k = 0; k < n; ++k) x[k] = qr[k];
+  oc_laplc(c, x, ax, impfl);
+  for (int it = 1; it <= c->p.itr; ++it) {
+    for (int i = 1; i <= lord; ++i) {
+      double *xi = x + (size_t)(i - 1) * n, *axi = ax + (size_t)(i - 1) * n;
+      double rax = 0.;
+      ax2[i - 1] = 0.;
+      for (int k = 0; k < n; ++k) {
+        rax = rax + r[k] * axi[k];
+        ax2[i - 1] = ax2[i - 1] + axi[k] * axi[k];
+      }
+      ax2[i - 1] = amax1(epa, ax2[i - 1]);
+      double beta = -rax / ax2[i - 1];
+      double dvmx = -1.e15, rl2 = 0.;
+      for (int k = 0; k < n; ++k) {
+        p[k] = p[k] + beta * xi[k];
+        r[k] = r[k] + beta * axi[k];
+        dvmx = amax1(dvmx, fabs(r[k]));
+        rl2 = rl2 + r[k] * r[k];
+      }
+      if (dvmx <= eps && it >= itmn) goto done;
+      if (rl2 >= rl20) goto done;
+      rl20 = amax1(rl2, epa);
+      oc_precon(c, r, qr, ar, impfl);
+      oc_laplc(c, qr, ar, impfl);
+      nlc = nlc + 1;
+      for (int ii = 1; ii <= i; ++ii) {
+        double *axii = ax + (size_t)(ii - 1) * n;
+        axar[ii - 1] = 0.;
+        for (int k = 0; k < n; ++k) axar[ii - 1] = axar[ii - 1] + axii[k] * ar[k];
+        del[ii - 1] = -axar[ii - 1] / ax2[ii - 1];
+      }
+      if (i < lord) {
+        double *xn = x + (size_t)i * n, *axn = ax + (size_t)i * n;
+        for (int k = 0; k < n; ++k) { xn[k] = qr[k]; axn[k] = ar[k]; }
+        for (int ii = 1; ii <= i; ++ii) {
+          double *xii = x + (size_t)(ii - 1) * n, *axii = ax + (size_t)(ii - 1) * n;
+          for (int k = 0; k < n; ++k) {
+            xn[k] = xn[k] + del[ii - 1] * xii[k];
+            axn[k] = axn[k] + del[ii - 1] * axii[k];
+          }
+        }
+      } else {
+        for (int k = 0; k < n; ++k) {
+          x[k] = qr[k] + del[0] * x[k];
+          ax[k] = ar[k] + del[0] * ax[k];
+        }
+        for (int ii = 2; ii <= i; ++ii) {
+          double *xii = x + (size_t)(ii - 1) * n, *axii = ax + (size_t)(ii - 1) * n;
+          for (int k = 0; k < n; ++k) {
+            x[k] = x[k] + del[ii - 1] * xii[k];
+            xii[k] = 0.;
+            ax[k] = ax[k] + del[ii - 1] * axii[k];
+            axii[k] = 0.;
+          }
+        }
+      }
+    }
+  }
+done:;
+  double eer = 0., eem = -1.e15;
+  for (int k = 0; k < n; ++k) { eer = eer + qr[k] * qr[k]; eem = amax1(eem, fabs(qr[k])); }
+  c->st.eer = eer / eer0;
+  c->st.eem = eem / eem0;
+  c->st.niter = nlc;
+  c->st.nitsm = c->st.nitsm + nlc;
+  c->st.icount = c->st.icount + 1;
+}
+
+/* mpdatam(uuu,unk,iflag), F:3029-3311: donor cell + one antidiffusive pass (iord=2)
+ * with the FCT limiter; bounds from the donor-cell solution only. */
+void oc_mpdatam(oc_ctx *c, const double *uuu, double *unk, int iflag) {
+  const oc_mesh *m = &c->m;
+  int n = c->n, E = m->nedge, B = m->nbface, Bp = m->nboun;
+  const double *U = uuu, *V = uuu + n, *W = uuu + 2 * (size_t)n;
+  double dt = c->p.dt;
+  double *aun = zalloc(E), *aunb = zalloc(B), *aunbb1 = zalloc(Bp);
+  double *divv = c->s[0], *arh2 = c->s[1], *gx = c->s[2], *gy = c->s[3], *gz = c->s[4];
+  double *mx = c->s[5], *mn = c->s[6], *rin = c->s[7], *rout = c->s[8], *cp = c->s[9], *cn = c->s[10];
+  double afree[5] = {c->p.u0, c->p.v0, c->p.w0, 0., 1.};
+  double tol = 1.e-10;
+  for (int i = 0; i < n; ++i) { divv[i] = 0.; mx[i] = -1.e15; mn[i] = 1.e15; }
+  for (int ipass = 1; ipass <= 2; ++ipass) {
+    for (int i = 0; i < n; ++i) arh2[i] = 0.;
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? E : B;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      const double *S = pass == 0 ? m->sn : m->sb;
+      double *a = pass == 0 ? aun : aunb;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+        double sx = S[e], sy = S[ne + e], sz = S[2 * (size_t)ne + e];
+        double fl;
+        if (ipass == 1) {
+          a[e] = (U[i1] + U[i2]) * 0.5 * sx + (V[i1] + V[i2]) * 0.5 * sy + (W[i1] + W[i2]) * 0.5 * sz;
+          fl = unk[i1] * amax1(0., a[e]) + unk[i2] * amin1(0., a[e]);
+        } else {
+          fl = a[e];
+        }
+        arh2[i1] = arh2[i1] + fl * (1. + 1.) * 0.5;
+        arh2[i2] = arh2[i2] - fl * (1. + 1.) * 0.5;
+        divv[i1] = divv[i1] + a[e];
+        divv[i2] = divv[i2] - a[e];
+      }
+    }
+    for (int b = 0; b < Bp; ++b) {
+      int i1 = m->ibpoin[b] - 1, f = m->ibpoin[Bp + b];
+      double fl1;
+      if (ipass == 1) {
+        aunbb1[b] = -divv[i1];
+        double af = iflag == 1 ? c->ue[i1] : afree[iflag - 1];
+        fl1 = af * amin1(0., aunbb1[b]) + unk[i1] * amax1(0., aunbb1[b]);
+      } else {
+        fl1 = 0.;
+      }
+      if (f == 11 || f == 5 || f == 6) { fl1 = 0.; aunbb1[b] = 0.; }
+      arh2[i1] = arh2[i1] + fl1 * 1.;
+    }
+    for (int i = 0; i < n; ++i) unk[i] = unk[i] - arh2[i] * dt / m->vol[i] / c->rh0[i];
+    if (ipass == 2) break;
+
+    oc_derivative(c, unk, gx, 1);
+    oc_derivative(c, unk, gy, 2);
+    oc_derivative(c, unk, gz, 3);
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? E : B;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      double *a = pass == 0 ? aun : aunb;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+        double un = fabs(a[e]) * (unk[i2] - unk[i1]) * 0.5;
+        double volt = 0.5 * (m->vol[i1] * c->rh0[i1] + m->vol[i2] * c->rh0[i2]);
+        double dpdx = 0.5 * (gx[i1] + gx[i2]) / volt;
+        double dpdy = 0.5 * (gy[i1] + gy[i2]) / volt;
+        double dpdz = 0.5 * (gz[i1] + gz[i2]) / volt;
+        double term = (U[i1] + U[i2]) * 0.5 * dpdx + (V[i1] + V[i2]) * 0.5 * dpdy + (W[i1] + W[i2]) * 0.5 * dpdz;
+        a[e] = un - a[e] * term * dt * 0.5;
+      }
+    }
+    for (int b = 0; b < Bp; ++b) aunbb1[b] = 0.;
+    /* FCT bounds from the donor-cell solution (F:3228-3247) */
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? E : B;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+        double p1 = unk[i1], p2 = unk[i2];
+        mx[i1] = amax1(amax1(mx[i1], p1), p2);
+        mn[i1] = amin1(amin1(mn[i1], p1), p2);
+        mx[i2] = amax1(amax1(mx[i2], p2), p1);
+        mn[i2] = amin1(amin1(mn[i2], p2), p1);
+      }
+    }
+    for (int i = 0; i < n; ++i) { rin[i] = 0.; rout[i] = 0.; cp[i] = 0.; cn[i] = 0.; }
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? E : B;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      double *a = pass == 0 ? aun : aunb;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+        double pp = amax1(0., a[e]), pn = -amin1(0., a[e]);
+        rin[i1] = rin[i1] + pn;
+        rout[i1] = rout[i1] + pp;
+        rin[i2] = rin[i2] + pp;
+        rout[i2] = rout[i2] + pn;
+      }
+    }
+    for (int i = 0; i < n; ++i) {
+      cp[i] = (mx[i] - unk[i]) * m->vol[i] * c->rh0[i] / (rin[i] * dt + tol);
+      cn[i] = (unk[i] - mn[i]) * m->vol[i] * c->rh0[i] / (rout[i] * dt + tol);
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? E : B;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      double *a = pass == 0 ? aun : aunb;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+        if (a[e] > 0.) a[e] = a[e] * amin1(amin1(1., cp[i2]), cn[i1]);
+        else a[e] = a[e] * amin1(amin1(1., cn[i2]), cp[i1]);
+      }
+    }
+  }
+  free(aun); free(aunb); free(aunbb1);
+}
+
+/* ---- NFT driver: mpsolver initialisation (F:177-266) and one DO ITIME step
+ * (F:285-439); diagnostics (div iflg=1, courant, stats) are not part of it. */
+double *oc_state(oc_ctx *c, const char *name) {
+  if (!strcmp(name, "unk")) return c->unk;
+  if (!strcmp(name, "frc")) return c->frc;
+  if (!strcmp(name, "uuu")) return c->uuu;
+  if (!strcmp(name, "unkdash")) return c->unkdash;
+  if (!strcmp(name, "prpot")) return c->prpot;
+  return NULL;
+}
+
+void oc_nft_init(oc_ctx *c) {
+  int n = c->n;
+  double *unk = c->unk, *frc = c->frc, *uuu = c->uuu, *ud = c->unkdash, *u1 = c->unk1;
+  for (int i = 0; i < 4 * n; ++i) frc[i] = 0.;
+  for (int i = 0; i < n; ++i) {
+    unk[i] = c->ue[i];
+    unk[n + i] = c->ve[i];
+    unk[2 * n + i] = 0.2 * frc[i];
+    unk[3 * n + i] = 0. + 0.001 * frc[i];
+    u1[i] = unk[i];
+    u1[n + i] = unk[n + i];
+    u1[2 * n + i] = unk[2 * n + i];
+    u1[3 * n + i] = 0.;
+    c->prpot[i] = 0.;
+  }
+  memset(&c->st, 0, sizeof(c->st));
+  oc_gcrk(c, c->prpot, u1, u1 + n, u1 + 2 * n, unk, unk + n, unk + 2 * n, 0);
+  oc_prfr(c, c->prpot, u1, u1 + n, u1 + 2 * n, unk, unk + n, unk + 2 * n, 0);
+  for (int i = 0; i < n; ++i) {
+    uuu[i] = unk[i]; uuu[n + i] = unk[n + i]; uuu[2 * n + i] = unk[2 * n + i];
+  }
+  for (int i = 0; i < n; ++i) {
+    c->prpot[i] = 0.;
+    frc[i] = 0.; frc[n + i] = 0.; frc[2 * n + i] = 0.; frc[3 * n + i] = 0.;
+  }
+  for (int i = 0; i < n; ++i) {
+    uuu[i] = uuu[i] * c->rh0[i];
+    uuu[n + i] = uuu[n + i] * c->rh0[i];
+    uuu[2 * n + i] = uuu[2 * n + i] * c->rh0[i];
+    ud[i] = unk[i]; ud[n + i] = unk[n + i]; ud[2 * n + i] = unk[2 * n + i];
+  }
+}
+
+void oc_nft_step(oc_ctx *c) {
+  int n = c->n;
+  double dt = c->p.dt;
+  double *unk = c->unk, *frc = c->frc, *uuu = c->uuu, *ud = c->unkdash, *u1 = c->unk1;
+  double *uuu0 = c->uuu0;
+  const double *al = c->alpha, *ue = c->ue, *ve = c->ve, *th0 = c->th0, *cw = c->coefw;
+  for (int i = 0; i < n; ++i) {
+    uuu0[i] = uuu[i]; uuu0[n + i] = uuu[n + i]; uuu0[2 * n + i] = uuu[2 * n + i];
+    for (int k = 0; k < 4; ++k) unk[k * n + i] = unk[k * n + i] + 0.5 * frc[k * n + i] * dt;
+  }
+  for (int k = 0; k < 4; ++k) oc_mpdatam(c, uuu, unk + (size_t)k * n, k + 1);
+  for (int mtri = 1; mtri <= 2; ++mtri) {
+    for (int i = 0; i < n; ++i) {
+      double frm1 = 0. - al[i] * (ud[i] - ue[i]);
+      double frm2 = 0. - al[i] * (ud[n + i] - ve[i]);
+      double frm3 = (0. - al[i] * ud[2 * n + i]) + G * unk[3 * n + i] / th0[i];
+      frc[i] = unk[i] + 0.5 * dt * frm1;
+      frc[n + i] = unk[n + i] + 0.5 * dt * frm2;
+      frc[2 * n + i] = (unk[2 * n + i] + 0.5 * dt * frm3) * cw[i];
+      u1[i] = frc[i];
+      u1[n + i] = frc[n + i];
+      u1[2 * n + i] = frc[2 * n + i];
+    }
+    oc_gcrk(c, c->prpot, u1, u1 + n, u1 + 2 * n, uuu, uuu + n, uuu + 2 * n, 1);
+    oc_prfr(c, c->prpot, u1, u1 + n, u1 + 2 * n, uuu, uuu + n, uuu + 2 * n, 1);
+    for (int i = 0; i < n; ++i) {
+      ud[i] = uuu[i];
+      ud[n + i] = uuu[n + i];
+      ud[2 * n + i] = uuu[2 * n + i];
+      if (mtri == 2) {
+        frc[i] = (ud[i] - frc[i]) * 2. / dt;
+        frc[n + i] = (ud[n + i] - frc[n + i]) * 2. / dt;
+        double frc33 = frc[2 * n + i] / cw[i] - (G * unk[3 * n + i] / th0[i]) * 0.5 * dt;
+        frc[2 * n + i] = (ud[2 * n + i] - frc33) * 2. / dt;
+        unk[i] = ud[i];
+        unk[n + i] = ud[n + i];
+        unk[2 * n + i] = ud[2 * n + i];
+        frc[3 * n + i] = -(unk[2 * n + i] * c->dthedz[i]);
+        unk[3 * n + i] = unk[3 * n + i] + 0.5 * dt * frc[3 * n + i];
+      }
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    double frm1 = 0. - al[i] * (unk[i] - ue[i]);
+    double frm2 = 0. - al[i] * (unk[n + i] - ve[i]);
+    double frm3 = 0. - al[i] * unk[2 * n + i];
+    double frm4 = -(al[i] * unk[3 * n + i]);
+    frc[i] = frc[i] + frm1;
+    frc[n + i] = frc[n + i] + frm2;
+    frc[2 * n + i] = frc[2 * n + i] + frm3;
+    frc[3 * n + i] = frc[3 * n + i] + frm4;
+    double rh = c->rh0[i];
+    uuu[i] = 1.5 * uuu[i] * rh - 0.5 * uuu0[i];
+    uuu[n + i] = 1.5 * uuu[n + i] * rh - 0.5 * uuu0[n + i];
+    uuu[2 * n + i] = 1.5 * uuu[2 * n + i] * rh - 0.5 * uuu0[2 * n + i];
+    double rhoi = 1. / rh;
+    uuu0[i] = uuu0[i] * rhoi;
+    uuu0[n + i] = uuu0[n + i] * rhoi;
+    uuu0[2 * n + i] = uuu0[2 * n + i] * rhoi;
+    ud[i] = 2. * unk[i] - uuu0[i];
+    ud[n + i] = 2. * unk[n + i] - uuu0[n + i];
+    ud[2 * n + i] = 2. * unk[2 * n + i] - uuu0[2 * n + i];
+  }
+}

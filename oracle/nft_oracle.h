@@ -1,0 +1,90 @@
+/* CPU oracle for the edge-based NFT step: TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C FP64 restatement of the reference Fortran
+ * (lippsLFroude_implgwopks3D_unstr3.f) for the Cartesian, non-periodic path
+ * (isfera=0, iper=0, lipps=0, itraj=0, implgw=1, icorio=0). Each function keeps the
+ * reference's loop order and operation order so that, compiled without FP contraction,
+ * it reproduces the reference's rounding. Only tests, the smoke check and the CPU leg
+ * of the benchmark use it, as the checker of the HIP path.
+ *
+ * Parity pinning (who checks each function):
+ *   derivative, laplc, prfr, div, bcv, precon(point), gcrk, mpdatam, compute_d,
+ *   absorbers, step driver  -> golden vectors written by the flang-compiled reference
+ *                              (tests/golden) plus closed forms in tests/test_oracle.py
+ *   line preconditioner     -> no reference counterpart: pinned by a dense solve of the
+ *                              tridiagonal system and by the closed form of the coefficients
+ */
+#ifndef NFT_ORACLE_H
+#define NFT_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oc_ctx oc_ctx;
+
+typedef struct {
+  int npoin, nedge, nbface, nboun;
+  const int32_t *iedge;   /* [2][nedge], 1-based */
+  const double *sn;       /* [3][nedge] */
+  const int32_t *kbface;  /* [2][nbface] */
+  const double *sb;       /* [3][nbface] */
+  const int32_t *ibpoin;  /* [2][nboun] (node, flag) */
+  const double *sbb;      /* [3][nboun] */
+  const double *anp;      /* [3][nboun] */
+  const double *coord;    /* [3][npoin] */
+  const double *vol;      /* [npoin] */
+  int ncol;               /* vertical columns (0: none) */
+  const int32_t *col_start; /* [ncol+1], 0-based node ranges */
+} oc_mesh;
+
+typedef struct {
+  double dt;              /* timestp (uniform) */
+  double u0, v0, w0;      /* free stream */
+  double strat;           /* N^2/g */
+  double theta0;          /* tetbar */
+  int lord, itr, exit_mode; double eps0;
+  int precon;             /* 0 none (iprc=0), 1 point relaxation, 2 vertical line */
+  int sm_flag;            /* point relaxation: 1 Jacobi, 2 Richardson, 3 mixed */
+  int itrp1; double omega;
+  /* sponge (irl=1 when any tow is nonzero) */
+  double dxabl, dxabr, dyabd, dyabu, dzabd, dzabu;
+  double towxl, towxr, towyd, towyu, towzd, towzu;
+  int irl;
+  /* domain box used by beta_timestep_unstr */
+  double dom[6];
+} oc_params;
+
+typedef struct { int niter, nitsm, icount; double eer, eem; } oc_stats;
+
+oc_ctx *oc_create(const oc_mesh *m, const oc_params *p);
+void oc_destroy(oc_ctx *c);
+
+/* profile arrays owned by the context (read/write access for tests) */
+double *oc_field(oc_ctx *c, const char *name);
+double oc_beta(oc_ctx *c);
+void oc_get_stats(oc_ctx *c, oc_stats *s);
+
+/* reference subroutines */
+void oc_derivative(oc_ctx *c, const double *unk, double *dpdxa, int idx);
+void oc_laplc(oc_ctx *c, const double *press, double *alap, int impfl);
+void oc_prfr(oc_ctx *c, const double *press, const double *ustr, const double *vstr,
+             const double *wstr, double *uout, double *vout, double *wout, int impfl);
+void oc_bcv(oc_ctx *c, double *u, double *v, double *w, double flag);
+void oc_div(oc_ctx *c, const double *ua, const double *va, const double *wa, double *rhs, int iflg);
+void oc_precon(oc_ctx *c, const double *rhs, double *p, double *r, int impfl);
+void oc_gcrk(oc_ctx *c, double *p, const double *u1, const double *v1, const double *w1,
+             double *u2, double *v2, double *w2, int impfl);
+void oc_mpdatam(oc_ctx *c, const double *uuu /*[3][npoin]*/, double *unk, int iflag);
+void oc_line_solve(oc_ctx *c, const double *rhs, double *z);
+
+/* NFT driver: state arrays are [npoin] blocks: unk[4], frc[4], uuu[3], unkdash[3], prpot */
+void oc_nft_init(oc_ctx *c);
+void oc_nft_step(oc_ctx *c);
+double *oc_state(oc_ctx *c, const char *name);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,277 @@
+"""Synthetic meshes and run configurations for the edge-based NFT step.
+
+The reference reads meshes it does not ship (`mesh3d.d`, input layout in
+lippsLFroude_implgwopks3D_unstr3.f:2343-2501), so meshes here are generated:
+jittered points -> 2-D Delaunay (Qhull through scipy) -> terrain-following
+extrusion, prism split and median-dual metrics in C++ (``csrc/meshgen.cpp``).
+
+Arrays follow the reference's column-major SoA layouts with 1-based node ids:
+``iedge[2, nedge]``, ``sn[3, nedge]``, ``kbface[2, nbface]``, ``sb[3, nbface]``,
+``ibpoin[2, nboun]`` (node, flag), ``sbb[3, nboun]``, ``anp[3, nboun]``,
+``coord[3, npoin]``, ``vol[npoin]``.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBDIR = os.path.join(_HERE, "lib")
+_lib = None
+
+
+def _meshlib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(_LIBDIR, "libnft_mesh.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run __graft_entry__.build() first")
+        lib = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int32)
+        lib.nft_meshgen_build.restype = ctypes.c_void_p
+        lib.nft_meshgen_build.argtypes = [ctypes.c_int, dp, ctypes.c_int, ip, dp, ctypes.c_int,
+                                          dp, ctypes.c_double, dp, ctypes.c_char_p, ctypes.c_int]
+        lib.nft_meshgen_sizes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
+        lib.nft_meshgen_export.argtypes = [ctypes.c_void_p, dp, dp, ip, dp, ip, dp, ip, dp, dp]
+        lib.nft_meshgen_write_text.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        lib.nft_meshgen_write_text.restype = ctypes.c_int
+        lib.nft_meshgen_free.argtypes = [ctypes.c_void_p]
+        lib.nft_meshgen_points.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_uint64, dp]
+        _lib = lib
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+@dataclasses.dataclass
+class MeshSpec:
+    nx: int
+    ny: int
+    nlev: int
+    lx: float
+    ly: float
+    htop: float
+    jitter: float = 0.3
+    seed: int = 1
+    hill_h0: float = 0.0
+    hill_a: float = 2000.0
+    dz_bottom: float = 0.0   # > 0 with dz_top: power-law stretched levels
+    dz_top: float = 0.0
+
+
+@dataclasses.dataclass
+class Mesh:
+    spec: MeshSpec
+    nh: int
+    nlev: int
+    coord: np.ndarray
+    vol: np.ndarray
+    iedge: np.ndarray
+    sn: np.ndarray
+    kbface: np.ndarray
+    sb: np.ndarray
+    ibpoin: np.ndarray
+    sbb: np.ndarray
+    anp: np.ndarray
+    tri: np.ndarray
+    _handle: Optional[int] = None
+
+    @property
+    def npoin(self):
+        return self.vol.shape[0]
+
+    @property
+    def nedge(self):
+        return self.iedge.shape[1]
+
+    @property
+    def nbface(self):
+        return self.kbface.shape[1]
+
+    @property
+    def nboun(self):
+        return self.ibpoin.shape[1]
+
+    def col_start(self) -> np.ndarray:
+        """Node ranges of the vertical columns (bottom to top), int32[nh+1]."""
+        return (np.arange(self.nh + 1, dtype=np.int64) * self.nlev).astype(np.int32)
+
+    def write_text(self, path: str) -> None:
+        lib = _meshlib()
+        if self._handle is None:
+            raise RuntimeError("mesh handle released")
+        rc = lib.nft_meshgen_write_text(ctypes.c_void_p(self._handle), path.encode())
+        if rc != 0:
+            raise OSError(f"writing {path} failed ({rc})")
+
+    def release(self):
+        if self._handle is not None:
+            _meshlib().nft_meshgen_free(ctypes.c_void_p(self._handle))
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def zeta_levels(nlev: int, htop: float, dz_bottom: float = 0.0, dz_top: float = 0.0) -> np.ndarray:
+    """Level heights of the flat-terrain column. Uniform unless dz_bottom/dz_top are given;
+    then dz_k = dz_bottom + (dz_top - dz_bottom) * (k/(n-1))**p with p chosen so the
+    column reaches htop, and the spacings are rescaled to end exactly at htop."""
+    n = nlev - 1
+    if dz_bottom <= 0 or dz_top <= 0:
+        return np.array([k * htop / n for k in range(nlev)], dtype=np.float64)
+    t = np.arange(n, dtype=np.float64) / max(n - 1, 1)
+
+    def total(p):
+        return float(np.sum(dz_bottom + (dz_top - dz_bottom) * t ** p))
+
+    lo, hi = 1e-3, 50.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if total(mid) > htop:
+            lo = mid
+        else:
+            hi = mid
+    dz = dz_bottom + (dz_top - dz_bottom) * t ** (0.5 * (lo + hi))
+    dz *= htop / dz.sum()
+    z = np.concatenate([[0.0], np.cumsum(dz)])
+    z[-1] = htop
+    return z
+
+
+def jittered_points(spec: MeshSpec) -> np.ndarray:
+    nh = spec.nx * spec.ny
+    xy = np.zeros((2, nh), dtype=np.float64)
+    _meshlib().nft_meshgen_points(spec.nx, spec.ny, spec.lx, spec.ly, spec.jitter,
+                                  ctypes.c_uint64(spec.seed), _dp(xy))
+    return xy
+
+
+def delaunay(xy: np.ndarray) -> np.ndarray:
+    """2-D Delaunay triangles [3, nt] (0-based column ids) of the point set."""
+    from scipy.spatial import Delaunay
+    d = Delaunay(xy.T.copy())
+    tri = np.ascontiguousarray(d.simplices.T.astype(np.int32))
+    used = np.zeros(xy.shape[1], dtype=bool)
+    used[tri.ravel()] = True
+    if not used.all():
+        raise RuntimeError("Delaunay triangulation dropped points")
+    return tri
+
+
+def terrain(spec: MeshSpec, xy: np.ndarray) -> np.ndarray:
+    """Gaussian hill centred in the box: h0 * 2**(-(r/a)**2) (height h0/2 at r = a)."""
+    if spec.hill_h0 == 0.0:
+        return np.zeros(xy.shape[1])
+    xc, yc = 0.5 * spec.lx, 0.5 * spec.ly
+    r2 = ((xy[0] - xc) ** 2 + (xy[1] - yc) ** 2) / spec.hill_a ** 2
+    return spec.hill_h0 * np.exp(-math.log(2.0) * r2)
+
+
+def generate(spec: MeshSpec) -> Mesh:
+    lib = _meshlib()
+    xy = jittered_points(spec)
+    tri = delaunay(xy)
+    zs = np.ascontiguousarray(terrain(spec, xy))
+    zeta = np.ascontiguousarray(zeta_levels(spec.nlev, spec.htop, spec.dz_bottom, spec.dz_top))
+    box = np.array([0.0, spec.lx, 0.0, spec.ly])
+    err = ctypes.create_string_buffer(256)
+    h = lib.nft_meshgen_build(xy.shape[1], _dp(xy), tri.shape[1], _ip(tri), _dp(zs), spec.nlev,
+                              _dp(zeta), spec.htop, _dp(box), err, 256)
+    if not h:
+        raise RuntimeError(err.value.decode())
+    sz = (ctypes.c_int64 * 7)()
+    lib.nft_meshgen_sizes(ctypes.c_void_p(h), sz)
+    N, E, B, Bp = sz[0], sz[1], sz[2], sz[3]
+    coord = np.zeros((3, N)); vol = np.zeros(N)
+    iedge = np.zeros((2, E), np.int32); sn = np.zeros((3, E))
+    kbface = np.zeros((2, B), np.int32); sb = np.zeros((3, B))
+    ibpoin = np.zeros((2, Bp), np.int32); sbb = np.zeros((3, Bp)); anp = np.zeros((3, Bp))
+    lib.nft_meshgen_export(ctypes.c_void_p(h), _dp(coord), _dp(vol), _ip(iedge), _dp(sn),
+                           _ip(kbface), _dp(sb), _ip(ibpoin), _dp(sbb), _dp(anp))
+    return Mesh(spec=spec, nh=xy.shape[1], nlev=spec.nlev, coord=coord, vol=vol, iedge=iedge,
+                sn=sn, kbface=kbface, sb=sb, ibpoin=ibpoin, sbb=sbb, anp=anp, tri=tri, _handle=h)
+
+
+# ---------------------------------------------------------------------------------------
+# Run configurations (mesh + flow + solver knobs). Names describe the workloads.
+# ---------------------------------------------------------------------------------------
+
+G = 9.80616  # gravity used by the reference (lippsLFroude_implgwopks3D_unstr3.f:94)
+
+
+@dataclasses.dataclass
+class RunConfig:
+    name: str
+    mesh: MeshSpec
+    u0: float = 10.0
+    bvf: float = 0.01          # Brunt-Vaisala frequency N; strat = N^2/g
+    dt: float = 10.0
+    steps: int = 100
+    lord: int = 3              # GCR(k)
+    eps0: float = 1e-8         # GCR tolerance (see exit_mode)
+    exit_mode: int = 1         # 0: max|r| <= eps0/dt (reference), 1: max|r| <= eps0*max|r0|
+    itr: int = 200             # max GCR restart cycles
+    precon: int = 2            # 0 none, 1 point Jacobi (reference precon), 2 vertical line
+    itrp1: int = 1             # preconditioner sweeps
+    omega: float = 1.0
+    sponge_depth: float = 3000.0
+    sponge_tau: float = 300.0
+    theta0: float = 300.0
+
+    @property
+    def strat(self):
+        return self.bvf * self.bvf / G
+
+
+def tiny_spec(nx=6, ny=5, nlev=4, seed=1):
+    return MeshSpec(nx=nx, ny=ny, nlev=nlev, lx=2500.0, ly=2000.0, htop=1500.0, seed=seed,
+                    hill_h0=200.0, hill_a=600.0)
+
+
+def small_spec():
+    """2,000 jittered points (40 x 50) over 20 km x 20 km, 20 uniform levels over 10 km,
+    Gaussian hill h0 = 500 m, half-width 2 km (40,000 nodes)."""
+    return MeshSpec(nx=40, ny=50, nlev=20, lx=20000.0, ly=20000.0, htop=10000.0,
+                    hill_h0=500.0, hill_a=2000.0)
+
+
+def production_spec():
+    """65,536 jittered points (256 x 256) over 200 km x 200 km, 64 uniform levels over
+    20 km, hill h0 = 1 km, half-width 2 km (4,194,304 nodes)."""
+    return MeshSpec(nx=256, ny=256, nlev=64, lx=200000.0, ly=200000.0, htop=20000.0,
+                    hill_h0=1000.0, hill_a=2000.0)
+
+
+def stiff_spec():
+    """131,072 jittered points (512 x 256) over 300 km x 300 km, 128 levels over 25 km
+    stretched from 50 m to 400 m, hill h0 = 2 km (16,777,216 nodes)."""
+    return MeshSpec(nx=512, ny=256, nlev=128, lx=300000.0, ly=300000.0, htop=25000.0,
+                    hill_h0=2000.0, hill_a=2000.0, dz_bottom=50.0, dz_top=400.0)
+
+
+CONFIGS = {
+    # plain FP64, small: dt=10 s, 100 steps, GCR(3), U=10, N=0.01 (Fr=2)
+    "small": RunConfig("small", small_spec(), u0=10.0, bvf=0.01, dt=10.0, steps=100, lord=3),
+    # plain FP64 at production size: dt=20 s, 1000 steps, GCR(4), N=0.01 (Fr=1)
+    "production": RunConfig("production", production_spec(), u0=10.0, bvf=0.01, dt=20.0,
+                            steps=1000, lord=4),
+    # low-Froude stress case: U=5, N=0.025 (Fr=0.1), dt=15 s, 2000 steps, GCR(4)
+    "stiff": RunConfig("stiff", stiff_spec(), u0=5.0, bvf=0.025, dt=15.0, steps=2000, lord=4),
+}
