@@ -1,0 +1,136 @@
+/* nft.h -- C ABI of the MI355X (gfx950) edge-based NFT time step.
+ *
+ * Drop-in boundary for the hot path of lippsLFroude_implgwopks3D_unstr3.f (the reference):
+ * MPDATA advection over median-dual edges and the GCR(k) solve for the anelastic pressure
+ * with implicit gravity waves. The reference's Fortran-77 subroutines keep their names and
+ * argument lists in src_amd/fortran/nft_hip.f90 and call the functions below through
+ * iso_c_binding; Python reaches them through ctypes (src_amd/nft.py).
+ *
+ * Conventions
+ *  - All arrays are host arrays in the reference's column-major SoA layouts; integer node
+ *    ids are 1-based as in the reference. Leading dimensions (the reference's compile-time
+ *    npoinx / nedgex / nfacesx) are passed explicitly so COMMON arrays can be passed as is.
+ *  - The mesh, the profiles and the time-step state live on the GPU after nft_init; the
+ *    per-subroutine entry points copy their arguments in and out, the step driver does not.
+ *  - Return codes: 0 ok; <0 argument or HIP error (text in nft_last_error()); >0 the GCR
+ *    solve stopped before reaching its tolerance (iteration limit or stagnation).
+ *  - One context per process thread; calls on one context are not reentrant.
+ */
+#ifndef NFT_H
+#define NFT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nft_ctx nft_ctx;
+
+/* Mesh as read by the reference's `input` (:2343-2501, COMMON /n/ /xyz/ /volume/ /edg/
+ * /wge/ /bound/ /bpoints/ /terms/). col_start (optional, ncol > 0) lists 0-based node
+ * ranges of vertical columns, bottom to top, for the vertical line preconditioner. */
+typedef struct {
+  int npoin, nedge, nbface, nboun;
+  int ld_poin, ld_edge, ld_face;  /* leading dimensions (>= counts) */
+  const int32_t *iedge;   /* iedge(ld_edge,2) */
+  const double *sn;       /* sn(ld_edge,3) */
+  const int32_t *kbface;  /* kbface(ld_face,2) */
+  const double *sb;       /* sb(ld_face,3) */
+  const int32_t *ibpoin;  /* ibpoin(ld_face,2): node, flag */
+  const double *sbb;      /* sbb(ld_face,3) */
+  const double *anp;      /* anp(ld_face,3) */
+  const double *coord;    /* coord(ld_poin,3) */
+  const double *vol;      /* vol(ld_poin) */
+  int ncol;
+  const int32_t *col_start; /* [ncol+1] */
+} nft_mesh;
+
+/* Run parameters: the reference's inputinc.d / input_mg_v3.d knobs and the constants
+ * hard-coded in its mpsolver (Cartesian, lipps=0, implgw=1, icorio=0, itraj=0). */
+typedef struct {
+  double dt;                 /* delt -> timestp */
+  double u0, v0, w0;         /* u0inf v0inf w0inf */
+  double strat;              /* N^2/g */
+  double theta0;             /* tetbar */
+  int lord;                  /* GCR(k), 1..16 */
+  int itr;                   /* max outer GCR cycles */
+  int exit_mode;             /* 0: max|r| <= eps0/dt (reference); 1: max|r| <= eps0*max|b| */
+  double eps0;
+  int precon;                /* 0 none (iprc=0), 1 point relaxation (reference), 2 vertical line */
+  int sm_flag;               /* point relaxation: 1 Jacobi, 2 Richardson, 3 mixed */
+  int itrp1;                 /* preconditioner sweeps */
+  double omega;
+  double dxabl, dxabr, dyabd, dyabu, dzabd, dzabu;  /* absorber depths */
+  double towxl, towxr, towyd, towyu, towzd, towzu;  /* absorber time scales */
+  int irl;
+  double dom[6];             /* box used by beta_timestep_unstr (reference: input :2255-2260) */
+  int precision;             /* 0: FP64 everywhere; 1: mixed (FP32 MPDATA + preconditioner) */
+} nft_params;
+
+/* GCR statistics, as /itero/ of the reference (:1953-1963) plus timing. */
+typedef struct {
+  int niter;       /* preconditioned iterations of the last solve (nlc) */
+  int nitsm;       /* running sum of niter */
+  int icount;      /* number of solves */
+  int converged;   /* last solve met its tolerance */
+  double eer, eem; /* final/initial preconditioned residual ratios (L2^2, max) */
+  double res_max;  /* max|r| at exit of the last solve */
+  double tol;      /* absolute tolerance the last solve used */
+} nft_gcr_stats;
+
+typedef struct {
+  int steps;
+  int gcr_solves;
+  int gcr_iters;          /* sum of niter over the solves */
+  int gcr_unconverged;    /* solves that stopped early */
+  double ms_total;        /* device time of the steps (hipEvent) */
+  double ms_gcr;          /* device time inside gcrk calls */
+  double ms_mpdata;       /* device time inside the four mpdatam calls */
+} nft_run_stats;
+
+/* ---- life cycle ---------------------------------------------------------------------- */
+/* Replaces `input`'s hand-off to mpsolver's set-up (rhprof, thprof, absorbers,
+ * beta_timestep_unstr(1), compute_d_unstr(1), coefw; :140-170, :250-252): uploads the
+ * mesh, builds the node-sorted incidence and the column factors. */
+int nft_init(const nft_mesh *mesh, const nft_params *params, nft_ctx **out);
+int nft_finalize(nft_ctx *ctx);
+const char *nft_last_error(void);
+int nft_device_name(char *buf, int len);
+
+/* ---- reference subroutines (host arrays in/out) ------------------------------------- */
+/* derivative(unk,dpdxa,idx)                     reference :3943-4022 */
+int nft_derivative(nft_ctx *ctx, const double *unk, double *dpdxa, int idx);
+/* laplc(press,alap,impfl)                       reference :1288-1353 */
+int nft_laplc(nft_ctx *ctx, const double *press, double *alap, int impfl);
+/* prfr(press,ustr,vstr,wstr,uout,vout,wout,impfl) reference :1536-1580 */
+int nft_prfr(nft_ctx *ctx, const double *press, const double *ustr, const double *vstr,
+             const double *wstr, double *uout, double *vout, double *wout, int impfl);
+/* bcv(uout,vout,wout,flag)                      reference :1583-1643 */
+int nft_bcv(nft_ctx *ctx, double *u, double *v, double *w, double flag);
+/* div(ua,va,wa,rhs,iflg), without the iflg=1 print  reference :2096-2146 */
+int nft_div(nft_ctx *ctx, const double *ua, const double *va, const double *wa, double *rhs,
+            int iflg);
+/* precon(rhs,p,r,iprc,elas,impfl), fine-grid smoothing  reference :482-612 */
+int nft_precon(nft_ctx *ctx, const double *rhs, double *p, int impfl);
+/* gcrk(p,u1,v1,w1,u2,v2,w2,impfl)               reference :1776-1972 */
+int nft_gcrk(nft_ctx *ctx, double *p, const double *u1, const double *v1, const double *w1,
+             double *u2, double *v2, double *w2, int impfl, nft_gcr_stats *stats);
+/* mpdatam(uuu,unk,iflag); uuu(ld,3)             reference :3029-3311 */
+int nft_mpdatam(nft_ctx *ctx, const double *uuu, int ld, double *unk, int iflag);
+
+/* ---- device-resident time stepping (mpsolver, reference :58-463) -------------------- */
+/* mpsolver initialisation: potential-flow projection, coefw, advector (:177-266) */
+int nft_state_init(nft_ctx *ctx);
+/* nsteps iterations of the DO ITIME loop (:285-439), state kept on the GPU */
+int nft_run_steps(nft_ctx *ctx, int nsteps, nft_run_stats *stats);
+/* state blocks: "unk"(4), "frc"(4), "uuu"(3), "unkdash"(3), "prpot"(1); [k][npoin] */
+int nft_get_state(nft_ctx *ctx, const char *name, double *host, int ld);
+int nft_set_state(nft_ctx *ctx, const char *name, const double *host, int ld);
+/* set-up fields: "alpha","d1","coefw","rh0","th0","ue","line_a","line_inv","line_c" */
+int nft_get_field(nft_ctx *ctx, const char *name, double *host);
+int nft_get_gcr_stats(nft_ctx *ctx, nft_gcr_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NFT_H */

@@ -25,7 +25,7 @@ struct oc_ctx {
   /* scratch */
   double *s[16];
   /* gcrk work */
-  double *gx, *gax, *gr, *gqr, *gar;
+  double *gx, *gax, *gr, *gqr, *gar, *gb[4];
   /* nft state */
   double *unk, *frc, *uuu, *unkdash, *prpot, *unk1, *uuu0;
   oc_stats st;
@@ -52,6 +52,7 @@ oc_ctx *oc_create(const oc_mesh *m, const oc_params *p) {
   int lord = p->lord > 0 ? p->lord : 1;
   c->gx = zalloc((size_t)lord * n); c->gax = zalloc((size_t)lord * n);
   c->gr = zalloc(n); c->gqr = zalloc(n); c->gar = zalloc(n);
+  for (int i = 0; i < 4; ++i) c->gb[i] = zalloc(n);
   c->unk = zalloc(4 * (size_t)n); c->frc = zalloc(4 * (size_t)n); c->uuu = zalloc(3 * (size_t)n);
   c->unkdash = zalloc(3 * (size_t)n); c->prpot = zalloc(n); c->unk1 = zalloc(4 * (size_t)n);
   c->uuu0 = zalloc(3 * (size_t)n);
@@ -171,6 +172,7 @@ void oc_destroy(oc_ctx *c) {
                    c->uuu, c->unkdash, c->prpot, c->unk1, c->uuu0};
   for (size_t i = 0; i < sizeof(all) / sizeof(all[0]); ++i) free(all[i]);
   for (int i = 0; i < 16; ++i) free(c->s[i]);
+  for (int i = 0; i < 4; ++i) free(c->gb[i]);
   free(c);
 }
 
@@ -419,7 +421,8 @@ void oc_precon(oc_ctx *c, const double *rhs, double *p, double *r, int impfl) {
 
 /* gcrk(p,u1,v1,w1,u2,v2,w2,impfl), F:1776-1972 (ielast=0).
  * exit_mode 0: max|r| <= eps0/dt (reference, F:1805, F:1895);
- * exit_mode 1: max|r| <= eps0 * max|r0| with r0 the residual on entry. */
+ * exit_mode 1: max|r| <= eps0 * max|b|, b = the residual of the zero initial guess
+ *              (the right-hand side -div(rho*u*)/(vol*rho) after the velocity bc). */
 void oc_gcrk(oc_ctx *c, double *p, const double *u1, const double *v1, const double *w1,
              double *u2, double *v2, double *w2, int impfl) {
   int n = c->n, lord = c->p.lord;
@@ -430,20 +433,27 @@ void oc_gcrk(oc_ctx *c, double *p, const double *u1, const double *v1, const dou
   double eps = c->p.eps0 / c->p.dt;
   for (int k = 0; k < n; ++k) { r[k] = 0.; ar[k] = 0.; qr[k] = 0.; }
   for (size_t k = 0; k < (size_t)lord * n; ++k) { x[k] = 0.; ax[k] = 0.; }
+  double bmx = 0.;
+  if (c->p.exit_mode == 1) {
+    double *z = c->gb[3];
+    for (int k = 0; k < n; ++k) z[k] = 0.;
+    oc_prfr(c, z, u1, v1, w1, c->gb[0], c->gb[1], c->gb[2], impfl);
+    oc_div(c, c->gb[0], c->gb[1], c->gb[2], z, -1);
+    for (int k = 0; k < n; ++k) bmx = amax1(bmx, fabs(z[k]));
+    eps = c->p.eps0 * bmx;
+  }
   oc_prfr(c, p, u1, v1, w1, u2, v2, w2, impfl);
   oc_div(c, u2, v2, w2, r, -1);
   oc_precon(c, r, qr, ar, impfl);
-  double eer0 = 0., eem0 = -1.e15, rl20 = 0., r0mx = 0.;
+  double eer0 = 0., eem0 = -1.e15, rl20 = 0.;
   for (int k = 0; k < n; ++k) {
     eer0 = eer0 + qr[k] * qr[k];
     eem0 = amax1(eem0, fabs(qr[k]));
     rl20 = rl20 + r[k] * r[k];
-    r0mx = amax1(r0mx, fabs(r[k]));
   }
   eer0 = amax1(eer0, epa);
   eem0 = amax1(eem0, epa);
   rl20 = amax1(rl20, epa);
-  if (c->p.exit_mode == 1) eps = c->p.eps0 * r0mx;
   for (int k = 0; k < n; ++k) x[k] = qr[k];
   oc_laplc(c, x, ax, impfl);
   for (int it = 1; it <= c->p.itr; ++it) {

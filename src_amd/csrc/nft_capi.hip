@@ -1,0 +1,1019 @@
+// C ABI of the gfx950 NFT step (declarations and contracts in include/nft.h).
+//
+// nft_init builds the device-resident mesh (node-sorted SELL-64 incidence, edge metrics,
+// boundary entries grouped per node), computes the once-per-run set-up fields on the host
+// in the reference's operation order, and allocates every work array, so the time-step
+// path allocates nothing. The reference subroutines map onto kernels as
+//   derivative -> k_grad<GRAD_RAW>             prfr  -> k_grad<GRAD_PRFR>
+//   laplc      -> k_grad<GRAD_LAPLC> + k_div<DIV_LAPLC>
+//   div        -> k_div<DIV_DIV>               bcv   -> fused into the epilogues (k_bcv alone)
+//   precon     -> k_line (vertical Thomas) | k_point (+ laplc for later sweeps)
+//   gcrk       -> the loop below: k_gcr_update, precon, laplc with fused <ax_ii,ar>, k_gcr_dir
+//   mpdatam    -> k_mp_donor, k_mp_grad, k_mp_fct, k_mp_limit (all fields of a step at once)
+// Only the GCR exit test needs the host: two doubles per inner iteration are read back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nft.h"
+#include "nft_kernels.h"
+
+using namespace nft;
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const std::string &m) {
+  g_err = m;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return set_err(-2, std::string("HIP error: ") + hipGetErrorString(e_) + \
+                                                 " at " #expr);                               \
+  } while (0)
+
+static const double kG = 9.80616;  // reference :94
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace
+
+struct nft_ctx {
+  nft_params P;
+  int n = 0, nedge = 0, nbface = 0, nboun = 0, etot = 0;
+  int ncol = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {};
+  std::vector<void *> allocs;
+
+  Incidence inc;
+  int gather_grid = 0;   // launched blocks of gather kernels
+  int vec_grid = 0;      // fixed grid of grid-stride vector kernels
+  int nodes_grid = 0;    // one thread per node
+  EdgeS<double> S;
+  EdgeS<float> Sf;
+
+  // node fields (device)
+  double *vol, *rh0, *th0, *ue, *ve, *coefw, *alpha, *d1, *dthedz, *pstep;
+  float *volf, *rh0f;
+  int32_t *col_start;
+  double *la, *linv, *lc;
+  float *laf, *linvf, *lcf;
+  double *zbuf;
+  float *zbuff;
+
+  // host copies of set-up fields
+  std::vector<double> h_alpha, h_d1, h_coefw, h_la, h_linv, h_lc;
+  double beta1 = 0;
+
+  // state
+  double *unk[4], *frc[4], *uuu[3], *uuu0[3], *ud[3], *prpot;
+  float *uuuf[3];
+
+  // work
+  double *t[8];          // prfr/div/laplc temporaries
+  double *gx[NFT_MAXD], *gax[NFT_MAXD];
+  double *r, *qr, *ar, *pw, *rw;
+  double *mp_psi1[4];
+  void *mp_g[12], *mp_c[8];
+  double *partials;
+  unsigned *counter;
+  double *scal;
+  double *h_scal;  // pinned
+  // io staging for the per-subroutine entry points
+  double *io[8];
+
+  nft_gcr_stats st{};
+};
+
+namespace {
+
+template <typename T>
+int dalloc(nft_ctx *c, T **p, size_t n) {
+  void *q = nullptr;
+  HIPCHK(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+  c->allocs.push_back(q);
+  *p = (T *)q;
+  return 0;
+}
+
+template <typename T>
+int upload(nft_ctx *c, T **p, const T *h, size_t n) {
+  int rc = dalloc(c, p, n);
+  if (rc) return rc;
+  if (n) HIPCHK(hipMemcpy(*p, h, n * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+inline int blocks_for(long n) { return (int)((n + NFT_BLOCK - 1) / NFT_BLOCK); }
+
+// ---- set-up fields on the host, operation for operation as the reference -------------
+void setup_fields(nft_ctx *c, const nft_mesh *m, std::vector<double> &rh0, std::vector<double> &th0,
+                  std::vector<double> &ue, std::vector<double> &ve, std::vector<double> &dthedz,
+                  std::vector<double> &pstep) {
+  const nft_params &p = c->P;
+  const int n = m->npoin, ldp = m->ld_poin;
+  const double *X = m->coord, *Y = m->coord + ldp, *Z = m->coord + 2 * (size_t)ldp;
+  rh0.assign(n, 1.0); th0.assign(n, p.theta0); ue.assign(n, p.u0); ve.assign(n, p.v0);
+  dthedz.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) dthedz[i] = p.theta0 * p.strat * 1.0;  // :192
+  // absorbers (:2915-3026)
+  c->h_alpha.assign(n, 0.0);
+  if (p.irl == 1) {
+    auto amax1 = [](double a, double b) { return a > b ? a : b; };
+    auto amin1 = [](double a, double b) { return a < b ? a : b; };
+    double eps = 1.e-07;
+    double abxmax = -1.e+07, abxmin = 1.e+07, abymax = -1.e+07, abymin = 1.e+07, abzmax = -1.e+07,
+           abzmin = 1.e+07;
+    for (int i = 0; i < n; ++i) {
+      abxmax = amax1(X[i], abxmax); abxmin = amin1(X[i], abxmin);
+      abymax = amax1(Y[i], abymax); abymin = amin1(Y[i], abymin);
+      abzmax = amax1(Z[i], abzmax); abzmin = amin1(Z[i], abzmin);
+    }
+    double r = 1.0;
+    double xabr = abxmax * r - p.dxabr, xabl = abxmin * r + p.dxabl;
+    double yabu = abymax * r - p.dyabu, yabd = abymin * r + p.dyabd;
+    double zabu = abzmax - p.dzabu, zabd = abzmin + p.dzabd;
+    double tolil = p.towxl != 0. ? 1. / p.towxl : 0., tolir = p.towxr != 0. ? 1. / p.towxr : 0.;
+    double toliu = p.towyu != 0. ? 1. / p.towyu : 0., tolid = p.towyd != 0. ? 1. / p.towyd : 0.;
+    double toliuz = p.towzu != 0. ? 1. / p.towzu : 0., tolidz = p.towzd != 0. ? 1. / p.towzd : 0.;
+    for (int i = 0; i < n; ++i) {
+      double relx, rely, relz, r0, r1, r2, relb;
+      r0 = X[i] * r; r1 = 0.; r2 = 0.; relb = 0.;
+      if (p.dxabl != 0.) r1 = amax1(0., xabl - r0) / p.dxabl;
+      if (p.dxabr != 0.) r2 = amax1(0., r0 - xabr) / p.dxabr;
+      if ((tolil + tolir) != 0.)
+        relb = ((tolil * r1) * (tolil * r1) + (tolir * r2) * (tolir * r2)) / (tolil * r1 + tolir * r2 + eps);
+      relx = relb;
+      r1 = 0.; r2 = 0.; relb = 0.; r0 = Y[i] * r;
+      if (p.dyabd != 0.) r1 = amax1(0., -r0 + yabd) / p.dyabd;
+      if (p.dyabu != 0.) r2 = amax1(0., r0 - yabu) / p.dyabu;
+      if ((tolid + toliu) != 0.)
+        relb = ((tolid * r1) * (tolid * r1) + (toliu * r2) * (toliu * r2)) / (tolid * r1 + toliu * r2 + eps);
+      rely = relb;
+      r1 = 0.; r2 = 0.; relb = 0.; r0 = Z[i];
+      if (p.dzabd != 0.) r1 = amax1(0., -r0 + zabd) / p.dzabd;
+      if (p.dzabu != 0.) r2 = amax1(0., r0 - zabu) / p.dzabu;
+      if ((tolidz + toliuz) != 0.)
+        relb = ((tolidz * r1) * (tolidz * r1) + (toliuz * r2) * (toliuz * r2)) / (tolidz * r1 + toliuz * r2 + eps);
+      relz = relb;
+      double alpha1 = (rely * rely + relx * relx) / (rely + relx + eps);
+      c->h_alpha[i] = (alpha1 * alpha1 + relz * relz) / (alpha1 + relz + eps);
+    }
+  }
+  // beta_timestep_unstr(1) (:953-1109)
+  {
+    double ddx = 10.e+06, hill = 1500.0;
+    const double *d = p.dom;
+    const int E = m->nedge, lde = m->ld_edge;
+    for (int e = 0; e < E; ++e) {
+      int i1 = m->iedge[e] - 1, i2 = m->iedge[lde + e] - 1;
+      if (X[i1] != d[0] && X[i1] != d[1] && Y[i1] != d[2] && Y[i1] != d[3] && Z[i1] != d[4] &&
+          Z[i1] != d[5] && X[i2] != d[0] && X[i2] != d[1] && Y[i2] != d[2] && Y[i2] != d[3] &&
+          Z[i2] != d[4] && Z[i2] != d[5] && Z[i1] > hill && Z[i2] > hill) {
+        double hx = std::fabs(m->sn[e]), hy = std::fabs(m->sn[lde + e]), hz = std::fabs(m->sn[2 * (size_t)lde + e]);
+        double h = std::max(std::max(hx, hy), hz);
+        ddx = std::min(std::min(ddx, m->vol[i1] / h), m->vol[i2] / h);
+      }
+    }
+    c->beta1 = 0.5 * ddx * ddx * 1.0;
+  }
+  // compute_d_unstr(1) (:1112-1284)
+  {
+    std::vector<double> dix(n, 0.), diy(n, 0.), diz(n, 0.);
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? m->nedge : m->nbface;
+      int ld = pass == 0 ? m->ld_edge : m->ld_face;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      const double *S = pass == 0 ? m->sn : m->sb;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ld + e] - 1;
+        double sx = S[e], sy = S[ld + e], sz = S[2 * (size_t)ld + e];
+        dix[i1] = dix[i1] + sx * sx * rh0[i2] / m->vol[i2];
+        dix[i2] = dix[i2] + sx * sx * rh0[i1] / m->vol[i1];
+        diy[i1] = diy[i1] + sy * sy * rh0[i2] / m->vol[i2];
+        diy[i2] = diy[i2] + sy * sy * rh0[i1] / m->vol[i1];
+        diz[i1] = diz[i1] + sz * sz * rh0[i2] / m->vol[i2];
+        diz[i2] = diz[i2] + sz * sz * rh0[i1] / m->vol[i1];
+      }
+    }
+    c->h_d1.assign(n, 0.);
+    for (int i = 0; i < n; ++i)
+      c->h_d1[i] = -(0.5 * p.dt * (dix[i] + diy[i] + diz[i]) / (4 * m->vol[i] * rh0[i]));
+  }
+  // coefw (:250-252)
+  c->h_coefw.assign(n, 0.);
+  for (int i = 0; i < n; ++i) {
+    double gth0 = kG / th0[i];
+    double cw = 1. + 0.25 * (p.dt * p.dt) * gth0 * dthedz[i];
+    c->h_coefw[i] = 1. / cw;
+  }
+  // point relaxation step (precon :587-599)
+  pstep.assign(n, 0.);
+  for (int i = 0; i < n; ++i) {
+    if (p.sm_flag == 1) pstep[i] = (-1.) / c->h_d1[i];
+    else if (p.sm_flag == 2) pstep[i] = c->beta1;
+    else pstep[i] = c->beta1 / (1 - c->h_d1[i] * c->beta1);
+  }
+  // vertical line preconditioner (no reference counterpart): two-point-flux edge Laplacian
+  // c_ij = 0.5*dt * 0.5(rho_i+rho_j) (Sx^2+Sy^2+cw Sz^2) / max(S.d, 1e-3|S||d|) / (vol_i rho_i),
+  // M_ii = -sum_j c_ij, M_{i,i+-1} = c_ij along the column; Thomas factors per column.
+  c->h_la.assign(n, 0.); c->h_linv.assign(n, 0.); c->h_lc.assign(n, 0.);
+  if (m->ncol > 0) {
+    std::vector<double> diag(n, 0.), lo(n, 0.), up(n, 0.);
+    std::vector<int> colof(n, -1), levof(n, -1);
+    for (int k = 0; k < m->ncol; ++k)
+      for (int i = m->col_start[k]; i < m->col_start[k + 1]; ++i) { colof[i] = k; levof[i] = i - m->col_start[k]; }
+    double hdt = 0.5 * p.dt;
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? m->nedge : m->nbface;
+      int ld = pass == 0 ? m->ld_edge : m->ld_face;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      const double *S = pass == 0 ? m->sn : m->sb;
+      for (int e = 0; e < ne; ++e) {
+        int i = ie[e] - 1, j = ie[ld + e] - 1;
+        double sx = S[e], sy = S[ld + e], sz = S[2 * (size_t)ld + e];
+        double dx = X[j] - X[i], dy = Y[j] - Y[i], dz = Z[j] - Z[i];
+        double sd = sx * dx + sy * dy + sz * dz;
+        double sl = std::sqrt(sx * sx + sy * sy + sz * sz), dl = std::sqrt(dx * dx + dy * dy + dz * dz);
+        double den = sd > 1e-3 * sl * dl ? sd : 1e-3 * sl * dl;
+        double cw = 0.5 * (c->h_coefw[i] + c->h_coefw[j]);
+        double rb = 0.5 * (rh0[i] + rh0[j]);
+        double k0 = hdt * rb * (sx * sx + sy * sy + cw * sz * sz) / den;
+        double ci = k0 / (m->vol[i] * rh0[i]);
+        double cj = k0 / (m->vol[j] * rh0[j]);
+        diag[i] -= ci;
+        diag[j] -= cj;
+        if (colof[i] >= 0 && colof[i] == colof[j]) {
+          if (levof[j] == levof[i] + 1) { up[i] = ci; lo[j] = cj; }
+          else if (levof[i] == levof[j] + 1) { lo[i] = ci; up[j] = cj; }
+        }
+      }
+    }
+    for (int k = 0; k < m->ncol; ++k) {
+      int s0 = m->col_start[k], s1 = m->col_start[k + 1];
+      double cprev = 0.;
+      for (int i = s0; i < s1; ++i) {
+        double a = (i == s0) ? 0. : lo[i];
+        double den = diag[i] - a * cprev;
+        double inv = 1. / den;
+        double cp = (i == s1 - 1) ? 0. : up[i] * inv;
+        c->h_la[i] = a; c->h_linv[i] = inv; c->h_lc[i] = cp;
+        cprev = cp;
+      }
+    }
+  }
+}
+
+// ---- device operators -------------------------------------------------------------------
+int grad_launch(nft_ctx *c, int mode, const GradArgs &a) {
+  dim3 g(c->gather_grid), b(NFT_BLOCK);
+  switch (mode) {
+    case GRAD_RAW: hipLaunchKernelGGL(k_grad<GRAD_RAW>, g, b, 0, c->stream, c->inc, c->S, a); break;
+    case GRAD_PRFR: hipLaunchKernelGGL(k_grad<GRAD_PRFR>, g, b, 0, c->stream, c->inc, c->S, a); break;
+    case GRAD_LAPLC: hipLaunchKernelGGL(k_grad<GRAD_LAPLC>, g, b, 0, c->stream, c->inc, c->S, a); break;
+    default: hipLaunchKernelGGL(k_grad<GRAD_ZERO_PRFR>, g, b, 0, c->stream, c->inc, c->S, a); break;
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+GradArgs grad_args(nft_ctx *c) {
+  GradArgs a{};
+  a.vol = c->vol; a.rh0 = c->rh0; a.coefw = c->coefw; a.ue = c->ue; a.dt = c->P.dt;
+  return a;
+}
+
+DivArgs div_args(nft_ctx *c) {
+  DivArgs a{};
+  a.vol = c->vol; a.rh0 = c->rh0; a.partials = c->partials; a.counter = c->counter;
+  a.scal = c->scal; a.epa = 1.e-38;
+  return a;
+}
+
+int div_launch(nft_ctx *c, int mode, const DivArgs &a) {
+  dim3 g(c->gather_grid), b(NFT_BLOCK);
+  if (mode == DIV_LAPLC) hipLaunchKernelGGL(k_div<DIV_LAPLC>, g, b, 0, c->stream, c->inc, c->S, a);
+  else hipLaunchKernelGGL(k_div<DIV_DIV>, g, b, 0, c->stream, c->inc, c->S, a);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// laplc(press, alap, impfl) with optional fused dots on alap
+int laplc_dev(nft_ctx *c, const double *press, double *alap, int impfl, const DotList *dots) {
+  GradArgs g = grad_args(c);
+  g.f = press; g.o0 = c->t[0]; g.o1 = c->t[1]; g.o2 = c->t[2]; g.impfl = impfl;
+  int rc = grad_launch(c, GRAD_LAPLC, g);
+  if (rc) return rc;
+  DivArgs d = div_args(c);
+  d.q0 = c->t[0]; d.q1 = c->t[1]; d.q2 = c->t[2]; d.out = alap;
+  if (dots) d.dots = *dots;
+  return div_launch(c, DIV_LAPLC, d);
+}
+
+int prfr_dev(nft_ctx *c, const double *p, const double *us, const double *vs, const double *ws,
+             double *uo, double *vo, double *wo, int impfl) {
+  GradArgs g = grad_args(c);
+  g.f = p; g.s0 = us; g.s1 = vs; g.s2 = ws; g.o0 = uo; g.o1 = vo; g.o2 = wo; g.impfl = impfl;
+  return grad_launch(c, p ? GRAD_PRFR : GRAD_ZERO_PRFR, g);
+}
+
+int div_dev(nft_ctx *c, const double *ua, const double *va, const double *wa, double *rhs, int iflg,
+            const DotList *dots) {
+  DivArgs d = div_args(c);
+  d.q0 = ua; d.q1 = va; d.q2 = wa; d.out = rhs; d.iflg = iflg;
+  if (dots) d.dots = *dots;
+  return div_launch(c, DIV_DIV, d);
+}
+
+int precon_dev(nft_ctx *c, const double *rhs, double *p, int impfl) {
+  const nft_params &P = c->P;
+  const int n = c->n;
+  if (P.precon == 0) {
+    hipLaunchKernelGGL(k_copy, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, n, rhs, p);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
+  for (int it = 1; it <= P.itrp1; ++it) {
+    if (it > 1) {
+      int rc = laplc_dev(c, p, c->rw, impfl, nullptr);
+      if (rc) return rc;
+    }
+    if (P.precon == 1) {
+      hipLaunchKernelGGL(k_point, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, n, c->pstep,
+                         c->rw, rhs, p, P.omega, it == 1 ? 1 : 0);
+    } else {
+      const double *res = rhs;
+      if (it > 1) {
+        hipLaunchKernelGGL(k_sub, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, n, rhs, c->rw, c->t[3]);
+        res = c->t[3];
+      }
+      int cg = blocks_for(c->ncol);
+      if (P.precision == 1)
+        hipLaunchKernelGGL(k_line<float>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->col_start,
+                           c->laf, c->linvf, c->lcf, res, it == 1 ? nullptr : p, p, P.omega, c->zbuff);
+      else
+        hipLaunchKernelGGL(k_line<double>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->col_start,
+                           c->la, c->linv, c->lc, res, it == 1 ? nullptr : p, p, P.omega, c->zbuf);
+    }
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
+int read_scal(nft_ctx *c, int first, int count) {
+  HIPCHK(hipMemcpyAsync(c->h_scal + first, c->scal + first, count * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int norms_dev(nft_ctx *c, const double *a, int sum_slot, int max_slot) {
+  hipLaunchKernelGGL(k_norms, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, a, (double *)nullptr,
+                     c->scal, sum_slot, max_slot, c->partials, c->counter);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// gcrk (reference :1776-1972) on device arrays. Returns 0 when converged, 1 otherwise.
+int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const double *w1, double *u2,
+             double *v2, double *w2, int impfl) {
+  const nft_params &P = c->P;
+  const int lord = P.lord, itmn = 3;
+  const double epa = 1.e-38;
+  int rc;
+  double eps = P.eps0 / P.dt;
+  if (P.exit_mode == 1) {
+    // b = residual of the zero initial guess
+    if ((rc = prfr_dev(c, nullptr, u1, v1, w1, c->t[4], c->t[5], c->t[6], impfl))) return rc;
+    if ((rc = div_dev(c, c->t[4], c->t[5], c->t[6], c->t[7], -1, nullptr))) return rc;
+    if ((rc = norms_dev(c, c->t[7], SC_TMP, SC_BMX))) return rc;
+  }
+  if ((rc = prfr_dev(c, p, u1, v1, w1, u2, v2, w2, impfl))) return rc;
+  {
+    DotList dl{};
+    dl.nd = 1; dl.a[0] = nullptr; dl.nmax = 1; dl.fin = FIN_NORMS;
+    if ((rc = div_dev(c, u2, v2, w2, c->r, -1, &dl))) return rc;
+  }
+  double *x0 = c->gx[0], *ax0 = c->gax[0];
+  if ((rc = precon_dev(c, c->r, x0, impfl))) return rc;
+  if ((rc = norms_dev(c, x0, SC_EER, SC_EEM))) return rc;
+  {
+    DotList dl{};
+    dl.nd = 2; dl.a[0] = c->r; dl.a[1] = nullptr; dl.fin = FIN_BETA; dl.slot = 0;
+    if ((rc = laplc_dev(c, x0, ax0, impfl, &dl))) return rc;
+  }
+  if ((rc = read_scal(c, 0, SC_TOTAL))) return rc;
+  if (P.exit_mode == 1) eps = P.eps0 * c->h_scal[SC_BMX];
+  double eer0 = std::max(c->h_scal[SC_EER], epa);
+  double eem0 = std::max(c->h_scal[SC_EEM], epa);
+  double rl20 = std::max(c->h_scal[SC_R2], epa);
+  const double *qr_last = x0;
+  int nlc = 0, converged = 0;
+  double dvmx = 0;
+  for (int it = 1; it <= P.itr; ++it) {
+    for (int i = 1; i <= lord; ++i) {
+      hipLaunchKernelGGL(k_gcr_update, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, p, c->r,
+                         c->gx[i - 1], c->gax[i - 1], c->scal, c->partials, c->counter);
+      HIPCHK(hipGetLastError());
+      if ((rc = read_scal(c, SC_DVMX, 2))) return rc;
+      dvmx = c->h_scal[SC_DVMX];
+      double rl2 = c->h_scal[SC_RL2];
+      if (dvmx <= eps && it >= itmn) { converged = 1; goto done; }
+      if (rl2 >= rl20) goto done;
+      rl20 = std::max(rl2, epa);
+      if ((rc = precon_dev(c, c->r, c->qr, impfl))) return rc;
+      qr_last = c->qr;
+      {
+        DotList dl{};
+        dl.nd = i;
+        for (int ii = 0; ii < i; ++ii) dl.a[ii] = c->gax[ii];
+        dl.fin = FIN_DEL;
+        if ((rc = laplc_dev(c, c->qr, c->ar, impfl, &dl))) return rc;
+      }
+      nlc = nlc + 1;
+      {
+        DirArgs d{};
+        d.n = c->n; d.nx = i;
+        int slot = (i < lord) ? i : 0;
+        d.xn = c->gx[slot]; d.axn = c->gax[slot]; d.slot = slot;
+        for (int ii = 0; ii < i; ++ii) { d.x[ii] = c->gx[ii]; d.ax[ii] = c->gax[ii]; }
+        d.qr = c->qr; d.ar = c->ar; d.r = c->r;
+        d.scal = c->scal; d.partials = c->partials; d.counter = c->counter; d.epa = epa;
+        hipLaunchKernelGGL(k_gcr_dir, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, d);
+        HIPCHK(hipGetLastError());
+      }
+    }
+  }
+done:
+  if ((rc = norms_dev(c, qr_last, SC_EER, SC_EEM))) return rc;
+  if ((rc = read_scal(c, SC_EER, 2))) return rc;
+  c->st.eer = c->h_scal[SC_EER] / eer0;
+  c->st.eem = c->h_scal[SC_EEM] / eem0;
+  c->st.niter = nlc;
+  c->st.nitsm += nlc;
+  c->st.icount += 1;
+  c->st.converged = converged;
+  c->st.res_max = dvmx;
+  c->st.tol = eps;
+  return converged ? 0 : 1;
+}
+
+template <typename T, int NF>
+int mpdata_dev(nft_ctx *c, const T *U, const T *V, const T *W, double *const *fields, const int *iflags) {
+  MpArgs<T, NF> a{};
+  a.U = U; a.V = V; a.W = W;
+  for (int f = 0; f < NF; ++f) {
+    a.psi[f] = fields[f];
+    a.psi1[f] = c->mp_psi1[f];
+    a.gx[f] = (T *)c->mp_g[3 * f];
+    a.gy[f] = (T *)c->mp_g[3 * f + 1];
+    a.gz[f] = (T *)c->mp_g[3 * f + 2];
+    a.cp[f] = (T *)c->mp_c[2 * f];
+    a.cn[f] = (T *)c->mp_c[2 * f + 1];
+    int fl = iflags[f];
+    double afree[5] = {c->P.u0, c->P.v0, c->P.w0, 0., 1.};
+    a.use_ue[f] = fl == 1;
+    a.afree[f] = (T)afree[fl - 1];
+  }
+  if (sizeof(T) == 8) { a.vol = (const T *)c->vol; a.rh0 = (const T *)c->rh0; }
+  else { a.vol = (const T *)c->volf; a.rh0 = (const T *)c->rh0f; }
+  a.ue = c->ue;
+  a.dt = (T)c->P.dt;
+  EdgeS<T> S;
+  if (sizeof(T) == 8) S = *(EdgeS<T> *)&c->S;
+  else S = *(EdgeS<T> *)&c->Sf;
+  dim3 g(c->gather_grid), b(NFT_BLOCK);
+  hipLaunchKernelGGL((k_mp_donor<T, NF>), g, b, 0, c->stream, c->inc, S, a);
+  // pass 2 reads the donor-cell fields
+  hipLaunchKernelGGL((k_mp_grad<T, NF>), g, b, 0, c->stream, c->inc, S, a);
+  hipLaunchKernelGGL((k_mp_fct<T, NF>), g, b, 0, c->stream, c->inc, S, a);
+  double *const *out = (double *const *)c->io[7];
+  hipLaunchKernelGGL((k_mp_limit<T, NF>), g, b, 0, c->stream, c->inc, S, a, out);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+StepArgs step_args(nft_ctx *c) {
+  StepArgs a{};
+  a.n = c->n;
+  for (int k = 0; k < 4; ++k) { a.unk[k] = c->unk[k]; a.frc[k] = c->frc[k]; }
+  for (int k = 0; k < 3; ++k) { a.uuu[k] = c->uuu[k]; a.uuu0[k] = c->uuu0[k]; a.ud[k] = c->ud[k]; }
+  a.alpha = c->alpha; a.ue = c->ue; a.ve = c->ve; a.th0 = c->th0; a.coefw = c->coefw;
+  a.rh0 = c->rh0; a.dthedz = c->dthedz;
+  a.dt = c->P.dt; a.g = kG;
+  for (int k = 0; k < 3; ++k) a.uuuf[k] = c->P.precision == 1 ? c->uuuf[k] : nullptr;
+  return a;
+}
+
+int set_out_fields(nft_ctx *c, double *const *fields, int nf) {
+  HIPCHK(hipMemcpyAsync(c->io[7], fields, nf * sizeof(double *), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+int mpdata_step(nft_ctx *c) {
+  int fl[4] = {1, 2, 3, 4};
+  int rc = set_out_fields(c, c->unk, 4);
+  if (rc) return rc;
+  if (c->P.precision == 1) return mpdata_dev<float, 4>(c, c->uuuf[0], c->uuuf[1], c->uuuf[2], c->unk, fl);
+  return mpdata_dev<double, 4>(c, c->uuu[0], c->uuu[1], c->uuu[2], c->unk, fl);
+}
+
+int one_step(nft_ctx *c, nft_run_stats *rs) {
+  StepArgs a = step_args(c);
+  dim3 g(c->nodes_grid), b(NFT_BLOCK);
+  int rc;
+  hipLaunchKernelGGL(k_step_pre, g, b, 0, c->stream, a);
+  HIPCHK(hipEventRecord(c->ev[2], c->stream));
+  if ((rc = mpdata_step(c))) return rc;
+  HIPCHK(hipEventRecord(c->ev[3], c->stream));
+  float ms = 0;
+  for (int mtri = 1; mtri <= 2; ++mtri) {
+    hipLaunchKernelGGL(k_step_rhs, g, b, 0, c->stream, a);
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    rc = gcrk_dev(c, c->prpot, c->frc[0], c->frc[1], c->frc[2], c->uuu[0], c->uuu[1], c->uuu[2], 1);
+    if (rc < 0) return rc;
+    if (rs) { rs->gcr_solves++; rs->gcr_iters += c->st.niter; rs->gcr_unconverged += rc; }
+    if ((rc = prfr_dev(c, c->prpot, c->frc[0], c->frc[1], c->frc[2], c->uuu[0], c->uuu[1], c->uuu[2], 1)))
+      return rc;
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(hipEventSynchronize(c->ev[1]));
+    if (rs) { HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); rs->ms_gcr += ms; }
+    hipLaunchKernelGGL(k_step_post, g, b, 0, c->stream, a, mtri == 2 ? 1 : 0);
+  }
+  hipLaunchKernelGGL(k_step_end, g, b, 0, c->stream, a);
+  HIPCHK(hipGetLastError());
+  if (rs) { HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3])); rs->ms_mpdata += ms; }
+  return 0;
+}
+
+int copy_in(nft_ctx *c, double *dst, const double *src) {
+  HIPCHK(hipMemcpyAsync(dst, src, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+int copy_out(nft_ctx *c, double *dst, const double *src) {
+  HIPCHK(hipMemcpyAsync(dst, src, c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return 0;
+}
+int sync(nft_ctx *c) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // namespace
+
+// =====================================================================================
+extern "C" {
+
+const char *nft_last_error(void) { return g_err.c_str(); }
+
+int nft_device_name(char *buf, int len) {
+  hipDeviceProp_t prop;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  std::snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  return 0;
+}
+
+int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
+  if (!m || !P || !out) return set_err(-1, "nft_init: null argument");
+  *out = nullptr;
+  if (m->npoin <= 0 || m->nedge < 0 || m->nbface < 0 || m->nboun < 0)
+    return set_err(-1, "nft_init: bad mesh sizes");
+  if (m->ld_poin < m->npoin || m->ld_edge < m->nedge || m->ld_face < std::max(m->nbface, m->nboun))
+    return set_err(-1, "nft_init: leading dimension smaller than count");
+  if (P->lord < 1 || P->lord > NFT_MAXD) return set_err(-1, "nft_init: lord out of range 1..18");
+  if (P->precon == 2 && m->ncol <= 0) return set_err(-1, "nft_init: line preconditioner needs columns");
+  if (P->precon < 0 || P->precon > 2) return set_err(-1, "nft_init: unknown preconditioner");
+  if (!(P->dt > 0)) return set_err(-1, "nft_init: dt must be positive");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(-3, "nft_init: no HIP device");
+  const int n = m->npoin;
+  const long etot = (long)m->nedge + m->nbface;
+  // validate connectivity before it reaches any kernel
+  auto bad_node = [&](int32_t v) { return v < 1 || v > n; };
+  for (int e = 0; e < m->nedge; ++e)
+    if (bad_node(m->iedge[e]) || bad_node(m->iedge[m->ld_edge + e]) || m->iedge[e] == m->iedge[m->ld_edge + e])
+      return set_err(-1, "nft_init: iedge entry out of range or degenerate at edge " + std::to_string(e + 1));
+  for (int e = 0; e < m->nbface; ++e)
+    if (bad_node(m->kbface[e]) || bad_node(m->kbface[m->ld_face + e]) || m->kbface[e] == m->kbface[m->ld_face + e])
+      return set_err(-1, "nft_init: kbface entry out of range at " + std::to_string(e + 1));
+  for (int b = 0; b < m->nboun; ++b)
+    if (bad_node(m->ibpoin[b])) return set_err(-1, "nft_init: ibpoin node out of range at " + std::to_string(b + 1));
+  if (m->ncol > 0) {
+    if (!m->col_start || m->col_start[0] != 0 || m->col_start[m->ncol] != n)
+      return set_err(-1, "nft_init: col_start must cover 0..npoin");
+    for (int k = 0; k < m->ncol; ++k)
+      if (m->col_start[k + 1] <= m->col_start[k]) return set_err(-1, "nft_init: empty column");
+  }
+  for (int i = 0; i < n; ++i)
+    if (!(m->vol[i] > 0)) return set_err(-1, "nft_init: non-positive volume at node " + std::to_string(i + 1));
+
+  nft_ctx *c = new nft_ctx();
+  c->P = *P;
+  c->n = n; c->nedge = m->nedge; c->nbface = m->nbface; c->nboun = m->nboun; c->etot = (int)etot;
+  c->ncol = m->ncol;
+  int rc = 0;
+  auto fail = [&](int code) { nft_finalize(c); return code; };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(set_err(-2, "stream"));
+  for (auto &e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(set_err(-2, "event"));
+
+  // ---- incidence: rows sorted by global edge index (interior, then semi-edges)
+  std::vector<int32_t> i1(etot), i2(etot);
+  std::vector<double> sx(etot), sy(etot), sz(etot);
+  for (int e = 0; e < m->nedge; ++e) {
+    i1[e] = m->iedge[e] - 1; i2[e] = m->iedge[m->ld_edge + e] - 1;
+    sx[e] = m->sn[e]; sy[e] = m->sn[m->ld_edge + e]; sz[e] = m->sn[2 * (size_t)m->ld_edge + e];
+  }
+  for (int e = 0; e < m->nbface; ++e) {
+    long k = m->nedge + e;
+    i1[k] = m->kbface[e] - 1; i2[k] = m->kbface[m->ld_face + e] - 1;
+    sx[k] = m->sb[e]; sy[k] = m->sb[m->ld_face + e]; sz[k] = m->sb[2 * (size_t)m->ld_face + e];
+  }
+  std::vector<int32_t> deg(n, 0);
+  for (long e = 0; e < etot; ++e) { deg[i1[e]]++; deg[i2[e]]++; }
+  const int nchunk = (n + NFT_WAVE - 1) / NFT_WAVE;
+  std::vector<int32_t> chunk_ptr(nchunk + 1, 0);
+  for (int ch = 0; ch < nchunk; ++ch) {
+    int w = 0;
+    for (int r = 0; r < NFT_WAVE; ++r) {
+      int i = ch * NFT_WAVE + r;
+      if (i < n) w = std::max(w, deg[i]);
+    }
+    if ((long)chunk_ptr[ch] + (long)NFT_WAVE * w > 2000000000L) return fail(set_err(-1, "incidence too large"));
+    chunk_ptr[ch + 1] = chunk_ptr[ch] + NFT_WAVE * w;
+  }
+  const long nslot = chunk_ptr[nchunk];
+  std::vector<int32_t> nbr(nslot), eid(nslot), fill(n, 0);
+  for (int i = 0; i < n; ++i) {
+    int ch = i / NFT_WAVE, r = i % NFT_WAVE;
+    int w = (chunk_ptr[ch + 1] - chunk_ptr[ch]) / NFT_WAVE;
+    for (int s = 0; s < w; ++s) { nbr[chunk_ptr[ch] + s * NFT_WAVE + r] = i; eid[chunk_ptr[ch] + s * NFT_WAVE + r] = 0; }
+  }
+  auto put = [&](int i, int j, int32_t code) {
+    int ch = i / NFT_WAVE, r = i % NFT_WAVE;
+    long slot = chunk_ptr[ch] + (long)fill[i] * NFT_WAVE + r;
+    nbr[slot] = j; eid[slot] = code; fill[i]++;
+  };
+  for (long e = 0; e < etot; ++e) {
+    put(i1[e], i2[e], (int32_t)e);
+    put(i2[e], i1[e], ~(int32_t)e);
+  }
+  // boundary points grouped per node, in entry order
+  std::vector<int32_t> bp_ptr(n + 1, 0);
+  for (int b = 0; b < m->nboun; ++b) bp_ptr[m->ibpoin[b]]++;
+  for (int i = 0; i < n; ++i) bp_ptr[i + 1] += bp_ptr[i];
+  const int nbp = m->nboun;
+  std::vector<int32_t> bfill(bp_ptr.begin(), bp_ptr.end() - 1), bflag(nbp);
+  std::vector<double> bsbb(3 * (size_t)nbp), banp(3 * (size_t)nbp);
+  for (int b = 0; b < nbp; ++b) {
+    int i = m->ibpoin[b] - 1;
+    int q = bfill[i]++;
+    bflag[q] = m->ibpoin[m->ld_face + b];
+    for (int d = 0; d < 3; ++d) {
+      bsbb[(size_t)d * nbp + q] = m->sbb[(size_t)d * m->ld_face + b];
+      banp[(size_t)d * nbp + q] = m->anp[(size_t)d * m->ld_face + b];
+    }
+  }
+  int32_t *d_chunk, *d_deg, *d_nbr, *d_eid, *d_bp_ptr, *d_bflag;
+  double *d_bsbb, *d_banp;
+  if ((rc = upload(c, &d_chunk, chunk_ptr.data(), chunk_ptr.size())) ||
+      (rc = upload(c, &d_deg, deg.data(), deg.size())) || (rc = upload(c, &d_nbr, nbr.data(), nbr.size())) ||
+      (rc = upload(c, &d_eid, eid.data(), eid.size())) || (rc = upload(c, &d_bp_ptr, bp_ptr.data(), bp_ptr.size())) ||
+      (rc = upload(c, &d_bflag, bflag.data(), bflag.size())) || (rc = upload(c, &d_bsbb, bsbb.data(), bsbb.size())) ||
+      (rc = upload(c, &d_banp, banp.data(), banp.size())))
+    return fail(rc);
+  Incidence &g = c->inc;
+  g.n = n; g.nchunk = nchunk; g.chunk_ptr = d_chunk; g.deg = d_deg; g.nbr = d_nbr; g.eid = d_eid;
+  g.bp_ptr = d_bp_ptr; g.bp_flag = d_bflag; g.bp_sbb = d_bsbb; g.bp_anp = d_banp; g.nbp = nbp;
+  g.nblocks = (nchunk + NFT_WPB - 1) / NFT_WPB;
+  if (g.nblocks >= 64) {
+    g.xcd_per = (g.nblocks + 7) / 8;
+    c->gather_grid = 8 * g.xcd_per;
+  } else {
+    g.xcd_per = 0;
+    c->gather_grid = g.nblocks;
+  }
+  c->vec_grid = std::min(blocks_for(n), 1024);
+  c->nodes_grid = blocks_for(n);
+  double *dsx, *dsy, *dsz;
+  if ((rc = upload(c, &dsx, sx.data(), etot)) || (rc = upload(c, &dsy, sy.data(), etot)) ||
+      (rc = upload(c, &dsz, sz.data(), etot)))
+    return fail(rc);
+  c->S = EdgeS<double>{dsx, dsy, dsz};
+
+  // ---- set-up fields
+  std::vector<double> rh0, th0, ue, ve, dthedz, pstep;
+  setup_fields(c, m, rh0, th0, ue, ve, dthedz, pstep);
+  std::vector<double> volv(m->vol, m->vol + n);
+  if ((rc = upload(c, &c->vol, volv.data(), n)) || (rc = upload(c, &c->rh0, rh0.data(), n)) ||
+      (rc = upload(c, &c->th0, th0.data(), n)) || (rc = upload(c, &c->ue, ue.data(), n)) ||
+      (rc = upload(c, &c->ve, ve.data(), n)) || (rc = upload(c, &c->coefw, c->h_coefw.data(), n)) ||
+      (rc = upload(c, &c->alpha, c->h_alpha.data(), n)) || (rc = upload(c, &c->d1, c->h_d1.data(), n)) ||
+      (rc = upload(c, &c->dthedz, dthedz.data(), n)) || (rc = upload(c, &c->pstep, pstep.data(), n)) ||
+      (rc = upload(c, &c->la, c->h_la.data(), n)) || (rc = upload(c, &c->linv, c->h_linv.data(), n)) ||
+      (rc = upload(c, &c->lc, c->h_lc.data(), n)))
+    return fail(rc);
+  if (m->ncol > 0) {
+    if ((rc = upload(c, &c->col_start, m->col_start, m->ncol + 1))) return fail(rc);
+  }
+  if ((rc = dalloc(c, &c->zbuf, n))) return fail(rc);
+  // FP32 copies for the mixed-precision form
+  if (P->precision == 1) {
+    auto tof = [](const std::vector<double> &v) { std::vector<float> f(v.size()); for (size_t i = 0; i < v.size(); ++i) f[i] = (float)v[i]; return f; };
+    std::vector<float> f;
+    float *fx, *fy, *fz;
+    if ((rc = upload(c, &fx, (f = tof(sx)).data(), etot)) || (rc = upload(c, &fy, (f = tof(sy)).data(), etot)) ||
+        (rc = upload(c, &fz, (f = tof(sz)).data(), etot)))
+      return fail(rc);
+    c->Sf = EdgeS<float>{fx, fy, fz};
+    if ((rc = upload(c, &c->volf, (f = tof(volv)).data(), n)) || (rc = upload(c, &c->rh0f, (f = tof(rh0)).data(), n)) ||
+        (rc = upload(c, &c->laf, (f = tof(c->h_la)).data(), n)) || (rc = upload(c, &c->linvf, (f = tof(c->h_linv)).data(), n)) ||
+        (rc = upload(c, &c->lcf, (f = tof(c->h_lc)).data(), n)) || (rc = dalloc(c, &c->zbuff, n)))
+      return fail(rc);
+    for (int k = 0; k < 3; ++k)
+      if ((rc = dalloc(c, &c->uuuf[k], n))) return fail(rc);
+  }
+
+  // ---- state and work arrays (allocated once)
+  for (int k = 0; k < 4; ++k)
+    if ((rc = dalloc(c, &c->unk[k], n)) || (rc = dalloc(c, &c->frc[k], n))) return fail(rc);
+  for (int k = 0; k < 3; ++k)
+    if ((rc = dalloc(c, &c->uuu[k], n)) || (rc = dalloc(c, &c->uuu0[k], n)) || (rc = dalloc(c, &c->ud[k], n)))
+      return fail(rc);
+  if ((rc = dalloc(c, &c->prpot, n))) return fail(rc);
+  for (int k = 0; k < 8; ++k)
+    if ((rc = dalloc(c, &c->t[k], n))) return fail(rc);
+  for (int k = 0; k < P->lord; ++k)
+    if ((rc = dalloc(c, &c->gx[k], n)) || (rc = dalloc(c, &c->gax[k], n))) return fail(rc);
+  if ((rc = dalloc(c, &c->r, n)) || (rc = dalloc(c, &c->qr, n)) || (rc = dalloc(c, &c->ar, n)) ||
+      (rc = dalloc(c, &c->pw, n)) || (rc = dalloc(c, &c->rw, n)))
+    return fail(rc);
+  const size_t tsz = P->precision == 1 ? sizeof(float) : sizeof(double);
+  for (int k = 0; k < 4; ++k)
+    if ((rc = dalloc(c, &c->mp_psi1[k], n))) return fail(rc);
+  for (int k = 0; k < 12; ++k) {
+    void *q;
+    if (hipMalloc(&q, n * tsz) != hipSuccess) return fail(set_err(-2, "alloc"));
+    c->allocs.push_back(q); c->mp_g[k] = q;
+  }
+  for (int k = 0; k < 8; ++k) {
+    void *q;
+    if (hipMalloc(&q, n * tsz) != hipSuccess) return fail(set_err(-2, "alloc"));
+    c->allocs.push_back(q); c->mp_c[k] = q;
+  }
+  for (int k = 0; k < 7; ++k)
+    if ((rc = dalloc(c, &c->io[k], n))) return fail(rc);
+  if ((rc = dalloc(c, &c->io[7], 8))) return fail(rc);  // device array of field pointers
+  size_t npart = (size_t)std::max(c->gather_grid, c->vec_grid) * (NFT_MAXD + 1);
+  if ((rc = dalloc(c, &c->partials, npart)) || (rc = dalloc(c, &c->counter, 4)) || (rc = dalloc(c, &c->scal, SC_TOTAL)))
+    return fail(rc);
+  if (hipMemset(c->counter, 0, 4 * sizeof(unsigned)) != hipSuccess) return fail(set_err(-2, "memset"));
+  if (hipMemset(c->scal, 0, SC_TOTAL * sizeof(double)) != hipSuccess) return fail(set_err(-2, "memset"));
+  if (hipHostMalloc(&c->h_scal, SC_TOTAL * sizeof(double)) != hipSuccess) return fail(set_err(-2, "host alloc"));
+  for (int k = 0; k < 4; ++k)
+    if (hipMemset(c->frc[k], 0, n * sizeof(double)) != hipSuccess) return fail(set_err(-2, "memset"));
+  if (hipDeviceSynchronize() != hipSuccess) return fail(set_err(-2, "sync"));
+  *out = c;
+  return 0;
+}
+
+int nft_finalize(nft_ctx *c) {
+  if (!c) return 0;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void *p : c->allocs) hipFree(p);
+  if (c->h_scal) hipHostFree(c->h_scal);
+  for (auto &e : c->ev)
+    if (e) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int nft_get_field(nft_ctx *c, const char *name, double *host) {
+  if (!c || !name || !host) return set_err(-1, "nft_get_field: null argument");
+  const std::vector<double> *v = nullptr;
+  std::string s(name);
+  if (s == "alpha") v = &c->h_alpha;
+  else if (s == "d1") v = &c->h_d1;
+  else if (s == "coefw") v = &c->h_coefw;
+  else if (s == "line_a") v = &c->h_la;
+  else if (s == "line_inv") v = &c->h_linv;
+  else if (s == "line_c") v = &c->h_lc;
+  if (v) { std::memcpy(host, v->data(), c->n * sizeof(double)); return 0; }
+  double *d = nullptr;
+  if (s == "rh0") d = c->rh0;
+  else if (s == "th0") d = c->th0;
+  else if (s == "ue") d = c->ue;
+  else if (s == "beta") { host[0] = c->beta1; return 0; }
+  if (!d) return set_err(-1, "nft_get_field: unknown field " + s);
+  HIPCHK(hipMemcpy(host, d, c->n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int nft_derivative(nft_ctx *c, const double *unk, double *dpdxa, int idx) {
+  if (!c || !unk || !dpdxa || idx < 1 || idx > 3) return set_err(-1, "nft_derivative: bad argument");
+  int rc;
+  if ((rc = copy_in(c, c->io[0], unk))) return rc;
+  GradArgs g = grad_args(c);
+  g.f = c->io[0]; g.o0 = c->io[1]; g.o1 = c->io[2]; g.o2 = c->io[3];
+  if ((rc = grad_launch(c, GRAD_RAW, g))) return rc;
+  if ((rc = copy_out(c, dpdxa, c->io[idx]))) return rc;
+  return sync(c);
+}
+
+int nft_laplc(nft_ctx *c, const double *press, double *alap, int impfl) {
+  if (!c || !press || !alap) return set_err(-1, "nft_laplc: null argument");
+  int rc;
+  if ((rc = copy_in(c, c->io[0], press))) return rc;
+  if ((rc = laplc_dev(c, c->io[0], c->io[1], impfl, nullptr))) return rc;
+  if ((rc = copy_out(c, alap, c->io[1]))) return rc;
+  return sync(c);
+}
+
+int nft_prfr(nft_ctx *c, const double *press, const double *ustr, const double *vstr, const double *wstr,
+             double *uout, double *vout, double *wout, int impfl) {
+  if (!c || !press || !ustr || !vstr || !wstr || !uout || !vout || !wout) return set_err(-1, "nft_prfr: null argument");
+  int rc;
+  if ((rc = copy_in(c, c->io[0], press)) || (rc = copy_in(c, c->io[1], ustr)) || (rc = copy_in(c, c->io[2], vstr)) ||
+      (rc = copy_in(c, c->io[3], wstr)))
+    return rc;
+  if ((rc = prfr_dev(c, c->io[0], c->io[1], c->io[2], c->io[3], c->io[4], c->io[5], c->io[6], impfl))) return rc;
+  if ((rc = copy_out(c, uout, c->io[4])) || (rc = copy_out(c, vout, c->io[5])) || (rc = copy_out(c, wout, c->io[6])))
+    return rc;
+  return sync(c);
+}
+
+int nft_bcv(nft_ctx *c, double *u, double *v, double *w, double flag) {
+  if (!c || !u || !v || !w) return set_err(-1, "nft_bcv: null argument");
+  int rc;
+  if ((rc = copy_in(c, c->io[0], u)) || (rc = copy_in(c, c->io[1], v)) || (rc = copy_in(c, c->io[2], w))) return rc;
+  hipLaunchKernelGGL(k_bcv, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, c->inc, c->io[0], c->io[1],
+                     c->io[2], c->ue, flag);
+  HIPCHK(hipGetLastError());
+  if ((rc = copy_out(c, u, c->io[0])) || (rc = copy_out(c, v, c->io[1])) || (rc = copy_out(c, w, c->io[2]))) return rc;
+  return sync(c);
+}
+
+int nft_div(nft_ctx *c, const double *ua, const double *va, const double *wa, double *rhs, int iflg) {
+  if (!c || !ua || !va || !wa || !rhs) return set_err(-1, "nft_div: null argument");
+  int rc;
+  if ((rc = copy_in(c, c->io[0], ua)) || (rc = copy_in(c, c->io[1], va)) || (rc = copy_in(c, c->io[2], wa))) return rc;
+  if ((rc = div_dev(c, c->io[0], c->io[1], c->io[2], c->io[3], iflg, nullptr))) return rc;
+  if ((rc = copy_out(c, rhs, c->io[3]))) return rc;
+  return sync(c);
+}
+
+int nft_precon(nft_ctx *c, const double *rhs, double *p, int impfl) {
+  if (!c || !rhs || !p) return set_err(-1, "nft_precon: null argument");
+  int rc;
+  if ((rc = copy_in(c, c->io[0], rhs))) return rc;
+  if ((rc = precon_dev(c, c->io[0], c->io[1], impfl))) return rc;
+  if ((rc = copy_out(c, p, c->io[1]))) return rc;
+  return sync(c);
+}
+
+int nft_gcrk(nft_ctx *c, double *p, const double *u1, const double *v1, const double *w1, double *u2, double *v2,
+             double *w2, int impfl, nft_gcr_stats *stats) {
+  if (!c || !p || !u1 || !v1 || !w1 || !u2 || !v2 || !w2) return set_err(-1, "nft_gcrk: null argument");
+  int rc;
+  if ((rc = copy_in(c, c->io[0], p)) || (rc = copy_in(c, c->io[1], u1)) || (rc = copy_in(c, c->io[2], v1)) ||
+      (rc = copy_in(c, c->io[3], w1)))
+    return rc;
+  int res = gcrk_dev(c, c->io[0], c->io[1], c->io[2], c->io[3], c->io[4], c->io[5], c->io[6], impfl);
+  if (res < 0) return res;
+  if ((rc = copy_out(c, p, c->io[0])) || (rc = copy_out(c, u2, c->io[4])) || (rc = copy_out(c, v2, c->io[5])) ||
+      (rc = copy_out(c, w2, c->io[6])))
+    return rc;
+  if ((rc = sync(c))) return rc;
+  if (stats) *stats = c->st;
+  return res;
+}
+
+int nft_mpdatam(nft_ctx *c, const double *uuu, int ld, double *unk, int iflag) {
+  if (!c || !uuu || !unk || iflag < 1 || iflag > 4 || ld < c->n) return set_err(-1, "nft_mpdatam: bad argument");
+  int rc;
+  if ((rc = copy_in(c, c->io[0], uuu)) || (rc = copy_in(c, c->io[1], uuu + ld)) ||
+      (rc = copy_in(c, c->io[2], uuu + 2 * (size_t)ld)) || (rc = copy_in(c, c->io[3], unk)))
+    return rc;
+  double *fields[1] = {c->io[3]};
+  int fl[1] = {iflag};
+  if ((rc = set_out_fields(c, fields, 1))) return rc;
+  if (c->P.precision == 1) {
+    float *uf[3] = {(float *)c->mp_g[9], (float *)c->mp_g[10], (float *)c->mp_g[11]};
+    // the NF=1 call uses only the first 3 gradient slots, so slots 9..11 are free for uuu in FP32
+    for (int k = 0; k < 3; ++k)
+      hipLaunchKernelGGL(k_to_t<float>, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, c->io[k], uf[k]);
+    rc = mpdata_dev<float, 1>(c, uf[0], uf[1], uf[2], fields, fl);
+  } else {
+    rc = mpdata_dev<double, 1>(c, c->io[0], c->io[1], c->io[2], fields, fl);
+  }
+  if (rc) return rc;
+  if ((rc = copy_out(c, unk, c->io[3]))) return rc;
+  return sync(c);
+}
+
+int nft_state_init(nft_ctx *c) {
+  if (!c) return set_err(-1, "nft_state_init: null context");
+  const int n = c->n;
+  dim3 g(c->nodes_grid), b(NFT_BLOCK);
+  int rc;
+  // unk = (ue, ve, 0, 0); unk1 (kept in frc[0..2]) = (ue, ve, 0); prpot = 0   (:177-206)
+  HIPCHK(hipMemcpyAsync(c->unk[0], c->ue, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->unk[1], c->ve, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_fill, g, b, 0, c->stream, n, c->unk[2], 0.0);
+  hipLaunchKernelGGL(k_fill, g, b, 0, c->stream, n, c->unk[3], 0.0);
+  hipLaunchKernelGGL(k_fill, g, b, 0, c->stream, n, c->prpot, 0.0);
+  HIPCHK(hipMemcpyAsync(c->frc[0], c->ue, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->frc[1], c->ve, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_fill, g, b, 0, c->stream, n, c->frc[2], 0.0);
+  c->st = nft_gcr_stats{};
+  rc = gcrk_dev(c, c->prpot, c->frc[0], c->frc[1], c->frc[2], c->unk[0], c->unk[1], c->unk[2], 0);
+  if (rc < 0) return rc;
+  if ((rc = prfr_dev(c, c->prpot, c->frc[0], c->frc[1], c->frc[2], c->unk[0], c->unk[1], c->unk[2], 0))) return rc;
+  hipLaunchKernelGGL(k_fill, g, b, 0, c->stream, n, c->prpot, 0.0);
+  StepArgs a = step_args(c);
+  hipLaunchKernelGGL(k_init_post, g, b, 0, c->stream, a);
+  HIPCHK(hipGetLastError());
+  return sync(c);
+}
+
+int nft_run_steps(nft_ctx *c, int nsteps, nft_run_stats *stats) {
+  if (!c || nsteps < 0) return set_err(-1, "nft_run_steps: bad argument");
+  nft_run_stats rs{};
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = one_step(c, &rs);
+    if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
+    rs.steps++;
+  }
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  rs.ms_total = ms;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (stats) *stats = rs;
+  return 0;
+}
+
+static int state_ptrs(nft_ctx *c, const char *name, double **p, int *k) {
+  std::string s(name ? name : "");
+  if (s == "unk") { for (int i = 0; i < 4; ++i) p[i] = c->unk[i]; *k = 4; }
+  else if (s == "frc") { for (int i = 0; i < 4; ++i) p[i] = c->frc[i]; *k = 4; }
+  else if (s == "uuu") { for (int i = 0; i < 3; ++i) p[i] = c->uuu[i]; *k = 3; }
+  else if (s == "unkdash") { for (int i = 0; i < 3; ++i) p[i] = c->ud[i]; *k = 3; }
+  else if (s == "prpot") { p[0] = c->prpot; *k = 1; }
+  else return set_err(-1, "unknown state block " + s);
+  return 0;
+}
+
+int nft_get_state(nft_ctx *c, const char *name, double *host, int ld) {
+  if (!c || !host || ld < c->n) return set_err(-1, "nft_get_state: bad argument");
+  double *p[4]; int k;
+  int rc = state_ptrs(c, name, p, &k);
+  if (rc) return rc;
+  for (int i = 0; i < k; ++i)
+    if ((rc = copy_out(c, host + (size_t)i * ld, p[i]))) return rc;
+  return sync(c);
+}
+
+int nft_set_state(nft_ctx *c, const char *name, const double *host, int ld) {
+  if (!c || !host || ld < c->n) return set_err(-1, "nft_set_state: bad argument");
+  double *p[4]; int k;
+  int rc = state_ptrs(c, name, p, &k);
+  if (rc) return rc;
+  for (int i = 0; i < k; ++i)
+    if ((rc = copy_in(c, p[i], host + (size_t)i * ld))) return rc;
+  if (std::string(name) == "uuu" && c->P.precision == 1)
+    for (int i = 0; i < 3; ++i)
+      hipLaunchKernelGGL(k_to_t<float>, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, c->uuu[i], c->uuuf[i]);
+  return sync(c);
+}
+
+int nft_get_gcr_stats(nft_ctx *c, nft_gcr_stats *s) {
+  if (!c || !s) return set_err(-1, "nft_get_gcr_stats: bad argument");
+  *s = c->st;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,844 @@
+// Device kernels of the edge-based NFT step for gfx950 (one translation unit with
+// nft_capi.hip).
+//
+// Layout in HBM
+//  * Node-sorted incidence in SELL-64: nodes are taken 64 at a time (one wavefront, one
+//    node per lane); chunk c owns slots [chunk_ptr[c], chunk_ptr[c+1]) laid out slot-major
+//    (slot m of lane r at chunk_ptr[c] + 64*m + r), so a wavefront reading slot m issues
+//    one coalesced load. A slot holds the neighbour node and the signed edge index:
+//    e >= 0 when the node is the edge's first node i1, ~e when it is i2. The slots of a
+//    node are in increasing global edge order (interior edges, then boundary semi-edges),
+//    which is exactly the order in which the reference's edge loops scatter into that
+//    node, so every gathered sum rounds as the reference's scatter does.
+//  * Edge metrics SoA: sx[e], sy[e], sz[e] over interior edges followed by semi-edges.
+//  * Boundary points grouped per node (CSR bp_ptr) in the reference's entry order:
+//    flag, sbb(3), anp(3).
+//  * Fields are node arrays [npoin]; columns of a generated mesh are level-contiguous, so
+//    the 64 lanes of a wavefront usually walk one column and their neighbour gathers and
+//    edge-metric loads are near-contiguous.
+// All arithmetic is written operand-for-operand like the reference Fortran and compiled
+// without FP contraction, so every non-reduction result equals the CPU oracle bit for bit.
+// Reductions (GCR inner products) use a fixed tree: per-wave shuffles, per-block shared
+// memory, then the last block to finish sums the per-block partials in block order, so
+// results are identical from run to run.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NFT_WAVE 64
+#define NFT_BLOCK 256
+#define NFT_WPB (NFT_BLOCK / NFT_WAVE)
+#define NFT_MAXD 18
+
+namespace nft {
+
+struct Incidence {
+  int n, nchunk;
+  const int32_t *chunk_ptr;  // [nchunk+1]
+  const int32_t *deg;        // [n]
+  const int32_t *nbr;        // slots
+  const int32_t *eid;        // slots
+  const int32_t *bp_ptr;     // [n+1]
+  const int32_t *bp_flag;    // [nbp]
+  const double *bp_sbb;      // [3][nbp]
+  const double *bp_anp;      // [3][nbp]
+  int nbp;
+  int xcd_per;               // logical blocks per XCD slice (remap), 0 = identity
+  int nblocks;               // logical blocks
+};
+
+template <typename T>
+struct EdgeS {
+  const T *sx, *sy, *sz;
+};
+
+// Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous range of chunks
+// so neighbouring columns share that XCD's L2.
+__device__ __forceinline__ int logical_block(const Incidence &g) {
+  int b = blockIdx.x;
+  if (g.xcd_per <= 0) return b;
+  return (b & 7) * g.xcd_per + (b >> 3);
+}
+
+struct Lane {
+  int i;      // node id, may be >= n for tail lanes
+  int base;   // slot base of the chunk
+  int width;  // slots per lane in this chunk
+  int deg;    // slots used by this node
+  bool valid;
+  int lane;
+};
+
+__device__ __forceinline__ bool lane_setup(const Incidence &g, Lane &L) {
+  int lb = logical_block(g);
+  if (lb >= g.nblocks) return false;
+  int chunk = lb * NFT_WPB + (threadIdx.x >> 6);
+  if (chunk >= g.nchunk) return false;
+  L.lane = threadIdx.x & 63;
+  L.i = chunk * NFT_WAVE + L.lane;
+  L.base = g.chunk_ptr[chunk];
+  L.width = (g.chunk_ptr[chunk + 1] - L.base) >> 6;
+  L.valid = L.i < g.n;
+  L.deg = L.valid ? g.deg[L.i] : 0;
+  return true;
+}
+
+// ------------------------------------------------------------------------------------
+// Deterministic reductions
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+// Reduces K values over the block (sum, or max where ismax bit k is set); result valid in
+// thread 0. Uses sh[K*NFT_WPB].
+template <int K>
+__device__ __forceinline__ void block_reduce(double (&v)[K], unsigned ismax, double *sh) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = ((ismax >> k) & 1) ? wave_max(v[k]) : wave_sum(v[k]);
+  if (l == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[k * NFT_WPB + w] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double a = sh[k * NFT_WPB];
+      for (int j = 1; j < NFT_WPB; ++j) a = ((ismax >> k) & 1) ? fmax(a, sh[k * NFT_WPB + j]) : a + sh[k * NFT_WPB + j];
+      v[k] = a;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void st_agent(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_agent(const double *p) {
+  return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Writes this block's K partials, and returns true in every thread of the block that
+// finishes last; that block then holds all partials summed in block order in out[K]
+// (valid in all threads).
+template <int K>
+__device__ bool grid_reduce(double (&v)[K], unsigned ismax, double *partials, unsigned *counter,
+                            int nblk, int myblk, double *sh, double (&out)[K]) {
+  block_reduce<K>(v, ismax, sh);
+  __shared__ int s_last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) st_agent(&partials[(size_t)myblk * K + k], v[k]);
+    __threadfence();
+    unsigned t = atomicAdd(counter, 1u);
+    s_last = (t == (unsigned)(nblk - 1));
+  }
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = ((ismax >> k) & 1) ? -1.0e300 : 0.0;
+  for (int b = threadIdx.x; b < nblk; b += NFT_BLOCK) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double p = ld_agent(&partials[(size_t)b * K + k]);
+      acc[k] = ((ismax >> k) & 1) ? fmax(acc[k], p) : acc[k] + p;
+    }
+  }
+  block_reduce<K>(acc, ismax, sh);
+  __shared__ double s_out[K];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) s_out[k] = acc[k];
+    atomicExch(counter, 0u);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[k] = s_out[k];
+  return true;
+}
+
+// ------------------------------------------------------------------------------------
+// Boundary conditions on one node (bcv, reference :1583-1643): entries in order.
+// ------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void bcv_node(const Incidence &g, int i, T &u, T &v, T &w, T ue_flag) {
+  int b0 = g.bp_ptr[i], b1 = g.bp_ptr[i + 1];
+  for (int b = b0; b < b1; ++b) {
+    int f = g.bp_flag[b];
+    if (f == 11 || f == 5 || f == 6) {
+      T ax = (T)g.bp_anp[b], ay = (T)g.bp_anp[g.nbp + b], az = (T)g.bp_anp[2 * g.nbp + b];
+      T vn = u * ax + v * ay + w * az;
+      u = u - ax * vn;
+      v = v - ay * vn;
+      w = w - az * vn;
+    }
+    if (f == 7 || f == 8) u = ue_flag;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Gradient of one field (derivative x3) with per-mode epilogues.
+// ------------------------------------------------------------------------------------
+enum GradMode {
+  GRAD_RAW = 0,     // out0..2 = vol-integrated gradient (derivative idx 1..3)
+  GRAD_PRFR = 1,    // out = str - 0.5*dt*g/vol (w: *fz), then bcv(...,1.)       (prfr)
+  GRAD_LAPLC = 2,   // out = rh0 * bcv0(0.5*(dt*g/vol), w: *fz)                    (laplc 1st half)
+  GRAD_ZERO_PRFR = 3  // pressure identically zero: out = bcv(str, 1.)
+};
+
+struct GradArgs {
+  const double *f;
+  double *o0, *o1, *o2;
+  const double *s0, *s1, *s2;  // ustr, vstr, wstr (PRFR)
+  const double *vol, *rh0, *coefw, *ue;
+  double dt;
+  int impfl;
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S, GradArgs a) {
+  Lane L;
+  if (!lane_setup(g, L)) return;
+  const int i = L.i;
+  double gx = 0., gy = 0., gz = 0.;
+  if (MODE != GRAD_ZERO_PRFR) {
+    double fi = L.valid ? a.f[i] : 0.;
+    for (int m = 0; m < L.width; ++m) {
+      if (m < L.deg) {
+        int slot = L.base + m * NFT_WAVE + L.lane;
+        int j = g.nbr[slot], e = g.eid[slot];
+        bool first = e >= 0;
+        int ee = first ? e : ~e;
+        double u = (fi + a.f[j]) * 0.5;
+        double sx = S.sx[ee], sy = S.sy[ee], sz = S.sz[ee];
+        if (first) { gx = gx + sx * u; gy = gy + sy * u; gz = gz + sz * u; }
+        else { gx = gx - sx * u; gy = gy - sy * u; gz = gz - sz * u; }
+      }
+    }
+    if (L.valid) {
+      for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
+        gx = gx + g.bp_sbb[b] * fi;
+        gy = gy + g.bp_sbb[g.nbp + b] * fi;
+        gz = gz + g.bp_sbb[2 * g.nbp + b] * fi;
+      }
+    }
+  }
+  if (!L.valid) return;
+  if (MODE == GRAD_RAW) {
+    a.o0[i] = gx; a.o1[i] = gy; a.o2[i] = gz;
+  } else if (MODE == GRAD_PRFR || MODE == GRAD_ZERO_PRFR) {
+    double u, v, w;
+    if (MODE == GRAD_PRFR) {
+      double vl = a.vol[i];
+      gx = gx / vl; gy = gy / vl; gz = gz / vl;
+      double fz = a.impfl ? a.coefw[i] : 1.0;
+      double h = 0.5 * a.dt;
+      u = a.s0[i] - h * gx;
+      v = a.s1[i] - h * gy;
+      w = a.s2[i] - h * gz * fz;
+    } else {
+      u = a.s0[i]; v = a.s1[i]; w = a.s2[i];
+    }
+    bcv_node<double>(g, i, u, v, w, a.ue[i] * 1.0);
+    a.o0[i] = u; a.o1[i] = v; a.o2[i] = w;
+  } else {  // GRAD_LAPLC
+    double vl = a.vol[i];
+    double fz = a.impfl ? a.coefw[i] : 1.0;
+    gx = a.dt * gx / vl; gy = a.dt * gy / vl; gz = a.dt * gz / vl;
+    gx = 0.5 * gx; gy = 0.5 * gy; gz = 0.5 * gz * fz;
+    bcv_node<double>(g, i, gx, gy, gz, a.ue[i] * 0.0);
+    double r = a.rh0[i];
+    a.o0[i] = gx * r; a.o1[i] = gy * r; a.o2[i] = gz * r;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Divergence of three fields (derivative(q1,.,1)+derivative(q2,.,2)+derivative(q3,.,3))
+// with optional fused inner products of the result against up to NFT_MAXD vectors.
+// ------------------------------------------------------------------------------------
+enum DivMode {
+  DIV_LAPLC = 0,   // out = ((ax+ay)+az)/(vol*rh0)                                (laplc 2nd half)
+  DIV_DIV = 1,     // out = iflg*((ax/vol+ay/vol)+az/vol)/rh0, inputs scaled by rh0 if iflg=-1
+};
+
+struct DotList {
+  int nd;                         // number of dot products a_k . out
+  const double *a[NFT_MAXD];      // nullptr means `out` itself
+  int fin;                        // finalize mode (see k_div)
+  int slot;                       // GCR slot for FIN_INIT / FIN_DEL
+  int nmax;                       // nonzero: also reduce max|out| (last entry)
+};
+
+enum Finalize { FIN_NONE = 0, FIN_BETA = 1, FIN_DEL = 2, FIN_NORMS = 3 };
+
+// scalar slots in the device scalar block
+enum Scal {
+  SC_BETA = 0, SC_RAX = 1, SC_DVMX = 2, SC_RL2 = 3, SC_BMX = 4, SC_EER = 5, SC_EEM = 6,
+  SC_R2 = 7, SC_RMX = 8, SC_TMP = 9,
+  SC_AX2 = 16,   // [16, 16+NFT_MAXD)
+  SC_DEL = 40,   // [40, 40+NFT_MAXD)
+  SC_TOTAL = 64
+};
+
+struct DivArgs {
+  const double *q0, *q1, *q2;
+  double *out;
+  const double *vol, *rh0;
+  int iflg;
+  DotList dots;
+  double *partials;
+  unsigned *counter;
+  double *scal;
+  double epa;
+};
+
+__device__ __forceinline__ void gcr_finalize(const DotList &d, const double *red, double *scal,
+                                             double epa) {
+  if (d.fin == FIN_BETA) {
+    // red[0] = <r, ax_s>, red[1] = <ax_s, ax_s>  (reference :1878-1885)
+    double ax2 = fmax(epa, red[1]);
+    scal[SC_AX2 + d.slot] = ax2;
+    scal[SC_RAX] = red[0];
+    scal[SC_BETA] = -red[0] / ax2;
+  } else if (d.fin == FIN_DEL) {
+    // red[ii] = <ax_ii, ar>, del_ii = -axar/ax2_ii  (reference :1906-1912)
+    for (int ii = 0; ii < d.nd; ++ii) scal[SC_DEL + ii] = -red[ii] / scal[SC_AX2 + ii];
+  } else if (d.fin == FIN_NORMS) {
+    scal[SC_R2] = red[0];
+    scal[SC_RMX] = red[NFT_MAXD];
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S, DivArgs a) {
+  __shared__ double sh[(NFT_MAXD + 1) * NFT_WPB];
+  Lane L;
+  bool active = lane_setup(g, L);
+  // lanes of blocks past the end still take part in the block reduction
+  double out = 0.;
+  if (active) {
+    const int i = L.i;
+    double ax = 0., ay = 0., az = 0.;
+    double r0 = (MODE == DIV_DIV && a.iflg == -1 && L.valid) ? a.rh0[i] : 1.0;
+    double qi0 = 0., qi1 = 0., qi2 = 0.;
+    if (L.valid) {
+      qi0 = a.q0[i]; qi1 = a.q1[i]; qi2 = a.q2[i];
+      if (MODE == DIV_DIV && a.iflg == -1) { qi0 = qi0 * r0; qi1 = qi1 * r0; qi2 = qi2 * r0; }
+    }
+    for (int m = 0; m < L.width; ++m) {
+      if (m < L.deg) {
+        int slot = L.base + m * NFT_WAVE + L.lane;
+        int j = g.nbr[slot], e = g.eid[slot];
+        bool first = e >= 0;
+        int ee = first ? e : ~e;
+        double qj0 = a.q0[j], qj1 = a.q1[j], qj2 = a.q2[j];
+        if (MODE == DIV_DIV && a.iflg == -1) {
+          double rj = a.rh0[j];
+          qj0 = qj0 * rj; qj1 = qj1 * rj; qj2 = qj2 * rj;
+        }
+        double u0 = (qi0 + qj0) * 0.5, u1 = (qi1 + qj1) * 0.5, u2 = (qi2 + qj2) * 0.5;
+        double sx = S.sx[ee], sy = S.sy[ee], sz = S.sz[ee];
+        if (first) { ax = ax + sx * u0; ay = ay + sy * u1; az = az + sz * u2; }
+        else { ax = ax - sx * u0; ay = ay - sy * u1; az = az - sz * u2; }
+      }
+    }
+    if (L.valid) {
+      for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
+        ax = ax + g.bp_sbb[b] * qi0;
+        ay = ay + g.bp_sbb[g.nbp + b] * qi1;
+        az = az + g.bp_sbb[2 * g.nbp + b] * qi2;
+      }
+      if (MODE == DIV_LAPLC) {
+        out = (ax + ay + az) / (a.vol[i] * a.rh0[i]);
+      } else {
+        double vl = a.vol[i];
+        out = (double)a.iflg * (ax / vl + ay / vl + az / vl) / a.rh0[i];
+      }
+      a.out[i] = out;
+    }
+    if (!L.valid) out = 0.;
+  }
+  if (a.dots.nd == 0 && a.dots.nmax == 0) return;
+  // fused reductions over the block grid (every block participates)
+  double v[NFT_MAXD + 1];
+#pragma unroll
+  for (int k = 0; k <= NFT_MAXD; ++k) v[k] = 0.;
+  if (active && L.valid) {
+#pragma unroll
+    for (int k = 0; k < NFT_MAXD; ++k)
+      if (k < a.dots.nd) v[k] = (a.dots.a[k] ? a.dots.a[k][L.i] : out) * out;
+    if (a.dots.nmax) v[NFT_MAXD] = fabs(out);
+  }
+  double red[NFT_MAXD + 1];
+  if (grid_reduce<NFT_MAXD + 1>(v, 1u << NFT_MAXD, a.partials, a.counter, gridDim.x, blockIdx.x, sh, red)) {
+    if (threadIdx.x == 0) gcr_finalize(a.dots, red, a.scal, a.epa);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Vector kernels of GCR (grid-stride, fixed grid -> deterministic partials)
+// ------------------------------------------------------------------------------------
+// p += beta*x; r += beta*ax; reduce max|r| and sum r^2   (reference :1886-1893)
+__global__ void __launch_bounds__(NFT_BLOCK) k_gcr_update(int n, double *p, double *r, const double *x,
+                                                          const double *ax, double *scal,
+                                                          double *partials, unsigned *counter) {
+  __shared__ double sh[2 * NFT_WPB];
+  const double beta = scal[SC_BETA];
+  double v[2] = {-1.e15, 0.};
+  for (int k = blockIdx.x * NFT_BLOCK + threadIdx.x; k < n; k += gridDim.x * NFT_BLOCK) {
+    double pk = p[k] + beta * x[k];
+    double rk = r[k] + beta * ax[k];
+    p[k] = pk;
+    r[k] = rk;
+    v[0] = fmax(v[0], fabs(rk));
+    v[1] = v[1] + rk * rk;
+  }
+  double red[2];
+  if (grid_reduce<2>(v, 1u, partials, counter, gridDim.x, blockIdx.x, sh, red)) {
+    if (threadIdx.x == 0) { scal[SC_DVMX] = red[0]; scal[SC_RL2] = red[1]; }
+  }
+}
+
+struct DirArgs {
+  int n, nx;                 // number of previous directions
+  double *xn, *axn;          // destination slot (may alias x[0]/ax[0] when collapsing)
+  const double *x[NFT_MAXD], *ax[NFT_MAXD];
+  const double *qr, *ar, *r;
+  double *scal, *partials;
+  unsigned *counter;
+  int slot;                  // slot index of xn
+  double epa;
+};
+
+// x_new = qr + sum del_ii x_ii ; ax_new = ar + sum del_ii ax_ii   (reference :1913-1936)
+// then <r, ax_new>, <ax_new, ax_new> -> beta for the next inner iteration
+__global__ void __launch_bounds__(NFT_BLOCK) k_gcr_dir(DirArgs a) {
+  __shared__ double sh[2 * NFT_WPB];
+  double del[NFT_MAXD];
+#pragma unroll
+  for (int ii = 0; ii < NFT_MAXD; ++ii) del[ii] = ii < a.nx ? a.scal[SC_DEL + ii] : 0.;
+  double v[2] = {0., 0.};
+  for (int k = blockIdx.x * NFT_BLOCK + threadIdx.x; k < a.n; k += gridDim.x * NFT_BLOCK) {
+    double xv = a.qr[k], av = a.ar[k];
+#pragma unroll
+    for (int ii = 0; ii < NFT_MAXD; ++ii) {
+      if (ii < a.nx) {
+        xv = xv + del[ii] * a.x[ii][k];
+        av = av + del[ii] * a.ax[ii][k];
+      }
+    }
+    a.xn[k] = xv;
+    a.axn[k] = av;
+    v[0] = v[0] + a.r[k] * av;
+    v[1] = v[1] + av * av;
+  }
+  double red[2];
+  if (grid_reduce<2>(v, 0u, a.partials, a.counter, gridDim.x, blockIdx.x, sh, red)) {
+    if (threadIdx.x == 0) {
+      double ax2 = fmax(a.epa, red[1]);
+      a.scal[SC_AX2 + a.slot] = ax2;
+      a.scal[SC_RAX] = red[0];
+      a.scal[SC_BETA] = -red[0] / ax2;
+    }
+  }
+}
+
+// sum a^2 and max|a| (and optionally copy a into dst)
+__global__ void __launch_bounds__(NFT_BLOCK) k_norms(int n, const double *a, double *dst, double *scal,
+                                                     int sum_slot, int max_slot, double *partials,
+                                                     unsigned *counter) {
+  __shared__ double sh[2 * NFT_WPB];
+  double v[2] = {0., -1.e15};
+  for (int k = blockIdx.x * NFT_BLOCK + threadIdx.x; k < n; k += gridDim.x * NFT_BLOCK) {
+    double x = a[k];
+    if (dst) dst[k] = x;
+    v[0] = v[0] + x * x;
+    v[1] = fmax(v[1], fabs(x));
+  }
+  double red[2];
+  if (grid_reduce<2>(v, 2u, partials, counter, gridDim.x, blockIdx.x, sh, red)) {
+    if (threadIdx.x == 0) { scal[sum_slot] = red[0]; scal[max_slot] = red[1]; }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Preconditioners
+// ------------------------------------------------------------------------------------
+// vertical line: forward/backward Thomas sweep per column, then p = p + omega*z
+// (it=1: p = 0 + omega*M^-1 rhs; it>1: rhs - L p is passed in res).
+template <typename T>
+__global__ void __launch_bounds__(NFT_BLOCK) k_line(int ncol, const int32_t *col_start, const T *la,
+                                                    const T *linv, const T *lc, const double *res,
+                                                    const double *pin, double *p, double omega,
+                                                    T *zbuf) {
+  int c = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (c >= ncol) return;
+  int s0 = col_start[c], s1 = col_start[c + 1];
+  T dprev = (T)0;
+  for (int i = s0; i < s1; ++i) {
+    T d = ((T)res[i] - la[i] * dprev) * linv[i];
+    zbuf[i] = d;
+    dprev = d;
+  }
+  T znext = (T)0;
+  for (int i = s1 - 1; i >= s0; --i) {
+    T z = (i == s1 - 1) ? zbuf[i] : zbuf[i] - lc[i] * znext;
+    znext = z;
+    double p0 = pin ? pin[i] : 0.;
+    p[i] = p0 + omega * (double)z;
+  }
+}
+
+// point relaxation: p = p + omega*pstep*(r - rhs)   (reference :607-612)
+__global__ void __launch_bounds__(NFT_BLOCK) k_point(int n, const double *pstep, const double *r,
+                                                     const double *rhs, double *p, double omega,
+                                                     int first) {
+  int k = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (k >= n) return;
+  double pk = first ? 0. : p[k];
+  double rk = first ? 0. : r[k];
+  p[k] = pk + omega * pstep[k] * (rk - rhs[k]);
+}
+
+__global__ void k_sub(int n, const double *a, const double *b, double *c) {
+  int k = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (k < n) c[k] = a[k] - b[k];
+}
+
+__global__ void k_copy(int n, const double *a, double *b) {
+  int k = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (k < n) b[k] = a[k];
+}
+
+__global__ void k_fill(int n, double *a, double v) {
+  int k = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (k < n) a[k] = v;
+}
+
+// bcv on stand-alone arrays
+__global__ void k_bcv(Incidence g, double *u, double *v, double *w, const double *ue, double flag) {
+  int i = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (i >= g.n) return;
+  double a = u[i], b = v[i], c = w[i];
+  bcv_node<double>(g, i, a, b, c, ue[i] * flag);
+  u[i] = a; v[i] = b; w[i] = c;
+}
+
+// ------------------------------------------------------------------------------------
+// MPDATA (reference mpdatam :3029-3311), NF fields advected by one momentum field.
+// T is the arithmetic type of fluxes, antidiffusive velocities and limiter; the fields
+// themselves stay FP64 and are updated by increments computed in T.
+// ------------------------------------------------------------------------------------
+template <typename T, int NF>
+struct MpArgs {
+  const T *U, *V, *W;              // advecting momentum uuu
+  const double *psi[NF];           // fields in (pass 1) / donor-cell fields (pass 2)
+  double *psi1[NF];                // donor-cell result / final result
+  T *gx[NF], *gy[NF], *gz[NF];     // gradients of the donor-cell fields
+  T *cp[NF], *cn[NF];              // limiter ratios
+  const T *vol, *rh0;              // T copies
+  const double *ue;
+  T afree[NF];                     // inflow value per field (0 => use ue)
+  int use_ue[NF];
+  T dt;
+};
+
+template <typename T>
+__device__ __forceinline__ T edge_aun(T U1, T U2, T V1, T V2, T W1, T W2, T sx, T sy, T sz) {
+  return (U1 + U2) * (T)0.5 * sx + (V1 + V2) * (T)0.5 * sy + (W1 + W2) * (T)0.5 * sz;
+}
+
+template <typename T>
+__device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+template <typename T>
+__device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+
+// pass 1: donor-cell fluxes, boundary closure and update (reference :3094-3174)
+template <typename T, int NF>
+__global__ void __launch_bounds__(NFT_BLOCK) k_mp_donor(Incidence g, EdgeS<T> S, MpArgs<T, NF> a) {
+  Lane L;
+  if (!lane_setup(g, L) || !L.valid) return;
+  const int i = L.i;
+  T Ui = a.U[i], Vi = a.V[i], Wi = a.W[i];
+  T pi[NF], arh2[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) { pi[f] = (T)a.psi[f][i]; arh2[f] = (T)0; }
+  T divv = (T)0;
+  for (int m = 0; m < L.deg; ++m) {
+    int slot = L.base + m * NFT_WAVE + L.lane;
+    int j = g.nbr[slot], e = g.eid[slot];
+    bool first = e >= 0;
+    int ee = first ? e : ~e;
+    T Uj = a.U[j], Vj = a.V[j], Wj = a.W[j];
+    T sx = S.sx[ee], sy = S.sy[ee], sz = S.sz[ee];
+    T aun = first ? edge_aun<T>(Ui, Uj, Vi, Vj, Wi, Wj, sx, sy, sz) : edge_aun<T>(Uj, Ui, Vj, Vi, Wj, Wi, sx, sy, sz);
+    T ap = tmax<T>((T)0, aun), an = tmin<T>((T)0, aun);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      T pj = (T)a.psi[f][j];
+      T fl = first ? pi[f] * ap + pj * an : pj * ap + pi[f] * an;
+      arh2[f] = first ? arh2[f] + fl : arh2[f] - fl;
+    }
+    divv = first ? divv + aun : divv - aun;
+  }
+  for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
+    int fl_ = g.bp_flag[b];
+    T aunbb1 = -divv;
+    bool slip = (fl_ == 11 || fl_ == 5 || fl_ == 6);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      T af = a.use_ue[f] ? (T)a.ue[i] : a.afree[f];
+      T fl1 = slip ? (T)0 : af * tmin<T>((T)0, aunbb1) + pi[f] * tmax<T>((T)0, aunbb1);
+      arh2[f] = arh2[f] + fl1 * (T)1;
+    }
+  }
+  T vl = a.vol[i], rh = a.rh0[i];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    T inc = arh2[f] * a.dt / vl / rh;
+    a.psi1[f][i] = a.psi[f][i] - (double)inc;
+  }
+}
+
+// gradients of the donor-cell fields (derivative x3 per field, reference :3182-3184)
+template <typename T, int NF>
+__global__ void __launch_bounds__(NFT_BLOCK) k_mp_grad(Incidence g, EdgeS<T> S, MpArgs<T, NF> a) {
+  Lane L;
+  if (!lane_setup(g, L) || !L.valid) return;
+  const int i = L.i;
+  T pi[NF], gx[NF], gy[NF], gz[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) { pi[f] = (T)a.psi1[f][i]; gx[f] = gy[f] = gz[f] = (T)0; }
+  for (int m = 0; m < L.deg; ++m) {
+    int slot = L.base + m * NFT_WAVE + L.lane;
+    int j = g.nbr[slot], e = g.eid[slot];
+    bool first = e >= 0;
+    int ee = first ? e : ~e;
+    T sx = S.sx[ee], sy = S.sy[ee], sz = S.sz[ee];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      T u = (pi[f] + (T)a.psi1[f][j]) * (T)0.5;
+      if (first) { gx[f] = gx[f] + sx * u; gy[f] = gy[f] + sy * u; gz[f] = gz[f] + sz * u; }
+      else { gx[f] = gx[f] - sx * u; gy[f] = gy[f] - sy * u; gz[f] = gz[f] - sz * u; }
+    }
+  }
+  for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
+    T bx = (T)g.bp_sbb[b], by = (T)g.bp_sbb[g.nbp + b], bz = (T)g.bp_sbb[2 * g.nbp + b];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) { gx[f] = gx[f] + bx * pi[f]; gy[f] = gy[f] + by * pi[f]; gz[f] = gz[f] + bz * pi[f]; }
+  }
+#pragma unroll
+  for (int f = 0; f < NF; ++f) { a.gx[f][i] = gx[f]; a.gy[f][i] = gy[f]; a.gz[f][i] = gz[f]; }
+}
+
+// antidiffusive velocity of one edge for field f, evaluated with the edge's own (i1,i2)
+// orientation so both end nodes obtain identical bits (reference :3187-3204)
+template <typename T, int NF>
+__device__ __forceinline__ T anti_flux(const MpArgs<T, NF> &a, int f, int i1, int i2, T aun, T volt,
+                                       T U12, T V12, T W12) {
+  T p1 = (T)a.psi1[f][i1], p2 = (T)a.psi1[f][i2];
+  T un = fabs(aun) * (p2 - p1) * (T)0.5;
+  T dpdx = (T)0.5 * (a.gx[f][i1] + a.gx[f][i2]) / volt;
+  T dpdy = (T)0.5 * (a.gy[f][i1] + a.gy[f][i2]) / volt;
+  T dpdz = (T)0.5 * (a.gz[f][i1] + a.gz[f][i2]) / volt;
+  T term = U12 * (T)0.5 * dpdx + V12 * (T)0.5 * dpdy + W12 * (T)0.5 * dpdz;
+  return un - aun * term * a.dt * (T)0.5;
+}
+
+// FCT bounds from the donor-cell solution, in/out fluxes and ratios (reference :3228-3280)
+template <typename T, int NF>
+__global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, EdgeS<T> S, MpArgs<T, NF> a) {
+  Lane L;
+  if (!lane_setup(g, L) || !L.valid) return;
+  const int i = L.i;
+  T Ui = a.U[i], Vi = a.V[i], Wi = a.W[i];
+  T vri = a.vol[i] * a.rh0[i];
+  T mx[NF], mn[NF], rin[NF], rout[NF], pi[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    mx[f] = (T)-1.e15; mn[f] = (T)1.e15; rin[f] = (T)0; rout[f] = (T)0;
+    pi[f] = (T)a.psi1[f][i];
+  }
+  for (int m = 0; m < L.deg; ++m) {
+    int slot = L.base + m * NFT_WAVE + L.lane;
+    int j = g.nbr[slot], e = g.eid[slot];
+    bool first = e >= 0;
+    int ee = first ? e : ~e;
+    int i1 = first ? i : j, i2 = first ? j : i;
+    T Uj = a.U[j], Vj = a.V[j], Wj = a.W[j];
+    T sx = S.sx[ee], sy = S.sy[ee], sz = S.sz[ee];
+    T aun = first ? edge_aun<T>(Ui, Uj, Vi, Vj, Wi, Wj, sx, sy, sz) : edge_aun<T>(Uj, Ui, Vj, Vi, Wj, Wi, sx, sy, sz);
+    T vrj = a.vol[j] * a.rh0[j];
+    T volt = (T)0.5 * (first ? vri + vrj : vrj + vri);
+    T U12 = Ui + Uj, V12 = Vi + Vj, W12 = Wi + Wj;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      T A = anti_flux<T, NF>(a, f, i1, i2, aun, volt, U12, V12, W12);
+      T pj = (T)a.psi1[f][j];
+      mx[f] = tmax<T>(tmax<T>(mx[f], pi[f]), pj);
+      mn[f] = tmin<T>(tmin<T>(mn[f], pi[f]), pj);
+      T pp = tmax<T>((T)0, A), pn = -tmin<T>((T)0, A);
+      if (first) { rin[f] = rin[f] + pn; rout[f] = rout[f] + pp; }
+      else { rin[f] = rin[f] + pp; rout[f] = rout[f] + pn; }
+    }
+  }
+  T vl = a.vol[i], rh = a.rh0[i];
+  const T tol = (T)1.e-10;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    a.cp[f][i] = (mx[f] - pi[f]) * vl * rh / (rin[f] * a.dt + tol);
+    a.cn[f][i] = (pi[f] - mn[f]) * vl * rh / (rout[f] * a.dt + tol);
+  }
+}
+
+// limited antidiffusive fluxes and the second update (reference :3284-3302, :3098-3174)
+template <typename T, int NF>
+__global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, EdgeS<T> S, MpArgs<T, NF> a,
+                                                        double *const *out) {
+  Lane L;
+  if (!lane_setup(g, L) || !L.valid) return;
+  const int i = L.i;
+  T Ui = a.U[i], Vi = a.V[i], Wi = a.W[i];
+  T vri = a.vol[i] * a.rh0[i];
+  T arh2[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) arh2[f] = (T)0;
+  for (int m = 0; m < L.deg; ++m) {
+    int slot = L.base + m * NFT_WAVE + L.lane;
+    int j = g.nbr[slot], e = g.eid[slot];
+    bool first = e >= 0;
+    int ee = first ? e : ~e;
+    int i1 = first ? i : j, i2 = first ? j : i;
+    T Uj = a.U[j], Vj = a.V[j], Wj = a.W[j];
+    T sx = S.sx[ee], sy = S.sy[ee], sz = S.sz[ee];
+    T aun = first ? edge_aun<T>(Ui, Uj, Vi, Vj, Wi, Wj, sx, sy, sz) : edge_aun<T>(Uj, Ui, Vj, Vi, Wj, Wi, sx, sy, sz);
+    T vrj = a.vol[j] * a.rh0[j];
+    T volt = (T)0.5 * (first ? vri + vrj : vrj + vri);
+    T U12 = Ui + Uj, V12 = Vi + Vj, W12 = Wi + Wj;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      T A = anti_flux<T, NF>(a, f, i1, i2, aun, volt, U12, V12, W12);
+      T lim = A > (T)0 ? tmin<T>(tmin<T>((T)1, a.cp[f][i2]), a.cn[f][i1])
+                       : tmin<T>(tmin<T>((T)1, a.cn[f][i2]), a.cp[f][i1]);
+      T fl = A * lim;
+      arh2[f] = first ? arh2[f] + fl : arh2[f] - fl;
+    }
+  }
+  for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) arh2[f] = arh2[f] + (T)0 * (T)1;
+  }
+  T vl = a.vol[i], rh = a.rh0[i];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    T inc = arh2[f] * a.dt / vl / rh;
+    out[f][i] = a.psi1[f][i] - (double)inc;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Pointwise loops of the time step (reference mpsolver)
+// ------------------------------------------------------------------------------------
+struct StepArgs {
+  int n;
+  double *unk[4], *frc[4], *uuu[3], *uuu0[3], *ud[3];
+  const double *alpha, *ue, *ve, *th0, *coefw, *rh0, *dthedz;
+  double dt, g;
+  float *uuuf[3];  // FP32 copy of the advector (mixed precision), may be null
+};
+
+// :290-301  uuu0 = uuu; unk += 0.5*frc*dt
+__global__ void k_step_pre(StepArgs a) {
+  int i = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (i >= a.n) return;
+  for (int k = 0; k < 3; ++k) a.uuu0[k][i] = a.uuu[k][i];
+  for (int k = 0; k < 4; ++k) a.unk[k][i] = a.unk[k][i] + 0.5 * a.frc[k][i] * a.dt;
+}
+
+// :324-359  explicit part of the trapezoidal step; frc(1..3) <- u*, which is also unk1
+__global__ void k_step_rhs(StepArgs a) {
+  int i = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (i >= a.n) return;
+  double al = a.alpha[i];
+  double frm1 = 0. - al * (a.ud[0][i] - a.ue[i]);
+  double frm2 = 0. - al * (a.ud[1][i] - a.ve[i]);
+  double frm3 = (0. - al * a.ud[2][i]) + a.g * a.unk[3][i] / a.th0[i];
+  double h = 0.5 * a.dt;
+  a.frc[0][i] = a.unk[0][i] + h * frm1;
+  a.frc[1][i] = a.unk[1][i] + h * frm2;
+  a.frc[2][i] = (a.unk[2][i] + h * frm3) * a.coefw[i];
+}
+
+// :367-383  after the implicit solve (last: forcing and theta update)
+__global__ void k_step_post(StepArgs a, int last) {
+  int i = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (i >= a.n) return;
+  double d0 = a.uuu[0][i], d1 = a.uuu[1][i], d2 = a.uuu[2][i];
+  a.ud[0][i] = d0; a.ud[1][i] = d1; a.ud[2][i] = d2;
+  if (last) {
+    double dt = a.dt;
+    a.frc[0][i] = (d0 - a.frc[0][i]) * 2. / dt;
+    a.frc[1][i] = (d1 - a.frc[1][i]) * 2. / dt;
+    double frc33 = a.frc[2][i] / a.coefw[i] - (a.g * a.unk[3][i] / a.th0[i]) * 0.5 * dt;
+    a.frc[2][i] = (d2 - frc33) * 2. / dt;
+    a.unk[0][i] = d0; a.unk[1][i] = d1; a.unk[2][i] = d2;
+    double f4 = -(d2 * a.dthedz[i]);
+    a.frc[3][i] = f4;
+    a.unk[3][i] = a.unk[3][i] + 0.5 * dt * f4;
+  }
+}
+
+// :387-439  explicit forcing for the next step and advector extrapolation
+__global__ void k_step_end(StepArgs a) {
+  int i = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (i >= a.n) return;
+  double al = a.alpha[i];
+  double u0 = a.unk[0][i], u1 = a.unk[1][i], u2 = a.unk[2][i], u3 = a.unk[3][i];
+  a.frc[0][i] = a.frc[0][i] + (0. - al * (u0 - a.ue[i]));
+  a.frc[1][i] = a.frc[1][i] + (0. - al * (u1 - a.ve[i]));
+  a.frc[2][i] = a.frc[2][i] + (0. - al * u2);
+  a.frc[3][i] = a.frc[3][i] + (-(al * u3));
+  double rh = a.rh0[i];
+  double rhoi = 1. / rh;
+  for (int k = 0; k < 3; ++k) {
+    double uu = a.uuu[k][i], u0k = a.uuu0[k][i];
+    double nu = 1.5 * uu * rh - 0.5 * u0k;
+    a.uuu[k][i] = nu;
+    if (a.uuuf[0]) a.uuuf[k][i] = (float)nu;
+    double o = u0k * rhoi;
+    a.uuu0[k][i] = o;
+    a.ud[k][i] = 2. * a.unk[k][i] - o;
+  }
+}
+
+// mpsolver initialisation after the potential-flow projection (:229-266)
+__global__ void k_init_post(StepArgs a) {
+  int i = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (i >= a.n) return;
+  double rh = a.rh0[i];
+  for (int k = 0; k < 3; ++k) {
+    double u = a.unk[k][i];
+    double m = u * rh;
+    a.uuu[k][i] = m;
+    if (a.uuuf[0]) a.uuuf[k][i] = (float)m;
+    a.ud[k][i] = u;
+  }
+  for (int k = 0; k < 4; ++k) a.frc[k][i] = 0.;
+}
+
+template <typename T>
+__global__ void k_to_t(int n, const double *a, T *b) {
+  int k = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (k < n) b[k] = (T)a[k];
+}
+
+}  // namespace nft

@@ -1,0 +1,262 @@
+"""ctypes front end of libnft_hip.so (the gfx950 NFT step; C ABI in include/nft.h).
+
+Mirrors the reference subroutines (lippsLFroude_implgwopks3D_unstr3.f) with the same
+names and argument meaning: derivative, laplc, prfr, bcv, div, precon, gcrk, mpdatam,
+plus the device-resident mpsolver loop (state_init / run_steps). Arrays are numpy
+float64 node arrays. There is no CPU fallback: without the built library or a GPU every
+call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "lib", "libnft_hip.so")
+_lib = None
+
+i32p = ctypes.POINTER(ctypes.c_int32)
+f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class NftMesh(ctypes.Structure):
+    _fields_ = [("npoin", ctypes.c_int), ("nedge", ctypes.c_int), ("nbface", ctypes.c_int),
+                ("nboun", ctypes.c_int), ("ld_poin", ctypes.c_int), ("ld_edge", ctypes.c_int),
+                ("ld_face", ctypes.c_int), ("iedge", i32p), ("sn", f64p), ("kbface", i32p),
+                ("sb", f64p), ("ibpoin", i32p), ("sbb", f64p), ("anp", f64p), ("coord", f64p),
+                ("vol", f64p), ("ncol", ctypes.c_int), ("col_start", i32p)]
+
+
+class NftParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("u0", ctypes.c_double), ("v0", ctypes.c_double),
+                ("w0", ctypes.c_double), ("strat", ctypes.c_double), ("theta0", ctypes.c_double),
+                ("lord", ctypes.c_int), ("itr", ctypes.c_int), ("exit_mode", ctypes.c_int),
+                ("eps0", ctypes.c_double), ("precon", ctypes.c_int), ("sm_flag", ctypes.c_int),
+                ("itrp1", ctypes.c_int), ("omega", ctypes.c_double),
+                ("dxabl", ctypes.c_double), ("dxabr", ctypes.c_double), ("dyabd", ctypes.c_double),
+                ("dyabu", ctypes.c_double), ("dzabd", ctypes.c_double), ("dzabu", ctypes.c_double),
+                ("towxl", ctypes.c_double), ("towxr", ctypes.c_double), ("towyd", ctypes.c_double),
+                ("towyu", ctypes.c_double), ("towzd", ctypes.c_double), ("towzu", ctypes.c_double),
+                ("irl", ctypes.c_int), ("dom", ctypes.c_double * 6), ("precision", ctypes.c_int)]
+
+
+class GcrStats(ctypes.Structure):
+    _fields_ = [("niter", ctypes.c_int), ("nitsm", ctypes.c_int), ("icount", ctypes.c_int),
+                ("converged", ctypes.c_int), ("eer", ctypes.c_double), ("eem", ctypes.c_double),
+                ("res_max", ctypes.c_double), ("tol", ctypes.c_double)]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int), ("gcr_solves", ctypes.c_int), ("gcr_iters", ctypes.c_int),
+                ("gcr_unconverged", ctypes.c_int), ("ms_total", ctypes.c_double),
+                ("ms_gcr", ctypes.c_double), ("ms_mpdata", ctypes.c_double)]
+
+
+# Every function the header declares, with its ctypes signature.
+_SIGS = {
+    "nft_init": (ctypes.c_int, [ctypes.POINTER(NftMesh), ctypes.POINTER(NftParams),
+                                ctypes.POINTER(ctypes.c_void_p)]),
+    "nft_finalize": (ctypes.c_int, [ctypes.c_void_p]),
+    "nft_last_error": (ctypes.c_char_p, []),
+    "nft_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+    "nft_derivative": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, ctypes.c_int]),
+    "nft_laplc": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, ctypes.c_int]),
+    "nft_prfr": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p, f64p, ctypes.c_int]),
+    "nft_bcv": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, f64p, ctypes.c_double]),
+    "nft_div": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, f64p, f64p, ctypes.c_int]),
+    "nft_precon": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, ctypes.c_int]),
+    "nft_gcrk": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p, f64p, ctypes.c_int,
+                                ctypes.POINTER(GcrStats)]),
+    "nft_mpdatam": (ctypes.c_int, [ctypes.c_void_p, f64p, ctypes.c_int, f64p, ctypes.c_int]),
+    "nft_state_init": (ctypes.c_int, [ctypes.c_void_p]),
+    "nft_run_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(RunStats)]),
+    "nft_get_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, f64p, ctypes.c_int]),
+    "nft_set_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, f64p, ctypes.c_int]),
+    "nft_get_field": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, f64p]),
+    "nft_get_gcr_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GcrStats)]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} is missing: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIBPATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class NftError(RuntimeError):
+    pass
+
+
+def _chk(rc, what):
+    if rc < 0:
+        raise NftError(f"{what} failed ({rc}): {lib().nft_last_error().decode()}")
+    return rc
+
+
+def _p(a):
+    assert a.flags["C_CONTIGUOUS"], "array must be contiguous"
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(f64p)
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(i32p)
+    raise TypeError(a.dtype)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+PRECISION = {"fp64": 0, "mixed": 1}
+
+
+def make_params(cfg, precon=None, sm_flag=1, itrp1=None, omega=None, eps0=None, exit_mode=None,
+                itr=None, lord=None, precision="fp64", dom=(0.0, 15000.0, -6000.0, 6000.0, 0.0, 6000.0)):
+    p = NftParams()
+    p.dt = cfg.dt; p.u0 = cfg.u0; p.v0 = 0.0; p.w0 = 0.0
+    p.strat = cfg.strat; p.theta0 = cfg.theta0
+    p.lord = cfg.lord if lord is None else lord
+    p.itr = cfg.itr if itr is None else itr
+    p.exit_mode = cfg.exit_mode if exit_mode is None else exit_mode
+    p.eps0 = cfg.eps0 if eps0 is None else eps0
+    p.precon = cfg.precon if precon is None else precon
+    p.sm_flag = sm_flag
+    p.itrp1 = cfg.itrp1 if itrp1 is None else itrp1
+    p.omega = cfg.omega if omega is None else omega
+    p.dzabu = cfg.sponge_depth
+    p.towzu = cfg.sponge_tau
+    p.irl = 1
+    for i, v in enumerate(dom):
+        p.dom[i] = v
+    p.precision = PRECISION[precision] if isinstance(precision, str) else int(precision)
+    return p
+
+
+class Solver:
+    """One device context: the mesh, set-up fields and the time-step state on the GPU."""
+
+    def __init__(self, mesh, params: NftParams, use_columns=True):
+        L = lib()
+        self.n = mesh.npoin
+        self._keep = [np.ascontiguousarray(a) for a in
+                      (mesh.iedge, mesh.sn, mesh.kbface, mesh.sb, mesh.ibpoin, mesh.sbb, mesh.anp,
+                       mesh.coord, mesh.vol, mesh.col_start())]
+        k = self._keep
+        ldf = max(mesh.nbface, mesh.nboun, 1)
+        # boundary arrays share the reference's nfacesx leading dimension; pad to ldf
+        kb = np.zeros((2, ldf), np.int32); kb[:, :mesh.nbface] = mesh.kbface
+        sb = np.zeros((3, ldf)); sb[:, :mesh.nbface] = mesh.sb
+        ib = np.zeros((2, ldf), np.int32); ib[:, :mesh.nboun] = mesh.ibpoin
+        sbb = np.zeros((3, ldf)); sbb[:, :mesh.nboun] = mesh.sbb
+        anp = np.zeros((3, ldf)); anp[:, :mesh.nboun] = mesh.anp
+        self._keep += [kb, sb, ib, sbb, anp]
+        m = NftMesh(mesh.npoin, mesh.nedge, mesh.nbface, mesh.nboun, mesh.npoin, max(mesh.nedge, 1), ldf,
+                    _p(k[0]), _p(k[1]), _p(kb), _p(sb), _p(ib), _p(sbb), _p(anp), _p(k[7]), _p(k[8]),
+                    mesh.nh if use_columns else 0, _p(k[9]))
+        self.params = params
+        h = ctypes.c_void_p()
+        _chk(L.nft_init(ctypes.byref(m), ctypes.byref(params), ctypes.byref(h)), "nft_init")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nft_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- reference subroutines -------------------------------------------------------
+    def derivative(self, unk, idx):
+        out = np.zeros(self.n)
+        _chk(lib().nft_derivative(self.h, _p(_f64(unk)), _p(out), idx), "nft_derivative")
+        return out
+
+    def laplc(self, p, impfl):
+        out = np.zeros(self.n)
+        _chk(lib().nft_laplc(self.h, _p(_f64(p)), _p(out), impfl), "nft_laplc")
+        return out
+
+    def prfr(self, p, us, vs, ws, impfl):
+        outs = [np.zeros(self.n) for _ in range(3)]
+        ins = [_f64(a) for a in (p, us, vs, ws)]
+        _chk(lib().nft_prfr(self.h, *[_p(a) for a in ins], *[_p(a) for a in outs], impfl), "nft_prfr")
+        return outs
+
+    def bcv(self, u, v, w, flag):
+        u, v, w = (np.array(a, dtype=np.float64) for a in (u, v, w))
+        _chk(lib().nft_bcv(self.h, _p(u), _p(v), _p(w), float(flag)), "nft_bcv")
+        return u, v, w
+
+    def div(self, u, v, w, iflg):
+        out = np.zeros(self.n)
+        ins = [_f64(a) for a in (u, v, w)]
+        _chk(lib().nft_div(self.h, *[_p(a) for a in ins], _p(out), iflg), "nft_div")
+        return out
+
+    def precon(self, rhs, impfl):
+        p = np.zeros(self.n)
+        _chk(lib().nft_precon(self.h, _p(_f64(rhs)), _p(p), impfl), "nft_precon")
+        return p
+
+    def gcrk(self, p, u1, v1, w1, impfl):
+        p = np.array(p, dtype=np.float64)
+        outs = [np.zeros(self.n) for _ in range(3)]
+        ins = [_f64(a) for a in (u1, v1, w1)]
+        st = GcrStats()
+        rc = _chk(lib().nft_gcrk(self.h, _p(p), *[_p(a) for a in ins], *[_p(a) for a in outs], impfl,
+                                 ctypes.byref(st)), "nft_gcrk")
+        return p, outs, st, rc
+
+    def mpdatam(self, uuu, unk, iflag):
+        unk = np.array(unk, dtype=np.float64)
+        uuu = _f64(uuu)
+        _chk(lib().nft_mpdatam(self.h, _p(uuu), uuu.shape[1], _p(unk), iflag), "nft_mpdatam")
+        return unk
+
+    # ---- device-resident stepping ----------------------------------------------------
+    def state_init(self):
+        _chk(lib().nft_state_init(self.h), "nft_state_init")
+
+    def run_steps(self, n):
+        st = RunStats()
+        _chk(lib().nft_run_steps(self.h, n, ctypes.byref(st)), "nft_run_steps")
+        return st
+
+    def state(self, name):
+        k = {"unk": 4, "frc": 4, "uuu": 3, "unkdash": 3, "prpot": 1}[name]
+        a = np.zeros((k, self.n))
+        _chk(lib().nft_get_state(self.h, name.encode(), _p(a), self.n), "nft_get_state")
+        return a if k > 1 else a[0]
+
+    def set_state(self, name, arr):
+        a = np.ascontiguousarray(np.atleast_2d(arr), dtype=np.float64)
+        _chk(lib().nft_set_state(self.h, name.encode(), _p(a), self.n), "nft_set_state")
+
+    def field(self, name):
+        a = np.zeros(self.n)
+        _chk(lib().nft_get_field(self.h, name.encode(), _p(a)), "nft_get_field")
+        return a
+
+    def gcr_stats(self):
+        st = GcrStats()
+        _chk(lib().nft_get_gcr_stats(self.h, ctypes.byref(st)), "nft_get_gcr_stats")
+        return st
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    _chk(lib().nft_device_name(buf, 256), "nft_device_name")
+    return buf.value.decode()

@@ -1,0 +1,54 @@
+"""Shared configurations and helpers of the test suite."""
+import functools
+
+import numpy as np
+
+from src_amd import mesh as M
+
+
+def tiny_cfg(**kw):
+    base = dict(u0=10.0, bvf=0.01, dt=10.0, steps=10, lord=3, eps0=1e-7, exit_mode=0, itr=30, precon=1,
+                itrp1=2, omega=0.5, sponge_depth=500.0, sponge_tau=300.0)
+    base.update(kw)
+    return M.RunConfig("tiny", M.tiny_spec(), **base)
+
+
+def small_cfg(**kw):
+    c = M.CONFIGS["small"]
+    base = dict(u0=c.u0, bvf=c.bvf, dt=c.dt, steps=100, lord=3, eps0=1e-8, exit_mode=0, itr=40, precon=1,
+                itrp1=2, omega=0.5, sponge_depth=3000.0, sponge_tau=300.0)
+    base.update(kw)
+    return M.RunConfig("small", c.mesh, **base)
+
+
+@functools.lru_cache(maxsize=None)
+def mesh_for(spec_key):
+    spec = {"tiny": M.tiny_spec(), "small": M.small_spec(), "tiny2": M.tiny_spec(nx=9, ny=7, nlev=6, seed=3)}[spec_key]
+    return M.generate(spec)
+
+
+def seeded_fields(m, seed=11):
+    rng = np.random.default_rng(seed)
+    n = m.npoin
+    x, y, z = m.coord
+    p = 50.0 * np.sin(x / 700.0) * np.cos(y / 900.0) + 10.0 * z / 1000.0 + rng.normal(0, 1.0, n)
+    u = 10.0 + rng.normal(0, 1.0, n)
+    v = rng.normal(0, 1.0, n)
+    w = rng.normal(0, 0.5, n)
+    r = rng.normal(0, 1e-3, n)
+    U = 10.0 + rng.normal(0, 0.5, n)
+    V = rng.normal(0, 0.5, n)
+    W = rng.normal(0, 0.2, n)
+    return dict(p=p, u=u, v=v, w=w, r=r, uuu=np.ascontiguousarray(np.stack([U, V, W])))
+
+
+def mesh_checksum(m):
+    return np.array([m.npoin, m.nedge, m.nbface, m.nboun, m.iedge.sum(), m.kbface.sum(),
+                     m.ibpoin.sum(), m.sn.sum(), np.abs(m.sn).sum(), m.sb.sum(), m.vol.sum(),
+                     m.sbb.sum(), m.coord.sum()], dtype=np.float64)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64); b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / nb if nb > 0 else np.linalg.norm(a - b)

@@ -43,7 +43,7 @@ def tiny_config():
 def small_config():
     c = M.CONFIGS["small"]
     return M.RunConfig("small", c.mesh, u0=c.u0, bvf=c.bvf, dt=c.dt, steps=100, lord=3,
-                       eps0=1e-8, exit_mode=0, itr=100, precon=1, itrp1=2, omega=0.5,
+                       eps0=1e-8, exit_mode=0, itr=40, precon=1, itrp1=2, omega=0.5,
                        sponge_depth=3000.0, sponge_tau=300.0)
 
 

@@ -1,0 +1,40 @@
+"""The C ABI library: built, loadable, and exporting what include/nft.h declares."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from src_amd import nft
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    with open(os.path.join(ROOT, "include", "nft.h")) as f:
+        txt = f.read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(nft_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_header_and_wrapper_agree():
+    assert set(declared()) == set(nft._SIGS)
+
+
+def test_library_exports_every_symbol():
+    lib = ctypes.CDLL(nft.LIBPATH)
+    for name in declared():
+        assert hasattr(lib, name), name
+
+
+def test_library_is_gfx950():
+    with open(nft.LIBPATH, "rb") as f:
+        blob = f.read()
+    assert b"gfx950" in blob
+
+
+def test_init_rejects_bad_arguments_without_touching_a_gpu():
+    L = nft.lib()
+    h = ctypes.c_void_p()
+    rc = L.nft_init(None, None, ctypes.byref(h))
+    assert rc < 0 and b"null" in L.nft_last_error()
