@@ -130,6 +130,16 @@ int nft_set_state(nft_ctx *ctx, const char *name, const double *host, int ld);
 int nft_get_field(nft_ctx *ctx, const char *name, double *host);
 int nft_get_gcr_stats(nft_ctx *ctx, nft_gcr_stats *stats);
 
+/* ---- measurement (no reference counterpart: the reference only times the whole run) -- */
+/* Kernel ids for the timer: 1 laplc gradient half, 2 laplc divergence half, 3 line solve,
+ * 4 GCR update, 5 GCR direction, 6..9 MPDATA donor/gradient/FCT/limit. While a kernel id
+ * is selected, every launch of that kernel is bracketed by HIP events on the context's
+ * stream. bytes_per_launch is the kernel's algorithmic traffic: every array it reads or
+ * writes counted once at its stored width (incidence slots actually used, edge metrics,
+ * node arrays). */
+int nft_kernel_timer(nft_ctx *ctx, int kernel);
+int nft_kernel_timer_read(nft_ctx *ctx, int *launches, double *total_ms, double *bytes_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

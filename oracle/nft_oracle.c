@@ -29,6 +29,9 @@ struct oc_ctx {
   /* nft state */
   double *unk, *frc, *uuu, *unkdash, *prpot, *unk1, *uuu0;
   oc_stats st;
+  /* launch geometry of the HIP path (nft_init) for dot_order = 1 */
+  int nchunk, nblocks, xcd_per, ggrid, vgrid;
+  double *part;
 };
 
 static double amax1(double a, double b) { return a > b ? a : b; }
@@ -162,14 +165,87 @@ oc_ctx *oc_create(const oc_mesh *m, const oc_params *p) {
     c->coefw[i] = 1. / cw;
   }
   build_line(c);
+  /* launch geometry of the HIP path (src_amd/csrc/nft_capi.hip, nft_init) */
+  c->nchunk = (n + 63) / 64;
+  c->nblocks = (c->nchunk + 3) / 4;
+  if (c->nblocks >= 64) { c->xcd_per = (c->nblocks + 7) / 8; c->ggrid = 8 * c->xcd_per; }
+  else { c->xcd_per = 0; c->ggrid = c->nblocks; }
+  c->vgrid = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
+  c->part = zalloc((size_t)(c->ggrid > c->vgrid ? c->ggrid : c->vgrid));
   return c;
+}
+
+/* ---- the HIP path's reduction tree (dot_order = 1) --------------------------------
+ * wave: lane l += lane l+off for off = 32,16,...,1 (shfl_down), result in lane 0;
+ * block: ((w0 + w1) + w2) + w3 over its four wavefronts;
+ * grid: thread t of the last block sums partials t, t+256, ... in order, then the block
+ * tree again. Inactive lanes hold 0. */
+static double tree64(double *v) {
+  for (int off = 32; off > 0; off >>= 1)
+    for (int l = 0; l < off; ++l) v[l] = v[l] + v[l + off];
+  return v[0];
+}
+
+static double block_sum(const double *t256) {
+  double w[64], r[4];
+  for (int q = 0; q < 4; ++q) {
+    for (int l = 0; l < 64; ++l) w[l] = t256[q * 64 + l];
+    r[q] = tree64(w);
+  }
+  double a = r[0];
+  for (int j = 1; j < 4; ++j) a = a + r[j];
+  return a;
+}
+
+static double grid_sum(const double *part, int nblk) {
+  double t[256];
+  for (int th = 0; th < 256; ++th) {
+    double acc = 0.0;
+    for (int b = th; b < nblk; b += 256) acc = acc + part[b];
+    t[th] = acc;
+  }
+  return block_sum(t);
+}
+
+/* sum over nodes of a[i]*b[i] (a NULL: b[i]*b[i]) as the fused dots of the gather kernels */
+static double dsum_gather(oc_ctx *c, const double *a, const double *b) {
+  double t[256];
+  for (int pb = 0; pb < c->ggrid; ++pb) {
+    int lb = c->xcd_per > 0 ? (pb & 7) * c->xcd_per + (pb >> 3) : pb;
+    for (int th = 0; th < 256; ++th) {
+      double v = 0.0;
+      if (lb < c->nblocks) {
+        int chunk = lb * 4 + th / 64;
+        int i = chunk * 64 + th % 64;
+        if (chunk < c->nchunk && i < c->n) v = (a ? a[i] : b[i]) * b[i];
+      }
+      t[th] = v;
+    }
+    c->part[pb] = block_sum(t);
+  }
+  return grid_sum(c->part, c->ggrid);
+}
+
+/* sum of a[k]*b[k] as the grid-stride vector kernels accumulate it */
+static double dsum_stride(oc_ctx *c, const double *a, const double *b) {
+  double t[256];
+  const int stride = c->vgrid * 256;
+  for (int pb = 0; pb < c->vgrid; ++pb) {
+    for (int th = 0; th < 256; ++th) {
+      double v = 0.0;
+      for (int k = pb * 256 + th; k < c->n; k += stride) v = v + a[k] * b[k];
+      t[th] = v;
+    }
+    c->part[pb] = block_sum(t);
+  }
+  return grid_sum(c->part, c->vgrid);
 }
 
 void oc_destroy(oc_ctx *c) {
   if (!c) return;
   double *all[] = {c->rh0, c->th0, c->the, c->ue, c->ve, c->coefw, c->alpha, c->d1, c->dthedz,
                    c->la, c->linv, c->lc, c->gx, c->gax, c->gr, c->gqr, c->gar, c->unk, c->frc,
-                   c->uuu, c->unkdash, c->prpot, c->unk1, c->uuu0};
+                   c->uuu, c->unkdash, c->prpot, c->unk1, c->uuu0, c->part};
   for (size_t i = 0; i < sizeof(all) / sizeof(all[0]); ++i) free(all[i]);
   for (int i = 0; i < 16; ++i) free(c->s[i]);
   for (int i = 0; i < 4; ++i) free(c->gb[i]);
@@ -446,10 +522,15 @@ void oc_gcrk(oc_ctx *c, double *p, const double *u1, const double *v1, const dou
   oc_div(c, u2, v2, w2, r, -1);
   oc_precon(c, r, qr, ar, impfl);
   double eer0 = 0., eem0 = -1.e15, rl20 = 0.;
+  const int dev = c->p.dot_order == 1;
   for (int k = 0; k < n; ++k) {
     eer0 = eer0 + qr[k] * qr[k];
     eem0 = amax1(eem0, fabs(qr[k]));
     rl20 = rl20 + r[k] * r[k];
+  }
+  if (dev) {
+    eer0 = dsum_stride(c, qr, qr);
+    rl20 = dsum_gather(c, NULL, r);
   }
   eer0 = amax1(eer0, epa);
   eem0 = amax1(eem0, epa);
@@ -461,9 +542,17 @@ void oc_gcrk(oc_ctx *c, double *p, const double *u1, const double *v1, const dou
       double *xi = x + (size_t)(i - 1) * n, *axi = ax + (size_t)(i - 1) * n;
       double rax = 0.;
       ax2[i - 1] = 0.;
-      for (int k = 0; k < n; ++k) {
-        rax = rax + r[k] * axi[k];
-        ax2[i - 1] = ax2[i - 1] + axi[k] * axi[k];
+      if (!dev) {
+        for (int k = 0; k < n; ++k) {
+          rax = rax + r[k] * axi[k];
+          ax2[i - 1] = ax2[i - 1] + axi[k] * axi[k];
+        }
+      } else if (it == 1 && i == 1) {   /* fused into the divergence kernel of laplc */
+        rax = dsum_gather(c, r, axi);
+        ax2[i - 1] = dsum_gather(c, NULL, axi);
+      } else {                          /* fused into the direction kernel */
+        rax = dsum_stride(c, r, axi);
+        ax2[i - 1] = dsum_stride(c, axi, axi);
       }
       ax2[i - 1] = amax1(epa, ax2[i - 1]);
       double beta = -rax / ax2[i - 1];
@@ -474,6 +563,7 @@ void oc_gcrk(oc_ctx *c, double *p, const double *u1, const double *v1, const dou
         dvmx = amax1(dvmx, fabs(r[k]));
         rl2 = rl2 + r[k] * r[k];
       }
+      if (dev) rl2 = dsum_stride(c, r, r);
       if (dvmx <= eps && it >= itmn) goto done;
       if (rl2 >= rl20) goto done;
       rl20 = amax1(rl2, epa);
@@ -483,7 +573,9 @@ void oc_gcrk(oc_ctx *c, double *p, const double *u1, const double *v1, const dou
       for (int ii = 1; ii <= i; ++ii) {
         double *axii = ax + (size_t)(ii - 1) * n;
         axar[ii - 1] = 0.;
-        for (int k = 0; k < n; ++k) axar[ii - 1] = axar[ii - 1] + axii[k] * ar[k];
+        if (dev) axar[ii - 1] = dsum_gather(c, axii, ar);
+        else
+          for (int k = 0; k < n; ++k) axar[ii - 1] = axar[ii - 1] + axii[k] * ar[k];
         del[ii - 1] = -axar[ii - 1] / ax2[ii - 1];
       }
       if (i < lord) {
@@ -516,6 +608,7 @@ void oc_gcrk(oc_ctx *c, double *p, const double *u1, const double *v1, const dou
 done:;
   double eer = 0., eem = -1.e15;
   for (int k = 0; k < n; ++k) { eer = eer + qr[k] * qr[k]; eem = amax1(eem, fabs(qr[k])); }
+  if (dev) eer = dsum_stride(c, qr, qr);
   c->st.eer = eer / eer0;
   c->st.eem = eem / eem0;
   c->st.niter = nlc;

@@ -54,6 +54,10 @@ typedef struct {
   int irl;
   /* domain box used by beta_timestep_unstr */
   double dom[6];
+  /* inner products: 0 = the reference's sequential sums; 1 = the HIP path's fixed
+   * reduction tree (same lane/block/grid assignment), so the two can be compared bit for
+   * bit and the effect of summation order measured on its own */
+  int dot_order;
 } oc_params;
 
 typedef struct { int niter, nitsm, icount; double eer, eem; } oc_stats;

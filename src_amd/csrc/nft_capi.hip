@@ -97,6 +97,13 @@ struct nft_ctx {
   double *io[8];
 
   nft_gcr_stats st{};
+
+  // kernel timer
+  int timer_kernel = 0;
+  std::vector<hipEvent_t> tev;   // pairs
+  int tev_used = 0;
+  int t_launches = 0;
+  double t_ms = 0;
 };
 
 namespace {
@@ -278,15 +285,36 @@ void setup_fields(nft_ctx *c, const nft_mesh *m, std::vector<double> &rh0, std::
   }
 }
 
+// ---- kernel timer ------------------------------------------------------------------------
+enum { KT_LGRAD = 1, KT_LDIV = 2, KT_LINE = 3, KT_UPD = 4, KT_DIR = 5, KT_MP0 = 6 };
+
+inline bool timing(nft_ctx *c, int k) { return c->timer_kernel == k && c->tev_used + 2 <= (int)c->tev.size(); }
+inline void t_begin(nft_ctx *c, int k) {
+  if (timing(c, k)) (void)hipEventRecord(c->tev[c->tev_used], c->stream);
+}
+inline void t_end(nft_ctx *c, int k) {
+  if (timing(c, k)) { (void)hipEventRecord(c->tev[c->tev_used + 1], c->stream); c->tev_used += 2; }
+}
+// after a stream synchronisation: fold finished event pairs into the totals
+void t_harvest(nft_ctx *c) {
+  for (int i = 0; i + 1 < c->tev_used; i += 2) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c->tev[i], c->tev[i + 1]) == hipSuccess) { c->t_ms += ms; c->t_launches++; }
+  }
+  c->tev_used = 0;
+}
+
 // ---- device operators -------------------------------------------------------------------
 int grad_launch(nft_ctx *c, int mode, const GradArgs &a) {
   dim3 g(c->gather_grid), b(NFT_BLOCK);
+  if (mode == GRAD_LAPLC) t_begin(c, KT_LGRAD);
   switch (mode) {
     case GRAD_RAW: hipLaunchKernelGGL(k_grad<GRAD_RAW>, g, b, 0, c->stream, c->inc, c->S, a); break;
     case GRAD_PRFR: hipLaunchKernelGGL(k_grad<GRAD_PRFR>, g, b, 0, c->stream, c->inc, c->S, a); break;
     case GRAD_LAPLC: hipLaunchKernelGGL(k_grad<GRAD_LAPLC>, g, b, 0, c->stream, c->inc, c->S, a); break;
     default: hipLaunchKernelGGL(k_grad<GRAD_ZERO_PRFR>, g, b, 0, c->stream, c->inc, c->S, a); break;
   }
+  if (mode == GRAD_LAPLC) t_end(c, KT_LGRAD);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -306,8 +334,13 @@ DivArgs div_args(nft_ctx *c) {
 
 int div_launch(nft_ctx *c, int mode, const DivArgs &a) {
   dim3 g(c->gather_grid), b(NFT_BLOCK);
-  if (mode == DIV_LAPLC) hipLaunchKernelGGL(k_div<DIV_LAPLC>, g, b, 0, c->stream, c->inc, c->S, a);
-  else hipLaunchKernelGGL(k_div<DIV_DIV>, g, b, 0, c->stream, c->inc, c->S, a);
+  if (mode == DIV_LAPLC) {
+    t_begin(c, KT_LDIV);
+    hipLaunchKernelGGL(k_div<DIV_LAPLC>, g, b, 0, c->stream, c->inc, c->S, a);
+    t_end(c, KT_LDIV);
+  } else {
+    hipLaunchKernelGGL(k_div<DIV_DIV>, g, b, 0, c->stream, c->inc, c->S, a);
+  }
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -362,12 +395,14 @@ int precon_dev(nft_ctx *c, const double *rhs, double *p, int impfl) {
         res = c->t[3];
       }
       int cg = blocks_for(c->ncol);
+      t_begin(c, KT_LINE);
       if (P.precision == 1)
         hipLaunchKernelGGL(k_line<float>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->col_start,
                            c->laf, c->linvf, c->lcf, res, it == 1 ? nullptr : p, p, P.omega, c->zbuff);
       else
         hipLaunchKernelGGL(k_line<double>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->col_start,
                            c->la, c->linv, c->lc, res, it == 1 ? nullptr : p, p, P.omega, c->zbuf);
+      t_end(c, KT_LINE);
     }
     HIPCHK(hipGetLastError());
   }
@@ -378,6 +413,7 @@ int read_scal(nft_ctx *c, int first, int count) {
   HIPCHK(hipMemcpyAsync(c->h_scal + first, c->scal + first, count * sizeof(double), hipMemcpyDeviceToHost,
                         c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  t_harvest(c);
   return 0;
 }
 
@@ -426,8 +462,10 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
   double dvmx = 0;
   for (int it = 1; it <= P.itr; ++it) {
     for (int i = 1; i <= lord; ++i) {
+      t_begin(c, KT_UPD);
       hipLaunchKernelGGL(k_gcr_update, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, p, c->r,
                          c->gx[i - 1], c->gax[i - 1], c->scal, c->partials, c->counter);
+      t_end(c, KT_UPD);
       HIPCHK(hipGetLastError());
       if ((rc = read_scal(c, SC_DVMX, 2))) return rc;
       dvmx = c->h_scal[SC_DVMX];
@@ -453,7 +491,9 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
         for (int ii = 0; ii < i; ++ii) { d.x[ii] = c->gx[ii]; d.ax[ii] = c->gax[ii]; }
         d.qr = c->qr; d.ar = c->ar; d.r = c->r;
         d.scal = c->scal; d.partials = c->partials; d.counter = c->counter; d.epa = epa;
+        t_begin(c, KT_DIR);
         hipLaunchKernelGGL(k_gcr_dir, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, d);
+        t_end(c, KT_DIR);
         HIPCHK(hipGetLastError());
       }
     }
@@ -497,12 +537,20 @@ int mpdata_dev(nft_ctx *c, const T *U, const T *V, const T *W, double *const *fi
   if (sizeof(T) == 8) S = *(EdgeS<T> *)&c->S;
   else S = *(EdgeS<T> *)&c->Sf;
   dim3 g(c->gather_grid), b(NFT_BLOCK);
+  t_begin(c, KT_MP0);
   hipLaunchKernelGGL((k_mp_donor<T, NF>), g, b, 0, c->stream, c->inc, S, a);
+  t_end(c, KT_MP0);
   // pass 2 reads the donor-cell fields
+  t_begin(c, KT_MP0 + 1);
   hipLaunchKernelGGL((k_mp_grad<T, NF>), g, b, 0, c->stream, c->inc, S, a);
+  t_end(c, KT_MP0 + 1);
+  t_begin(c, KT_MP0 + 2);
   hipLaunchKernelGGL((k_mp_fct<T, NF>), g, b, 0, c->stream, c->inc, S, a);
+  t_end(c, KT_MP0 + 2);
   double *const *out = (double *const *)c->io[7];
+  t_begin(c, KT_MP0 + 3);
   hipLaunchKernelGGL((k_mp_limit<T, NF>), g, b, 0, c->stream, c->inc, S, a, out);
+  t_end(c, KT_MP0 + 3);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -551,6 +599,7 @@ int one_step(nft_ctx *c, nft_run_stats *rs) {
       return rc;
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
     HIPCHK(hipEventSynchronize(c->ev[1]));
+    t_harvest(c);
     if (rs) { HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); rs->ms_gcr += ms; }
     hipLaunchKernelGGL(k_step_post, g, b, 0, c->stream, a, mtri == 2 ? 1 : 0);
   }
@@ -800,6 +849,8 @@ int nft_finalize(nft_ctx *c) {
   if (c->h_scal) hipHostFree(c->h_scal);
   for (auto &e : c->ev)
     if (e) hipEventDestroy(e);
+  for (auto &e : c->tev)
+    if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -1013,6 +1064,52 @@ int nft_set_state(nft_ctx *c, const char *name, const double *host, int ld) {
 int nft_get_gcr_stats(nft_ctx *c, nft_gcr_stats *s) {
   if (!c || !s) return set_err(-1, "nft_get_gcr_stats: bad argument");
   *s = c->st;
+  return 0;
+}
+
+int nft_kernel_timer(nft_ctx *c, int kernel) {
+  if (!c || kernel < 0 || kernel > 9) return set_err(-1, "nft_kernel_timer: bad argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  t_harvest(c);
+  if (c->tev.empty()) {
+    c->tev.resize(1024);
+    for (auto &e : c->tev) HIPCHK(hipEventCreate(&e));
+  }
+  c->timer_kernel = kernel;
+  c->tev_used = 0;
+  c->t_launches = 0;
+  c->t_ms = 0;
+  return 0;
+}
+
+// algorithmic bytes of one launch: each array read or written counted once at its width
+static double kernel_bytes(nft_ctx *c, int k) {
+  const double N = c->n, E = c->etot, NB = c->inc.nbp;
+  const double geom = 2 * E * 8 + 24 * E + 4 * (c->inc.nchunk + 1) + 4 * N + 4 * (N + 1) + NB * (4 + 48);
+  const double geomf = 2 * E * 8 + 12 * E + 4 * (c->inc.nchunk + 1) + 4 * N + 4 * (N + 1) + NB * (4 + 48);
+  const bool mixed = c->P.precision == 1;
+  const double ts = mixed ? 4 : 8;
+  switch (k) {
+    case KT_LGRAD: return geom + 8 * N /*p*/ + 4 * 8 * N /*vol rh0 coefw ue*/ + 3 * 8 * N /*q*/;
+    case KT_LDIV: return geom + 3 * 8 * N /*q*/ + 2 * 8 * N /*vol rh0*/ + 8 * N /*out*/;
+    case KT_LINE: return 4 * (c->ncol + 1) + 3 * ts * N /*factors*/ + 8 * N /*rhs*/ + 8 * N /*p*/ + 2 * ts * N /*z store+load*/;
+    case KT_UPD: return 6 * 8 * N;  /* p,r read+write; x, ax read */
+    case KT_DIR: return (3 + 2.0 * c->P.lord) * 8 * N;  /* qr, ar, r + up to lord x/ax; writes 2 */
+    case KT_MP0: return (mixed ? geomf : geom) + 3 * ts * N + 4 * 8 * N + 4 * 8 * N + 2 * ts * N + 8 * N;
+    case KT_MP0 + 1: return (mixed ? geomf : geom) + 4 * 8 * N + 12 * ts * N;
+    case KT_MP0 + 2: return (mixed ? geomf : geom) + 3 * ts * N + 4 * 8 * N + 12 * ts * N + 2 * ts * N + 8 * ts * N;
+    case KT_MP0 + 3: return (mixed ? geomf : geom) + 3 * ts * N + 4 * 8 * N + 12 * ts * N + 2 * ts * N + 8 * ts * N + 4 * 8 * N;
+  }
+  return 0;
+}
+
+int nft_kernel_timer_read(nft_ctx *c, int *launches, double *total_ms, double *bytes_per_launch) {
+  if (!c) return set_err(-1, "nft_kernel_timer_read: null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  t_harvest(c);
+  if (launches) *launches = c->t_launches;
+  if (total_ms) *total_ms = c->t_ms;
+  if (bytes_per_launch) *bytes_per_launch = kernel_bytes(c, c->timer_kernel);
   return 0;
 }
 

@@ -1,0 +1,331 @@
+! iso_c_binding layer over libnft_hip.so (C ABI in include/nft.h).
+!
+! Module nft_hip: bind(C) types and interfaces of every nft_* entry point, plus
+! nft_attach, which hands the reference's COMMON-block mesh and control data to nft_init.
+!
+! External subroutines derivative, laplc, prfr, bcv, div, precon, gcrk and mpdatam keep
+! the names and argument lists of the reference's subroutines
+! (lippsLFroude_implgwopks3D_unstr3.f :3943, :1288, :1536, :1583, :2096, :482, :1776,
+! :3029) so callers with implicit interfaces (mpsolver) link to them unchanged; they move
+! their arguments to the GPU, run the HIP kernels and copy the results back. gcrk fills
+! COMMON /itero/ as the reference does (:1953-1963); div with iflg=1 prints the same
+! diagnostics line (:2148-2168).
+!
+! Compile with the same default-real kind as the reference build (FP64:
+! -fdefault-real-8); npoinx must equal the reference's PARAMETER.
+module nft_hip
+  use iso_c_binding
+  implicit none
+  integer, parameter :: npoinx = 800000, nedgex = 4000000, nfacesx = 110000
+
+  type, bind(C) :: nft_mesh_t
+    integer(c_int) :: npoin, nedge, nbface, nboun, ld_poin, ld_edge, ld_face
+    type(c_ptr) :: iedge, sn, kbface, sb, ibpoin, sbb, anp, coord, vol
+    integer(c_int) :: ncol
+    type(c_ptr) :: col_start
+  end type
+
+  type, bind(C) :: nft_params_t
+    real(c_double) :: dt, u0, v0, w0, strat, theta0
+    integer(c_int) :: lord, itr, exit_mode
+    real(c_double) :: eps0
+    integer(c_int) :: precon, sm_flag, itrp1
+    real(c_double) :: omega
+    real(c_double) :: dxabl, dxabr, dyabd, dyabu, dzabd, dzabu
+    real(c_double) :: towxl, towxr, towyd, towyu, towzd, towzu
+    integer(c_int) :: irl
+    real(c_double) :: dom(6)
+    integer(c_int) :: precision
+  end type
+
+  type, bind(C) :: nft_gcr_stats_t
+    integer(c_int) :: niter, nitsm, icount, converged
+    real(c_double) :: eer, eem, res_max, tol
+  end type
+
+  type, bind(C) :: nft_run_stats_t
+    integer(c_int) :: steps, gcr_solves, gcr_iters, gcr_unconverged
+    real(c_double) :: ms_total, ms_gcr, ms_mpdata
+  end type
+
+  type(c_ptr), save :: nft_context = c_null_ptr
+  real(c_double), save :: nft_dt = 0.0d0
+
+  interface
+    integer(c_int) function nft_init(mesh, params, ctx) bind(C, name="nft_init")
+      import :: nft_mesh_t, nft_params_t, c_ptr, c_int
+      type(nft_mesh_t), intent(in) :: mesh
+      type(nft_params_t), intent(in) :: params
+      type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function nft_finalize(ctx) bind(C, name="nft_finalize")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    type(c_ptr) function nft_last_error() bind(C, name="nft_last_error")
+      import :: c_ptr
+    end function
+    integer(c_int) function nft_derivative(ctx, unk, dpdxa, idx) bind(C, name="nft_derivative")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: unk(*)
+      real(c_double), intent(out) :: dpdxa(*)
+      integer(c_int), value :: idx
+    end function
+    integer(c_int) function nft_laplc(ctx, press, alap, impfl) bind(C, name="nft_laplc")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: press(*)
+      real(c_double), intent(out) :: alap(*)
+      integer(c_int), value :: impfl
+    end function
+    integer(c_int) function nft_prfr(ctx, press, ustr, vstr, wstr, uout, vout, wout, impfl) &
+        bind(C, name="nft_prfr")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: press(*), ustr(*), vstr(*), wstr(*)
+      real(c_double), intent(out) :: uout(*), vout(*), wout(*)
+      integer(c_int), value :: impfl
+    end function
+    integer(c_int) function nft_bcv(ctx, u, v, w, flag) bind(C, name="nft_bcv")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: u(*), v(*), w(*)
+      real(c_double), value :: flag
+    end function
+    integer(c_int) function nft_div(ctx, ua, va, wa, rhs, iflg) bind(C, name="nft_div")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: ua(*), va(*), wa(*)
+      real(c_double), intent(out) :: rhs(*)
+      integer(c_int), value :: iflg
+    end function
+    integer(c_int) function nft_precon(ctx, rhs, p, impfl) bind(C, name="nft_precon")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: rhs(*)
+      real(c_double), intent(out) :: p(*)
+      integer(c_int), value :: impfl
+    end function
+    integer(c_int) function nft_gcrk(ctx, p, u1, v1, w1, u2, v2, w2, impfl, stats) &
+        bind(C, name="nft_gcrk")
+      import :: c_ptr, c_int, c_double, nft_gcr_stats_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: p(*)
+      real(c_double), intent(in) :: u1(*), v1(*), w1(*)
+      real(c_double), intent(out) :: u2(*), v2(*), w2(*)
+      integer(c_int), value :: impfl
+      type(nft_gcr_stats_t), intent(out) :: stats
+    end function
+    integer(c_int) function nft_mpdatam(ctx, uuu, ld, unk, iflag) bind(C, name="nft_mpdatam")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: uuu(*)
+      integer(c_int), value :: ld
+      real(c_double), intent(inout) :: unk(*)
+      integer(c_int), value :: iflag
+    end function
+    integer(c_int) function nft_state_init(ctx) bind(C, name="nft_state_init")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function nft_run_steps(ctx, nsteps, stats) bind(C, name="nft_run_steps")
+      import :: c_ptr, c_int, nft_run_stats_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps
+      type(nft_run_stats_t), intent(out) :: stats
+    end function
+    integer(c_int) function nft_get_state(ctx, name, host, ld) bind(C, name="nft_get_state")
+      import :: c_ptr, c_int, c_double, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: name(*)
+      real(c_double), intent(out) :: host(*)
+      integer(c_int), value :: ld
+    end function
+    integer(c_int) function nft_get_gcr_stats(ctx, stats) bind(C, name="nft_get_gcr_stats")
+      import :: c_ptr, c_int, nft_gcr_stats_t
+      type(c_ptr), value :: ctx
+      type(nft_gcr_stats_t), intent(out) :: stats
+    end function
+  end interface
+
+contains
+
+  subroutine nft_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    character(len=512) :: text
+    integer :: k
+    if (rc >= 0) return
+    call c_f_pointer(nft_last_error(), msg, [512])
+    text = ' '
+    do k = 1, 512
+      if (msg(k) == c_null_char) exit
+      text(k:k) = msg(k)
+    end do
+    write(*,'(a,a,a,i0,a,a)') 'nft: ', what, ' failed (', rc, '): ', trim(text)
+    error stop 1
+  end subroutine
+
+  ! Hands the reference's COMMON-block mesh and control data to nft_init. Call after the
+  ! reference's `input`. ncol/col_start: vertical columns for the line preconditioner
+  ! (ncol = 0: none); precon_kind: 1 point relaxation, 2 vertical line; precision: 0 FP64,
+  ! 1 mixed; strat/theta0: the values mpsolver hard-codes (:134-135).
+  subroutine nft_attach(ncol, col_start, precon_kind, precision, lord, strat, theta0)
+    integer, intent(in) :: ncol, precon_kind, precision, lord
+    integer(c_int), intent(in), target :: col_start(*)
+    real(8), intent(in) :: strat, theta0
+    integer :: npoin, nedge, nbface, nbpoin, nboun
+    common/n/npoin,nedge,nbface,nbpoin,nboun
+    real(8) :: coord, vol, sn, sb, sbb, anp
+    integer :: iedge, kbface, ibpoin
+    common/xyz/coord(npoinx,3)
+    common/volume/vol(npoinx)
+    common/edg/iedge(nedgex,2)
+    common/wge/sn(nedgex,3)
+    common/bound/kbface(nfacesx,1:2),sb(nfacesx,3)
+    common/bpoints/ibpoin(nfacesx,2),anp(nfacesx,3)
+    common/terms/sbb(nfacesx,3)
+    target :: coord, vol, sn, sb, sbb, anp, iedge, kbface, ibpoin
+    real(8) :: delt, adapt, afree(5), p0inf, rh0inf, u0inf, v0inf, w0inf
+    common/sss/delt,adapt
+    common/frflo/afree,p0inf,rh0inf,u0inf,v0inf,w0inf
+    integer :: itr, niter, nitsm, icount
+    real(8) :: eps0, eer, eem
+    common/itero/ itr,eps0,niter,nitsm,icount,eer,eem
+    real(8) :: dxabl,dxabr,dyabd,dyabu,dzabd,dzabu,towxl,towxr,towyd,towyu,towzd,towzu
+    integer :: irl
+    common/absspec/dxabl,dxabr,dyabd,dyabu,dzabd,dzabu,towxl,towxr,towyd,towyu,towzd,towzu,irl
+    integer :: mg_flag, itrp1, itrp2, itrp_c1, itrp_c2, itrp_cc
+    real(8) :: omega, sm_flag1, sm_flag2
+    common/vcycle/mg_flag,omega,itrp1,itrp2,itrp_c1,itrp_c2,itrp_cc
+    common/smooth/sm_flag1,sm_flag2
+    type(nft_mesh_t) :: m
+    type(nft_params_t) :: p
+    m%npoin = npoin; m%nedge = nedge; m%nbface = nbface; m%nboun = nboun
+    m%ld_poin = npoinx; m%ld_edge = nedgex; m%ld_face = nfacesx
+    m%iedge = c_loc(iedge); m%sn = c_loc(sn); m%kbface = c_loc(kbface); m%sb = c_loc(sb)
+    m%ibpoin = c_loc(ibpoin); m%sbb = c_loc(sbb); m%anp = c_loc(anp)
+    m%coord = c_loc(coord); m%vol = c_loc(vol)
+    m%ncol = ncol
+    m%col_start = c_loc(col_start)
+    p%dt = delt; p%u0 = u0inf; p%v0 = v0inf; p%w0 = w0inf
+    p%strat = strat; p%theta0 = theta0
+    p%lord = lord; p%itr = itr; p%exit_mode = 0; p%eps0 = eps0
+    p%precon = precon_kind; p%sm_flag = nint(sm_flag1); p%itrp1 = itrp1; p%omega = omega
+    p%dxabl = dxabl; p%dxabr = dxabr; p%dyabd = dyabd; p%dyabu = dyabu; p%dzabd = dzabd; p%dzabu = dzabu
+    p%towxl = towxl; p%towxr = towxr; p%towyd = towyd; p%towyu = towyu; p%towzd = towzd; p%towzu = towzu
+    p%irl = irl
+    p%dom = [0.0d0, 15000.0d0, -6000.0d0, 6000.0d0, 0.0d0, 6000.0d0]
+    p%precision = precision
+    nft_dt = delt
+    call nft_check(nft_init(m, p, nft_context), 'nft_init')
+  end subroutine
+
+end module nft_hip
+
+! ---- drop-in replacements of the reference subroutines ---------------------------------
+
+subroutine derivative(unk, dpdxa, idx)
+  use nft_hip
+  implicit none
+  real(8) :: unk(*), dpdxa(*)
+  integer :: idx
+  call nft_check(nft_derivative(nft_context, unk, dpdxa, idx), 'derivative')
+end subroutine
+
+subroutine laplc(press, alap, impfl)
+  use nft_hip
+  implicit none
+  real(8) :: press(*), alap(*)
+  integer :: impfl
+  call nft_check(nft_laplc(nft_context, press, alap, impfl), 'laplc')
+end subroutine
+
+subroutine prfr(press, ustr, vstr, wstr, uout, vout, wout, impfl)
+  use nft_hip
+  implicit none
+  real(8) :: press(*), ustr(*), vstr(*), wstr(*), uout(*), vout(*), wout(*)
+  integer :: impfl
+  call nft_check(nft_prfr(nft_context, press, ustr, vstr, wstr, uout, vout, wout, impfl), 'prfr')
+end subroutine
+
+subroutine bcv(uout, vout, wout, flag)
+  use nft_hip
+  implicit none
+  real(8) :: uout(*), vout(*), wout(*), flag
+  call nft_check(nft_bcv(nft_context, uout, vout, wout, flag), 'bcv')
+end subroutine
+
+subroutine div(ua, va, wa, rhs, iflg)
+  use nft_hip
+  implicit none
+  real(8) :: ua(*), va(*), wa(*), rhs(*)
+  integer :: iflg
+  integer :: npoin, nedge, nbface, nbpoin, nboun
+  common/n/npoin,nedge,nbface,nbpoin,nboun
+  integer :: itr, niter, nitsm, icount, nitav, ip
+  real(8) :: eps0, eer, eem, divmx, divmn, divav, divsd
+  common/itero/ itr,eps0,niter,nitsm,icount,eer,eem
+  call nft_check(nft_div(nft_context, ua, va, wa, rhs, iflg), 'div')
+  if (iflg == 1) then
+    divmx = -1.d15
+    divmn = 1.d15
+    divav = 0.d0
+    do ip = 1, npoin
+      divmx = max(divmx, rhs(ip)*nft_dt)
+      divmn = min(divmn, rhs(ip)*nft_dt)
+      divav = divav + rhs(ip)*nft_dt
+    end do
+    divav = divav/dble(npoin)
+    divsd = 0.d0
+    do ip = 1, npoin
+      divsd = divsd + (rhs(ip)*nft_dt - divav)**2
+    end do
+    divsd = sqrt(divsd/npoin)
+    nitav = nitsm/max(icount, 1)
+    print 205, divmx, divmn, divav, divsd, eer, eem, niter, nitav
+205 format(1x,'dvmx,dvmn,dvav,dvsd:',4e11.4/1x,' eer,eem:',2e11.4,1x,'niter,nitav:',4i4)
+  end if
+end subroutine
+
+subroutine precon(rhs, p, r, iprc, elas, impfl)
+  use nft_hip
+  implicit none
+  real(8) :: rhs(*), p(*), r(*), elas
+  integer :: iprc, impfl
+  integer :: npoin, nedge, nbface, nbpoin, nboun
+  common/n/npoin,nedge,nbface,nbpoin,nboun
+  if (iprc == 0) then
+    p(1:npoin) = rhs(1:npoin)
+    return
+  end if
+  call nft_check(nft_precon(nft_context, rhs, p, impfl), 'precon')
+end subroutine
+
+subroutine gcrk(p, u1, v1, w1, u2, v2, w2, impfl)
+  use nft_hip
+  implicit none
+  real(8) :: p(*), u1(*), v1(*), w1(*), u2(*), v2(*), w2(*)
+  integer :: impfl
+  integer :: itr, niter, nitsm, icount, rc
+  real(8) :: eps0, eer, eem
+  common/itero/ itr,eps0,niter,nitsm,icount,eer,eem
+  type(nft_gcr_stats_t) :: st
+  rc = nft_gcrk(nft_context, p, u1, v1, w1, u2, v2, w2, impfl, st)
+  call nft_check(rc, 'gcrk')
+  niter = st%niter
+  nitsm = nitsm + st%niter
+  icount = icount + 1
+  eer = st%eer
+  eem = st%eem
+end subroutine
+
+subroutine mpdatam(uuu, unk, iflag)
+  use nft_hip
+  implicit none
+  real(8) :: uuu(npoinx,3), unk(*)
+  integer :: iflag
+  call nft_check(nft_mpdatam(nft_context, uuu, npoinx, unk, iflag), 'mpdatam')
+end subroutine

@@ -1,0 +1,145 @@
+! nftdrv: Fortran driver of the device-resident NFT time loop.
+!
+! Reads the reference's own input files from the working directory (mesh3d.d,
+! inputinc.d, input_mg_v3.d; layouts of lippsLFroude_implgwopks3D_unstr3.f `input`,
+! :2265-2321 and :2343-2501), hands the mesh to nft_init through iso_c_binding, runs the
+! mpsolver initialisation and `ntime` steps of its DO ITIME loop on the GPU, prints the
+! reference's per-step iteration line and writes the final state (unk, uuu, prpot) as
+! raw float64 to nftdrv_state.bin.
+!
+! Usage: nftdrv <nlev> <precon> <precision> <lord> <strat> [exit_mode]
+!   nlev      levels per column when nodes are numbered column by column (0: no columns)
+!   precon    1 point relaxation (reference), 2 vertical line
+!   precision 0 FP64, 1 mixed
+!   lord      GCR order; strat: N^2/g (the reference hard-codes these in mpsolver)
+program nftdrv
+  use iso_c_binding
+  use nft_hip
+  implicit none
+  integer :: npoin, nedge, nbface, nbpoin, nboun, i, k, ii, nlev, pk, prec, lord, em
+  integer :: ntime, nrkstage, istst, mrestart, isfera, iper, ielast, itr, lordin, itrp, irl
+  integer :: mg_flag, itrp1, itrp2, itrp_c1, itrp_c2, itrp_cc, match_flag
+  real(8) :: cfl, delt, adapt, relax, p0inf, u0inf, v0inf, w0inf, re, eps0, betap, strat
+  real(8) :: sm1, sm2, omega
+  real(8) :: ab(6), tw(6)
+  real(8), allocatable, target :: coord(:,:), vol(:), sn(:,:), sb(:,:), sbb(:,:), anp(:,:)
+  integer(c_int), allocatable, target :: iedge(:,:), kbface(:,:), ibpoin(:,:), cols(:)
+  real(8), allocatable :: state(:,:)
+  type(nft_mesh_t) :: m
+  type(nft_params_t) :: p
+  type(nft_run_stats_t) :: rs
+  type(nft_gcr_stats_t) :: gs
+  character(len=64) :: arg
+  integer :: nitsm_total, step
+
+  call get_command_argument(1, arg); read(arg,*) nlev
+  call get_command_argument(2, arg); read(arg,*) pk
+  call get_command_argument(3, arg); read(arg,*) prec
+  call get_command_argument(4, arg); read(arg,*) lord
+  call get_command_argument(5, arg); read(arg,*) strat
+  em = 0
+  if (command_argument_count() >= 6) then
+    call get_command_argument(6, arg); read(arg,*) em
+  end if
+
+  open(8, file='input_mg_v3.d', status='old')
+  read(8,*); read(8,*) mg_flag; read(8,*); read(8,*); read(8,*) sm1, sm2
+  read(8,*); read(8,*) omega; read(8,*); read(8,*)
+  read(8,*) itrp1, itrp2, itrp_c1, itrp_c2, itrp_cc; read(8,*); read(8,*) match_flag
+  close(8)
+  open(1, file='inputinc.d', status='old')
+  read(1,*); read(1,*) ntime, nrkstage, istst, mrestart, isfera, iper
+  read(1,*); read(1,*) cfl, delt, adapt, relax
+  read(1,*); read(1,*) p0inf, u0inf, v0inf, w0inf, re
+  read(1,*); read(1,*); read(1,*) ielast, eps0, itr, lordin
+  read(1,*); read(1,*) itrp, betap
+  read(1,*); read(1,*) irl
+  ab = 0; tw = 0
+  if (irl == 1) then
+    read(1,*); read(1,*) ab
+    read(1,*); read(1,*) tw
+  end if
+  close(1)
+  if (isfera /= 0 .or. iper /= 0) then
+    write(*,*) 'nftdrv: only the Cartesian, non-periodic form is supported'
+    error stop 2
+  end if
+
+  open(5, file='mesh3d.d', status='old')
+  read(5,*) npoin, nedge, nbface, nbpoin, nboun
+  allocate(coord(npoin,3), vol(npoin), iedge(max(nedge,1),2), sn(max(nedge,1),3))
+  allocate(kbface(max(nbface,nboun,1),2), sb(max(nbface,nboun,1),3))
+  allocate(ibpoin(max(nbface,nboun,1),2), sbb(max(nbface,nboun,1),3), anp(max(nbface,nboun,1),3))
+  do i = 1, npoin
+    read(5,*) ii, (coord(i,k), k=1,3), vol(i)
+  end do
+  do i = 1, nedge
+    read(5,*) ii, iedge(i,1), iedge(i,2)
+  end do
+  do i = 1, nedge
+    read(5,*) ii, (sn(i,k), k=1,3)
+  end do
+  do i = 1, nbface
+    read(5,*) ii, kbface(i,1), kbface(i,2)
+  end do
+  do i = 1, nbface
+    read(5,*) ii, (sb(i,k), k=1,3)
+  end do
+  do i = 1, nboun
+    read(5,*) ii, ibpoin(i,1), ibpoin(i,2)
+  end do
+  do i = 1, nboun
+    read(5,*) ii, (sbb(i,k), k=1,3)
+  end do
+  do i = 1, nboun
+    read(5,*) ii, (anp(i,k), k=1,3)
+  end do
+  close(5)
+
+  if (nlev > 0) then
+    allocate(cols(npoin/nlev + 1))
+    do i = 0, npoin/nlev
+      cols(i+1) = i*nlev
+    end do
+  else
+    allocate(cols(1)); cols(1) = 0
+  end if
+
+  m%npoin = npoin; m%nedge = nedge; m%nbface = nbface; m%nboun = nboun
+  m%ld_poin = npoin; m%ld_edge = max(nedge,1); m%ld_face = max(nbface,nboun,1)
+  m%iedge = c_loc(iedge); m%sn = c_loc(sn); m%kbface = c_loc(kbface); m%sb = c_loc(sb)
+  m%ibpoin = c_loc(ibpoin); m%sbb = c_loc(sbb); m%anp = c_loc(anp)
+  m%coord = c_loc(coord); m%vol = c_loc(vol)
+  m%ncol = 0
+  if (nlev > 0) m%ncol = npoin/nlev
+  m%col_start = c_loc(cols)
+
+  p%dt = delt; p%u0 = u0inf; p%v0 = v0inf; p%w0 = w0inf
+  p%strat = strat; p%theta0 = 300.0d0
+  p%lord = lord; p%itr = itr; p%exit_mode = em; p%eps0 = eps0
+  p%precon = pk; p%sm_flag = nint(sm1); p%itrp1 = itrp1; p%omega = omega
+  p%dxabl = ab(1); p%dxabr = ab(2); p%dyabd = ab(3); p%dyabu = ab(4); p%dzabd = ab(5); p%dzabu = ab(6)
+  p%towxl = tw(1); p%towxr = tw(2); p%towyd = tw(3); p%towyu = tw(4); p%towzd = tw(5); p%towzu = tw(6)
+  p%irl = irl
+  p%dom = [0.0d0, 15000.0d0, -6000.0d0, 6000.0d0, 0.0d0, 6000.0d0]
+  p%precision = prec
+
+  call nft_check(nft_init(m, p, nft_context), 'nft_init')
+  call nft_check(nft_state_init(nft_context), 'nft_state_init')
+  call nft_check(nft_get_gcr_stats(nft_context, gs), 'stats')
+  write(*,'(a,i6)') ' initial projection niter', gs%niter
+  nitsm_total = 0
+  do step = 1, ntime
+    call nft_check(nft_run_steps(nft_context, 1, rs), 'nft_run_steps')
+    call nft_check(nft_get_gcr_stats(nft_context, gs), 'stats')
+    write(*,'(a,i6,a,i6,a,f10.3)') ' step', step, ' niter', gs%niter, ' ms', rs%ms_total
+  end do
+  allocate(state(npoin, 8))
+  call nft_check(nft_get_state(nft_context, 'unk'//c_null_char, state(1,1), npoin), 'state')
+  call nft_check(nft_get_state(nft_context, 'uuu'//c_null_char, state(1,5), npoin), 'state')
+  call nft_check(nft_get_state(nft_context, 'prpot'//c_null_char, state(1,8), npoin), 'state')
+  open(30, file='nftdrv_state.bin', access='stream', form='unformatted', status='replace')
+  write(30) state
+  close(30)
+  call nft_check(nft_finalize(nft_context), 'nft_finalize')
+end program nftdrv

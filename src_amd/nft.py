@@ -76,7 +76,13 @@ _SIGS = {
     "nft_set_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, f64p, ctypes.c_int]),
     "nft_get_field": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, f64p]),
     "nft_get_gcr_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GcrStats)]),
+    "nft_kernel_timer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "nft_kernel_timer_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
 }
+
+KERNELS = {1: "laplc_grad", 2: "laplc_div", 3: "line_solve", 4: "gcr_update", 5: "gcr_dir",
+           6: "mp_donor", 7: "mp_grad", 8: "mp_fct", 9: "mp_limit"}
 
 
 def lib():
@@ -249,6 +255,15 @@ class Solver:
         a = np.zeros(self.n)
         _chk(lib().nft_get_field(self.h, name.encode(), _p(a)), "nft_get_field")
         return a
+
+    def kernel_timer(self, kernel):
+        _chk(lib().nft_kernel_timer(self.h, int(kernel)), "nft_kernel_timer")
+
+    def kernel_timer_read(self):
+        n = ctypes.c_int(); ms = ctypes.c_double(); b = ctypes.c_double()
+        _chk(lib().nft_kernel_timer_read(self.h, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)),
+             "nft_kernel_timer_read")
+        return n.value, ms.value, b.value
 
     def gcr_stats(self):
         st = GcrStats()

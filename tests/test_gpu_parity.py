@@ -1,0 +1,266 @@
+"""HIP path (through the C ABI) against the CPU oracle and the reference's golden vectors.
+
+Bar: bit-exact where the arithmetic is a fixed-order restatement (gathers in the
+reference's scatter order, pointwise loops, Thomas sweeps, MPDATA); GCR inner products
+are tree reductions, so solver results are compared with tolerances that are stated in
+each test (FP64: 1e-12 relative L2 per field after the time steps).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from common import mesh_checksum, mesh_for, rel_l2, seeded_fields, small_cfg, tiny_cfg
+from oracle import Oracle
+from src_amd import nft
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def solver(m, cfg, **kw):
+    return nft.Solver(m, nft.make_params(cfg, **kw))
+
+
+@pytest.fixture(scope="module", params=["tiny", "small"])
+def pair(request):
+    m = mesh_for(request.param)
+    cfg = tiny_cfg() if request.param == "tiny" else small_cfg()
+    s = solver(m, cfg)
+    o = Oracle(m, cfg)
+    yield m, cfg, s, o
+    s.close()
+
+
+def test_setup_fields_bitwise(pair):
+    m, cfg, s, o = pair
+    for k in ("alpha", "d1", "coefw", "line_a", "line_inv", "line_c"):
+        assert np.array_equal(s.field(k), o.field(k)), k
+
+
+def test_derivative_bitwise(pair):
+    m, cfg, s, o = pair
+    f = seeded_fields(m)
+    for idx in (1, 2, 3):
+        assert np.array_equal(s.derivative(f["p"], idx), o.derivative(f["p"], idx)), idx
+
+
+def test_laplc_bitwise(pair):
+    m, cfg, s, o = pair
+    f = seeded_fields(m)
+    for impfl in (0, 1):
+        assert np.array_equal(s.laplc(f["p"], impfl), o.laplc(f["p"], impfl)), impfl
+
+
+def test_prfr_bcv_div_bitwise(pair):
+    m, cfg, s, o = pair
+    f = seeded_fields(m)
+    for impfl in (0, 1):
+        a = s.prfr(f["p"], f["u"], f["v"], f["w"], impfl)
+        b = o.prfr(f["p"], f["u"], f["v"], f["w"], impfl)
+        for k in range(3):
+            assert np.array_equal(a[k], b[k]), (impfl, k)
+    for flag in (0.0, 1.0):
+        a = s.bcv(f["u"], f["v"], f["w"], flag)
+        b = o.bcv(f["u"], f["v"], f["w"], flag)
+        for k in range(3):
+            assert np.array_equal(a[k], b[k]), (flag, k)
+    for iflg in (-1, 1):
+        assert np.array_equal(s.div(f["u"], f["v"], f["w"], iflg), o.div(f["u"], f["v"], f["w"], iflg))
+
+
+@pytest.mark.parametrize("key", ["tiny", "small"])
+@pytest.mark.parametrize("kind", [dict(precon=1, itrp1=2, omega=0.5), dict(precon=1, itrp1=1, omega=0.7),
+                                  dict(precon=2, itrp1=1, omega=1.0), dict(precon=2, itrp1=2, omega=0.8)])
+def test_precon_bitwise(key, kind):
+    m = mesh_for(key)
+    cfg = (tiny_cfg if key == "tiny" else small_cfg)(**kind)
+    s, o = solver(m, cfg), Oracle(m, cfg)
+    f = seeded_fields(m)
+    assert np.array_equal(s.precon(f["r"], 1), o.precon(f["r"], 1))
+    s.close()
+
+
+def test_mpdatam_bitwise(pair):
+    m, cfg, s, o = pair
+    f = seeded_fields(m)
+    for iflag in (1, 2, 3, 4):
+        a = s.mpdatam(f["uuu"], f["p"], iflag)
+        b = o.mpdatam(f["uuu"], f["p"], iflag)
+        assert np.array_equal(a, b), iflag
+
+
+def test_mpdatam_against_reference_vectors():
+    g = np.load(os.path.join(GOLD, "tiny_ops.npz"), allow_pickle=False)
+    m = mesh_for("tiny")
+    assert np.array_equal(g["mesh_checksum"], mesh_checksum(m))
+    s = solver(m, tiny_cfg())
+    v = g["vec_in"]
+    uuu = np.ascontiguousarray(v[5:8])
+    assert np.array_equal(s.mpdatam(uuu, v[0], 1), g["mpdatam_1"])
+    assert np.array_equal(s.mpdatam(uuu, v[0], 4), g["mpdatam_4"])
+    for k in (1, 2, 3):
+        assert np.array_equal(s.derivative(v[0], k), g[f"derivative_{k}"])
+    assert np.array_equal(s.laplc(v[0], 1), g["laplc_1"])
+    assert np.array_equal(s.precon(v[4], 1), g["precon"])
+    s.close()
+
+
+# The HIP path and the oracle in dot_order=1 use the same inner-product reduction tree,
+# so everything below is compared bit for bit. The oracle in dot_order=0 (the reference's
+# sequential sums) equals the reference bit for bit (tests/test_oracle.py).
+LINE = dict(precon=2, itrp1=1, omega=1.0, exit_mode=1, eps0=1e-8, itr=200)
+
+
+@pytest.mark.parametrize("key,kw", [
+    ("tiny", dict()),                                                 # reference parity mode
+    ("small", LINE),
+    ("small", dict(precon=1, itrp1=2, omega=0.5, itr=10)),
+])
+def test_gcrk_bitwise_device_order(key, kw):
+    m = mesh_for(key)
+    cfg = (tiny_cfg if key == "tiny" else small_cfg)(**kw)
+    s, o = solver(m, cfg), Oracle(m, cfg, dot_order=1)
+    f = seeded_fields(m)
+    p0 = np.zeros(m.npoin)
+    pg, ug, st, rc = s.gcrk(p0, f["u"], f["v"], 0.1 * f["w"], 1)
+    po, uo = o.gcrk(p0, f["u"], f["v"], 0.1 * f["w"], 1)
+    so = o.stats()
+    assert st.niter == so.niter
+    assert np.array_equal(pg, po)
+    for k in range(3):
+        assert np.array_equal(ug[k], uo[k])
+    assert st.eer == so.eer and st.eem == so.eem
+    s.close()
+
+
+def _steps(m, cfg, nsteps, **kw):
+    s = solver(m, cfg, **kw)
+    s.state_init()
+    its = [s.gcr_stats().niter]
+    for _ in range(nsteps):
+        s.run_steps(1)
+        its.append(s.gcr_stats().niter)
+    return s, its
+
+
+def _oracle_steps(m, cfg, nsteps, dot_order=1):
+    o = Oracle(m, cfg, dot_order=dot_order)
+    o.init()
+    its = [o.stats().niter]
+    for _ in range(nsteps):
+        o.step()
+        its.append(o.stats().niter)
+    return o, its
+
+
+@pytest.mark.parametrize("key,kw,nsteps", [("tiny", dict(), 10), ("small", dict(), 2), ("small", LINE, 5)])
+def test_steps_bitwise_device_order(key, kw, nsteps):
+    """Full NFT steps (MPDATA x4, two GCR solves, projections): the GPU state equals the
+    oracle's bit for bit, iteration counts included."""
+    m = mesh_for(key)
+    cfg = (tiny_cfg if key == "tiny" else small_cfg)(**kw)
+    s, its = _steps(m, cfg, nsteps)
+    o, ito = _oracle_steps(m, cfg, nsteps)
+    assert its == ito
+    for name in ("unk", "frc", "uuu", "unkdash", "prpot"):
+        assert np.array_equal(s.state(name), o.state(name)), name
+    s.close()
+
+
+def test_steps_small_100_line_vs_sequential_oracle():
+    """100 steps, line-preconditioned GCR(3) at 1e-8: the GPU against the oracle in the
+    reference's sequential summation order (fixture small_line_oracle.npz). Only the
+    inner-product summation order differs; tolerance: prognostic fields u, v, w, theta'
+    and the advector within 1e-12 relative L2 (sampled field and full-field norm), GCR
+    iterations within one per solve."""
+    path = os.path.join(GOLD, "small_line_oracle.npz")
+    if not os.path.exists(path):
+        pytest.skip("fixture not generated")
+    g = np.load(path, allow_pickle=False)
+    m = mesh_for("small")
+    assert np.array_equal(g["mesh_checksum"], mesh_checksum(m))
+    s = solver(m, small_cfg(**LINE))
+    s.state_init()
+    its = [s.gcr_stats().niter]
+    done = 0
+    for nt in (1, 10, 100):
+        while done < nt:
+            s.run_steps(1)
+            its.append(s.gcr_stats().niter)
+            done += 1
+        st = int(g[f"s{nt}_stride"][0])
+        ref_its = g[f"s{nt}_niter"].tolist()
+        assert all(abs(a - b) <= 1 for a, b in zip(its, ref_its)), (its, ref_its)
+        for name in ("unk", "uuu"):
+            a = s.state(name)
+            for k in range(a.shape[0]):
+                err = rel_l2(a[k, ::st], g[f"s{nt}_{name}_sample"][k])
+                print(f"step {nt} {name}[{k}] sampled rel L2 {err:.3e}")
+                assert err <= 1e-12, (nt, name, k, err)
+                l2 = np.sqrt((a[k] * a[k]).sum())
+                assert abs(l2 - g[f"s{nt}_{name}_l2"][k]) <= 1e-12 * g[f"s{nt}_{name}_l2"][k]
+    s.close()
+
+
+def test_steps_small_100_reference_parity_mode():
+    """100 steps in the reference's own settings (point Jacobi, 120 GCR iterations per
+    solve, never reaching eps0) against the flang reference (small_run.npz). In this
+    regime the GCR iterate is far from converged and round-off in the inner-product order
+    alone changes the fields by ~1e-3 after one step (oracle vs oracle, measured in
+    tests/test_oracle.py), so the check is: same iteration counts, and full-field norms
+    within 2%."""
+    g = np.load(os.path.join(GOLD, "small_run.npz"), allow_pickle=False)
+    m = mesh_for("small")
+    s, its = _steps(m, small_cfg(), 100)
+    assert its == g["s100_niter"].tolist()
+    for name in ("unk", "uuu"):
+        a = s.state(name)
+        for k in range(a.shape[0]):
+            l2 = np.sqrt((a[k] * a[k]).sum())
+            ref = g[f"s100_{name}_l2"][k]
+            print(f"{name}[{k}] norm {l2:.6e} reference {ref:.6e} rel {abs(l2 - ref) / ref:.2e}")
+            assert abs(l2 - ref) <= 0.02 * ref
+    s.close()
+
+
+def test_mixed_precision_close_to_fp64():
+    """FP32 MPDATA and line preconditioner, FP64 GCR: per-field deviation from the FP64
+    form after 10 steps below 1e-5 relative L2, and the solves still converge."""
+    m = mesh_for("small")
+    cfg = small_cfg(precon=2, itrp1=1, omega=1.0, exit_mode=1, eps0=1e-8, itr=200)
+    a = solver(m, cfg)
+    b = solver(m, cfg, precision="mixed")
+    for s in (a, b):
+        s.state_init()
+        st = s.run_steps(10)
+        assert st.gcr_unconverged == 0
+    for name in ("unk", "uuu"):
+        x, y = a.state(name), b.state(name)
+        for k in range(x.shape[0]):
+            err = rel_l2(y[k], x[k])
+            assert err <= 1e-5, (name, k, err)
+    a.close(); b.close()
+
+
+def test_gcr_reports_nonconvergence():
+    """A forced iteration limit (itr=1) ends the solve early and is reported (rc=1)."""
+    m = mesh_for("small")
+    cfg = small_cfg(precon=2, itrp1=1, omega=1.0, exit_mode=1, eps0=1e-12, itr=1)
+    s = solver(m, cfg)
+    f = seeded_fields(m)
+    _, _, st, rc = s.gcrk(np.zeros(m.npoin), f["u"], f["v"], f["w"], 1)
+    assert rc == 1 and st.converged == 0 and st.niter == cfg.lord
+    s.close()
+
+
+def test_init_rejects_corrupt_mesh():
+    m = mesh_for("tiny")
+    bad = m.iedge.copy()
+    bad[1, 3] = m.npoin + 5
+    import copy
+    m2 = copy.copy(m)
+    m2._handle = None   # the copy must not free the generator handle it shares
+    m2.iedge = bad
+    with pytest.raises(nft.NftError, match="out of range"):
+        nft.Solver(m2, nft.make_params(tiny_cfg()))
