@@ -20,8 +20,10 @@ struct oc_ctx {
   /* profiles and setup fields */
   double *rh0, *th0, *the, *ue, *ve, *coefw, *alpha, *d1, *dthedz;
   double beta1;
-  /* line preconditioner factors */
+  /* line preconditioner: Thomas factors and operator rows */
   double *la, *linv, *lc;
+  double *llo, *ldg, *lup;
+  int line_L;   /* uniform column length <= 64 (parallel cyclic reduction), else 0 (Thomas) */
   /* scratch */
   double *s[16];
   /* gcrk work */
@@ -51,6 +53,7 @@ oc_ctx *oc_create(const oc_mesh *m, const oc_params *p) {
   c->ve = zalloc(n); c->coefw = zalloc(n); c->alpha = zalloc(n); c->d1 = zalloc(n);
   c->dthedz = zalloc(n);
   c->la = zalloc(n); c->linv = zalloc(n); c->lc = zalloc(n);
+  c->llo = zalloc(n); c->ldg = zalloc(n); c->lup = zalloc(n);
   for (int i = 0; i < 16; ++i) c->s[i] = zalloc(n);
   int lord = p->lord > 0 ? p->lord : 1;
   c->gx = zalloc((size_t)lord * n); c->gax = zalloc((size_t)lord * n);
@@ -170,7 +173,7 @@ oc_ctx *oc_create(const oc_mesh *m, const oc_params *p) {
   c->nblocks = (c->nchunk + 3) / 4;
   if (c->nblocks >= 64) { c->xcd_per = (c->nblocks + 7) / 8; c->ggrid = 8 * c->xcd_per; }
   else { c->xcd_per = 0; c->ggrid = c->nblocks; }
-  c->vgrid = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
+  c->vgrid = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
   c->part = zalloc((size_t)(c->ggrid > c->vgrid ? c->ggrid : c->vgrid));
   return c;
 }
@@ -245,7 +248,7 @@ void oc_destroy(oc_ctx *c) {
   if (!c) return;
   double *all[] = {c->rh0, c->th0, c->the, c->ue, c->ve, c->coefw, c->alpha, c->d1, c->dthedz,
                    c->la, c->linv, c->lc, c->gx, c->gax, c->gr, c->gqr, c->gar, c->unk, c->frc,
-                   c->uuu, c->unkdash, c->prpot, c->unk1, c->uuu0, c->part};
+                   c->uuu, c->unkdash, c->prpot, c->unk1, c->uuu0, c->part, c->llo, c->ldg, c->lup};
   for (size_t i = 0; i < sizeof(all) / sizeof(all[0]); ++i) free(all[i]);
   for (int i = 0; i < 16; ++i) free(c->s[i]);
   for (int i = 0; i < 4; ++i) free(c->gb[i]);
@@ -265,6 +268,9 @@ double *oc_field(oc_ctx *c, const char *name) {
   if (!strcmp(name, "line_a")) return c->la;
   if (!strcmp(name, "line_inv")) return c->linv;
   if (!strcmp(name, "line_c")) return c->lc;
+  if (!strcmp(name, "line_lo")) return c->llo;
+  if (!strcmp(name, "line_dg")) return c->ldg;
+  if (!strcmp(name, "line_up")) return c->lup;
   return NULL;
 }
 
@@ -445,15 +451,57 @@ static void build_line(oc_ctx *c) {
       double cp = (i == s1 - 1) ? 0. : up[i] * inv;
       c->la[i] = a; c->linv[i] = inv; c->lc[i] = cp;
       cprev = cp;
+      c->llo[i] = a; c->ldg[i] = diag[i]; c->lup[i] = (i == s1 - 1) ? 0. : up[i];
     }
   }
   free(colof);
   free(levof);
+  int L0 = m->col_start[1] - m->col_start[0];
+  int uniform = L0 <= 64;
+  for (int k = 0; k < m->ncol && uniform; ++k)
+    if (m->col_start[k + 1] - m->col_start[k] != L0 || m->col_start[k] != k * L0) uniform = 0;
+  c->line_L = uniform ? L0 : 0;
 }
 
-/* z = M^-1 rhs, column by column (forward then backward Thomas sweep) */
+/* Parallel cyclic reduction of one column of L <= 64 rows padded with identity rows to
+ * 64: steps of distance 1, 2, ..., 32, in the operation order of the HIP path's
+ * wavefront solver (k_line_pcr). */
+static void pcr_column(const double *lo, const double *dg, const double *up, const double *rhs, int L,
+                       double *z) {
+  double a[64], b[64], cc[64], d[64], na[64], nb[64], nc[64], nd[64];
+  for (int l = 0; l < 64; ++l) {
+    if (l < L) { a[l] = lo[l]; b[l] = dg[l]; cc[l] = up[l]; d[l] = rhs[l]; }
+    else { a[l] = 0.; b[l] = 1.; cc[l] = 0.; d[l] = 0.; }
+  }
+  for (int s = 1; s < 64; s <<= 1) {
+    for (int l = 0; l < 64; ++l) {
+      int im = l - s, ip = l + s;
+      double am = 0., bm = 1., cm = 0., dm = 0., ap = 0., bp = 1., cp = 0., dp = 0.;
+      if (im >= 0) { am = a[im]; bm = b[im]; cm = cc[im]; dm = d[im]; }
+      if (ip <= 63) { ap = a[ip]; bp = b[ip]; cp = cc[ip]; dp = d[ip]; }
+      double k1 = a[l] / bm, k2 = cc[l] / bp;
+      na[l] = -(am * k1);
+      nc[l] = -(cp * k2);
+      nb[l] = b[l] - cm * k1 - ap * k2;
+      nd[l] = d[l] - dm * k1 - dp * k2;
+    }
+    for (int l = 0; l < 64; ++l) { a[l] = na[l]; b[l] = nb[l]; cc[l] = nc[l]; d[l] = nd[l]; }
+  }
+  for (int l = 0; l < L; ++l) z[l] = d[l] / b[l];
+}
+
+/* z = M^-1 rhs, column by column: parallel cyclic reduction for uniform columns of at
+ * most 64 levels, otherwise a forward then backward Thomas sweep */
 void oc_line_solve(oc_ctx *c, const double *rhs, double *z) {
   const oc_mesh *m = &c->m;
+  if (c->line_L > 0) {
+    int L = c->line_L;
+    for (int k = 0; k < m->ncol; ++k) {
+      int s0 = k * L;
+      pcr_column(c->llo + s0, c->ldg + s0, c->lup + s0, rhs + s0, L, z + s0);
+    }
+    return;
+  }
   for (int k = 0; k < m->ncol; ++k) {
     int s0 = m->col_start[k], s1 = m->col_start[k + 1];
     double dprev = 0.;

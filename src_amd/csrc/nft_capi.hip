@@ -68,15 +68,19 @@ struct nft_ctx {
 
   // node fields (device)
   double *vol, *rh0, *th0, *ue, *ve, *coefw, *alpha, *d1, *dthedz, *pstep;
+  NodeCoef nc{};
   float *volf, *rh0f;
   int32_t *col_start;
   double *la, *linv, *lc;
   float *laf, *linvf, *lcf;
+  int line_L = 0;             // uniform column length (0: not uniform or > 64 -> Thomas)
+  double *llo, *ldg, *lup;    // line operator rows (PCR)
+  float *llof, *ldgf, *lupf;
   double *zbuf;
   float *zbuff;
 
   // host copies of set-up fields
-  std::vector<double> h_alpha, h_d1, h_coefw, h_la, h_linv, h_lc;
+  std::vector<double> h_alpha, h_d1, h_coefw, h_la, h_linv, h_lc, h_lo, h_dg, h_up;
   double beta1 = 0;
 
   // state
@@ -270,6 +274,7 @@ void setup_fields(nft_ctx *c, const nft_mesh *m, std::vector<double> &rh0, std::
         }
       }
     }
+    c->h_lo.assign(n, 0.); c->h_dg.assign(n, 0.); c->h_up.assign(n, 0.);
     for (int k = 0; k < m->ncol; ++k) {
       int s0 = m->col_start[k], s1 = m->col_start[k + 1];
       double cprev = 0.;
@@ -280,8 +285,14 @@ void setup_fields(nft_ctx *c, const nft_mesh *m, std::vector<double> &rh0, std::
         double cp = (i == s1 - 1) ? 0. : up[i] * inv;
         c->h_la[i] = a; c->h_linv[i] = inv; c->h_lc[i] = cp;
         cprev = cp;
+        c->h_lo[i] = a; c->h_dg[i] = diag[i]; c->h_up[i] = (i == s1 - 1) ? 0. : up[i];
       }
     }
+    int L0 = m->col_start[1] - m->col_start[0];
+    bool uniform = L0 <= 64;
+    for (int k = 0; k < m->ncol && uniform; ++k)
+      if (m->col_start[k + 1] - m->col_start[k] != L0 || m->col_start[k] != k * L0) uniform = false;
+    c->line_L = uniform ? L0 : 0;
   }
 }
 
@@ -321,25 +332,45 @@ int grad_launch(nft_ctx *c, int mode, const GradArgs &a) {
 
 GradArgs grad_args(nft_ctx *c) {
   GradArgs a{};
-  a.vol = c->vol; a.rh0 = c->rh0; a.coefw = c->coefw; a.ue = c->ue; a.dt = c->P.dt;
+  a.nc = c->nc; a.dt = c->P.dt;
   return a;
 }
 
 DivArgs div_args(nft_ctx *c) {
   DivArgs a{};
-  a.vol = c->vol; a.rh0 = c->rh0; a.partials = c->partials; a.counter = c->counter;
+  a.nc = c->nc; a.partials = c->partials; a.counter = c->counter;
   a.scal = c->scal; a.epa = 1.e-38;
   return a;
 }
 
-int div_launch(nft_ctx *c, int mode, const DivArgs &a) {
-  dim3 g(c->gather_grid), b(NFT_BLOCK);
-  if (mode == DIV_LAPLC) {
-    t_begin(c, KT_LDIV);
-    hipLaunchKernelGGL(k_div<DIV_LAPLC>, g, b, 0, c->stream, c->inc, c->S, a);
-    t_end(c, KT_LDIV);
+template <int MODE, int ND, int NMAX>
+void div_go(nft_ctx *c, const DivArgs &a) {
+  hipLaunchKernelGGL((k_div<MODE, ND, NMAX>), dim3(c->gather_grid), dim3(NFT_BLOCK), 0, c->stream, c->inc, c->S, a);
+}
+
+template <int ND>
+bool div_laplc_nd(nft_ctx *c, const DivArgs &a, int nd) {
+  if constexpr (ND > NFT_MAXD) {
+    return false;
   } else {
-    hipLaunchKernelGGL(k_div<DIV_DIV>, g, b, 0, c->stream, c->inc, c->S, a);
+    if (nd == ND) { div_go<DIV_LAPLC, ND, 0>(c, a); return true; }
+    return div_laplc_nd<ND + 1>(c, a, nd);
+  }
+}
+
+int div_launch(nft_ctx *c, int mode, const DivArgs &a) {
+  const int nd = a.dots.nd, nmax = a.dots.nmax ? 1 : 0;
+  if (mode == DIV_LAPLC) {
+    if (nmax) return set_err(-1, "laplc: max reduction not supported");
+    t_begin(c, KT_LDIV);
+    bool ok = div_laplc_nd<0>(c, a, nd);
+    t_end(c, KT_LDIV);
+    if (!ok) return set_err(-1, "laplc: too many fused dots");
+  } else {
+    if (nd == 0 && nmax == 0) div_go<DIV_DIV, 0, 0>(c, a);
+    else if (nd == 1 && nmax == 1) div_go<DIV_DIV, 1, 1>(c, a);
+    else if (nd == 1 && nmax == 0) div_go<DIV_DIV, 1, 0>(c, a);
+    else return set_err(-1, "div: unsupported fused reduction");
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -394,14 +425,27 @@ int precon_dev(nft_ctx *c, const double *rhs, double *p, int impfl) {
         hipLaunchKernelGGL(k_sub, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, n, rhs, c->rw, c->t[3]);
         res = c->t[3];
       }
-      int cg = blocks_for(c->ncol);
       t_begin(c, KT_LINE);
-      if (P.precision == 1)
-        hipLaunchKernelGGL(k_line<float>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->col_start,
-                           c->laf, c->linvf, c->lcf, res, it == 1 ? nullptr : p, p, P.omega, c->zbuff);
-      else
-        hipLaunchKernelGGL(k_line<double>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->col_start,
-                           c->la, c->linv, c->lc, res, it == 1 ? nullptr : p, p, P.omega, c->zbuf);
+      const double *pin = it == 1 ? nullptr : p;
+      if (c->line_L > 0) {
+        int cpw = 64 / c->line_L;
+        long waves = (c->ncol + cpw - 1) / cpw;
+        int cg = (int)((waves + NFT_WPB - 1) / NFT_WPB);
+        if (P.precision == 1)
+          hipLaunchKernelGGL(k_line_pcr<float>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->line_L,
+                             c->llof, c->ldgf, c->lupf, res, pin, p, P.omega);
+        else
+          hipLaunchKernelGGL(k_line_pcr<double>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->line_L,
+                             c->llo, c->ldg, c->lup, res, pin, p, P.omega);
+      } else {
+        int cg = blocks_for(c->ncol);
+        if (P.precision == 1)
+          hipLaunchKernelGGL(k_line<float>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->col_start,
+                             c->laf, c->linvf, c->lcf, res, pin, p, P.omega, c->zbuff);
+        else
+          hipLaunchKernelGGL(k_line<double>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->col_start,
+                             c->la, c->linv, c->lc, res, pin, p, P.omega, c->zbuf);
+      }
       t_end(c, KT_LINE);
     }
     HIPCHK(hipGetLastError());
@@ -758,7 +802,7 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
     g.xcd_per = 0;
     c->gather_grid = g.nblocks;
   }
-  c->vec_grid = std::min(blocks_for(n), 1024);
+  c->vec_grid = std::min(blocks_for(n), 4096);
   c->nodes_grid = blocks_for(n);
   double *dsx, *dsy, *dsz;
   if ((rc = upload(c, &dsx, sx.data(), etot)) || (rc = upload(c, &dsy, sy.data(), etot)) ||
@@ -779,8 +823,19 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
       (rc = upload(c, &c->lc, c->h_lc.data(), n)))
     return fail(rc);
   if (m->ncol > 0) {
+    if ((rc = upload(c, &c->llo, c->h_lo.data(), n)) || (rc = upload(c, &c->ldg, c->h_dg.data(), n)) ||
+        (rc = upload(c, &c->lup, c->h_up.data(), n)))
+      return fail(rc);
+  }
+  if (m->ncol > 0) {
     if ((rc = upload(c, &c->col_start, m->col_start, m->ncol + 1))) return fail(rc);
   }
+  // exact shortcuts for uniform node coefficients
+  c->nc.vol = c->vol; c->nc.rh0 = c->rh0; c->nc.coefw = c->coefw; c->nc.ue = c->ue;
+  c->nc.rh0_one = std::all_of(rh0.begin(), rh0.end(), [](double v) { return v == 1.0; }) ? 1 : 0;
+  c->nc.coefw0 = c->h_coefw[0];
+  c->nc.coefw_uniform =
+      std::all_of(c->h_coefw.begin(), c->h_coefw.end(), [&](double v) { return v == c->nc.coefw0; }) ? 1 : 0;
   if ((rc = dalloc(c, &c->zbuf, n))) return fail(rc);
   // FP32 copies for the mixed-precision form
   if (P->precision == 1) {
@@ -795,6 +850,11 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
         (rc = upload(c, &c->laf, (f = tof(c->h_la)).data(), n)) || (rc = upload(c, &c->linvf, (f = tof(c->h_linv)).data(), n)) ||
         (rc = upload(c, &c->lcf, (f = tof(c->h_lc)).data(), n)) || (rc = dalloc(c, &c->zbuff, n)))
       return fail(rc);
+    if (m->ncol > 0) {
+      if ((rc = upload(c, &c->llof, (f = tof(c->h_lo)).data(), n)) || (rc = upload(c, &c->ldgf, (f = tof(c->h_dg)).data(), n)) ||
+          (rc = upload(c, &c->lupf, (f = tof(c->h_up)).data(), n)))
+        return fail(rc);
+    }
     for (int k = 0; k < 3; ++k)
       if ((rc = dalloc(c, &c->uuuf[k], n))) return fail(rc);
   }
@@ -1092,7 +1152,8 @@ static double kernel_bytes(nft_ctx *c, int k) {
   switch (k) {
     case KT_LGRAD: return geom + 8 * N /*p*/ + 4 * 8 * N /*vol rh0 coefw ue*/ + 3 * 8 * N /*q*/;
     case KT_LDIV: return geom + 3 * 8 * N /*q*/ + 2 * 8 * N /*vol rh0*/ + 8 * N /*out*/;
-    case KT_LINE: return 4 * (c->ncol + 1) + 3 * ts * N /*factors*/ + 8 * N /*rhs*/ + 8 * N /*p*/ + 2 * ts * N /*z store+load*/;
+    case KT_LINE: return c->line_L > 0 ? 3 * ts * N /*rows*/ + 8 * N /*rhs*/ + 8 * N /*p*/
+                                       : 4 * (c->ncol + 1) + 3 * ts * N /*factors*/ + 8 * N + 8 * N + 2 * ts * N /*z*/;
     case KT_UPD: return 6 * 8 * N;  /* p,r read+write; x, ax read */
     case KT_DIR: return (3 + 2.0 * c->P.lord) * 8 * N;  /* qr, ar, r + up to lord x/ax; writes 2 */
     case KT_MP0: return (mixed ? geomf : geom) + 3 * ts * N + 4 * 8 * N + 4 * 8 * N + 2 * ts * N + 8 * N;

@@ -168,9 +168,11 @@ __device__ bool grid_reduce(double (&v)[K], unsigned ismax, double *partials, un
 
 // ------------------------------------------------------------------------------------
 // Boundary conditions on one node (bcv, reference :1583-1643): entries in order.
+// u at flags 7/8 becomes ue*flag; ue is read only there.
 // ------------------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void bcv_node(const Incidence &g, int i, T &u, T &v, T &w, T ue_flag) {
+__device__ __forceinline__ void bcv_node(const Incidence &g, int i, T &u, T &v, T &w, const double *ue,
+                                         double flag) {
   int b0 = g.bp_ptr[i], b1 = g.bp_ptr[i + 1];
   for (int b = b0; b < b1; ++b) {
     int f = g.bp_flag[b];
@@ -181,9 +183,26 @@ __device__ __forceinline__ void bcv_node(const Incidence &g, int i, T &u, T &v, 
       v = v - ay * vn;
       w = w - az * vn;
     }
-    if (f == 7 || f == 8) u = ue_flag;
+    if (f == 7 || f == 8) u = (T)(ue[i] * flag);
   }
 }
+
+// Node coefficients with exact shortcuts: when every rh0 is 1.0 the reference's products
+// and quotients by rh0 are identities and rh0 is not read; a uniform coefw is passed by value.
+struct NodeCoef {
+  const double *vol, *rh0, *coefw, *ue;
+  int rh0_one;
+  int coefw_uniform;
+  double coefw0;
+};
+
+__device__ __forceinline__ double rho(const NodeCoef &c, int i) { return c.rh0_one ? 1.0 : c.rh0[i]; }
+__device__ __forceinline__ double cw(const NodeCoef &c, int i) { return c.coefw_uniform ? c.coefw0 : c.coefw[i]; }
+
+// Slots are consumed NFT_BATCH at a time: the batch's indices, then all its gathers, are
+// issued before any use, so a wavefront keeps several independent loads in flight; the
+// sums still run slot by slot in the reference's edge order.
+#define NFT_BATCH 4
 
 // ------------------------------------------------------------------------------------
 // Gradient of one field (derivative x3) with per-mode epilogues.
@@ -199,7 +218,7 @@ struct GradArgs {
   const double *f;
   double *o0, *o1, *o2;
   const double *s0, *s1, *s2;  // ustr, vstr, wstr (PRFR)
-  const double *vol, *rh0, *coefw, *ue;
+  NodeCoef nc;
   double dt;
   int impfl;
 };
@@ -212,16 +231,29 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S
   double gx = 0., gy = 0., gz = 0.;
   if (MODE != GRAD_ZERO_PRFR) {
     double fi = L.valid ? a.f[i] : 0.;
-    for (int m = 0; m < L.width; ++m) {
-      if (m < L.deg) {
-        int slot = L.base + m * NFT_WAVE + L.lane;
-        int j = g.nbr[slot], e = g.eid[slot];
-        bool first = e >= 0;
-        int ee = first ? e : ~e;
-        double u = (fi + a.f[j]) * 0.5;
-        double sx = S.sx[ee], sy = S.sy[ee], sz = S.sz[ee];
-        if (first) { gx = gx + sx * u; gy = gy + sy * u; gz = gz + sz * u; }
-        else { gx = gx - sx * u; gy = gy - sy * u; gz = gz - sz * u; }
+    for (int m0 = 0; m0 < L.width; m0 += NFT_BATCH) {
+      int jj[NFT_BATCH], ee[NFT_BATCH];
+#pragma unroll
+      for (int u = 0; u < NFT_BATCH; ++u) {
+        int slot = L.base + (m0 + u) * NFT_WAVE + L.lane;
+        bool ok = m0 + u < L.deg;
+        jj[u] = ok ? g.nbr[slot] : 0;
+        ee[u] = ok ? g.eid[slot] : 0;
+      }
+      double fj[NFT_BATCH], sx[NFT_BATCH], sy[NFT_BATCH], sz[NFT_BATCH];
+#pragma unroll
+      for (int u = 0; u < NFT_BATCH; ++u) {
+        int e = ee[u] >= 0 ? ee[u] : ~ee[u];
+        fj[u] = a.f[jj[u]];
+        sx[u] = S.sx[e]; sy[u] = S.sy[e]; sz[u] = S.sz[e];
+      }
+#pragma unroll
+      for (int u = 0; u < NFT_BATCH; ++u) {
+        if (m0 + u < L.deg) {
+          double v = (fi + fj[u]) * 0.5;
+          if (ee[u] >= 0) { gx = gx + sx[u] * v; gy = gy + sy[u] * v; gz = gz + sz[u] * v; }
+          else { gx = gx - sx[u] * v; gy = gy - sy[u] * v; gz = gz - sz[u] * v; }
+        }
       }
     }
     if (L.valid) {
@@ -238,9 +270,9 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S
   } else if (MODE == GRAD_PRFR || MODE == GRAD_ZERO_PRFR) {
     double u, v, w;
     if (MODE == GRAD_PRFR) {
-      double vl = a.vol[i];
+      double vl = a.nc.vol[i];
       gx = gx / vl; gy = gy / vl; gz = gz / vl;
-      double fz = a.impfl ? a.coefw[i] : 1.0;
+      double fz = a.impfl ? cw(a.nc, i) : 1.0;
       double h = 0.5 * a.dt;
       u = a.s0[i] - h * gx;
       v = a.s1[i] - h * gy;
@@ -248,22 +280,26 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S
     } else {
       u = a.s0[i]; v = a.s1[i]; w = a.s2[i];
     }
-    bcv_node<double>(g, i, u, v, w, a.ue[i] * 1.0);
+    bcv_node<double>(g, i, u, v, w, a.nc.ue, 1.0);
     a.o0[i] = u; a.o1[i] = v; a.o2[i] = w;
   } else {  // GRAD_LAPLC
-    double vl = a.vol[i];
-    double fz = a.impfl ? a.coefw[i] : 1.0;
+    double vl = a.nc.vol[i];
+    double fz = a.impfl ? cw(a.nc, i) : 1.0;
     gx = a.dt * gx / vl; gy = a.dt * gy / vl; gz = a.dt * gz / vl;
     gx = 0.5 * gx; gy = 0.5 * gy; gz = 0.5 * gz * fz;
-    bcv_node<double>(g, i, gx, gy, gz, a.ue[i] * 0.0);
-    double r = a.rh0[i];
-    a.o0[i] = gx * r; a.o1[i] = gy * r; a.o2[i] = gz * r;
+    bcv_node<double>(g, i, gx, gy, gz, a.nc.ue, 0.0);
+    if (a.nc.rh0_one) {
+      a.o0[i] = gx; a.o1[i] = gy; a.o2[i] = gz;
+    } else {
+      double r = a.nc.rh0[i];
+      a.o0[i] = gx * r; a.o1[i] = gy * r; a.o2[i] = gz * r;
+    }
   }
 }
 
 // ------------------------------------------------------------------------------------
 // Divergence of three fields (derivative(q1,.,1)+derivative(q2,.,2)+derivative(q3,.,3))
-// with optional fused inner products of the result against up to NFT_MAXD vectors.
+// with ND fused inner products of the result (and optionally max|out|).
 // ------------------------------------------------------------------------------------
 enum DivMode {
   DIV_LAPLC = 0,   // out = ((ax+ay)+az)/(vol*rh0)                                (laplc 2nd half)
@@ -273,9 +309,9 @@ enum DivMode {
 struct DotList {
   int nd;                         // number of dot products a_k . out
   const double *a[NFT_MAXD];      // nullptr means `out` itself
-  int fin;                        // finalize mode (see k_div)
-  int slot;                       // GCR slot for FIN_INIT / FIN_DEL
-  int nmax;                       // nonzero: also reduce max|out| (last entry)
+  int fin;                        // finalize mode (see gcr_finalize)
+  int slot;                       // GCR slot for FIN_BETA
+  int nmax;                       // nonzero: also reduce max|out| (entry nd)
 };
 
 enum Finalize { FIN_NONE = 0, FIN_BETA = 1, FIN_DEL = 2, FIN_NORMS = 3 };
@@ -292,7 +328,7 @@ enum Scal {
 struct DivArgs {
   const double *q0, *q1, *q2;
   double *out;
-  const double *vol, *rh0;
+  NodeCoef nc;
   int iflg;
   DotList dots;
   double *partials;
@@ -314,41 +350,53 @@ __device__ __forceinline__ void gcr_finalize(const DotList &d, const double *red
     for (int ii = 0; ii < d.nd; ++ii) scal[SC_DEL + ii] = -red[ii] / scal[SC_AX2 + ii];
   } else if (d.fin == FIN_NORMS) {
     scal[SC_R2] = red[0];
-    scal[SC_RMX] = red[NFT_MAXD];
+    if (d.nmax) scal[SC_RMX] = red[d.nd];
   }
 }
 
-template <int MODE>
+template <int MODE, int ND, int NMAX>
 __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S, DivArgs a) {
-  __shared__ double sh[(NFT_MAXD + 1) * NFT_WPB];
+  constexpr int K = ND + NMAX;
+  __shared__ double sh[(K > 0 ? K : 1) * NFT_WPB];
   Lane L;
   bool active = lane_setup(g, L);
+  if (K == 0 && !active) return;
   // lanes of blocks past the end still take part in the block reduction
   double out = 0.;
   if (active) {
     const int i = L.i;
+    const bool scale = MODE == DIV_DIV && a.iflg == -1 && !a.nc.rh0_one;
     double ax = 0., ay = 0., az = 0.;
-    double r0 = (MODE == DIV_DIV && a.iflg == -1 && L.valid) ? a.rh0[i] : 1.0;
     double qi0 = 0., qi1 = 0., qi2 = 0.;
     if (L.valid) {
       qi0 = a.q0[i]; qi1 = a.q1[i]; qi2 = a.q2[i];
-      if (MODE == DIV_DIV && a.iflg == -1) { qi0 = qi0 * r0; qi1 = qi1 * r0; qi2 = qi2 * r0; }
+      if (scale) { double r0 = a.nc.rh0[i]; qi0 = qi0 * r0; qi1 = qi1 * r0; qi2 = qi2 * r0; }
     }
-    for (int m = 0; m < L.width; ++m) {
-      if (m < L.deg) {
-        int slot = L.base + m * NFT_WAVE + L.lane;
-        int j = g.nbr[slot], e = g.eid[slot];
-        bool first = e >= 0;
-        int ee = first ? e : ~e;
-        double qj0 = a.q0[j], qj1 = a.q1[j], qj2 = a.q2[j];
-        if (MODE == DIV_DIV && a.iflg == -1) {
-          double rj = a.rh0[j];
-          qj0 = qj0 * rj; qj1 = qj1 * rj; qj2 = qj2 * rj;
+    for (int m0 = 0; m0 < L.width; m0 += NFT_BATCH) {
+      int jj[NFT_BATCH], ee[NFT_BATCH];
+#pragma unroll
+      for (int u = 0; u < NFT_BATCH; ++u) {
+        int slot = L.base + (m0 + u) * NFT_WAVE + L.lane;
+        bool ok = m0 + u < L.deg;
+        jj[u] = ok ? g.nbr[slot] : 0;
+        ee[u] = ok ? g.eid[slot] : 0;
+      }
+      double q0[NFT_BATCH], q1[NFT_BATCH], q2[NFT_BATCH], sx[NFT_BATCH], sy[NFT_BATCH], sz[NFT_BATCH];
+#pragma unroll
+      for (int u = 0; u < NFT_BATCH; ++u) {
+        int e = ee[u] >= 0 ? ee[u] : ~ee[u];
+        int j = jj[u];
+        q0[u] = a.q0[j]; q1[u] = a.q1[j]; q2[u] = a.q2[j];
+        sx[u] = S.sx[e]; sy[u] = S.sy[e]; sz[u] = S.sz[e];
+        if (scale) { double rj = a.nc.rh0[j]; q0[u] = q0[u] * rj; q1[u] = q1[u] * rj; q2[u] = q2[u] * rj; }
+      }
+#pragma unroll
+      for (int u = 0; u < NFT_BATCH; ++u) {
+        if (m0 + u < L.deg) {
+          double u0 = (qi0 + q0[u]) * 0.5, u1 = (qi1 + q1[u]) * 0.5, u2 = (qi2 + q2[u]) * 0.5;
+          if (ee[u] >= 0) { ax = ax + sx[u] * u0; ay = ay + sy[u] * u1; az = az + sz[u] * u2; }
+          else { ax = ax - sx[u] * u0; ay = ay - sy[u] * u1; az = az - sz[u] * u2; }
         }
-        double u0 = (qi0 + qj0) * 0.5, u1 = (qi1 + qj1) * 0.5, u2 = (qi2 + qj2) * 0.5;
-        double sx = S.sx[ee], sy = S.sy[ee], sz = S.sz[ee];
-        if (first) { ax = ax + sx * u0; ay = ay + sy * u1; az = az + sz * u2; }
-        else { ax = ax - sx * u0; ay = ay - sy * u1; az = az - sz * u2; }
       }
     }
     if (L.valid) {
@@ -358,29 +406,30 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
         az = az + g.bp_sbb[2 * g.nbp + b] * qi2;
       }
       if (MODE == DIV_LAPLC) {
-        out = (ax + ay + az) / (a.vol[i] * a.rh0[i]);
+        out = a.nc.rh0_one ? (ax + ay + az) / a.nc.vol[i] : (ax + ay + az) / (a.nc.vol[i] * a.nc.rh0[i]);
       } else {
-        double vl = a.vol[i];
-        out = (double)a.iflg * (ax / vl + ay / vl + az / vl) / a.rh0[i];
+        double vl = a.nc.vol[i];
+        double t = (double)a.iflg * (ax / vl + ay / vl + az / vl);
+        out = a.nc.rh0_one ? t : t / a.nc.rh0[i];
       }
       a.out[i] = out;
     }
     if (!L.valid) out = 0.;
   }
-  if (a.dots.nd == 0 && a.dots.nmax == 0) return;
-  // fused reductions over the block grid (every block participates)
-  double v[NFT_MAXD + 1];
+  if constexpr (K > 0) {
+    // fused reductions over the block grid (every block participates)
+    double v[K];
 #pragma unroll
-  for (int k = 0; k <= NFT_MAXD; ++k) v[k] = 0.;
-  if (active && L.valid) {
+    for (int k = 0; k < K; ++k) v[k] = 0.;
+    if (active && L.valid) {
 #pragma unroll
-    for (int k = 0; k < NFT_MAXD; ++k)
-      if (k < a.dots.nd) v[k] = (a.dots.a[k] ? a.dots.a[k][L.i] : out) * out;
-    if (a.dots.nmax) v[NFT_MAXD] = fabs(out);
-  }
-  double red[NFT_MAXD + 1];
-  if (grid_reduce<NFT_MAXD + 1>(v, 1u << NFT_MAXD, a.partials, a.counter, gridDim.x, blockIdx.x, sh, red)) {
-    if (threadIdx.x == 0) gcr_finalize(a.dots, red, a.scal, a.epa);
+      for (int k = 0; k < ND; ++k) v[k] = (a.dots.a[k] ? a.dots.a[k][L.i] : out) * out;
+      if (NMAX) v[ND] = fabs(out);
+    }
+    double red[K];
+    if (grid_reduce<K>(v, NMAX ? (1u << ND) : 0u, a.partials, a.counter, gridDim.x, blockIdx.x, sh, red)) {
+      if (threadIdx.x == 0) gcr_finalize(a.dots, red, a.scal, a.epa);
+    }
   }
 }
 
@@ -394,13 +443,26 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_gcr_update(int n, double *p, doub
   __shared__ double sh[2 * NFT_WPB];
   const double beta = scal[SC_BETA];
   double v[2] = {-1.e15, 0.};
-  for (int k = blockIdx.x * NFT_BLOCK + threadIdx.x; k < n; k += gridDim.x * NFT_BLOCK) {
-    double pk = p[k] + beta * x[k];
-    double rk = r[k] + beta * ax[k];
-    p[k] = pk;
-    r[k] = rk;
-    v[0] = fmax(v[0], fabs(rk));
-    v[1] = v[1] + rk * rk;
+  const int stride = gridDim.x * NFT_BLOCK;
+  for (int k0 = blockIdx.x * NFT_BLOCK + threadIdx.x; k0 < n; k0 += 4 * stride) {
+    double pk[4], rk[4], xk[4], ak[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      int k = k0 + u * stride;
+      if (k < n) { pk[u] = p[k]; rk[u] = r[k]; xk[u] = x[k]; ak[u] = ax[k]; }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      int k = k0 + u * stride;
+      if (k < n) {
+        double pn = pk[u] + beta * xk[u];
+        double rn = rk[u] + beta * ak[u];
+        p[k] = pn;
+        r[k] = rn;
+        v[0] = fmax(v[0], fabs(rn));
+        v[1] = v[1] + rn * rn;
+      }
+    }
   }
   double red[2];
   if (grid_reduce<2>(v, 1u, partials, counter, gridDim.x, blockIdx.x, sh, red)) {
@@ -473,8 +535,8 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_norms(int n, const double *a, dou
 // ------------------------------------------------------------------------------------
 // Preconditioners
 // ------------------------------------------------------------------------------------
-// vertical line: forward/backward Thomas sweep per column, then p = p + omega*z
-// (it=1: p = 0 + omega*M^-1 rhs; it>1: rhs - L p is passed in res).
+// vertical line, general columns: forward/backward Thomas sweep per column, then
+// p = p + omega*z (it=1: p = 0 + omega*M^-1 rhs; it>1: rhs - L p is passed in res).
 template <typename T>
 __global__ void __launch_bounds__(NFT_BLOCK) k_line(int ncol, const int32_t *col_start, const T *la,
                                                     const T *linv, const T *lc, const double *res,
@@ -493,6 +555,47 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_line(int ncol, const int32_t *col
   for (int i = s1 - 1; i >= s0; --i) {
     T z = (i == s1 - 1) ? zbuf[i] : zbuf[i] - lc[i] * znext;
     znext = z;
+    double p0 = pin ? pin[i] : 0.;
+    p[i] = p0 + omega * (double)z;
+  }
+}
+
+// vertical line, columns of equal length L <= 64: a wavefront holds 64/L columns, one
+// level per lane, and solves them by parallel cyclic reduction (6 steps of distance
+// 1..32, lanes past the columns are identity rows). Rows couple only inside their column:
+// the end rows have a = 0 / c = 0, which zeroes every cross-column term exactly.
+template <typename T>
+__device__ __forceinline__ T lane_get(T v, int src) { return __shfl(v, src, 64); }
+
+template <typename T>
+__global__ void __launch_bounds__(NFT_BLOCK) k_line_pcr(int ncol, int L, const T *lo, const T *dg,
+                                                        const T *up, const double *res, const double *pin,
+                                                        double *p, double omega) {
+  const int lane = threadIdx.x & 63;
+  const int wave = (blockIdx.x * NFT_BLOCK + threadIdx.x) >> 6;
+  const int cpw = 64 / L;
+  const int col = wave * cpw + lane / L;
+  const bool real = lane < cpw * L && col < ncol;
+  const int i = col * L + lane % L;
+  T a = (T)0, b = (T)1, c = (T)0, d = (T)0;
+  if (real) { a = lo[i]; b = dg[i]; c = up[i]; d = (T)res[i]; }
+  for (int s = 1; s < 64; s <<= 1) {
+    int im = lane - s, ip = lane + s;
+    T am = lane_get(a, im < 0 ? lane : im), bm = lane_get(b, im < 0 ? lane : im);
+    T cm = lane_get(c, im < 0 ? lane : im), dm = lane_get(d, im < 0 ? lane : im);
+    T ap = lane_get(a, ip > 63 ? lane : ip), bp = lane_get(b, ip > 63 ? lane : ip);
+    T cp = lane_get(c, ip > 63 ? lane : ip), dp = lane_get(d, ip > 63 ? lane : ip);
+    if (im < 0) { am = (T)0; bm = (T)1; cm = (T)0; dm = (T)0; }
+    if (ip > 63) { ap = (T)0; bp = (T)1; cp = (T)0; dp = (T)0; }
+    T k1 = a / bm, k2 = c / bp;
+    T na = -(am * k1);
+    T nc = -(cp * k2);
+    T nb = b - cm * k1 - ap * k2;
+    T nd = d - dm * k1 - dp * k2;
+    a = na; b = nb; c = nc; d = nd;
+  }
+  if (real) {
+    T z = d / b;
     double p0 = pin ? pin[i] : 0.;
     p[i] = p0 + omega * (double)z;
   }
@@ -529,7 +632,7 @@ __global__ void k_bcv(Incidence g, double *u, double *v, double *w, const double
   int i = blockIdx.x * NFT_BLOCK + threadIdx.x;
   if (i >= g.n) return;
   double a = u[i], b = v[i], c = w[i];
-  bcv_node<double>(g, i, a, b, c, ue[i] * flag);
+  bcv_node<double>(g, i, a, b, c, ue, flag);
   u[i] = a; v[i] = b; w[i] = c;
 }
 
