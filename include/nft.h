@@ -140,6 +140,31 @@ int nft_get_gcr_stats(nft_ctx *ctx, nft_gcr_stats *stats);
 int nft_kernel_timer(nft_ctx *ctx, int kernel);
 int nft_kernel_timer_read(nft_ctx *ctx, int *launches, double *total_ms, double *bytes_per_launch);
 
+/* ---- learned preconditioner (no reference counterpart) ------------------------------
+ * A column MLP with two horizontal graph convolutions (bf16 MFMA, FP32 accumulation)
+ * adds a learned correction to the vertical line preconditioner inside the flexible GCR;
+ * when an iteration stalls (rl2 > stall_ratio * rl2_prev, or no decrease) the solve drops
+ * the correction and continues with the line preconditioner alone. Needs uniform columns
+ * and precon = 2. Parameters: per layer W (K x N, row-major) then b (N); layers
+ * Kp->H, H->H, 2H->H, 2H->H, H->H, H->Kp with Kp = levels rounded up to a multiple of 8. */
+int nft_net_init(nft_ctx *ctx, int hidden, uint64_t seed, int max_batch);
+int nft_net_info(nft_ctx *ctx, int *kp, int *levels, int *ncol, int64_t *nparams);
+int nft_net_params(nft_ctx *ctx, float *host, int64_t count, int set);
+/* forward (and with T, loss + backward) on a host batch [nsamples*ncol][Kp] */
+int nft_net_eval(nft_ctx *ctx, const float *X, const float *T, int nsamples, float *Y, float *grad,
+                 double *loss);
+/* capture training pairs X = r/s, T = ((p_k - p_final) - M^-1 r)/s, s = max|r|, from the
+ * converged GCR solves that follow (max_stage iterations staged per solve) */
+int nft_dump_pairs(nft_ctx *ctx, int max_pairs, int max_stage);
+/* pair store: count, copy out [first, first+n_get), append n_put host pairs */
+int nft_pairs(nft_ctx *ctx, int *count, float *X, float *T, int first, int n_get, const float *Xin,
+              const float *Tin, int n_put);
+int nft_net_train(nft_ctx *ctx, int steps, int batch, double lr, uint64_t seed, double *loss_first,
+                  double *loss_last);
+/* kind 0: line preconditioner only; 1: learned correction + line fallback. kind < 0 only
+ * reads the number of fallbacks so far. */
+int nft_set_learned(nft_ctx *ctx, int kind, double stall_ratio, int *fallbacks);
+
 #ifdef __cplusplus
 }
 #endif

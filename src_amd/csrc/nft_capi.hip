@@ -22,6 +22,7 @@
 
 #include "../../include/nft.h"
 #include "nft_kernels.h"
+#include "nft_net.h"
 
 using namespace nft;
 
@@ -74,6 +75,7 @@ struct nft_ctx {
   double *la, *linv, *lc;
   float *laf, *linvf, *lcf;
   int line_L = 0;             // uniform column length (0: not uniform or > 64 -> Thomas)
+  int col_L = 0;              // uniform column length with nodes numbered column by column (0: not)
   double *llo, *ldg, *lup;    // line operator rows (PCR)
   float *llof, *ldgf, *lupf;
   double *zbuf;
@@ -81,6 +83,7 @@ struct nft_ctx {
 
   // host copies of set-up fields
   std::vector<double> h_alpha, h_d1, h_coefw, h_la, h_linv, h_lc, h_lo, h_dg, h_up;
+  std::vector<std::pair<int, int>> h_hedges;   // column pairs joined by same-level edges
   double beta1 = 0;
 
   // state
@@ -101,6 +104,30 @@ struct nft_ctx {
   double *io[8];
 
   nft_gcr_stats st{};
+
+  // learned preconditioner (network, training pairs, inference state)
+  struct Net {
+    int ready = 0, H = 128, L = 0, Kp = 0, ncol = 0, max_rows = 0;
+    int32_t *nptr = nullptr, *nidx = nullptr;
+    int K[6] = {}, N[6] = {};
+    float *w[12] = {}, *g[12] = {}, *m[12] = {}, *v[12] = {};
+    uint16_t *wb[6] = {}, *wt[6] = {};
+    float *X = nullptr, *A1 = nullptr, *C2 = nullptr, *C3 = nullptr, *A4 = nullptr, *A5 = nullptr, *Y = nullptr;
+    float *T = nullptr, *dY = nullptr, *dZ5 = nullptr, *dZ4 = nullptr, *dC3 = nullptr, *dZ3 = nullptr;
+    float *dC2 = nullptr, *dZ2 = nullptr, *dZ1 = nullptr, *P = nullptr, *lpart = nullptr;
+    size_t P_count = 0;
+    int step = 0;
+    float *pX = nullptr, *pT = nullptr;
+    int npairs = 0, max_pairs = 0;
+    double *stage = nullptr;
+    int stage_max = 0, stage_n = 0;
+    std::vector<double> stage_s;
+    int dumping = 0;
+    int kind = 0;          // 0 off, 1 learned correction on top of the line preconditioner
+    double stall = 0.99;   // fall back to the line preconditioner when rl2/rl2_prev exceeds this
+    int active = 0;        // per solve: learned correction in use
+    int fallbacks = 0;
+  } net;
 
   // kernel timer
   int timer_kernel = 0;
@@ -289,10 +316,11 @@ void setup_fields(nft_ctx *c, const nft_mesh *m, std::vector<double> &rh0, std::
       }
     }
     int L0 = m->col_start[1] - m->col_start[0];
-    bool uniform = L0 <= 64;
+    bool uniform = true;
     for (int k = 0; k < m->ncol && uniform; ++k)
       if (m->col_start[k + 1] - m->col_start[k] != L0 || m->col_start[k] != k * L0) uniform = false;
-    c->line_L = uniform ? L0 : 0;
+    c->col_L = uniform ? L0 : 0;
+    c->line_L = uniform && L0 <= 64 ? L0 : 0;
   }
 }
 
@@ -358,6 +386,12 @@ bool div_laplc_nd(nft_ctx *c, const DivArgs &a, int nd) {
   }
 }
 
+int final_launch(nft_ctx *c, int nblk, const FinalOp &op) {
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(NFT_BLOCK), 0, c->stream, c->partials, nblk, op, c->scal);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int div_launch(nft_ctx *c, int mode, const DivArgs &a) {
   const int nd = a.dots.nd, nmax = a.dots.nmax ? 1 : 0;
   if (mode == DIV_LAPLC) {
@@ -373,6 +407,11 @@ int div_launch(nft_ctx *c, int mode, const DivArgs &a) {
     else return set_err(-1, "div: unsupported fused reduction");
   }
   HIPCHK(hipGetLastError());
+  if (nd + nmax > 0) {
+    FinalOp op{};
+    op.kind = FK_DIV; op.K = nd + nmax; op.ismax = nmax ? (1u << nd) : 0u; op.dots = a.dots; op.epa = a.epa;
+    return final_launch(c, c->gather_grid, op);
+  }
   return 0;
 }
 
@@ -453,6 +492,176 @@ int precon_dev(nft_ctx *c, const double *rhs, double *p, int impfl) {
   return 0;
 }
 
+
+// ---- learned preconditioner ------------------------------------------------------------
+constexpr int NET_SPLIT_ROWS = 1024;
+
+int net_gemm(nft_ctx *c, int M, int N, int K, const float *A, int lda, const uint16_t *Bt, const float *bias,
+             int relu, const float *mask, int ldm, float *C, int ldc) {
+  dim3 grid((M + 31) / 32, (N + 31) / 32);
+  hipLaunchKernelGGL(k_net_gemm, grid, dim3(64), 0, c->stream, M, N, K, A, lda, Bt, bias, relu, mask, ldm, C, ldc);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// gradient of weight l from activations A [M][K] and output gradient D [M][N]
+int net_wgrad(nft_ctx *c, int l, int M, const float *A, int lda, const float *D, int ldd) {
+  auto &n = c->net;
+  const int K = n.K[l], N = n.N[l];
+  const int nsplit = (M + NET_SPLIT_ROWS - 1) / NET_SPLIT_ROWS;
+  hipLaunchKernelGGL(k_net_wgrad, dim3((K + 31) / 32, (N + 31) / 32, nsplit), dim3(64), 0, c->stream, M, K, N, A,
+                     lda, D, ldd, NET_SPLIT_ROWS, n.P);
+  hipLaunchKernelGGL(k_net_reduce_splits, dim3((K * N + 255) / 256), dim3(256), 0, c->stream, nsplit, K * N, n.P,
+                     n.g[2 * l]);
+  hipLaunchKernelGGL(k_net_colsum, dim3((N + 127) / 128, nsplit), dim3(128), 0, c->stream, M, N, D, ldd,
+                     NET_SPLIT_ROWS, n.P);
+  hipLaunchKernelGGL(k_net_reduce_splits, dim3((N + 255) / 256), dim3(256), 0, c->stream, nsplit, N, n.P,
+                     n.g[2 * l + 1]);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// forward pass on M = nsamples*ncol rows of n.X -> n.Y (activations kept for backward)
+int net_forward(nft_ctx *c, int M) {
+  auto &n = c->net;
+  const int H = n.H, H2 = 2 * H;
+  int rc;
+  int tb = 256, gb = (M * H + tb - 1) / tb;
+  if ((rc = net_gemm(c, M, H, n.Kp, n.X, n.Kp, n.wt[0], n.w[1], 1, nullptr, 0, n.A1, H))) return rc;
+  if ((rc = net_gemm(c, M, H, H, n.A1, H, n.wt[1], n.w[3], 1, nullptr, 0, n.C2, H2))) return rc;
+  hipLaunchKernelGGL(k_net_agg, dim3(gb), dim3(tb), 0, c->stream, M, n.ncol, H, n.nptr, n.nidx, n.C2, H2, n.C2 + H, H2);
+  if ((rc = net_gemm(c, M, H, H2, n.C2, H2, n.wt[2], n.w[5], 1, nullptr, 0, n.C3, H2))) return rc;
+  hipLaunchKernelGGL(k_net_agg, dim3(gb), dim3(tb), 0, c->stream, M, n.ncol, H, n.nptr, n.nidx, n.C3, H2, n.C3 + H, H2);
+  if ((rc = net_gemm(c, M, H, H2, n.C3, H2, n.wt[3], n.w[7], 1, nullptr, 0, n.A4, H))) return rc;
+  if ((rc = net_gemm(c, M, H, H, n.A4, H, n.wt[4], n.w[9], 1, nullptr, 0, n.A5, H))) return rc;
+  if ((rc = net_gemm(c, M, n.Kp, H, n.A5, H, n.wt[5], n.w[11], 0, nullptr, 0, n.Y, n.Kp))) return rc;
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// loss (MSE over the first L of Kp outputs) and gradients of every parameter into n.g
+int net_backward(nft_ctx *c, int M, double *loss) {
+  auto &n = c->net;
+  const int H = n.H, H2 = 2 * H, Kp = n.Kp;
+  int rc;
+  const int nl = (M * Kp + 255) / 256;
+  const float inv = 1.0f / (float)((size_t)M * n.L);
+  hipLaunchKernelGGL(k_net_loss, dim3(nl), dim3(256), 0, c->stream, M, n.L, Kp, n.Y, n.T, inv, n.dY, n.lpart);
+  int tb = 256, gb = (M * H + tb - 1) / tb;
+  // layer 6: Y = A5 W6 + b6
+  if ((rc = net_wgrad(c, 5, M, n.A5, H, n.dY, Kp))) return rc;
+  if ((rc = net_gemm(c, M, H, Kp, n.dY, Kp, n.wb[5], nullptr, 0, n.A5, H, n.dZ5, H))) return rc;
+  // layer 5: A5 = relu(A4 W5 + b5)
+  if ((rc = net_wgrad(c, 4, M, n.A4, H, n.dZ5, H))) return rc;
+  if ((rc = net_gemm(c, M, H, H, n.dZ5, H, n.wb[4], nullptr, 0, n.A4, H, n.dZ4, H))) return rc;
+  // layer 4: A4 = relu([A3|Agg A3] W4 + b4)
+  if ((rc = net_wgrad(c, 3, M, n.C3, H2, n.dZ4, H))) return rc;
+  if ((rc = net_gemm(c, M, H2, H, n.dZ4, H, n.wb[3], nullptr, 0, nullptr, 0, n.dC3, H2))) return rc;
+  hipLaunchKernelGGL(k_net_agg_back, dim3(gb), dim3(tb), 0, c->stream, M, n.ncol, H, n.nptr, n.nidx, n.dC3, H2, n.C3,
+                     H2, n.dZ3, H);
+  // layer 3: A3 = relu([A2|Agg A2] W3 + b3)
+  if ((rc = net_wgrad(c, 2, M, n.C2, H2, n.dZ3, H))) return rc;
+  if ((rc = net_gemm(c, M, H2, H, n.dZ3, H, n.wb[2], nullptr, 0, nullptr, 0, n.dC2, H2))) return rc;
+  hipLaunchKernelGGL(k_net_agg_back, dim3(gb), dim3(tb), 0, c->stream, M, n.ncol, H, n.nptr, n.nidx, n.dC2, H2, n.C2,
+                     H2, n.dZ2, H);
+  // layer 2: A2 = relu(A1 W2 + b2)
+  if ((rc = net_wgrad(c, 1, M, n.A1, H, n.dZ2, H))) return rc;
+  if ((rc = net_gemm(c, M, H, H, n.dZ2, H, n.wb[1], nullptr, 0, n.A1, H, n.dZ1, H))) return rc;
+  // layer 1: A1 = relu(X W1 + b1)
+  if ((rc = net_wgrad(c, 0, M, n.X, Kp, n.dZ1, H))) return rc;
+  HIPCHK(hipGetLastError());
+  if (loss) {
+    std::vector<float> part(nl);
+    HIPCHK(hipMemcpyAsync(part.data(), n.lpart, nl * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double a = 0;
+    for (int i = 0; i < nl; ++i) a += part[i];
+    *loss = a * inv;
+  }
+  return 0;
+}
+
+int net_alloc(nft_ctx *c, int H, int max_rows, uint64_t seed) {
+  auto &n = c->net;
+  int rc;
+  n.H = H;
+  n.L = c->col_L;
+  n.Kp = (n.L + 7) / 8 * 8;
+  n.ncol = c->ncol;
+  n.max_rows = max_rows;
+  const int Kp = n.Kp, H2 = 2 * H;
+  const int Ks[6] = {Kp, H, H2, H2, H, H}, Ns[6] = {H, H, H, H, H, Kp};
+  // He initialisation from a fixed-seed 64-bit generator (Box-Muller), output layer x0.1
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
+  auto next = [&]() { st ^= st << 13; st ^= st >> 7; st ^= st << 17; return st; };
+  auto unif = [&]() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
+  for (int l = 0; l < 6; ++l) {
+    n.K[l] = Ks[l]; n.N[l] = Ns[l];
+    size_t cnt = (size_t)Ks[l] * Ns[l];
+    std::vector<float> w(cnt), z(cnt, 0.f), zb(Ns[l], 0.f);
+    double std_ = std::sqrt(2.0 / Ks[l]) * (l == 5 ? 0.1 : 1.0);
+    for (size_t i = 0; i < cnt; ++i) {
+      double u1 = unif(), u2 = unif();
+      w[i] = (float)(std_ * std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
+    }
+    // rows of W1 beyond L see only zero padding; keep them zero
+    if (l == 0)
+      for (int k = n.L; k < Kp; ++k)
+        for (int j = 0; j < Ns[l]; ++j) w[(size_t)k * Ns[l] + j] = 0.f;
+    if ((rc = upload(c, &n.w[2 * l], w.data(), cnt)) || (rc = upload(c, &n.g[2 * l], z.data(), cnt)) ||
+        (rc = upload(c, &n.m[2 * l], z.data(), cnt)) || (rc = upload(c, &n.v[2 * l], z.data(), cnt)) ||
+        (rc = upload(c, &n.w[2 * l + 1], zb.data(), Ns[l])) || (rc = upload(c, &n.g[2 * l + 1], zb.data(), Ns[l])) ||
+        (rc = upload(c, &n.m[2 * l + 1], zb.data(), Ns[l])) || (rc = upload(c, &n.v[2 * l + 1], zb.data(), Ns[l])) ||
+        (rc = dalloc(c, &n.wb[l], cnt)) || (rc = dalloc(c, &n.wt[l], cnt)))
+      return rc;
+    hipLaunchKernelGGL(k_net_to_bf16, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, Ks[l], Ns[l], n.w[2 * l],
+                       n.wb[l], n.wt[l]);
+  }
+  const size_t R = max_rows;
+  if ((rc = dalloc(c, &n.X, R * Kp)) || (rc = dalloc(c, &n.A1, R * H)) || (rc = dalloc(c, &n.C2, R * H2)) ||
+      (rc = dalloc(c, &n.C3, R * H2)) || (rc = dalloc(c, &n.A4, R * H)) || (rc = dalloc(c, &n.A5, R * H)) ||
+      (rc = dalloc(c, &n.Y, R * Kp)) || (rc = dalloc(c, &n.T, R * Kp)) || (rc = dalloc(c, &n.dY, R * Kp)) ||
+      (rc = dalloc(c, &n.dZ5, R * H)) || (rc = dalloc(c, &n.dZ4, R * H)) || (rc = dalloc(c, &n.dC3, R * H2)) ||
+      (rc = dalloc(c, &n.dZ3, R * H)) || (rc = dalloc(c, &n.dC2, R * H2)) || (rc = dalloc(c, &n.dZ2, R * H)) ||
+      (rc = dalloc(c, &n.dZ1, R * H)) || (rc = dalloc(c, &n.lpart, (R * Kp + 255) / 256 + 1)))
+    return rc;
+  n.P_count = (size_t)((R + NET_SPLIT_ROWS - 1) / NET_SPLIT_ROWS) * H2 * std::max(H, Kp);
+  if ((rc = dalloc(c, &n.P, n.P_count))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  n.ready = 1;
+  return 0;
+}
+
+// learned correction on top of a line-preconditioned qr: qr += s * N(r / s), s = scal[sidx]
+int net_apply_precon(nft_ctx *c, const double *r, double *qr, int sidx) {
+  auto &n = c->net;
+  const int M = n.ncol;
+  hipLaunchKernelGGL(k_net_input, dim3((M * n.Kp + 255) / 256), dim3(256), 0, c->stream, n.ncol, n.L, n.Kp, r,
+                     c->scal, sidx, n.X);
+  int rc = net_forward(c, M);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_net_output, dim3((M * n.L + 255) / 256), dim3(256), 0, c->stream, n.ncol, n.L, n.Kp, n.Y,
+                     c->scal, sidx, qr);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// training pairs of a converged solve from the staged (r, qr, p) iterates
+int net_harvest(nft_ctx *c, const double *pfinal) {
+  auto &n = c->net;
+  const size_t N = c->n, per = (size_t)n.ncol * n.Kp;
+  for (int k = 0; k < n.stage_n && n.npairs < n.max_pairs; ++k) {
+    const double *base = n.stage + (size_t)k * 3 * N;
+    hipLaunchKernelGGL(k_net_make_pair, dim3((per + 255) / 256), dim3(256), 0, c->stream, n.ncol, n.L, n.Kp, base,
+                       base + N, base + 2 * N, pfinal, n.stage_s[k], n.pX + per * n.npairs, n.pT + per * n.npairs);
+    n.npairs++;
+  }
+  n.stage_n = 0;
+  n.stage_s.clear();
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int read_scal(nft_ctx *c, int first, int count) {
   HIPCHK(hipMemcpyAsync(c->h_scal + first, c->scal + first, count * sizeof(double), hipMemcpyDeviceToHost,
                         c->stream));
@@ -465,7 +674,9 @@ int norms_dev(nft_ctx *c, const double *a, int sum_slot, int max_slot) {
   hipLaunchKernelGGL(k_norms, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, a, (double *)nullptr,
                      c->scal, sum_slot, max_slot, c->partials, c->counter);
   HIPCHK(hipGetLastError());
-  return 0;
+  FinalOp op{};
+  op.kind = FK_NORMS; op.K = 2; op.ismax = 2u; op.sum_slot = sum_slot; op.max_slot = max_slot;
+  return final_launch(c, c->vec_grid, op);
 }
 
 // gcrk (reference :1776-1972) on device arrays. Returns 0 when converged, 1 otherwise.
@@ -489,7 +700,11 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
     if ((rc = div_dev(c, u2, v2, w2, c->r, -1, &dl))) return rc;
   }
   double *x0 = c->gx[0], *ax0 = c->gax[0];
+  c->net.active = c->net.kind == 1 && c->net.ready;
+  c->net.stage_n = 0;
+  c->net.stage_s.clear();
   if ((rc = precon_dev(c, c->r, x0, impfl))) return rc;
+  if (c->net.active && (rc = net_apply_precon(c, c->r, x0, SC_RMX))) return rc;
   if ((rc = norms_dev(c, x0, SC_EER, SC_EEM))) return rc;
   {
     DotList dl{};
@@ -510,15 +725,38 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
       hipLaunchKernelGGL(k_gcr_update, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, p, c->r,
                          c->gx[i - 1], c->gax[i - 1], c->scal, c->partials, c->counter);
       t_end(c, KT_UPD);
+      {
+        FinalOp op{};
+        op.kind = FK_UPDATE; op.K = 2; op.ismax = 1u;
+        if ((rc = final_launch(c, c->vec_grid, op))) return rc;
+      }
       HIPCHK(hipGetLastError());
       if ((rc = read_scal(c, SC_DVMX, 2))) return rc;
       dvmx = c->h_scal[SC_DVMX];
       double rl2 = c->h_scal[SC_RL2];
       if (dvmx <= eps && it >= itmn) { converged = 1; goto done; }
-      if (rl2 >= rl20) goto done;
+      if (rl2 >= rl20) {
+        // stagnation: with the learned correction active, drop it and continue with the
+        // line preconditioner alone instead of ending the solve
+        if (!c->net.active) goto done;
+        c->net.active = 0;
+        c->net.fallbacks++;
+      } else if (c->net.active && rl2 > c->net.stall * rl20) {
+        c->net.active = 0;
+        c->net.fallbacks++;
+      }
       rl20 = std::max(rl2, epa);
       if ((rc = precon_dev(c, c->r, c->qr, impfl))) return rc;
+      if (c->net.active && (rc = net_apply_precon(c, c->r, c->qr, SC_DVMX))) return rc;
       qr_last = c->qr;
+      if (c->net.dumping && c->net.stage_n < c->net.stage_max) {
+        double *dst = c->net.stage + (size_t)c->net.stage_n * 3 * c->n;
+        HIPCHK(hipMemcpyAsync(dst, c->r, c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(dst + c->n, c->qr, c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(dst + 2 * (size_t)c->n, p, c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        c->net.stage_s.push_back(dvmx);
+        c->net.stage_n++;
+      }
       {
         DotList dl{};
         dl.nd = i;
@@ -539,6 +777,9 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
         hipLaunchKernelGGL(k_gcr_dir, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, d);
         t_end(c, KT_DIR);
         HIPCHK(hipGetLastError());
+        FinalOp op{};
+        op.kind = FK_DIR; op.K = 2; op.ismax = 0u; op.slot = slot; op.epa = epa;
+        if ((rc = final_launch(c, c->vec_grid, op))) return rc;
       }
     }
   }
@@ -551,6 +792,11 @@ done:
   c->st.nitsm += nlc;
   c->st.icount += 1;
   c->st.converged = converged;
+  if (c->net.dumping) {
+    if (converged && (rc = net_harvest(c, p))) return rc;
+    c->net.stage_n = 0;
+    c->net.stage_s.clear();
+  }
   c->st.res_max = dvmx;
   c->st.tol = eps;
   return converged ? 0 : 1;
@@ -740,6 +986,18 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   }
   std::vector<int32_t> deg(n, 0);
   for (long e = 0; e < etot; ++e) { deg[i1[e]]++; deg[i2[e]]++; }
+  if (m->ncol > 0) {
+    // horizontal column adjacency (edges between different columns at one level)
+    std::vector<int> colof(n), levof(n);
+    for (int k = 0; k < m->ncol; ++k)
+      for (int i = m->col_start[k]; i < m->col_start[k + 1]; ++i) { colof[i] = k; levof[i] = i - m->col_start[k]; }
+    for (long e = 0; e < etot; ++e) {
+      int a = colof[i1[e]], b = colof[i2[e]];
+      if (a != b && levof[i1[e]] == levof[i2[e]]) c->h_hedges.push_back({std::min(a, b), std::max(a, b)});
+    }
+    std::sort(c->h_hedges.begin(), c->h_hedges.end());
+    c->h_hedges.erase(std::unique(c->h_hedges.begin(), c->h_hedges.end()), c->h_hedges.end());
+  }
   const int nchunk = (n + NFT_WAVE - 1) / NFT_WAVE;
   std::vector<int32_t> chunk_ptr(nchunk + 1, 0);
   for (int ch = 0; ch < nchunk; ++ch) {
@@ -1171,6 +1429,186 @@ int nft_kernel_timer_read(nft_ctx *c, int *launches, double *total_ms, double *b
   if (launches) *launches = c->t_launches;
   if (total_ms) *total_ms = c->t_ms;
   if (bytes_per_launch) *bytes_per_launch = kernel_bytes(c, c->timer_kernel);
+  return 0;
+}
+
+
+// ---- learned preconditioner ------------------------------------------------------------
+int nft_net_init(nft_ctx *c, int hidden, uint64_t seed, int max_batch) {
+  if (!c || hidden < 32 || hidden % 32 || max_batch < 1) return set_err(-1, "nft_net_init: bad argument");
+  if (c->ncol <= 0 || c->col_L == 0) return set_err(-1, "nft_net_init: needs uniform columns numbered column by column");
+  if (c->net.ready) return set_err(-1, "nft_net_init: already initialised");
+  // horizontal column graph from the edges joining different columns at one level
+  std::vector<std::vector<int>> nb(c->ncol);
+  for (const auto &e : c->h_hedges) {
+    int a = e.first, b = e.second;
+    nb[a].push_back(b);
+    nb[b].push_back(a);
+  }
+  std::vector<int32_t> ptr(c->ncol + 1, 0), idx;
+  for (int k = 0; k < c->ncol; ++k) {
+    std::sort(nb[k].begin(), nb[k].end());
+    nb[k].erase(std::unique(nb[k].begin(), nb[k].end()), nb[k].end());
+    if (nb[k].empty()) return set_err(-1, "nft_net_init: a column without horizontal neighbours");
+    ptr[k + 1] = ptr[k] + (int)nb[k].size();
+    idx.insert(idx.end(), nb[k].begin(), nb[k].end());
+  }
+  int rc;
+  if ((rc = upload(c, &c->net.nptr, ptr.data(), ptr.size())) || (rc = upload(c, &c->net.nidx, idx.data(), idx.size())))
+    return rc;
+  return net_alloc(c, hidden, max_batch * c->ncol, seed);
+}
+
+int nft_net_info(nft_ctx *c, int *kp, int *levels, int *ncol, int64_t *nparams) {
+  if (!c || !c->net.ready) return set_err(-1, "nft_net_info: network not initialised");
+  auto &n = c->net;
+  int64_t cnt = 0;
+  for (int l = 0; l < 6; ++l) cnt += (int64_t)n.K[l] * n.N[l] + n.N[l];
+  if (kp) *kp = n.Kp;
+  if (levels) *levels = n.L;
+  if (ncol) *ncol = n.ncol;
+  if (nparams) *nparams = cnt;
+  return 0;
+}
+
+// flat parameter vector: for each layer W (K x N row-major) then b (N)
+int nft_net_params(nft_ctx *c, float *host, int64_t count, int set) {
+  if (!c || !c->net.ready || !host) return set_err(-1, "nft_net_params: bad argument");
+  auto &n = c->net;
+  int64_t off = 0;
+  for (int l = 0; l < 6; ++l) {
+    size_t kw = (size_t)n.K[l] * n.N[l], kb = n.N[l];
+    if (off + (int64_t)(kw + kb) > count) return set_err(-1, "nft_net_params: buffer too small");
+    if (set) {
+      HIPCHK(hipMemcpy(n.w[2 * l], host + off, kw * sizeof(float), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(n.w[2 * l + 1], host + off + kw, kb * sizeof(float), hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(k_net_to_bf16, dim3((kw + 255) / 256), dim3(256), 0, c->stream, n.K[l], n.N[l], n.w[2 * l],
+                         n.wb[l], n.wt[l]);
+    } else {
+      HIPCHK(hipMemcpy(host + off, n.w[2 * l], kw * sizeof(float), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(host + off + kw, n.w[2 * l + 1], kb * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    off += kw + kb;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// forward (and, with T, backward) on a host batch X/T [nsamples*ncol][Kp]; outputs Y and
+// the flat gradient vector (same layout as nft_net_params) when given
+int nft_net_eval(nft_ctx *c, const float *X, const float *T, int nsamples, float *Y, float *grad, double *loss) {
+  if (!c || !c->net.ready || !X || nsamples < 1) return set_err(-1, "nft_net_eval: bad argument");
+  auto &n = c->net;
+  const int M = nsamples * n.ncol;
+  if (M > n.max_rows) return set_err(-1, "nft_net_eval: batch larger than max_batch");
+  HIPCHK(hipMemcpyAsync(n.X, X, (size_t)M * n.Kp * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  int rc;
+  if ((rc = net_forward(c, M))) return rc;
+  if (T) {
+    HIPCHK(hipMemcpyAsync(n.T, T, (size_t)M * n.Kp * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if ((rc = net_backward(c, M, loss))) return rc;
+  }
+  if (Y) HIPCHK(hipMemcpyAsync(Y, n.Y, (size_t)M * n.Kp * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  if (grad) {
+    int64_t off = 0;
+    for (int l = 0; l < 6; ++l) {
+      size_t kw = (size_t)n.K[l] * n.N[l], kb = n.N[l];
+      HIPCHK(hipMemcpyAsync(grad + off, n.g[2 * l], kw * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(grad + off + kw, n.g[2 * l + 1], kb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      off += kw + kb;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// capture (residual, correction) pairs from the converged GCR solves that follow
+int nft_dump_pairs(nft_ctx *c, int max_pairs, int max_stage) {
+  if (!c || !c->net.ready || max_pairs < 0 || max_stage < 1) return set_err(-1, "nft_dump_pairs: bad argument");
+  auto &n = c->net;
+  const size_t per = (size_t)n.ncol * n.Kp;
+  int rc;
+  if (max_pairs > n.max_pairs) {
+    if ((rc = dalloc(c, &n.pX, per * max_pairs)) || (rc = dalloc(c, &n.pT, per * max_pairs))) return rc;
+    n.max_pairs = max_pairs;
+    n.npairs = 0;
+  }
+  if (max_stage > n.stage_max) {
+    if ((rc = dalloc(c, &n.stage, (size_t)3 * c->n * max_stage))) return rc;
+    n.stage_max = max_stage;
+  }
+  n.dumping = max_pairs > 0;
+  return 0;
+}
+
+int nft_pairs(nft_ctx *c, int *count, float *X, float *T, int first, int n_get, const float *Xin,
+              const float *Tin, int n_put) {
+  if (!c || !c->net.ready) return set_err(-1, "nft_pairs: network not initialised");
+  auto &n = c->net;
+  const size_t per = (size_t)n.ncol * n.Kp;
+  if (n_get > 0) {
+    if (first < 0 || first + n_get > n.npairs || !X || !T) return set_err(-1, "nft_pairs: range");
+    HIPCHK(hipMemcpy(X, n.pX + per * first, per * n_get * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(T, n.pT + per * first, per * n_get * sizeof(float), hipMemcpyDeviceToHost));
+  }
+  if (n_put > 0) {
+    if (n.npairs + n_put > n.max_pairs || !Xin || !Tin) return set_err(-1, "nft_pairs: capacity");
+    HIPCHK(hipMemcpy(n.pX + per * n.npairs, Xin, per * n_put * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(n.pT + per * n.npairs, Tin, per * n_put * sizeof(float), hipMemcpyHostToDevice));
+    n.npairs += n_put;
+  }
+  if (count) *count = n.npairs;
+  return 0;
+}
+
+// Adam on the captured pairs: `steps` minibatches of `batch` pairs drawn with a fixed-seed
+// generator; losses of the first and last minibatch returned
+int nft_net_train(nft_ctx *c, int steps, int batch, double lr, uint64_t seed, double *loss_first, double *loss_last) {
+  if (!c || !c->net.ready || steps < 1 || batch < 1) return set_err(-1, "nft_net_train: bad argument");
+  auto &n = c->net;
+  if (n.npairs < 1) return set_err(-1, "nft_net_train: no training pairs");
+  if (batch * n.ncol > n.max_rows) return set_err(-1, "nft_net_train: batch larger than max_batch");
+  const size_t per = (size_t)n.ncol * n.Kp;
+  const int M = batch * n.ncol;
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 7;
+  auto next = [&]() { st ^= st << 13; st ^= st >> 7; st ^= st << 17; return st; };
+  int rc;
+  for (int it = 0; it < steps; ++it) {
+    for (int b = 0; b < batch; ++b) {
+      int k = (int)(next() % (uint64_t)n.npairs);
+      HIPCHK(hipMemcpyAsync(n.X + per * b, n.pX + per * k, per * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(n.T + per * b, n.pT + per * k, per * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    }
+    if ((rc = net_forward(c, M))) return rc;
+    double loss = 0;
+    bool want = (it == 0 && loss_first) || (it == steps - 1 && loss_last);
+    if ((rc = net_backward(c, M, want ? &loss : nullptr))) return rc;
+    if (it == 0 && loss_first) *loss_first = loss;
+    if (it == steps - 1 && loss_last) *loss_last = loss;
+    n.step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - std::pow(b1, (float)n.step), c2 = 1.0f - std::pow(b2, (float)n.step);
+    for (int l = 0; l < 6; ++l) {
+      size_t kw = (size_t)n.K[l] * n.N[l], kb = n.N[l];
+      hipLaunchKernelGGL(k_net_adam, dim3((kw + 255) / 256), dim3(256), 0, c->stream, (int)kw, n.K[l], n.N[l],
+                         n.w[2 * l], n.g[2 * l], n.m[2 * l], n.v[2 * l], (float)lr, b1, b2, c1, c2, 1e-8f, n.wb[l],
+                         n.wt[l]);
+      hipLaunchKernelGGL(k_net_adam, dim3((kb + 255) / 256), dim3(256), 0, c->stream, (int)kb, 1, (int)kb,
+                         n.w[2 * l + 1], n.g[2 * l + 1], n.m[2 * l + 1], n.v[2 * l + 1], (float)lr, b1, b2, c1, c2,
+                         1e-8f, (uint16_t *)nullptr, (uint16_t *)nullptr);
+    }
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int nft_set_learned(nft_ctx *c, int kind, double stall_ratio, int *fallbacks) {
+  if (!c) return set_err(-1, "nft_set_learned: null context");
+  if (kind == 1 && !c->net.ready) return set_err(-1, "nft_set_learned: network not initialised");
+  if (kind == 1 && c->P.precon != 2) return set_err(-1, "nft_set_learned: needs the line preconditioner");
+  if (kind >= 0) { c->net.kind = kind; c->net.stall = stall_ratio; }
+  if (fallbacks) *fallbacks = c->net.fallbacks;
   return 0;
 }
 

@@ -125,45 +125,16 @@ __device__ __forceinline__ double ld_agent(const double *p) {
   return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Writes this block's K partials, and returns true in every thread of the block that
-// finishes last; that block then holds all partials summed in block order in out[K]
-// (valid in all threads).
+// Writes this block's K partial results (block tree, result of thread 0) to
+// partials[blockIdx.x*K + k]. The grid-level sum is k_reduce_final, launched next on the
+// same stream: the kernel boundary makes every partial visible without a fence per block.
 template <int K>
-__device__ bool grid_reduce(double (&v)[K], unsigned ismax, double *partials, unsigned *counter,
-                            int nblk, int myblk, double *sh, double (&out)[K]) {
+__device__ __forceinline__ void block_partial(double (&v)[K], unsigned ismax, double *partials, double *sh) {
   block_reduce<K>(v, ismax, sh);
-  __shared__ int s_last;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) st_agent(&partials[(size_t)myblk * K + k], v[k]);
-    __threadfence();
-    unsigned t = atomicAdd(counter, 1u);
-    s_last = (t == (unsigned)(nblk - 1));
+    for (int k = 0; k < K; ++k) partials[(size_t)blockIdx.x * K + k] = v[k];
   }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  double acc[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) acc[k] = ((ismax >> k) & 1) ? -1.0e300 : 0.0;
-  for (int b = threadIdx.x; b < nblk; b += NFT_BLOCK) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double p = ld_agent(&partials[(size_t)b * K + k]);
-      acc[k] = ((ismax >> k) & 1) ? fmax(acc[k], p) : acc[k] + p;
-    }
-  }
-  block_reduce<K>(acc, ismax, sh);
-  __shared__ double s_out[K];
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) s_out[k] = acc[k];
-    atomicExch(counter, 0u);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < K; ++k) out[k] = s_out[k];
-  return true;
 }
 
 // ------------------------------------------------------------------------------------
@@ -426,10 +397,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
       for (int k = 0; k < ND; ++k) v[k] = (a.dots.a[k] ? a.dots.a[k][L.i] : out) * out;
       if (NMAX) v[ND] = fabs(out);
     }
-    double red[K];
-    if (grid_reduce<K>(v, NMAX ? (1u << ND) : 0u, a.partials, a.counter, gridDim.x, blockIdx.x, sh, red)) {
-      if (threadIdx.x == 0) gcr_finalize(a.dots, red, a.scal, a.epa);
-    }
+    block_partial<K>(v, NMAX ? (1u << ND) : 0u, a.partials, sh);
   }
 }
 
@@ -464,10 +432,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_gcr_update(int n, double *p, doub
       }
     }
   }
-  double red[2];
-  if (grid_reduce<2>(v, 1u, partials, counter, gridDim.x, blockIdx.x, sh, red)) {
-    if (threadIdx.x == 0) { scal[SC_DVMX] = red[0]; scal[SC_RL2] = red[1]; }
-  }
+  block_partial<2>(v, 1u, partials, sh);
 }
 
 struct DirArgs {
@@ -503,15 +468,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_gcr_dir(DirArgs a) {
     v[0] = v[0] + a.r[k] * av;
     v[1] = v[1] + av * av;
   }
-  double red[2];
-  if (grid_reduce<2>(v, 0u, a.partials, a.counter, gridDim.x, blockIdx.x, sh, red)) {
-    if (threadIdx.x == 0) {
-      double ax2 = fmax(a.epa, red[1]);
-      a.scal[SC_AX2 + a.slot] = ax2;
-      a.scal[SC_RAX] = red[0];
-      a.scal[SC_BETA] = -red[0] / ax2;
-    }
-  }
+  block_partial<2>(v, 0u, a.partials, sh);
 }
 
 // sum a^2 and max|a| (and optionally copy a into dst)
@@ -526,9 +483,54 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_norms(int n, const double *a, dou
     v[0] = v[0] + x * x;
     v[1] = fmax(v[1], fabs(x));
   }
-  double red[2];
-  if (grid_reduce<2>(v, 2u, partials, counter, gridDim.x, blockIdx.x, sh, red)) {
-    if (threadIdx.x == 0) { scal[sum_slot] = red[0]; scal[max_slot] = red[1]; }
+  block_partial<2>(v, 2u, partials, sh);
+}
+
+// Grid-level end of every fused reduction: one block sums the per-block partials in a
+// fixed order (thread t: blocks t, t+256, ... ; then the block tree) and applies the
+// solver's scalar update.
+enum FinalKind { FK_DIV = 0, FK_UPDATE = 1, FK_DIR = 2, FK_NORMS = 3 };
+
+struct FinalOp {
+  int kind;
+  int K;            // values per block
+  unsigned ismax;   // bit k: value k is a max
+  DotList dots;     // FK_DIV
+  int slot;         // FK_DIR: slot of the new direction
+  int sum_slot, max_slot;  // FK_NORMS
+  double epa;
+};
+
+__global__ void __launch_bounds__(NFT_BLOCK) k_reduce_final(const double *partials, int nblk, FinalOp op,
+                                                            double *scal) {
+  __shared__ double sh[(NFT_MAXD + 1) * NFT_WPB];
+  double acc[NFT_MAXD + 1];
+#pragma unroll
+  for (int k = 0; k <= NFT_MAXD; ++k) acc[k] = ((op.ismax >> k) & 1) ? -1.0e300 : 0.0;
+  for (int b = threadIdx.x; b < nblk; b += NFT_BLOCK) {
+#pragma unroll
+    for (int k = 0; k <= NFT_MAXD; ++k) {
+      if (k < op.K) {
+        double p = partials[(size_t)b * op.K + k];
+        acc[k] = ((op.ismax >> k) & 1) ? fmax(acc[k], p) : acc[k] + p;
+      }
+    }
+  }
+  block_reduce<NFT_MAXD + 1>(acc, op.ismax, sh);
+  if (threadIdx.x != 0) return;
+  if (op.kind == FK_DIV) {
+    gcr_finalize(op.dots, acc, scal, op.epa);
+  } else if (op.kind == FK_UPDATE) {
+    scal[SC_DVMX] = acc[0];
+    scal[SC_RL2] = acc[1];
+  } else if (op.kind == FK_DIR) {
+    double ax2 = fmax(op.epa, acc[1]);
+    scal[SC_AX2 + op.slot] = ax2;
+    scal[SC_RAX] = acc[0];
+    scal[SC_BETA] = -acc[0] / ax2;
+  } else {
+    scal[op.sum_slot] = acc[0];
+    scal[op.max_slot] = acc[1];
   }
 }
 

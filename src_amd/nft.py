@@ -81,6 +81,33 @@ _SIGS = {
                                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
 }
 
+_SIGS.update({
+    "nft_net_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_int]),
+    "nft_net_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64)]),
+    "nft_net_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int64, ctypes.c_int]),
+    "nft_net_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                    ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                    ctypes.POINTER(ctypes.c_double)]),
+    "nft_dump_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "nft_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float),
+                                 ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int,
+                                 ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_int]),
+    "nft_net_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64,
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "nft_set_learned": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]),
+})
+
+f32p = ctypes.POINTER(ctypes.c_float)
+
+
+def _pf(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(f32p)
+
+
 KERNELS = {1: "laplc_grad", 2: "laplc_div", 3: "line_solve", 4: "gcr_update", 5: "gcr_dir",
            6: "mp_donor", 7: "mp_grad", 8: "mp_fct", 9: "mp_limit"}
 
@@ -264,6 +291,68 @@ class Solver:
         _chk(lib().nft_kernel_timer_read(self.h, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)),
              "nft_kernel_timer_read")
         return n.value, ms.value, b.value
+
+    # ---- learned preconditioner -------------------------------------------------------
+    def net_init(self, hidden=128, seed=0, max_batch=16):
+        _chk(lib().nft_net_init(self.h, hidden, ctypes.c_uint64(seed), max_batch), "nft_net_init")
+
+    def net_info(self):
+        kp, L, nc, npar = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+        _chk(lib().nft_net_info(self.h, ctypes.byref(kp), ctypes.byref(L), ctypes.byref(nc), ctypes.byref(npar)),
+             "nft_net_info")
+        return dict(kp=kp.value, levels=L.value, ncol=nc.value, nparams=npar.value)
+
+    def net_params(self, values=None):
+        n = self.net_info()["nparams"]
+        if values is None:
+            out = np.zeros(n, np.float32)
+            _chk(lib().nft_net_params(self.h, _pf(out), n, 0), "nft_net_params")
+            return out
+        v = np.ascontiguousarray(values, np.float32)
+        _chk(lib().nft_net_params(self.h, _pf(v), v.size, 1), "nft_net_params")
+
+    def net_eval(self, X, T=None):
+        info = self.net_info()
+        X = np.ascontiguousarray(X, np.float32)
+        ns = X.shape[0] // info["ncol"]
+        Y = np.zeros_like(X)
+        grad = np.zeros(info["nparams"], np.float32) if T is not None else None
+        loss = ctypes.c_double()
+        Tc = np.ascontiguousarray(T, np.float32) if T is not None else None
+        _chk(lib().nft_net_eval(self.h, _pf(X), _pf(Tc), ns, _pf(Y), _pf(grad), ctypes.byref(loss)), "nft_net_eval")
+        return Y, grad, loss.value
+
+    def dump_pairs(self, max_pairs, max_stage=400):
+        _chk(lib().nft_dump_pairs(self.h, max_pairs, max_stage), "nft_dump_pairs")
+
+    def pair_count(self):
+        n = ctypes.c_int()
+        _chk(lib().nft_pairs(self.h, ctypes.byref(n), None, None, 0, 0, None, None, 0), "nft_pairs")
+        return n.value
+
+    def pairs_get(self, first, count):
+        info = self.net_info()
+        per = info["ncol"] * info["kp"]
+        X = np.zeros(per * count, np.float32); T = np.zeros(per * count, np.float32)
+        _chk(lib().nft_pairs(self.h, None, _pf(X), _pf(T), first, count, None, None, 0), "nft_pairs")
+        return X.reshape(count, -1), T.reshape(count, -1)
+
+    def pairs_put(self, X, T):
+        X = np.ascontiguousarray(X, np.float32); T = np.ascontiguousarray(T, np.float32)
+        n = ctypes.c_int()
+        _chk(lib().nft_pairs(self.h, ctypes.byref(n), None, None, 0, 0, _pf(X), _pf(T), X.shape[0]), "nft_pairs")
+        return n.value
+
+    def net_train(self, steps, batch=8, lr=1e-3, seed=0):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _chk(lib().nft_net_train(self.h, steps, batch, lr, ctypes.c_uint64(seed), ctypes.byref(a), ctypes.byref(b)),
+             "nft_net_train")
+        return a.value, b.value
+
+    def set_learned(self, kind, stall=0.99):
+        fb = ctypes.c_int()
+        _chk(lib().nft_set_learned(self.h, kind, stall, ctypes.byref(fb)), "nft_set_learned")
+        return fb.value
 
     def gcr_stats(self):
         st = GcrStats()
