@@ -74,6 +74,35 @@ def timed_run(mesh, cfg, precision, steps, warmup, profile=True):
     return out
 
 
+def learned_run(mesh, cfg, steps, train_steps, log_prefix="[learned]"):
+    """Plain FP64 solver with the learned correction: pairs (r, M_line^-1 r) are stored
+    during one plain step, the network is trained on them (residual objective), then
+    `steps` steps run with the correction (and its stall fallback) in every GCR solve."""
+    from src_amd import nft
+    s = nft.Solver(mesh, nft.make_params(cfg))
+    s.net_init(hidden=128, seed=1, max_batch=2)
+    s.state_init()
+    s.dump_pairs(max_pairs=64, stride=4)
+    s.run_steps(1)
+    s.dump_pairs(0)
+    npairs = s.pair_count()
+    t0 = time.perf_counter()
+    l0, l1 = s.net_train(steps=train_steps, batch=2, lr=1e-3, seed=3)
+    t_train = time.perf_counter() - t0
+    log(f"{log_prefix} {npairs} pairs, {train_steps} Adam steps in {t_train:.1f} s, loss {l0:.4f} -> {l1:.4f}")
+    s.set_learned(1, stall=0.99)
+    st = s.run_steps(steps)
+    fb, li = s.set_learned(-1)
+    s.close()
+    return dict(value=round(1000.0 / (st.ms_total / steps), 4), unit="timesteps/s",
+                iters_per_solve=round(st.gcr_iters / max(st.gcr_solves, 1), 2),
+                solve_ms=round(st.ms_gcr / max(st.gcr_solves, 1), 3), steps=steps,
+                unconverged=st.gcr_unconverged, fallbacks=fb, learned_iterations=li,
+                training={"pairs": npairs, "adam_steps": train_steps, "batch": 2, "seconds": round(t_train, 2),
+                          "loss_first": round(l0, 5), "loss_last": round(l1, 5),
+                          "objective": "|L z - r|^2/|r|^2, z = M_line^-1 r + s N(r/s)"})
+
+
 def cpu_baseline(mesh, cfg, iters_per_solve):
     """Serial FP64 oracle on the production mesh: one MPDATA call per field, one prfr,
     div, line preconditioner and laplc, and one GCR cycle are timed and scaled to a step
@@ -128,6 +157,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--learned-train-steps", type=int, default=100)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="also run N plain device copies of one node array (byte-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the "
@@ -143,6 +175,12 @@ def main():
         f"{mesh.nboun} boundary points ({time.perf_counter() - t0:.1f} s)")
     log("device:", nft.device_name())
 
+    if args.calibrate_copy:
+        s0 = nft.Solver(mesh, nft.make_params(cfg))
+        ms = s0.copy_calibration(args.calibrate_copy)
+        log(f"copy calibration: {mesh.npoin * 8} bytes read + written per copy, {ms * 1e3:.1f} us, "
+            f"{2 * mesh.npoin * 8 / (ms * 1e-3) / 1e9:.0f} GB/s")
+        s0.close()
     r = timed_run(mesh, cfg, "fp64", args.steps, args.warmup)
     st = r["st"]
     K = args.steps
@@ -178,7 +216,8 @@ def main():
                          "N=0.01 1/s (Fr=1), dt=20 s; MPDATA + 2 x GCR(4) to 1e-8 relative residual with "
                          "the vertical line preconditioner, FP64"),
             "nodes": mesh.npoin, "edges": mesh.nedge + mesh.nbface, "gcr_order": cfg.lord,
-            "tolerance": cfg.eps0, "preconditioner": "vertical line (Thomas)", "precision": "fp64",
+            "tolerance": cfg.eps0, "preconditioner": "vertical line (tridiagonal, wavefront PCR)",
+            "precision": "fp64",
         },
         "gcr": {"iters_per_solve": round(iters, 2), "solve_ms": round(st.ms_gcr / max(st.gcr_solves, 1), 3),
                 "solves": st.gcr_solves, "unconverged": st.gcr_unconverged},
@@ -198,6 +237,7 @@ def main():
             "iters_per_solve": round(sm.gcr_iters / max(sm.gcr_solves, 1), 2),
             "solve_ms": round(sm.ms_gcr / max(sm.gcr_solves, 1), 3),
             "mpdata_ms_per_step": round(sm.ms_mpdata / mk, 3), "steps": mk}}
+        line["variants"]["learned"] = learned_run(mesh, cfg, mk, args.learned_train_steps)
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         line["cpu_baseline"] = cpu_baseline(mesh, cfg, iters)

@@ -102,6 +102,9 @@ int nft_device_name(char *buf, int len);
 int nft_derivative(nft_ctx *ctx, const double *unk, double *dpdxa, int idx);
 /* laplc(press,alap,impfl)                       reference :1288-1353 */
 int nft_laplc(nft_ctx *ctx, const double *press, double *alap, int impfl);
+/* out = L^T y, the transpose of laplc (no reference counterpart: trains the learned
+ * preconditioner) */
+int nft_laplc_t(nft_ctx *ctx, const double *y, double *out, int impfl);
 /* prfr(press,ustr,vstr,wstr,uout,vout,wout,impfl) reference :1536-1580 */
 int nft_prfr(nft_ctx *ctx, const double *press, const double *ustr, const double *vstr,
              const double *wstr, double *uout, double *vout, double *wout, int impfl);
@@ -139,6 +142,9 @@ int nft_get_gcr_stats(nft_ctx *ctx, nft_gcr_stats *stats);
  * node arrays). */
 int nft_kernel_timer(nft_ctx *ctx, int kernel);
 int nft_kernel_timer_read(nft_ctx *ctx, int *launches, double *total_ms, double *bytes_per_launch);
+/* plain device copy of one node array (8*npoin bytes read + written) `reps` times, mean ms
+ * per copy: the reference point for hardware byte counters */
+int nft_copy_calibration(nft_ctx *ctx, int reps, double *ms);
 
 /* ---- learned preconditioner (no reference counterpart) ------------------------------
  * A column MLP with two horizontal graph convolutions (bf16 MFMA, FP32 accumulation)
@@ -153,17 +159,25 @@ int nft_net_params(nft_ctx *ctx, float *host, int64_t count, int set);
 /* forward (and with T, loss + backward) on a host batch [nsamples*ncol][Kp] */
 int nft_net_eval(nft_ctx *ctx, const float *X, const float *T, int nsamples, float *Y, float *grad,
                  double *loss);
-/* capture training pairs X = r/s, T = ((p_k - p_final) - M^-1 r)/s, s = max|r|, from the
- * converged GCR solves that follow (max_stage iterations staged per solve) */
-int nft_dump_pairs(nft_ctx *ctx, int max_pairs, int max_stage);
-/* pair store: count, copy out [first, first+n_get), append n_put host pairs */
-int nft_pairs(nft_ctx *ctx, int *count, float *X, float *T, int first, int n_get, const float *Xin,
-              const float *Tin, int n_put);
-int nft_net_train(nft_ctx *ctx, int steps, int batch, double lr, uint64_t seed, double *loss_first,
-                  double *loss_last);
+/* store every stride-th pair (r, q = M_line^-1 r) of the GCR iterations that follow, with
+ * s = max|r| and |r|^2, until max_pairs are held (device memory: 16*npoin bytes per pair) */
+int nft_dump_pairs(nft_ctx *ctx, int max_pairs, int stride);
+/* pair store: count, and copies of pairs [first, first+n_get): r and q as n_get x npoin */
+int nft_pairs(nft_ctx *ctx, int *count, int first, int n_get, double *r, double *q, double *s,
+              double *r2);
+/* Training objective (physics-informed): with z = q + s*N(r/s), the mean over pairs of
+ * |L z - r|^2 / |r|^2, L = laplc; its gradient is s * L^T(2(Lz - r)/(B|r|^2)) per sample.
+ * nft_net_pair_loss evaluates it on pairs [first, first+count); nft_net_train runs Adam. */
+int nft_net_pair_loss(nft_ctx *ctx, int first, int count, double *loss, float *grad);
+/* Adam over pairs [0, use_pairs) (all when use_pairs <= 0) */
+int nft_net_train(nft_ctx *ctx, int steps, int batch, double lr, uint64_t seed, int use_pairs,
+                  double *loss_first, double *loss_last);
+/* training objective: 0 residual |Lz - r|^2/|r|^2, 1 angle 1 - <r,Lz>^2/(|r|^2 |Lz|^2);
+ * linear = 1: identity activations and biases held at zero (the correction is linear in r) */
+int nft_net_options(nft_ctx *ctx, int objective, int linear);
 /* kind 0: line preconditioner only; 1: learned correction + line fallback. kind < 0 only
- * reads the number of fallbacks so far. */
-int nft_set_learned(nft_ctx *ctx, int kind, double stall_ratio, int *fallbacks);
+ * reads the counters: fallbacks so far, GCR iterations that used the learned correction. */
+int nft_set_learned(nft_ctx *ctx, int kind, double stall_ratio, int *fallbacks, int *learned_iters);
 
 #ifdef __cplusplus
 }

@@ -117,12 +117,17 @@ struct nft_ctx {
     float *dC2 = nullptr, *dZ2 = nullptr, *dZ1 = nullptr, *P = nullptr, *lpart = nullptr;
     size_t P_count = 0;
     int step = 0;
-    float *pX = nullptr, *pT = nullptr;
-    int npairs = 0, max_pairs = 0;
-    double *stage = nullptr;
-    int stage_max = 0, stage_n = 0;
-    std::vector<double> stage_s;
+    // training pairs: residual r and its line-preconditioned q, with s = max|r| and |r|^2
+    double *pR = nullptr, *pQ = nullptr;
+    std::vector<double> ps, pr2;
+    std::vector<int> pimp;   // impfl of the solve each pair came from
+    int npairs = 0, max_pairs = 0, stride = 1, seen = 0;
+    // training work arrays (node length) and loss partials
+    double *tz = nullptr, *tLz = nullptr, *ty = nullptr, *tg = nullptr, *tpart = nullptr;
     int dumping = 0;
+    int objective = 0;     // training objective: 0 residual, 1 angle
+    int linear = 0;        // identity activations and zero biases: a fixed linear map
+    int learned_iters = 0; // GCR iterations that used the learned correction
     int kind = 0;          // 0 off, 1 learned correction on top of the line preconditioner
     double stall = 0.99;   // fall back to the line preconditioner when rl2/rl2_prev exceeds this
     int active = 0;        // per solve: learned correction in use
@@ -427,6 +432,18 @@ int laplc_dev(nft_ctx *c, const double *press, double *alap, int impfl, const Do
   return div_launch(c, DIV_LAPLC, d);
 }
 
+// out = L^T y (transpose of laplc_dev, two gathers; uses t[0..2])
+int laplt_dev(nft_ctx *c, const double *y, double *out, int impfl) {
+  LaplTArgs a{};
+  a.y = y; a.g0 = c->t[0]; a.g1 = c->t[1]; a.g2 = c->t[2]; a.out = out;
+  a.nc = c->nc; a.dt = c->P.dt; a.impfl = impfl;
+  dim3 g(c->gather_grid), b(NFT_BLOCK);
+  hipLaunchKernelGGL(k_laplT_1, g, b, 0, c->stream, c->inc, c->S, a);
+  hipLaunchKernelGGL(k_laplT_2, g, b, 0, c->stream, c->inc, c->S, a);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int prfr_dev(nft_ctx *c, const double *p, const double *us, const double *vs, const double *ws,
              double *uo, double *vo, double *wo, int impfl) {
   GradArgs g = grad_args(c);
@@ -527,49 +544,58 @@ int net_forward(nft_ctx *c, int M) {
   const int H = n.H, H2 = 2 * H;
   int rc;
   int tb = 256, gb = (M * H + tb - 1) / tb;
-  if ((rc = net_gemm(c, M, H, n.Kp, n.X, n.Kp, n.wt[0], n.w[1], 1, nullptr, 0, n.A1, H))) return rc;
-  if ((rc = net_gemm(c, M, H, H, n.A1, H, n.wt[1], n.w[3], 1, nullptr, 0, n.C2, H2))) return rc;
+  const int act = n.linear ? 0 : 1;
+  if ((rc = net_gemm(c, M, H, n.Kp, n.X, n.Kp, n.wt[0], n.w[1], act, nullptr, 0, n.A1, H))) return rc;
+  if ((rc = net_gemm(c, M, H, H, n.A1, H, n.wt[1], n.w[3], act, nullptr, 0, n.C2, H2))) return rc;
   hipLaunchKernelGGL(k_net_agg, dim3(gb), dim3(tb), 0, c->stream, M, n.ncol, H, n.nptr, n.nidx, n.C2, H2, n.C2 + H, H2);
-  if ((rc = net_gemm(c, M, H, H2, n.C2, H2, n.wt[2], n.w[5], 1, nullptr, 0, n.C3, H2))) return rc;
+  if ((rc = net_gemm(c, M, H, H2, n.C2, H2, n.wt[2], n.w[5], act, nullptr, 0, n.C3, H2))) return rc;
   hipLaunchKernelGGL(k_net_agg, dim3(gb), dim3(tb), 0, c->stream, M, n.ncol, H, n.nptr, n.nidx, n.C3, H2, n.C3 + H, H2);
-  if ((rc = net_gemm(c, M, H, H2, n.C3, H2, n.wt[3], n.w[7], 1, nullptr, 0, n.A4, H))) return rc;
-  if ((rc = net_gemm(c, M, H, H, n.A4, H, n.wt[4], n.w[9], 1, nullptr, 0, n.A5, H))) return rc;
+  if ((rc = net_gemm(c, M, H, H2, n.C3, H2, n.wt[3], n.w[7], act, nullptr, 0, n.A4, H))) return rc;
+  if ((rc = net_gemm(c, M, H, H, n.A4, H, n.wt[4], n.w[9], act, nullptr, 0, n.A5, H))) return rc;
   if ((rc = net_gemm(c, M, n.Kp, H, n.A5, H, n.wt[5], n.w[11], 0, nullptr, 0, n.Y, n.Kp))) return rc;
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-// loss (MSE over the first L of Kp outputs) and gradients of every parameter into n.g
-int net_backward(nft_ctx *c, int M, double *loss) {
+// gradients of every parameter into n.g from the output gradient n.dY
+int net_backprop(nft_ctx *c, int M) {
   auto &n = c->net;
   const int H = n.H, H2 = 2 * H, Kp = n.Kp;
   int rc;
-  const int nl = (M * Kp + 255) / 256;
-  const float inv = 1.0f / (float)((size_t)M * n.L);
-  hipLaunchKernelGGL(k_net_loss, dim3(nl), dim3(256), 0, c->stream, M, n.L, Kp, n.Y, n.T, inv, n.dY, n.lpart);
   int tb = 256, gb = (M * H + tb - 1) / tb;
   // layer 6: Y = A5 W6 + b6
   if ((rc = net_wgrad(c, 5, M, n.A5, H, n.dY, Kp))) return rc;
-  if ((rc = net_gemm(c, M, H, Kp, n.dY, Kp, n.wb[5], nullptr, 0, n.A5, H, n.dZ5, H))) return rc;
+  if ((rc = net_gemm(c, M, H, Kp, n.dY, Kp, n.wb[5], nullptr, 0, n.linear ? nullptr : n.A5, H, n.dZ5, H))) return rc;
   // layer 5: A5 = relu(A4 W5 + b5)
   if ((rc = net_wgrad(c, 4, M, n.A4, H, n.dZ5, H))) return rc;
-  if ((rc = net_gemm(c, M, H, H, n.dZ5, H, n.wb[4], nullptr, 0, n.A4, H, n.dZ4, H))) return rc;
+  if ((rc = net_gemm(c, M, H, H, n.dZ5, H, n.wb[4], nullptr, 0, n.linear ? nullptr : n.A4, H, n.dZ4, H))) return rc;
   // layer 4: A4 = relu([A3|Agg A3] W4 + b4)
   if ((rc = net_wgrad(c, 3, M, n.C3, H2, n.dZ4, H))) return rc;
   if ((rc = net_gemm(c, M, H2, H, n.dZ4, H, n.wb[3], nullptr, 0, nullptr, 0, n.dC3, H2))) return rc;
-  hipLaunchKernelGGL(k_net_agg_back, dim3(gb), dim3(tb), 0, c->stream, M, n.ncol, H, n.nptr, n.nidx, n.dC3, H2, n.C3,
+  hipLaunchKernelGGL(k_net_agg_back, dim3(gb), dim3(tb), 0, c->stream, M, n.ncol, H, n.nptr, n.nidx, n.dC3, H2, n.linear ? nullptr : n.C3,
                      H2, n.dZ3, H);
   // layer 3: A3 = relu([A2|Agg A2] W3 + b3)
   if ((rc = net_wgrad(c, 2, M, n.C2, H2, n.dZ3, H))) return rc;
   if ((rc = net_gemm(c, M, H2, H, n.dZ3, H, n.wb[2], nullptr, 0, nullptr, 0, n.dC2, H2))) return rc;
-  hipLaunchKernelGGL(k_net_agg_back, dim3(gb), dim3(tb), 0, c->stream, M, n.ncol, H, n.nptr, n.nidx, n.dC2, H2, n.C2,
+  hipLaunchKernelGGL(k_net_agg_back, dim3(gb), dim3(tb), 0, c->stream, M, n.ncol, H, n.nptr, n.nidx, n.dC2, H2, n.linear ? nullptr : n.C2,
                      H2, n.dZ2, H);
   // layer 2: A2 = relu(A1 W2 + b2)
   if ((rc = net_wgrad(c, 1, M, n.A1, H, n.dZ2, H))) return rc;
-  if ((rc = net_gemm(c, M, H, H, n.dZ2, H, n.wb[1], nullptr, 0, n.A1, H, n.dZ1, H))) return rc;
+  if ((rc = net_gemm(c, M, H, H, n.dZ2, H, n.wb[1], nullptr, 0, n.linear ? nullptr : n.A1, H, n.dZ1, H))) return rc;
   // layer 1: A1 = relu(X W1 + b1)
   if ((rc = net_wgrad(c, 0, M, n.X, Kp, n.dZ1, H))) return rc;
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// loss (MSE over the first L of Kp outputs against n.T) and gradients of every parameter
+int net_backward(nft_ctx *c, int M, double *loss) {
+  auto &n = c->net;
+  const int nl = (M * n.Kp + 255) / 256;
+  const float inv = 1.0f / (float)((size_t)M * n.L);
+  hipLaunchKernelGGL(k_net_loss, dim3(nl), dim3(256), 0, c->stream, M, n.L, n.Kp, n.Y, n.T, inv, n.dY, n.lpart);
+  int rc = net_backprop(c, M);
+  if (rc) return rc;
   if (loss) {
     std::vector<float> part(nl);
     HIPCHK(hipMemcpyAsync(part.data(), n.lpart, nl * sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -646,19 +672,17 @@ int net_apply_precon(nft_ctx *c, const double *r, double *qr, int sidx) {
   return 0;
 }
 
-// training pairs of a converged solve from the staged (r, qr, p) iterates
-int net_harvest(nft_ctx *c, const double *pfinal) {
+// keep every stride-th (r, line-preconditioned q) of the solves while the store has room
+int net_store_pair(nft_ctx *c, double s, double r2, int impfl) {
   auto &n = c->net;
-  const size_t N = c->n, per = (size_t)n.ncol * n.Kp;
-  for (int k = 0; k < n.stage_n && n.npairs < n.max_pairs; ++k) {
-    const double *base = n.stage + (size_t)k * 3 * N;
-    hipLaunchKernelGGL(k_net_make_pair, dim3((per + 255) / 256), dim3(256), 0, c->stream, n.ncol, n.L, n.Kp, base,
-                       base + N, base + 2 * N, pfinal, n.stage_s[k], n.pX + per * n.npairs, n.pT + per * n.npairs);
-    n.npairs++;
-  }
-  n.stage_n = 0;
-  n.stage_s.clear();
-  HIPCHK(hipGetLastError());
+  if (n.seen++ % n.stride != 0 || n.npairs >= n.max_pairs || !(s > 0) || !(r2 > 0)) return 0;
+  const size_t N = c->n;
+  HIPCHK(hipMemcpyAsync(n.pR + N * n.npairs, c->r, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(n.pQ + N * n.npairs, c->qr, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  n.ps.push_back(s);
+  n.pr2.push_back(r2);
+  n.pimp.push_back(impfl);
+  n.npairs++;
   return 0;
 }
 
@@ -701,8 +725,6 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
   }
   double *x0 = c->gx[0], *ax0 = c->gax[0];
   c->net.active = c->net.kind == 1 && c->net.ready;
-  c->net.stage_n = 0;
-  c->net.stage_s.clear();
   if ((rc = precon_dev(c, c->r, x0, impfl))) return rc;
   if (c->net.active && (rc = net_apply_precon(c, c->r, x0, SC_RMX))) return rc;
   if ((rc = norms_dev(c, x0, SC_EER, SC_EEM))) return rc;
@@ -747,16 +769,12 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
       }
       rl20 = std::max(rl2, epa);
       if ((rc = precon_dev(c, c->r, c->qr, impfl))) return rc;
-      if (c->net.active && (rc = net_apply_precon(c, c->r, c->qr, SC_DVMX))) return rc;
-      qr_last = c->qr;
-      if (c->net.dumping && c->net.stage_n < c->net.stage_max) {
-        double *dst = c->net.stage + (size_t)c->net.stage_n * 3 * c->n;
-        HIPCHK(hipMemcpyAsync(dst, c->r, c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(dst + c->n, c->qr, c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(dst + 2 * (size_t)c->n, p, c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        c->net.stage_s.push_back(dvmx);
-        c->net.stage_n++;
+      if (c->net.dumping && (rc = net_store_pair(c, dvmx, rl2, impfl))) return rc;
+      if (c->net.active) {
+        if ((rc = net_apply_precon(c, c->r, c->qr, SC_DVMX))) return rc;
+        c->net.learned_iters++;
       }
+      qr_last = c->qr;
       {
         DotList dl{};
         dl.nd = i;
@@ -792,11 +810,6 @@ done:
   c->st.nitsm += nlc;
   c->st.icount += 1;
   c->st.converged = converged;
-  if (c->net.dumping) {
-    if (converged && (rc = net_harvest(c, p))) return rc;
-    c->net.stage_n = 0;
-    c->net.stage_s.clear();
-  }
   c->st.res_max = dvmx;
   c->st.tol = eps;
   return converged ? 0 : 1;
@@ -1215,6 +1228,15 @@ int nft_laplc(nft_ctx *c, const double *press, double *alap, int impfl) {
   return sync(c);
 }
 
+int nft_laplc_t(nft_ctx *c, const double *y, double *out, int impfl) {
+  if (!c || !y || !out) return set_err(-1, "nft_laplc_t: null argument");
+  int rc;
+  if ((rc = copy_in(c, c->io[0], y))) return rc;
+  if ((rc = laplt_dev(c, c->io[0], c->io[1], impfl))) return rc;
+  if ((rc = copy_out(c, out, c->io[1]))) return rc;
+  return sync(c);
+}
+
 int nft_prfr(nft_ctx *c, const double *press, const double *ustr, const double *vstr, const double *wstr,
              double *uout, double *vout, double *wout, int impfl) {
   if (!c || !press || !ustr || !vstr || !wstr || !uout || !vout || !wout) return set_err(-1, "nft_prfr: null argument");
@@ -1522,67 +1544,171 @@ int nft_net_eval(nft_ctx *c, const float *X, const float *T, int nsamples, float
   return 0;
 }
 
-// capture (residual, correction) pairs from the converged GCR solves that follow
-int nft_dump_pairs(nft_ctx *c, int max_pairs, int max_stage) {
-  if (!c || !c->net.ready || max_pairs < 0 || max_stage < 1) return set_err(-1, "nft_dump_pairs: bad argument");
+// store every stride-th (r, M_line^-1 r) pair of the GCR solves that follow, up to max_pairs
+int nft_dump_pairs(nft_ctx *c, int max_pairs, int stride) {
+  if (!c || !c->net.ready || max_pairs < 0 || stride < 1) return set_err(-1, "nft_dump_pairs: bad argument");
+  if (max_pairs > 0 && c->P.precon != 2) return set_err(-1, "nft_dump_pairs: needs the line preconditioner");
   auto &n = c->net;
-  const size_t per = (size_t)n.ncol * n.Kp;
+  const size_t N = c->n;
   int rc;
   if (max_pairs > n.max_pairs) {
-    if ((rc = dalloc(c, &n.pX, per * max_pairs)) || (rc = dalloc(c, &n.pT, per * max_pairs))) return rc;
+    if ((rc = dalloc(c, &n.pR, N * max_pairs)) || (rc = dalloc(c, &n.pQ, N * max_pairs))) return rc;
     n.max_pairs = max_pairs;
     n.npairs = 0;
+    n.ps.clear();
+    n.pr2.clear();
+    n.pimp.clear();
   }
-  if (max_stage > n.stage_max) {
-    if ((rc = dalloc(c, &n.stage, (size_t)3 * c->n * max_stage))) return rc;
-    n.stage_max = max_stage;
-  }
+  n.stride = stride;
+  n.seen = 0;
   n.dumping = max_pairs > 0;
   return 0;
 }
 
-int nft_pairs(nft_ctx *c, int *count, float *X, float *T, int first, int n_get, const float *Xin,
-              const float *Tin, int n_put) {
+// pair store: count, and copies of pairs [first, first+n_get) (r, q: n_get x npoin; s, r2)
+int nft_pairs(nft_ctx *c, int *count, int first, int n_get, double *r, double *q, double *s, double *r2) {
   if (!c || !c->net.ready) return set_err(-1, "nft_pairs: network not initialised");
   auto &n = c->net;
-  const size_t per = (size_t)n.ncol * n.Kp;
+  const size_t N = c->n;
   if (n_get > 0) {
-    if (first < 0 || first + n_get > n.npairs || !X || !T) return set_err(-1, "nft_pairs: range");
-    HIPCHK(hipMemcpy(X, n.pX + per * first, per * n_get * sizeof(float), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(T, n.pT + per * first, per * n_get * sizeof(float), hipMemcpyDeviceToHost));
-  }
-  if (n_put > 0) {
-    if (n.npairs + n_put > n.max_pairs || !Xin || !Tin) return set_err(-1, "nft_pairs: capacity");
-    HIPCHK(hipMemcpy(n.pX + per * n.npairs, Xin, per * n_put * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(n.pT + per * n.npairs, Tin, per * n_put * sizeof(float), hipMemcpyHostToDevice));
-    n.npairs += n_put;
+    if (first < 0 || first + n_get > n.npairs || !r || !q || !s || !r2) return set_err(-1, "nft_pairs: range");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(r, n.pR + N * first, N * n_get * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(q, n.pQ + N * first, N * n_get * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < n_get; ++k) { s[k] = n.ps[first + k]; r2[k] = n.pr2[first + k]; }
   }
   if (count) *count = n.npairs;
   return 0;
 }
 
-// Adam on the captured pairs: `steps` minibatches of `batch` pairs drawn with a fixed-seed
-// generator; losses of the first and last minibatch returned
-int nft_net_train(nft_ctx *c, int steps, int batch, double lr, uint64_t seed, double *loss_first, double *loss_last) {
+// Physics-informed loss of one minibatch. For sample b with pair (r, q, s, a = |r|^2),
+// z_b = q + s N(r/s), v = L z_b, b = <v,v>, c = <r,v>:
+//   objective 0 (residual): l_b = |v - r|^2 / a = (b - 2c + a)/a,  dl/dv = 2(v - r)/a;
+//   objective 1 (angle):    l_b = 1 - c^2/(a b),                 dl/dv = (2c/(ab)) ((c/b) v - r);
+// loss = mean over the B samples, dloss/dY_b = s L^T(dl_b/dv) / B.
+// Forward of the network on the B samples must already be in n.Y.
+static int net_physics_grad(nft_ctx *c, const int *idx, int B, double *loss) {
+  auto &n = c->net;
+  const int N = c->n;
+  const size_t per = (size_t)n.ncol * n.Kp;
+  const int nb = (N + 255) / 256;
+  int rc;
+  double total = 0;
+  std::vector<double> part(2 * (size_t)nb);
+  for (int bi = 0; bi < B; ++bi) {
+    const int k = idx[bi];
+    const double s = n.ps[k], a = n.pr2[k];
+    const double *r = n.pR + (size_t)N * k, *q = n.pQ + (size_t)N * k;
+    hipLaunchKernelGGL(k_net_z, dim3(nb), dim3(256), 0, c->stream, n.ncol, n.L, n.Kp, q, n.Y + per * bi, s, n.tz);
+    if ((rc = laplc_dev(c, n.tz, n.tLz, n.pimp[k], nullptr))) return rc;
+    hipLaunchKernelGGL(k_net_dots, dim3(nb), dim3(256), 0, c->stream, N, n.tLz, r, n.tpart);
+    HIPCHK(hipMemcpyAsync(part.data(), n.tpart, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double bb = 0, cc = 0;
+    for (int i = 0; i < nb; ++i) { bb += part[2 * i]; cc += part[2 * i + 1]; }
+    double alpha, beta, l;
+    if (n.objective == 1 && bb > 0) {
+      l = 1.0 - cc * cc / (a * bb);
+      alpha = 2.0 * cc * cc / (a * bb * bb);
+      beta = -2.0 * cc / (a * bb);
+    } else {
+      l = (bb - 2.0 * cc + a) / a;
+      alpha = 2.0 / a;
+      beta = -2.0 / a;
+    }
+    total += l;
+    hipLaunchKernelGGL(k_net_axpby, dim3(nb), dim3(256), 0, c->stream, N, alpha / B, n.tLz, beta / B, r, n.ty);
+    if ((rc = laplt_dev(c, n.ty, n.tg, n.pimp[k]))) return rc;
+    hipLaunchKernelGGL(k_net_dY, dim3((per + 255) / 256), dim3(256), 0, c->stream, n.ncol, n.L, n.Kp, n.tg, s,
+                       n.dY + per * bi);
+    HIPCHK(hipGetLastError());
+  }
+  if (loss) *loss = total / B;
+  return 0;
+}
+
+static int net_train_alloc(nft_ctx *c) {
+  auto &n = c->net;
+  if (n.tz) return 0;
+  const size_t N = c->n;
+  int rc;
+  if ((rc = dalloc(c, &n.tz, N)) || (rc = dalloc(c, &n.tLz, N)) || (rc = dalloc(c, &n.ty, N)) ||
+      (rc = dalloc(c, &n.tg, N)) || (rc = dalloc(c, &n.tpart, 2 * ((N + 255) / 256))))
+    return rc;
+  return 0;
+}
+
+// Forward on the stored pairs idx[0..B) (network inputs r/s)
+static int net_forward_pairs(nft_ctx *c, const int *idx, int B) {
+  auto &n = c->net;
+  const size_t per = (size_t)n.ncol * n.Kp;
+  for (int b = 0; b < B; ++b) {
+    const int k = idx[b];
+    hipLaunchKernelGGL(k_net_input_s, dim3((per + 255) / 256), dim3(256), 0, c->stream, n.ncol, n.L, n.Kp,
+                       n.pR + (size_t)c->n * k, n.ps[k], n.X + per * b);
+  }
+  HIPCHK(hipGetLastError());
+  return net_forward(c, B * n.ncol);
+}
+
+// physics-informed loss over the stored pairs [first, first+count) with the current
+// network (the zero-network value is |L q - r|^2/|r|^2); with grad (count <= max_batch),
+// also the flat parameter gradient of that loss (layout of nft_net_params)
+int nft_net_pair_loss(nft_ctx *c, int first, int count, double *loss, float *grad) {
+  if (!c || !c->net.ready || !loss || count < 1) return set_err(-1, "nft_net_pair_loss: bad argument");
+  auto &n = c->net;
+  if (first < 0 || first + count > n.npairs) return set_err(-1, "nft_net_pair_loss: range");
+  const int maxb = n.max_rows / n.ncol;
+  if (grad && count > maxb) return set_err(-1, "nft_net_pair_loss: gradient needs count <= max_batch");
+  int rc;
+  if ((rc = net_train_alloc(c))) return rc;
+  double acc = 0;
+  for (int b0 = 0; b0 < count; b0 += maxb) {
+    int B = std::min(maxb, count - b0);
+    std::vector<int> idx(B);
+    for (int b = 0; b < B; ++b) idx[b] = first + b0 + b;
+    double l = 0;
+    if ((rc = net_forward_pairs(c, idx.data(), B)) || (rc = net_physics_grad(c, idx.data(), B, &l))) return rc;
+    acc += l * B;
+    if (grad && (rc = net_backprop(c, B * n.ncol))) return rc;
+  }
+  *loss = acc / count;
+  if (grad) {
+    int64_t off = 0;
+    for (int l = 0; l < 6; ++l) {
+      size_t kw = (size_t)n.K[l] * n.N[l], kb = n.N[l];
+      HIPCHK(hipMemcpyAsync(grad + off, n.g[2 * l], kw * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(grad + off + kw, n.g[2 * l + 1], kb * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      off += kw + kb;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+// Adam on the stored pairs [0, use_pairs) (all when use_pairs <= 0) with the physics-informed
+// loss: `steps` minibatches of `batch` pairs drawn with a fixed-seed generator; losses of the
+// first and last minibatch returned
+int nft_net_train(nft_ctx *c, int steps, int batch, double lr, uint64_t seed, int use_pairs, double *loss_first,
+                  double *loss_last) {
   if (!c || !c->net.ready || steps < 1 || batch < 1) return set_err(-1, "nft_net_train: bad argument");
   auto &n = c->net;
-  if (n.npairs < 1) return set_err(-1, "nft_net_train: no training pairs");
+  const int np = use_pairs > 0 ? std::min(use_pairs, n.npairs) : n.npairs;
+  if (np < 1) return set_err(-1, "nft_net_train: no training pairs");
   if (batch * n.ncol > n.max_rows) return set_err(-1, "nft_net_train: batch larger than max_batch");
-  const size_t per = (size_t)n.ncol * n.Kp;
   const int M = batch * n.ncol;
   uint64_t st = seed * 0x9E3779B97F4A7C15ull + 7;
   auto next = [&]() { st ^= st << 13; st ^= st >> 7; st ^= st << 17; return st; };
   int rc;
+  if ((rc = net_train_alloc(c))) return rc;
+  std::vector<int> idx(batch);
   for (int it = 0; it < steps; ++it) {
-    for (int b = 0; b < batch; ++b) {
-      int k = (int)(next() % (uint64_t)n.npairs);
-      HIPCHK(hipMemcpyAsync(n.X + per * b, n.pX + per * k, per * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(n.T + per * b, n.pT + per * k, per * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-    }
-    if ((rc = net_forward(c, M))) return rc;
+    for (int b = 0; b < batch; ++b) idx[b] = (int)(next() % (uint64_t)np);
+    if ((rc = net_forward_pairs(c, idx.data(), batch))) return rc;
     double loss = 0;
     bool want = (it == 0 && loss_first) || (it == steps - 1 && loss_last);
-    if ((rc = net_backward(c, M, want ? &loss : nullptr))) return rc;
+    if ((rc = net_physics_grad(c, idx.data(), batch, want ? &loss : nullptr))) return rc;
+    if ((rc = net_backprop(c, M))) return rc;
     if (it == 0 && loss_first) *loss_first = loss;
     if (it == steps - 1 && loss_last) *loss_last = loss;
     n.step++;
@@ -1593,9 +1719,10 @@ int nft_net_train(nft_ctx *c, int steps, int batch, double lr, uint64_t seed, do
       hipLaunchKernelGGL(k_net_adam, dim3((kw + 255) / 256), dim3(256), 0, c->stream, (int)kw, n.K[l], n.N[l],
                          n.w[2 * l], n.g[2 * l], n.m[2 * l], n.v[2 * l], (float)lr, b1, b2, c1, c2, 1e-8f, n.wb[l],
                          n.wt[l]);
-      hipLaunchKernelGGL(k_net_adam, dim3((kb + 255) / 256), dim3(256), 0, c->stream, (int)kb, 1, (int)kb,
-                         n.w[2 * l + 1], n.g[2 * l + 1], n.m[2 * l + 1], n.v[2 * l + 1], (float)lr, b1, b2, c1, c2,
-                         1e-8f, (uint16_t *)nullptr, (uint16_t *)nullptr);
+      if (!n.linear)
+        hipLaunchKernelGGL(k_net_adam, dim3((kb + 255) / 256), dim3(256), 0, c->stream, (int)kb, 1, (int)kb,
+                           n.w[2 * l + 1], n.g[2 * l + 1], n.m[2 * l + 1], n.v[2 * l + 1], (float)lr, b1, b2, c1,
+                           c2, 1e-8f, (uint16_t *)nullptr, (uint16_t *)nullptr);
     }
     HIPCHK(hipGetLastError());
   }
@@ -1603,12 +1730,46 @@ int nft_net_train(nft_ctx *c, int steps, int batch, double lr, uint64_t seed, do
   return 0;
 }
 
-int nft_set_learned(nft_ctx *c, int kind, double stall_ratio, int *fallbacks) {
+// device copy of one node array (read 8N + write 8N bytes), for calibrating byte counters
+int nft_copy_calibration(nft_ctx *c, int reps, double *ms) {
+  if (!c || reps < 1) return set_err(-1, "nft_copy_calibration: bad argument");
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(k_copy, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, c->io[0], c->io[1]);
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms) *ms = t / reps;
+  return 0;
+}
+
+// training objective (0 residual, 1 angle) and activation (linear = 1: identity activations
+// with the biases held at zero, so the correction is a fixed linear map of r)
+int nft_net_options(nft_ctx *c, int objective, int linear) {
+  if (!c || !c->net.ready || objective < 0 || objective > 1 || linear < 0 || linear > 1)
+    return set_err(-1, "nft_net_options: bad argument");
+  auto &n = c->net;
+  n.objective = objective;
+  n.linear = linear;
+  if (linear)
+    for (int l = 0; l < 6; ++l) HIPCHK(hipMemsetAsync(n.w[2 * l + 1], 0, n.N[l] * sizeof(float), c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int nft_set_learned(nft_ctx *c, int kind, double stall_ratio, int *fallbacks, int *learned_iters) {
   if (!c) return set_err(-1, "nft_set_learned: null context");
   if (kind == 1 && !c->net.ready) return set_err(-1, "nft_set_learned: network not initialised");
   if (kind == 1 && c->P.precon != 2) return set_err(-1, "nft_set_learned: needs the line preconditioner");
   if (kind >= 0) { c->net.kind = kind; c->net.stall = stall_ratio; }
   if (fallbacks) *fallbacks = c->net.fallbacks;
+  if (learned_iters) *learned_iters = c->net.learned_iters;
   return 0;
 }
 

@@ -402,6 +402,75 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
 }
 
 // ------------------------------------------------------------------------------------
+// Transpose of laplc (used to train the learned preconditioner). With yh = y/(vol*rho):
+//   stage 1 (divergence^T and the per-node maps): for each node j and component c
+//     w_j = sum_{e at j} s_je S_e 0.5 (yh_j - yh_i) + sbb_j yh_j,
+//     gt_j = D_j P_j^T (rho_j w_j), D_j = diag(0.5 dt/vol_j (1, 1, fz)), P_j^T = the node's
+//     boundary maps (slip projections, x-zeroing at flags 7/8) in reverse entry order;
+//   stage 2 (gradient^T): out_j = sum_c [ sum_{e at j} s_je S_e^c 0.5 (gt_j^c - gt_i^c) + sbb_j^c gt_j^c ].
+// ------------------------------------------------------------------------------------
+struct LaplTArgs {
+  const double *y;
+  double *g0, *g1, *g2;
+  double *out;
+  NodeCoef nc;
+  double dt;
+  int impfl;
+};
+
+__global__ void __launch_bounds__(NFT_BLOCK) k_laplT_1(Incidence g, EdgeS<double> S, LaplTArgs a) {
+  Lane L;
+  if (!lane_setup(g, L) || !L.valid) return;
+  const int i = L.i;
+  const double yi = a.y[i] / (a.nc.vol[i] * rho(a.nc, i));
+  double wx = 0., wy = 0., wz = 0.;
+  for (int m = 0; m < L.deg; ++m) {
+    int slot = L.base + m * NFT_WAVE + L.lane;
+    int j = g.nbr[slot], e = g.eid[slot];
+    double sg = e >= 0 ? 0.5 : -0.5;
+    int ee = e >= 0 ? e : ~e;
+    double d = yi - a.y[j] / (a.nc.vol[j] * rho(a.nc, j));
+    wx += sg * S.sx[ee] * d; wy += sg * S.sy[ee] * d; wz += sg * S.sz[ee] * d;
+  }
+  int b0 = g.bp_ptr[i], b1 = g.bp_ptr[i + 1];
+  for (int b = b0; b < b1; ++b) {
+    wx += g.bp_sbb[b] * yi; wy += g.bp_sbb[g.nbp + b] * yi; wz += g.bp_sbb[2 * g.nbp + b] * yi;
+  }
+  double r = rho(a.nc, i);
+  wx *= r; wy *= r; wz *= r;
+  for (int b = b1 - 1; b >= b0; --b) {
+    int f = g.bp_flag[b];
+    if (f == 11 || f == 5 || f == 6) {
+      double nx = g.bp_anp[b], ny = g.bp_anp[g.nbp + b], nz = g.bp_anp[2 * g.nbp + b];
+      double vn = wx * nx + wy * ny + wz * nz;
+      wx -= nx * vn; wy -= ny * vn; wz -= nz * vn;
+    }
+    if (f == 7 || f == 8) wx = 0.;
+  }
+  double h = 0.5 * a.dt / a.nc.vol[i];
+  double fz = a.impfl ? cw(a.nc, i) : 1.0;
+  a.g0[i] = h * wx; a.g1[i] = h * wy; a.g2[i] = h * wz * fz;
+}
+
+__global__ void __launch_bounds__(NFT_BLOCK) k_laplT_2(Incidence g, EdgeS<double> S, LaplTArgs a) {
+  Lane L;
+  if (!lane_setup(g, L) || !L.valid) return;
+  const int i = L.i;
+  const double gx = a.g0[i], gy = a.g1[i], gz = a.g2[i];
+  double acc = 0.;
+  for (int m = 0; m < L.deg; ++m) {
+    int slot = L.base + m * NFT_WAVE + L.lane;
+    int j = g.nbr[slot], e = g.eid[slot];
+    double sg = e >= 0 ? 0.5 : -0.5;
+    int ee = e >= 0 ? e : ~e;
+    acc += sg * (S.sx[ee] * (gx - a.g0[j]) + S.sy[ee] * (gy - a.g1[j]) + S.sz[ee] * (gz - a.g2[j]));
+  }
+  for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b)
+    acc += g.bp_sbb[b] * gx + g.bp_sbb[g.nbp + b] * gy + g.bp_sbb[2 * g.nbp + b] * gz;
+  a.out[i] = acc;
+}
+
+// ------------------------------------------------------------------------------------
 // Vector kernels of GCR (grid-stride, fixed grid -> deterministic partials)
 // ------------------------------------------------------------------------------------
 // p += beta*x; r += beta*ax; reduce max|r| and sum r^2   (reference :1886-1893)
@@ -507,13 +576,24 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_reduce_final(const double *partia
   double acc[NFT_MAXD + 1];
 #pragma unroll
   for (int k = 0; k <= NFT_MAXD; ++k) acc[k] = ((op.ismax >> k) & 1) ? -1.0e300 : 0.0;
-  for (int b = threadIdx.x; b < nblk; b += NFT_BLOCK) {
+  // eight blocks' partials are loaded before they are added (in block order)
+  constexpr int U = 8;
+  for (int b0 = threadIdx.x; b0 < nblk; b0 += U * NFT_BLOCK) {
+    for (int k = 0; k < op.K; ++k) {
+      double pv[U];
 #pragma unroll
-    for (int k = 0; k <= NFT_MAXD; ++k) {
-      if (k < op.K) {
-        double p = partials[(size_t)b * op.K + k];
-        acc[k] = ((op.ismax >> k) & 1) ? fmax(acc[k], p) : acc[k] + p;
+      for (int u = 0; u < U; ++u) {
+        int b = b0 + u * NFT_BLOCK;
+        pv[u] = b < nblk ? partials[(size_t)b * op.K + k] : 0.0;
       }
+      double a = acc[0];
+      for (int kk = 0; kk <= NFT_MAXD; ++kk) if (kk == k) a = acc[kk];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (b0 + u * NFT_BLOCK < nblk) a = ((op.ismax >> k) & 1) ? fmax(a, pv[u]) : a + pv[u];
+      }
+#pragma unroll
+      for (int kk = 0; kk <= NFT_MAXD; ++kk) if (kk == k) acc[kk] = a;
     }
   }
   block_reduce<NFT_MAXD + 1>(acc, op.ismax, sh);

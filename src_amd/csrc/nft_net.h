@@ -132,6 +132,7 @@ __global__ void k_net_agg(int nrows, int ncol, int H, const int32_t *nptr, const
 }
 
 // dA[row][h] = (dA_direct[row][h] + sum_{c' in N(c)} dG[c'][h] / deg(c')) * (A[row][h] > 0)
+// (no mask when A is null: identity activation)
 __global__ void k_net_agg_back(int nrows, int ncol, int H, const int32_t *nptr, const int32_t *nidx,
                                const float *dC, int ldc, const float *A, int lda, float *dA, int ldo) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -144,7 +145,7 @@ __global__ void k_net_agg_back(int nrows, int ncol, int H, const int32_t *nptr, 
     g += dC[((size_t)sample * ncol + cn) * ldc + H + h] / (float)(nptr[cn + 1] - nptr[cn]);
   }
   float v = dC[(size_t)row * ldc + h] + g;
-  if (A[(size_t)row * lda + h] <= 0.0f) v = 0.0f;
+  if (A && A[(size_t)row * lda + h] <= 0.0f) v = 0.0f;
   dA[(size_t)row * ldo + h] = v;
 }
 
@@ -219,20 +220,59 @@ __global__ void k_net_loss(int M, int L, int Kp, const float *Y, const float *T,
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// training pairs of one finished solve: X = r_k / s_k, T = ((p_k - p_final) - qr_k) / s_k
-__global__ void k_net_make_pair(int ncol, int L, int Kp, const double *r, const double *qr, const double *pk,
-                                const double *pfin, double s, float *X, float *T) {
+// network input of a stored residual with its scale given by value
+__global__ void k_net_input_s(int ncol, int L, int Kp, const double *r, double s, float *X) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ncol * Kp) return;
   int c = t / Kp, k = t % Kp;
-  if (k < L && s > 0) {
-    size_t i = (size_t)c * L + k;
-    X[t] = (float)(r[i] / s);
-    T[t] = (float)(((pk[i] - pfin[i]) - qr[i]) / s);
-  } else {
-    X[t] = 0.0f;
-    T[t] = 0.0f;
+  X[t] = (k < L && s > 0) ? (float)(r[(size_t)c * L + k] / s) : 0.0f;
+}
+
+// z = q + s * Y (one sample's preconditioned direction)
+__global__ void k_net_z(int ncol, int L, int Kp, const double *q, const float *Y, double s, double *z) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncol * L) return;
+  int c = t / L, k = t % L;
+  z[t] = q[t] + s * (double)Y[(size_t)c * Kp + k];
+}
+
+// per-block partials (fixed tree) of <v,v> and <r,v>: part[2*block + {0,1}]
+__global__ void k_net_dots(int n, const double *v, const double *r, double *part) {
+  __shared__ double sh[2][256];
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double vv = 0.0, rv = 0.0;
+  if (t < n) {
+    vv = v[t] * v[t];
+    rv = r[t] * v[t];
   }
+  sh[0][threadIdx.x] = vv;
+  sh[1][threadIdx.x] = rv;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sh[0][0];
+    part[2 * blockIdx.x + 1] = sh[1][0];
+  }
+}
+
+// y = alpha * v + beta * r
+__global__ void k_net_axpby(int n, double alpha, const double *v, double beta, const double *r, double *y) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = alpha * v[t] + beta * r[t];
+}
+
+// dY of one sample = s * (L^T y) in the network's [column][Kp] layout
+__global__ void k_net_dY(int ncol, int L, int Kp, const double *g, double s, float *dY) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncol * Kp) return;
+  int c = t / Kp, k = t % Kp;
+  dY[t] = (k < L) ? (float)(s * g[(size_t)c * L + k]) : 0.0f;
 }
 
 }  // namespace nft

@@ -63,6 +63,7 @@ _SIGS = {
     "nft_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     "nft_derivative": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, ctypes.c_int]),
     "nft_laplc": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, ctypes.c_int]),
+    "nft_laplc_t": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, ctypes.c_int]),
     "nft_prfr": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p, f64p, ctypes.c_int]),
     "nft_bcv": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, f64p, ctypes.c_double]),
     "nft_div": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, f64p, f64p, ctypes.c_int]),
@@ -90,12 +91,16 @@ _SIGS.update({
                                     ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                     ctypes.POINTER(ctypes.c_double)]),
     "nft_dump_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
-    "nft_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float),
-                                 ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int,
-                                 ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_int]),
+    "nft_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
+                                 f64p, f64p, f64p, f64p]),
+    "nft_net_pair_loss": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_float)]),
     "nft_net_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64,
-                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
-    "nft_set_learned": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]),
+                                     ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "nft_set_learned": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_int)]),
+    "nft_net_options": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "nft_copy_calibration": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
 })
 
 f32p = ctypes.POINTER(ctypes.c_float)
@@ -222,6 +227,12 @@ class Solver:
         _chk(lib().nft_laplc(self.h, _p(_f64(p)), _p(out), impfl), "nft_laplc")
         return out
 
+    def laplc_t(self, y, impfl):
+        """Transpose of laplc (no reference counterpart)."""
+        out = np.zeros(self.n)
+        _chk(lib().nft_laplc_t(self.h, _p(_f64(y)), _p(out), impfl), "nft_laplc_t")
+        return out
+
     def prfr(self, p, us, vs, ws, impfl):
         outs = [np.zeros(self.n) for _ in range(3)]
         ins = [_f64(a) for a in (p, us, vs, ws)]
@@ -322,37 +333,49 @@ class Solver:
         _chk(lib().nft_net_eval(self.h, _pf(X), _pf(Tc), ns, _pf(Y), _pf(grad), ctypes.byref(loss)), "nft_net_eval")
         return Y, grad, loss.value
 
-    def dump_pairs(self, max_pairs, max_stage=400):
-        _chk(lib().nft_dump_pairs(self.h, max_pairs, max_stage), "nft_dump_pairs")
+    def dump_pairs(self, max_pairs, stride=1):
+        _chk(lib().nft_dump_pairs(self.h, max_pairs, stride), "nft_dump_pairs")
 
     def pair_count(self):
         n = ctypes.c_int()
-        _chk(lib().nft_pairs(self.h, ctypes.byref(n), None, None, 0, 0, None, None, 0), "nft_pairs")
+        _chk(lib().nft_pairs(self.h, ctypes.byref(n), 0, 0, None, None, None, None), "nft_pairs")
         return n.value
 
     def pairs_get(self, first, count):
-        info = self.net_info()
-        per = info["ncol"] * info["kp"]
-        X = np.zeros(per * count, np.float32); T = np.zeros(per * count, np.float32)
-        _chk(lib().nft_pairs(self.h, None, _pf(X), _pf(T), first, count, None, None, 0), "nft_pairs")
-        return X.reshape(count, -1), T.reshape(count, -1)
+        """Stored pairs [first, first+count): r, q (count x npoin), s, |r|^2."""
+        r = np.zeros((count, self.n)); q = np.zeros((count, self.n))
+        s = np.zeros(count); r2 = np.zeros(count)
+        _chk(lib().nft_pairs(self.h, None, first, count, _p(r), _p(q), _p(s), _p(r2)), "nft_pairs")
+        return r, q, s, r2
 
-    def pairs_put(self, X, T):
-        X = np.ascontiguousarray(X, np.float32); T = np.ascontiguousarray(T, np.float32)
-        n = ctypes.c_int()
-        _chk(lib().nft_pairs(self.h, ctypes.byref(n), None, None, 0, 0, _pf(X), _pf(T), X.shape[0]), "nft_pairs")
-        return n.value
+    def pair_loss(self, first, count, grad=False):
+        """Physics-informed loss on stored pairs; with grad=True also the parameter gradient."""
+        v = ctypes.c_double()
+        g = np.zeros(self.net_info()["nparams"], np.float32) if grad else None
+        _chk(lib().nft_net_pair_loss(self.h, first, count, ctypes.byref(v), _pf(g)), "nft_net_pair_loss")
+        return (v.value, g) if grad else v.value
 
-    def net_train(self, steps, batch=8, lr=1e-3, seed=0):
+    def net_train(self, steps, batch=8, lr=1e-3, seed=0, use_pairs=0):
+        """Adam on stored pairs [0, use_pairs) (all by default); first and last minibatch loss."""
         a, b = ctypes.c_double(), ctypes.c_double()
-        _chk(lib().nft_net_train(self.h, steps, batch, lr, ctypes.c_uint64(seed), ctypes.byref(a), ctypes.byref(b)),
-             "nft_net_train")
+        _chk(lib().nft_net_train(self.h, steps, batch, lr, ctypes.c_uint64(seed), use_pairs, ctypes.byref(a),
+                                 ctypes.byref(b)), "nft_net_train")
         return a.value, b.value
 
     def set_learned(self, kind, stall=0.99):
-        fb = ctypes.c_int()
-        _chk(lib().nft_set_learned(self.h, kind, stall, ctypes.byref(fb)), "nft_set_learned")
-        return fb.value
+        """Select the preconditioner (kind < 0: no change); returns (fallbacks, learned iterations)."""
+        fb, li = ctypes.c_int(), ctypes.c_int()
+        _chk(lib().nft_set_learned(self.h, kind, stall, ctypes.byref(fb), ctypes.byref(li)), "nft_set_learned")
+        return fb.value, li.value
+
+    def net_options(self, objective=0, linear=0):
+        """Training objective (0 residual, 1 angle); linear=1 makes the correction linear in r."""
+        _chk(lib().nft_net_options(self.h, objective, linear), "nft_net_options")
+
+    def copy_calibration(self, reps=10):
+        ms = ctypes.c_double()
+        _chk(lib().nft_copy_calibration(self.h, reps, ctypes.byref(ms)), "nft_copy_calibration")
+        return ms.value
 
     def gcr_stats(self):
         st = GcrStats()

@@ -1,11 +1,13 @@
 """Learned preconditioner on the GPU: MFMA forward and backward against the FP64 oracle
-(oracle/netref.py, bf16-rounded operands), training on pairs the plain solver dumps, and
-what the trained network changes in the flexible GCR solve."""
+(oracle/netref.py, bf16-rounded operands), the transpose of laplc against the dense
+transpose of the CPU oracle's laplc, the physics-informed training loss and its gradient,
+and what the trained network changes in the flexible GCR solve."""
 import numpy as np
 import pytest
 
 import netref
-from common import mesh_for, rel_l2, small_cfg
+from common import mesh_for, rel_l2, small_cfg, tiny_cfg
+from oracle import Oracle
 from src_amd import nft
 
 pytestmark = pytest.mark.gpu
@@ -75,23 +77,134 @@ def test_gradients_match_oracle(small_net):
         assert e <= 3e-2, name
 
 
+@pytest.mark.parametrize("impfl", [0, 1])
+def test_laplc_transpose_matches_dense_oracle(impfl):
+    """L^T y from the GPU against the transpose of the dense matrix assembled column by
+    column from the CPU oracle's laplc (tiny2 mesh: 378 nodes, boundary maps of every kind)."""
+    m = mesh_for("tiny2")
+    cfg = tiny_cfg()
+    o = Oracle(m, cfg)
+    n = m.npoin
+    A = np.zeros((n, n))
+    e = np.zeros(n)
+    for j in range(n):
+        e[j] = 1.0
+        A[:, j] = o.laplc(e, impfl)
+        e[j] = 0.0
+    o.close()
+    s = nft.Solver(m, nft.make_params(cfg))
+    rng = np.random.default_rng(7 + impfl)
+    for _ in range(3):
+        y = rng.normal(size=n)
+        got = s.laplc_t(y, impfl)
+        err = rel_l2(got, A.T @ y)
+        print("L^T rel L2", err)
+        assert err <= 1e-13
+    s.close()
+
+
+@pytest.mark.parametrize("impfl", [0, 1])
+def test_laplc_transpose_adjoint(impfl):
+    """<L x, y> = <x, L^T y> on the 40,000-node mesh."""
+    m = mesh_for("small")
+    s = nft.Solver(m, nft.make_params(small_cfg(**LINE)))
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        x, y = rng.normal(size=m.npoin), rng.normal(size=m.npoin)
+        a, b = np.dot(s.laplc(x, impfl), y), np.dot(x, s.laplc_t(y, impfl))
+        print("adjoint", a, b)
+        assert abs(a - b) <= 1e-12 * (np.linalg.norm(s.laplc(x, impfl)) * np.linalg.norm(y))
+    s.close()
+
+
+@pytest.fixture(scope="module")
+def small_pairs():
+    m = mesh_for("small")
+    cfg = small_cfg(**LINE)
+    s = nft.Solver(m, nft.make_params(cfg))
+    s.net_init(hidden=128, seed=9, max_batch=4)
+    s.state_init()
+    s.dump_pairs(max_pairs=24, stride=5)
+    s.run_steps(2)
+    s.dump_pairs(0)
+    yield m, cfg, s
+    s.close()
+
+
+def test_pair_loss_matches_oracle(small_pairs):
+    """With a zero network z = q, so the loss is the mean of |L q - r|^2/|r|^2 over the
+    pairs: compared with the CPU oracle's laplc applied to the stored pairs."""
+    m, cfg, s = small_pairs
+    npairs = s.pair_count()
+    assert npairs >= 8
+    keep = s.net_params()
+    s.net_params(np.zeros_like(keep))
+    got = s.pair_loss(0, 8)
+    s.net_params(keep)
+    r, q, sc, r2 = s.pairs_get(0, 8)
+    assert np.allclose(sc, np.abs(r).max(axis=1), rtol=0, atol=0)
+    assert np.allclose(r2, (r * r).sum(axis=1), rtol=1e-12)
+    o = Oracle(m, cfg)
+    ref = np.mean([np.sum((o.laplc(q[k], 1) - r[k]) ** 2) / r2[k] for k in range(8)])
+    o.close()
+    print("line-only loss", got, ref)
+    assert abs(got - ref) <= 1e-10 * ref
+
+
+@pytest.mark.parametrize("objective,linear", [(0, 0), (1, 0), (0, 1)])
+def test_pair_loss_gradient(small_pairs, objective, linear):
+    """Central difference of the loss along a random direction of the output bias b6
+    against the directional derivative from the gradient: checks dY = s L^T(dl/dv)/B and its
+    column sums. The residual objective is exactly quadratic in b6 (central differences are
+    exact up to rounding); the angle objective is smooth, with an O(eps^2) difference error."""
+    m, cfg, s = small_pairs
+    info = s.net_info()
+    flat = s.net_params()
+    s.net_options(objective, linear)
+    base = s.net_params()
+    nb6 = info["kp"]
+    off = flat.size - nb6
+    l0, g = s.pair_loss(0, 4, grad=True)
+    rng = np.random.default_rng(1)
+    v = np.zeros(nb6, np.float32)
+    v[:info["levels"]] = rng.normal(size=info["levels"])
+    eps = 1e-2 if objective == 0 else 2e-3
+    lp, lm = [], []
+    for sign, out in ((1, lp), (-1, lm)):
+        f = base.copy()
+        f[off:] += np.float32(sign * eps) * v
+        s.net_params(f)
+        out.append(s.pair_loss(0, 4))
+    s.net_options(0, 0)
+    s.net_params(flat)
+    fd = (lp[0] - lm[0]) / (2 * eps)
+    an = float(np.dot(g[off:].astype(np.float64), v))
+    print("loss", l0, "directional derivative", fd, an)
+    assert abs(fd - an) <= 1e-3 * abs(fd) + 1e-9
+
+
 def test_learned_preconditioner_end_to_end():
-    """Train on (residual, correction) pairs dumped by plain line-preconditioned solves on
-    the 40,000-node mesh, then run the same steps with the learned correction: every solve
-    still reaches 1e-8, the fields agree with the plain run to the solver tolerance, and
-    the iteration counts of both runs are reported."""
+    """Train on (r, M_line^-1 r) pairs stored by plain line-preconditioned solves on the
+    40,000-node mesh (residual objective), check that the loss falls on held-out pairs,
+    then run the same steps with the learned correction: every solve still reaches 1e-8,
+    the fields agree with the plain run to the solver tolerance, and the iteration counts
+    and solve times of both runs are reported (on this problem the learned correction
+    lowers the one-step residual but not the GCR iteration count)."""
     m = mesh_for("small")
     cfg = small_cfg(lord=4, **LINE)
     plain = nft.Solver(m, nft.make_params(cfg))
     plain.net_init(hidden=128, seed=1, max_batch=8)
-    plain.dump_pairs(max_pairs=600, max_stage=300)
+    plain.dump_pairs(max_pairs=400, stride=2)
     plain.state_init()
-    st_dump = plain.run_steps(6)
+    plain.run_steps(6)
     npairs = plain.pair_count()
     assert npairs > 100
-    l0, l1 = plain.net_train(steps=600, batch=8, lr=2e-3, seed=3)
-    print(f"pairs {npairs}, training loss {l0:.4e} -> {l1:.4e}")
-    assert l1 < l0
+    held = min(32, npairs // 4)
+    before = plain.pair_loss(npairs - held, held)
+    l0, l1 = plain.net_train(steps=600, batch=8, lr=1e-3, seed=3, use_pairs=npairs - held)
+    after = plain.pair_loss(npairs - held, held)
+    print(f"pairs {npairs}, training loss {l0:.4e} -> {l1:.4e}; held-out {held} pairs {before:.4e} -> {after:.4e}")
+    assert l1 < l0 and after < before
     params = plain.net_params()
     plain.close()
 
@@ -103,16 +216,16 @@ def test_learned_preconditioner_end_to_end():
             s.set_learned(1, stall=0.99)
         s.state_init()
         st = s.run_steps(6)
-        fb = s.set_learned(-1)
+        fb, li = s.set_learned(-1)
         unk = s.state("unk")
         s.close()
-        return st, unk, fb
+        return st, unk, (fb, li)
 
     sp, up, _ = run(False)
     sl, ul, fb = run(True)
     gain = sp.gcr_iters / max(sl.gcr_iters, 1)
     print(f"plain: {sp.gcr_iters} iterations in {sp.gcr_solves} solves, {sp.ms_gcr:.1f} ms; "
-          f"learned: {sl.gcr_iters} iterations, {sl.ms_gcr:.1f} ms, fallbacks {fb}; "
+          f"learned: {sl.gcr_iters} iterations, {sl.ms_gcr:.1f} ms, (fallbacks, learned iterations) {fb}; "
           f"iteration ratio {gain:.3f}, time ratio {sp.ms_gcr / max(sl.ms_gcr, 1e-9):.3f}")
     assert sl.gcr_unconverged == 0 and sp.gcr_unconverged == 0
     for k in range(4):
