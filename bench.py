@@ -32,10 +32,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def production_config():
+def production_config(column_order="grid"):
     from src_amd import mesh as M
     c = M.CONFIGS["production"]
-    return M.RunConfig(c.name, c.mesh, u0=c.u0, bvf=c.bvf, dt=c.dt, steps=c.steps, lord=4,
+    return M.RunConfig(c.name, M.production_spec(column_order), u0=c.u0, bvf=c.bvf, dt=c.dt, steps=c.steps, lord=4,
                        eps0=1e-8, exit_mode=1, itr=400, precon=2, itrp1=1, omega=1.0,
                        sponge_depth=3000.0, sponge_tau=300.0)
 
@@ -158,6 +158,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--learned-train-steps", type=int, default=100)
+    ap.add_argument("--column-order", default="grid", choices=("grid", "hilbert"),
+                    help="numbering of the production mesh's columns")
+    ap.add_argument("--no-roofline", action="store_true",
+                    help="skip the per-kernel timing pass (for runs under a counter profiler)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run N plain device copies of one node array (byte-counter reference)")
     args = ap.parse_args()
@@ -168,7 +172,7 @@ def main():
     from src_amd import mesh as M
     from src_amd import nft
 
-    cfg = production_config()
+    cfg = production_config(args.column_order)
     t0 = time.perf_counter()
     mesh = M.generate(cfg.mesh)
     log(f"mesh: {mesh.npoin} nodes, {mesh.nedge} edges + {mesh.nbface} semi-edges, "
@@ -181,13 +185,13 @@ def main():
         log(f"copy calibration: {mesh.npoin * 8} bytes read + written per copy, {ms * 1e3:.1f} us, "
             f"{2 * mesh.npoin * 8 / (ms * 1e-3) / 1e9:.0f} GB/s")
         s0.close()
-    r = timed_run(mesh, cfg, "fp64", args.steps, args.warmup)
+    r = timed_run(mesh, cfg, "fp64", args.steps, args.warmup, profile=not args.no_roofline)
     st = r["st"]
     K = args.steps
     ms_step = st.ms_total / max(K, 1)
     value = 1000.0 / ms_step if ms_step > 0 else 0.0
     iters = st.gcr_iters / max(st.gcr_solves, 1)
-    kname, nl, kms, kbytes = r["kernel"]
+    kname, nl, kms, kbytes = r.get("kernel", (None, 0, 0.0, 0.0))
     avg_ms = kms / max(nl, 1)
     achieved = kbytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     unk = r["solver"].state("unk")

@@ -132,6 +132,11 @@ int nft_set_state(nft_ctx *ctx, const char *name, const double *host, int ld);
 /* set-up fields: "alpha","d1","coefw","rh0","th0","ue","line_a","line_inv","line_c" */
 int nft_get_field(nft_ctx *ctx, const char *name, double *host);
 int nft_get_gcr_stats(nft_ctx *ctx, nft_gcr_stats *stats);
+/* Incidence used by the gradient/divergence sweeps (no reference counterpart; results are
+ * identical either way): 1 = columnar (per-column slot template for the interior levels,
+ * available when the mesh is extruded with identical interior levels; the default then),
+ * 0 = node-sorted SELL-64. columnar < 0 only reports availability. */
+int nft_set_incidence(nft_ctx *ctx, int columnar, int *available);
 
 /* ---- measurement (no reference counterpart: the reference only times the whole run) -- */
 /* Kernel ids for the timer: 1 laplc gradient half, 2 laplc divergence half, 3 line solve,

@@ -181,8 +181,8 @@ oc_ctx *oc_create(const oc_mesh *m, const oc_params *p) {
 /* ---- the HIP path's reduction tree (dot_order = 1) --------------------------------
  * wave: lane l += lane l+off for off = 32,16,...,1 (shfl_down), result in lane 0;
  * block: ((w0 + w1) + w2) + w3 over its four wavefronts;
- * grid: thread t of the last block sums partials t, t+256, ... in order, then the block
- * tree again. Inactive lanes hold 0. */
+ * grid: one block of 1024 threads; thread t sums partials t, t+1024, ... in order, then
+ * the wavefront trees and the 16 wavefront results in order. Inactive lanes hold 0. */
 static double tree64(double *v) {
   for (int off = 32; off > 0; off >>= 1)
     for (int l = 0; l < off; ++l) v[l] = v[l] + v[l + off];
@@ -201,13 +201,19 @@ static double block_sum(const double *t256) {
 }
 
 static double grid_sum(const double *part, int nblk) {
-  double t[256];
-  for (int th = 0; th < 256; ++th) {
+  double t[1024], w[64];
+  for (int th = 0; th < 1024; ++th) {
     double acc = 0.0;
-    for (int b = th; b < nblk; b += 256) acc = acc + part[b];
+    for (int b = th; b < nblk; b += 1024) acc = acc + part[b];
     t[th] = acc;
   }
-  return block_sum(t);
+  double a = 0.0;
+  for (int q = 0; q < 16; ++q) {
+    for (int l = 0; l < 64; ++l) w[l] = t[q * 64 + l];
+    double r = tree64(w);
+    a = q == 0 ? r : a + r;
+  }
+  return a;
 }
 
 /* sum over nodes of a[i]*b[i] (a NULL: b[i]*b[i]) as the fused dots of the gather kernels */

@@ -5,8 +5,9 @@
 // jittered points (the triangulation itself is an input):
 //   * columns c = 0..nh-1, levels k = 0..L-1, node id n = c*L + k (level fastest),
 //     z = zs(c) + zeta_k * (H - zs(c)) / H (Gal-Chen terrain following);
-//   * every prism (triangle a<b<c, layer k) is split into 3 tets with the diagonal of
-//     each quad face leaving its lowest-numbered node, which makes the split conforming:
+//   * every prism (triangle a<b<c by split rank, layer k; the rank is the column id unless
+//     given) is split into 3 tets with the diagonal of each quad face leaving its
+//     lowest-ranked node, which makes the split conforming:
 //     {a0,b0,c0,c1}, {a0,b0,b1,c1}, {a0,a1,b1,c1};
 //   * median-dual metrics: per tet edge (p,q) with the other nodes (r,s) in positive
 //     orientation the dual facet area vector is S_pq = ((r+s)-(p+q)) x (s-r) / 24, and
@@ -70,11 +71,14 @@ struct nft_genmesh;
 
 // Builds the mesh. xy: [2][nh] column positions (x block then y block); tri: [3][nt]
 // 0-based column ids (any orientation); zs: terrain height per column; zeta: L level
-// heights in [0,H] (zeta[0]=0, zeta[L-1]=H); box = {xmin,xmax,ymin,ymax}.
+// heights in [0,H] (zeta[0]=0, zeta[L-1]=H); box = {xmin,xmax,ymin,ymax}; rank: optional
+// [nh] distinct split ranks (NULL: the column id). Prisms are split from their
+// lowest-ranked column, so a renumbering of the columns that passes the old ids as ranks
+// yields the same tetrahedra.
 // Returns NULL on failure (message in *errbuf when given).
 nft_genmesh* nft_meshgen_build(int nh, const double* xy, int nt, const int32_t* tri,
                                const double* zs, int L, const double* zeta, double H,
-                               const double* box, char* errbuf, int errlen) {
+                               const double* box, const int32_t* rank, char* errbuf, int errlen) {
   auto fail = [&](const char* m) -> nft_genmesh* {
     if (errbuf && errlen > 0) { std::snprintf(errbuf, errlen, "%s", m); }
     return nullptr;
@@ -85,6 +89,7 @@ nft_genmesh* nft_meshgen_build(int nh, const double* xy, int nt, const int32_t* 
   m->nh = nh; m->L = L; m->npoin = (int64_t)nh * L;
   const int64_t N = m->npoin;
   const double xmin = box[0], xmax = box[1], ymin = box[2], ymax = box[3];
+  auto rk = [&](int c) -> int64_t { return rank ? (int64_t)rank[c] : (int64_t)c; };
 
   // ---- node coordinates
   m->coord.assign(3 * N, 0.0);
@@ -163,9 +168,11 @@ nft_genmesh* nft_meshgen_build(int nh, const double* xy, int nt, const int32_t* 
         r1[e] = (int32_t)((int64_t)c * L + k); r2[e] = (int32_t)((int64_t)c2 * L + k);
         rsurf[e] = (k == 0 || k == L - 1 || hull[h]) ? 1 : 0;
       }
+      // the quad's diagonal rises from the lower-ranked column; endpoints stored id-ascending
+      const bool up = rk(c) < rk(c2);
       for (int k = 0; k < L - 1; ++k) {
         int64_t e = dbase[h] + k;
-        r1[e] = (int32_t)((int64_t)c * L + k); r2[e] = (int32_t)((int64_t)c2 * L + k + 1);
+        r1[e] = (int32_t)((int64_t)c * L + k + (up ? 0 : 1)); r2[e] = (int32_t)((int64_t)c2 * L + k + (up ? 1 : 0));
         rsurf[e] = hull[h];
       }
     }
@@ -188,7 +195,7 @@ nft_genmesh* nft_meshgen_build(int nh, const double* xy, int nt, const int32_t* 
   double minvol = 1e300;
   for (int t = 0; t < nt; ++t) {
     int cc[3] = {tri[t], tri[nt + t], tri[2 * nt + t]};
-    std::sort(cc, cc + 3);
+    std::sort(cc, cc + 3, [&](int x, int y) { return rk(x) < rk(y); });
     const int a = cc[0], b = cc[1], c = cc[2];
     int64_t hh[3] = {hid(a, b), hid(b, c), hid(c, a)};
     for (int k = 0; k < L - 1; ++k) {
@@ -262,8 +269,10 @@ nft_genmesh* nft_meshgen_build(int nh, const double* xy, int nt, const int32_t* 
     else if (y1 == ymin && y2 == ymin) { part = SOUTH; out = {0, -1, 0}; }
     else if (y1 == ymax && y2 == ymax) { part = NORTH; out = {0, 1, 0}; }
     else { delete m; return fail("meshgen: hull edge not on the box"); }
+    // the side quad is split along the prism's diagonal, which rises from the lower-ranked column
+    const int lo = rk(c1) < rk(c2) ? c1 : c2, hi = lo == c1 ? c2 : c1;
     for (int k = 0; k < L - 1; ++k) {
-      int64_t p0 = (int64_t)c1 * L + k, q0 = (int64_t)c2 * L + k;
+      int64_t p0 = (int64_t)lo * L + k, q0 = (int64_t)hi * L + k;
       addface(part, p0, q0, q0 + 1, out);
       addface(part, p0, q0 + 1, p0 + 1, out);
     }

@@ -76,6 +76,8 @@ struct nft_ctx {
   float *laf, *linvf, *lcf;
   int line_L = 0;             // uniform column length (0: not uniform or > 64 -> Thomas)
   int col_L = 0;              // uniform column length with nodes numbered column by column (0: not)
+  int col_on = 0;             // gathers use the columnar incidence (inc.col_L > 0)
+  double col_bytes = 0;       // bytes of the columnar tables
   double *llo, *ldg, *lup;    // line operator rows (PCR)
   float *llof, *ldgf, *lupf;
   double *zbuf;
@@ -100,6 +102,11 @@ struct nft_ctx {
   unsigned *counter;
   double *scal;
   double *h_scal;  // pinned
+  // device-controlled GCR loop: exit flag handed to the loop kernels (null outside it) and
+  // the same decision mirrored into mapped host memory for the host's poll
+  const double *skip = nullptr;
+  double *h_flag = nullptr, *d_hflag = nullptr;
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};
   // io staging for the per-subroutine entry points
   double *io[8];
 
@@ -339,24 +346,38 @@ inline void t_begin(nft_ctx *c, int k) {
 inline void t_end(nft_ctx *c, int k) {
   if (timing(c, k)) { (void)hipEventRecord(c->tev[c->tev_used + 1], c->stream); c->tev_used += 2; }
 }
-// after a stream synchronisation: fold finished event pairs into the totals
-void t_harvest(nft_ctx *c) {
-  for (int i = 0; i + 1 < c->tev_used; i += 2) {
+// fold the first `upto` recorded events (finished pairs) into the totals and move the rest
+// to the front
+void t_harvest_prefix(nft_ctx *c, int upto) {
+  upto = std::min(upto, c->tev_used) & ~1;
+  for (int i = 0; i + 1 < upto; i += 2) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, c->tev[i], c->tev[i + 1]) == hipSuccess) { c->t_ms += ms; c->t_launches++; }
   }
-  c->tev_used = 0;
+  std::rotate(c->tev.begin(), c->tev.begin() + upto, c->tev.begin() + c->tev_used);
+  c->tev_used -= upto;
 }
+// after a stream synchronisation: fold finished event pairs into the totals
+void t_harvest(nft_ctx *c) { t_harvest_prefix(c, c->tev_used); }
 
 // ---- device operators -------------------------------------------------------------------
 int grad_launch(nft_ctx *c, int mode, const GradArgs &a) {
   dim3 g(c->gather_grid), b(NFT_BLOCK);
   if (mode == GRAD_LAPLC) t_begin(c, KT_LGRAD);
-  switch (mode) {
-    case GRAD_RAW: hipLaunchKernelGGL(k_grad<GRAD_RAW>, g, b, 0, c->stream, c->inc, c->S, a); break;
-    case GRAD_PRFR: hipLaunchKernelGGL(k_grad<GRAD_PRFR>, g, b, 0, c->stream, c->inc, c->S, a); break;
-    case GRAD_LAPLC: hipLaunchKernelGGL(k_grad<GRAD_LAPLC>, g, b, 0, c->stream, c->inc, c->S, a); break;
-    default: hipLaunchKernelGGL(k_grad<GRAD_ZERO_PRFR>, g, b, 0, c->stream, c->inc, c->S, a); break;
+  if (c->col_on) {
+    switch (mode) {
+      case GRAD_RAW: hipLaunchKernelGGL((k_grad<GRAD_RAW, true>), g, b, 0, c->stream, c->inc, c->S, a); break;
+      case GRAD_PRFR: hipLaunchKernelGGL((k_grad<GRAD_PRFR, true>), g, b, 0, c->stream, c->inc, c->S, a); break;
+      case GRAD_LAPLC: hipLaunchKernelGGL((k_grad<GRAD_LAPLC, true>), g, b, 0, c->stream, c->inc, c->S, a); break;
+      default: hipLaunchKernelGGL((k_grad<GRAD_ZERO_PRFR, true>), g, b, 0, c->stream, c->inc, c->S, a); break;
+    }
+  } else {
+    switch (mode) {
+      case GRAD_RAW: hipLaunchKernelGGL((k_grad<GRAD_RAW, false>), g, b, 0, c->stream, c->inc, c->S, a); break;
+      case GRAD_PRFR: hipLaunchKernelGGL((k_grad<GRAD_PRFR, false>), g, b, 0, c->stream, c->inc, c->S, a); break;
+      case GRAD_LAPLC: hipLaunchKernelGGL((k_grad<GRAD_LAPLC, false>), g, b, 0, c->stream, c->inc, c->S, a); break;
+      default: hipLaunchKernelGGL((k_grad<GRAD_ZERO_PRFR, false>), g, b, 0, c->stream, c->inc, c->S, a); break;
+    }
   }
   if (mode == GRAD_LAPLC) t_end(c, KT_LGRAD);
   HIPCHK(hipGetLastError());
@@ -365,20 +386,25 @@ int grad_launch(nft_ctx *c, int mode, const GradArgs &a) {
 
 GradArgs grad_args(nft_ctx *c) {
   GradArgs a{};
-  a.nc = c->nc; a.dt = c->P.dt;
+  a.nc = c->nc; a.dt = c->P.dt; a.skip = c->skip;
   return a;
 }
 
 DivArgs div_args(nft_ctx *c) {
   DivArgs a{};
   a.nc = c->nc; a.partials = c->partials; a.counter = c->counter;
-  a.scal = c->scal; a.epa = 1.e-38;
+  a.scal = c->scal; a.epa = 1.e-38; a.skip = c->skip;
   return a;
 }
 
 template <int MODE, int ND, int NMAX>
 void div_go(nft_ctx *c, const DivArgs &a) {
-  hipLaunchKernelGGL((k_div<MODE, ND, NMAX>), dim3(c->gather_grid), dim3(NFT_BLOCK), 0, c->stream, c->inc, c->S, a);
+  if (c->col_on)
+    hipLaunchKernelGGL((k_div<MODE, ND, NMAX, true>), dim3(c->gather_grid), dim3(NFT_BLOCK), 0, c->stream, c->inc,
+                       c->S, a);
+  else
+    hipLaunchKernelGGL((k_div<MODE, ND, NMAX, false>), dim3(c->gather_grid), dim3(NFT_BLOCK), 0, c->stream, c->inc,
+                       c->S, a);
 }
 
 template <int ND>
@@ -391,8 +417,24 @@ bool div_laplc_nd(nft_ctx *c, const DivArgs &a, int nd) {
   }
 }
 
-int final_launch(nft_ctx *c, int nblk, const FinalOp &op) {
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(NFT_BLOCK), 0, c->stream, c->partials, nblk, op, c->scal);
+template <int K>
+bool final_go(nft_ctx *c, int nblk, const FinalOp &op) {
+  if constexpr (K > NFT_MAXD + 1) {
+    return false;
+  } else {
+    if (op.K == K) {
+      hipLaunchKernelGGL(k_reduce_final<K>, dim3(1), dim3(NFT_FIN_THREADS), 0, c->stream, c->partials, nblk, op,
+                         c->scal);
+      return true;
+    }
+    return final_go<K + 1>(c, nblk, op);
+  }
+}
+
+int final_launch(nft_ctx *c, int nblk, const FinalOp &op0) {
+  FinalOp op = op0;
+  op.skip = c->skip;
+  if (!final_go<1>(c, nblk, op)) return set_err(-1, "reduction: too many values per block");
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -463,7 +505,7 @@ int precon_dev(nft_ctx *c, const double *rhs, double *p, int impfl) {
   const nft_params &P = c->P;
   const int n = c->n;
   if (P.precon == 0) {
-    hipLaunchKernelGGL(k_copy, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, n, rhs, p);
+    hipLaunchKernelGGL(k_copy, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, n, rhs, p, c->skip);
     HIPCHK(hipGetLastError());
     return 0;
   }
@@ -474,11 +516,12 @@ int precon_dev(nft_ctx *c, const double *rhs, double *p, int impfl) {
     }
     if (P.precon == 1) {
       hipLaunchKernelGGL(k_point, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, n, c->pstep,
-                         c->rw, rhs, p, P.omega, it == 1 ? 1 : 0);
+                         c->rw, rhs, p, P.omega, it == 1 ? 1 : 0, c->skip);
     } else {
       const double *res = rhs;
       if (it > 1) {
-        hipLaunchKernelGGL(k_sub, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, n, rhs, c->rw, c->t[3]);
+        hipLaunchKernelGGL(k_sub, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, n, rhs, c->rw, c->t[3],
+                           c->skip);
         res = c->t[3];
       }
       t_begin(c, KT_LINE);
@@ -489,18 +532,18 @@ int precon_dev(nft_ctx *c, const double *rhs, double *p, int impfl) {
         int cg = (int)((waves + NFT_WPB - 1) / NFT_WPB);
         if (P.precision == 1)
           hipLaunchKernelGGL(k_line_pcr<float>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->line_L,
-                             c->llof, c->ldgf, c->lupf, res, pin, p, P.omega);
+                             c->llof, c->ldgf, c->lupf, res, pin, p, P.omega, c->skip);
         else
           hipLaunchKernelGGL(k_line_pcr<double>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->line_L,
-                             c->llo, c->ldg, c->lup, res, pin, p, P.omega);
+                             c->llo, c->ldg, c->lup, res, pin, p, P.omega, c->skip);
       } else {
         int cg = blocks_for(c->ncol);
         if (P.precision == 1)
           hipLaunchKernelGGL(k_line<float>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->col_start,
-                             c->laf, c->linvf, c->lcf, res, pin, p, P.omega, c->zbuff);
+                             c->laf, c->linvf, c->lcf, res, pin, p, P.omega, c->zbuff, c->skip);
         else
           hipLaunchKernelGGL(k_line<double>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->col_start,
-                             c->la, c->linv, c->lc, res, pin, p, P.omega, c->zbuf);
+                             c->la, c->linv, c->lc, res, pin, p, P.omega, c->zbuf, c->skip);
       }
       t_end(c, KT_LINE);
     }
@@ -703,6 +746,108 @@ int norms_dev(nft_ctx *c, const double *a, int sum_slot, int max_slot) {
   return final_launch(c, c->vec_grid, op);
 }
 
+// One GCR inner iteration (reference :1886-1936) as launched by both loops: update with
+// its reductions, preconditioner, laplc of the preconditioned residual with the fused
+// <ax_ii, ar>, new direction with <r, ax>, <ax, ax>. `ctl` is the finalize of the update
+// (device exit test when ctl.ctl is set).
+static int gcr_iteration_update(nft_ctx *c, double *p, int i, const FinalOp &ctl) {
+  t_begin(c, KT_UPD);
+  hipLaunchKernelGGL(k_gcr_update, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, p, c->r, c->gx[i - 1],
+                     c->gax[i - 1], c->scal, c->partials, c->skip);
+  t_end(c, KT_UPD);
+  HIPCHK(hipGetLastError());
+  return final_launch(c, c->vec_grid, ctl);
+}
+
+static int gcr_iteration_rest(nft_ctx *c, int i, int impfl, double epa) {
+  const int lord = c->P.lord;
+  int rc;
+  if ((rc = precon_dev(c, c->r, c->qr, impfl))) return rc;
+  {
+    DotList dl{};
+    dl.nd = i;
+    for (int ii = 0; ii < i; ++ii) dl.a[ii] = c->gax[ii];
+    dl.fin = FIN_DEL;
+    if ((rc = laplc_dev(c, c->qr, c->ar, impfl, &dl))) return rc;
+  }
+  DirArgs d{};
+  d.n = c->n; d.nx = i;
+  int slot = (i < lord) ? i : 0;
+  d.xn = c->gx[slot]; d.axn = c->gax[slot]; d.slot = slot;
+  for (int ii = 0; ii < i; ++ii) { d.x[ii] = c->gx[ii]; d.ax[ii] = c->gax[ii]; }
+  d.qr = c->qr; d.ar = c->ar; d.r = c->r;
+  d.scal = c->scal; d.partials = c->partials; d.counter = c->counter; d.epa = epa; d.skip = c->skip;
+  t_begin(c, KT_DIR);
+  hipLaunchKernelGGL(k_gcr_dir, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, d);
+  t_end(c, KT_DIR);
+  HIPCHK(hipGetLastError());
+  FinalOp op{};
+  op.kind = FK_DIR; op.K = 2; op.ismax = 0u; op.slot = slot; op.epa = epa;
+  return final_launch(c, c->vec_grid, op);
+}
+
+// The inner loop with the exit test on the device: every iteration is enqueued without a
+// host round trip; the host reads the mapped exit flag one iteration behind (the GPU
+// always has the next iteration queued) and stops enqueueing once it is set. Iterations
+// enqueued past the exit return at once. Same arithmetic and exit rule as the host loop.
+static int gcr_loop_device(nft_ctx *c, double *p, int impfl, double eps, double rl20, const double *x0, int *nlc,
+                           int *converged, double *dvmx) {
+  const nft_params &P = c->P;
+  const int lord = P.lord, itmn = 3;
+  const double epa = 1.e-38;
+  int rc = 0;
+  (void)x0;
+  *c->h_flag = 0.0;
+  hipLaunchKernelGGL(k_gcr_ctl_begin, dim3(1), dim3(64), 0, c->stream, c->scal, rl20);
+  HIPCHK(hipGetLastError());
+  c->skip = c->scal + SC_DONE;
+  hipEvent_t ev[2] = {c->poll_ev[0], c->poll_ev[1]};
+  // timer events in use before each enqueued iteration, counted from the loop's start
+  std::vector<long> ev_mark;
+  long ev_off = 0;   // events already folded into the totals
+  int enq = 0;
+  bool stop = false;
+  for (int it = 1; it <= P.itr && !stop && !rc; ++it) {
+    for (int i = 1; i <= lord && !stop; ++i) {
+      ev_mark.push_back(ev_off + c->tev_used);
+      FinalOp op{};
+      op.kind = FK_UPDATE; op.K = 2; op.ismax = 1u;
+      op.ctl = 1; op.it = it; op.itmn = itmn; op.eps = eps; op.epa = epa; op.hflag = c->d_hflag;
+      if ((rc = gcr_iteration_update(c, p, i, op))) break;
+      if ((rc = gcr_iteration_rest(c, i, impfl, epa))) break;
+      if (hipEventRecord(ev[enq & 1], c->stream) != hipSuccess) { rc = set_err(-2, "hipEventRecord"); break; }
+      ++enq;
+      if (enq > 1) {
+        if (hipEventSynchronize(ev[(enq - 2) & 1]) != hipSuccess) { rc = set_err(-2, "hipEventSynchronize"); break; }
+        if (*(volatile double *)c->h_flag != 0.0) {
+          stop = true;
+        } else if (c->timer_kernel && c->tev_used > (int)c->tev.size() / 2) {
+          // iterations before enq-1 all passed and have finished: their timings are final
+          long upto = ev_mark[enq - 1] - ev_off;
+          t_harvest_prefix(c, (int)upto);
+          ev_off += upto;
+        }
+      }
+    }
+  }
+  c->skip = nullptr;
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(c->h_scal + SC_DVMX, c->scal + SC_DVMX, sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(c->h_scal + SC_DONE, c->scal + SC_DONE, 3 * sizeof(double), hipMemcpyDeviceToHost));
+  *nlc = (int)c->h_scal[SC_NLC];
+  *converged = c->h_scal[SC_DONE] == 1.0;
+  *dvmx = c->h_scal[SC_DVMX];
+  // launches past the exit did no work: drop their timer events before they are read
+  if (*nlc < enq) {
+    long keep = ev_mark[*nlc] - ev_off;
+    if (c->timer_kernel == KT_UPD) keep += 2;   // the update of the deciding iteration ran
+    if (keep >= 0 && keep < c->tev_used) c->tev_used = (int)keep;
+  }
+  t_harvest(c);
+  return 0;
+}
+
 // gcrk (reference :1776-1972) on device arrays. Returns 0 when converged, 1 otherwise.
 int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const double *w1, double *u2,
              double *v2, double *w2, int impfl) {
@@ -741,11 +886,16 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
   const double *qr_last = x0;
   int nlc = 0, converged = 0;
   double dvmx = 0;
+  if (!c->net.active && !c->net.dumping) {
+    if ((rc = gcr_loop_device(c, p, impfl, eps, rl20, x0, &nlc, &converged, &dvmx))) return rc;
+    if (nlc > 0) qr_last = c->qr;
+    goto done;
+  }
   for (int it = 1; it <= P.itr; ++it) {
     for (int i = 1; i <= lord; ++i) {
       t_begin(c, KT_UPD);
       hipLaunchKernelGGL(k_gcr_update, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, p, c->r,
-                         c->gx[i - 1], c->gax[i - 1], c->scal, c->partials, c->counter);
+                         c->gx[i - 1], c->gax[i - 1], c->scal, c->partials, (const double *)nullptr);
       t_end(c, KT_UPD);
       {
         FinalOp op{};
@@ -1038,6 +1188,53 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
     put(i1[e], i2[e], (int32_t)e);
     put(i2[e], i1[e], ~(int32_t)e);
   }
+  // columnar form: template of level 1 per column, checked against every interior level
+  std::vector<int32_t> tptr, tnb, tco, xptr, xnb, xco;
+  int colL = 0;
+  if (m->ncol > 0 && n % m->ncol == 0 && n / m->ncol >= 3) {
+    const int Lc = n / m->ncol;
+    bool ok = true;
+    for (int k = 0; k <= m->ncol && ok; ++k) ok = m->col_start[k] == k * Lc;
+    auto sl = [&](int i, int q) { int ch = i / NFT_WAVE, r = i % NFT_WAVE; return chunk_ptr[ch] + (long)q * NFT_WAVE + r; };
+    tptr.assign(m->ncol + 1, 0);
+    xptr.assign(m->ncol + 1, 0);
+    for (int cc = 0; cc < m->ncol && ok; ++cc) {
+      const int i1n = cc * Lc + 1, w = deg[i1n];
+      for (int q = 0; q < w; ++q) {
+        long s1 = sl(i1n, q);
+        int e = eid[s1];
+        // edge code base: ((edge - k + 1) << 1) | (i2 role), edge - k + 1 >= 0
+        int ea = e >= 0 ? e : ~e;
+        if (ea >= (1 << 29)) { ok = false; break; }
+        tnb.push_back(nbr[s1] - 1);
+        tco.push_back((ea << 1) | (e < 0 ? 1 : 0));
+      }
+      tptr[cc + 1] = tptr[cc] + w;
+      for (int k = 1; k < Lc - 1 && ok; ++k) {
+        const int i = cc * Lc + k;
+        if (deg[i] != w) { ok = false; break; }
+        for (int q = 0; q < w; ++q) {
+          long sk = sl(i, q);
+          int b = tco[tptr[cc] + q];
+          int ea = (b >> 1) - 1 + k;
+          int e = (b & 1) ? ~ea : ea;
+          if (nbr[sk] != tnb[tptr[cc] + q] + k || eid[sk] != e) { ok = false; break; }
+        }
+      }
+      const int ib = cc * Lc, it = cc * Lc + Lc - 1;
+      const int xw = std::max(deg[ib], deg[it]);
+      for (int q = 0; q < xw; ++q)
+        for (int t = 0; t < 2; ++t) {
+          int i = t ? it : ib;
+          bool has = q < deg[i];
+          xnb.push_back(has ? nbr[sl(i, q)] : i);
+          xco.push_back(has ? eid[sl(i, q)] : 0);
+        }
+      xptr[cc + 1] = xptr[cc] + 2 * xw;
+    }
+    if (ok) colL = Lc;
+  }
+
   // boundary points grouped per node, in entry order
   std::vector<int32_t> bp_ptr(n + 1, 0);
   for (int b = 0; b < m->nboun; ++b) bp_ptr[m->ibpoin[b]]++;
@@ -1064,6 +1261,18 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
     return fail(rc);
   Incidence &g = c->inc;
   g.n = n; g.nchunk = nchunk; g.chunk_ptr = d_chunk; g.deg = d_deg; g.nbr = d_nbr; g.eid = d_eid;
+  g.col_L = 0;
+  if (colL > 0) {
+    int32_t *d_tptr, *d_tnb, *d_tco, *d_xptr, *d_xnb, *d_xco;
+    if ((rc = upload(c, &d_tptr, tptr.data(), tptr.size())) || (rc = upload(c, &d_tnb, tnb.data(), tnb.size())) ||
+        (rc = upload(c, &d_tco, tco.data(), tco.size())) || (rc = upload(c, &d_xptr, xptr.data(), xptr.size())) ||
+        (rc = upload(c, &d_xnb, xnb.data(), xnb.size())) || (rc = upload(c, &d_xco, xco.data(), xco.size())))
+      return fail(rc);
+    g.col_L = colL;
+    g.tptr = d_tptr; g.tnb = d_tnb; g.tco = d_tco; g.xptr = d_xptr; g.xnb = d_xnb; g.xco = d_xco;
+    c->col_on = 1;
+    c->col_bytes = 4.0 * (tptr.size() + tnb.size() + tco.size() + xptr.size() + xnb.size() + xco.size());
+  }
   g.bp_ptr = d_bp_ptr; g.bp_flag = d_bflag; g.bp_sbb = d_bsbb; g.bp_anp = d_banp; g.nbp = nbp;
   g.nblocks = (nchunk + NFT_WPB - 1) / NFT_WPB;
   if (g.nblocks >= 64) {
@@ -1166,6 +1375,14 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   if (hipMemset(c->counter, 0, 4 * sizeof(unsigned)) != hipSuccess) return fail(set_err(-2, "memset"));
   if (hipMemset(c->scal, 0, SC_TOTAL * sizeof(double)) != hipSuccess) return fail(set_err(-2, "memset"));
   if (hipHostMalloc(&c->h_scal, SC_TOTAL * sizeof(double)) != hipSuccess) return fail(set_err(-2, "host alloc"));
+  if (hipHostMalloc(&c->h_flag, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+    return fail(set_err(-2, "host alloc"));
+  *c->h_flag = 0.0;
+  if (hipHostGetDevicePointer((void **)&c->d_hflag, c->h_flag, 0) != hipSuccess)
+    return fail(set_err(-2, "hipHostGetDevicePointer"));
+  if (hipEventCreateWithFlags(&c->poll_ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->poll_ev[1], hipEventDisableTiming) != hipSuccess)
+    return fail(set_err(-2, "event"));
   for (int k = 0; k < 4; ++k)
     if (hipMemset(c->frc[k], 0, n * sizeof(double)) != hipSuccess) return fail(set_err(-2, "memset"));
   if (hipDeviceSynchronize() != hipSuccess) return fail(set_err(-2, "sync"));
@@ -1178,6 +1395,9 @@ int nft_finalize(nft_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   for (void *p : c->allocs) hipFree(p);
   if (c->h_scal) hipHostFree(c->h_scal);
+  if (c->h_flag) hipHostFree(c->h_flag);
+  for (auto &e : c->poll_ev)
+    if (e) (void)hipEventDestroy(e);
   for (auto &e : c->ev)
     if (e) hipEventDestroy(e);
   for (auto &e : c->tev)
@@ -1425,13 +1645,16 @@ int nft_kernel_timer(nft_ctx *c, int kernel) {
 // algorithmic bytes of one launch: each array read or written counted once at its width
 static double kernel_bytes(nft_ctx *c, int k) {
   const double N = c->n, E = c->etot, NB = c->inc.nbp;
-  const double geom = 2 * E * 8 + 24 * E + 4 * (c->inc.nchunk + 1) + 4 * N + 4 * (N + 1) + NB * (4 + 48);
+  const double geom0 = 24 * E + 4 * (c->inc.nchunk + 1) + 4 * N + 4 * (N + 1) + NB * (4 + 48);
+  const double geom = geom0 + 2 * E * 8;
+  // gradient / divergence sweeps read the columnar tables instead of nbr/eid when in use
+  const double geomg = c->col_on ? geom0 + c->col_bytes : geom;
   const double geomf = 2 * E * 8 + 12 * E + 4 * (c->inc.nchunk + 1) + 4 * N + 4 * (N + 1) + NB * (4 + 48);
   const bool mixed = c->P.precision == 1;
   const double ts = mixed ? 4 : 8;
   switch (k) {
-    case KT_LGRAD: return geom + 8 * N /*p*/ + 4 * 8 * N /*vol rh0 coefw ue*/ + 3 * 8 * N /*q*/;
-    case KT_LDIV: return geom + 3 * 8 * N /*q*/ + 2 * 8 * N /*vol rh0*/ + 8 * N /*out*/;
+    case KT_LGRAD: return geomg + 8 * N /*p*/ + 4 * 8 * N /*vol rh0 coefw ue*/ + 3 * 8 * N /*q*/;
+    case KT_LDIV: return geomg + 3 * 8 * N /*q*/ + 2 * 8 * N /*vol rh0*/ + 8 * N /*out*/;
     case KT_LINE: return c->line_L > 0 ? 3 * ts * N /*rows*/ + 8 * N /*rhs*/ + 8 * N /*p*/
                                        : 4 * (c->ncol + 1) + 3 * ts * N /*factors*/ + 8 * N + 8 * N + 2 * ts * N /*z*/;
     case KT_UPD: return 6 * 8 * N;  /* p,r read+write; x, ax read */
@@ -1738,7 +1961,8 @@ int nft_copy_calibration(nft_ctx *c, int reps, double *ms) {
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, c->stream));
   for (int r = 0; r < reps; ++r)
-    hipLaunchKernelGGL(k_copy, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, c->io[0], c->io[1]);
+    hipLaunchKernelGGL(k_copy, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, c->io[0], c->io[1],
+                       (const double *)nullptr);
   HIPCHK(hipEventRecord(e1, c->stream));
   HIPCHK(hipEventSynchronize(e1));
   float t = 0;
@@ -1760,6 +1984,16 @@ int nft_net_options(nft_ctx *c, int objective, int linear) {
   if (linear)
     for (int l = 0; l < 6; ++l) HIPCHK(hipMemsetAsync(n.w[2 * l + 1], 0, n.N[l] * sizeof(float), c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// gradient / divergence gathers: 1 columnar incidence (when the mesh has one), 0 SELL-64
+int nft_set_incidence(nft_ctx *c, int columnar, int *available) {
+  if (!c) return set_err(-1, "nft_set_incidence: null context");
+  if (available) *available = c->inc.col_L > 0;
+  if (columnar < 0) return 0;
+  if (columnar && c->inc.col_L == 0) return set_err(-1, "nft_set_incidence: mesh has no columnar form");
+  c->col_on = columnar ? 1 : 0;
   return 0;
 }
 

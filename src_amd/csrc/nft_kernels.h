@@ -45,7 +45,43 @@ struct Incidence {
   int nbp;
   int xcd_per;               // logical blocks per XCD slice (remap), 0 = identity
   int nblocks;               // logical blocks
+  // Columnar form of the same slots (extruded meshes, node i = c*col_L + k): the interior
+  // levels 0 < k < col_L-1 of column c share one slot template t in [tptr[c], tptr[c+1]):
+  // neighbour tnb[t]+k, edge (tco[t]>>1) - 1 + k in the i2 role when tco[t]&1; the bottom and top nodes keep
+  // explicit slots xnb/xco at xptr[c] + 2*m + (0 bottom, 1 top). Built only when it
+  // reproduces nbr/eid exactly; col_L = 0 otherwise.
+  int col_L;
+  const int32_t *tptr, *tnb, *tco, *xptr, *xnb, *xco;
 };
+
+// Per-lane view of the columnar form.
+struct ColLane {
+  int k, t0, x0, xt;
+  bool inner;
+};
+
+__device__ __forceinline__ void col_setup(const Incidence &g, int i, ColLane &C) {
+  int c = i / g.col_L;
+  C.k = i - c * g.col_L;
+  C.inner = C.k > 0 && C.k < g.col_L - 1;
+  C.t0 = g.tptr[c];
+  C.x0 = g.xptr[c];
+  C.xt = C.k == 0 ? 0 : 1;
+}
+
+// neighbour and signed edge code of slot m (valid lanes with m < deg only)
+__device__ __forceinline__ void col_slot(const Incidence &g, const ColLane &C, int m, int &j, int &e) {
+  if (C.inner) {
+    j = g.tnb[C.t0 + m] + C.k;
+    int b = g.tco[C.t0 + m];
+    int ea = (b >> 1) - 1 + C.k;
+    e = (b & 1) ? ~ea : ea;
+  } else {
+    int s = C.x0 + 2 * m + C.xt;
+    j = g.xnb[s];
+    e = g.xco[s];
+  }
+}
 
 template <typename T>
 struct EdgeS {
@@ -81,6 +117,20 @@ __device__ __forceinline__ bool lane_setup(const Incidence &g, Lane &L) {
   L.valid = L.i < g.n;
   L.deg = L.valid ? g.deg[L.i] : 0;
   return true;
+}
+
+// slot m of this lane (SELL-64 or columnar); j = e = 0 when !ok
+template <bool COL>
+__device__ __forceinline__ void slot_at(const Incidence &g, const Lane &L, const ColLane &C, int m, bool ok, int &j,
+                                        int &e) {
+  if (COL) {
+    if (ok) col_slot(g, C, m, j, e);
+    else { j = 0; e = 0; }
+  } else {
+    int slot = L.base + m * NFT_WAVE + L.lane;
+    j = ok ? g.nbr[slot] : 0;
+    e = ok ? g.eid[slot] : 0;
+  }
 }
 
 // ------------------------------------------------------------------------------------
@@ -170,6 +220,11 @@ struct NodeCoef {
 __device__ __forceinline__ double rho(const NodeCoef &c, int i) { return c.rh0_one ? 1.0 : c.rh0[i]; }
 __device__ __forceinline__ double cw(const NodeCoef &c, int i) { return c.coefw_uniform ? c.coefw0 : c.coefw[i]; }
 
+// Kernels of the GCR loop take the device exit flag (scal + SC_DONE, or null outside the
+// loop) and return at once when the solve has ended: iterations enqueued ahead of the
+// host's check then do no work. The flag is uniform, so whole blocks leave together.
+__device__ __forceinline__ bool skipped(const double *flag) { return flag && *flag != 0.0; }
+
 // Slots are consumed NFT_BATCH at a time: the batch's indices, then all its gathers, are
 // issued before any use, so a wavefront keeps several independent loads in flight; the
 // sums still run slot by slot in the reference's edge order.
@@ -192,25 +247,24 @@ struct GradArgs {
   NodeCoef nc;
   double dt;
   int impfl;
+  const double *skip;
 };
 
-template <int MODE>
+template <int MODE, bool COL>
 __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S, GradArgs a) {
+  if (skipped(a.skip)) return;
   Lane L;
   if (!lane_setup(g, L)) return;
   const int i = L.i;
+  ColLane C{};
+  if (COL && L.valid) col_setup(g, i, C);
   double gx = 0., gy = 0., gz = 0.;
   if (MODE != GRAD_ZERO_PRFR) {
     double fi = L.valid ? a.f[i] : 0.;
     for (int m0 = 0; m0 < L.width; m0 += NFT_BATCH) {
       int jj[NFT_BATCH], ee[NFT_BATCH];
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH; ++u) {
-        int slot = L.base + (m0 + u) * NFT_WAVE + L.lane;
-        bool ok = m0 + u < L.deg;
-        jj[u] = ok ? g.nbr[slot] : 0;
-        ee[u] = ok ? g.eid[slot] : 0;
-      }
+      for (int u = 0; u < NFT_BATCH; ++u) slot_at<COL>(g, L, C, m0 + u, m0 + u < L.deg, jj[u], ee[u]);
       double fj[NFT_BATCH], sx[NFT_BATCH], sy[NFT_BATCH], sz[NFT_BATCH];
 #pragma unroll
       for (int u = 0; u < NFT_BATCH; ++u) {
@@ -291,6 +345,9 @@ enum Finalize { FIN_NONE = 0, FIN_BETA = 1, FIN_DEL = 2, FIN_NORMS = 3 };
 enum Scal {
   SC_BETA = 0, SC_RAX = 1, SC_DVMX = 2, SC_RL2 = 3, SC_BMX = 4, SC_EER = 5, SC_EEM = 6,
   SC_R2 = 7, SC_RMX = 8, SC_TMP = 9,
+  SC_DONE = 10,  // GCR exit decided on the device: 0 running, 1 converged, 2 stagnated
+  SC_NLC = 11,   // inner iterations that passed the exit test
+  SC_RL20 = 12,  // previous sum r^2 (stagnation test)
   SC_AX2 = 16,   // [16, 16+NFT_MAXD)
   SC_DEL = 40,   // [40, 40+NFT_MAXD)
   SC_TOTAL = 64
@@ -306,6 +363,7 @@ struct DivArgs {
   unsigned *counter;
   double *scal;
   double epa;
+  const double *skip;
 };
 
 __device__ __forceinline__ void gcr_finalize(const DotList &d, const double *red, double *scal,
@@ -325,10 +383,11 @@ __device__ __forceinline__ void gcr_finalize(const DotList &d, const double *red
   }
 }
 
-template <int MODE, int ND, int NMAX>
+template <int MODE, int ND, int NMAX, bool COL>
 __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S, DivArgs a) {
   constexpr int K = ND + NMAX;
   __shared__ double sh[(K > 0 ? K : 1) * NFT_WPB];
+  if (skipped(a.skip)) return;
   Lane L;
   bool active = lane_setup(g, L);
   if (K == 0 && !active) return;
@@ -336,6 +395,8 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
   double out = 0.;
   if (active) {
     const int i = L.i;
+    ColLane C{};
+    if (COL && L.valid) col_setup(g, i, C);
     const bool scale = MODE == DIV_DIV && a.iflg == -1 && !a.nc.rh0_one;
     double ax = 0., ay = 0., az = 0.;
     double qi0 = 0., qi1 = 0., qi2 = 0.;
@@ -346,12 +407,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
     for (int m0 = 0; m0 < L.width; m0 += NFT_BATCH) {
       int jj[NFT_BATCH], ee[NFT_BATCH];
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH; ++u) {
-        int slot = L.base + (m0 + u) * NFT_WAVE + L.lane;
-        bool ok = m0 + u < L.deg;
-        jj[u] = ok ? g.nbr[slot] : 0;
-        ee[u] = ok ? g.eid[slot] : 0;
-      }
+      for (int u = 0; u < NFT_BATCH; ++u) slot_at<COL>(g, L, C, m0 + u, m0 + u < L.deg, jj[u], ee[u]);
       double q0[NFT_BATCH], q1[NFT_BATCH], q2[NFT_BATCH], sx[NFT_BATCH], sy[NFT_BATCH], sz[NFT_BATCH];
 #pragma unroll
       for (int u = 0; u < NFT_BATCH; ++u) {
@@ -476,8 +532,9 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_laplT_2(Incidence g, EdgeS<double
 // p += beta*x; r += beta*ax; reduce max|r| and sum r^2   (reference :1886-1893)
 __global__ void __launch_bounds__(NFT_BLOCK) k_gcr_update(int n, double *p, double *r, const double *x,
                                                           const double *ax, double *scal,
-                                                          double *partials, unsigned *counter) {
+                                                          double *partials, const double *skip) {
   __shared__ double sh[2 * NFT_WPB];
+  if (skipped(skip)) return;
   const double beta = scal[SC_BETA];
   double v[2] = {-1.e15, 0.};
   const int stride = gridDim.x * NFT_BLOCK;
@@ -513,12 +570,14 @@ struct DirArgs {
   unsigned *counter;
   int slot;                  // slot index of xn
   double epa;
+  const double *skip;
 };
 
 // x_new = qr + sum del_ii x_ii ; ax_new = ar + sum del_ii ax_ii   (reference :1913-1936)
 // then <r, ax_new>, <ax_new, ax_new> -> beta for the next inner iteration
 __global__ void __launch_bounds__(NFT_BLOCK) k_gcr_dir(DirArgs a) {
   __shared__ double sh[2 * NFT_WPB];
+  if (skipped(a.skip)) return;
   double del[NFT_MAXD];
 #pragma unroll
   for (int ii = 0; ii < NFT_MAXD; ++ii) del[ii] = ii < a.nx ? a.scal[SC_DEL + ii] : 0.;
@@ -555,9 +614,9 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_norms(int n, const double *a, dou
   block_partial<2>(v, 2u, partials, sh);
 }
 
-// Grid-level end of every fused reduction: one block sums the per-block partials in a
-// fixed order (thread t: blocks t, t+256, ... ; then the block tree) and applies the
-// solver's scalar update.
+// Grid-level end of every fused reduction: one block of 1024 threads sums the per-block
+// partials in a fixed order (thread t: blocks t, t+1024, ...; wavefront trees; the 16
+// wavefront results in order) and applies the solver's scalar update.
 enum FinalKind { FK_DIV = 0, FK_UPDATE = 1, FK_DIR = 2, FK_NORMS = 3 };
 
 struct FinalOp {
@@ -568,41 +627,79 @@ struct FinalOp {
   int slot;         // FK_DIR: slot of the new direction
   int sum_slot, max_slot;  // FK_NORMS
   double epa;
+  const double *skip;
+  // FK_UPDATE with ctl: the exit test of the reference's inner loop (:1893-1895), decided
+  // here: converged when max|r| <= eps at outer iteration it >= itmn, stagnated when
+  // sum r^2 >= its previous value; otherwise the previous value becomes max(sum r^2, epa)
+  // and the pass is counted. The decision is also stored to *hflag (mapped host memory).
+  int ctl, it, itmn;
+  double eps;
+  double *hflag;
 };
 
-__global__ void __launch_bounds__(NFT_BLOCK) k_reduce_final(const double *partials, int nblk, FinalOp op,
-                                                            double *scal) {
-  __shared__ double sh[(NFT_MAXD + 1) * NFT_WPB];
-  double acc[NFT_MAXD + 1];
+#define NFT_FIN_THREADS 1024
+template <int K>
+__global__ void __launch_bounds__(NFT_FIN_THREADS) k_reduce_final(const double *partials, int nblk, FinalOp op,
+                                                                  double *scal) {
+  constexpr int NW = NFT_FIN_THREADS / 64;
+  constexpr int U = K <= 4 ? 8 : (K <= 8 ? 4 : (K <= 16 ? 2 : 1));
+  __shared__ double sh[K * NW];
+  if (skipped(op.skip)) return;
+  double acc[K];
 #pragma unroll
-  for (int k = 0; k <= NFT_MAXD; ++k) acc[k] = ((op.ismax >> k) & 1) ? -1.0e300 : 0.0;
-  // eight blocks' partials are loaded before they are added (in block order)
-  constexpr int U = 8;
-  for (int b0 = threadIdx.x; b0 < nblk; b0 += U * NFT_BLOCK) {
-    for (int k = 0; k < op.K; ++k) {
-      double pv[U];
+  for (int k = 0; k < K; ++k) acc[k] = ((op.ismax >> k) & 1) ? -1.0e300 : 0.0;
+  // thread t: partials of blocks t, t+1024, ... in that order; U blocks' K values are
+  // loaded before they are added
+  for (int b0 = threadIdx.x; b0 < nblk; b0 += U * NFT_FIN_THREADS) {
+    double pv[U][K];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        int b = b0 + u * NFT_BLOCK;
-        pv[u] = b < nblk ? partials[(size_t)b * op.K + k] : 0.0;
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u * NFT_FIN_THREADS;
+#pragma unroll
+      for (int k = 0; k < K; ++k) pv[u][k] = b < nblk ? partials[(size_t)b * K + k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (b0 + u * NFT_FIN_THREADS < nblk) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] = ((op.ismax >> k) & 1) ? fmax(acc[k], pv[u][k]) : acc[k] + pv[u][k];
       }
-      double a = acc[0];
-      for (int kk = 0; kk <= NFT_MAXD; ++kk) if (kk == k) a = acc[kk];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (b0 + u * NFT_BLOCK < nblk) a = ((op.ismax >> k) & 1) ? fmax(a, pv[u]) : a + pv[u];
-      }
-#pragma unroll
-      for (int kk = 0; kk <= NFT_MAXD; ++kk) if (kk == k) acc[kk] = a;
     }
   }
-  block_reduce<NFT_MAXD + 1>(acc, op.ismax, sh);
+  // wavefront trees, then the 16 wavefront results in order
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = ((op.ismax >> k) & 1) ? wave_max(acc[k]) : wave_sum(acc[k]);
+  if (l == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[k * NW + w] = acc[k];
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double a = sh[k * NW];
+    for (int j = 1; j < NW; ++j) a = ((op.ismax >> k) & 1) ? fmax(a, sh[k * NW + j]) : a + sh[k * NW + j];
+    acc[k] = a;
+  }
   if (op.kind == FK_DIV) {
     gcr_finalize(op.dots, acc, scal, op.epa);
   } else if (op.kind == FK_UPDATE) {
     scal[SC_DVMX] = acc[0];
     scal[SC_RL2] = acc[1];
+    if (op.ctl) {
+      double done = 0.0;
+      if (acc[0] <= op.eps && op.it >= op.itmn) done = 1.0;
+      else if (acc[1] >= scal[SC_RL20]) done = 2.0;
+      else {
+        scal[SC_RL20] = fmax(acc[1], op.epa);
+        scal[SC_NLC] = scal[SC_NLC] + 1.0;
+      }
+      if (done != 0.0) {
+        scal[SC_DONE] = done;
+        if (op.hflag) *op.hflag = done;
+      }
+    }
   } else if (op.kind == FK_DIR) {
     double ax2 = fmax(op.epa, acc[1]);
     scal[SC_AX2 + op.slot] = ax2;
@@ -623,7 +720,8 @@ template <typename T>
 __global__ void __launch_bounds__(NFT_BLOCK) k_line(int ncol, const int32_t *col_start, const T *la,
                                                     const T *linv, const T *lc, const double *res,
                                                     const double *pin, double *p, double omega,
-                                                    T *zbuf) {
+                                                    T *zbuf, const double *skip) {
+  if (skipped(skip)) return;
   int c = blockIdx.x * NFT_BLOCK + threadIdx.x;
   if (c >= ncol) return;
   int s0 = col_start[c], s1 = col_start[c + 1];
@@ -652,7 +750,8 @@ __device__ __forceinline__ T lane_get(T v, int src) { return __shfl(v, src, 64);
 template <typename T>
 __global__ void __launch_bounds__(NFT_BLOCK) k_line_pcr(int ncol, int L, const T *lo, const T *dg,
                                                         const T *up, const double *res, const double *pin,
-                                                        double *p, double omega) {
+                                                        double *p, double omega, const double *skip) {
+  if (skipped(skip)) return;
   const int lane = threadIdx.x & 63;
   const int wave = (blockIdx.x * NFT_BLOCK + threadIdx.x) >> 6;
   const int cpw = 64 / L;
@@ -686,7 +785,8 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_line_pcr(int ncol, int L, const T
 // point relaxation: p = p + omega*pstep*(r - rhs)   (reference :607-612)
 __global__ void __launch_bounds__(NFT_BLOCK) k_point(int n, const double *pstep, const double *r,
                                                      const double *rhs, double *p, double omega,
-                                                     int first) {
+                                                     int first, const double *skip) {
+  if (skipped(skip)) return;
   int k = blockIdx.x * NFT_BLOCK + threadIdx.x;
   if (k >= n) return;
   double pk = first ? 0. : p[k];
@@ -694,14 +794,25 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_point(int n, const double *pstep,
   p[k] = pk + omega * pstep[k] * (rk - rhs[k]);
 }
 
-__global__ void k_sub(int n, const double *a, const double *b, double *c) {
+__global__ void k_sub(int n, const double *a, const double *b, double *c, const double *skip) {
+  if (skipped(skip)) return;
   int k = blockIdx.x * NFT_BLOCK + threadIdx.x;
   if (k < n) c[k] = a[k] - b[k];
 }
 
-__global__ void k_copy(int n, const double *a, double *b) {
+__global__ void k_copy(int n, const double *a, double *b, const double *skip) {
+  if (skipped(skip)) return;
   int k = blockIdx.x * NFT_BLOCK + threadIdx.x;
   if (k < n) b[k] = a[k];
+}
+
+// start of a device-controlled GCR loop: flag cleared, pass count 0, previous sum r^2
+__global__ void k_gcr_ctl_begin(double *scal, double rl20) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    scal[SC_DONE] = 0.0;
+    scal[SC_NLC] = 0.0;
+    scal[SC_RL20] = rl20;
+  }
 }
 
 __global__ void k_fill(int n, double *a, double v) {

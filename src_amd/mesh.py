@@ -36,7 +36,7 @@ def _meshlib():
         ip = ctypes.POINTER(ctypes.c_int32)
         lib.nft_meshgen_build.restype = ctypes.c_void_p
         lib.nft_meshgen_build.argtypes = [ctypes.c_int, dp, ctypes.c_int, ip, dp, ctypes.c_int,
-                                          dp, ctypes.c_double, dp, ctypes.c_char_p, ctypes.c_int]
+                                          dp, ctypes.c_double, dp, ip, ctypes.c_char_p, ctypes.c_int]
         lib.nft_meshgen_sizes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
         lib.nft_meshgen_export.argtypes = [ctypes.c_void_p, dp, dp, ip, dp, ip, dp, ip, dp, dp]
         lib.nft_meshgen_write_text.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
@@ -70,6 +70,10 @@ class MeshSpec:
     hill_a: float = 2000.0
     dz_bottom: float = 0.0   # > 0 with dz_top: power-law stretched levels
     dz_top: float = 0.0
+    # column numbering: "grid" (row by row) or "hilbert" (along a Hilbert curve through the
+    # grid cells, so columns that are neighbours in the plane get nearby ids and the
+    # metrics of their shared edges are read close together in time)
+    column_order: str = "grid"
 
 
 @dataclasses.dataclass
@@ -163,6 +167,39 @@ def jittered_points(spec: MeshSpec) -> np.ndarray:
     return xy
 
 
+def hilbert_index(ix: np.ndarray, iy: np.ndarray, order: int) -> np.ndarray:
+    """Distance along the Hilbert curve of side 2**order of the cells (ix, iy) (the
+    classic rotate-and-flip walk, vectorised)."""
+    x = ix.astype(np.int64).copy()
+    y = iy.astype(np.int64).copy()
+    d = np.zeros_like(x)
+    s = 1 << (order - 1)
+    while s > 0:
+        rx = ((x & s) > 0).astype(np.int64)
+        ry = ((y & s) > 0).astype(np.int64)
+        d += s * s * ((3 * rx) ^ ry)
+        # rotate the quadrant so the sub-curve has the canonical orientation
+        flip = (ry == 0) & (rx == 1)
+        x = np.where(flip, s - 1 - x, x)
+        y = np.where(flip, s - 1 - y, y)
+        swap = ry == 0
+        x, y = np.where(swap, y, x), np.where(swap, x, y)
+        s >>= 1
+    return d
+
+
+def column_permutation(spec: MeshSpec) -> Optional[np.ndarray]:
+    """Point permutation for spec.column_order (None for the grid order)."""
+    if spec.column_order == "grid":
+        return None
+    if spec.column_order != "hilbert":
+        raise ValueError(f"unknown column order {spec.column_order!r}")
+    c = np.arange(spec.nx * spec.ny)
+    ix, iy = c % spec.nx, c // spec.nx
+    order = max(1, math.ceil(math.log2(max(spec.nx, spec.ny))))
+    return np.argsort(hilbert_index(ix, iy, order), kind="stable")
+
+
 def delaunay(xy: np.ndarray) -> np.ndarray:
     """2-D Delaunay triangles [3, nt] (0-based column ids) of the point set."""
     from scipy.spatial import Delaunay
@@ -187,13 +224,21 @@ def terrain(spec: MeshSpec, xy: np.ndarray) -> np.ndarray:
 def generate(spec: MeshSpec) -> Mesh:
     lib = _meshlib()
     xy = jittered_points(spec)
+    perm = column_permutation(spec)
+    rank = None
+    if perm is not None:
+        # renumbered columns keep the grid order's prism split (ranks = grid ids), so the
+        # tetrahedra are those of the grid-ordered mesh
+        xy = np.ascontiguousarray(xy[:, perm])
+        rank = np.ascontiguousarray(perm.astype(np.int32))
     tri = delaunay(xy)
     zs = np.ascontiguousarray(terrain(spec, xy))
     zeta = np.ascontiguousarray(zeta_levels(spec.nlev, spec.htop, spec.dz_bottom, spec.dz_top))
     box = np.array([0.0, spec.lx, 0.0, spec.ly])
     err = ctypes.create_string_buffer(256)
     h = lib.nft_meshgen_build(xy.shape[1], _dp(xy), tri.shape[1], _ip(tri), _dp(zs), spec.nlev,
-                              _dp(zeta), spec.htop, _dp(box), err, 256)
+                              _dp(zeta), spec.htop, _dp(box), _ip(rank) if rank is not None else None,
+                              err, 256)
     if not h:
         raise RuntimeError(err.value.decode())
     sz = (ctypes.c_int64 * 7)()
@@ -252,18 +297,19 @@ def small_spec():
                     hill_h0=500.0, hill_a=2000.0)
 
 
-def production_spec():
+def production_spec(column_order="grid"):
     """65,536 jittered points (256 x 256) over 200 km x 200 km, 64 uniform levels over
-    20 km, hill h0 = 1 km, half-width 2 km (4,194,304 nodes)."""
+    20 km, hill h0 = 1 km, half-width 2 km (4,194,304 nodes); columns row by row unless
+    column_order="hilbert" (same mesh, renumbered)."""
     return MeshSpec(nx=256, ny=256, nlev=64, lx=200000.0, ly=200000.0, htop=20000.0,
-                    hill_h0=1000.0, hill_a=2000.0)
+                    hill_h0=1000.0, hill_a=2000.0, column_order=column_order)
 
 
-def stiff_spec():
+def stiff_spec(column_order="grid"):
     """131,072 jittered points (512 x 256) over 300 km x 300 km, 128 levels over 25 km
     stretched from 50 m to 400 m, hill h0 = 2 km (16,777,216 nodes)."""
     return MeshSpec(nx=512, ny=256, nlev=128, lx=300000.0, ly=300000.0, htop=25000.0,
-                    hill_h0=2000.0, hill_a=2000.0, dz_bottom=50.0, dz_top=400.0)
+                    hill_h0=2000.0, hill_a=2000.0, dz_bottom=50.0, dz_top=400.0, column_order=column_order)
 
 
 CONFIGS = {

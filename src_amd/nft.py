@@ -100,6 +100,7 @@ _SIGS.update({
     "nft_set_learned": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_int)]),
     "nft_net_options": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "nft_set_incidence": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "nft_copy_calibration": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
 })
 
@@ -376,6 +377,12 @@ class Solver:
         ms = ctypes.c_double()
         _chk(lib().nft_copy_calibration(self.h, reps, ctypes.byref(ms)), "nft_copy_calibration")
         return ms.value
+
+    def set_incidence(self, columnar):
+        """1: columnar gathers (when available), 0: SELL-64, -1: no change. Returns availability."""
+        av = ctypes.c_int()
+        _chk(lib().nft_set_incidence(self.h, columnar, ctypes.byref(av)), "nft_set_incidence")
+        return bool(av.value)
 
     def gcr_stats(self):
         st = GcrStats()

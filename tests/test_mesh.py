@@ -113,3 +113,63 @@ def test_stretched_levels():
     assert abs(z[-1] - 25000.0) < 1e-6 and z[0] == 0.0
     assert np.all(dz > 0) and np.all(np.diff(dz) > -1e-9)
     assert 45 < dz[0] < 55 and 350 < dz[-1] < 450
+
+
+def test_hilbert_curve_is_a_unit_step_bijection():
+    o = 4
+    n = 1 << o
+    ix, iy = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    ix, iy = ix.ravel(), iy.ravel()
+    d = M.hilbert_index(ix, iy, o)
+    assert sorted(d.tolist()) == list(range(n * n))
+    p = np.argsort(d)
+    assert (np.abs(np.diff(ix[p])) + np.abs(np.diff(iy[p]))).max() == 1
+
+
+def test_hilbert_numbering_is_the_same_mesh():
+    """Columns renumbered along the Hilbert curve (prism split ranked by the grid ids):
+    after mapping node ids back, the edge set, the interior/boundary split, the dual
+    metrics and the volumes are those of the grid-numbered mesh (metrics to rounding,
+    since the tets are visited in another order)."""
+    import dataclasses
+    g = M.generate(M.small_spec())
+    sp = dataclasses.replace(M.small_spec(), column_order="hilbert")
+    h = M.generate(sp)
+    perm = M.column_permutation(sp)
+    L = sp.nlev
+    hmap = (perm[:, None] * L + np.arange(L)[None, :]).ravel()
+
+    def edges(m, nmap=None):
+        i1 = np.concatenate([m.iedge[0], m.kbface[0]]) - 1
+        i2 = np.concatenate([m.iedge[1], m.kbface[1]]) - 1
+        S = np.concatenate([m.sn, m.sb], axis=1)
+        if nmap is not None:
+            i1, i2 = nmap[i1], nmap[i2]
+        sw = i1 > i2
+        a, b = np.where(sw, i2, i1), np.where(sw, i1, i2)
+        S = np.where(sw[None, :], -S, S)
+        key = a.astype(np.int64) * m.npoin + b
+        o = np.argsort(key)
+        bnd = np.concatenate([np.zeros(m.nedge, bool), np.ones(m.nbface, bool)])
+        return key[o], S[:, o], bnd[o]
+
+    kg, Sg, bg = edges(g)
+    kh, Sh, bh = edges(h, hmap)
+    assert np.array_equal(kg, kh) and np.array_equal(bg, bh)
+    assert np.abs(Sg - Sh).max() <= 1e-14 * np.abs(Sg).max()
+    vh = np.zeros(g.npoin)
+    vh[hmap] = h.vol
+    assert np.abs(vh - g.vol).max() <= 1e-14 * g.vol.max()
+
+    def bpoints(m, nmap=None):
+        ib = m.ibpoin[0] - 1
+        if nmap is not None:
+            ib = nmap[ib]
+        o = np.lexsort((m.ibpoin[1], ib))
+        return ib[o], m.ibpoin[1][o], m.sbb[:, o], m.anp[:, o]
+
+    bg_, fg, sg, ag = bpoints(g)
+    bh_, fh, sh, ah = bpoints(h, hmap)
+    assert np.array_equal(bg_, bh_) and np.array_equal(fg, fh)
+    assert np.abs(sg - sh).max() <= 1e-14 * np.abs(sg).max()
+    assert np.abs(ag - ah).max() <= 1e-14
