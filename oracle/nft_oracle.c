@@ -23,7 +23,7 @@ struct oc_ctx {
   /* line preconditioner: Thomas factors and operator rows */
   double *la, *linv, *lc;
   double *llo, *ldg, *lup;
-  int line_L;   /* uniform column length <= 64 (parallel cyclic reduction), else 0 (Thomas) */
+  int line_L;   /* uniform column length <= 64 (wavefront scans), else 0 (Thomas) */
   /* scratch */
   double *s[16];
   /* gcrk work */
@@ -469,42 +469,48 @@ static void build_line(oc_ctx *c) {
   c->line_L = uniform ? L0 : 0;
 }
 
-/* Parallel cyclic reduction of one column of L <= 64 rows padded with identity rows to
- * 64: steps of distance 1, 2, ..., 32, in the operation order of the HIP path's
- * wavefront solver (k_line_pcr). */
-static void pcr_column(const double *lo, const double *dg, const double *up, const double *rhs, int L,
-                       double *z) {
-  double a[64], b[64], cc[64], d[64], na[64], nb[64], nc[64], nd[64];
+/* One column of L <= 64 rows in a 64-lane frame (rows past L carry the zero map), solved
+ * with the fixed Thomas factors as two scans of affine maps, in the operation order of the
+ * HIP path's wavefront solver (k_line_scan):
+ *   forward  (beta, alpha) = (d*inv, -(a*inv)); for s = 1..32, lanes l >= s:
+ *            beta = beta + alpha*beta[l-s]; alpha = alpha*alpha[l-s]   (old values)
+ *   backward (beta, gamma) = (y, -c');         for s = 1..32, lanes l+s < 64:
+ *            beta = beta + gamma*beta[l+s]; gamma = gamma*gamma[l+s]
+ * A column's result does not depend on its lane offset in a wavefront: alpha = 0 at its
+ * first row and gamma = 0 at its last row turn every cross-column term into an exact 0. */
+static void scan_column(const double *la, const double *linv, const double *lc, const double *rhs, int L,
+                        double *z) {
+  double al[64], be[64], ga[64], na[64], nb[64];
   for (int l = 0; l < 64; ++l) {
-    if (l < L) { a[l] = lo[l]; b[l] = dg[l]; cc[l] = up[l]; d[l] = rhs[l]; }
-    else { a[l] = 0.; b[l] = 1.; cc[l] = 0.; d[l] = 0.; }
+    if (l < L) { al[l] = -(la[l] * linv[l]); be[l] = rhs[l] * linv[l]; ga[l] = -lc[l]; }
+    else { al[l] = 0.; be[l] = 0.; ga[l] = 0.; }
   }
   for (int s = 1; s < 64; s <<= 1) {
     for (int l = 0; l < 64; ++l) {
-      int im = l - s, ip = l + s;
-      double am = 0., bm = 1., cm = 0., dm = 0., ap = 0., bp = 1., cp = 0., dp = 0.;
-      if (im >= 0) { am = a[im]; bm = b[im]; cm = cc[im]; dm = d[im]; }
-      if (ip <= 63) { ap = a[ip]; bp = b[ip]; cp = cc[ip]; dp = d[ip]; }
-      double k1 = a[l] / bm, k2 = cc[l] / bp;
-      na[l] = -(am * k1);
-      nc[l] = -(cp * k2);
-      nb[l] = b[l] - cm * k1 - ap * k2;
-      nd[l] = d[l] - dm * k1 - dp * k2;
+      if (l >= s) { nb[l] = be[l] + al[l] * be[l - s]; na[l] = al[l] * al[l - s]; }
+      else { nb[l] = be[l]; na[l] = al[l]; }
     }
-    for (int l = 0; l < 64; ++l) { a[l] = na[l]; b[l] = nb[l]; cc[l] = nc[l]; d[l] = nd[l]; }
+    for (int l = 0; l < 64; ++l) { be[l] = nb[l]; al[l] = na[l]; }
   }
-  for (int l = 0; l < L; ++l) z[l] = d[l] / b[l];
+  for (int s = 1; s < 64; s <<= 1) {
+    for (int l = 0; l < 64; ++l) {
+      if (l + s < 64) { nb[l] = be[l] + ga[l] * be[l + s]; na[l] = ga[l] * ga[l + s]; }
+      else { nb[l] = be[l]; na[l] = ga[l]; }
+    }
+    for (int l = 0; l < 64; ++l) { be[l] = nb[l]; ga[l] = na[l]; }
+  }
+  for (int l = 0; l < L; ++l) z[l] = be[l];
 }
 
-/* z = M^-1 rhs, column by column: parallel cyclic reduction for uniform columns of at
- * most 64 levels, otherwise a forward then backward Thomas sweep */
+/* z = M^-1 rhs, column by column: wavefront scans for uniform columns of at most 64
+ * levels, otherwise a forward then backward Thomas sweep */
 void oc_line_solve(oc_ctx *c, const double *rhs, double *z) {
   const oc_mesh *m = &c->m;
   if (c->line_L > 0) {
     int L = c->line_L;
     for (int k = 0; k < m->ncol; ++k) {
       int s0 = k * L;
-      pcr_column(c->llo + s0, c->ldg + s0, c->lup + s0, rhs + s0, L, z + s0);
+      scan_column(c->la + s0, c->linv + s0, c->lc + s0, rhs + s0, L, z + s0);
     }
     return;
   }

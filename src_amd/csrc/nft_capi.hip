@@ -78,13 +78,11 @@ struct nft_ctx {
   int col_L = 0;              // uniform column length with nodes numbered column by column (0: not)
   int col_on = 0;             // gathers use the columnar incidence (inc.col_L > 0)
   double col_bytes = 0;       // bytes of the columnar tables
-  double *llo, *ldg, *lup;    // line operator rows (PCR)
-  float *llof, *ldgf, *lupf;
   double *zbuf;
   float *zbuff;
 
   // host copies of set-up fields
-  std::vector<double> h_alpha, h_d1, h_coefw, h_la, h_linv, h_lc, h_lo, h_dg, h_up;
+  std::vector<double> h_alpha, h_d1, h_coefw, h_la, h_linv, h_lc;
   std::vector<std::pair<int, int>> h_hedges;   // column pairs joined by same-level edges
   double beta1 = 0;
 
@@ -313,7 +311,6 @@ void setup_fields(nft_ctx *c, const nft_mesh *m, std::vector<double> &rh0, std::
         }
       }
     }
-    c->h_lo.assign(n, 0.); c->h_dg.assign(n, 0.); c->h_up.assign(n, 0.);
     for (int k = 0; k < m->ncol; ++k) {
       int s0 = m->col_start[k], s1 = m->col_start[k + 1];
       double cprev = 0.;
@@ -324,7 +321,6 @@ void setup_fields(nft_ctx *c, const nft_mesh *m, std::vector<double> &rh0, std::
         double cp = (i == s1 - 1) ? 0. : up[i] * inv;
         c->h_la[i] = a; c->h_linv[i] = inv; c->h_lc[i] = cp;
         cprev = cp;
-        c->h_lo[i] = a; c->h_dg[i] = diag[i]; c->h_up[i] = (i == s1 - 1) ? 0. : up[i];
       }
     }
     int L0 = m->col_start[1] - m->col_start[0];
@@ -531,11 +527,11 @@ int precon_dev(nft_ctx *c, const double *rhs, double *p, int impfl) {
         long waves = (c->ncol + cpw - 1) / cpw;
         int cg = (int)((waves + NFT_WPB - 1) / NFT_WPB);
         if (P.precision == 1)
-          hipLaunchKernelGGL(k_line_pcr<float>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->line_L,
-                             c->llof, c->ldgf, c->lupf, res, pin, p, P.omega, c->skip);
+          hipLaunchKernelGGL(k_line_scan<float>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->line_L,
+                             c->laf, c->linvf, c->lcf, res, pin, p, P.omega, c->skip);
         else
-          hipLaunchKernelGGL(k_line_pcr<double>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->line_L,
-                             c->llo, c->ldg, c->lup, res, pin, p, P.omega, c->skip);
+          hipLaunchKernelGGL(k_line_scan<double>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->line_L,
+                             c->la, c->linv, c->lc, res, pin, p, P.omega, c->skip);
       } else {
         int cg = blocks_for(c->ncol);
         if (P.precision == 1)
@@ -1303,11 +1299,6 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
       (rc = upload(c, &c->lc, c->h_lc.data(), n)))
     return fail(rc);
   if (m->ncol > 0) {
-    if ((rc = upload(c, &c->llo, c->h_lo.data(), n)) || (rc = upload(c, &c->ldg, c->h_dg.data(), n)) ||
-        (rc = upload(c, &c->lup, c->h_up.data(), n)))
-      return fail(rc);
-  }
-  if (m->ncol > 0) {
     if ((rc = upload(c, &c->col_start, m->col_start, m->ncol + 1))) return fail(rc);
   }
   // exact shortcuts for uniform node coefficients
@@ -1330,11 +1321,6 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
         (rc = upload(c, &c->laf, (f = tof(c->h_la)).data(), n)) || (rc = upload(c, &c->linvf, (f = tof(c->h_linv)).data(), n)) ||
         (rc = upload(c, &c->lcf, (f = tof(c->h_lc)).data(), n)) || (rc = dalloc(c, &c->zbuff, n)))
       return fail(rc);
-    if (m->ncol > 0) {
-      if ((rc = upload(c, &c->llof, (f = tof(c->h_lo)).data(), n)) || (rc = upload(c, &c->ldgf, (f = tof(c->h_dg)).data(), n)) ||
-          (rc = upload(c, &c->lupf, (f = tof(c->h_up)).data(), n)))
-        return fail(rc);
-    }
     for (int k = 0; k < 3; ++k)
       if ((rc = dalloc(c, &c->uuuf[k], n))) return fail(rc);
   }

@@ -225,10 +225,15 @@ __device__ __forceinline__ double cw(const NodeCoef &c, int i) { return c.coefw_
 // host's check then do no work. The flag is uniform, so whole blocks leave together.
 __device__ __forceinline__ bool skipped(const double *flag) { return flag && *flag != 0.0; }
 
-// Slots are consumed NFT_BATCH at a time: the batch's indices, then all its gathers, are
+// Slots are consumed NFT_BATCH_G (gradient) or NFT_BATCH_D (divergence) at a time: the batch's indices, then all its gathers, are
 // issued before any use, so a wavefront keeps several independent loads in flight; the
 // sums still run slot by slot in the reference's edge order.
-#define NFT_BATCH 4
+#ifndef NFT_BATCH_G
+#define NFT_BATCH_G 2   // gradient sweeps (2 measured best on gfx950, 1..8 tried)
+#endif
+#ifndef NFT_BATCH_D
+#define NFT_BATCH_D 2   // divergence sweeps
+#endif
 
 // ------------------------------------------------------------------------------------
 // Gradient of one field (derivative x3) with per-mode epilogues.
@@ -261,19 +266,19 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S
   double gx = 0., gy = 0., gz = 0.;
   if (MODE != GRAD_ZERO_PRFR) {
     double fi = L.valid ? a.f[i] : 0.;
-    for (int m0 = 0; m0 < L.width; m0 += NFT_BATCH) {
-      int jj[NFT_BATCH], ee[NFT_BATCH];
+    for (int m0 = 0; m0 < L.width; m0 += NFT_BATCH_G) {
+      int jj[NFT_BATCH_G], ee[NFT_BATCH_G];
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH; ++u) slot_at<COL>(g, L, C, m0 + u, m0 + u < L.deg, jj[u], ee[u]);
-      double fj[NFT_BATCH], sx[NFT_BATCH], sy[NFT_BATCH], sz[NFT_BATCH];
+      for (int u = 0; u < NFT_BATCH_G; ++u) slot_at<COL>(g, L, C, m0 + u, m0 + u < L.deg, jj[u], ee[u]);
+      double fj[NFT_BATCH_G], sx[NFT_BATCH_G], sy[NFT_BATCH_G], sz[NFT_BATCH_G];
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH; ++u) {
+      for (int u = 0; u < NFT_BATCH_G; ++u) {
         int e = ee[u] >= 0 ? ee[u] : ~ee[u];
         fj[u] = a.f[jj[u]];
         sx[u] = S.sx[e]; sy[u] = S.sy[e]; sz[u] = S.sz[e];
       }
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH; ++u) {
+      for (int u = 0; u < NFT_BATCH_G; ++u) {
         if (m0 + u < L.deg) {
           double v = (fi + fj[u]) * 0.5;
           if (ee[u] >= 0) { gx = gx + sx[u] * v; gy = gy + sy[u] * v; gz = gz + sz[u] * v; }
@@ -404,13 +409,13 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
       qi0 = a.q0[i]; qi1 = a.q1[i]; qi2 = a.q2[i];
       if (scale) { double r0 = a.nc.rh0[i]; qi0 = qi0 * r0; qi1 = qi1 * r0; qi2 = qi2 * r0; }
     }
-    for (int m0 = 0; m0 < L.width; m0 += NFT_BATCH) {
-      int jj[NFT_BATCH], ee[NFT_BATCH];
+    for (int m0 = 0; m0 < L.width; m0 += NFT_BATCH_D) {
+      int jj[NFT_BATCH_D], ee[NFT_BATCH_D];
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH; ++u) slot_at<COL>(g, L, C, m0 + u, m0 + u < L.deg, jj[u], ee[u]);
-      double q0[NFT_BATCH], q1[NFT_BATCH], q2[NFT_BATCH], sx[NFT_BATCH], sy[NFT_BATCH], sz[NFT_BATCH];
+      for (int u = 0; u < NFT_BATCH_D; ++u) slot_at<COL>(g, L, C, m0 + u, m0 + u < L.deg, jj[u], ee[u]);
+      double q0[NFT_BATCH_D], q1[NFT_BATCH_D], q2[NFT_BATCH_D], sx[NFT_BATCH_D], sy[NFT_BATCH_D], sz[NFT_BATCH_D];
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH; ++u) {
+      for (int u = 0; u < NFT_BATCH_D; ++u) {
         int e = ee[u] >= 0 ? ee[u] : ~ee[u];
         int j = jj[u];
         q0[u] = a.q0[j]; q1[u] = a.q1[j]; q2[u] = a.q2[j];
@@ -418,7 +423,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
         if (scale) { double rj = a.nc.rh0[j]; q0[u] = q0[u] * rj; q1[u] = q1[u] * rj; q2[u] = q2[u] * rj; }
       }
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH; ++u) {
+      for (int u = 0; u < NFT_BATCH_D; ++u) {
         if (m0 + u < L.deg) {
           double u0 = (qi0 + q0[u]) * 0.5, u1 = (qi1 + q1[u]) * 0.5, u2 = (qi2 + q2[u]) * 0.5;
           if (ee[u] >= 0) { ax = ax + sx[u] * u0; ay = ay + sy[u] * u1; az = az + sz[u] * u2; }
@@ -741,16 +746,22 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_line(int ncol, const int32_t *col
 }
 
 // vertical line, columns of equal length L <= 64: a wavefront holds 64/L columns, one
-// level per lane, and solves them by parallel cyclic reduction (6 steps of distance
-// 1..32, lanes past the columns are identity rows). Rows couple only inside their column:
-// the end rows have a = 0 / c = 0, which zeroes every cross-column term exactly.
+// level per lane. The factors of the Thomas algorithm are fixed (a: sub-diagonal, inv:
+// 1/pivot, c': scaled super-diagonal), so the solve is two first-order linear recurrences
+//   y_k = d_k*inv_k + alpha_k*y_(k-1),  alpha_k = -(a_k*inv_k)      (forward)
+//   z_k = y_k + gamma_k*z_(k+1),        gamma_k = -c'_k               (backward)
+// each evaluated as an inclusive scan of affine maps across the wavefront in 6 steps
+// (distance 1..32): (beta, alpha) <- (beta + alpha*beta', alpha*alpha') with the map s
+// lanes away. alpha = 0 at a column's first row and gamma = 0 at its last row (a = 0,
+// c' = 0 there), which drops every term from a neighbouring column exactly; lanes past
+// the columns carry the zero map.
 template <typename T>
 __device__ __forceinline__ T lane_get(T v, int src) { return __shfl(v, src, 64); }
 
 template <typename T>
-__global__ void __launch_bounds__(NFT_BLOCK) k_line_pcr(int ncol, int L, const T *lo, const T *dg,
-                                                        const T *up, const double *res, const double *pin,
-                                                        double *p, double omega, const double *skip) {
+__global__ void __launch_bounds__(NFT_BLOCK) k_line_scan(int ncol, int L, const T *la, const T *linv,
+                                                         const T *lc, const double *res, const double *pin,
+                                                         double *p, double omega, const double *skip) {
   if (skipped(skip)) return;
   const int lane = threadIdx.x & 63;
   const int wave = (blockIdx.x * NFT_BLOCK + threadIdx.x) >> 6;
@@ -758,27 +769,32 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_line_pcr(int ncol, int L, const T
   const int col = wave * cpw + lane / L;
   const bool real = lane < cpw * L && col < ncol;
   const int i = col * L + lane % L;
-  T a = (T)0, b = (T)1, c = (T)0, d = (T)0;
-  if (real) { a = lo[i]; b = dg[i]; c = up[i]; d = (T)res[i]; }
+  T al = (T)0, be = (T)0, ga = (T)0;
+  if (real) {
+    T iv = linv[i];
+    al = -(la[i] * iv);
+    be = (T)res[i] * iv;
+    ga = -lc[i];
+  }
+#pragma unroll
   for (int s = 1; s < 64; s <<= 1) {
-    int im = lane - s, ip = lane + s;
-    T am = lane_get(a, im < 0 ? lane : im), bm = lane_get(b, im < 0 ? lane : im);
-    T cm = lane_get(c, im < 0 ? lane : im), dm = lane_get(d, im < 0 ? lane : im);
-    T ap = lane_get(a, ip > 63 ? lane : ip), bp = lane_get(b, ip > 63 ? lane : ip);
-    T cp = lane_get(c, ip > 63 ? lane : ip), dp = lane_get(d, ip > 63 ? lane : ip);
-    if (im < 0) { am = (T)0; bm = (T)1; cm = (T)0; dm = (T)0; }
-    if (ip > 63) { ap = (T)0; bp = (T)1; cp = (T)0; dp = (T)0; }
-    T k1 = a / bm, k2 = c / bp;
-    T na = -(am * k1);
-    T nc = -(cp * k2);
-    T nb = b - cm * k1 - ap * k2;
-    T nd = d - dm * k1 - dp * k2;
-    a = na; b = nb; c = nc; d = nd;
+    T ap = lane_get(al, lane >= s ? lane - s : lane), bp = lane_get(be, lane >= s ? lane - s : lane);
+    if (lane >= s) {
+      be = be + al * bp;
+      al = al * ap;
+    }
+  }
+#pragma unroll
+  for (int s = 1; s < 64; s <<= 1) {
+    T gp = lane_get(ga, lane + s < 64 ? lane + s : lane), bp = lane_get(be, lane + s < 64 ? lane + s : lane);
+    if (lane + s < 64) {
+      be = be + ga * bp;
+      ga = ga * gp;
+    }
   }
   if (real) {
-    T z = d / b;
     double p0 = pin ? pin[i] : 0.;
-    p[i] = p0 + omega * (double)z;
+    p[i] = p0 + omega * (double)be;
   }
 }
 
