@@ -92,6 +92,7 @@ struct nft_ctx {
 
   // work
   double *t[8];          // prfr/div/laplc temporaries
+  double2 *txy;          // laplc: (x, y) of the scaled gradient (z in t[2])
   double *gx[NFT_MAXD], *gax[NFT_MAXD];
   double *r, *qr, *ar, *pw, *rw;
   double *mp_psi1[4];
@@ -461,11 +462,11 @@ int div_launch(nft_ctx *c, int mode, const DivArgs &a) {
 // laplc(press, alap, impfl) with optional fused dots on alap
 int laplc_dev(nft_ctx *c, const double *press, double *alap, int impfl, const DotList *dots) {
   GradArgs g = grad_args(c);
-  g.f = press; g.o0 = c->t[0]; g.o1 = c->t[1]; g.o2 = c->t[2]; g.impfl = impfl;
+  g.f = press; g.oxy = c->txy; g.o2 = c->t[2]; g.impfl = impfl;
   int rc = grad_launch(c, GRAD_LAPLC, g);
   if (rc) return rc;
   DivArgs d = div_args(c);
-  d.q0 = c->t[0]; d.q1 = c->t[1]; d.q2 = c->t[2]; d.out = alap;
+  d.qxy = c->txy; d.q2 = c->t[2]; d.out = alap;
   if (dots) d.dots = *dots;
   return div_launch(c, DIV_LAPLC, d);
 }
@@ -1259,15 +1260,18 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   g.n = n; g.nchunk = nchunk; g.chunk_ptr = d_chunk; g.deg = d_deg; g.nbr = d_nbr; g.eid = d_eid;
   g.col_L = 0;
   if (colL > 0) {
-    int32_t *d_tptr, *d_tnb, *d_tco, *d_xptr, *d_xnb, *d_xco;
-    if ((rc = upload(c, &d_tptr, tptr.data(), tptr.size())) || (rc = upload(c, &d_tnb, tnb.data(), tnb.size())) ||
-        (rc = upload(c, &d_tco, tco.data(), tco.size())) || (rc = upload(c, &d_xptr, xptr.data(), xptr.size())) ||
-        (rc = upload(c, &d_xnb, xnb.data(), xnb.size())) || (rc = upload(c, &d_xco, xco.data(), xco.size())))
+    std::vector<int2> tab(tnb.size() + xnb.size());
+    for (size_t q = 0; q < tnb.size(); ++q) tab[q] = make_int2(tnb[q], tco[q]);
+    for (size_t q = 0; q < xnb.size(); ++q) tab[tnb.size() + q] = make_int2(xnb[q], xco[q]);
+    int32_t *d_tptr, *d_xptr;
+    int2 *d_tab;
+    if ((rc = upload(c, &d_tptr, tptr.data(), tptr.size())) || (rc = upload(c, &d_xptr, xptr.data(), xptr.size())) ||
+        (rc = upload(c, &d_tab, tab.data(), tab.size())))
       return fail(rc);
     g.col_L = colL;
-    g.tptr = d_tptr; g.tnb = d_tnb; g.tco = d_tco; g.xptr = d_xptr; g.xnb = d_xnb; g.xco = d_xco;
+    g.tptr = d_tptr; g.xptr = d_xptr; g.tab = d_tab; g.xoff = (int)tnb.size();
     c->col_on = 1;
-    c->col_bytes = 4.0 * (tptr.size() + tnb.size() + tco.size() + xptr.size() + xnb.size() + xco.size());
+    c->col_bytes = 4.0 * (tptr.size() + xptr.size()) + 8.0 * tab.size();
   }
   g.bp_ptr = d_bp_ptr; g.bp_flag = d_bflag; g.bp_sbb = d_bsbb; g.bp_anp = d_banp; g.nbp = nbp;
   g.nblocks = (nchunk + NFT_WPB - 1) / NFT_WPB;
@@ -1285,6 +1289,14 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
       (rc = upload(c, &dsz, sz.data(), etot)))
     return fail(rc);
   c->S = EdgeS<double>{dsx, dsy, dsz};
+  {
+    std::vector<double2> sxy(etot);
+    for (long e = 0; e < etot; ++e) sxy[e] = make_double2(sx[e], sy[e]);
+    double2 *d_sxy;
+    if ((rc = upload(c, &d_sxy, sxy.data(), etot))) return fail(rc);
+    c->inc.sxy = d_sxy;
+    c->inc.szz = dsz;
+  }
 
   // ---- set-up fields
   std::vector<double> rh0, th0, ue, ve, dthedz, pstep;
@@ -1334,6 +1346,7 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   if ((rc = dalloc(c, &c->prpot, n))) return fail(rc);
   for (int k = 0; k < 8; ++k)
     if ((rc = dalloc(c, &c->t[k], n))) return fail(rc);
+  if ((rc = dalloc(c, &c->txy, n))) return fail(rc);
   for (int k = 0; k < P->lord; ++k)
     if ((rc = dalloc(c, &c->gx[k], n)) || (rc = dalloc(c, &c->gax[k], n))) return fail(rc);
   if ((rc = dalloc(c, &c->r, n)) || (rc = dalloc(c, &c->qr, n)) || (rc = dalloc(c, &c->ar, n)) ||

@@ -47,40 +47,47 @@ struct Incidence {
   int nblocks;               // logical blocks
   // Columnar form of the same slots (extruded meshes, node i = c*col_L + k): the interior
   // levels 0 < k < col_L-1 of column c share one slot template t in [tptr[c], tptr[c+1]):
-  // neighbour tnb[t]+k, edge (tco[t]>>1) - 1 + k in the i2 role when tco[t]&1; the bottom and top nodes keep
-  // explicit slots xnb/xco at xptr[c] + 2*m + (0 bottom, 1 top). Built only when it
-  // reproduces nbr/eid exactly; col_L = 0 otherwise.
+  // neighbour tab[t].x + k, edge (tab[t].y >> 1) - 1 + k, in the i2 role when tab[t].y & 1;
+  // the bottom and top nodes keep explicit (neighbour, signed edge) slots at
+  // tab[xoff + xptr[c] + 2*m + (0 bottom, 1 top)]. Built only when it reproduces nbr/eid
+  // exactly; col_L = 0 otherwise.
   int col_L;
-  const int32_t *tptr, *tnb, *tco, *xptr, *xnb, *xco;
+  const int32_t *tptr, *xptr;
+  // (neighbour, code) pairs: the templates (tptr) then, from xoff, the explicit slots of
+  // the bottom and top nodes (xptr), so every lane reads its slot with one 8-byte load
+  const int2 *tab;
+  int xoff;
+  // edge metrics for the gradient/divergence sweeps as a 16-byte (Sx, Sy) pair and Sz:
+  // two aligned loads per slot instead of three
+  const double2 *sxy;
+  const double *szz;
 };
 
 // Per-lane view of the columnar form.
 struct ColLane {
-  int k, t0, x0, xt;
-  bool inner;
+  int k;       // level
+  int base;    // tab index of slot 0
+  int stride;  // 1 (template) or 2 (explicit bottom/top slots)
+  int kk;      // level offset added to template entries (0 for explicit slots)
 };
 
 __device__ __forceinline__ void col_setup(const Incidence &g, int i, ColLane &C) {
   int c = i / g.col_L;
   C.k = i - c * g.col_L;
-  C.inner = C.k > 0 && C.k < g.col_L - 1;
-  C.t0 = g.tptr[c];
-  C.x0 = g.xptr[c];
-  C.xt = C.k == 0 ? 0 : 1;
+  const bool inner = C.k > 0 && C.k < g.col_L - 1;
+  C.base = inner ? g.tptr[c] : g.xoff + g.xptr[c] + (C.k == 0 ? 0 : 1);
+  C.stride = inner ? 1 : 2;
+  C.kk = inner ? C.k : -1;
 }
 
-// neighbour and signed edge code of slot m (valid lanes with m < deg only)
+// neighbour and signed edge code of slot m (valid lanes with m < deg only), branch free
 __device__ __forceinline__ void col_slot(const Incidence &g, const ColLane &C, int m, int &j, int &e) {
-  if (C.inner) {
-    j = g.tnb[C.t0 + m] + C.k;
-    int b = g.tco[C.t0 + m];
-    int ea = (b >> 1) - 1 + C.k;
-    e = (b & 1) ? ~ea : ea;
-  } else {
-    int s = C.x0 + 2 * m + C.xt;
-    j = g.xnb[s];
-    e = g.xco[s];
-  }
+  const int2 v = g.tab[C.base + m * C.stride];
+  const bool tmpl = C.kk >= 0;
+  j = v.x + (tmpl ? C.kk : 0);
+  const int ea = (v.y >> 1) - 1 + C.kk;
+  const int et = (v.y & 1) ? ~ea : ea;
+  e = tmpl ? et : v.y;
 }
 
 template <typename T>
@@ -119,17 +126,20 @@ __device__ __forceinline__ bool lane_setup(const Incidence &g, Lane &L) {
   return true;
 }
 
-// slot m of this lane (SELL-64 or columnar); j = e = 0 when !ok
+// slot m of this lane (SELL-64 or columnar). Loads are unconditional so that a batch's
+// loads are all in flight together: slots past the node's degree read its last slot
+// again and lanes past the mesh read slot 0 of their chunk (both valid addresses); the
+// callers drop those slots when they accumulate.
 template <bool COL>
-__device__ __forceinline__ void slot_at(const Incidence &g, const Lane &L, const ColLane &C, int m, bool ok, int &j,
+__device__ __forceinline__ void slot_at(const Incidence &g, const Lane &L, const ColLane &C, int m, int &j,
                                         int &e) {
+  const int mm = L.deg > 0 ? min(m, L.deg - 1) : 0;
   if (COL) {
-    if (ok) col_slot(g, C, m, j, e);
-    else { j = 0; e = 0; }
+    col_slot(g, C, mm, j, e);
   } else {
-    int slot = L.base + m * NFT_WAVE + L.lane;
-    j = ok ? g.nbr[slot] : 0;
-    e = ok ? g.eid[slot] : 0;
+    const int slot = L.base + mm * NFT_WAVE + L.lane;
+    j = g.nbr[slot];
+    e = g.eid[slot];
   }
 }
 
@@ -241,13 +251,14 @@ __device__ __forceinline__ bool skipped(const double *flag) { return flag && *fl
 enum GradMode {
   GRAD_RAW = 0,     // out0..2 = vol-integrated gradient (derivative idx 1..3)
   GRAD_PRFR = 1,    // out = str - 0.5*dt*g/vol (w: *fz), then bcv(...,1.)       (prfr)
-  GRAD_LAPLC = 2,   // out = rh0 * bcv0(0.5*(dt*g/vol), w: *fz)                    (laplc 1st half)
+  GRAD_LAPLC = 2,   // out = rh0 * bcv0(0.5*(dt*g/vol), w: *fz) -> (oxy, o2)        (laplc 1st half)
   GRAD_ZERO_PRFR = 3  // pressure identically zero: out = bcv(str, 1.)
 };
 
 struct GradArgs {
   const double *f;
   double *o0, *o1, *o2;
+  double2 *oxy;                // GRAD_LAPLC: (x, y) components; z in o2
   const double *s0, *s1, *s2;  // ustr, vstr, wstr (PRFR)
   NodeCoef nc;
   double dt;
@@ -262,28 +273,31 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S
   if (!lane_setup(g, L)) return;
   const int i = L.i;
   ColLane C{};
-  if (COL && L.valid) col_setup(g, i, C);
+  if (COL) col_setup(g, L.valid ? i : g.n - 1, C);  // lanes past the mesh: a valid column, no slots used
   double gx = 0., gy = 0., gz = 0.;
   if (MODE != GRAD_ZERO_PRFR) {
     double fi = L.valid ? a.f[i] : 0.;
     for (int m0 = 0; m0 < L.width; m0 += NFT_BATCH_G) {
       int jj[NFT_BATCH_G], ee[NFT_BATCH_G];
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH_G; ++u) slot_at<COL>(g, L, C, m0 + u, m0 + u < L.deg, jj[u], ee[u]);
+      for (int u = 0; u < NFT_BATCH_G; ++u) slot_at<COL>(g, L, C, m0 + u, jj[u], ee[u]);
       double fj[NFT_BATCH_G], sx[NFT_BATCH_G], sy[NFT_BATCH_G], sz[NFT_BATCH_G];
 #pragma unroll
       for (int u = 0; u < NFT_BATCH_G; ++u) {
         int e = ee[u] >= 0 ? ee[u] : ~ee[u];
         fj[u] = a.f[jj[u]];
-        sx[u] = S.sx[e]; sy[u] = S.sy[e]; sz[u] = S.sz[e];
+        const double2 sxy = g.sxy[e];
+        sx[u] = sxy.x; sy[u] = sxy.y; sz[u] = g.szz[e];
       }
 #pragma unroll
       for (int u = 0; u < NFT_BATCH_G; ++u) {
-        if (m0 + u < L.deg) {
-          double v = (fi + fj[u]) * 0.5;
-          if (ee[u] >= 0) { gx = gx + sx[u] * v; gy = gy + sy[u] * v; gz = gz + sz[u] * v; }
-          else { gx = gx - sx[u] * v; gy = gy - sy[u] * v; gz = gz - sz[u] * v; }
-        }
+        // selects, not branches: slots past the degree leave the sums exactly as they are
+        const bool ok = m0 + u < L.deg;
+        const double v = (fi + fj[u]) * 0.5;
+        const double tx = sx[u] * v, ty = sy[u] * v, tz = sz[u] * v;
+        const bool pos = ee[u] >= 0;
+        const double nx = pos ? gx + tx : gx - tx, ny = pos ? gy + ty : gy - ty, nz = pos ? gz + tz : gz - tz;
+        gx = ok ? nx : gx; gy = ok ? ny : gy; gz = ok ? nz : gz;
       }
     }
     if (L.valid) {
@@ -318,12 +332,12 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S
     gx = a.dt * gx / vl; gy = a.dt * gy / vl; gz = a.dt * gz / vl;
     gx = 0.5 * gx; gy = 0.5 * gy; gz = 0.5 * gz * fz;
     bcv_node<double>(g, i, gx, gy, gz, a.nc.ue, 0.0);
-    if (a.nc.rh0_one) {
-      a.o0[i] = gx; a.o1[i] = gy; a.o2[i] = gz;
-    } else {
+    if (!a.nc.rh0_one) {
       double r = a.nc.rh0[i];
-      a.o0[i] = gx * r; a.o1[i] = gy * r; a.o2[i] = gz * r;
+      gx = gx * r; gy = gy * r; gz = gz * r;
     }
+    a.oxy[i] = make_double2(gx, gy);
+    a.o2[i] = gz;
   }
 }
 
@@ -332,7 +346,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S
 // with ND fused inner products of the result (and optionally max|out|).
 // ------------------------------------------------------------------------------------
 enum DivMode {
-  DIV_LAPLC = 0,   // out = ((ax+ay)+az)/(vol*rh0)                                (laplc 2nd half)
+  DIV_LAPLC = 0,   // out = ((ax+ay)+az)/(vol*rh0), inputs (qxy, q2)              (laplc 2nd half)
   DIV_DIV = 1,     // out = iflg*((ax/vol+ay/vol)+az/vol)/rh0, inputs scaled by rh0 if iflg=-1
 };
 
@@ -360,6 +374,7 @@ enum Scal {
 
 struct DivArgs {
   const double *q0, *q1, *q2;
+  const double2 *qxy;             // DIV_LAPLC: (x, y) components; z in q2
   double *out;
   NodeCoef nc;
   int iflg;
@@ -401,34 +416,40 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
   if (active) {
     const int i = L.i;
     ColLane C{};
-    if (COL && L.valid) col_setup(g, i, C);
+    if (COL) col_setup(g, L.valid ? i : g.n - 1, C);
     const bool scale = MODE == DIV_DIV && a.iflg == -1 && !a.nc.rh0_one;
     double ax = 0., ay = 0., az = 0.;
     double qi0 = 0., qi1 = 0., qi2 = 0.;
     if (L.valid) {
-      qi0 = a.q0[i]; qi1 = a.q1[i]; qi2 = a.q2[i];
+      if (MODE == DIV_LAPLC) { const double2 v = a.qxy[i]; qi0 = v.x; qi1 = v.y; }
+      else { qi0 = a.q0[i]; qi1 = a.q1[i]; }
+      qi2 = a.q2[i];
       if (scale) { double r0 = a.nc.rh0[i]; qi0 = qi0 * r0; qi1 = qi1 * r0; qi2 = qi2 * r0; }
     }
     for (int m0 = 0; m0 < L.width; m0 += NFT_BATCH_D) {
       int jj[NFT_BATCH_D], ee[NFT_BATCH_D];
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH_D; ++u) slot_at<COL>(g, L, C, m0 + u, m0 + u < L.deg, jj[u], ee[u]);
+      for (int u = 0; u < NFT_BATCH_D; ++u) slot_at<COL>(g, L, C, m0 + u, jj[u], ee[u]);
       double q0[NFT_BATCH_D], q1[NFT_BATCH_D], q2[NFT_BATCH_D], sx[NFT_BATCH_D], sy[NFT_BATCH_D], sz[NFT_BATCH_D];
 #pragma unroll
       for (int u = 0; u < NFT_BATCH_D; ++u) {
         int e = ee[u] >= 0 ? ee[u] : ~ee[u];
         int j = jj[u];
-        q0[u] = a.q0[j]; q1[u] = a.q1[j]; q2[u] = a.q2[j];
-        sx[u] = S.sx[e]; sy[u] = S.sy[e]; sz[u] = S.sz[e];
+        if (MODE == DIV_LAPLC) { const double2 v = a.qxy[j]; q0[u] = v.x; q1[u] = v.y; }
+        else { q0[u] = a.q0[j]; q1[u] = a.q1[j]; }
+        q2[u] = a.q2[j];
+        const double2 sxy = g.sxy[e];
+        sx[u] = sxy.x; sy[u] = sxy.y; sz[u] = g.szz[e];
         if (scale) { double rj = a.nc.rh0[j]; q0[u] = q0[u] * rj; q1[u] = q1[u] * rj; q2[u] = q2[u] * rj; }
       }
 #pragma unroll
       for (int u = 0; u < NFT_BATCH_D; ++u) {
-        if (m0 + u < L.deg) {
-          double u0 = (qi0 + q0[u]) * 0.5, u1 = (qi1 + q1[u]) * 0.5, u2 = (qi2 + q2[u]) * 0.5;
-          if (ee[u] >= 0) { ax = ax + sx[u] * u0; ay = ay + sy[u] * u1; az = az + sz[u] * u2; }
-          else { ax = ax - sx[u] * u0; ay = ay - sy[u] * u1; az = az - sz[u] * u2; }
-        }
+        const bool ok = m0 + u < L.deg;
+        const double u0 = (qi0 + q0[u]) * 0.5, u1 = (qi1 + q1[u]) * 0.5, u2 = (qi2 + q2[u]) * 0.5;
+        const double tx = sx[u] * u0, ty = sy[u] * u1, tz = sz[u] * u2;
+        const bool pos = ee[u] >= 0;
+        const double nx = pos ? ax + tx : ax - tx, ny = pos ? ay + ty : ay - ty, nz = pos ? az + tz : az - tz;
+        ax = ok ? nx : ax; ay = ok ? ny : ay; az = ok ? nz : az;
       }
     }
     if (L.valid) {
