@@ -80,16 +80,6 @@ __device__ __forceinline__ void col_setup(const Incidence &g, int i, ColLane &C)
   C.kk = inner ? C.k : -1;
 }
 
-// neighbour and signed edge code of slot m (valid lanes with m < deg only), branch free
-__device__ __forceinline__ void col_slot(const Incidence &g, const ColLane &C, int m, int &j, int &e) {
-  const int2 v = g.tab[C.base + m * C.stride];
-  const bool tmpl = C.kk >= 0;
-  j = v.x + (tmpl ? C.kk : 0);
-  const int ea = (v.y >> 1) - 1 + C.kk;
-  const int et = (v.y & 1) ? ~ea : ea;
-  e = tmpl ? et : v.y;
-}
-
 template <typename T>
 struct EdgeS {
   const T *sx, *sy, *sz;
@@ -119,27 +109,38 @@ __device__ __forceinline__ bool lane_setup(const Incidence &g, Lane &L) {
   if (chunk >= g.nchunk) return false;
   L.lane = threadIdx.x & 63;
   L.i = chunk * NFT_WAVE + L.lane;
-  L.base = g.chunk_ptr[chunk];
-  L.width = (g.chunk_ptr[chunk + 1] - L.base) >> 6;
+  // chunk values are the same in every lane of the wavefront: keep them in scalar registers
+  L.base = __builtin_amdgcn_readfirstlane(g.chunk_ptr[chunk]);
+  L.width = __builtin_amdgcn_readfirstlane((g.chunk_ptr[chunk + 1] - L.base) >> 6);
   L.valid = L.i < g.n;
   L.deg = L.valid ? g.deg[L.i] : 0;
   return true;
 }
 
-// slot m of this lane (SELL-64 or columnar). Loads are unconditional so that a batch's
-// loads are all in flight together: slots past the node's degree read its last slot
-// again and lanes past the mesh read slot 0 of their chunk (both valid addresses); the
+// Slot m of this lane (SELL-64 or columnar) in two steps: the raw 8-byte entry, then its
+// decoding into (neighbour, signed edge), so a batch's entries can be loaded one batch
+// ahead. Loads are unconditional: slots past the node's degree read its last slot again
+// and lanes past the mesh read slot 0 of their chunk (valid addresses in both cases); the
 // callers drop those slots when they accumulate.
 template <bool COL>
-__device__ __forceinline__ void slot_at(const Incidence &g, const Lane &L, const ColLane &C, int m, int &j,
-                                        int &e) {
+__device__ __forceinline__ int2 slot_raw(const Incidence &g, const Lane &L, const ColLane &C, int m) {
   const int mm = L.deg > 0 ? min(m, L.deg - 1) : 0;
+  if (COL) return g.tab[C.base + mm * C.stride];
+  const int slot = L.base + mm * NFT_WAVE + L.lane;
+  return make_int2(g.nbr[slot], g.eid[slot]);
+}
+
+template <bool COL>
+__device__ __forceinline__ void slot_decode(const ColLane &C, int2 v, int &j, int &e) {
   if (COL) {
-    col_slot(g, C, mm, j, e);
+    const bool tmpl = C.kk >= 0;
+    j = v.x + (tmpl ? C.kk : 0);
+    const int ea = (v.y >> 1) - 1 + C.kk;
+    const int et = (v.y & 1) ? ~ea : ea;
+    e = tmpl ? et : v.y;
   } else {
-    const int slot = L.base + mm * NFT_WAVE + L.lane;
-    j = g.nbr[slot];
-    e = g.eid[slot];
+    j = v.x;
+    e = v.y;
   }
 }
 
@@ -277,10 +278,18 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S
   double gx = 0., gy = 0., gz = 0.;
   if (MODE != GRAD_ZERO_PRFR) {
     double fi = L.valid ? a.f[i] : 0.;
+    // the next batch's slot entries are loaded while this batch's gathers are in flight
+    int2 rn[NFT_BATCH_G];
+#pragma unroll
+    for (int u = 0; u < NFT_BATCH_G; ++u) rn[u] = slot_raw<COL>(g, L, C, u);
     for (int m0 = 0; m0 < L.width; m0 += NFT_BATCH_G) {
       int jj[NFT_BATCH_G], ee[NFT_BATCH_G];
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH_G; ++u) slot_at<COL>(g, L, C, m0 + u, jj[u], ee[u]);
+      for (int u = 0; u < NFT_BATCH_G; ++u) slot_decode<COL>(C, rn[u], jj[u], ee[u]);
+      if (m0 + NFT_BATCH_G < L.width) {
+#pragma unroll
+        for (int u = 0; u < NFT_BATCH_G; ++u) rn[u] = slot_raw<COL>(g, L, C, m0 + NFT_BATCH_G + u);
+      }
       double fj[NFT_BATCH_G], sx[NFT_BATCH_G], sy[NFT_BATCH_G], sz[NFT_BATCH_G];
 #pragma unroll
       for (int u = 0; u < NFT_BATCH_G; ++u) {
@@ -426,10 +435,17 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
       qi2 = a.q2[i];
       if (scale) { double r0 = a.nc.rh0[i]; qi0 = qi0 * r0; qi1 = qi1 * r0; qi2 = qi2 * r0; }
     }
+    int2 rn[NFT_BATCH_D];
+#pragma unroll
+    for (int u = 0; u < NFT_BATCH_D; ++u) rn[u] = slot_raw<COL>(g, L, C, u);
     for (int m0 = 0; m0 < L.width; m0 += NFT_BATCH_D) {
       int jj[NFT_BATCH_D], ee[NFT_BATCH_D];
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH_D; ++u) slot_at<COL>(g, L, C, m0 + u, jj[u], ee[u]);
+      for (int u = 0; u < NFT_BATCH_D; ++u) slot_decode<COL>(C, rn[u], jj[u], ee[u]);
+      if (m0 + NFT_BATCH_D < L.width) {
+#pragma unroll
+        for (int u = 0; u < NFT_BATCH_D; ++u) rn[u] = slot_raw<COL>(g, L, C, m0 + NFT_BATCH_D + u);
+      }
       double q0[NFT_BATCH_D], q1[NFT_BATCH_D], q2[NFT_BATCH_D], sx[NFT_BATCH_D], sy[NFT_BATCH_D], sz[NFT_BATCH_D];
 #pragma unroll
       for (int u = 0; u < NFT_BATCH_D; ++u) {
