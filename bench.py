@@ -103,6 +103,20 @@ def learned_run(mesh, cfg, steps, train_steps, log_prefix="[learned]"):
                           "objective": "|L z - r|^2/|r|^2, z = M_line^-1 r + s N(r/s)"})
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/production_fp64_traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+    passes over this bench, each scaled by what the counter reads for a plain copy of
+    known size), or None when the summary has no entry for it."""
+    path = os.path.join(ROOT, "profiles", "production_fp64_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d["kernels"][kernel]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(mesh, cfg, iters_per_solve):
     """Serial FP64 oracle on the production mesh: one MPDATA call per field, one prfr,
     div, line preconditioner and laplc, and one GCR cycle are timed and scaled to a step
@@ -228,7 +242,8 @@ def main():
         "mpdata_ms_per_step": round(st.ms_mpdata / max(K, 1), 3),
         "state_finite": finite,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(kname) if kname else None,
                      "launches": nl, "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": int(kbytes)},
     }
     if not args.no_variants:
