@@ -234,7 +234,7 @@ def main():
                          "N=0.01 1/s (Fr=1), dt=20 s; MPDATA + 2 x GCR(4) to 1e-8 relative residual with "
                          "the vertical line preconditioner, FP64"),
             "nodes": mesh.npoin, "edges": mesh.nedge + mesh.nbface, "gcr_order": cfg.lord,
-            "tolerance": cfg.eps0, "preconditioner": "vertical line (tridiagonal, wavefront PCR)",
+            "tolerance": cfg.eps0, "preconditioner": "vertical line (tridiagonal, wavefront scans of the Thomas factors)",
             "precision": "fp64",
         },
         "gcr": {"iters_per_solve": round(iters, 2), "solve_ms": round(st.ms_gcr / max(st.gcr_solves, 1), 3),
@@ -243,7 +243,7 @@ def main():
         "state_finite": finite,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(kname) if kname else None,
+                     "traffic": (round(measured_traffic(kname)) if kname and measured_traffic(kname) else None),
                      "launches": nl, "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": int(kbytes)},
     }
     if not args.no_variants:
