@@ -23,7 +23,8 @@ struct oc_ctx {
   /* line preconditioner: Thomas factors and operator rows */
   double *la, *linv, *lc;
   double *llo, *ldg, *lup;
-  int line_L;   /* uniform column length <= 64 (wavefront scans), else 0 (Thomas) */
+  int line_L;   /* uniform column length <= 64 (wavefront scans), else 0 */
+  int line_R;   /* rows per lane of the coarsened scans for uniform 64 < L <= 512, else 0 (Thomas) */
   /* scratch */
   double *s[16];
   /* gcrk work */
@@ -467,6 +468,13 @@ static void build_line(oc_ctx *c) {
   for (int k = 0; k < m->ncol && uniform; ++k)
     if (m->col_start[k + 1] - m->col_start[k] != L0 || m->col_start[k] != k * L0) uniform = 0;
   c->line_L = uniform ? L0 : 0;
+  c->line_R = 0;
+  if (!uniform && m->ncol > 0) {
+    int longcols = L0 > 64 && L0 <= 512;
+    for (int k = 0; k < m->ncol && longcols; ++k)
+      if (m->col_start[k + 1] - m->col_start[k] != L0 || m->col_start[k] != k * L0) longcols = 0;
+    if (longcols) c->line_R = (L0 + 63) / 64;
+  }
 }
 
 /* One column of L <= 64 rows in a 64-lane frame (rows past L carry the zero map), solved
@@ -502,8 +510,61 @@ static void scan_column(const double *la, const double *linv, const double *lc, 
   for (int l = 0; l < L; ++l) z[l] = be[l];
 }
 
+/* One column of 64 < L <= 64*R rows, lane l holding rows l*R .. l*R+R-1 (rows >= L the
+ * zero map), in the operation order of the HIP path's coarsened scan (k_line_scan_r):
+ * each lane composes its rows' maps in order into (A, B) from (1, 0); the 64 lane maps
+ * are scanned as in scan_column; lane l replays its rows from y_in = B of lane l-1 (0 for
+ * lane 0); then the same backwards with gamma = -c' from the last row down. */
+static void scan_column_r(const double *la, const double *linv, const double *lc, const double *rhs, int L,
+                          int R, double *z) {
+  double al[512], be[512], ga[512];
+  double A[64], B[64], nA[64], nB[64], yin[64];
+  for (int q = 0; q < 64 * R; ++q) {
+    if (q < L) { al[q] = -(la[q] * linv[q]); be[q] = rhs[q] * linv[q]; ga[q] = -lc[q]; }
+    else { al[q] = 0.; be[q] = 0.; ga[q] = 0.; }
+  }
+  for (int l = 0; l < 64; ++l) {
+    double a = 1., b = 0.;
+    for (int r = 0; r < R; ++r) { b = be[l * R + r] + al[l * R + r] * b; a = al[l * R + r] * a; }
+    A[l] = a; B[l] = b;
+  }
+  for (int s = 1; s < 64; s <<= 1) {
+    for (int l = 0; l < 64; ++l) {
+      if (l >= s) { nB[l] = B[l] + A[l] * B[l - s]; nA[l] = A[l] * A[l - s]; }
+      else { nB[l] = B[l]; nA[l] = A[l]; }
+    }
+    for (int l = 0; l < 64; ++l) { A[l] = nA[l]; B[l] = nB[l]; }
+  }
+  for (int l = 0; l < 64; ++l) yin[l] = l > 0 ? B[l - 1] : 0.;
+  for (int l = 0; l < 64; ++l) {
+    double y = yin[l];
+    for (int r = 0; r < R; ++r) { y = be[l * R + r] + al[l * R + r] * y; be[l * R + r] = y; }
+  }
+  for (int l = 0; l < 64; ++l) {
+    double g = 1., b = 0.;
+    for (int r = R - 1; r >= 0; --r) { b = be[l * R + r] + ga[l * R + r] * b; g = ga[l * R + r] * g; }
+    A[l] = g; B[l] = b;
+  }
+  for (int s = 1; s < 64; s <<= 1) {
+    for (int l = 0; l < 64; ++l) {
+      if (l + s < 64) { nB[l] = B[l] + A[l] * B[l + s]; nA[l] = A[l] * A[l + s]; }
+      else { nB[l] = B[l]; nA[l] = A[l]; }
+    }
+    for (int l = 0; l < 64; ++l) { A[l] = nA[l]; B[l] = nB[l]; }
+  }
+  for (int l = 0; l < 64; ++l) yin[l] = l < 63 ? B[l + 1] : 0.;
+  for (int l = 0; l < 64; ++l) {
+    double zz = yin[l];
+    for (int r = R - 1; r >= 0; --r) {
+      zz = be[l * R + r] + ga[l * R + r] * zz;
+      if (l * R + r < L) z[l * R + r] = zz;
+    }
+  }
+}
+
 /* z = M^-1 rhs, column by column: wavefront scans for uniform columns of at most 64
- * levels, otherwise a forward then backward Thomas sweep */
+ * levels, coarsened scans up to 512 levels, otherwise a forward then backward Thomas
+ * sweep */
 void oc_line_solve(oc_ctx *c, const double *rhs, double *z) {
   const oc_mesh *m = &c->m;
   if (c->line_L > 0) {
@@ -511,6 +572,14 @@ void oc_line_solve(oc_ctx *c, const double *rhs, double *z) {
     for (int k = 0; k < m->ncol; ++k) {
       int s0 = k * L;
       scan_column(c->la + s0, c->linv + s0, c->lc + s0, rhs + s0, L, z + s0);
+    }
+    return;
+  }
+  if (c->line_R > 0) {
+    int L = m->col_start[1], R = c->line_R;
+    for (int k = 0; k < m->ncol; ++k) {
+      int s0 = k * L;
+      scan_column_r(c->la + s0, c->linv + s0, c->lc + s0, rhs + s0, L, R, z + s0);
     }
     return;
   }

@@ -74,7 +74,8 @@ struct nft_ctx {
   int32_t *col_start;
   double *la, *linv, *lc;
   float *laf, *linvf, *lcf;
-  int line_L = 0;             // uniform column length (0: not uniform or > 64 -> Thomas)
+  int line_L = 0;             // uniform column length <= 64 (wavefront scans), else 0
+  int line_R = 0;             // rows per lane of the coarsened scan for 64 < L <= 512, else 0 (Thomas)
   int col_L = 0;              // uniform column length with nodes numbered column by column (0: not)
   int col_on = 0;             // gathers use the columnar incidence (inc.col_L > 0)
   double col_bytes = 0;       // bytes of the columnar tables
@@ -330,6 +331,7 @@ void setup_fields(nft_ctx *c, const nft_mesh *m, std::vector<double> &rh0, std::
       if (m->col_start[k + 1] - m->col_start[k] != L0 || m->col_start[k] != k * L0) uniform = false;
     c->col_L = uniform ? L0 : 0;
     c->line_L = uniform && L0 <= 64 ? L0 : 0;
+    c->line_R = uniform && L0 > 64 && L0 <= 64 * 8 ? (L0 + 63) / 64 : 0;
   }
 }
 
@@ -498,6 +500,23 @@ int div_dev(nft_ctx *c, const double *ua, const double *va, const double *wa, do
   return div_launch(c, DIV_DIV, d);
 }
 
+// coarsened line scan for uniform columns of 64 < L <= 64*R rows (R = line_R)
+template <typename T>
+void line_scan_r_go(nft_ctx *c, int cg, const T *la, const T *linv, const T *lc, const double *res,
+                    const double *pin, double *p) {
+  const int L = c->col_L;
+  const double om = c->P.omega;
+  switch (c->line_R) {
+    case 2: hipLaunchKernelGGL((k_line_scan_r<T, 2>), dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, L, la, linv, lc, res, pin, p, om, c->skip); break;
+    case 3: hipLaunchKernelGGL((k_line_scan_r<T, 3>), dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, L, la, linv, lc, res, pin, p, om, c->skip); break;
+    case 4: hipLaunchKernelGGL((k_line_scan_r<T, 4>), dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, L, la, linv, lc, res, pin, p, om, c->skip); break;
+    case 5: hipLaunchKernelGGL((k_line_scan_r<T, 5>), dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, L, la, linv, lc, res, pin, p, om, c->skip); break;
+    case 6: hipLaunchKernelGGL((k_line_scan_r<T, 6>), dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, L, la, linv, lc, res, pin, p, om, c->skip); break;
+    case 7: hipLaunchKernelGGL((k_line_scan_r<T, 7>), dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, L, la, linv, lc, res, pin, p, om, c->skip); break;
+    default: hipLaunchKernelGGL((k_line_scan_r<T, 8>), dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, L, la, linv, lc, res, pin, p, om, c->skip); break;
+  }
+}
+
 int precon_dev(nft_ctx *c, const double *rhs, double *p, int impfl) {
   const nft_params &P = c->P;
   const int n = c->n;
@@ -533,6 +552,10 @@ int precon_dev(nft_ctx *c, const double *rhs, double *p, int impfl) {
         else
           hipLaunchKernelGGL(k_line_scan<double>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->line_L,
                              c->la, c->linv, c->lc, res, pin, p, P.omega, c->skip);
+      } else if (c->line_R > 0) {
+        int cg = (int)((c->ncol + NFT_WPB - 1) / NFT_WPB);
+        if (P.precision == 1) line_scan_r_go<float>(c, cg, c->laf, c->linvf, c->lcf, res, pin, p);
+        else line_scan_r_go<double>(c, cg, c->la, c->linv, c->lc, res, pin, p);
       } else {
         int cg = blocks_for(c->ncol);
         if (P.precision == 1)

@@ -835,6 +835,79 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_line_scan(int ncol, int L, const 
   }
 }
 
+// vertical line, columns of equal length 64 < L <= 64*R: one wavefront per column, lane l
+// holding rows l*R .. l*R+R-1 (rows >= L carry the zero map). Same recurrences as
+// k_line_scan, evaluated as a thread-coarsened scan:
+//   forward:  each lane composes its rows' maps in order into (A, B) (A = 1, B = 0 to
+//             start; B = beta + alpha*B, A = alpha*A), the wavefront scans the lane maps
+//             as k_line_scan does, lane l takes y_in = B of lane l-1 (0 for lane 0) and
+//             replays its rows: y = beta + alpha*y;
+//   backward: the same from the last row down with gamma = -c' and the forward results.
+template <typename T, int R>
+__global__ void __launch_bounds__(NFT_BLOCK) k_line_scan_r(int ncol, int L, const T *la, const T *linv,
+                                                           const T *lc, const double *res, const double *pin,
+                                                           double *p, double omega, const double *skip) {
+  if (skipped(skip)) return;
+  const int lane = threadIdx.x & 63;
+  const int col = (blockIdx.x * NFT_BLOCK + threadIdx.x) >> 6;
+  if (col >= ncol) return;
+  const size_t c0 = (size_t)col * L;
+  T al[R], be[R], ga[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int row = lane * R + r;
+    al[r] = (T)0; be[r] = (T)0; ga[r] = (T)0;
+    if (row < L) {
+      const size_t i = c0 + row;
+      T iv = linv[i];
+      al[r] = -(la[i] * iv);
+      be[r] = (T)res[i] * iv;
+      ga[r] = -lc[i];
+    }
+  }
+  // forward
+  T A = (T)1, B = (T)0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) { B = be[r] + al[r] * B; A = al[r] * A; }
+#pragma unroll
+  for (int s = 1; s < 64; s <<= 1) {
+    T ap = lane_get(A, lane >= s ? lane - s : lane), bp = lane_get(B, lane >= s ? lane - s : lane);
+    if (lane >= s) {
+      B = B + A * bp;
+      A = A * ap;
+    }
+  }
+  T y = lane_get(B, lane > 0 ? lane - 1 : 0);
+  if (lane == 0) y = (T)0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) { y = be[r] + al[r] * y; be[r] = y; }
+  // backward (be now holds the forward results)
+  T G = (T)1;
+  B = (T)0;
+#pragma unroll
+  for (int r = R - 1; r >= 0; --r) { B = be[r] + ga[r] * B; G = ga[r] * G; }
+#pragma unroll
+  for (int s = 1; s < 64; s <<= 1) {
+    T gp = lane_get(G, lane + s < 64 ? lane + s : lane), bp = lane_get(B, lane + s < 64 ? lane + s : lane);
+    if (lane + s < 64) {
+      B = B + G * bp;
+      G = G * gp;
+    }
+  }
+  T z = lane_get(B, lane < 63 ? lane + 1 : 63);
+  if (lane == 63) z = (T)0;
+#pragma unroll
+  for (int r = R - 1; r >= 0; --r) {
+    z = be[r] + ga[r] * z;
+    const int row = lane * R + r;
+    if (row < L) {
+      const size_t i = c0 + row;
+      double p0 = pin ? pin[i] : 0.;
+      p[i] = p0 + omega * (double)z;
+    }
+  }
+}
+
 // point relaxation: p = p + omega*pstep*(r - rhs)   (reference :607-612)
 __global__ void __launch_bounds__(NFT_BLOCK) k_point(int n, const double *pstep, const double *r,
                                                      const double *rhs, double *p, double omega,

@@ -84,6 +84,23 @@ def test_precon_bitwise(key, kind):
     s.close()
 
 
+@pytest.mark.parametrize("key", ["tall", "tall3"])
+def test_long_column_line_solve_bitwise(key):
+    """Columns of 100 and 130 levels: the coarsened line scans equal the oracle's
+    restatement bit for bit, alone and inside a GCR solve."""
+    m = mesh_for(key)
+    cfg = tiny_cfg(precon=2, itrp1=1, omega=1.0, exit_mode=1, eps0=1e-8, itr=100)
+    s, o = solver(m, cfg), Oracle(m, cfg, dot_order=1)
+    f = seeded_fields(m)
+    assert np.array_equal(s.precon(f["r"], 1), o.precon(f["r"], 1))
+    p0 = np.zeros(m.npoin)
+    pg, ug, st, rc = s.gcrk(p0, f["u"], f["v"], 0.1 * f["w"], 1)
+    po, uo = o.gcrk(p0, f["u"], f["v"], 0.1 * f["w"], 1)
+    assert st.niter == o.stats().niter
+    assert np.array_equal(pg, po)
+    s.close()
+
+
 def test_mpdatam_bitwise(pair):
     m, cfg, s, o = pair
     f = seeded_fields(m)

@@ -111,8 +111,11 @@ def test_small_mesh_steps_against_reference():
 
 
 # ---- closed forms ------------------------------------------------------------------
-def test_line_solve_matches_dense():
-    m = mesh_for("tiny2")
+@pytest.mark.parametrize("key", ["tiny2", "tall", "tall3"])
+def test_line_solve_matches_dense(key):
+    """Wavefront scans (6 levels), coarsened scans (100 and 130 levels: 2 and 3 rows per
+    lane) against a dense solve of each column's tridiagonal system."""
+    m = mesh_for(key)
     o = Oracle(m, tiny_cfg(precon=2, itrp1=1, omega=1.0))
     rhs = np.random.default_rng(3).normal(size=m.npoin)
     z = o.line_solve(rhs)
