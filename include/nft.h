@@ -138,6 +138,23 @@ int nft_get_gcr_stats(nft_ctx *ctx, nft_gcr_stats *stats);
  * 0 = node-sorted SELL-64. columnar < 0 only reports availability. */
 int nft_set_incidence(nft_ctx *ctx, int columnar, int *available);
 
+/* ---- files ---------------------------------------------------------------------------
+ * Full-precision state file, replacing the reference's f14.6 text restart
+ * (Restart_Out_old / Restart_In_old :4779-4890, called at :472 and :277): the step-boundary
+ * state unk, frc, uuu, unkdash, prpot plus the step and GCR counters; resuming from it
+ * continues the run bit for bit. */
+int nft_save_state(nft_ctx *ctx, const char *path);
+int nft_load_state(nft_ctx *ctx, const char *path);
+/* time steps since nft_state_init (or as loaded with a state file) */
+int nft_steps_done(nft_ctx *ctx, int64_t *steps);
+/* Binary mesh file with the content of the reference's mesh3d.d (`input` :2343-2501) and
+ * the column table: host-only, no GPU needed. nft_mesh_load allocates the arrays (leading
+ * dimensions = counts; kbface/sb and ibpoin/sbb/anp share one, max(nbface, nboun), as in
+ * the reference's COMMON layout); release them with nft_mesh_release. */
+int nft_mesh_save(const char *path, const nft_mesh *mesh);
+int nft_mesh_load(const char *path, nft_mesh **out);
+int nft_mesh_release(nft_mesh *mesh);
+
 /* ---- measurement (no reference counterpart: the reference only times the whole run) -- */
 /* Kernel ids for the timer: 1 laplc gradient half, 2 laplc divergence half, 3 line solve,
  * 4 GCR update, 5 GCR direction, 6..9 MPDATA donor/gradient/FCT/limit. While a kernel id

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/nft.h"
@@ -78,6 +79,7 @@ struct nft_ctx {
   int line_R = 0;             // rows per lane of the coarsened scan for 64 < L <= 512, else 0 (Thomas)
   int col_L = 0;              // uniform column length with nodes numbered column by column (0: not)
   int col_on = 0;             // gathers use the columnar incidence (inc.col_L > 0)
+  int64_t steps_done = 0;     // time steps since nft_state_init (or as loaded with the state)
   double col_bytes = 0;       // bytes of the columnar tables
   double *zbuf;
   float *zbuff;
@@ -1563,6 +1565,7 @@ int nft_mpdatam(nft_ctx *c, const double *uuu, int ld, double *unk, int iflag) {
 
 int nft_state_init(nft_ctx *c) {
   if (!c) return set_err(-1, "nft_state_init: null context");
+  c->steps_done = 0;
   const int n = c->n;
   dim3 g(c->nodes_grid), b(NFT_BLOCK);
   int rc;
@@ -1597,6 +1600,7 @@ int nft_run_steps(nft_ctx *c, int nsteps, nft_run_stats *stats) {
     int rc = one_step(c, &rs);
     if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
     rs.steps++;
+    c->steps_done++;
   }
   HIPCHK(hipEventRecord(e1, c->stream));
   HIPCHK(hipEventSynchronize(e1));
@@ -2026,6 +2030,172 @@ int nft_set_learned(nft_ctx *c, int kind, double stall_ratio, int *fallbacks, in
   if (kind >= 0) { c->net.kind = kind; c->net.stall = stall_ratio; }
   if (fallbacks) *fallbacks = c->net.fallbacks;
   if (learned_iters) *learned_iters = c->net.learned_iters;
+  return 0;
+}
+
+
+// ---- full-precision state and mesh files ----------------------------------------------
+// State file (replaces the reference's f14.6 text restart, Restart_Out_old/Restart_In_old
+// :4779-4890, which loses digits): "NFTSTAT1", int64 npoin, steps, nitsm, icount, then
+// the step-boundary state as little-endian float64 blocks unk(4), frc(4), uuu(3),
+// unkdash(3), prpot(1), each [k][npoin]. Resuming from it continues the run bit for bit.
+static const char kStateMagic[8] = {'N', 'F', 'T', 'S', 'T', 'A', 'T', '1'};
+static const char *kStateBlocks[5] = {"unk", "frc", "uuu", "unkdash", "prpot"};
+static const int kStateK[5] = {4, 4, 3, 3, 1};
+
+int nft_save_state(nft_ctx *c, const char *path) {
+  if (!c || !path) return set_err(-1, "nft_save_state: bad argument");
+  const size_t n = c->n;
+  std::vector<double> buf(4 * n);
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return set_err(-1, std::string("nft_save_state: cannot open ") + path);
+  int64_t hdr[4] = {(int64_t)n, c->steps_done, (int64_t)c->st.nitsm, (int64_t)c->st.icount};
+  bool ok = std::fwrite(kStateMagic, 1, 8, f) == 8 && std::fwrite(hdr, sizeof(int64_t), 4, f) == 4;
+  for (int b = 0; b < 5 && ok; ++b) {
+    int rc = nft_get_state(c, kStateBlocks[b], buf.data(), (int)n);
+    if (rc) { std::fclose(f); return rc; }
+    ok = std::fwrite(buf.data(), sizeof(double), kStateK[b] * n, f) == kStateK[b] * n;
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? 0 : set_err(-1, std::string("nft_save_state: write failed for ") + path);
+}
+
+int nft_load_state(nft_ctx *c, const char *path) {
+  if (!c || !path) return set_err(-1, "nft_load_state: bad argument");
+  const size_t n = c->n;
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return set_err(-1, std::string("nft_load_state: cannot open ") + path);
+  char magic[8];
+  int64_t hdr[4];
+  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, kStateMagic, 8) != 0 ||
+      std::fread(hdr, sizeof(int64_t), 4, f) != 4) {
+    std::fclose(f);
+    return set_err(-1, std::string("nft_load_state: not a state file: ") + path);
+  }
+  if (hdr[0] != (int64_t)n) {
+    std::fclose(f);
+    return set_err(-1, "nft_load_state: state is for " + std::to_string(hdr[0]) + " nodes, mesh has " +
+                           std::to_string(n));
+  }
+  std::vector<double> buf(4 * n);
+  for (int b = 0; b < 5; ++b) {
+    if (std::fread(buf.data(), sizeof(double), kStateK[b] * n, f) != kStateK[b] * n) {
+      std::fclose(f);
+      return set_err(-1, std::string("nft_load_state: truncated file ") + path);
+    }
+    int rc = nft_set_state(c, kStateBlocks[b], buf.data(), (int)n);
+    if (rc) { std::fclose(f); return rc; }
+  }
+  std::fclose(f);
+  c->steps_done = hdr[1];
+  c->st.nitsm = (int)hdr[2];
+  c->st.icount = (int)hdr[3];
+  return 0;
+}
+
+int nft_steps_done(nft_ctx *c, int64_t *steps) {
+  if (!c || !steps) return set_err(-1, "nft_steps_done: bad argument");
+  *steps = c->steps_done;
+  return 0;
+}
+
+// Mesh file: binary form of the reference's mesh3d.d (:2343-2501), with the column table:
+// "NFTMESH1", int64 npoin, nedge, nbface, nboun, ncol, then iedge(nedge,2) int32,
+// sn(nedge,3) float64, kbface(nbface,2) int32, sb(nbface,3), ibpoin(nboun,2) int32,
+// sbb(nboun,3), anp(nboun,3), coord(npoin,3), vol(npoin), col_start(ncol+1) int32, all
+// column-major with leading dimension = count and 1-based node ids as in the reference.
+// Reading it is a few bulk reads instead of parsing ~10^8 text numbers.
+static const char kMeshMagic[8] = {'N', 'F', 'T', 'M', 'E', 'S', 'H', '1'};
+
+}  // extern "C"
+
+namespace {
+struct OwnedMesh {
+  nft_mesh m{};
+  std::vector<int32_t> iedge, kbface, ibpoin, col;
+  std::vector<double> sn, sb, sbb, anp, coord, vol;
+};
+
+template <typename T>
+bool write_cols(FILE *f, const T *a, int ld, int count, int k) {
+  for (int j = 0; j < k; ++j)
+    if (count > 0 && std::fwrite(a + (size_t)j * ld, sizeof(T), count, f) != (size_t)count) return false;
+  return true;
+}
+}  // namespace
+
+extern "C" {
+
+int nft_mesh_save(const char *path, const nft_mesh *m) {
+  if (!path || !m || m->npoin <= 0 || m->nedge < 0 || m->nbface < 0 || m->nboun < 0)
+    return set_err(-1, "nft_mesh_save: bad argument");
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return set_err(-1, std::string("nft_mesh_save: cannot open ") + path);
+  const int ncol = m->ncol > 0 && m->col_start ? m->ncol : 0;
+  int64_t hdr[5] = {m->npoin, m->nedge, m->nbface, m->nboun, ncol};
+  bool ok = std::fwrite(kMeshMagic, 1, 8, f) == 8 && std::fwrite(hdr, sizeof(int64_t), 5, f) == 5 &&
+            write_cols(f, m->iedge, m->ld_edge, m->nedge, 2) && write_cols(f, m->sn, m->ld_edge, m->nedge, 3) &&
+            write_cols(f, m->kbface, m->ld_face, m->nbface, 2) && write_cols(f, m->sb, m->ld_face, m->nbface, 3) &&
+            write_cols(f, m->ibpoin, m->ld_face, m->nboun, 2) && write_cols(f, m->sbb, m->ld_face, m->nboun, 3) &&
+            write_cols(f, m->anp, m->ld_face, m->nboun, 3) && write_cols(f, m->coord, m->ld_poin, m->npoin, 3) &&
+            write_cols(f, m->vol, m->ld_poin, m->npoin, 1) &&
+            (ncol == 0 || write_cols(f, m->col_start, ncol + 1, ncol + 1, 1));
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? 0 : set_err(-1, std::string("nft_mesh_save: write failed for ") + path);
+}
+
+int nft_mesh_load(const char *path, nft_mesh **out) {
+  if (!path || !out) return set_err(-1, "nft_mesh_load: bad argument");
+  *out = nullptr;
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return set_err(-1, std::string("nft_mesh_load: cannot open ") + path);
+  char magic[8];
+  int64_t h[5];
+  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, kMeshMagic, 8) != 0 ||
+      std::fread(h, sizeof(int64_t), 5, f) != 5 || h[0] <= 0 || h[0] > 2000000000LL || h[1] < 0 ||
+      h[1] > 2000000000LL || h[2] < 0 || h[2] > 2000000000LL || h[3] < 0 || h[3] > 2000000000LL || h[4] < 0 ||
+      h[4] > h[0]) {
+    std::fclose(f);
+    return set_err(-1, std::string("nft_mesh_load: not a mesh file: ") + path);
+  }
+  auto *o = new OwnedMesh();
+  const size_t N = h[0], E = h[1], B = h[2], P = h[3], C = h[4];
+  auto rd = [&](auto &v, size_t cnt) {
+    v.resize(cnt);
+    return cnt == 0 || std::fread(v.data(), sizeof(v[0]), cnt, f) == cnt;
+  };
+  bool ok = rd(o->iedge, 2 * E) && rd(o->sn, 3 * E) && rd(o->kbface, 2 * B) && rd(o->sb, 3 * B) &&
+            rd(o->ibpoin, 2 * P) && rd(o->sbb, 3 * P) && rd(o->anp, 3 * P) && rd(o->coord, 3 * N) &&
+            rd(o->vol, N) && (C == 0 || rd(o->col, C + 1));
+  std::fclose(f);
+  if (!ok) {
+    delete o;
+    return set_err(-1, std::string("nft_mesh_load: truncated file ") + path);
+  }
+  nft_mesh &m = o->m;
+  m.npoin = (int)N; m.nedge = (int)E; m.nbface = (int)B; m.nboun = (int)P;
+  m.ld_poin = (int)N; m.ld_edge = (int)E; m.ld_face = (int)std::max(B, P);
+  // the reference's layout shares one leading dimension for face and boundary arrays
+  if (B != P) {
+    const size_t ld = m.ld_face;
+    auto widen = [&](auto &v, size_t cnt, int k) {
+      std::remove_reference_t<decltype(v)> w(ld * k, 0);
+      for (int j = 0; j < k; ++j) std::copy(v.begin() + j * cnt, v.begin() + (j + 1) * cnt, w.begin() + j * ld);
+      v.swap(w);
+    };
+    widen(o->kbface, B, 2); widen(o->sb, B, 3);
+    widen(o->ibpoin, P, 2); widen(o->sbb, P, 3); widen(o->anp, P, 3);
+  }
+  m.iedge = o->iedge.data(); m.sn = o->sn.data(); m.kbface = o->kbface.data(); m.sb = o->sb.data();
+  m.ibpoin = o->ibpoin.data(); m.sbb = o->sbb.data(); m.anp = o->anp.data(); m.coord = o->coord.data();
+  m.vol = o->vol.data(); m.ncol = (int)C; m.col_start = C ? o->col.data() : nullptr;
+  *out = &o->m;
+  return 0;
+}
+
+int nft_mesh_release(nft_mesh *m) {
+  // m is the first member of the OwnedMesh nft_mesh_load allocated
+  if (m) delete reinterpret_cast<OwnedMesh *>(m);
   return 0;
 }
 

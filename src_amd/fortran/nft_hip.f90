@@ -135,6 +135,17 @@ module nft_hip
       integer(c_int), value :: nsteps
       type(nft_run_stats_t), intent(out) :: stats
     end function
+    ! full-precision state file (replaces Restart_Out_old / Restart_In_old :4779-4890)
+    integer(c_int) function nft_save_state(ctx, path) bind(C, name="nft_save_state")
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+    end function
+    integer(c_int) function nft_load_state(ctx, path) bind(C, name="nft_load_state")
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+    end function
     integer(c_int) function nft_get_state(ctx, name, host, ld) bind(C, name="nft_get_state")
       import :: c_ptr, c_int, c_double, c_char
       type(c_ptr), value :: ctx

@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:f41c3b968e33daee
+﻿!mod$ v1 sum:ea5fa0f2257ce7b9
 !need$ 0bde2ac47243ead2 i iso_c_binding
 module nft_hip
 use,intrinsic::iso_c_binding,only:c_associated
@@ -286,6 +286,22 @@ type(c_ptr),value::ctx
 integer(4),value::nsteps
 type(nft_run_stats_t),intent(out)::stats
 integer(4)::nft_run_steps
+end
+end interface
+interface
+function nft_save_state(ctx,path) bind(c,name="nft_save_state")
+import::c_ptr
+type(c_ptr),value::ctx
+character(1_8,1),intent(in)::path(1_8:*)
+integer(4)::nft_save_state
+end
+end interface
+interface
+function nft_load_state(ctx,path) bind(c,name="nft_load_state")
+import::c_ptr
+type(c_ptr),value::ctx
+character(1_8,1),intent(in)::path(1_8:*)
+integer(4)::nft_load_state
 end
 end interface
 interface

@@ -101,6 +101,12 @@ _SIGS.update({
                                        ctypes.POINTER(ctypes.c_int)]),
     "nft_net_options": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "nft_set_incidence": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "nft_save_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "nft_load_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "nft_steps_done": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "nft_mesh_save": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_void_p]),
+    "nft_mesh_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "nft_mesh_release": (ctypes.c_int, [ctypes.c_void_p]),
     "nft_copy_calibration": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
 })
 
@@ -180,27 +186,62 @@ def make_params(cfg, precon=None, sm_flag=1, itrp1=None, omega=None, eps0=None, 
     return p
 
 
+def _mesh_struct(mesh, use_columns=True):
+    """NftMesh view of a src_amd.mesh.Mesh (and the arrays it points to, kept alive)."""
+    keep = [np.ascontiguousarray(a) for a in
+            (mesh.iedge, mesh.sn, mesh.kbface, mesh.sb, mesh.ibpoin, mesh.sbb, mesh.anp,
+             mesh.coord, mesh.vol, mesh.col_start())]
+    k = keep
+    ldf = max(mesh.nbface, mesh.nboun, 1)
+    # boundary arrays share the reference's nfacesx leading dimension; pad to ldf
+    kb = np.zeros((2, ldf), np.int32); kb[:, :mesh.nbface] = mesh.kbface
+    sb = np.zeros((3, ldf)); sb[:, :mesh.nbface] = mesh.sb
+    ib = np.zeros((2, ldf), np.int32); ib[:, :mesh.nboun] = mesh.ibpoin
+    sbb = np.zeros((3, ldf)); sbb[:, :mesh.nboun] = mesh.sbb
+    anp = np.zeros((3, ldf)); anp[:, :mesh.nboun] = mesh.anp
+    keep += [kb, sb, ib, sbb, anp]
+    m = NftMesh(mesh.npoin, mesh.nedge, mesh.nbface, mesh.nboun, mesh.npoin, max(mesh.nedge, 1), ldf,
+                _p(k[0]), _p(k[1]), _p(kb), _p(sb), _p(ib), _p(sbb), _p(anp), _p(k[7]), _p(k[8]),
+                mesh.nh if use_columns else 0, _p(k[9]))
+    return m, keep
+
+
+def mesh_save(path, mesh):
+    """Binary mesh file (nft_mesh_save); host only."""
+    m, keep = _mesh_struct(mesh)
+    _chk(lib().nft_mesh_save(os.fsencode(path), ctypes.byref(m)), "nft_mesh_save")
+
+
+def mesh_load(path):
+    """Arrays of a binary mesh file (nft_mesh_load), copied out in the Mesh layouts."""
+    ptr = ctypes.c_void_p()
+    _chk(lib().nft_mesh_load(os.fsencode(path), ctypes.byref(ptr)), "nft_mesh_load")
+    try:
+        m = ctypes.cast(ptr, ctypes.POINTER(NftMesh)).contents
+
+        def cols(p, ld, count, k, dtype):
+            flat = np.ctypeslib.as_array(p, shape=(ld * k,)) if count else np.zeros(0, dtype)
+            return np.array(flat.reshape(k, ld)[:, :count] if count else np.zeros((k, 0), dtype), dtype=dtype)
+
+        out = dict(iedge=cols(m.iedge, m.ld_edge, m.nedge, 2, np.int32), sn=cols(m.sn, m.ld_edge, m.nedge, 3, np.float64),
+                   kbface=cols(m.kbface, m.ld_face, m.nbface, 2, np.int32), sb=cols(m.sb, m.ld_face, m.nbface, 3, np.float64),
+                   ibpoin=cols(m.ibpoin, m.ld_face, m.nboun, 2, np.int32), sbb=cols(m.sbb, m.ld_face, m.nboun, 3, np.float64),
+                   anp=cols(m.anp, m.ld_face, m.nboun, 3, np.float64), coord=cols(m.coord, m.ld_poin, m.npoin, 3, np.float64),
+                   vol=cols(m.vol, m.ld_poin, m.npoin, 1, np.float64)[0])
+        out["col_start"] = (np.array(np.ctypeslib.as_array(m.col_start, shape=(m.ncol + 1,)), dtype=np.int32)
+                            if m.ncol else np.zeros(0, np.int32))
+        return out
+    finally:
+        lib().nft_mesh_release(ptr)
+
+
 class Solver:
     """One device context: the mesh, set-up fields and the time-step state on the GPU."""
 
     def __init__(self, mesh, params: NftParams, use_columns=True):
         L = lib()
         self.n = mesh.npoin
-        self._keep = [np.ascontiguousarray(a) for a in
-                      (mesh.iedge, mesh.sn, mesh.kbface, mesh.sb, mesh.ibpoin, mesh.sbb, mesh.anp,
-                       mesh.coord, mesh.vol, mesh.col_start())]
-        k = self._keep
-        ldf = max(mesh.nbface, mesh.nboun, 1)
-        # boundary arrays share the reference's nfacesx leading dimension; pad to ldf
-        kb = np.zeros((2, ldf), np.int32); kb[:, :mesh.nbface] = mesh.kbface
-        sb = np.zeros((3, ldf)); sb[:, :mesh.nbface] = mesh.sb
-        ib = np.zeros((2, ldf), np.int32); ib[:, :mesh.nboun] = mesh.ibpoin
-        sbb = np.zeros((3, ldf)); sbb[:, :mesh.nboun] = mesh.sbb
-        anp = np.zeros((3, ldf)); anp[:, :mesh.nboun] = mesh.anp
-        self._keep += [kb, sb, ib, sbb, anp]
-        m = NftMesh(mesh.npoin, mesh.nedge, mesh.nbface, mesh.nboun, mesh.npoin, max(mesh.nedge, 1), ldf,
-                    _p(k[0]), _p(k[1]), _p(kb), _p(sb), _p(ib), _p(sbb), _p(anp), _p(k[7]), _p(k[8]),
-                    mesh.nh if use_columns else 0, _p(k[9]))
+        m, self._keep = _mesh_struct(mesh, use_columns)
         self.params = params
         h = ctypes.c_void_p()
         _chk(L.nft_init(ctypes.byref(m), ctypes.byref(params), ctypes.byref(h)), "nft_init")
@@ -383,6 +424,18 @@ class Solver:
         av = ctypes.c_int()
         _chk(lib().nft_set_incidence(self.h, columnar, ctypes.byref(av)), "nft_set_incidence")
         return bool(av.value)
+
+    def save_state(self, path):
+        """Full-precision state file (step-boundary state and counters)."""
+        _chk(lib().nft_save_state(self.h, os.fsencode(path)), "nft_save_state")
+
+    def load_state(self, path):
+        _chk(lib().nft_load_state(self.h, os.fsencode(path)), "nft_load_state")
+
+    def steps_done(self):
+        v = ctypes.c_int64()
+        _chk(lib().nft_steps_done(self.h, ctypes.byref(v)), "nft_steps_done")
+        return v.value
 
     def gcr_stats(self):
         st = GcrStats()
