@@ -129,7 +129,8 @@ int nft_run_steps(nft_ctx *ctx, int nsteps, nft_run_stats *stats);
 /* state blocks: "unk"(4), "frc"(4), "uuu"(3), "unkdash"(3), "prpot"(1); [k][npoin] */
 int nft_get_state(nft_ctx *ctx, const char *name, double *host, int ld);
 int nft_set_state(nft_ctx *ctx, const char *name, const double *host, int ld);
-/* set-up fields: "alpha","d1","coefw","rh0","th0","ue","line_a","line_inv","line_c" */
+/* set-up fields: "alpha","d1","coefw","rh0","th0","ue","line_a","line_inv","line_c";
+ * "courant": the per-node Courant numbers of the last nft_diagnostics call */
 int nft_get_field(nft_ctx *ctx, const char *name, double *host);
 int nft_get_gcr_stats(nft_ctx *ctx, nft_gcr_stats *stats);
 /* Incidence used by the gradient/divergence sweeps (no reference counterpart; results are
@@ -137,6 +138,26 @@ int nft_get_gcr_stats(nft_ctx *ctx, nft_gcr_stats *stats);
  * available when the mesh is extruded with identical interior levels; the default then),
  * 0 = node-sorted SELL-64. columnar < 0 only reports availability. */
 int nft_set_incidence(nft_ctx *ctx, int columnar, int *available);
+
+/* ---- diagnostics (the reference prints them after every step, :456-459) ---------------
+ * div(uuu,rhs,1) statistics of rhs*timestp (:2148-2168), courant(uuu,cour,iflag)
+ * (:1974-2094) and stats(unk) (:2847-2912), computed on the GPU with fused reductions. */
+typedef struct {
+  double div[4];        /* dvmx, dvmn, dvav, dvsd */
+  double eer, eem;      /* of the last GCR solve (/itero/) */
+  int niter, nitav;     /* niter, nitsm/icount */
+  int crnic;            /* 1-based node of the last Courant-number maximum */
+  double cour[4];       /* crnmx, crnmn, crnav, crnsd */
+  double u[4], v[4], w[4], t[4];  /* umx, umn, uav, usd ... tmx, tmn, tav, tsd */
+  int64_t step;         /* time steps done when recorded */
+} nft_diag;
+/* the three diagnostics of the current state (courant with iflag 1, 2 or 3) */
+int nft_diagnostics(nft_ctx *ctx, int courant_flag, nft_diag *out);
+/* record them on the GPU every `every` steps of nft_run_steps (0: off), keeping the last
+ * `capacity` records; nft_get_diagnostics copies the records made since the previous call
+ * (oldest first, at most max) and sets *count to their number */
+int nft_set_diagnostics(nft_ctx *ctx, int every, int capacity, int courant_flag);
+int nft_get_diagnostics(nft_ctx *ctx, int *count, nft_diag *out, int max);
 
 /* ---- files ---------------------------------------------------------------------------
  * Full-precision state file, replacing the reference's f14.6 text restart

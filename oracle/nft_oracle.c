@@ -861,8 +861,95 @@ void oc_mpdatam(oc_ctx *c, const double *uuu, double *unk, int iflag) {
   free(aun); free(aunb); free(aunbb1);
 }
 
+/* ---- per-step diagnostics (F:456-459), in the reference's loop order -------------------
+ * out[25]: div(uuu,rhs,1) statistics of rhs*timestp (F:2148-2164: dvmx, dvmn, dvav, dvsd),
+ * courant(uuu,cour,flag) (F:1974-2094: crnmx, crnmn, crnav, crnsd, ic), stats(unk)
+ * (F:2847-2912: umx, umn, uav, usd, then v, w, t). */
+void oc_diagnostics(oc_ctx *c, int flag, double *out) {
+  const oc_mesh *m = &c->m;
+  const int n = c->n;
+  const double ts = c->p.dt;
+  const double *u = c->uuu, *v = c->uuu + n, *w = c->uuu + 2 * (size_t)n;
+  double *rhs = (double *)malloc(sizeof(double) * (size_t)n);
+  double *cour = (double *)malloc(sizeof(double) * (size_t)n);
+  oc_div(c, u, v, w, rhs, 1);
+  double mx = -1.e15, mn = 1.e15, av = 0., sd = 0.;
+  for (int ip = 0; ip < n; ++ip) {
+    mx = fmax(mx, rhs[ip] * ts);
+    mn = fmin(mn, rhs[ip] * ts);
+    av = av + rhs[ip] * ts;
+  }
+  av = av / (double)n;
+  for (int ip = 0; ip < n; ++ip) { double d = rhs[ip] * ts - av; sd = sd + d * d; }
+  sd = sqrt(sd / n);
+  out[0] = mx; out[1] = mn; out[2] = av; out[3] = sd;
+  /* courant: the edge loops scatter into both nodes, then the boundary points */
+  for (int ip = 0; ip < n; ++ip) cour[ip] = 0.;
+  for (int pass = 0; pass < 2; ++pass) {
+    int ne = pass == 0 ? m->nedge : m->nbface;
+    const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+    const double *S = pass == 0 ? m->sn : m->sb;
+    for (int e = 0; e < ne; ++e) {
+      int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+      double sx = S[e], sy = S[ne + e], sz = S[2 * (size_t)ne + e];
+      double rhav = 0.5 * (c->rh0[i1] + c->rh0[i2]);
+      double aun = (u[i1] + u[i2]) * 0.5 * sx + (v[i1] + v[i2]) * 0.5 * sy + (w[i1] + w[i2]) * 0.5 * sz;
+      aun = aun / rhav;
+      if (flag == 1) { cour[i1] = cour[i1] + fabs(aun); cour[i2] = cour[i2] + fabs(aun); }
+      if (flag == 2) { cour[i1] = cour[i1] + aun; cour[i2] = cour[i2] - aun; }
+      if (flag == 3) { cour[i1] = cour[i1] + fmax(0., aun); cour[i2] = cour[i2] - fmin(0., aun); }
+    }
+  }
+  for (int b = 0; b < m->nboun; ++b) {
+    int i1 = m->ibpoin[b] - 1;
+    double sx = m->sbb[b], sy = m->sbb[m->nboun + b], sz = m->sbb[2 * (size_t)m->nboun + b];
+    double aun = u[i1] * sx + v[i1] * sy + w[i1] * sz;
+    aun = aun / c->rh0[i1];
+    if (flag == 1) cour[i1] = cour[i1] + fabs(aun);
+    if (flag == 2) cour[i1] = cour[i1] + aun;
+    if (flag == 3) cour[i1] = cour[i1] + fmax(0., aun);
+  }
+  for (int ip = 0; ip < n; ++ip) cour[ip] = cour[ip] * ts / m->vol[ip] / c->rh0[ip];
+  mx = -1.e9; mn = 1.e9; av = 0.; sd = 0.;
+  int ic = 0;
+  for (int ip = 0; ip < n; ++ip) {
+    av = av + cour[ip];
+    mx = fmax(mx, cour[ip]);
+    mn = fmin(mn, cour[ip]);
+    if (mx == cour[ip]) ic = ip + 1;
+  }
+  av = av / n;
+  for (int ip = 0; ip < n; ++ip) sd = sd + (cour[ip] - av) * (cour[ip] - av);
+  sd = sqrt(sd / n);
+  out[4] = mx; out[5] = mn; out[6] = av; out[7] = sd; out[8] = ic;
+  /* stats(unk): rh0-weighted means and deviations; tsd divides by npoin (F:2905) */
+  double rav = 0., fav[4] = {0, 0, 0, 0}, fmx[4], fmn[4], fsd[4] = {0, 0, 0, 0};
+  for (int k = 0; k < 4; ++k) { fmx[k] = -1.e10; fmn[k] = 1.e10; }
+  for (int ip = 0; ip < n; ++ip) {
+    rav = rav + c->rh0[ip];
+    for (int k = 0; k < 4; ++k) {
+      double x = c->unk[(size_t)k * n + ip];
+      fav[k] = fav[k] + c->rh0[ip] * x;
+      fmx[k] = fmax(fmx[k], x);
+      fmn[k] = fmin(fmn[k], x);
+    }
+  }
+  for (int k = 0; k < 4; ++k) fav[k] = fav[k] / rav;
+  for (int ip = 0; ip < n; ++ip)
+    for (int k = 0; k < 4; ++k) {
+      double d = c->unk[(size_t)k * n + ip] - fav[k];
+      fsd[k] = fsd[k] + c->rh0[ip] * (d * d);
+    }
+  for (int k = 0; k < 4; ++k) {
+    out[9 + 4 * k] = fmx[k]; out[10 + 4 * k] = fmn[k]; out[11 + 4 * k] = fav[k];
+    out[12 + 4 * k] = sqrt(fsd[k] / (k < 3 ? rav : (double)n));
+  }
+  free(rhs);
+  free(cour);
+}
+
 /* ---- NFT driver: mpsolver initialisation (F:177-266) and one DO ITIME step
- * (F:285-439); diagnostics (div iflg=1, courant, stats) are not part of it. */
+ * (F:285-439); the diagnostics above are separate calls. */
 double *oc_state(oc_ctx *c, const char *name) {
   if (!strcmp(name, "unk")) return c->unk;
   if (!strcmp(name, "frc")) return c->frc;

@@ -86,6 +86,8 @@ void oc_line_solve(oc_ctx *c, const double *rhs, double *z);
 /* NFT driver: state arrays are [npoin] blocks: unk[4], frc[4], uuu[3], unkdash[3], prpot */
 void oc_nft_init(oc_ctx *c);
 void oc_nft_step(oc_ctx *c);
+/* per-step diagnostics: div(uuu,rhs,1), courant(uuu,cour,flag), stats(unk) -> out[25] */
+void oc_diagnostics(oc_ctx *c, int flag, double *out);
 double *oc_state(oc_ctx *c, const char *name);
 
 #ifdef __cplusplus

@@ -75,6 +75,7 @@ def lib():
         L.oc_line_solve.argtypes = [vp, f64p, f64p]
         L.oc_nft_init.argtypes = [vp]
         L.oc_nft_step.argtypes = [vp]
+        L.oc_diagnostics.argtypes = [vp, ctypes.c_int, f64p]
         _lib = L
     return _lib
 
@@ -209,3 +210,10 @@ class Oracle:
     def step(self, n=1):
         for _ in range(n):
             lib().oc_nft_step(self.h)
+
+    def diagnostics(self, flag=1):
+        """The reference's per-step diagnostics of the current state: [dvmx, dvmn, dvav,
+        dvsd, crnmx, crnmn, crnav, crnsd, ic, umx, umn, uav, usd, ..., tmx, tmn, tav, tsd]."""
+        out = np.zeros(25)
+        lib().oc_diagnostics(self.h, flag, _p(out))
+        return out

@@ -80,6 +80,11 @@ struct nft_ctx {
   int col_L = 0;              // uniform column length with nodes numbered column by column (0: not)
   int col_on = 0;             // gathers use the columnar incidence (inc.col_L > 0)
   int64_t steps_done = 0;     // time steps since nft_state_init (or as loaded with the state)
+  // diagnostics recorded by nft_run_steps: device ring of records, host side of each record
+  int diag_every = 0, diag_cap = 0, diag_flag = 1;
+  double *diag_ring = nullptr, *diag_now = nullptr;
+  std::vector<nft_diag> diag_meta;   // eer, eem, niter, nitav, step of each pending record
+  std::vector<int> diag_slot;
   double col_bytes = 0;       // bytes of the columnar tables
   double *zbuf;
   float *zbuff;
@@ -1055,6 +1060,65 @@ int mpdata_step(nft_ctx *c) {
   return mpdata_dev<double, 4>(c, c->uuu[0], c->uuu[1], c->uuu[2], c->unk, fl);
 }
 
+// ---- diagnostics ----------------------------------------------------------------------
+constexpr int kDiagLen = 25;   // div 4, cour 4, crnic, u 4, v 4, w 4, t 4
+
+template <int NF>
+int diag_stats(nft_ctx *c, const DiagSpec &d) {
+  hipLaunchKernelGGL(k_diag_mom<NF>, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, d, c->partials);
+  FinalOp op{};
+  op.kind = FK_DIAG_MOM; op.K = 3 * NF + 1; op.diag = d;
+  for (int f = 0; f < NF; ++f) op.ismax |= (1u << (3 * f)) | (1u << (3 * f + 1));
+  int rc = final_launch(c, c->vec_grid, op);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_diag_var<NF>, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, d, c->partials);
+  FinalOp op2{};
+  op2.kind = FK_DIAG_VAR; op2.K = NF + 1; op2.ismax = 1u << NF; op2.diag = d;
+  return final_launch(c, c->vec_grid, op2);
+}
+
+// the reference's per-step diagnostics of the current state into rec[kDiagLen] (device)
+int diag_dev(nft_ctx *c, int flag, double *rec) {
+  int rc;
+  if ((rc = div_dev(c, c->uuu[0], c->uuu[1], c->uuu[2], c->t[4], 1, nullptr))) return rc;
+  DiagSpec d{};
+  d.nf = 1; d.x[0] = c->t[4]; d.scale = c->P.dt; d.n = (double)c->n;
+  d.init_mx = -1.e15; d.init_mn = 1.e15; d.out = rec;
+  if ((rc = diag_stats<1>(c, d))) return rc;
+  dim3 g(c->gather_grid), b(NFT_BLOCK);
+#define NFT_COUR(F, COL)                                                                                        \
+  hipLaunchKernelGGL((k_courant<F, COL>), g, b, 0, c->stream, c->inc, c->S, c->uuu[0], c->uuu[1], c->uuu[2], c->nc, \
+                     c->P.dt, c->t[5])
+  if (c->col_on) {
+    if (flag == 2) NFT_COUR(2, true); else if (flag == 3) NFT_COUR(3, true); else NFT_COUR(1, true);
+  } else {
+    if (flag == 2) NFT_COUR(2, false); else if (flag == 3) NFT_COUR(3, false); else NFT_COUR(1, false);
+  }
+#undef NFT_COUR
+  HIPCHK(hipGetLastError());
+  DiagSpec cd{};
+  cd.nf = 1; cd.x[0] = c->t[5]; cd.scale = 1.0; cd.n = (double)c->n;
+  cd.init_mx = -1.e9; cd.init_mn = 1.e9; cd.argmax = 1; cd.out = rec + 4;
+  if ((rc = diag_stats<1>(c, cd))) return rc;
+  DiagSpec sd{};
+  sd.nf = 4; sd.w = c->rh0; sd.scale = 1.0; sd.n = (double)c->n;
+  sd.init_mx = -1.e10; sd.init_mn = 1.e10; sd.out = rec + 9;
+  for (int f = 0; f < 4; ++f) { sd.x[f] = c->unk[f]; sd.mean_by_w[f] = 1; sd.sd_by_w[f] = f < 3 ? 1 : 0; }
+  return diag_stats<4>(c, sd);
+}
+
+void diag_unpack(const double *r, nft_diag &d) {
+  for (int k = 0; k < 4; ++k) { d.div[k] = r[k]; d.cour[k] = r[4 + k]; }
+  d.crnic = (int)r[8];
+  for (int k = 0; k < 4; ++k) { d.u[k] = r[9 + k]; d.v[k] = r[13 + k]; d.w[k] = r[17 + k]; d.t[k] = r[21 + k]; }
+}
+
+void diag_meta_fill(nft_ctx *c, nft_diag &d) {
+  d.eer = c->st.eer; d.eem = c->st.eem; d.niter = c->st.niter;
+  d.nitav = c->st.nitsm / std::max(c->st.icount, 1);
+  d.step = c->steps_done;
+}
+
 int one_step(nft_ctx *c, nft_run_stats *rs) {
   StepArgs a = step_args(c);
   dim3 g(c->nodes_grid), b(NFT_BLOCK);
@@ -1447,6 +1511,7 @@ int nft_get_field(nft_ctx *c, const char *name, double *host) {
   else if (s == "th0") d = c->th0;
   else if (s == "ue") d = c->ue;
   else if (s == "beta") { host[0] = c->beta1; return 0; }
+  else if (s == "courant") d = c->t[5];   // Courant numbers of the last nft_diagnostics call
   if (!d) return set_err(-1, "nft_get_field: unknown field " + s);
   HIPCHK(hipMemcpy(host, d, c->n * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
@@ -1601,6 +1666,15 @@ int nft_run_steps(nft_ctx *c, int nsteps, nft_run_stats *stats) {
     if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
     rs.steps++;
     c->steps_done++;
+    if (c->diag_every > 0 && c->steps_done % c->diag_every == 0) {
+      const int slot = (int)(c->diag_meta.size() % c->diag_cap);
+      rc = diag_dev(c, c->diag_flag, c->diag_ring + (size_t)slot * kDiagLen);
+      if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
+      nft_diag m{};
+      diag_meta_fill(c, m);
+      c->diag_meta.push_back(m);
+      c->diag_slot.push_back(slot);
+    }
   }
   HIPCHK(hipEventRecord(e1, c->stream));
   HIPCHK(hipEventSynchronize(e1));
@@ -2010,6 +2084,55 @@ int nft_net_options(nft_ctx *c, int objective, int linear) {
   if (linear)
     for (int l = 0; l < 6; ++l) HIPCHK(hipMemsetAsync(n.w[2 * l + 1], 0, n.N[l] * sizeof(float), c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int nft_diagnostics(nft_ctx *c, int courant_flag, nft_diag *out) {
+  if (!c || !out || courant_flag < 1 || courant_flag > 3) return set_err(-1, "nft_diagnostics: bad argument");
+  int rc;
+  if (!c->diag_now && (rc = dalloc(c, &c->diag_now, kDiagLen))) return rc;
+  if ((rc = diag_dev(c, courant_flag, c->diag_now))) return rc;
+  double r[kDiagLen];
+  HIPCHK(hipMemcpyAsync(r, c->diag_now, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *out = nft_diag{};
+  diag_unpack(r, *out);
+  diag_meta_fill(c, *out);
+  return 0;
+}
+
+int nft_set_diagnostics(nft_ctx *c, int every, int capacity, int courant_flag) {
+  if (!c || every < 0 || (every > 0 && capacity < 1) || courant_flag < 1 || courant_flag > 3)
+    return set_err(-1, "nft_set_diagnostics: bad argument");
+  if (every > 0 && capacity > c->diag_cap) {
+    int rc;
+    if ((rc = dalloc(c, &c->diag_ring, (size_t)capacity * kDiagLen))) return rc;
+    c->diag_cap = capacity;
+  }
+  c->diag_every = every;
+  c->diag_flag = courant_flag;
+  c->diag_meta.clear();
+  c->diag_slot.clear();
+  return 0;
+}
+
+int nft_get_diagnostics(nft_ctx *c, int *count, nft_diag *out, int max) {
+  if (!c || !count || (max > 0 && !out)) return set_err(-1, "nft_get_diagnostics: bad argument");
+  const int total = (int)c->diag_meta.size();
+  const int first = std::max(0, total - c->diag_cap);   // older records were overwritten
+  const int n = std::min(total - first, std::max(max, 0));
+  *count = n;
+  if (n > 0) {
+    std::vector<double> ring((size_t)c->diag_cap * kDiagLen);
+    HIPCHK(hipMemcpyAsync(ring.data(), c->diag_ring, ring.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < n; ++k) {
+      out[k] = c->diag_meta[first + k];
+      diag_unpack(ring.data() + (size_t)c->diag_slot[first + k] * kDiagLen, out[k]);
+    }
+  }
+  c->diag_meta.clear();
+  c->diag_slot.clear();
   return 0;
 }
 

@@ -659,7 +659,23 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_norms(int n, const double *a, dou
 // Grid-level end of every fused reduction: one block of 1024 threads sums the per-block
 // partials in a fixed order (thread t: blocks t, t+1024, ...; wavefront trees; the 16
 // wavefront results in order) and applies the solver's scalar update.
-enum FinalKind { FK_DIV = 0, FK_UPDATE = 1, FK_DIR = 2, FK_NORMS = 3 };
+enum FinalKind { FK_DIV = 0, FK_UPDATE = 1, FK_DIR = 2, FK_NORMS = 3, FK_DIAG_MOM = 4, FK_DIAG_VAR = 5 };
+
+// Diagnostic statistics of up to 4 node fields (the reference's div(...,1), courant and
+// stats print blocks): per field max, min, mean and standard deviation, written to out[].
+#define NFT_DIAG_MAXF 4
+struct DiagSpec {
+  int nf;                            // fields
+  const double *x[NFT_DIAG_MAXF];    // field values
+  const double *w;                   // weight (null: 1 per node)
+  double scale;                      // each value is x*scale (1: unscaled)
+  double n;                          // node count (as the reference's float(npoin))
+  double init_mx, init_mn;           // the reference's starting values of its max/min loops
+  int mean_by_w[NFT_DIAG_MAXF];      // mean denominator: sum of weights (1) or n (0)
+  int sd_by_w[NFT_DIAG_MAXF];        // sd denominator: sum of weights (1) or n (0)
+  int argmax;                        // also the 1-based index of the last maximum of field 0
+  double *out;                       // [4*nf] (+1 for argmax): max, min, mean, sd per field
+};
 
 struct FinalOp {
   int kind;
@@ -677,6 +693,7 @@ struct FinalOp {
   int ctl, it, itmn;
   double eps;
   double *hflag;
+  DiagSpec diag;   // FK_DIAG_MOM / FK_DIAG_VAR
 };
 
 #define NFT_FIN_THREADS 1024
@@ -724,6 +741,28 @@ __global__ void __launch_bounds__(NFT_FIN_THREADS) k_reduce_final(const double *
     for (int j = 1; j < NW; ++j) a = ((op.ismax >> k) & 1) ? fmax(a, sh[k * NW + j]) : a + sh[k * NW + j];
     acc[k] = a;
   }
+  if (op.kind == FK_DIAG_MOM) {
+    // acc: per field (max, min, sum of w*x), then the sum of weights
+    const DiagSpec &d = op.diag;
+    const double W = d.w ? acc[3 * d.nf] : d.n;
+    for (int f = 0; f < d.nf; ++f) {
+      d.out[4 * f + 0] = fmax(d.init_mx, acc[3 * f + 0]);
+      d.out[4 * f + 1] = fmin(d.init_mn, -acc[3 * f + 1]);
+      d.out[4 * f + 2] = acc[3 * f + 2] / (d.mean_by_w[f] ? W : d.n);
+      d.out[4 * f + 3] = W;   // kept for the second pass (overwritten by the sd)
+    }
+    return;
+  }
+  if (op.kind == FK_DIAG_VAR) {
+    // acc: per field sum of w*(x-mean)^2, then (argmax) the largest index attaining the max
+    const DiagSpec &d = op.diag;
+    for (int f = 0; f < d.nf; ++f) {
+      const double W = d.out[4 * f + 3];
+      d.out[4 * f + 3] = sqrt(acc[f] / (d.sd_by_w[f] ? W : d.n));
+    }
+    if (d.argmax) d.out[4 * d.nf] = acc[d.nf];
+    return;
+  }
   if (op.kind == FK_DIV) {
     gcr_finalize(op.dots, acc, scal, op.epa);
   } else if (op.kind == FK_UPDATE) {
@@ -751,6 +790,115 @@ __global__ void __launch_bounds__(NFT_FIN_THREADS) k_reduce_final(const double *
     scal[op.sum_slot] = acc[0];
     scal[op.max_slot] = acc[1];
   }
+}
+
+// ------------------------------------------------------------------------------------
+// Diagnostics (the reference prints them after every step, :456-459)
+// ------------------------------------------------------------------------------------
+// First pass: per field max, min and sum of w*x (x = value*scale), and the sum of weights.
+// Sums run over the grid-stride order of the vector kernels (block partials, then
+// k_reduce_final); max and min are exact in any order.
+template <int NF>
+__device__ __forceinline__ unsigned mins_mask() {
+  unsigned m = 0;
+  for (int f = 0; f < NF; ++f) m |= 1u << (3 * f + 1);
+  return m;
+}
+
+template <int NF>
+__global__ void __launch_bounds__(NFT_BLOCK) k_diag_mom(int n, DiagSpec d, double *partials) {
+  constexpr int K = 3 * NF + 1;
+  __shared__ double sh[K * NFT_WPB];
+  double v[K];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) { v[3 * f] = -1.0e300; v[3 * f + 1] = 1.0e300; v[3 * f + 2] = 0.0; }
+  v[3 * NF] = 0.0;
+  for (int i = blockIdx.x * NFT_BLOCK + threadIdx.x; i < n; i += gridDim.x * NFT_BLOCK) {
+    const double w = d.w ? d.w[i] : 1.0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const double x = d.scale == 1.0 ? d.x[f][i] : d.x[f][i] * d.scale;
+      v[3 * f] = fmax(v[3 * f], x);
+      v[3 * f + 1] = fmin(v[3 * f + 1], x);
+      v[3 * f + 2] = v[3 * f + 2] + (d.w ? w * x : x);
+    }
+    v[3 * NF] = v[3 * NF] + w;
+  }
+  unsigned ismax = 0;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) ismax |= 1u << (3 * f);
+  // the min is reduced as the max of -x
+#pragma unroll
+  for (int f = 0; f < NF; ++f) v[3 * f + 1] = -v[3 * f + 1];
+  block_partial<K>(v, ismax | mins_mask<NF>(), partials, sh);
+}
+
+// Second pass: per field sum of w*(x-mean)^2 and, for the argmax field, the largest
+// 1-based index whose value equals the max (the reference's running `if (crnmx.eq.cour(ip))
+// ic=ip` ends there).
+template <int NF>
+__global__ void __launch_bounds__(NFT_BLOCK) k_diag_var(int n, DiagSpec d, double *partials) {
+  constexpr int K = NF + 1;
+  __shared__ double sh[K * NFT_WPB];
+  double v[K];
+  double mean[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) { v[f] = 0.0; mean[f] = d.out[4 * f + 2]; }
+  v[NF] = 0.0;
+  const double mx = d.out[0];
+  for (int i = blockIdx.x * NFT_BLOCK + threadIdx.x; i < n; i += gridDim.x * NFT_BLOCK) {
+    const double w = d.w ? d.w[i] : 1.0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const double x = d.scale == 1.0 ? d.x[f][i] : d.x[f][i] * d.scale;
+      const double dv = x - mean[f];
+      const double q = dv * dv;
+      v[f] = v[f] + (d.w ? w * q : q);
+    }
+    if (d.argmax) {
+      const double x0 = d.scale == 1.0 ? d.x[0][i] : d.x[0][i] * d.scale;
+      if (x0 == mx) v[NF] = fmax(v[NF], (double)(i + 1));
+    }
+  }
+  block_partial<K>(v, 1u << NF, partials, sh);
+}
+
+// courant(uuu,cour,iflag) (:1974-2094): per node the sum over its edges, in the reference's
+// edge order, of |a_e| (iflag 1), +-a_e (2) or the signed upwind parts (3), with
+// a_e = ((u_i1+u_i2)*0.5*Sx + (v_i1+v_i2)*0.5*Sy + (w_i1+w_i2)*0.5*Sz) / (0.5*(rho_i1+rho_i2)),
+// then the boundary points' (u*sbbx + v*sbby + w*sbbz)/rho, then *dt/vol/rho.
+template <int FLAG, bool COL>
+__global__ void __launch_bounds__(NFT_BLOCK) k_courant(Incidence g, EdgeS<double> S, const double *u,
+                                                       const double *v, const double *w, NodeCoef nc, double dt,
+                                                       double *cour) {
+  Lane L;
+  if (!lane_setup(g, L)) return;
+  const int i = L.i;
+  ColLane C{};
+  if (COL) col_setup(g, L.valid ? i : g.n - 1, C);
+  if (!L.valid) return;
+  const double ui = u[i], vi = v[i], wi = w[i], ri = rho(nc, i);
+  double acc = 0.;
+  for (int m = 0; m < L.deg; ++m) {
+    int j, e;
+    slot_decode<COL>(C, slot_raw<COL>(g, L, C, m), j, e);
+    const int ea = e >= 0 ? e : ~e;
+    double a = (ui + u[j]) * 0.5 * S.sx[ea] + (vi + v[j]) * 0.5 * S.sy[ea] + (wi + w[j]) * 0.5 * S.sz[ea];
+    if (!nc.rh0_one) a = a / (0.5 * (ri + rho(nc, j)));
+    if (FLAG == 1) acc = acc + fabs(a);
+    else if (FLAG == 2) acc = e >= 0 ? acc + a : acc - a;
+    else acc = e >= 0 ? acc + fmax(0., a) : acc - fmin(0., a);
+  }
+  for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
+    double a = ui * g.bp_sbb[b] + vi * g.bp_sbb[g.nbp + b] + wi * g.bp_sbb[2 * g.nbp + b];
+    if (!nc.rh0_one) a = a / ri;
+    if (FLAG == 1) acc = acc + fabs(a);
+    else if (FLAG == 2) acc = acc + a;
+    else acc = acc + fmax(0., a);
+  }
+  double c = acc * dt / nc.vol[i];
+  if (!nc.rh0_one) c = c / ri;
+  cour[i] = c;
 }
 
 // ------------------------------------------------------------------------------------

@@ -48,6 +48,16 @@ module nft_hip
     real(c_double) :: ms_total, ms_gcr, ms_mpdata
   end type
 
+  ! per-step diagnostics (nft_diag in nft.h)
+  type, bind(C) :: nft_diag_t
+    real(c_double) :: div(4)
+    real(c_double) :: eer, eem
+    integer(c_int) :: niter, nitav, crnic
+    real(c_double) :: cour(4)
+    real(c_double) :: u(4), v(4), w(4), t(4)
+    integer(c_int64_t) :: step
+  end type
+
   type(c_ptr), save :: nft_context = c_null_ptr
   real(c_double), save :: nft_dt = 0.0d0
 
@@ -153,6 +163,25 @@ module nft_hip
       real(c_double), intent(out) :: host(*)
       integer(c_int), value :: ld
     end function
+    integer(c_int) function nft_set_state(ctx, name, host, ld) bind(C, name="nft_set_state")
+      import :: c_ptr, c_int, c_double, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: name(*)
+      real(c_double), intent(in) :: host(*)
+      integer(c_int), value :: ld
+    end function
+    integer(c_int) function nft_diagnostics(ctx, courant_flag, d) bind(C, name="nft_diagnostics")
+      import :: c_ptr, c_int, nft_diag_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: courant_flag
+      type(nft_diag_t), intent(out) :: d
+    end function
+    integer(c_int) function nft_get_field(ctx, name, host) bind(C, name="nft_get_field")
+      import :: c_ptr, c_int, c_double, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: name(*)
+      real(c_double), intent(out) :: host(*)
+    end function
     integer(c_int) function nft_get_gcr_stats(ctx, stats) bind(C, name="nft_get_gcr_stats")
       import :: c_ptr, c_int, nft_gcr_stats_t
       type(c_ptr), value :: ctx
@@ -161,6 +190,23 @@ module nft_hip
   end interface
 
 contains
+
+  ! the reference's diagnostic lines (div :2165-2167, courant :2090-2092, stats :2906-2910);
+  ! parts: 1 div, 2 courant, 4 stats (sums of these)
+  subroutine nft_print_diag(d, parts)
+    type(nft_diag_t), intent(in) :: d
+    integer, intent(in) :: parts
+    if (iand(parts, 1) /= 0) print 201, d%div, d%eer, d%eem, d%niter, d%nitav
+    if (iand(parts, 2) /= 0) then
+      write(6,*) d%crnic
+      print 202, d%cour
+    end if
+    if (iand(parts, 4) /= 0) print 203, d%u, d%v, d%w, d%t
+201 format(1x,'dvmx,dvmn,dvav,dvsd:',4e11.4/1x,' eer,eem:',2e11.4,1x,'niter,nitav:',4i4)
+202 format(1x,'crnmx,crnmn,crnav,crnsd:',4e11.4)
+203 format(1x,'umx,umn,uav,usd:',4e11.4/1x,'vmx,vmn,vav,vsd:',4e11.4/ &
+           1x,'wmx,wmn,wav,wsd:',4e11.4/1x,'tmx,tmn,tav,tsd:',4e11.4)
+  end subroutine
 
   subroutine nft_check(rc, what)
     integer(c_int), intent(in) :: rc
@@ -339,4 +385,29 @@ subroutine mpdatam(uuu, unk, iflag)
   real(8) :: uuu(npoinx,3), unk(*)
   integer :: iflag
   call nft_check(nft_mpdatam(nft_context, uuu, npoinx, unk, iflag), 'mpdatam')
+end subroutine
+
+! courant(uuu,cour,iflag) (reference :1974-2094) on the GPU: uploads uuu, computes the
+! Courant numbers and their statistics, returns cour and prints the reference's lines.
+subroutine courant(uuu, cour, iflag)
+  use nft_hip
+  implicit none
+  real(8) :: uuu(npoinx,3), cour(*)
+  integer :: iflag
+  type(nft_diag_t) :: d
+  call nft_check(nft_set_state(nft_context, 'uuu'//c_null_char, uuu, npoinx), 'courant')
+  call nft_check(nft_diagnostics(nft_context, iflag, d), 'courant')
+  call nft_check(nft_get_field(nft_context, 'courant'//c_null_char, cour), 'courant')
+  call nft_print_diag(d, 2)
+end subroutine
+
+! stats(unk) (reference :2847-2912) on the GPU: uploads unk and prints the reference's lines.
+subroutine stats(unk)
+  use nft_hip
+  implicit none
+  real(8) :: unk(npoinx,4)
+  type(nft_diag_t) :: d
+  call nft_check(nft_set_state(nft_context, 'unk'//c_null_char, unk, npoinx), 'stats')
+  call nft_check(nft_diagnostics(nft_context, 1, d), 'stats')
+  call nft_print_diag(d, 4)
 end subroutine

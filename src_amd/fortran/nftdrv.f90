@@ -28,6 +28,8 @@ program nftdrv
   type(nft_mesh_t) :: m
   type(nft_params_t) :: p
   type(nft_run_stats_t) :: rs
+  type(nft_diag_t) :: dg
+  real(8) :: ms_total
   type(nft_gcr_stats_t) :: gs
   character(len=64) :: arg
   integer :: nitsm_total, step
@@ -126,14 +128,21 @@ program nftdrv
 
   call nft_check(nft_init(m, p, nft_context), 'nft_init')
   call nft_check(nft_state_init(nft_context), 'nft_state_init')
-  call nft_check(nft_get_gcr_stats(nft_context, gs), 'stats')
-  write(*,'(a,i6)') ' initial projection niter', gs%niter
+  ! the reference's diagnostics after the initialisation (:270-272) and every step
+  ! (:456-459), computed on the GPU
+  call nft_check(nft_diagnostics(nft_context, 1, dg), 'diagnostics')
+  call nft_print_diag(dg, 7)
+  write(*,*) 'end of initialisation, the time loop starts'
   nitsm_total = 0
+  ms_total = 0.0d0
   do step = 1, ntime
     call nft_check(nft_run_steps(nft_context, 1, rs), 'nft_run_steps')
-    call nft_check(nft_get_gcr_stats(nft_context, gs), 'stats')
-    write(*,'(a,i6,a,i6,a,f10.3)') ' step', step, ' niter', gs%niter, ' ms', rs%ms_total
+    ms_total = ms_total + rs%ms_total
+    write(*,*) step, p%dt, p%dt
+    call nft_check(nft_diagnostics(nft_context, 1, dg), 'diagnostics')
+    call nft_print_diag(dg, 7)
   end do
+  write(*,'(a,f12.3,a)') ' GPU time in the time loop ', ms_total, ' ms'
   allocate(state(npoin, 8))
   call nft_check(nft_get_state(nft_context, 'unk'//c_null_char, state(1,1), npoin), 'state')
   call nft_check(nft_get_state(nft_context, 'uuu'//c_null_char, state(1,5), npoin), 'state')

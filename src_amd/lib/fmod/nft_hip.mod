@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:ea5fa0f2257ce7b9
+﻿!mod$ v1 sum:8f8a07c1bd035d15
 !need$ 0bde2ac47243ead2 i iso_c_binding
 module nft_hip
 use,intrinsic::iso_c_binding,only:c_associated
@@ -148,6 +148,20 @@ integer(4)::gcr_unconverged
 real(8)::ms_total
 real(8)::ms_gcr
 real(8)::ms_mpdata
+end type
+type,bind(c)::nft_diag_t
+real(8)::div(1_8:4_8)
+real(8)::eer
+real(8)::eem
+integer(4)::niter
+integer(4)::nitav
+integer(4)::crnic
+real(8)::cour(1_8:4_8)
+real(8)::u(1_8:4_8)
+real(8)::v(1_8:4_8)
+real(8)::w(1_8:4_8)
+real(8)::t(1_8:4_8)
+integer(8)::step
 end type
 type(c_ptr),save::nft_context
 real(8),save::nft_dt
@@ -315,6 +329,35 @@ integer(4)::nft_get_state
 end
 end interface
 interface
+function nft_set_state(ctx,name,host,ld) bind(c,name="nft_set_state")
+import::c_ptr
+type(c_ptr),value::ctx
+character(1_8,1),intent(in)::name(1_8:*)
+real(8),intent(in)::host(1_8:*)
+integer(4),value::ld
+integer(4)::nft_set_state
+end
+end interface
+interface
+function nft_diagnostics(ctx,courant_flag,d) bind(c,name="nft_diagnostics")
+import::c_ptr
+import::nft_diag_t
+type(c_ptr),value::ctx
+integer(4),value::courant_flag
+type(nft_diag_t),intent(out)::d
+integer(4)::nft_diagnostics
+end
+end interface
+interface
+function nft_get_field(ctx,name,host) bind(c,name="nft_get_field")
+import::c_ptr
+type(c_ptr),value::ctx
+character(1_8,1),intent(in)::name(1_8:*)
+real(8),intent(out)::host(1_8:*)
+integer(4)::nft_get_field
+end
+end interface
+interface
 function nft_get_gcr_stats(ctx,stats) bind(c,name="nft_get_gcr_stats")
 import::c_ptr
 import::nft_gcr_stats_t
@@ -324,6 +367,10 @@ integer(4)::nft_get_gcr_stats
 end
 end interface
 contains
+subroutine nft_print_diag(d,parts)
+type(nft_diag_t),intent(in)::d
+integer(4),intent(in)::parts
+end
 subroutine nft_check(rc,what)
 integer(4),intent(in)::rc
 character(*,1),intent(in)::what

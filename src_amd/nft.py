@@ -42,6 +42,19 @@ class NftParams(ctypes.Structure):
                 ("irl", ctypes.c_int), ("dom", ctypes.c_double * 6), ("precision", ctypes.c_int)]
 
 
+class NftDiag(ctypes.Structure):
+    _fields_ = [("div", ctypes.c_double * 4), ("eer", ctypes.c_double), ("eem", ctypes.c_double),
+                ("niter", ctypes.c_int), ("nitav", ctypes.c_int), ("crnic", ctypes.c_int),
+                ("cour", ctypes.c_double * 4), ("u", ctypes.c_double * 4), ("v", ctypes.c_double * 4),
+                ("w", ctypes.c_double * 4), ("t", ctypes.c_double * 4), ("step", ctypes.c_int64)]
+
+    def row(self):
+        """[dvmx, dvmn, dvav, dvsd, eer, eem, niter, nitav, ic, crnmx, crnmn, crnav, crnsd,
+        umx, umn, uav, usd, ..., tmx, tmn, tav, tsd] (the reference's print order)."""
+        return np.array(list(self.div) + [self.eer, self.eem, self.niter, self.nitav, self.crnic] +
+                        list(self.cour) + list(self.u) + list(self.v) + list(self.w) + list(self.t))
+
+
 class GcrStats(ctypes.Structure):
     _fields_ = [("niter", ctypes.c_int), ("nitsm", ctypes.c_int), ("icount", ctypes.c_int),
                 ("converged", ctypes.c_int), ("eer", ctypes.c_double), ("eem", ctypes.c_double),
@@ -101,6 +114,10 @@ _SIGS.update({
                                        ctypes.POINTER(ctypes.c_int)]),
     "nft_net_options": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "nft_set_incidence": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "nft_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(NftDiag)]),
+    "nft_set_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "nft_get_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(NftDiag),
+                                           ctypes.c_int]),
     "nft_save_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "nft_load_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "nft_steps_done": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
@@ -424,6 +441,21 @@ class Solver:
         av = ctypes.c_int()
         _chk(lib().nft_set_incidence(self.h, columnar, ctypes.byref(av)), "nft_set_incidence")
         return bool(av.value)
+
+    def diagnostics(self, courant_flag=1):
+        """The reference's per-step diagnostics of the current state (NftDiag)."""
+        d = NftDiag()
+        _chk(lib().nft_diagnostics(self.h, courant_flag, ctypes.byref(d)), "nft_diagnostics")
+        return d
+
+    def set_diagnostics(self, every, capacity=1024, courant_flag=1):
+        _chk(lib().nft_set_diagnostics(self.h, every, capacity, courant_flag), "nft_set_diagnostics")
+
+    def get_diagnostics(self, max_records=1024):
+        out = (NftDiag * max_records)()
+        n = ctypes.c_int()
+        _chk(lib().nft_get_diagnostics(self.h, ctypes.byref(n), out, max_records), "nft_get_diagnostics")
+        return [out[k] for k in range(n.value)]
 
     def save_state(self, path):
         """Full-precision state file (step-boundary state and counters)."""

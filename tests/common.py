@@ -55,3 +55,30 @@ def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64); b = np.asarray(b, dtype=np.float64)
     nb = np.linalg.norm(b)
     return np.linalg.norm(a - b) / nb if nb > 0 else np.linalg.norm(a - b)
+
+
+def _e11(text, k):
+    """k fixed-width e11.4 fields (the reference's `4e11.4` prints run them together)."""
+    text = text.rstrip("\n")
+    return [float(text[11 * j: 11 * (j + 1)]) for j in range(k)]
+
+
+def parse_diagnostics(stdout):
+    """The reference's per-step diagnostic blocks (F:270-272 after the initialisation,
+    F:456-459 after every step) as rows [dvmx, dvmn, dvav, dvsd, eer, eem, niter, nitav,
+    ic, crnmx, crnmn, crnav, crnsd, umx, umn, uav, usd, vmx, ..., tmx, tmn, tav, tsd]."""
+    lines = stdout.splitlines()
+    rows = []
+    for k, line in enumerate(lines):
+        if "dvmx,dvmn,dvav,dvsd:" not in line:
+            continue
+        row = _e11(line.split(":", 1)[1], 4)
+        ee = lines[k + 1]
+        row += _e11(ee.split("eer,eem:", 1)[1], 2)
+        row += [float(x) for x in ee.split("niter,nitav:", 1)[1].split()[:2]]
+        row.append(float(lines[k + 2].split()[0]))
+        row += _e11(lines[k + 3].split(":", 1)[1], 4)
+        for j in range(4):
+            row += _e11(lines[k + 4 + j].split(":", 1)[1], 4)
+        rows.append(row)
+    return np.array(rows)

@@ -1,13 +1,14 @@
 """Writes the golden fixtures of tests/golden from the flang-compiled reference.
 
 Run in the build container (it needs the reference source to build oracle/_ref):
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--tiny-only]
 It builds the reference with oracle/build_ref.sh, writes the reference's input files for
 generated meshes, runs
   * the harness in `ops` mode: the reference's own derivative, laplc, bcv, prfr, div,
     precon, mpdatam and gcrk on seeded vectors over the tiny mesh (tiny_ops.npz), and
   * the reference program itself (`run` mode) for 1 and 10 steps on the tiny mesh
-    (tiny_run.npz) and for 1, 10 and 100 steps on the 40,000-node mesh (small_run.npz),
+    (tiny_run.npz, with the diagnostics it prints after the initialisation and every
+    step) and for 1, 10 and 100 steps on the 40,000-node mesh (small_run.npz),
 and stores inputs and outputs. Only numbers are stored.
 """
 from __future__ import annotations
@@ -24,9 +25,11 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from src_amd import mesh as M  # noqa: E402
 from src_amd.refio import write_reference_inputs  # noqa: E402
+from common import parse_diagnostics  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 
@@ -154,10 +157,12 @@ def run_steps(exe, m, cfg, d, ntime):
     st = read_restart(d, m.npoin)
     niter = [int(a) for a, _ in NITER_RE.findall(r.stdout)]
     st["niter"] = np.array(niter, dtype=np.int32)  # init solve first, then one per step
+    st["diag"] = parse_diagnostics(r.stdout)       # init block first, then one per step
     return st
 
 
 def main():
+    only_tiny = "--tiny-only" in sys.argv
     tcfg = tiny_config()
     exe = build_ref(tcfg)
     m = M.generate(tcfg.mesh)
@@ -172,6 +177,8 @@ def main():
                 runs[f"s{nt}_{k}"] = val
         np.savez_compressed(os.path.join(OUT, "tiny_run.npz"), mesh_checksum=mesh_checksum(m), **runs)
     print("tiny fixtures written")
+    if only_tiny:
+        return
 
     scfg = small_config()
     exe = build_ref(scfg)

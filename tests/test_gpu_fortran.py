@@ -7,7 +7,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from common import mesh_for, tiny_cfg, rel_l2
+from common import mesh_for, parse_diagnostics, tiny_cfg, rel_l2
 from oracle import Oracle
 from src_amd import nft
 from src_amd.refio import write_reference_inputs
@@ -46,3 +46,25 @@ def test_fortran_driver_matches_ctypes_path(tmp_path):
     o.init()
     o.step(nsteps)
     assert np.array_equal(st[0:4], o.state("unk"))
+    # The diagnostics it prints (computed on the GPU): the block after the initialisation
+    # against the reference's own print (same trajectory); the blocks after each step
+    # against the oracle in the GPU's summation order (in this absolute-tolerance mode the
+    # GCR iterates, and so the later blocks, depend on the summation order).
+    rows = parse_diagnostics(r.stdout)
+    assert rows.shape == (nsteps + 1, 29)
+    ref0 = np.load(os.path.join(ROOT, "tests", "golden", "tiny_run.npz"))["s10_diag"][0]
+    o2 = Oracle(m, cfg, dot_order=1)
+    o2.init()
+    for k in range(nsteps + 1):
+        if k:
+            o2.step()
+        d = o2.diagnostics(1)
+        st2 = o2.stats()
+        ora = np.concatenate([d[:4], [st2.eer, st2.eem, st2.niter, st2.nitsm // max(st2.icount, 1), d[8]], d[4:8], d[9:]])
+        for ref in ([ref0, ora] if k == 0 else [ora]):
+            for j in range(29):
+                x, y = rows[k, j], ref[j]
+                if j in (6, 7, 8):
+                    assert x == y, (k, j, x, y)
+                else:
+                    assert (x == 0 if y == 0 else abs(x - y) <= 1.2e-4 * 10.0 ** (np.floor(np.log10(abs(y))) + 1)), (k, j, x, y)

@@ -302,3 +302,50 @@ def test_init_rejects_corrupt_mesh():
     m2.iedge = bad
     with pytest.raises(nft.NftError, match="out of range"):
         nft.Solver(m2, nft.make_params(tiny_cfg()))
+
+
+def _diag_close(mine, ref):
+    """Extrema and indices exact; means and deviations (tree vs sequential sums) to 1e-10
+    of the field's magnitude."""
+    mine, ref = np.asarray(mine), np.asarray(ref)
+    groups = [(0, 4), (4, 8)] + [(9 + 4 * k, 13 + 4 * k) for k in range(4)]
+    for a, b in groups:
+        assert mine[a] == ref[a] and mine[a + 1] == ref[a + 1], (a, mine[a:b], ref[a:b])
+        scale = max(abs(ref[a]), abs(ref[a + 1]), 1e-300)
+        assert abs(mine[a + 2] - ref[a + 2]) <= 1e-10 * scale, (a, mine[a + 2], ref[a + 2])
+        assert abs(mine[a + 3] - ref[a + 3]) <= 1e-10 * scale, (a, mine[a + 3], ref[a + 3])
+    assert mine[8] == ref[8]
+
+
+@pytest.mark.parametrize("key,columnar", [("tiny", 1), ("tiny", 0), ("small", 1)])
+def test_diagnostics_match_oracle(key, columnar):
+    """div(uuu,rhs,1), courant (iflag 1, 2, 3) and stats computed on the GPU after the
+    initialisation and after each step, against the oracle's restatement of the reference's
+    loops; the records made inside nft_run_steps equal the direct calls bit for bit."""
+    m = mesh_for(key)
+    cfg = tiny_cfg() if key == "tiny" else small_cfg(**LINE)
+    s, o = solver(m, cfg), Oracle(m, cfg, dot_order=1)
+    s.set_incidence(columnar)
+    s.state_init()
+    o.init()
+    direct = []
+    for step in range(4):
+        for flag in (1, 2, 3):
+            d = s.diagnostics(flag)
+            r = o.diagnostics(flag)
+            mine = np.concatenate([np.array(d.div), np.array(d.cour), [d.crnic], d.u, d.v, d.w, d.t])
+            _diag_close(mine, r)
+        st = o.stats()
+        assert d.niter == st.niter and d.nitav == st.nitsm // max(st.icount, 1) and d.eer == st.eer
+        if step == 0:
+            s.set_diagnostics(every=1, capacity=8, courant_flag=1)
+        else:
+            direct.append(s.diagnostics(1).row())
+        if step < 3:
+            s.run_steps(1)
+            o.step()
+    recs = s.get_diagnostics()
+    assert [r.step for r in recs] == [1, 2, 3]
+    for r, d in zip(recs, direct):
+        assert np.array_equal(r.row(), d)
+    s.close()

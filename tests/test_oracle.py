@@ -86,6 +86,36 @@ def test_time_steps_bitwise():
         assert its == g[f"s{nt}_niter"].tolist()
 
 
+def _printed(x, ref):
+    """x agrees with the reference's e11.4 print `ref` to within 0.6 of its last digit."""
+    if ref == 0.0:
+        return abs(x) < 1e-300
+    e = np.floor(np.log10(abs(ref))) + 1
+    return abs(x - ref) <= 0.6e-4 * 10.0 ** e
+
+
+def test_diagnostics_against_reference_prints():
+    """div(uuu,rhs,1), courant and stats after the initialisation and after each of 10
+    steps (tiny mesh): the oracle's values against the reference's printed blocks."""
+    g = load("tiny_run.npz")
+    rows = g["s10_diag"]
+    m = mesh_for("tiny")
+    o = Oracle(m, tiny_cfg())
+    o.init()
+    for k in range(rows.shape[0]):
+        if k:
+            o.step()
+        d = o.diagnostics(1)
+        st = o.stats()
+        mine = np.concatenate([d[:4], [st.eer, st.eem, st.niter, st.nitsm // max(st.icount, 1), d[8]], d[4:8], d[9:]])
+        ref = rows[k]
+        for j in range(29):
+            if j in (6, 7, 8):
+                assert mine[j] == ref[j], (k, j, mine[j], ref[j])
+            else:
+                assert _printed(mine[j], ref[j]), (k, j, mine[j], ref[j])
+
+
 def test_small_mesh_steps_against_reference():
     """40,000-node mesh, point-Jacobi GCR(3): 1 and 10 steps against the reference."""
     g = load("small_run.npz")
