@@ -40,6 +40,26 @@ def production_config(column_order="grid"):
                        sponge_depth=3000.0, sponge_tau=300.0)
 
 
+def stress_config(column_order="grid"):
+    """The 16.8M-node stress mesh (131,072 columns x 128 stretched levels, Fr = 0.1) with
+    the same solver settings; a secondary workload, not the headline metric."""
+    from src_amd import mesh as M
+    c = M.CONFIGS["stiff"]
+    return M.RunConfig(c.name, M.stiff_spec(column_order), u0=c.u0, bvf=c.bvf, dt=c.dt, steps=c.steps, lord=4,
+                       eps0=1e-8, exit_mode=1, itr=400, precon=2, itrp1=1, omega=1.0,
+                       sponge_depth=3000.0, sponge_tau=300.0)
+
+
+WORKLOADS = {
+    "production": ("production mesh: 65,536 jittered columns x 64 levels over 200 km x 200 km x 20 km "
+                   "({nodes} nodes, {edges} edges), hill h0=1 km, U=10 m/s, N=0.01 1/s (Fr=1), dt=20 s; MPDATA + "
+                   "2 x GCR(4) to 1e-8 relative residual with the vertical line preconditioner, FP64"),
+    "stress": ("stress mesh: 131,072 jittered columns x 128 levels stretched 50-400 m over 300 km x 300 km x 25 km "
+               "({nodes} nodes, {edges} edges), hill h0=2 km, U=5 m/s, N=0.025 1/s (Fr=0.1), dt=15 s; MPDATA + "
+               "2 x GCR(4) to 1e-8 relative residual with the vertical line preconditioner, FP64"),
+}
+
+
 def timed_run(mesh, cfg, precision, steps, warmup, profile=True):
     from src_amd import nft
     s = nft.Solver(mesh, nft.make_params(cfg, precision=precision))
@@ -117,8 +137,8 @@ def measured_traffic(kernel):
         return None
 
 
-def cpu_baseline(mesh, cfg, iters_per_solve):
-    """Serial FP64 oracle on the production mesh: one MPDATA call per field, one prfr,
+def cpu_baseline(mesh, cfg, iters_per_solve, mesh_name="production"):
+    """Serial FP64 oracle on the bench mesh: one MPDATA call per field, one prfr,
     div, line preconditioner and laplc, and one GCR cycle are timed and scaled to a step
     with the GPU run's mean iteration count."""
     import numpy as np
@@ -158,7 +178,7 @@ def cpu_baseline(mesh, cfg, iters_per_solve):
     step = t_mp + 2 * (t_setup + iters_per_solve * t_iter + t_prfr)
     o.close(); o2.close()
     return dict(value=1.0 / step, unit="timesteps/s", cores=1, kind="port",
-                sample=(f"FP64 C oracle, 1 thread, production mesh ({n} nodes): mpdatam x4 {t_mp:.2f} s, "
+                sample=(f"FP64 C oracle, 1 thread, {mesh_name} mesh ({n} nodes): mpdatam x4 {t_mp:.2f} s, "
                         f"GCR iteration {t_iter:.2f} s (precon {t_pre:.2f} s + laplc {t_lap:.2f} s + "
                         f"vector work), scaled to one step with {iters_per_solve:.1f} iterations per "
                         f"solve (2 solves) -> {step:.1f} s/step; on identical work the oracle takes 0.80x "
@@ -173,6 +193,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--learned-train-steps", type=int, default=100)
+    ap.add_argument("--mesh", default="production", choices=("production", "stress"),
+                    help="workload: the production mesh (headline metric) or the 16.8M-node stress mesh")
     ap.add_argument("--column-order", default="grid", choices=("grid", "hilbert"),
                     help="numbering of the production mesh's columns")
     ap.add_argument("--no-roofline", action="store_true",
@@ -187,7 +209,7 @@ def main():
     from src_amd import mesh as M
     from src_amd import nft
 
-    cfg = production_config(args.column_order)
+    cfg = (production_config if args.mesh == "production" else stress_config)(args.column_order)
     t0 = time.perf_counter()
     mesh = M.generate(cfg.mesh)
     log(f"mesh: {mesh.npoin} nodes, {mesh.nedge} edges + {mesh.nbface} semi-edges, "
@@ -230,10 +252,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": ("production mesh: 65,536 jittered columns x 64 levels over 200 km x 200 km x 20 km "
-                         f"({mesh.npoin} nodes, {mesh.nedge + mesh.nbface} edges), hill h0=1 km, U=10 m/s, "
-                         "N=0.01 1/s (Fr=1), dt=20 s; MPDATA + 2 x GCR(4) to 1e-8 relative residual with "
-                         "the vertical line preconditioner, FP64"),
+            "workload": WORKLOADS[args.mesh].format(nodes=mesh.npoin, edges=mesh.nedge + mesh.nbface),
             "nodes": mesh.npoin, "edges": mesh.nedge + mesh.nbface, "gcr_order": cfg.lord,
             "tolerance": cfg.eps0, "preconditioner": "vertical line (tridiagonal, wavefront scans of the Thomas factors)",
             "precision": "fp64",
@@ -244,7 +263,8 @@ def main():
         "state_finite": finite,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": (round(measured_traffic(kname)) if kname and measured_traffic(kname) else None),
+                     "traffic": (round(measured_traffic(kname))
+                                 if kname and args.mesh == "production" and measured_traffic(kname) else None),
                      "launches": nl, "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": int(kbytes)},
     }
     if not args.no_variants:
@@ -260,7 +280,7 @@ def main():
         line["variants"]["learned"] = learned_run(mesh, cfg, mk, args.learned_train_steps)
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
-        line["cpu_baseline"] = cpu_baseline(mesh, cfg, iters)
+        line["cpu_baseline"] = cpu_baseline(mesh, cfg, iters, args.mesh)
         log(f"cpu baseline sample took {time.perf_counter() - t0:.1f} s")
     print(json.dumps(line), flush=True)
 

@@ -933,6 +933,7 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
       if ((rc = read_scal(c, SC_DVMX, 2))) return rc;
       dvmx = c->h_scal[SC_DVMX];
       double rl2 = c->h_scal[SC_RL2];
+      if (std::isnan(rl2)) goto done;   // NaN residual: stop, reported as not converged
       if (dvmx <= eps && it >= itmn) { converged = 1; goto done; }
       if (rl2 >= rl20) {
         // stagnation: with the learned correction active, drop it and continue with the

@@ -770,7 +770,8 @@ __global__ void __launch_bounds__(NFT_FIN_THREADS) k_reduce_final(const double *
     scal[SC_RL2] = acc[1];
     if (op.ctl) {
       double done = 0.0;
-      if (acc[0] <= op.eps && op.it >= op.itmn) done = 1.0;
+      if (!(acc[1] == acc[1])) done = 2.0;   // NaN residual: stop, reported as not converged
+      else if (acc[0] <= op.eps && op.it >= op.itmn) done = 1.0;
       else if (acc[1] >= scal[SC_RL20]) done = 2.0;
       else {
         scal[SC_RL20] = fmax(acc[1], op.epa);

@@ -230,3 +230,34 @@ def test_learned_preconditioner_end_to_end():
     assert sl.gcr_unconverged == 0 and sp.gcr_unconverged == 0
     for k in range(4):
         assert rel_l2(ul[k], up[k]) <= 1e-5
+
+
+def test_garbage_network_falls_back_to_line_preconditioner():
+    """Failure injection: network weights replaced by large random values. The learned
+    correction then ruins the first directions; the solve must drop it (fallbacks > 0)
+    and still reach the tolerance with the line preconditioner, ending at the plain
+    run's solution to the solver tolerance."""
+    m = mesh_for("small")
+    cfg = small_cfg(**LINE)
+
+    def run(garbage):
+        s = nft.Solver(m, nft.make_params(cfg))
+        if garbage:
+            s.net_init(hidden=128, seed=2, max_batch=1)
+            rng = np.random.default_rng(0)
+            s.net_params((50.0 * rng.normal(size=s.net_info()["nparams"])).astype(np.float32))
+            s.set_learned(1, stall=0.99)
+        s.state_init()
+        st = s.run_steps(2)
+        fb = s.set_learned(-1)[0]
+        unk = s.state("unk")
+        s.close()
+        return st, fb, unk
+
+    sp, _, up = run(False)
+    sg, fb, ug = run(True)
+    print("fallbacks", fb, "iterations", sp.gcr_iters, sg.gcr_iters)
+    assert fb > 0
+    assert sg.gcr_unconverged == 0
+    for k in range(4):
+        assert rel_l2(ug[k], up[k]) <= 1e-5

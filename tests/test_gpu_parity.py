@@ -349,3 +349,20 @@ def test_diagnostics_match_oracle(key, columnar):
     for r, d in zip(recs, direct):
         assert np.array_equal(r.row(), d)
     s.close()
+
+
+def test_nan_seeded_state_is_reported_not_hung():
+    """A NaN planted in the state (failure injection from the host): the device exit test
+    never passes, every solve stops at its iteration limit and is reported unconverged,
+    and the run returns instead of hanging."""
+    m = mesh_for("tiny")
+    cfg = tiny_cfg(itr=5)
+    s = solver(m, cfg)
+    s.state_init()
+    unk = s.state("unk")
+    unk[0, m.npoin // 2] = np.nan
+    s.set_state("unk", unk)
+    st = s.run_steps(1)
+    assert st.gcr_solves == 2 and st.gcr_unconverged == 2
+    assert not np.isfinite(s.state("unk")).all()
+    s.close()
