@@ -139,6 +139,33 @@ int nft_get_gcr_stats(nft_ctx *ctx, nft_gcr_stats *stats);
  * 0 = node-sorted SELL-64. columnar < 0 only reports availability. */
 int nft_set_incidence(nft_ctx *ctx, int columnar, int *available);
 
+/* ---- multigrid branch of precon (mg_flag=1, reference :618-748) --------------------
+ * After the fine sweeps, a three-level V-cycle: the fine residual restricted to coarse
+ * level 1 (restriction_unstr :757-822), point relaxation there (laplc_c :1356-1446), its
+ * residual restricted to level 2 and relaxed (laplc_cc :1450-1535), the corrections
+ * prolonged back (prolongation_unstr :828-940) with relaxation on level 1 and itrp2 fine
+ * sweeps. The coarse meshes are the reference's mesh3d_c.d / mesh3d_cc.d (COMMON /edg_c/
+ * /wge_c/ ... as read; the reference reads the level-2 sn y/z columns from level 1's array,
+ * :2467-2468, so pass its COMMON arrays as they are to reproduce it), match1 the 1-based
+ * fine node of each level-1 node and match2 the level-1 node of each level-2 node
+ * (match1.d / match2.d). Coarse levels use the point step of sm_flag with d2/d3 and
+ * betaunstr2/3 (compute_d_unstr, beta_timestep_unstr :953-1284) computed here. */
+typedef struct {
+  int mg_flag;                        /* 1: run the V-cycle inside precon */
+  int itrp2, itrp_c1, itrp_c2, itrp_cc;  /* sweeps as in input_mg_v3.d */
+  int transfer;  /* 0: restriction/prolongation index the fine array by the neighbour
+                    counter, as the reference does (:817, :913); 1: by the neighbour list */
+} nft_multigrid;
+/* Coarse levels and V-cycle settings. With m1 == NULL only mg_flag and the sweep counts
+ * change (the levels of an earlier call stay). */
+int nft_set_multigrid(nft_ctx *ctx, const nft_mesh *m1, const int32_t *match1,
+                      const nft_mesh *m2, const int32_t *match2, const nft_multigrid *mg);
+/* per-level set-up values: "d" (d2 / d3), "beta" (1 value), "rh0", "pstep"; *count = length */
+int nft_multigrid_field(nft_ctx *ctx, int level, const char *name, double *host, int *count);
+/* the V-cycle's pieces on host arrays: op 0 restriction_unstr(in fine, out level),
+ * op 1 prolongation_unstr(in level, out finer), op 2 laplc on the level (laplc_c/_cc) */
+int nft_multigrid_op(nft_ctx *ctx, int op, int level, const double *in, double *out, int impfl);
+
 /* ---- diagnostics (the reference prints them after every step, :456-459) ---------------
  * div(uuu,rhs,1) statistics of rhs*timestp (:2148-2168), courant(uuu,cour,iflag)
  * (:1974-2094) and stats(unk) (:2847-2912), computed on the GPU with fused reductions. */
@@ -178,7 +205,8 @@ int nft_mesh_release(nft_mesh *mesh);
 
 /* ---- measurement (no reference counterpart: the reference only times the whole run) -- */
 /* Kernel ids for the timer: 1 laplc gradient half, 2 laplc divergence half, 3 line solve,
- * 4 GCR update, 5 GCR direction, 6..9 MPDATA donor/gradient/FCT/limit. While a kernel id
+ * 4 GCR update, 5 GCR direction, 6..9 MPDATA donor/gradient/FCT/limit, 10 the coarse-level
+ * part of the multigrid V-cycle (a chain of kernels; bytes_per_launch 0). While a kernel id
  * is selected, every launch of that kernel is bracketed by HIP events on the context's
  * stream. bytes_per_launch is the kernel's algorithmic traffic: every array it reads or
  * writes counted once at its stored width (incidence slots actually used, edge metrics,

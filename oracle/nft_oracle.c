@@ -35,6 +35,16 @@ struct oc_ctx {
   /* launch geometry of the HIP path (nft_init) for dot_order = 1 */
   int nchunk, nblocks, xcd_per, ggrid, vgrid;
   double *part;
+  /* neighbour lists in neigh_definition's order (F:2726-2767): interior edges, then the
+   * boundary-face edges, each in input order */
+  int32_t *nb_ptr, *nb_idx;
+  /* multigrid branch of precon (F:618-748): coarse levels 1 and 2, each a context of its
+   * own mesh; match[l] = 0-based node of level l-1 (0 = fine) for each node of level l,
+   * imatch[l] the inverse over level l-1 (-1 where no coarse node sits) */
+  struct oc_ctx *lv[2];
+  int32_t *lv_match[2], *lv_imatch[2];
+  int mg_on, mg_itrp2, mg_c1, mg_c2, mg_cc, mg_transfer;
+  double *mg_w[12];
 };
 
 static double amax1(double a, double b) { return a > b ? a : b; }
@@ -43,6 +53,31 @@ static double amin1(double a, double b) { return a < b ? a : b; }
 static double *zalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
 
 static void build_line(oc_ctx *c);
+
+/* compute_d_unstr (F:1112-1284) over the context's own mesh with density rh0_u */
+static void compute_d(oc_ctx *c, const double *rh0u, double *d) {
+  const oc_mesh *m = &c->m;
+  int n = c->n;
+  double *dix = c->s[0], *diy = c->s[1], *diz = c->s[2];
+  for (int i = 0; i < n; ++i) { dix[i] = 0.; diy[i] = 0.; diz[i] = 0.; }
+  for (int pass = 0; pass < 2; ++pass) {
+    int ne = pass == 0 ? m->nedge : m->nbface;
+    const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+    const double *S = pass == 0 ? m->sn : m->sb;
+    for (int e = 0; e < ne; ++e) {
+      int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+      double sx = S[e], sy = S[ne + e], sz = S[2 * (size_t)ne + e];
+      dix[i1] = dix[i1] + sx * sx * rh0u[i2] / m->vol[i2];
+      dix[i2] = dix[i2] + sx * sx * rh0u[i1] / m->vol[i1];
+      diy[i1] = diy[i1] + sy * sy * rh0u[i2] / m->vol[i2];
+      diy[i2] = diy[i2] + sy * sy * rh0u[i1] / m->vol[i1];
+      diz[i1] = diz[i1] + sz * sz * rh0u[i2] / m->vol[i2];
+      diz[i2] = diz[i2] + sz * sz * rh0u[i1] / m->vol[i1];
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    d[i] = -(0.5 * c->p.dt * (dix[i] + diy[i] + diz[i]) / (4 * m->vol[i] * rh0u[i]));
+}
 
 oc_ctx *oc_create(const oc_mesh *m, const oc_params *p) {
   oc_ctx *c = (oc_ctx *)calloc(1, sizeof(oc_ctx));
@@ -141,26 +176,28 @@ oc_ctx *oc_create(const oc_mesh *m, const oc_params *p) {
     c->beta1 = 0.5 * ddx * ddx * 1.0;
   }
   /* compute_d_unstr(1) (F:1112-1284) */
+  compute_d(c, c->rh0, c->d1);
+  /* neigh_definition (F:2726-2767) */
   {
-    double *dix = c->s[0], *diy = c->s[1], *diz = c->s[2];
-    for (int i = 0; i < n; ++i) { dix[i] = 0.; diy[i] = 0.; diz[i] = 0.; }
+    c->nb_ptr = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
     for (int pass = 0; pass < 2; ++pass) {
       int ne = pass == 0 ? m->nedge : m->nbface;
       const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
-      const double *S = pass == 0 ? m->sn : m->sb;
+      for (int e = 0; e < ne; ++e) { c->nb_ptr[ie[e]]++; c->nb_ptr[ie[ne + e]]++; }
+    }
+    for (int i = 0; i < n; ++i) c->nb_ptr[i + 1] += c->nb_ptr[i];
+    c->nb_idx = (int32_t *)calloc((size_t)c->nb_ptr[n] + 1, sizeof(int32_t));
+    int32_t *fill = (int32_t *)calloc((size_t)n, sizeof(int32_t));
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? m->nedge : m->nbface;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
       for (int e = 0; e < ne; ++e) {
         int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
-        double sx = S[e], sy = S[ne + e], sz = S[2 * (size_t)ne + e];
-        dix[i1] = dix[i1] + sx * sx * c->rh0[i2] / m->vol[i2];
-        dix[i2] = dix[i2] + sx * sx * c->rh0[i1] / m->vol[i1];
-        diy[i1] = diy[i1] + sy * sy * c->rh0[i2] / m->vol[i2];
-        diy[i2] = diy[i2] + sy * sy * c->rh0[i1] / m->vol[i1];
-        diz[i1] = diz[i1] + sz * sz * c->rh0[i2] / m->vol[i2];
-        diz[i2] = diz[i2] + sz * sz * c->rh0[i1] / m->vol[i1];
+        c->nb_idx[c->nb_ptr[i1] + fill[i1]++] = i2;
+        c->nb_idx[c->nb_ptr[i2] + fill[i2]++] = i1;
       }
     }
-    for (int i = 0; i < n; ++i)
-      c->d1[i] = -(0.5 * p->dt * (dix[i] + diy[i] + diz[i]) / (4 * m->vol[i] * c->rh0[i]));
+    free(fill);
   }
   /* coefw (F:250-252); set here so laplc/precon can be called stand-alone */
   for (int i = 0; i < n; ++i) {
@@ -259,6 +296,9 @@ void oc_destroy(oc_ctx *c) {
   for (size_t i = 0; i < sizeof(all) / sizeof(all[0]); ++i) free(all[i]);
   for (int i = 0; i < 16; ++i) free(c->s[i]);
   for (int i = 0; i < 4; ++i) free(c->gb[i]);
+  free(c->nb_ptr); free(c->nb_idx);
+  for (int l = 0; l < 2; ++l) { oc_destroy(c->lv[l]); free(c->lv_match[l]); free(c->lv_imatch[l]); }
+  for (int i = 0; i < 12; ++i) free(c->mg_w[i]);
   free(c);
 }
 
@@ -595,34 +635,200 @@ void oc_line_solve(oc_ctx *c, const double *rhs, double *z) {
   }
 }
 
-/* precon(rhs,p,r,iprc,elas,impfl), F:482-612 fine-grid smoothing path (mg_flag=0).
- * precon=2 replaces the point step pstep*(r-rhs) by the line solve M^-1 (rhs-r). */
+/* point-relaxation step per node (precon F:587-599 on the fine mesh, F:632-653 on the
+ * coarse ones): d and beta of the level */
+static void pstep_fill(int sm_flag, const double *d, double beta, int n, double *pstep) {
+  for (int i = 0; i < n; ++i) {
+    if (sm_flag == 1) pstep[i] = (-1.) / d[i];
+    else if (sm_flag == 2) pstep[i] = beta;
+    else pstep[i] = beta / (1 - d[i] * beta);
+  }
+}
+
+/* one fine-mesh smoothing sweep of precon (F:607-612, F:740-745): r = L p (skipped on the
+ * first sweep from p = 0, where it is zero), then the point step p += omega*pstep*(r-rhs),
+ * or with precon=2 the line step p += omega * M^-1 (rhs - r) */
+static void fine_sweep(oc_ctx *c, const double *rhs, double *p, double *r, int impfl, int first) {
+  int n = c->n;
+  double om = c->p.omega;
+  if (!first) oc_laplc(c, p, r, impfl);
+  if (c->p.precon == 1) {
+    const double *pstep = c->s[15];
+    for (int i = 0; i < n; ++i) p[i] = p[i] + om * pstep[i] * (r[i] - rhs[i]);
+  } else {
+    double *res = c->s[15], *z = c->s[2];
+    for (int i = 0; i < n; ++i) res[i] = rhs[i] - r[i];
+    oc_line_solve(c, res, z);
+    for (int i = 0; i < n; ++i) p[i] = p[i] + om * z[i];
+  }
+}
+
+/* restriction_unstr(scalar,scalar_c,flg), F:757-822: the mean over the fine node's
+ * neighbours of the edge averages. transfer 0 indexes the fine array by the neighbour
+ * counter itself, as F:817 does (scalar(inei), inei = 1..count); transfer 1 by the
+ * neighbour list. f = the finer level's context. */
+static void mg_restrict(const oc_ctx *f, const int32_t *match, int nc, const double *src, double *out,
+                        int transfer) {
+  for (int ic = 0; ic < nc; ++ic) {
+    int ip = match[ic];
+    int cnt = f->nb_ptr[ip + 1] - f->nb_ptr[ip];
+    double sc = 0.;
+    for (int q = 0; q < cnt; ++q) {
+      int j = transfer ? f->nb_idx[f->nb_ptr[ip] + q] : q;
+      sc = sc + (src[ip] + src[j]) / 2;
+    }
+    out[ic] = sc / cnt;
+  }
+}
+
+/* prolongation_unstr(scalar_c,scalar,flg), F:828-940: injection at the matched nodes, the
+ * mean of the matched neighbours elsewhere (0 where there is none). A node counts as
+ * matched when its injected value differs from the reference's sentinel -99999; the
+ * neighbours are the counter values 1..count with transfer 0 (F:913-914) and the
+ * neighbour list with transfer 1. */
+static void mg_prolong(const oc_ctx *f, const int32_t *imatch, const double *coarse, double *out,
+                       int transfer) {
+  const double sentinel = -99999.;
+  int n = f->n;
+  double *aux = f->s[14];
+  for (int ip = 0; ip < n; ++ip) { out[ip] = 0.; aux[ip] = sentinel; }
+  for (int ip = 0; ip < n; ++ip)
+    if (imatch[ip] >= 0) { out[ip] = coarse[imatch[ip]]; aux[ip] = coarse[imatch[ip]]; }
+  for (int ip = 0; ip < n; ++ip) {
+    if (aux[ip] != sentinel) continue;
+    int cnt = f->nb_ptr[ip + 1] - f->nb_ptr[ip];
+    double sc = out[ip], filled = 0.;
+    for (int q = 0; q < cnt; ++q) {
+      int j = transfer ? f->nb_idx[f->nb_ptr[ip] + q] : q;
+      if (aux[j] != sentinel) { sc = sc + out[j]; filled = filled + 1; }
+    }
+    out[ip] = filled != 0. ? sc / filled : sc;
+  }
+}
+
+/* the V-cycle of precon's multigrid branch, F:618-748 */
+static void mg_cycle(oc_ctx *c, const double *rhs, double *p, double *r, int impfl) {
+  oc_ctx *c1 = c->lv[0], *c2 = c->lv[1];
+  int n = c->n, n1 = c1->n, n2 = c2->n, tr = c->mg_transfer;
+  double om = c->p.omega;
+  double *rrhs = c->mg_w[0], *e = c->mg_w[1];
+  double *p_c = c->mg_w[2], *r_c = c->mg_w[3], *rrhs_c = c->mg_w[4], *rout_c = c->mg_w[5];
+  double *e_c = c->mg_w[6], *ps_c = c->mg_w[7];
+  double *p_cc = c->mg_w[8], *r_cc = c->mg_w[9], *rrhs_cc = c->mg_w[10], *ps_cc = c->mg_w[11];
+  pstep_fill(c->p.sm_flag, c1->d1, c1->beta1, n1, ps_c);
+  pstep_fill(c->p.sm_flag, c2->d1, c2->beta1, n2, ps_cc);
+  for (int i = 0; i < n1; ++i) { p_c[i] = 0.; r_c[i] = 0.; e_c[i] = 0.; }
+  for (int i = 0; i < n2; ++i) { p_cc[i] = 0.; r_cc[i] = 0.; }
+  /* fine residual, restricted */
+  oc_laplc(c, p, r, impfl);
+  for (int i = 0; i < n; ++i) rrhs[i] = rhs[i] - r[i];
+  mg_restrict(c, c->lv_match[0], n1, rrhs, rrhs_c, tr);
+  /* coarse 1: relax, residual, restrict */
+  for (int i = 0; i < n1; ++i) p_c[i] = -(om * ps_c[i] * rrhs_c[i]);
+  for (int it = 1; it <= c->mg_c1; ++it) {
+    oc_laplc(c1, p_c, r_c, impfl);
+    for (int i = 0; i < n1; ++i) p_c[i] = p_c[i] + om * ps_c[i] * (r_c[i] - rrhs_c[i]);
+  }
+  oc_laplc(c1, p_c, r_c, impfl);
+  for (int i = 0; i < n1; ++i) rout_c[i] = rrhs_c[i] - r_c[i];
+  mg_restrict(c1, c->lv_match[1], n2, rout_c, rrhs_cc, tr);
+  /* coarse 2: relax */
+  for (int i = 0; i < n2; ++i) p_cc[i] = -(rrhs_cc[i] * ps_cc[i] * om);
+  for (int it = 1; it <= c->mg_cc; ++it) {
+    oc_laplc(c2, p_cc, r_cc, impfl);
+    for (int i = 0; i < n2; ++i) p_cc[i] = p_cc[i] + om * ps_cc[i] * (r_cc[i] - rrhs_cc[i]);
+  }
+  /* back up: prolong, correct, relax */
+  mg_prolong(c1, c->lv_imatch[1], p_cc, e_c, tr);
+  for (int i = 0; i < n1; ++i) p_c[i] = p_c[i] + e_c[i];
+  for (int it = 1; it <= c->mg_c2; ++it) {
+    oc_laplc(c1, p_c, r_c, impfl);
+    for (int i = 0; i < n1; ++i) p_c[i] = p_c[i] + om * ps_c[i] * (r_c[i] - rrhs_c[i]);
+  }
+  mg_prolong(c, c->lv_imatch[0], p_c, e, tr);
+  for (int i = 0; i < n; ++i) p[i] = p[i] + e[i];
+  for (int it = 1; it <= c->mg_itrp2; ++it) fine_sweep(c, rhs, p, r, impfl, 0);
+}
+
+/* precon(rhs,p,r,iprc,elas,impfl), F:482-748: fine-grid smoothing, then the multigrid
+ * V-cycle when oc_set_multigrid has turned it on (mg_flag=1).
+ * precon=2 replaces the point step pstep*(r-rhs) by the line solve M^-1 (rhs-r) on the
+ * fine mesh; the coarse meshes keep the point step. */
 void oc_precon(oc_ctx *c, const double *rhs, double *p, double *r, int impfl) {
   int n = c->n;
   if (c->p.precon == 0) { for (int i = 0; i < n; ++i) p[i] = rhs[i]; return; }
   for (int i = 0; i < n; ++i) { p[i] = 0.; r[i] = 0.; }
-  double om = c->p.omega;
-  if (c->p.precon == 1) {
-    double *pstep = c->s[15];
-    for (int i = 0; i < n; ++i) {
-      if (c->p.sm_flag == 1) pstep[i] = (-1.) / c->d1[i];
-      else if (c->p.sm_flag == 2) pstep[i] = c->beta1;
-      else pstep[i] = c->beta1 / (1 - c->d1[i] * c->beta1);
-    }
-    for (int it = 1; it <= c->p.itrp1; ++it) {
-      if (it > 1) oc_laplc(c, p, r, impfl);
-      for (int i = 0; i < n; ++i) p[i] = p[i] + om * pstep[i] * (r[i] - rhs[i]);
-    }
-  } else {
-    double *res = c->s[15], *z = c->s[2];
-    for (int it = 1; it <= c->p.itrp1; ++it) {
-      if (it > 1) oc_laplc(c, p, r, impfl);
-      for (int i = 0; i < n; ++i) res[i] = rhs[i] - r[i];
-      oc_line_solve(c, res, z);
-      for (int i = 0; i < n; ++i) p[i] = p[i] + om * z[i];
+  if (c->p.precon == 1) pstep_fill(c->p.sm_flag, c->d1, c->beta1, n, c->s[15]);
+  for (int it = 1; it <= c->p.itrp1; ++it) fine_sweep(c, rhs, p, r, impfl, it == 1);
+  if (c->mg_on) mg_cycle(c, rhs, p, r, impfl);
+}
+
+int oc_set_multigrid(oc_ctx *c, const oc_mesh *m1, const int32_t *match1, const oc_mesh *m2,
+                     const int32_t *match2, int itrp2, int itrp_c1, int itrp_c2, int itrp_cc,
+                     int transfer) {
+  if (!m1) { c->mg_on = 0; return 0; }
+  const oc_mesh *ms[2] = {m1, m2};
+  const int32_t *mt[2] = {match1, match2};
+  int nf[2] = {c->n, m1->npoin};
+  for (int l = 0; l < 2; ++l) {
+    if (ms[l]->npoin > nf[l]) return -1;  /* coarse arrays are indexed inside the fine ones */
+    for (int i = 0; i < ms[l]->npoin; ++i)
+      if (mt[l][i] < 1 || mt[l][i] > nf[l]) return -1;
+  }
+  for (int l = 0; l < 2; ++l) {
+    oc_destroy(c->lv[l]); free(c->lv_match[l]); free(c->lv_imatch[l]);
+    c->lv[l] = oc_create(ms[l], &c->p);
+    int nc = ms[l]->npoin;
+    c->lv_match[l] = (int32_t *)malloc(sizeof(int32_t) * (size_t)nc);
+    c->lv_imatch[l] = (int32_t *)malloc(sizeof(int32_t) * (size_t)nf[l]);
+    for (int i = 0; i < nf[l]; ++i) c->lv_imatch[l][i] = -1;
+    for (int i = 0; i < nc; ++i) {
+      c->lv_match[l][i] = mt[l][i] - 1;
+      c->lv_imatch[l][mt[l][i] - 1] = i;
     }
   }
+  oc_ctx *c1 = c->lv[0], *c2 = c->lv[1];
+  /* laplc_c / laplc_cc (F:1356-1535): rh0 of the matched fine node; coefw, timestp and ue
+   * indexed by the coarse node id in the fine arrays, as the reference does */
+  for (int i = 0; i < c1->n; ++i) {
+    c1->rh0[i] = c->rh0[c->lv_match[0][i]];
+    c1->coefw[i] = c->coefw[i];
+    c1->ue[i] = c->ue[i];
+  }
+  for (int i = 0; i < c2->n; ++i) {
+    c2->rh0[i] = c->rh0[c->lv_match[0][c->lv_match[1][i]]];
+    c2->coefw[i] = c->coefw[i];
+    c2->ue[i] = c->ue[i];
+  }
+  c->mg_transfer = transfer;
+  /* compute_d_unstr(2), (3) (F:1181-1199): density restricted from the fine mesh */
+  double *r1 = (double *)calloc((size_t)c1->n, sizeof(double));
+  double *r2 = (double *)calloc((size_t)c2->n, sizeof(double));
+  mg_restrict(c, c->lv_match[0], c1->n, c->rh0, r1, transfer);
+  compute_d(c1, r1, c1->d1);
+  mg_restrict(c1, c->lv_match[1], c2->n, r1, r2, transfer);
+  compute_d(c2, r2, c2->d1);
+  free(r1); free(r2);
+  for (int i = 0; i < 12; ++i) {
+    free(c->mg_w[i]);
+    c->mg_w[i] = zalloc((size_t)c->n);
+  }
+  c->mg_itrp2 = itrp2; c->mg_c1 = itrp_c1; c->mg_c2 = itrp_c2; c->mg_cc = itrp_cc;
+  c->mg_on = 1;
+  return 0;
 }
+
+void oc_mg_restrict(oc_ctx *c, int level, const double *src, double *out) {
+  const oc_ctx *f = level == 1 ? c : c->lv[0];
+  mg_restrict(f, c->lv_match[level - 1], c->lv[level - 1]->n, src, out, c->mg_transfer);
+}
+
+void oc_mg_prolong(oc_ctx *c, int level, const double *coarse, double *out) {
+  const oc_ctx *f = level == 1 ? c : c->lv[0];
+  mg_prolong(f, c->lv_imatch[level - 1], coarse, out, c->mg_transfer);
+}
+
+oc_ctx *oc_level(oc_ctx *c, int level) { return level == 0 ? c : c->lv[level - 1]; }
 
 /* gcrk(p,u1,v1,w1,u2,v2,w2,impfl), F:1776-1972 (ielast=0).
  * exit_mode 0: max|r| <= eps0/dt (reference, F:1805, F:1895);

@@ -11,6 +11,9 @@
  *   derivative, laplc, prfr, div, bcv, precon(point), gcrk, mpdatam, compute_d,
  *   absorbers, step driver  -> golden vectors written by the flang-compiled reference
  *                              (tests/golden) plus closed forms in tests/test_oracle.py
+ *   multigrid branch of precon (restriction, prolongation, coarse laplc, d2/d3, the
+ *   V-cycle)                -> golden vectors of the reference run with mg_flag=1 on
+ *                              generated nested meshes (tests/golden)
  *   line preconditioner     -> no reference counterpart: pinned by a dense solve of the
  *                              tridiagonal system and by the closed form of the coefficients
  */
@@ -82,6 +85,22 @@ void oc_gcrk(oc_ctx *c, double *p, const double *u1, const double *v1, const dou
              double *u2, double *v2, double *w2, int impfl);
 void oc_mpdatam(oc_ctx *c, const double *uuu /*[3][npoin]*/, double *unk, int iflag);
 void oc_line_solve(oc_ctx *c, const double *rhs, double *z);
+
+/* precon's multigrid branch (mg_flag=1, F:618-748). Level 1 = coarse mesh 1 with match1
+ * (1-based fine node of each level-1 node), level 2 = coarse mesh 2 with match2 (1-based
+ * level-1 node). transfer 0: restriction/prolongation as the reference indexes them (the
+ * neighbour counter used as the node id, F:817, F:913); 1: the neighbour lists.
+ * Returns -1 when a match is out of range or a coarse mesh is larger than its parent.
+ * m1 == NULL turns the branch off. */
+int oc_set_multigrid(oc_ctx *c, const oc_mesh *m1, const int32_t *match1, const oc_mesh *m2,
+                     const int32_t *match2, int itrp2, int itrp_c1, int itrp_c2, int itrp_cc,
+                     int transfer);
+/* restriction_unstr(.,.,level) and prolongation_unstr(.,.,level) */
+void oc_mg_restrict(oc_ctx *c, int level, const double *src, double *out);
+void oc_mg_prolong(oc_ctx *c, int level, const double *coarse, double *out);
+/* context of a level (0 = fine): its oc_laplc is laplc_c / laplc_cc, its "d1" field is
+ * d2 / d3 and its oc_beta is betaunstr2 / betaunstr3 */
+oc_ctx *oc_level(oc_ctx *c, int level);
 
 /* NFT driver: state arrays are [npoin] blocks: unk[4], frc[4], uuu[3], unkdash[3], prpot */
 void oc_nft_init(oc_ctx *c);

@@ -76,6 +76,13 @@ def lib():
         L.oc_nft_init.argtypes = [vp]
         L.oc_nft_step.argtypes = [vp]
         L.oc_diagnostics.argtypes = [vp, ctypes.c_int, f64p]
+        L.oc_set_multigrid.restype = ctypes.c_int
+        L.oc_set_multigrid.argtypes = [vp, ctypes.POINTER(OcMesh), i32p, ctypes.POINTER(OcMesh), i32p,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.oc_mg_restrict.argtypes = [vp, ctypes.c_int, f64p, f64p]
+        L.oc_mg_prolong.argtypes = [vp, ctypes.c_int, f64p, f64p]
+        L.oc_level.restype = vp
+        L.oc_level.argtypes = [vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -210,6 +217,48 @@ class Oracle:
     def step(self, n=1):
         for _ in range(n):
             lib().oc_nft_step(self.h)
+
+    def set_multigrid(self, levels, itrp2=0, itrp_c1=1, itrp_c2=1, itrp_cc=1, transfer=0):
+        """Turns on precon's multigrid branch (mg_flag=1) with levels = [(mesh_c, match1),
+        (mesh_cc, match2)] as mesh.multigrid_meshes returns them."""
+        (m1, t1), (m2, t2) = levels
+        keep, structs = [], []
+        for mm in (m1, m2):
+            arrs = [np.ascontiguousarray(a) for a in (mm.iedge, mm.sn, mm.kbface, mm.sb, mm.ibpoin,
+                                                      mm.sbb, mm.anp, mm.coord, mm.vol)]
+            keep.append(arrs)
+            structs.append(OcMesh(mm.npoin, mm.nedge, mm.nbface, mm.nboun, *[_p(a) for a in arrs],
+                                  0, ctypes.cast(None, i32p)))
+        t1 = np.ascontiguousarray(t1, np.int32); t2 = np.ascontiguousarray(t2, np.int32)
+        rc = lib().oc_set_multigrid(self.h, ctypes.byref(structs[0]), _p(t1), ctypes.byref(structs[1]),
+                                    _p(t2), itrp2, itrp_c1, itrp_c2, itrp_cc, transfer)
+        if rc != 0:
+            raise ValueError("oc_set_multigrid rejected the levels")
+        self._mg_keep = (keep, structs, t1, t2)
+        self._mg_n = (self.n, m1.npoin, m2.npoin)
+
+    def restrict(self, level, src):
+        out = np.zeros(self._mg_n[level])
+        lib().oc_mg_restrict(self.h, level, _p(np.ascontiguousarray(src, np.float64)), _p(out))
+        return out
+
+    def prolong(self, level, coarse):
+        out = np.zeros(self._mg_n[level - 1])
+        lib().oc_mg_prolong(self.h, level, _p(np.ascontiguousarray(coarse, np.float64)), _p(out))
+        return out
+
+    def level_laplc(self, level, p, impfl):
+        n = self._mg_n[level]
+        out = np.zeros(n)
+        lib().oc_laplc(lib().oc_level(self.h, level), _p(np.ascontiguousarray(p, np.float64)), _p(out), impfl)
+        return out
+
+    def level_field(self, level, name):
+        ptr = lib().oc_field(lib().oc_level(self.h, level), name.encode())
+        return np.ctypeslib.as_array(ptr, shape=(self._mg_n[level],)).copy()
+
+    def level_beta(self, level):
+        return lib().oc_beta(lib().oc_level(self.h, level))
 
     def diagnostics(self, flag=1):
         """The reference's per-step diagnostics of the current state: [dvmx, dvmn, dvav,

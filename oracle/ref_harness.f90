@@ -7,6 +7,10 @@
 !                 of mpsolver (lippsLFroude_implgwopks3D_unstr3.f:108-170, 250-252), then
 !                 call the reference's own derivative/laplc/bcv/prfr/div/precon/mpdatam/gcrk
 !                 on the vectors of vec_in.bin and write every result to vec_out.bin.
+!   nft_ref mgops : the same set-up including the coarse meshes (beta_timestep_unstr and
+!                 compute_d_unstr for levels 2 and 3), then the multigrid pieces:
+!                 restriction_unstr, prolongation_unstr, laplc_c, laplc_cc, and precon and
+!                 gcrk with the V-cycle as input_mg_v3.d sets it (mg_flag=1).
 ! The COMMON declarations below restate the reference's block layouts at its default
 ! PARAMETER sizes (they must match for the blocks to alias).
 program nft_ref_harness
@@ -26,6 +30,8 @@ program nft_ref_harness
   common/itero/ itr,eps0,niter,nitsm,icount,eer,eem
   common/sf/isfera
   common/elast/ cfels,iflels
+  common/n_c/npoin_c,nedge_c,nbface_c,nbpoin_c,nboun_c
+  common/n_cc/npoin_cc,nedge_cc,nbface_cc,nbpoin_cc,nboun_cc
   real, save :: alpha(npoinx), v(npoinx,8), o1(npoinx), o2(npoinx), o3(npoinx), o4(npoinx)
   real, save :: uuu(npoinx,3), wk(npoinx)
   character(len=16) :: mode
@@ -93,6 +99,40 @@ program nft_ref_harness
   end do
   close(22)
   open(23,file='vec_out.bin',access='stream',form='unformatted',status='replace')
+  if (trim(mode) == 'mgops') then
+    call beta_timestep_unstr(2)
+    call beta_timestep_unstr(3)
+    call compute_d_unstr(2)
+    call compute_d_unstr(3)
+    write(23) (d2(ip),ip=1,npoin_c)
+    write(23) (d3(ip),ip=1,npoin_cc)
+    write(23) betaunstr2, betaunstr3
+    call restriction_unstr(v(1,1),o1,1)
+    write(23) (o1(ip),ip=1,npoin_c)
+    call restriction_unstr(v(1,2),o1,2)
+    write(23) (o1(ip),ip=1,npoin_cc)
+    call prolongation_unstr(v(1,3),o1,1)
+    write(23) (o1(ip),ip=1,npoin)
+    call prolongation_unstr(v(1,4),o1,2)
+    write(23) (o1(ip),ip=1,npoin_c)
+    call laplc_c(v(1,1),o1,1)
+    write(23) (o1(ip),ip=1,npoin_c)
+    call laplc_c(v(1,1),o1,0)
+    write(23) (o1(ip),ip=1,npoin_c)
+    call laplc_cc(v(1,1),o1,1)
+    write(23) (o1(ip),ip=1,npoin_cc)
+    call precon(v(1,5),o1,wk,1,elas,1)
+    write(23) (o1(ip),ip=1,npoin)
+    o4(1:npoin)=0.
+    nitsm=0
+    icount=0
+    call gcrk(o4,v(1,2),v(1,3),v(1,4),o1,o2,o3,1)
+    write(23) (o4(ip),ip=1,npoin)
+    write(23) (o1(ip),ip=1,npoin), (o2(ip),ip=1,npoin), (o3(ip),ip=1,npoin)
+    write(23) real(niter), eer, eem
+    close(23)
+    stop
+  end if
   write(23) (alpha(ip),ip=1,npoin)
   write(23) (d1(ip),ip=1,npoin)
   write(23) (coefw(ip),ip=1,npoin)

@@ -119,6 +119,22 @@ struct nft_ctx {
 
   nft_gcr_stats st{};
 
+  // multigrid branch of precon (mg_flag=1): coarse levels 1 and 2 on meshes of their own
+  struct MgLevel {
+    int n = 0, nodes_grid = 0, gather_grid = 0, col_on = 0;
+    Incidence inc{};
+    EdgeS<double> S{};
+    NodeCoef nc{};
+    double *vol = nullptr, *rho = nullptr, *pstep = nullptr;
+    int32_t *match = nullptr;    // [n] 0-based node of the next finer level
+    int32_t *imatch = nullptr;   // [n of the finer level] node of this level, -1 where none
+    double *p = nullptr, *r = nullptr, *rrhs = nullptr, *rout = nullptr, *e = nullptr;
+    std::vector<double> h_d;     // d2 / d3 of compute_d_unstr
+    double beta = 0;             // betaunstr2 / betaunstr3
+  } mg[2];
+  int mg_ready = 0, mg_on = 0, mg_itrp2 = 0, mg_c1 = 0, mg_c2 = 0, mg_cc = 0, mg_transfer = 0;
+  double *mg_rrhs = nullptr, *mg_e = nullptr;
+
   // learned preconditioner (network, training pairs, inference state)
   struct Net {
     int ready = 0, H = 128, L = 0, Kp = 0, ncol = 0, max_rows = 0;
@@ -178,6 +194,60 @@ int upload(nft_ctx *c, T **p, const T *h, size_t n) {
 inline int blocks_for(long n) { return (int)((n + NFT_BLOCK - 1) / NFT_BLOCK); }
 
 // ---- set-up fields on the host, operation for operation as the reference -------------
+// beta_timestep_unstr (:953-1109) over one mesh: half the square of the smallest
+// vol/max|S| over interior edges off the box faces and above the hard-coded 1500 m
+double beta_host(const nft_mesh *m, const double *d) {
+  const int ldp = m->ld_poin;
+  const double *X = m->coord, *Y = m->coord + ldp, *Z = m->coord + 2 * (size_t)ldp;
+  double ddx = 10.e+06, hill = 1500.0;
+  const int E = m->nedge, lde = m->ld_edge;
+  for (int e = 0; e < E; ++e) {
+    int i1 = m->iedge[e] - 1, i2 = m->iedge[lde + e] - 1;
+    if (X[i1] != d[0] && X[i1] != d[1] && Y[i1] != d[2] && Y[i1] != d[3] && Z[i1] != d[4] &&
+        Z[i1] != d[5] && X[i2] != d[0] && X[i2] != d[1] && Y[i2] != d[2] && Y[i2] != d[3] &&
+        Z[i2] != d[4] && Z[i2] != d[5] && Z[i1] > hill && Z[i2] > hill) {
+      double hx = std::fabs(m->sn[e]), hy = std::fabs(m->sn[lde + e]), hz = std::fabs(m->sn[2 * (size_t)lde + e]);
+      double h = std::max(std::max(hx, hy), hz);
+      ddx = std::min(std::min(ddx, m->vol[i1] / h), m->vol[i2] / h);
+    }
+  }
+  return 0.5 * ddx * ddx * 1.0;
+}
+
+// compute_d_unstr (:1112-1284) over one mesh with density rh0u
+void compute_d_host(const nft_mesh *m, const double *rh0, double dt, std::vector<double> &out) {
+  const int n = m->npoin;
+  std::vector<double> dix(n, 0.), diy(n, 0.), diz(n, 0.);
+  for (int pass = 0; pass < 2; ++pass) {
+    int ne = pass == 0 ? m->nedge : m->nbface;
+    int ld = pass == 0 ? m->ld_edge : m->ld_face;
+    const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+    const double *S = pass == 0 ? m->sn : m->sb;
+    for (int e = 0; e < ne; ++e) {
+      int i1 = ie[e] - 1, i2 = ie[ld + e] - 1;
+      double sx = S[e], sy = S[ld + e], sz = S[2 * (size_t)ld + e];
+      dix[i1] = dix[i1] + sx * sx * rh0[i2] / m->vol[i2];
+      dix[i2] = dix[i2] + sx * sx * rh0[i1] / m->vol[i1];
+      diy[i1] = diy[i1] + sy * sy * rh0[i2] / m->vol[i2];
+      diy[i2] = diy[i2] + sy * sy * rh0[i1] / m->vol[i1];
+      diz[i1] = diz[i1] + sz * sz * rh0[i2] / m->vol[i2];
+      diz[i2] = diz[i2] + sz * sz * rh0[i1] / m->vol[i1];
+    }
+  }
+  out.assign(n, 0.);
+  for (int i = 0; i < n; ++i) out[i] = -(0.5 * dt * (dix[i] + diy[i] + diz[i]) / (4 * m->vol[i] * rh0[i]));
+}
+
+// point-relaxation step of precon (:587-599 fine, :632-653 coarse) from d and beta
+void pstep_host(int sm_flag, const std::vector<double> &d, double beta, std::vector<double> &pstep) {
+  pstep.assign(d.size(), 0.);
+  for (size_t i = 0; i < d.size(); ++i) {
+    if (sm_flag == 1) pstep[i] = (-1.) / d[i];
+    else if (sm_flag == 2) pstep[i] = beta;
+    else pstep[i] = beta / (1 - d[i] * beta);
+  }
+}
+
 void setup_fields(nft_ctx *c, const nft_mesh *m, std::vector<double> &rh0, std::vector<double> &th0,
                   std::vector<double> &ue, std::vector<double> &ve, std::vector<double> &dthedz,
                   std::vector<double> &pstep) {
@@ -232,45 +302,9 @@ void setup_fields(nft_ctx *c, const nft_mesh *m, std::vector<double> &rh0, std::
     }
   }
   // beta_timestep_unstr(1) (:953-1109)
-  {
-    double ddx = 10.e+06, hill = 1500.0;
-    const double *d = p.dom;
-    const int E = m->nedge, lde = m->ld_edge;
-    for (int e = 0; e < E; ++e) {
-      int i1 = m->iedge[e] - 1, i2 = m->iedge[lde + e] - 1;
-      if (X[i1] != d[0] && X[i1] != d[1] && Y[i1] != d[2] && Y[i1] != d[3] && Z[i1] != d[4] &&
-          Z[i1] != d[5] && X[i2] != d[0] && X[i2] != d[1] && Y[i2] != d[2] && Y[i2] != d[3] &&
-          Z[i2] != d[4] && Z[i2] != d[5] && Z[i1] > hill && Z[i2] > hill) {
-        double hx = std::fabs(m->sn[e]), hy = std::fabs(m->sn[lde + e]), hz = std::fabs(m->sn[2 * (size_t)lde + e]);
-        double h = std::max(std::max(hx, hy), hz);
-        ddx = std::min(std::min(ddx, m->vol[i1] / h), m->vol[i2] / h);
-      }
-    }
-    c->beta1 = 0.5 * ddx * ddx * 1.0;
-  }
+  c->beta1 = beta_host(m, p.dom);
   // compute_d_unstr(1) (:1112-1284)
-  {
-    std::vector<double> dix(n, 0.), diy(n, 0.), diz(n, 0.);
-    for (int pass = 0; pass < 2; ++pass) {
-      int ne = pass == 0 ? m->nedge : m->nbface;
-      int ld = pass == 0 ? m->ld_edge : m->ld_face;
-      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
-      const double *S = pass == 0 ? m->sn : m->sb;
-      for (int e = 0; e < ne; ++e) {
-        int i1 = ie[e] - 1, i2 = ie[ld + e] - 1;
-        double sx = S[e], sy = S[ld + e], sz = S[2 * (size_t)ld + e];
-        dix[i1] = dix[i1] + sx * sx * rh0[i2] / m->vol[i2];
-        dix[i2] = dix[i2] + sx * sx * rh0[i1] / m->vol[i1];
-        diy[i1] = diy[i1] + sy * sy * rh0[i2] / m->vol[i2];
-        diy[i2] = diy[i2] + sy * sy * rh0[i1] / m->vol[i1];
-        diz[i1] = diz[i1] + sz * sz * rh0[i2] / m->vol[i2];
-        diz[i2] = diz[i2] + sz * sz * rh0[i1] / m->vol[i1];
-      }
-    }
-    c->h_d1.assign(n, 0.);
-    for (int i = 0; i < n; ++i)
-      c->h_d1[i] = -(0.5 * p.dt * (dix[i] + diy[i] + diz[i]) / (4 * m->vol[i] * rh0[i]));
-  }
+  compute_d_host(m, rh0.data(), p.dt, c->h_d1);
   // coefw (:250-252)
   c->h_coefw.assign(n, 0.);
   for (int i = 0; i < n; ++i) {
@@ -279,12 +313,7 @@ void setup_fields(nft_ctx *c, const nft_mesh *m, std::vector<double> &rh0, std::
     c->h_coefw[i] = 1. / cw;
   }
   // point relaxation step (precon :587-599)
-  pstep.assign(n, 0.);
-  for (int i = 0; i < n; ++i) {
-    if (p.sm_flag == 1) pstep[i] = (-1.) / c->h_d1[i];
-    else if (p.sm_flag == 2) pstep[i] = c->beta1;
-    else pstep[i] = c->beta1 / (1 - c->h_d1[i] * c->beta1);
-  }
+  pstep_host(p.sm_flag, c->h_d1, c->beta1, pstep);
   // vertical line preconditioner (no reference counterpart): two-point-flux edge Laplacian
   // c_ij = 0.5*dt * 0.5(rho_i+rho_j) (Sx^2+Sy^2+cw Sz^2) / max(S.d, 1e-3|S||d|) / (vol_i rho_i),
   // M_ii = -sum_j c_ij, M_{i,i+-1} = c_ij along the column; Thomas factors per column.
@@ -343,7 +372,7 @@ void setup_fields(nft_ctx *c, const nft_mesh *m, std::vector<double> &rh0, std::
 }
 
 // ---- kernel timer ------------------------------------------------------------------------
-enum { KT_LGRAD = 1, KT_LDIV = 2, KT_LINE = 3, KT_UPD = 4, KT_DIR = 5, KT_MP0 = 6 };
+enum { KT_LGRAD = 1, KT_LDIV = 2, KT_LINE = 3, KT_UPD = 4, KT_DIR = 5, KT_MP0 = 6, KT_MG = 10 };
 
 inline bool timing(nft_ctx *c, int k) { return c->timer_kernel == k && c->tev_used + 2 <= (int)c->tev.size(); }
 inline void t_begin(nft_ctx *c, int k) {
@@ -524,58 +553,145 @@ void line_scan_r_go(nft_ctx *c, int cg, const T *la, const T *linv, const T *lc,
   }
 }
 
-int precon_dev(nft_ctx *c, const double *rhs, double *p, int impfl) {
+// one fine-mesh smoothing sweep of precon (:607-612, :740-745): r = L p (not on the first
+// sweep from p = 0, where it is zero), then the point step p += omega*pstep*(r-rhs), or
+// with the line preconditioner p += omega * M^-1 (rhs - r)
+int fine_sweep(nft_ctx *c, const double *rhs, double *p, int impfl, bool first) {
   const nft_params &P = c->P;
   const int n = c->n;
+  if (!first) {
+    int rc = laplc_dev(c, p, c->rw, impfl, nullptr);
+    if (rc) return rc;
+  }
+  if (P.precon == 1) {
+    hipLaunchKernelGGL(k_point, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, n, c->pstep,
+                       c->rw, rhs, p, P.omega, first ? 1 : 0, c->skip);
+  } else {
+    const double *res = rhs;
+    if (!first) {
+      hipLaunchKernelGGL(k_sub, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, n, rhs, c->rw, c->t[3],
+                         c->skip);
+      res = c->t[3];
+    }
+    t_begin(c, KT_LINE);
+    const double *pin = first ? nullptr : p;
+    if (c->line_L > 0) {
+      int cpw = 64 / c->line_L;
+      long waves = (c->ncol + cpw - 1) / cpw;
+      int cg = (int)((waves + NFT_WPB - 1) / NFT_WPB);
+      if (P.precision == 1)
+        hipLaunchKernelGGL(k_line_scan<float>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->line_L,
+                           c->laf, c->linvf, c->lcf, res, pin, p, P.omega, c->skip);
+      else
+        hipLaunchKernelGGL(k_line_scan<double>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->line_L,
+                           c->la, c->linv, c->lc, res, pin, p, P.omega, c->skip);
+    } else if (c->line_R > 0) {
+      int cg = (int)((c->ncol + NFT_WPB - 1) / NFT_WPB);
+      if (P.precision == 1) line_scan_r_go<float>(c, cg, c->laf, c->linvf, c->lcf, res, pin, p);
+      else line_scan_r_go<double>(c, cg, c->la, c->linv, c->lc, res, pin, p);
+    } else {
+      int cg = blocks_for(c->ncol);
+      if (P.precision == 1)
+        hipLaunchKernelGGL(k_line<float>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->col_start,
+                           c->laf, c->linvf, c->lcf, res, pin, p, P.omega, c->zbuff, c->skip);
+      else
+        hipLaunchKernelGGL(k_line<double>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->col_start,
+                           c->la, c->linv, c->lc, res, pin, p, P.omega, c->zbuf, c->skip);
+    }
+    t_end(c, KT_LINE);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// laplc_c / laplc_cc (:1356-1535): laplc over a coarse level's mesh; the gradient
+// temporaries are the fine mesh's (coarse levels are smaller)
+int laplc_level(nft_ctx *c, const nft_ctx::MgLevel &L, const double *press, double *alap, int impfl) {
+  GradArgs g{};
+  g.nc = L.nc; g.dt = c->P.dt; g.skip = c->skip;
+  g.f = press; g.oxy = c->txy; g.o2 = c->t[2]; g.impfl = impfl;
+  DivArgs d{};
+  d.nc = L.nc; d.partials = c->partials; d.counter = c->counter; d.scal = c->scal; d.epa = 1.e-38;
+  d.skip = c->skip; d.qxy = c->txy; d.q2 = c->t[2]; d.out = alap;
+  dim3 gg(L.gather_grid), b(NFT_BLOCK);
+  if (L.col_on) {
+    hipLaunchKernelGGL((k_grad<GRAD_LAPLC, true>), gg, b, 0, c->stream, L.inc, L.S, g);
+    hipLaunchKernelGGL((k_div<DIV_LAPLC, 0, 0, true>), gg, b, 0, c->stream, L.inc, L.S, d);
+  } else {
+    hipLaunchKernelGGL((k_grad<GRAD_LAPLC, false>), gg, b, 0, c->stream, L.inc, L.S, g);
+    hipLaunchKernelGGL((k_div<DIV_LAPLC, 0, 0, false>), gg, b, 0, c->stream, L.inc, L.S, d);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// point relaxation on a coarse level (:676-681, :703-708, :723-728): it sweeps of
+// r = L p, p += omega*pstep*(r - rrhs)
+int relax_level(nft_ctx *c, nft_ctx::MgLevel &L, int sweeps, int impfl) {
+  for (int it = 1; it <= sweeps; ++it) {
+    int rc = laplc_level(c, L, L.p, L.r, impfl);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_point, dim3(L.nodes_grid), dim3(NFT_BLOCK), 0, c->stream, L.n, L.pstep, L.r, L.rrhs,
+                       L.p, c->P.omega, 0, c->skip);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// the V-cycle of precon's multigrid branch (:618-748), after the fine smoothing
+int mg_cycle_dev(nft_ctx *c, const double *rhs, double *p, int impfl) {
+  auto &A = c->mg[0], &B = c->mg[1];
+  const int n = c->n, tr = c->mg_transfer;
+  const double om = c->P.omega;
+  const dim3 b(NFT_BLOCK);
+  int rc;
+  // fine residual, restricted to level 1
+  if ((rc = laplc_dev(c, p, c->rw, impfl, nullptr))) return rc;
+  t_begin(c, KT_MG);
+  hipLaunchKernelGGL(k_sub, dim3(c->nodes_grid), b, 0, c->stream, n, rhs, c->rw, c->mg_rrhs, c->skip);
+  hipLaunchKernelGGL(k_mg_restrict, dim3(A.nodes_grid), b, 0, c->stream, A.n, A.match, c->inc, c->mg_rrhs,
+                     A.rrhs, tr, c->skip);
+  // level 1: p = -(omega*pstep*rrhs), relax, residual, restrict to level 2
+  hipLaunchKernelGGL(k_point, dim3(A.nodes_grid), b, 0, c->stream, A.n, A.pstep, A.r, A.rrhs, A.p, om, 1,
+                     c->skip);
+  if ((rc = relax_level(c, A, c->mg_c1, impfl))) return rc;
+  if ((rc = laplc_level(c, A, A.p, A.r, impfl))) return rc;
+  hipLaunchKernelGGL(k_sub, dim3(A.nodes_grid), b, 0, c->stream, A.n, A.rrhs, A.r, A.rout, c->skip);
+  hipLaunchKernelGGL(k_mg_restrict, dim3(B.nodes_grid), b, 0, c->stream, B.n, B.match, A.inc, A.rout, B.rrhs,
+                     tr, c->skip);
+  // level 2: p = -(rrhs*pstep*omega), relax
+  hipLaunchKernelGGL(k_mg_start_cc, dim3(B.nodes_grid), b, 0, c->stream, B.n, B.pstep, B.rrhs, B.p, om, c->skip);
+  if ((rc = relax_level(c, B, c->mg_cc, impfl))) return rc;
+  // back to level 1: correct, relax
+  hipLaunchKernelGGL(k_mg_prolong, dim3(A.nodes_grid), b, 0, c->stream, A.n, B.imatch, A.inc, B.p, A.e, tr,
+                     c->skip);
+  hipLaunchKernelGGL(k_mg_add, dim3(A.nodes_grid), b, 0, c->stream, A.n, A.p, A.e, c->skip);
+  if ((rc = relax_level(c, A, c->mg_c2, impfl))) return rc;
+  // back to the fine mesh: correct, relax
+  hipLaunchKernelGGL(k_mg_prolong, dim3(c->nodes_grid), b, 0, c->stream, n, A.imatch, c->inc, A.p, c->mg_e, tr,
+                     c->skip);
+  hipLaunchKernelGGL(k_mg_add, dim3(c->nodes_grid), b, 0, c->stream, n, p, c->mg_e, c->skip);
+  t_end(c, KT_MG);
+  HIPCHK(hipGetLastError());
+  for (int it = 1; it <= c->mg_itrp2; ++it)
+    if ((rc = fine_sweep(c, rhs, p, impfl, false))) return rc;
+  return 0;
+}
+
+// precon(rhs,p,r,iprc,elas,impfl) (:482-748): itrp1 fine sweeps (point relaxation or the
+// vertical line preconditioner), then the V-cycle when the multigrid branch is on
+int precon_dev(nft_ctx *c, const double *rhs, double *p, int impfl) {
+  const nft_params &P = c->P;
   if (P.precon == 0) {
-    hipLaunchKernelGGL(k_copy, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, n, rhs, p, c->skip);
+    hipLaunchKernelGGL(k_copy, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, rhs, p, c->skip);
     HIPCHK(hipGetLastError());
     return 0;
   }
   for (int it = 1; it <= P.itrp1; ++it) {
-    if (it > 1) {
-      int rc = laplc_dev(c, p, c->rw, impfl, nullptr);
-      if (rc) return rc;
-    }
-    if (P.precon == 1) {
-      hipLaunchKernelGGL(k_point, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, n, c->pstep,
-                         c->rw, rhs, p, P.omega, it == 1 ? 1 : 0, c->skip);
-    } else {
-      const double *res = rhs;
-      if (it > 1) {
-        hipLaunchKernelGGL(k_sub, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, n, rhs, c->rw, c->t[3],
-                           c->skip);
-        res = c->t[3];
-      }
-      t_begin(c, KT_LINE);
-      const double *pin = it == 1 ? nullptr : p;
-      if (c->line_L > 0) {
-        int cpw = 64 / c->line_L;
-        long waves = (c->ncol + cpw - 1) / cpw;
-        int cg = (int)((waves + NFT_WPB - 1) / NFT_WPB);
-        if (P.precision == 1)
-          hipLaunchKernelGGL(k_line_scan<float>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->line_L,
-                             c->laf, c->linvf, c->lcf, res, pin, p, P.omega, c->skip);
-        else
-          hipLaunchKernelGGL(k_line_scan<double>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->line_L,
-                             c->la, c->linv, c->lc, res, pin, p, P.omega, c->skip);
-      } else if (c->line_R > 0) {
-        int cg = (int)((c->ncol + NFT_WPB - 1) / NFT_WPB);
-        if (P.precision == 1) line_scan_r_go<float>(c, cg, c->laf, c->linvf, c->lcf, res, pin, p);
-        else line_scan_r_go<double>(c, cg, c->la, c->linv, c->lc, res, pin, p);
-      } else {
-        int cg = blocks_for(c->ncol);
-        if (P.precision == 1)
-          hipLaunchKernelGGL(k_line<float>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->col_start,
-                             c->laf, c->linvf, c->lcf, res, pin, p, P.omega, c->zbuff, c->skip);
-        else
-          hipLaunchKernelGGL(k_line<double>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->col_start,
-                             c->la, c->linv, c->lc, res, pin, p, P.omega, c->zbuf, c->skip);
-      }
-      t_end(c, KT_LINE);
-    }
-    HIPCHK(hipGetLastError());
+    int rc = fine_sweep(c, rhs, p, impfl, it == 1);
+    if (rc) return rc;
   }
+  if (c->mg_on) return mg_cycle_dev(c, rhs, p, impfl);
   return 0;
 }
 
@@ -1162,69 +1278,55 @@ int sync(nft_ctx *c) {
   return 0;
 }
 
-}  // namespace
-
-// =====================================================================================
-extern "C" {
-
-const char *nft_last_error(void) { return g_err.c_str(); }
-
-int nft_device_name(char *buf, int len) {
-  hipDeviceProp_t prop;
-  int dev = 0;
-  HIPCHK(hipGetDevice(&dev));
-  HIPCHK(hipGetDeviceProperties(&prop, dev));
-  std::snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
-  return 0;
-}
-
-int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
-  if (!m || !P || !out) return set_err(-1, "nft_init: null argument");
-  *out = nullptr;
-  if (m->npoin <= 0 || m->nedge < 0 || m->nbface < 0 || m->nboun < 0)
-    return set_err(-1, "nft_init: bad mesh sizes");
+// connectivity and volume checks of a mesh before it reaches any kernel
+int validate_mesh(const nft_mesh *m, const char *who) {
+  const std::string w(who);
+  if (m->npoin <= 0 || m->nedge < 0 || m->nbface < 0 || m->nboun < 0) return set_err(-1, w + ": bad mesh sizes");
   if (m->ld_poin < m->npoin || m->ld_edge < m->nedge || m->ld_face < std::max(m->nbface, m->nboun))
-    return set_err(-1, "nft_init: leading dimension smaller than count");
-  if (P->lord < 1 || P->lord > NFT_MAXD) return set_err(-1, "nft_init: lord out of range 1..18");
-  if (P->precon == 2 && m->ncol <= 0) return set_err(-1, "nft_init: line preconditioner needs columns");
-  if (P->precon < 0 || P->precon > 2) return set_err(-1, "nft_init: unknown preconditioner");
-  if (!(P->dt > 0)) return set_err(-1, "nft_init: dt must be positive");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(-3, "nft_init: no HIP device");
+    return set_err(-1, w + ": leading dimension smaller than count");
   const int n = m->npoin;
-  const long etot = (long)m->nedge + m->nbface;
-  // validate connectivity before it reaches any kernel
   auto bad_node = [&](int32_t v) { return v < 1 || v > n; };
   for (int e = 0; e < m->nedge; ++e)
     if (bad_node(m->iedge[e]) || bad_node(m->iedge[m->ld_edge + e]) || m->iedge[e] == m->iedge[m->ld_edge + e])
-      return set_err(-1, "nft_init: iedge entry out of range or degenerate at edge " + std::to_string(e + 1));
+      return set_err(-1, w + ": iedge entry out of range or degenerate at edge " + std::to_string(e + 1));
   for (int e = 0; e < m->nbface; ++e)
     if (bad_node(m->kbface[e]) || bad_node(m->kbface[m->ld_face + e]) || m->kbface[e] == m->kbface[m->ld_face + e])
-      return set_err(-1, "nft_init: kbface entry out of range at " + std::to_string(e + 1));
+      return set_err(-1, w + ": kbface entry out of range at " + std::to_string(e + 1));
   for (int b = 0; b < m->nboun; ++b)
-    if (bad_node(m->ibpoin[b])) return set_err(-1, "nft_init: ibpoin node out of range at " + std::to_string(b + 1));
+    if (bad_node(m->ibpoin[b])) return set_err(-1, w + ": ibpoin node out of range at " + std::to_string(b + 1));
   if (m->ncol > 0) {
     if (!m->col_start || m->col_start[0] != 0 || m->col_start[m->ncol] != n)
-      return set_err(-1, "nft_init: col_start must cover 0..npoin");
+      return set_err(-1, w + ": col_start must cover 0..npoin");
     for (int k = 0; k < m->ncol; ++k)
-      if (m->col_start[k + 1] <= m->col_start[k]) return set_err(-1, "nft_init: empty column");
+      if (m->col_start[k + 1] <= m->col_start[k]) return set_err(-1, w + ": empty column");
   }
   for (int i = 0; i < n; ++i)
-    if (!(m->vol[i] > 0)) return set_err(-1, "nft_init: non-positive volume at node " + std::to_string(i + 1));
+    if (!(m->vol[i] > 0)) return set_err(-1, w + ": non-positive volume at node " + std::to_string(i + 1));
+  return 0;
+}
 
-  nft_ctx *c = new nft_ctx();
-  c->P = *P;
-  c->n = n; c->nedge = m->nedge; c->nbface = m->nbface; c->nboun = m->nboun; c->etot = (int)etot;
-  c->ncol = m->ncol;
+// Device geometry of one mesh: the node-sorted SELL-64 incidence (each node's slots in the
+// global edge order: interior edges, then boundary-face edges), its columnar form when the
+// mesh is extruded with identical interior levels, the per-node boundary-point tables and
+// the edge metrics. Host copies of the edge lists and metrics stay in G for the callers.
+struct Geometry {
+  Incidence inc{};
+  EdgeS<double> S{};
+  int gather_grid = 0;
+  int col_on = 0;
+  double col_bytes = 0;
+  std::vector<int32_t> i1, i2;
+  std::vector<double> sx, sy, sz;
+};
+
+int build_geometry(nft_ctx *c, const nft_mesh *m, Geometry &G) {
+  const int n = m->npoin;
+  const long etot = (long)m->nedge + m->nbface;
   int rc = 0;
-  auto fail = [&](int code) { nft_finalize(c); return code; };
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(set_err(-2, "stream"));
-  for (auto &e : c->ev)
-    if (hipEventCreate(&e) != hipSuccess) return fail(set_err(-2, "event"));
-
-  // ---- incidence: rows sorted by global edge index (interior, then semi-edges)
-  std::vector<int32_t> i1(etot), i2(etot);
-  std::vector<double> sx(etot), sy(etot), sz(etot);
+  G.i1.assign(etot, 0); G.i2.assign(etot, 0);
+  G.sx.assign(etot, 0.); G.sy.assign(etot, 0.); G.sz.assign(etot, 0.);
+  std::vector<int32_t> &i1 = G.i1, &i2 = G.i2;
+  std::vector<double> &sx = G.sx, &sy = G.sy, &sz = G.sz;
   for (int e = 0; e < m->nedge; ++e) {
     i1[e] = m->iedge[e] - 1; i2[e] = m->iedge[m->ld_edge + e] - 1;
     sx[e] = m->sn[e]; sy[e] = m->sn[m->ld_edge + e]; sz[e] = m->sn[2 * (size_t)m->ld_edge + e];
@@ -1236,18 +1338,6 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   }
   std::vector<int32_t> deg(n, 0);
   for (long e = 0; e < etot; ++e) { deg[i1[e]]++; deg[i2[e]]++; }
-  if (m->ncol > 0) {
-    // horizontal column adjacency (edges between different columns at one level)
-    std::vector<int> colof(n), levof(n);
-    for (int k = 0; k < m->ncol; ++k)
-      for (int i = m->col_start[k]; i < m->col_start[k + 1]; ++i) { colof[i] = k; levof[i] = i - m->col_start[k]; }
-    for (long e = 0; e < etot; ++e) {
-      int a = colof[i1[e]], b = colof[i2[e]];
-      if (a != b && levof[i1[e]] == levof[i2[e]]) c->h_hedges.push_back({std::min(a, b), std::max(a, b)});
-    }
-    std::sort(c->h_hedges.begin(), c->h_hedges.end());
-    c->h_hedges.erase(std::unique(c->h_hedges.begin(), c->h_hedges.end()), c->h_hedges.end());
-  }
   const int nchunk = (n + NFT_WAVE - 1) / NFT_WAVE;
   std::vector<int32_t> chunk_ptr(nchunk + 1, 0);
   for (int ch = 0; ch < nchunk; ++ch) {
@@ -1256,7 +1346,7 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
       int i = ch * NFT_WAVE + r;
       if (i < n) w = std::max(w, deg[i]);
     }
-    if ((long)chunk_ptr[ch] + (long)NFT_WAVE * w > 2000000000L) return fail(set_err(-1, "incidence too large"));
+    if ((long)chunk_ptr[ch] + (long)NFT_WAVE * w > 2000000000L) return set_err(-1, "incidence too large");
     chunk_ptr[ch + 1] = chunk_ptr[ch] + NFT_WAVE * w;
   }
   const long nslot = chunk_ptr[nchunk];
@@ -1345,8 +1435,8 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
       (rc = upload(c, &d_eid, eid.data(), eid.size())) || (rc = upload(c, &d_bp_ptr, bp_ptr.data(), bp_ptr.size())) ||
       (rc = upload(c, &d_bflag, bflag.data(), bflag.size())) || (rc = upload(c, &d_bsbb, bsbb.data(), bsbb.size())) ||
       (rc = upload(c, &d_banp, banp.data(), banp.size())))
-    return fail(rc);
-  Incidence &g = c->inc;
+    return rc;
+  Incidence &g = G.inc;
   g.n = n; g.nchunk = nchunk; g.chunk_ptr = d_chunk; g.deg = d_deg; g.nbr = d_nbr; g.eid = d_eid;
   g.col_L = 0;
   if (colL > 0) {
@@ -1357,37 +1447,99 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
     int2 *d_tab;
     if ((rc = upload(c, &d_tptr, tptr.data(), tptr.size())) || (rc = upload(c, &d_xptr, xptr.data(), xptr.size())) ||
         (rc = upload(c, &d_tab, tab.data(), tab.size())))
-      return fail(rc);
+      return rc;
     g.col_L = colL;
     g.tptr = d_tptr; g.xptr = d_xptr; g.tab = d_tab; g.xoff = (int)tnb.size();
-    c->col_on = 1;
-    c->col_bytes = 4.0 * (tptr.size() + xptr.size()) + 8.0 * tab.size();
+    G.col_on = 1;
+    G.col_bytes = 4.0 * (tptr.size() + xptr.size()) + 8.0 * tab.size();
   }
   g.bp_ptr = d_bp_ptr; g.bp_flag = d_bflag; g.bp_sbb = d_bsbb; g.bp_anp = d_banp; g.nbp = nbp;
   g.nblocks = (nchunk + NFT_WPB - 1) / NFT_WPB;
   if (g.nblocks >= 64) {
     g.xcd_per = (g.nblocks + 7) / 8;
-    c->gather_grid = 8 * g.xcd_per;
+    G.gather_grid = 8 * g.xcd_per;
   } else {
     g.xcd_per = 0;
-    c->gather_grid = g.nblocks;
+    G.gather_grid = g.nblocks;
   }
-  c->vec_grid = std::min(blocks_for(n), 4096);
-  c->nodes_grid = blocks_for(n);
   double *dsx, *dsy, *dsz;
   if ((rc = upload(c, &dsx, sx.data(), etot)) || (rc = upload(c, &dsy, sy.data(), etot)) ||
       (rc = upload(c, &dsz, sz.data(), etot)))
-    return fail(rc);
-  c->S = EdgeS<double>{dsx, dsy, dsz};
+    return rc;
+  G.S = EdgeS<double>{dsx, dsy, dsz};
   {
     std::vector<double2> sxy(etot);
     for (long e = 0; e < etot; ++e) sxy[e] = make_double2(sx[e], sy[e]);
     double2 *d_sxy;
-    if ((rc = upload(c, &d_sxy, sxy.data(), etot))) return fail(rc);
-    c->inc.sxy = d_sxy;
-    c->inc.szz = dsz;
+    if ((rc = upload(c, &d_sxy, sxy.data(), etot))) return rc;
+    g.sxy = d_sxy;
+    g.szz = dsz;
   }
+  return 0;
+}
 
+}  // namespace
+
+// =====================================================================================
+extern "C" {
+
+const char *nft_last_error(void) { return g_err.c_str(); }
+
+int nft_device_name(char *buf, int len) {
+  hipDeviceProp_t prop;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  std::snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  return 0;
+}
+
+int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
+  if (!m || !P || !out) return set_err(-1, "nft_init: null argument");
+  *out = nullptr;
+  if (P->lord < 1 || P->lord > NFT_MAXD) return set_err(-1, "nft_init: lord out of range 1..18");
+  if (P->precon == 2 && m->ncol <= 0) return set_err(-1, "nft_init: line preconditioner needs columns");
+  if (P->precon < 0 || P->precon > 2) return set_err(-1, "nft_init: unknown preconditioner");
+  if (!(P->dt > 0)) return set_err(-1, "nft_init: dt must be positive");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(-3, "nft_init: no HIP device");
+  const int n = m->npoin;
+  const long etot = (long)m->nedge + m->nbface;
+  {
+    int rc0 = validate_mesh(m, "nft_init");
+    if (rc0) return rc0;
+  }
+  nft_ctx *c = new nft_ctx();
+  c->P = *P;
+  c->n = n; c->nedge = m->nedge; c->nbface = m->nbface; c->nboun = m->nboun; c->etot = (int)etot;
+  c->ncol = m->ncol;
+  int rc = 0;
+  auto fail = [&](int code) { nft_finalize(c); return code; };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(set_err(-2, "stream"));
+  for (auto &e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(set_err(-2, "event"));
+
+  // ---- geometry: incidence, columnar form, boundary tables, edge metrics
+  Geometry geo;
+  if ((rc = build_geometry(c, m, geo))) return fail(rc);
+  c->inc = geo.inc; c->S = geo.S; c->gather_grid = geo.gather_grid;
+  c->col_on = geo.col_on; c->col_bytes = geo.col_bytes;
+  c->vec_grid = std::min(blocks_for(n), 4096);
+  c->nodes_grid = blocks_for(n);
+  const std::vector<int32_t> &i1 = geo.i1, &i2 = geo.i2;
+  const std::vector<double> &sx = geo.sx, &sy = geo.sy, &sz = geo.sz;
+  if (m->ncol > 0) {
+    // horizontal column adjacency (edges between different columns at one level)
+    std::vector<int> colof(n), levof(n);
+    for (int k = 0; k < m->ncol; ++k)
+      for (int i = m->col_start[k]; i < m->col_start[k + 1]; ++i) { colof[i] = k; levof[i] = i - m->col_start[k]; }
+    for (long e = 0; e < etot; ++e) {
+      int a = colof[i1[e]], b = colof[i2[e]];
+      if (a != b && levof[i1[e]] == levof[i2[e]]) c->h_hedges.push_back({std::min(a, b), std::max(a, b)});
+    }
+    std::sort(c->h_hedges.begin(), c->h_hedges.end());
+    c->h_hedges.erase(std::unique(c->h_hedges.begin(), c->h_hedges.end()), c->h_hedges.end());
+  }
   // ---- set-up fields
   std::vector<double> rh0, th0, ue, ve, dthedz, pstep;
   setup_fields(c, m, rh0, th0, ue, ve, dthedz, pstep);
@@ -1729,7 +1881,7 @@ int nft_get_gcr_stats(nft_ctx *c, nft_gcr_stats *s) {
 }
 
 int nft_kernel_timer(nft_ctx *c, int kernel) {
-  if (!c || kernel < 0 || kernel > 9) return set_err(-1, "nft_kernel_timer: bad argument");
+  if (!c || kernel < 0 || kernel > 10) return set_err(-1, "nft_kernel_timer: bad argument");
   HIPCHK(hipStreamSynchronize(c->stream));
   t_harvest(c);
   if (c->tev.empty()) {
@@ -2163,6 +2315,164 @@ int nft_set_learned(nft_ctx *c, int kind, double stall_ratio, int *fallbacks, in
 // :4779-4890, which loses digits): "NFTSTAT1", int64 npoin, steps, nitsm, icount, then
 // the step-boundary state as little-endian float64 blocks unk(4), frc(4), uuu(3),
 // unkdash(3), prpot(1), each [k][npoin]. Resuming from it continues the run bit for bit.
+// ---- multigrid branch of precon ---------------------------------------------------------
+int nft_set_multigrid(nft_ctx *c, const nft_mesh *m1, const int32_t *match1, const nft_mesh *m2,
+                      const int32_t *match2, const nft_multigrid *mgp) {
+  if (!c || !mgp) return set_err(-1, "nft_set_multigrid: null argument");
+  if (mgp->itrp2 < 0 || mgp->itrp_c1 < 0 || mgp->itrp_c2 < 0 || mgp->itrp_cc < 0)
+    return set_err(-1, "nft_set_multigrid: negative sweep count");
+  if (mgp->transfer != MG_AS_REFERENCE && mgp->transfer != MG_NEIGHBOURS)
+    return set_err(-1, "nft_set_multigrid: transfer must be 0 or 1");
+  if (!m1 != !m2 || !m1 != !match1 || !m2 != !match2)
+    return set_err(-1, "nft_set_multigrid: give both coarse meshes and both matches, or none");
+  if (!m1) {
+    if (mgp->mg_flag && !c->mg_ready) return set_err(-1, "nft_set_multigrid: no coarse levels set");
+    if (mgp->mg_flag && mgp->transfer != c->mg_transfer)
+      return set_err(-1, "nft_set_multigrid: changing the transfer needs the coarse meshes again");
+  } else {
+    if (c->P.precon == 0) return set_err(-1, "nft_set_multigrid: precon 0 has no multigrid branch");
+    const nft_mesh *ms[2] = {m1, m2};
+    const int32_t *mt[2] = {match1, match2};
+    const int nfine[2] = {c->n, m1->npoin};
+    for (int l = 0; l < 2; ++l) {
+      int rc = validate_mesh(ms[l], "nft_set_multigrid");
+      if (rc) return rc;
+      // the coarse levels index the fine mesh's coefw and ue by their own node ids, as the
+      // reference does (laplc_c, :1387-1392)
+      if (ms[l]->npoin > nfine[l]) return set_err(-1, "nft_set_multigrid: coarse mesh larger than its parent");
+      std::vector<char> seen(nfine[l], 0);
+      for (int i = 0; i < ms[l]->npoin; ++i) {
+        int32_t f = mt[l][i];
+        if (f < 1 || f > nfine[l]) return set_err(-1, "nft_set_multigrid: match entry out of range at " + std::to_string(i + 1));
+        if (seen[f - 1]) return set_err(-1, "nft_set_multigrid: two coarse nodes match fine node " + std::to_string(f));
+        seen[f - 1] = 1;
+      }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->mg_on = 0; c->mg_ready = 0;
+    int rc;
+    std::vector<double> rh0(c->n);
+    HIPCHK(hipMemcpy(rh0.data(), c->rh0, c->n * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<int32_t> chain(m1->npoin);   // fine node of each level-1 node
+    for (int l = 0; l < 2; ++l) {
+      auto &L = c->mg[l];
+      const nft_mesh *cm = ms[l];
+      const int nc = cm->npoin, nf = nfine[l];
+      L = nft_ctx::MgLevel{};
+      L.n = nc;
+      Geometry G;
+      if ((rc = build_geometry(c, cm, G))) return rc;
+      L.inc = G.inc; L.S = G.S; L.gather_grid = G.gather_grid; L.col_on = G.col_on;
+      L.nodes_grid = blocks_for(nc);
+      std::vector<int32_t> match(nc), imatch(nf, -1);
+      for (int i = 0; i < nc; ++i) { match[i] = mt[l][i] - 1; imatch[match[i]] = i; }
+      // density of the matched fine node (laplc_c :1419-1426, laplc_cc :1512-1518)
+      std::vector<double> rho(nc);
+      for (int i = 0; i < nc; ++i) {
+        int f = l == 0 ? match[i] : chain[match[i]];
+        if (l == 0) chain[i] = f;
+        rho[i] = rh0[f];
+      }
+      std::vector<double> vol(cm->vol, cm->vol + nc);
+      if ((rc = upload(c, &L.match, match.data(), nc)) || (rc = upload(c, &L.imatch, imatch.data(), nf)) ||
+          (rc = upload(c, &L.rho, rho.data(), nc)) || (rc = upload(c, &L.vol, vol.data(), nc)) ||
+          (rc = dalloc(c, &L.pstep, nc)) || (rc = dalloc(c, &L.p, nc)) || (rc = dalloc(c, &L.r, nc)) ||
+          (rc = dalloc(c, &L.rrhs, nc)) || (rc = dalloc(c, &L.rout, nc)) || (rc = dalloc(c, &L.e, nc)))
+        return rc;
+      L.nc = c->nc;
+      L.nc.vol = L.vol;
+      L.nc.rh0 = L.rho;
+      L.nc.rh0_one = std::all_of(rho.begin(), rho.end(), [](double v) { return v == 1.0; }) ? 1 : 0;
+      // coefw and ue stay the fine arrays (indexed by the coarse node id)
+      L.beta = beta_host(cm, c->P.dom);
+    }
+    // compute_d_unstr(2), (3) (:1181-1199) with the density restricted from the fine mesh
+    c->mg_transfer = mgp->transfer;
+    {
+      auto &A = c->mg[0], &B = c->mg[1];
+      const dim3 b(NFT_BLOCK);
+      hipLaunchKernelGGL(k_mg_restrict, dim3(A.nodes_grid), b, 0, c->stream, A.n, A.match, c->inc, c->rh0, A.r,
+                         mgp->transfer, nullptr);
+      hipLaunchKernelGGL(k_mg_restrict, dim3(B.nodes_grid), b, 0, c->stream, B.n, B.match, A.inc, A.r, B.r,
+                         mgp->transfer, nullptr);
+      HIPCHK(hipGetLastError());
+      std::vector<double> r1(A.n), r2(B.n);
+      HIPCHK(hipMemcpyAsync(r1.data(), A.r, A.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(r2.data(), B.r, B.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      compute_d_host(m1, r1.data(), c->P.dt, A.h_d);
+      compute_d_host(m2, r2.data(), c->P.dt, B.h_d);
+      for (auto *L : {&A, &B}) {
+        std::vector<double> ps;
+        pstep_host(c->P.sm_flag, L->h_d, L->beta, ps);
+        HIPCHK(hipMemcpy(L->pstep, ps.data(), ps.size() * sizeof(double), hipMemcpyHostToDevice));
+      }
+    }
+    if (!c->mg_rrhs) {
+      if ((rc = dalloc(c, &c->mg_rrhs, c->n)) || (rc = dalloc(c, &c->mg_e, c->n))) return rc;
+    }
+    c->mg_ready = 1;
+  }
+  c->mg_itrp2 = mgp->itrp2; c->mg_c1 = mgp->itrp_c1; c->mg_c2 = mgp->itrp_c2; c->mg_cc = mgp->itrp_cc;
+  c->mg_on = mgp->mg_flag ? 1 : 0;
+  return 0;
+}
+
+int nft_multigrid_field(nft_ctx *c, int level, const char *name, double *host, int *count) {
+  if (!c || !name || level < 1 || level > 2) return set_err(-1, "nft_multigrid_field: bad argument");
+  if (!c->mg_ready) return set_err(-1, "nft_multigrid_field: no coarse levels set");
+  const auto &L = c->mg[level - 1];
+  const std::string s(name);
+  if (count) *count = L.n;
+  if (s == "d") {
+    if (host) std::copy(L.h_d.begin(), L.h_d.end(), host);
+  } else if (s == "beta") {
+    if (count) *count = 1;
+    if (host) host[0] = L.beta;
+  } else if (s == "rh0" || s == "pstep") {
+    if (host) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      HIPCHK(hipMemcpy(host, s == "rh0" ? L.rho : L.pstep, L.n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+  } else {
+    return set_err(-1, "nft_multigrid_field: unknown field " + s);
+  }
+  return 0;
+}
+
+// restriction_unstr / prolongation_unstr / laplc_c / laplc_cc on host arrays
+int nft_multigrid_op(nft_ctx *c, int op, int level, const double *in, double *out, int impfl) {
+  if (!c || !in || !out || level < 1 || level > 2) return set_err(-1, "nft_multigrid_op: bad argument");
+  if (!c->mg_ready) return set_err(-1, "nft_multigrid_op: no coarse levels set");
+  auto &L = c->mg[level - 1];
+  const int nfine = level == 1 ? c->n : c->mg[0].n;
+  const Incidence &fine = level == 1 ? c->inc : c->mg[0].inc;
+  const dim3 b(NFT_BLOCK);
+  double *din = c->io[0], *dout = c->io[1];
+  int nin, nout;
+  switch (op) {
+    case 0: nin = nfine; nout = L.n; break;    // restriction
+    case 1: nin = L.n; nout = nfine; break;    // prolongation
+    case 2: nin = L.n; nout = L.n; break;      // laplc on the level
+    default: return set_err(-1, "nft_multigrid_op: op must be 0, 1 or 2");
+  }
+  HIPCHK(hipMemcpyAsync(din, in, nin * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (op == 0)
+    hipLaunchKernelGGL(k_mg_restrict, dim3(L.nodes_grid), b, 0, c->stream, L.n, L.match, fine, din, dout,
+                       c->mg_transfer, nullptr);
+  else if (op == 1)
+    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks_for(nfine)), b, 0, c->stream, nfine, L.imatch, fine, din, dout,
+                       c->mg_transfer, nullptr);
+  else {
+    int rc = laplc_level(c, L, din, dout, impfl);
+    if (rc) return rc;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, dout, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 static const char kStateMagic[8] = {'N', 'F', 'T', 'S', 'T', 'A', 'T', '1'};
 static const char *kStateBlocks[5] = {"unk", "frc", "uuu", "unkdash", "prpot"};
 static const int kStateK[5] = {4, 4, 3, 3, 1};

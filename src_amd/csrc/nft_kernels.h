@@ -1081,6 +1081,78 @@ __global__ void k_copy(int n, const double *a, double *b, const double *skip) {
   if (k < n) b[k] = a[k];
 }
 
+// ------------------------------------------------------------------------------------
+// Multigrid branch of precon (lippsLFroude_implgwopks3D_unstr3.f:618-748): grid transfers
+// and the coarse-level point steps. One thread per node; the neighbours of a fine node
+// are its incidence slots, which hold them in neigh_definition's order (interior edges,
+// then boundary-face edges, each in input order).
+// ------------------------------------------------------------------------------------
+enum MgTransfer {
+  MG_AS_REFERENCE = 0,  // the neighbour counter q = 1..count used as a node id (:817, :913)
+  MG_NEIGHBOURS = 1,    // the neighbour list
+};
+
+__device__ __forceinline__ int mg_neighbour(const Incidence &f, int ip, int q, int transfer) {
+  if (transfer == MG_AS_REFERENCE) return q;
+  const int ch = ip / NFT_WAVE, r = ip % NFT_WAVE;
+  return f.nbr[f.chunk_ptr[ch] + (long)q * NFT_WAVE + r];
+}
+
+// restriction_unstr (:757-822): out[ic] = (sum_q (f[ip] + f[j_q]) / 2) / count, ip = match[ic]
+__global__ void __launch_bounds__(NFT_BLOCK) k_mg_restrict(int nc, const int32_t *match, Incidence fine,
+                                                           const double *f, double *out, int transfer,
+                                                           const double *skip) {
+  if (skipped(skip)) return;
+  const int ic = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (ic >= nc) return;
+  const int ip = match[ic];
+  const int cnt = fine.deg[ip];
+  const double fi = f[ip];
+  double sc = 0.;
+  for (int q = 0; q < cnt; ++q) sc = sc + (fi + f[mg_neighbour(fine, ip, q, transfer)]) / 2.0;
+  out[ic] = sc / (double)cnt;
+}
+
+// prolongation_unstr (:828-940): injection at the fine nodes that carry a coarse node, the
+// mean of such neighbours elsewhere (0 where there is none). A node counts as carrying a
+// value when it differs from the reference's marker -99999.
+__global__ void __launch_bounds__(NFT_BLOCK) k_mg_prolong(int nf, const int32_t *imatch, Incidence fine,
+                                                          const double *coarse, double *out, int transfer,
+                                                          const double *skip) {
+  if (skipped(skip)) return;
+  const int ip = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (ip >= nf) return;
+  const double marker = -99999.;
+  const int ic = imatch[ip];
+  const double v0 = ic >= 0 ? coarse[ic] : 0.;
+  if (ic >= 0 && v0 != marker) { out[ip] = v0; return; }
+  const int cnt = fine.deg[ip];
+  double sc = v0, filled = 0.;
+  for (int q = 0; q < cnt; ++q) {
+    const int jc = imatch[mg_neighbour(fine, ip, q, transfer)];
+    if (jc >= 0) {
+      const double vj = coarse[jc];
+      if (vj != marker) { sc = sc + vj; filled = filled + 1.; }
+    }
+  }
+  out[ip] = filled != 0. ? sc / filled : sc;
+}
+
+// start of the coarsest relaxation (:701-703): p = -(rrhs * pstep * omega), in that order
+__global__ void k_mg_start_cc(int n, const double *pstep, const double *rrhs, double *p, double omega,
+                              const double *skip) {
+  if (skipped(skip)) return;
+  const int k = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (k < n) p[k] = -(rrhs[k] * pstep[k] * omega);
+}
+
+// coarse-grid correction (:716-718, :733-735): a += b
+__global__ void k_mg_add(int n, double *a, const double *b, const double *skip) {
+  if (skipped(skip)) return;
+  const int k = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (k < n) a[k] = a[k] + b[k];
+}
+
 // start of a device-controlled GCR loop: flag cleared, pass count 0, previous sum r^2
 __global__ void k_gcr_ctl_begin(double *scal, double rl20) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {

@@ -222,7 +222,6 @@ def terrain(spec: MeshSpec, xy: np.ndarray) -> np.ndarray:
 
 
 def generate(spec: MeshSpec) -> Mesh:
-    lib = _meshlib()
     xy = jittered_points(spec)
     perm = column_permutation(spec)
     rank = None
@@ -231,9 +230,65 @@ def generate(spec: MeshSpec) -> Mesh:
         # tetrahedra are those of the grid-ordered mesh
         xy = np.ascontiguousarray(xy[:, perm])
         rank = np.ascontiguousarray(perm.astype(np.int32))
+    zeta = zeta_levels(spec.nlev, spec.htop, spec.dz_bottom, spec.dz_top)
+    return _build(spec, xy, zeta, rank)
+
+
+def _coarse_index(n: int) -> np.ndarray:
+    """Every other index of 0..n-1, always keeping the last one (the box edge / model top)."""
+    idx = list(range(0, n, 2))
+    if idx[-1] != n - 1:
+        idx.append(n - 1)
+    return np.array(idx, dtype=np.int64)
+
+
+def multigrid_meshes(spec: MeshSpec, levels: int = 2) -> list:
+    """Nested coarse meshes for precon's multigrid branch (the reference reads them from
+    mesh3d_c.d / mesh3d_cc.d and the node maps from match1.d / match2.d,
+    lippsLFroude_implgwopks3D_unstr3.f:2415-2672).
+
+    Level l keeps every other grid column in x and y and every other level of level l-1
+    (the box edges and the model top are always kept), re-triangulated and extruded with the
+    same terrain and level heights, so every coarse node sits exactly on a node of the
+    finer mesh. Returns [(mesh, match), ...] from the first coarse level down, match being
+    the 1-based id in the next finer mesh of each coarse node (the reference's match1 /
+    match2). Grid column order only."""
+    if spec.column_order != "grid":
+        raise ValueError("multigrid meshes are built for the grid column order")
+    xy = jittered_points(spec)
+    zeta = zeta_levels(spec.nlev, spec.htop, spec.dz_bottom, spec.dz_top)
+    ix, iy, kz = (np.arange(spec.nx), np.arange(spec.ny), np.arange(spec.nlev))
+    out = []
+    for _ in range(levels):
+        sx, sy, sk = _coarse_index(len(ix)), _coarse_index(len(iy)), _coarse_index(len(kz))
+        cx, cy, ck = ix[sx], iy[sy], kz[sk]
+        ncx, ncy, ncl = len(cx), len(cy), len(ck)
+        fine_cols = (cy[:, None] * spec.nx + cx[None, :]).ravel()         # fine-grid column ids
+        prev_cols = (sy[:, None] * len(ix) + sx[None, :]).ravel()         # ids in level l-1
+        sub = dataclasses.replace(spec, nx=ncx, ny=ncy, nlev=ncl)
+        m = _build(sub, np.ascontiguousarray(xy[:, fine_cols]), zeta[ck], None)
+        match = (prev_cols[:, None] * len(kz) + sk[None, :]).ravel() + 1
+        out.append((m, np.ascontiguousarray(match.astype(np.int32))))
+        ix, iy, kz = cx, cy, ck
+    return out
+
+
+def as_reference_reads(coarse2: Mesh, coarse1: Mesh) -> Mesh:
+    """The second coarse mesh as the reference holds it after `input`: its edge metrics'
+    y and z components are read from the first coarse mesh's array
+    (lippsLFroude_implgwopks3D_unstr3.f:2467-2468, sn_cc(ie,2:3) = sn_c(ie,2:3))."""
+    if coarse2.nedge > coarse1.nedge:
+        raise ValueError("coarse mesh 2 has more edges than coarse mesh 1")
+    sn = coarse2.sn.copy()
+    sn[1:3, :] = coarse1.sn[1:3, :coarse2.nedge]
+    return dataclasses.replace(coarse2, sn=sn, _handle=None)
+
+
+def _build(spec: MeshSpec, xy: np.ndarray, zeta: np.ndarray, rank) -> Mesh:
+    lib = _meshlib()
     tri = delaunay(xy)
     zs = np.ascontiguousarray(terrain(spec, xy))
-    zeta = np.ascontiguousarray(zeta_levels(spec.nlev, spec.htop, spec.dz_bottom, spec.dz_top))
+    zeta = np.ascontiguousarray(zeta, dtype=np.float64)
     box = np.array([0.0, spec.lx, 0.0, spec.ly])
     err = ctypes.create_string_buffer(256)
     h = lib.nft_meshgen_build(xy.shape[1], _dp(xy), tri.shape[1], _ip(tri), _dp(zs), spec.nlev,

@@ -114,6 +114,11 @@ _SIGS.update({
                                        ctypes.POINTER(ctypes.c_int)]),
     "nft_net_options": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "nft_set_incidence": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "nft_set_multigrid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
+                                         ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]),
+    "nft_multigrid_field": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, f64p,
+                                           ctypes.POINTER(ctypes.c_int)]),
+    "nft_multigrid_op": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, f64p, f64p, ctypes.c_int]),
     "nft_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(NftDiag)]),
     "nft_set_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "nft_get_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(NftDiag),
@@ -201,6 +206,11 @@ def make_params(cfg, precon=None, sm_flag=1, itrp1=None, omega=None, eps0=None, 
         p.dom[i] = v
     p.precision = PRECISION[precision] if isinstance(precision, str) else int(precision)
     return p
+
+
+class NftMultigrid(ctypes.Structure):
+    _fields_ = [("mg_flag", ctypes.c_int), ("itrp2", ctypes.c_int), ("itrp_c1", ctypes.c_int),
+                ("itrp_c2", ctypes.c_int), ("itrp_cc", ctypes.c_int), ("transfer", ctypes.c_int)]
 
 
 def _mesh_struct(mesh, use_columns=True):
@@ -435,6 +445,38 @@ class Solver:
         ms = ctypes.c_double()
         _chk(lib().nft_copy_calibration(self.h, reps, ctypes.byref(ms)), "nft_copy_calibration")
         return ms.value
+
+    def set_multigrid(self, levels=None, mg_flag=1, itrp2=0, itrp_c1=1, itrp_c2=1, itrp_cc=1, transfer=0):
+        """precon's multigrid branch: levels = [(mesh_c, match1), (mesh_cc, match2)] as
+        mesh.multigrid_meshes returns them (None keeps the levels of an earlier call)."""
+        mg = NftMultigrid(mg_flag, itrp2, itrp_c1, itrp_c2, itrp_cc, transfer)
+        if levels is None:
+            _chk(lib().nft_set_multigrid(self.h, None, None, None, None, ctypes.byref(mg)), "nft_set_multigrid")
+            return
+        (m1, t1), (m2, t2) = levels
+        s1, k1 = _mesh_struct(m1)
+        s2, k2 = _mesh_struct(m2)
+        t1 = np.ascontiguousarray(t1, np.int32); t2 = np.ascontiguousarray(t2, np.int32)
+        _chk(lib().nft_set_multigrid(self.h, ctypes.byref(s1), _p(t1), ctypes.byref(s2), _p(t2), ctypes.byref(mg)),
+             "nft_set_multigrid")
+        self._mg_n = (self.n, m1.npoin, m2.npoin)
+
+    def multigrid_field(self, level, name):
+        cnt = ctypes.c_int()
+        _chk(lib().nft_multigrid_field(self.h, level, name.encode(), None, ctypes.byref(cnt)), "nft_multigrid_field")
+        out = np.zeros(cnt.value)
+        _chk(lib().nft_multigrid_field(self.h, level, name.encode(), _p(out), ctypes.byref(cnt)),
+             "nft_multigrid_field")
+        return out
+
+    def multigrid_op(self, op, level, x, impfl=1):
+        """op "restrict" (fine -> level), "prolong" (level -> finer), "laplc" (on the level)."""
+        code = {"restrict": 0, "prolong": 1, "laplc": 2}[op]
+        nout = {0: self._mg_n[level], 1: self._mg_n[level - 1], 2: self._mg_n[level]}[code]
+        out = np.zeros(nout)
+        _chk(lib().nft_multigrid_op(self.h, code, level, _p(np.ascontiguousarray(x, np.float64)), _p(out), impfl),
+             "nft_multigrid_op")
+        return out
 
     def set_incidence(self, columnar):
         """1: columnar gathers (when available), 0: SELL-64, -1: no change. Returns availability."""

@@ -82,3 +82,19 @@ def parse_diagnostics(stdout):
             row += _e11(lines[k + 4 + j].split(":", 1)[1], 4)
         rows.append(row)
     return np.array(rows)
+
+
+# sweeps of precon's V-cycle (input_mg_v3.d: itrp2, itrp_c1, itrp_c2, itrp_cc) used by the
+# multigrid fixtures and tests
+MG_SWEEPS = dict(itrp2=1, itrp_c1=2, itrp_c2=2, itrp_cc=3)
+
+
+@functools.lru_cache(maxsize=None)
+def mg_levels_for(spec_key, as_read=False):
+    """Nested coarse meshes of mesh_for(spec_key); as_read=True gives the second one as the
+    reference holds it after reading (its sn y/z columns taken from the first)."""
+    spec = mesh_for(spec_key).spec
+    (m1, t1), (m2, t2) = M.multigrid_meshes(spec)
+    if as_read:
+        m2 = M.as_reference_reads(m2, m1)
+    return [(m1, t1), (m2, t2)]

@@ -9,6 +9,8 @@ generated meshes, runs
   * the reference program itself (`run` mode) for 1 and 10 steps on the tiny mesh
     (tiny_run.npz, with the diagnostics it prints after the initialisation and every
     step) and for 1, 10 and 100 steps on the 40,000-node mesh (small_run.npz),
+  * the multigrid branch of precon (mg_flag=1) on the tiny mesh's generated nested coarse
+    meshes: its pieces in `mgops` mode and 1 and 10 steps of the program (tiny_mg.npz),
 and stores inputs and outputs. Only numbers are stored.
 """
 from __future__ import annotations
@@ -124,6 +126,41 @@ def run_ops(exe, m, cfg, d):
     return out
 
 
+# multigrid branch (mg_flag=1): sweeps of the V-cycle on the tiny mesh's nested levels
+MG_SWEEPS = dict(itrp2=1, itrp_c1=2, itrp_c2=2, itrp_cc=3)
+
+
+def run_mgops(exe, m, levels, cfg, d):
+    write_reference_inputs(d, m, cfg, ntime=1, mg=dict(levels=levels, **MG_SWEEPS), **PARITY)
+    with open(os.path.join(d, "harness.d"), "w") as f:
+        f.write(strat_literal(cfg) + "\n")
+    v = seeded_vectors(m)
+    with open(os.path.join(d, "vec_in.bin"), "wb") as f:
+        for k in range(8):
+            f.write(v[k].tobytes())
+    r = subprocess.run([exe, "mgops"], cwd=d, capture_output=True, text=True,
+                       preexec_fn=_unlimited_stack)
+    if r.returncode != 0:
+        raise RuntimeError(r.stdout + r.stderr)
+    raw = np.fromfile(os.path.join(d, "vec_out.bin"), dtype=np.float64)
+    n, n1, n2 = m.npoin, levels[0][0].npoin, levels[1][0].npoin
+    out, pos = {}, 0
+
+    def take(name, k):
+        nonlocal pos
+        out[name] = raw[pos:pos + k].copy()
+        pos += k
+
+    take("d2", n1); take("d3", n2); take("beta23", 2)
+    take("restrict_1", n1); take("restrict_2", n2)
+    take("prolong_1", n); take("prolong_2", n1)
+    take("laplc_c_1", n1); take("laplc_c_0", n1); take("laplc_cc_1", n2)
+    take("precon", n)
+    take("gcrk_p", n); take("gcrk_u", 3 * n); take("gcrk_stats", 3)
+    assert pos == raw.size, (pos, raw.size)
+    return out
+
+
 def read_restart(d, n):
     files = glob.glob(os.path.join(d, "Rstart_Out*.d"))
     assert len(files) == 1, files
@@ -148,8 +185,8 @@ def read_restart(d, n):
 NITER_RE = re.compile(r"niter,nitav:\s*(\d+)\s+(\d+)")
 
 
-def run_steps(exe, m, cfg, d, ntime):
-    write_reference_inputs(d, m, cfg, ntime=ntime, **PARITY)
+def run_steps(exe, m, cfg, d, ntime, mg=None):
+    write_reference_inputs(d, m, cfg, ntime=ntime, mg=mg, **PARITY)
     r = subprocess.run([exe, "run"], cwd=d, capture_output=True, text=True,
                        preexec_fn=_unlimited_stack)
     if r.returncode != 0:
@@ -176,6 +213,18 @@ def main():
             for k, val in st.items():
                 runs[f"s{nt}_{k}"] = val
         np.savez_compressed(os.path.join(OUT, "tiny_run.npz"), mesh_checksum=mesh_checksum(m), **runs)
+        # multigrid branch on generated nested coarse meshes
+        levels = M.multigrid_meshes(tcfg.mesh)
+        mg = run_mgops(exe, m, levels, tcfg, d)
+        for nt in (1, 10):
+            st = run_steps(exe, m, tcfg, d, nt, mg=dict(levels=levels, **MG_SWEEPS))
+            for k in ("unk", "uuu", "frc", "unkdash", "prpot", "niter"):
+                mg[f"s{nt}_{k}"] = st[k]
+        np.savez_compressed(os.path.join(OUT, "tiny_mg.npz"), mesh_checksum=mesh_checksum(m),
+                            level_checksums=np.stack([mesh_checksum(lm) for lm, _ in levels]),
+                            match1=levels[0][1], match2=levels[1][1],
+                            sweeps=np.array([MG_SWEEPS[k] for k in ("itrp2", "itrp_c1", "itrp_c2", "itrp_cc")]),
+                            **mg)
     print("tiny fixtures written")
     if only_tiny:
         return
