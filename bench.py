@@ -123,6 +123,34 @@ def learned_run(mesh, cfg, steps, train_steps, log_prefix="[learned]"):
                           "objective": "|L z - r|^2/|r|^2, z = M_line^-1 r + s N(r/s)"})
 
 
+def multigrid_run(mesh, cfg, steps, kind):
+    """precon's multigrid branch on the same workload. kind "reference": the reference's
+    own preconditioner settings (two point-Jacobi sweeps at omega 0.5, V-cycle sweeps
+    1/2/2/3, its transfers); kind "line": the vertical line preconditioner as the fine
+    smoother with the corrected transfers, coarse sweeps 0/4/4/16 at omega 0.5."""
+    import dataclasses
+    from src_amd import mesh as M
+    from src_amd import nft
+    levels = M.multigrid_meshes(cfg.mesh)
+    if kind == "reference":
+        c = dataclasses.replace(cfg, precon=1, itrp1=2, omega=0.5)
+        mg = dict(transfer=0, itrp2=1, itrp_c1=2, itrp_c2=2, itrp_cc=3)
+    else:
+        c = cfg
+        mg = dict(transfer=1, itrp2=0, itrp_c1=4, itrp_c2=4, itrp_cc=16, omega_c=0.5)
+    s = nft.Solver(mesh, nft.make_params(c))
+    s.set_multigrid(levels, **mg)
+    s.state_init()
+    st = s.run_steps(steps)
+    s.close()
+    return dict(value=round(1000.0 / (st.ms_total / steps), 4), unit="timesteps/s",
+                iters_per_solve=round(st.gcr_iters / max(st.gcr_solves, 1), 2),
+                solve_ms=round(st.ms_gcr / max(st.gcr_solves, 1), 3), steps=steps,
+                unconverged=st.gcr_unconverged,
+                levels=[mesh.npoin] + [lm.npoin for lm, _ in levels],
+                settings=dict(precon="point Jacobi" if kind == "reference" else "vertical line", **mg))
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (profiles/production_fp64_traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
@@ -278,6 +306,12 @@ def main():
             "solve_ms": round(sm.ms_gcr / max(sm.gcr_solves, 1), 3),
             "mpdata_ms_per_step": round(sm.ms_mpdata / mk, 3), "steps": mk}}
         line["variants"]["learned"] = learned_run(mesh, cfg, mk, args.learned_train_steps)
+        if args.mesh == "production":
+            for kind in ("reference", "line"):
+                t0 = time.perf_counter()
+                line["variants"][f"multigrid_{kind}"] = multigrid_run(mesh, cfg, 2, kind)
+                log(f"[multigrid {kind}] {line['variants'][f'multigrid_{kind}']} "
+                    f"({time.perf_counter() - t0:.1f} s)")
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         line["cpu_baseline"] = cpu_baseline(mesh, cfg, iters, args.mesh)

@@ -155,6 +155,9 @@ typedef struct {
   int itrp2, itrp_c1, itrp_c2, itrp_cc;  /* sweeps as in input_mg_v3.d */
   int transfer;  /* 0: restriction/prolongation index the fine array by the neighbour
                     counter, as the reference does (:817, :913); 1: by the neighbour list */
+  double omega_c;  /* relaxation factor of the coarse-level point steps; <= 0: omega, as
+                      the reference (no counterpart otherwise: its coarse Jacobi with the
+                      d2/d3 diagonal estimates diverges for omega >= 0.7) */
 } nft_multigrid;
 /* Coarse levels and V-cycle settings. With m1 == NULL only mg_flag and the sweep counts
  * change (the levels of an earlier call stay). */

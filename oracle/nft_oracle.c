@@ -44,6 +44,7 @@ struct oc_ctx {
   struct oc_ctx *lv[2];
   int32_t *lv_match[2], *lv_imatch[2];
   int mg_on, mg_itrp2, mg_c1, mg_c2, mg_cc, mg_transfer;
+  double mg_omega;   /* relaxation factor of the coarse levels (the reference: omega) */
   double *mg_w[12];
 };
 
@@ -710,7 +711,7 @@ static void mg_prolong(const oc_ctx *f, const int32_t *imatch, const double *coa
 static void mg_cycle(oc_ctx *c, const double *rhs, double *p, double *r, int impfl) {
   oc_ctx *c1 = c->lv[0], *c2 = c->lv[1];
   int n = c->n, n1 = c1->n, n2 = c2->n, tr = c->mg_transfer;
-  double om = c->p.omega;
+  double om = c->mg_omega;
   double *rrhs = c->mg_w[0], *e = c->mg_w[1];
   double *p_c = c->mg_w[2], *r_c = c->mg_w[3], *rrhs_c = c->mg_w[4], *rout_c = c->mg_w[5];
   double *e_c = c->mg_w[6], *ps_c = c->mg_w[7];
@@ -814,9 +815,12 @@ int oc_set_multigrid(oc_ctx *c, const oc_mesh *m1, const int32_t *match1, const 
     c->mg_w[i] = zalloc((size_t)c->n);
   }
   c->mg_itrp2 = itrp2; c->mg_c1 = itrp_c1; c->mg_c2 = itrp_c2; c->mg_cc = itrp_cc;
+  c->mg_omega = c->p.omega;
   c->mg_on = 1;
   return 0;
 }
+
+void oc_set_multigrid_omega(oc_ctx *c, double omega_c) { c->mg_omega = omega_c > 0 ? omega_c : c->p.omega; }
 
 void oc_mg_restrict(oc_ctx *c, int level, const double *src, double *out) {
   const oc_ctx *f = level == 1 ? c : c->lv[0];

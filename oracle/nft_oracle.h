@@ -95,6 +95,9 @@ void oc_line_solve(oc_ctx *c, const double *rhs, double *z);
 int oc_set_multigrid(oc_ctx *c, const oc_mesh *m1, const int32_t *match1, const oc_mesh *m2,
                      const int32_t *match2, int itrp2, int itrp_c1, int itrp_c2, int itrp_cc,
                      int transfer);
+/* relaxation factor of the coarse-level point steps (no reference counterpart: the
+ * reference uses omega on every level, which is also the default; <= 0 restores it) */
+void oc_set_multigrid_omega(oc_ctx *c, double omega_c);
 /* restriction_unstr(.,.,level) and prolongation_unstr(.,.,level) */
 void oc_mg_restrict(oc_ctx *c, int level, const double *src, double *out);
 void oc_mg_prolong(oc_ctx *c, int level, const double *coarse, double *out);

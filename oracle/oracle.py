@@ -79,6 +79,7 @@ def lib():
         L.oc_set_multigrid.restype = ctypes.c_int
         L.oc_set_multigrid.argtypes = [vp, ctypes.POINTER(OcMesh), i32p, ctypes.POINTER(OcMesh), i32p,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.oc_set_multigrid_omega.argtypes = [vp, ctypes.c_double]
         L.oc_mg_restrict.argtypes = [vp, ctypes.c_int, f64p, f64p]
         L.oc_mg_prolong.argtypes = [vp, ctypes.c_int, f64p, f64p]
         L.oc_level.restype = vp
@@ -218,7 +219,7 @@ class Oracle:
         for _ in range(n):
             lib().oc_nft_step(self.h)
 
-    def set_multigrid(self, levels, itrp2=0, itrp_c1=1, itrp_c2=1, itrp_cc=1, transfer=0):
+    def set_multigrid(self, levels, itrp2=0, itrp_c1=1, itrp_c2=1, itrp_cc=1, transfer=0, omega_c=0.0):
         """Turns on precon's multigrid branch (mg_flag=1) with levels = [(mesh_c, match1),
         (mesh_cc, match2)] as mesh.multigrid_meshes returns them."""
         (m1, t1), (m2, t2) = levels
@@ -234,6 +235,7 @@ class Oracle:
                                     _p(t2), itrp2, itrp_c1, itrp_c2, itrp_cc, transfer)
         if rc != 0:
             raise ValueError("oc_set_multigrid rejected the levels")
+        lib().oc_set_multigrid_omega(self.h, omega_c)
         self._mg_keep = (keep, structs, t1, t2)
         self._mg_n = (self.n, m1.npoin, m2.npoin)
 

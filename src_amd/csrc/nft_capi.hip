@@ -133,6 +133,7 @@ struct nft_ctx {
     double beta = 0;             // betaunstr2 / betaunstr3
   } mg[2];
   int mg_ready = 0, mg_on = 0, mg_itrp2 = 0, mg_c1 = 0, mg_c2 = 0, mg_cc = 0, mg_transfer = 0;
+  double mg_omega = 0;   // relaxation factor of the coarse levels
   double *mg_rrhs = nullptr, *mg_e = nullptr;
 
   // learned preconditioner (network, training pairs, inference state)
@@ -632,7 +633,7 @@ int relax_level(nft_ctx *c, nft_ctx::MgLevel &L, int sweeps, int impfl) {
     int rc = laplc_level(c, L, L.p, L.r, impfl);
     if (rc) return rc;
     hipLaunchKernelGGL(k_point, dim3(L.nodes_grid), dim3(NFT_BLOCK), 0, c->stream, L.n, L.pstep, L.r, L.rrhs,
-                       L.p, c->P.omega, 0, c->skip);
+                       L.p, c->mg_omega, 0, c->skip);
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -642,7 +643,7 @@ int relax_level(nft_ctx *c, nft_ctx::MgLevel &L, int sweeps, int impfl) {
 int mg_cycle_dev(nft_ctx *c, const double *rhs, double *p, int impfl) {
   auto &A = c->mg[0], &B = c->mg[1];
   const int n = c->n, tr = c->mg_transfer;
-  const double om = c->P.omega;
+  const double om = c->mg_omega;
   const dim3 b(NFT_BLOCK);
   int rc;
   // fine residual, restricted to level 1
@@ -2414,6 +2415,7 @@ int nft_set_multigrid(nft_ctx *c, const nft_mesh *m1, const int32_t *match1, con
     c->mg_ready = 1;
   }
   c->mg_itrp2 = mgp->itrp2; c->mg_c1 = mgp->itrp_c1; c->mg_c2 = mgp->itrp_c2; c->mg_cc = mgp->itrp_cc;
+  c->mg_omega = mgp->omega_c > 0 ? mgp->omega_c : c->P.omega;
   c->mg_on = mgp->mg_flag ? 1 : 0;
   return 0;
 }

@@ -58,6 +58,12 @@ module nft_hip
     integer(c_int64_t) :: step
   end type
 
+  ! V-cycle settings of precon's multigrid branch (nft_multigrid in nft.h)
+  type, bind(C) :: nft_multigrid_t
+    integer(c_int) :: mg_flag, itrp2, itrp_c1, itrp_c2, itrp_cc, transfer
+    real(c_double) :: omega_c
+  end type
+
   type(c_ptr), save :: nft_context = c_null_ptr
   real(c_double), save :: nft_dt = 0.0d0
 
@@ -67,6 +73,13 @@ module nft_hip
       type(nft_mesh_t), intent(in) :: mesh
       type(nft_params_t), intent(in) :: params
       type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function nft_set_multigrid(ctx, m1, match1, m2, match2, mg) bind(C, name="nft_set_multigrid")
+      import :: nft_mesh_t, nft_multigrid_t, c_ptr, c_int
+      type(c_ptr), value :: ctx
+      type(nft_mesh_t), intent(in) :: m1, m2
+      type(c_ptr), value :: match1, match2
+      type(nft_multigrid_t), intent(in) :: mg
     end function
     integer(c_int) function nft_finalize(ctx) bind(C, name="nft_finalize")
       import :: c_ptr, c_int
@@ -278,6 +291,57 @@ contains
     p%precision = precision
     nft_dt = delt
     call nft_check(nft_init(m, p, nft_context), 'nft_init')
+    if (mg_flag == 1) call nft_attach_multigrid(itrp2, itrp_c1, itrp_c2, itrp_cc)
+  end subroutine
+
+  ! The coarse meshes and node maps of precon's multigrid branch as `input` leaves them in
+  ! COMMON (:2343-2672; /edg_c/ ... /match/), handed to nft_set_multigrid. The arrays are
+  ! passed as read, so the reference's level-2 metrics (sn_cc y/z from sn_c, :2467-2468)
+  ! and its transfers (transfer 0) are reproduced.
+  subroutine nft_attach_multigrid(itrp2, itrp_c1, itrp_c2, itrp_cc)
+    integer, intent(in) :: itrp2, itrp_c1, itrp_c2, itrp_cc
+    integer :: npoin_c, nedge_c, nbface_c, nbpoin_c, nboun_c
+    integer :: npoin_cc, nedge_cc, nbface_cc, nbpoin_cc, nboun_cc
+    common/n_c/npoin_c,nedge_c,nbface_c,nbpoin_c,nboun_c
+    common/n_cc/npoin_cc,nedge_cc,nbface_cc,nbpoin_cc,nboun_cc
+    real(8) :: coord_c, vol_c, sn_c, sb_c, sbb_c, anp_c
+    real(8) :: coord_cc, vol_cc, sn_cc, sb_cc, sbb_cc, anp_cc
+    integer :: iedge_c, kbface_c, ibpoin_c, iedge_cc, kbface_cc, ibpoin_cc
+    common/xyz_c/coord_c(npoinx,3)
+    common/volume_c/vol_c(npoinx)
+    common/edg_c/iedge_c(nedgex,2)
+    common/wge_c/sn_c(nedgex,3)
+    common/bound_c/kbface_c(nfacesx,1:2),sb_c(nfacesx,3)
+    common/bpoints_c/ibpoin_c(nfacesx,2),anp_c(nfacesx,3)
+    common/terms_c/sbb_c(nfacesx,3)
+    common/xyz_cc/coord_cc(npoinx,3)
+    common/volume_cc/vol_cc(npoinx)
+    common/edg_cc/iedge_cc(nedgex,2)
+    common/wge_cc/sn_cc(nedgex,3)
+    common/bound_cc/kbface_cc(nfacesx,1:2),sb_cc(nfacesx,3)
+    common/bpoints_cc/ibpoin_cc(nfacesx,2),anp_cc(nfacesx,3)
+    common/terms_cc/sbb_cc(nfacesx,3)
+    integer :: match1, match2
+    common/match/match1(npoinx),match2(npoinx)
+    target :: coord_c, vol_c, sn_c, sb_c, sbb_c, anp_c, iedge_c, kbface_c, ibpoin_c
+    target :: coord_cc, vol_cc, sn_cc, sb_cc, sbb_cc, anp_cc, iedge_cc, kbface_cc, ibpoin_cc
+    target :: match1, match2
+    type(nft_mesh_t) :: m1, m2
+    type(nft_multigrid_t) :: mg
+    m1%npoin = npoin_c; m1%nedge = nedge_c; m1%nbface = nbface_c; m1%nboun = nboun_c
+    m1%ld_poin = npoinx; m1%ld_edge = nedgex; m1%ld_face = nfacesx
+    m1%iedge = c_loc(iedge_c); m1%sn = c_loc(sn_c); m1%kbface = c_loc(kbface_c); m1%sb = c_loc(sb_c)
+    m1%ibpoin = c_loc(ibpoin_c); m1%sbb = c_loc(sbb_c); m1%anp = c_loc(anp_c)
+    m1%coord = c_loc(coord_c); m1%vol = c_loc(vol_c); m1%ncol = 0; m1%col_start = c_null_ptr
+    m2%npoin = npoin_cc; m2%nedge = nedge_cc; m2%nbface = nbface_cc; m2%nboun = nboun_cc
+    m2%ld_poin = npoinx; m2%ld_edge = nedgex; m2%ld_face = nfacesx
+    m2%iedge = c_loc(iedge_cc); m2%sn = c_loc(sn_cc); m2%kbface = c_loc(kbface_cc); m2%sb = c_loc(sb_cc)
+    m2%ibpoin = c_loc(ibpoin_cc); m2%sbb = c_loc(sbb_cc); m2%anp = c_loc(anp_cc)
+    m2%coord = c_loc(coord_cc); m2%vol = c_loc(vol_cc); m2%ncol = 0; m2%col_start = c_null_ptr
+    mg%mg_flag = 1; mg%itrp2 = itrp2; mg%itrp_c1 = itrp_c1; mg%itrp_c2 = itrp_c2; mg%itrp_cc = itrp_cc
+    mg%transfer = 0; mg%omega_c = 0.0d0
+    call nft_check(nft_set_multigrid(nft_context, m1, c_loc(match1), m2, c_loc(match2), mg), &
+                   'nft_set_multigrid')
   end subroutine
 
 end module nft_hip

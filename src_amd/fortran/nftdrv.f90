@@ -5,7 +5,9 @@
 ! :2265-2321 and :2343-2501), hands the mesh to nft_init through iso_c_binding, runs the
 ! mpsolver initialisation and `ntime` steps of its DO ITIME loop on the GPU, prints the
 ! reference's per-step iteration line and writes the final state (unk, uuu, prpot) as
-! raw float64 to nftdrv_state.bin.
+! raw float64 to nftdrv_state.bin. With mg_flag=1 in input_mg_v3.d it also reads the coarse
+! meshes mesh3d_c.d / mesh3d_cc.d and the node maps match1.d / match2.d and runs precon's
+! multigrid branch.
 !
 ! Usage: nftdrv <nlev> <precon> <precision> <lord> <strat> [exit_mode]
 !   nlev      levels per column when nodes are numbered column by column (0: no columns)
@@ -22,8 +24,15 @@ program nftdrv
   real(8) :: cfl, delt, adapt, relax, p0inf, u0inf, v0inf, w0inf, re, eps0, betap, strat
   real(8) :: sm1, sm2, omega
   real(8) :: ab(6), tw(6)
-  real(8), allocatable, target :: coord(:,:), vol(:), sn(:,:), sb(:,:), sbb(:,:), anp(:,:)
-  integer(c_int), allocatable, target :: iedge(:,:), kbface(:,:), ibpoin(:,:), cols(:)
+  type mesh_arrays
+    integer :: npoin = 0, nedge = 0, nbface = 0, nboun = 0
+    real(8), allocatable :: coord(:,:), vol(:), sn(:,:), sb(:,:), sbb(:,:), anp(:,:)
+    integer(c_int), allocatable :: iedge(:,:), kbface(:,:), ibpoin(:,:)
+  end type
+  type(mesh_arrays), target :: fine, lv(2)
+  integer(c_int), allocatable, target :: cols(:), match1(:), match2(:)
+  type(nft_mesh_t) :: m1, m2
+  type(nft_multigrid_t) :: mg
   real(8), allocatable :: state(:,:)
   type(nft_mesh_t) :: m
   type(nft_params_t) :: p
@@ -67,36 +76,8 @@ program nftdrv
     error stop 2
   end if
 
-  open(5, file='mesh3d.d', status='old')
-  read(5,*) npoin, nedge, nbface, nbpoin, nboun
-  allocate(coord(npoin,3), vol(npoin), iedge(max(nedge,1),2), sn(max(nedge,1),3))
-  allocate(kbface(max(nbface,nboun,1),2), sb(max(nbface,nboun,1),3))
-  allocate(ibpoin(max(nbface,nboun,1),2), sbb(max(nbface,nboun,1),3), anp(max(nbface,nboun,1),3))
-  do i = 1, npoin
-    read(5,*) ii, (coord(i,k), k=1,3), vol(i)
-  end do
-  do i = 1, nedge
-    read(5,*) ii, iedge(i,1), iedge(i,2)
-  end do
-  do i = 1, nedge
-    read(5,*) ii, (sn(i,k), k=1,3)
-  end do
-  do i = 1, nbface
-    read(5,*) ii, kbface(i,1), kbface(i,2)
-  end do
-  do i = 1, nbface
-    read(5,*) ii, (sb(i,k), k=1,3)
-  end do
-  do i = 1, nboun
-    read(5,*) ii, ibpoin(i,1), ibpoin(i,2)
-  end do
-  do i = 1, nboun
-    read(5,*) ii, (sbb(i,k), k=1,3)
-  end do
-  do i = 1, nboun
-    read(5,*) ii, (anp(i,k), k=1,3)
-  end do
-  close(5)
+  call read_mesh('mesh3d.d', fine)
+  npoin = fine%npoin; nedge = fine%nedge; nbface = fine%nbface; nboun = fine%nboun
 
   if (nlev > 0) then
     allocate(cols(npoin/nlev + 1))
@@ -107,11 +88,7 @@ program nftdrv
     allocate(cols(1)); cols(1) = 0
   end if
 
-  m%npoin = npoin; m%nedge = nedge; m%nbface = nbface; m%nboun = nboun
-  m%ld_poin = npoin; m%ld_edge = max(nedge,1); m%ld_face = max(nbface,nboun,1)
-  m%iedge = c_loc(iedge); m%sn = c_loc(sn); m%kbface = c_loc(kbface); m%sb = c_loc(sb)
-  m%ibpoin = c_loc(ibpoin); m%sbb = c_loc(sbb); m%anp = c_loc(anp)
-  m%coord = c_loc(coord); m%vol = c_loc(vol)
+  call c_view(fine, m)
   m%ncol = 0
   if (nlev > 0) m%ncol = npoin/nlev
   m%col_start = c_loc(cols)
@@ -127,6 +104,26 @@ program nftdrv
   p%precision = prec
 
   call nft_check(nft_init(m, p, nft_context), 'nft_init')
+  if (mg_flag == 1) then
+    ! precon's multigrid branch: coarse meshes and node maps as `input` reads them
+    ! (:2415-2672), including its level-2 metrics quirk (sn_cc y/z from sn_c, :2467-2468)
+    call read_mesh('mesh3d_c.d', lv(1))
+    call read_mesh('mesh3d_cc.d', lv(2))
+    lv(2)%sn(1:lv(2)%nedge, 2:3) = lv(1)%sn(1:lv(2)%nedge, 2:3)
+    allocate(match1(lv(1)%npoin), match2(lv(2)%npoin))
+    open(55, file='match1.d', status='old')
+    read(55,*) match1
+    close(55)
+    open(56, file='match2.d', status='old')
+    read(56,*) match2
+    close(56)
+    call c_view(lv(1), m1)
+    call c_view(lv(2), m2)
+    mg%mg_flag = 1; mg%itrp2 = itrp2; mg%itrp_c1 = itrp_c1; mg%itrp_c2 = itrp_c2; mg%itrp_cc = itrp_cc
+    mg%transfer = 0; mg%omega_c = 0.0d0
+    call nft_check(nft_set_multigrid(nft_context, m1, c_loc(match1), m2, c_loc(match2), mg), &
+                   'nft_set_multigrid')
+  end if
   call nft_check(nft_state_init(nft_context), 'nft_state_init')
   ! the reference's diagnostics after the initialisation (:270-272) and every step
   ! (:456-459), computed on the GPU
@@ -151,4 +148,54 @@ program nftdrv
   write(30) state
   close(30)
   call nft_check(nft_finalize(nft_context), 'nft_finalize')
+
+contains
+
+  ! a mesh file in the layout `input` reads (:2343-2501), Cartesian and non-periodic
+  subroutine read_mesh(fname, mt)
+    character(len=*), intent(in) :: fname
+    type(mesh_arrays), intent(inout) :: mt
+    integer :: nbp, i, k, ii, ldf
+    open(5, file=fname, status='old')
+    read(5,*) mt%npoin, mt%nedge, mt%nbface, nbp, mt%nboun
+    ldf = max(mt%nbface, mt%nboun, 1)
+    allocate(mt%coord(mt%npoin,3), mt%vol(mt%npoin), mt%iedge(max(mt%nedge,1),2), mt%sn(max(mt%nedge,1),3))
+    allocate(mt%kbface(ldf,2), mt%sb(ldf,3), mt%ibpoin(ldf,2), mt%sbb(ldf,3), mt%anp(ldf,3))
+    do i = 1, mt%npoin
+      read(5,*) ii, (mt%coord(i,k), k=1,3), mt%vol(i)
+    end do
+    do i = 1, mt%nedge
+      read(5,*) ii, mt%iedge(i,1), mt%iedge(i,2)
+    end do
+    do i = 1, mt%nedge
+      read(5,*) ii, (mt%sn(i,k), k=1,3)
+    end do
+    do i = 1, mt%nbface
+      read(5,*) ii, mt%kbface(i,1), mt%kbface(i,2)
+    end do
+    do i = 1, mt%nbface
+      read(5,*) ii, (mt%sb(i,k), k=1,3)
+    end do
+    do i = 1, mt%nboun
+      read(5,*) ii, mt%ibpoin(i,1), mt%ibpoin(i,2)
+    end do
+    do i = 1, mt%nboun
+      read(5,*) ii, (mt%sbb(i,k), k=1,3)
+    end do
+    do i = 1, mt%nboun
+      read(5,*) ii, (mt%anp(i,k), k=1,3)
+    end do
+    close(5)
+  end subroutine
+
+  subroutine c_view(mt, mc)
+    type(mesh_arrays), target, intent(in) :: mt
+    type(nft_mesh_t), intent(out) :: mc
+    mc%npoin = mt%npoin; mc%nedge = mt%nedge; mc%nbface = mt%nbface; mc%nboun = mt%nboun
+    mc%ld_poin = mt%npoin; mc%ld_edge = max(mt%nedge,1); mc%ld_face = max(mt%nbface,mt%nboun,1)
+    mc%iedge = c_loc(mt%iedge); mc%sn = c_loc(mt%sn); mc%kbface = c_loc(mt%kbface); mc%sb = c_loc(mt%sb)
+    mc%ibpoin = c_loc(mt%ibpoin); mc%sbb = c_loc(mt%sbb); mc%anp = c_loc(mt%anp)
+    mc%coord = c_loc(mt%coord); mc%vol = c_loc(mt%vol)
+    mc%ncol = 0; mc%col_start = c_null_ptr
+  end subroutine
 end program nftdrv

@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:8f8a07c1bd035d15
+﻿!mod$ v1 sum:80e054baf3325f9f
 !need$ 0bde2ac47243ead2 i iso_c_binding
 module nft_hip
 use,intrinsic::iso_c_binding,only:c_associated
@@ -163,6 +163,15 @@ real(8)::w(1_8:4_8)
 real(8)::t(1_8:4_8)
 integer(8)::step
 end type
+type,bind(c)::nft_multigrid_t
+integer(4)::mg_flag
+integer(4)::itrp2
+integer(4)::itrp_c1
+integer(4)::itrp_c2
+integer(4)::itrp_cc
+integer(4)::transfer
+real(8)::omega_c
+end type
 type(c_ptr),save::nft_context
 real(8),save::nft_dt
 interface
@@ -174,6 +183,20 @@ type(nft_mesh_t),intent(in)::mesh
 type(nft_params_t),intent(in)::params
 type(c_ptr),intent(out)::ctx
 integer(4)::nft_init
+end
+end interface
+interface
+function nft_set_multigrid(ctx,m1,match1,m2,match2,mg) bind(c,name="nft_set_multigrid")
+import::c_ptr
+import::nft_mesh_t
+import::nft_multigrid_t
+type(c_ptr),value::ctx
+type(nft_mesh_t),intent(in)::m1
+type(c_ptr),value::match1
+type(nft_mesh_t),intent(in)::m2
+type(c_ptr),value::match2
+type(nft_multigrid_t),intent(in)::mg
+integer(4)::nft_set_multigrid
 end
 end interface
 interface
@@ -383,5 +406,11 @@ integer(4),intent(in)::precision
 integer(4),intent(in)::lord
 real(8),intent(in)::strat
 real(8),intent(in)::theta0
+end
+subroutine nft_attach_multigrid(itrp2,itrp_c1,itrp_c2,itrp_cc)
+integer(4),intent(in)::itrp2
+integer(4),intent(in)::itrp_c1
+integer(4),intent(in)::itrp_c2
+integer(4),intent(in)::itrp_cc
 end
 end

@@ -210,7 +210,8 @@ def make_params(cfg, precon=None, sm_flag=1, itrp1=None, omega=None, eps0=None, 
 
 class NftMultigrid(ctypes.Structure):
     _fields_ = [("mg_flag", ctypes.c_int), ("itrp2", ctypes.c_int), ("itrp_c1", ctypes.c_int),
-                ("itrp_c2", ctypes.c_int), ("itrp_cc", ctypes.c_int), ("transfer", ctypes.c_int)]
+                ("itrp_c2", ctypes.c_int), ("itrp_cc", ctypes.c_int), ("transfer", ctypes.c_int),
+                ("omega_c", ctypes.c_double)]
 
 
 def _mesh_struct(mesh, use_columns=True):
@@ -446,10 +447,12 @@ class Solver:
         _chk(lib().nft_copy_calibration(self.h, reps, ctypes.byref(ms)), "nft_copy_calibration")
         return ms.value
 
-    def set_multigrid(self, levels=None, mg_flag=1, itrp2=0, itrp_c1=1, itrp_c2=1, itrp_cc=1, transfer=0):
+    def set_multigrid(self, levels=None, mg_flag=1, itrp2=0, itrp_c1=1, itrp_c2=1, itrp_cc=1, transfer=0,
+                      omega_c=0.0):
         """precon's multigrid branch: levels = [(mesh_c, match1), (mesh_cc, match2)] as
-        mesh.multigrid_meshes returns them (None keeps the levels of an earlier call)."""
-        mg = NftMultigrid(mg_flag, itrp2, itrp_c1, itrp_c2, itrp_cc, transfer)
+        mesh.multigrid_meshes returns them (None keeps the levels of an earlier call);
+        omega_c <= 0 relaxes the coarse levels with the fine omega, as the reference."""
+        mg = NftMultigrid(mg_flag, itrp2, itrp_c1, itrp_c2, itrp_cc, transfer, omega_c)
         if levels is None:
             _chk(lib().nft_set_multigrid(self.h, None, None, None, None, ctypes.byref(mg)), "nft_set_multigrid")
             return

@@ -68,3 +68,32 @@ def test_fortran_driver_matches_ctypes_path(tmp_path):
                     assert x == y, (k, j, x, y)
                 else:
                     assert (x == 0 if y == 0 else abs(x - y) <= 1.2e-4 * 10.0 ** (np.floor(np.log10(abs(y))) + 1)), (k, j, x, y)
+
+
+@pytest.mark.gpu
+def test_fortran_driver_multigrid(tmp_path):
+    """mg_flag=1 in input_mg_v3.d: nftdrv reads mesh3d_c.d, mesh3d_cc.d, match1.d and
+    match2.d as the reference's `input` does (level-2 metrics quirk included) and runs
+    precon's V-cycle on the GPU; its state equals the oracle's (multigrid on, GPU summation
+    order) bit for bit."""
+    if not os.path.exists(DRV):
+        pytest.skip("nftdrv not built")
+    from common import MG_SWEEPS, mg_levels_for
+    m = mesh_for("tiny")
+    cfg = tiny_cfg()
+    nsteps = 3
+    d = str(tmp_path)
+    write_reference_inputs(d, m, cfg, ntime=nsteps, sm_flag=1, itrp1=cfg.itrp1, omega=cfg.omega,
+                           mg=dict(levels=mg_levels_for("tiny"), **MG_SWEEPS))
+    r = subprocess.run([DRV, str(m.nlev), "1", "0", str(cfg.lord), repr(cfg.strat), "0"], cwd=d,
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    st = np.fromfile(os.path.join(d, "nftdrv_state.bin"), dtype=np.float64).reshape(8, m.npoin)
+    o = Oracle(m, cfg, dot_order=1)
+    o.set_multigrid(mg_levels_for("tiny", as_read=True), transfer=0, **MG_SWEEPS)
+    o.init()
+    o.step(nsteps)
+    assert np.array_equal(st[0:4], o.state("unk"))
+    assert np.array_equal(st[4:7], o.state("uuu"))
+    assert np.array_equal(st[7], o.state("prpot"))
