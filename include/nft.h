@@ -46,7 +46,7 @@ typedef struct {
 } nft_mesh;
 
 /* Run parameters: the reference's inputinc.d / input_mg_v3.d knobs and the constants
- * hard-coded in its mpsolver (Cartesian, lipps=0, implgw=1, icorio=0, itraj=0). */
+ * hard-coded in its mpsolver (Cartesian, lipps=0, implgw=1, icorio=0; itraj below). */
 typedef struct {
   double dt;                 /* delt -> timestp */
   double u0, v0, w0;         /* u0inf v0inf w0inf */
@@ -65,6 +65,9 @@ typedef struct {
   int irl;
   double dom[6];             /* box used by beta_timestep_unstr (reference: input :2255-2260) */
   int precision;             /* 0: FP64 everywhere; 1: mixed (FP32 MPDATA + preconditioner) */
+  int itraj;                 /* advector at n+1/2: 0 extrapolation (reference default, :427-438);
+                                1 velprd's first-order prediction (:4448-4531, one more GCR
+                                solve per step) */
 } nft_params;
 
 /* GCR statistics, as /itero/ of the reference (:1953-1963) plus timing. */

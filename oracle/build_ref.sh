@@ -1,9 +1,10 @@
 #!/bin/bash
 # Builds the reference Fortran program with flang into oracle/_ref/ (TEST INFRASTRUCTURE).
 #
-# Usage: oracle/build_ref.sh <lord> <strat>
+# Usage: oracle/build_ref.sh <lord> <strat> [itraj]
 #   lord  : GCR order written into `parameter (lordx=..,lord=..)` (reference default 9)
 #   strat : stratification N^2/g written over the hard-coded `strat=1.e-05`
+#   itraj : advector scheme written over the hard-coded `itraj=0` (1: velprd)
 #
 # The source is read where it lies (REF_SRC, default under /root/reference) and a patched
 # copy is written to oracle/_ref/ only (git-ignored, never committed). Patches:
@@ -11,7 +12,7 @@
 #   2. neighbour lists widened from 10 to 32 entries (extruded tet meshes have up to ~22
 #      neighbours; 10 overflows the array)
 #   3. GCR order `lord`
-#   4. stratification constant
+#   4. stratification constant (and, when given, the advector scheme itraj)
 #   5. restart-file formats f14.6 -> es26.17e3 so the final state is written at full
 #      precision
 # The build is FP64 (-fdefault-real-8 -fdefault-double-8), as the reference's REAL*4
@@ -21,9 +22,11 @@ HERE=$(cd "$(dirname "$0")" && pwd)
 REF_SRC=${REF_SRC:-/root/reference/lippsLFroude_implgwopks3D_unstr3.f}
 LORD=${1:-3}
 STRAT=${2:-1.e-05}
+ITRAJ=${3:-0}
 FC=${FC:-flang}
 OUT="$HERE/_ref"
 TAG="l${LORD}_s${STRAT}"
+if [ "$ITRAJ" != "0" ]; then TAG="${TAG}_t${ITRAJ}"; fi
 mkdir -p "$OUT"
 if [ ! -f "$REF_SRC" ]; then
   echo "reference source not found: $REF_SRC" >&2
@@ -33,6 +36,7 @@ sed -e '1s/^      program nftincv10/      subroutine nftmain/' \
     -e 's/(npoinx,10)/(npoinx,32)/g' \
     -e "s/parameter (lordx=9,nn=npoinx,lord=9)/parameter (lordx=${LORD},nn=npoinx,lord=${LORD})/" \
     -e "s/^      strat=1.e-05/      strat=${STRAT}/" \
+    -e "s/^      itraj=0\(\r\?\)\$/      itraj=${ITRAJ}\1/" \
     -e 's/format(3f14.6)/format(3es26.17e3)/' \
     -e 's/format(4f14.6)/format(4es26.17e3)/' \
     "$REF_SRC" > "$OUT/ref_${TAG}.f"

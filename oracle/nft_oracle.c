@@ -46,6 +46,8 @@ struct oc_ctx {
   int mg_on, mg_itrp2, mg_c1, mg_c2, mg_cc, mg_transfer;
   double mg_omega;   /* relaxation factor of the coarse levels (the reference: omega) */
   double *mg_w[12];
+  /* velprd work (itraj = 1): uuu1[3], ff[3], pprpot */
+  double *vp[7];
 };
 
 static double amax1(double a, double b) { return a > b ? a : b; }
@@ -99,6 +101,8 @@ oc_ctx *oc_create(const oc_mesh *m, const oc_params *p) {
   c->unk = zalloc(4 * (size_t)n); c->frc = zalloc(4 * (size_t)n); c->uuu = zalloc(3 * (size_t)n);
   c->unkdash = zalloc(3 * (size_t)n); c->prpot = zalloc(n); c->unk1 = zalloc(4 * (size_t)n);
   c->uuu0 = zalloc(3 * (size_t)n);
+  if (p->itraj == 1)
+    for (int i = 0; i < 7; ++i) c->vp[i] = zalloc(n);
 
   /* rhprof/thprof with lipps=0 (F:4549-4579, F:4532-4547); metric factors are 1 in the
    * Cartesian case (F:149-157) */
@@ -298,6 +302,7 @@ void oc_destroy(oc_ctx *c) {
   for (int i = 0; i < 16; ++i) free(c->s[i]);
   for (int i = 0; i < 4; ++i) free(c->gb[i]);
   free(c->nb_ptr); free(c->nb_idx);
+  for (int i = 0; i < 7; ++i) free(c->vp[i]);
   for (int l = 0; l < 2; ++l) { oc_destroy(c->lv[l]); free(c->lv_match[l]); free(c->lv_imatch[l]); }
   for (int i = 0; i < 12; ++i) free(c->mg_w[i]);
   free(c);
@@ -1169,6 +1174,44 @@ double *oc_state(oc_ctx *c, const char *name) {
   return NULL;
 }
 
+/* velprd(unk,frc,uuu,uuu1,unkdash), F:4448-4531 (Cartesian: the metric factors are 1):
+ * a first-order prediction of u at n+1 from u^n + dt*frc advected by uuu, projected by a
+ * GCR solve (impfl=0) and prfr; uuu becomes the mean of uuu and the prediction times rho
+ * (the advector at n+1/2), unkdash the prediction. */
+static void oc_velprd(oc_ctx *c) {
+  int n = c->n;
+  double dt = c->p.dt;
+  double *unk = c->unk, *frc = c->frc, *uuu = c->uuu, *ud = c->unkdash;
+  double *u1 = c->vp[0], *ff1 = c->vp[3], *ff2 = c->vp[4], *ff3 = c->vp[5], *pp = c->vp[6];
+  double *v1 = c->vp[1], *w1 = c->vp[2];
+  double *uo[3] = {u1, v1, w1};
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) ud[(size_t)k * n + i] = unk[(size_t)k * n + i] + dt * frc[(size_t)k * n + i];
+  for (int k = 0; k < 3; ++k) {
+    const double *f = ud + (size_t)k * n;
+    oc_derivative(c, f, ff1, 1);
+    oc_derivative(c, f, ff2, 2);
+    oc_derivative(c, f, ff3, 3);
+    for (int i = 0; i < n; ++i) {
+      double v = c->m.vol[i];
+      ff1[i] = ff1[i] / v;
+      ff2[i] = ff2[i] / v;
+      ff3[i] = ff3[i] / v;
+    }
+    for (int i = 0; i < n; ++i)
+      uo[k][i] = f[i] - dt * (uuu[i] * ff1[i] + uuu[n + i] * ff2[i] + uuu[2 * n + i] * ff3[i]);
+  }
+  for (int i = 0; i < n; ++i) { pp[i] = 0.; ff1[i] = 0.; ff2[i] = 0.; ff3[i] = 0.; }
+  oc_gcrk(c, pp, u1, v1, w1, ff1, ff2, ff3, 0);
+  oc_prfr(c, pp, u1, v1, w1, ff1, ff2, ff3, 0);
+  double *ff[3] = {ff1, ff2, ff3};
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      uuu[(size_t)k * n + i] = (uuu[(size_t)k * n + i] + ff[k][i]) * 0.5 * c->rh0[i];
+      ud[(size_t)k * n + i] = ff[k][i];
+    }
+}
+
 void oc_nft_init(oc_ctx *c) {
   int n = c->n;
   double *unk = c->unk, *frc = c->frc, *uuu = c->uuu, *ud = c->unkdash, *u1 = c->unk1;
@@ -1194,6 +1237,7 @@ void oc_nft_init(oc_ctx *c) {
     c->prpot[i] = 0.;
     frc[i] = 0.; frc[n + i] = 0.; frc[2 * n + i] = 0.; frc[3 * n + i] = 0.;
   }
+  if (c->p.itraj == 1) { oc_velprd(c); return; }
   for (int i = 0; i < n; ++i) {
     uuu[i] = uuu[i] * c->rh0[i];
     uuu[n + i] = uuu[n + i] * c->rh0[i];
@@ -1208,8 +1252,9 @@ void oc_nft_step(oc_ctx *c) {
   double *unk = c->unk, *frc = c->frc, *uuu = c->uuu, *ud = c->unkdash, *u1 = c->unk1;
   double *uuu0 = c->uuu0;
   const double *al = c->alpha, *ue = c->ue, *ve = c->ve, *th0 = c->th0, *cw = c->coefw;
+  const int traj = c->p.itraj == 1;
   for (int i = 0; i < n; ++i) {
-    uuu0[i] = uuu[i]; uuu0[n + i] = uuu[n + i]; uuu0[2 * n + i] = uuu[2 * n + i];
+    if (!traj) { uuu0[i] = uuu[i]; uuu0[n + i] = uuu[n + i]; uuu0[2 * n + i] = uuu[2 * n + i]; }
     for (int k = 0; k < 4; ++k) unk[k * n + i] = unk[k * n + i] + 0.5 * frc[k * n + i] * dt;
   }
   for (int k = 0; k < 4; ++k) oc_mpdatam(c, uuu, unk + (size_t)k * n, k + 1);
@@ -1253,6 +1298,7 @@ void oc_nft_step(oc_ctx *c) {
     frc[n + i] = frc[n + i] + frm2;
     frc[2 * n + i] = frc[2 * n + i] + frm3;
     frc[3 * n + i] = frc[3 * n + i] + frm4;
+    if (traj) continue;
     double rh = c->rh0[i];
     uuu[i] = 1.5 * uuu[i] * rh - 0.5 * uuu0[i];
     uuu[n + i] = 1.5 * uuu[n + i] * rh - 0.5 * uuu0[n + i];
@@ -1265,4 +1311,5 @@ void oc_nft_step(oc_ctx *c) {
     ud[n + i] = 2. * unk[n + i] - uuu0[n + i];
     ud[2 * n + i] = 2. * unk[2 * n + i] - uuu0[2 * n + i];
   }
+  if (traj) oc_velprd(c);
 }

@@ -2,7 +2,7 @@
  *
  * A plain-C FP64 restatement of the reference Fortran
  * (lippsLFroude_implgwopks3D_unstr3.f) for the Cartesian, non-periodic path
- * (isfera=0, iper=0, lipps=0, itraj=0, implgw=1, icorio=0). Each function keeps the
+ * (isfera=0, iper=0, lipps=0, itraj=0 or 1, implgw=1, icorio=0). Each function keeps the
  * reference's loop order and operation order so that, compiled without FP contraction,
  * it reproduces the reference's rounding. Only tests, the smoke check and the CPU leg
  * of the benchmark use it, as the checker of the HIP path.
@@ -61,6 +61,9 @@ typedef struct {
    * reduction tree (same lane/block/grid assignment), so the two can be compared bit for
    * bit and the effect of summation order measured on its own */
   int dot_order;
+  /* advector at n+1/2: 0 extrapolation 1.5 rho u^n - 0.5 u^(n-1) (F:427-438); 1 the
+   * first-order prediction velprd (F:4448-4531), which adds a GCR solve per step */
+  int itraj;
 } oc_params;
 
 typedef struct { int niter, nitsm, icount; double eer, eem; } oc_stats;

@@ -34,7 +34,8 @@ class OcParams(ctypes.Structure):
                 ("dyabu", ctypes.c_double), ("dzabd", ctypes.c_double), ("dzabu", ctypes.c_double),
                 ("towxl", ctypes.c_double), ("towxr", ctypes.c_double), ("towyd", ctypes.c_double),
                 ("towyu", ctypes.c_double), ("towzd", ctypes.c_double), ("towzu", ctypes.c_double),
-                ("irl", ctypes.c_int), ("dom", ctypes.c_double * 6), ("dot_order", ctypes.c_int)]
+                ("irl", ctypes.c_int), ("dom", ctypes.c_double * 6), ("dot_order", ctypes.c_int),
+                ("itraj", ctypes.c_int)]
 
 
 class OcStats(ctypes.Structure):
@@ -101,7 +102,7 @@ class Oracle:
     """FP64 CPU restatement bound to one mesh and run configuration."""
 
     def __init__(self, mesh, cfg, precon=None, sm_flag=1, itrp1=None, omega=None, eps0=None,
-                 exit_mode=None, itr=None, lord=None, use_columns=True, dot_order=0):
+                 exit_mode=None, itr=None, lord=None, use_columns=True, dot_order=0, itraj=0):
         L = lib()
         self.mesh = mesh
         self.n = mesh.npoin
@@ -129,6 +130,7 @@ class Oracle:
         for i, v in enumerate(REF_DOM):
             p.dom[i] = v
         p.dot_order = dot_order
+        p.itraj = itraj
         self._m, self._params = m, p
         self.h = L.oc_create(ctypes.byref(m), ctypes.byref(p))
 

@@ -136,6 +136,10 @@ struct nft_ctx {
   double mg_omega = 0;   // relaxation factor of the coarse levels
   double *mg_rrhs = nullptr, *mg_e = nullptr;
 
+  // velprd (itraj = 1): its pressure and projected prediction (host and device pointer lists)
+  double *vp_p = nullptr, *vp_ff[3] = {nullptr, nullptr, nullptr};
+  double **vp_ffd = nullptr;
+
   // learned preconditioner (network, training pairs, inference state)
   struct Net {
     int ready = 0, H = 128, L = 0, Kp = 0, ncol = 0, max_rows = 0;
@@ -405,6 +409,7 @@ int grad_launch(nft_ctx *c, int mode, const GradArgs &a) {
       case GRAD_RAW: hipLaunchKernelGGL((k_grad<GRAD_RAW, true>), g, b, 0, c->stream, c->inc, c->S, a); break;
       case GRAD_PRFR: hipLaunchKernelGGL((k_grad<GRAD_PRFR, true>), g, b, 0, c->stream, c->inc, c->S, a); break;
       case GRAD_LAPLC: hipLaunchKernelGGL((k_grad<GRAD_LAPLC, true>), g, b, 0, c->stream, c->inc, c->S, a); break;
+      case GRAD_ADV: hipLaunchKernelGGL((k_grad<GRAD_ADV, true>), g, b, 0, c->stream, c->inc, c->S, a); break;
       default: hipLaunchKernelGGL((k_grad<GRAD_ZERO_PRFR, true>), g, b, 0, c->stream, c->inc, c->S, a); break;
     }
   } else {
@@ -412,6 +417,7 @@ int grad_launch(nft_ctx *c, int mode, const GradArgs &a) {
       case GRAD_RAW: hipLaunchKernelGGL((k_grad<GRAD_RAW, false>), g, b, 0, c->stream, c->inc, c->S, a); break;
       case GRAD_PRFR: hipLaunchKernelGGL((k_grad<GRAD_PRFR, false>), g, b, 0, c->stream, c->inc, c->S, a); break;
       case GRAD_LAPLC: hipLaunchKernelGGL((k_grad<GRAD_LAPLC, false>), g, b, 0, c->stream, c->inc, c->S, a); break;
+      case GRAD_ADV: hipLaunchKernelGGL((k_grad<GRAD_ADV, false>), g, b, 0, c->stream, c->inc, c->S, a); break;
       default: hipLaunchKernelGGL((k_grad<GRAD_ZERO_PRFR, false>), g, b, 0, c->stream, c->inc, c->S, a); break;
     }
   }
@@ -1161,8 +1167,43 @@ StepArgs step_args(nft_ctx *c) {
   a.alpha = c->alpha; a.ue = c->ue; a.ve = c->ve; a.th0 = c->th0; a.coefw = c->coefw;
   a.rh0 = c->rh0; a.dthedz = c->dthedz;
   a.dt = c->P.dt; a.g = kG;
+  a.itraj = c->P.itraj;
   for (int k = 0; k < 3; ++k) a.uuuf[k] = c->P.precision == 1 ? c->uuuf[k] : nullptr;
   return a;
+}
+
+// velprd(unk,frc,uuu,uuu1,unkdash) (:4448-4531, Cartesian): predicts u at n+1 by advecting
+// unk + dt*frc with uuu (gradient and advection fused in k_grad<GRAD_ADV>), projects it
+// with a GCR solve (impfl = 0) and prfr, and sets uuu = (uuu + prediction)*0.5*rh0,
+// unkdash = prediction. uuu1 lives in uuu0 (unused with itraj = 1).
+int velprd_dev(nft_ctx *c, nft_run_stats *rs) {
+  StepArgs a = step_args(c);
+  dim3 g(c->nodes_grid), b(NFT_BLOCK);
+  int rc;
+  hipLaunchKernelGGL(k_vp_start, g, b, 0, c->stream, a);
+  for (int k = 0; k < 3; ++k) {
+    GradArgs ga = grad_args(c);
+    ga.f = c->ud[k]; ga.s0 = c->uuu[0]; ga.s1 = c->uuu[1]; ga.s2 = c->uuu[2]; ga.o0 = c->uuu0[k];
+    if ((rc = grad_launch(c, GRAD_ADV, ga))) return rc;
+  }
+  hipLaunchKernelGGL(k_fill, g, b, 0, c->stream, c->n, c->vp_p, 0.0);
+  float ms = 0;
+  HIPCHK(hipEventRecord(c->ev[0], c->stream));
+  rc = gcrk_dev(c, c->vp_p, c->uuu0[0], c->uuu0[1], c->uuu0[2], c->vp_ff[0], c->vp_ff[1], c->vp_ff[2], 0);
+  if (rc < 0) return rc;
+  if (rs) { rs->gcr_solves++; rs->gcr_iters += c->st.niter; rs->gcr_unconverged += rc; }
+  if ((rc = prfr_dev(c, c->vp_p, c->uuu0[0], c->uuu0[1], c->uuu0[2], c->vp_ff[0], c->vp_ff[1], c->vp_ff[2], 0)))
+    return rc;
+  HIPCHK(hipEventRecord(c->ev[1], c->stream));
+  if (rs) {
+    HIPCHK(hipEventSynchronize(c->ev[1]));
+    t_harvest(c);
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    rs->ms_gcr += ms;
+  }
+  hipLaunchKernelGGL(k_vp_end, g, b, 0, c->stream, a, (const double *const *)c->vp_ffd);
+  HIPCHK(hipGetLastError());
+  return 0;
 }
 
 int set_out_fields(nft_ctx *c, double *const *fields, int nf) {
@@ -1263,6 +1304,7 @@ int one_step(nft_ctx *c, nft_run_stats *rs) {
   hipLaunchKernelGGL(k_step_end, g, b, 0, c->stream, a);
   HIPCHK(hipGetLastError());
   if (rs) { HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3])); rs->ms_mpdata += ms; }
+  if (c->P.itraj == 1) return velprd_dev(c, rs);
   return 0;
 }
 
@@ -1502,6 +1544,7 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   if (P->precon == 2 && m->ncol <= 0) return set_err(-1, "nft_init: line preconditioner needs columns");
   if (P->precon < 0 || P->precon > 2) return set_err(-1, "nft_init: unknown preconditioner");
   if (!(P->dt > 0)) return set_err(-1, "nft_init: dt must be positive");
+  if (P->itraj != 0 && P->itraj != 1) return set_err(-1, "nft_init: itraj must be 0 or 1");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(-3, "nft_init: no HIP device");
   const int n = m->npoin;
@@ -1610,6 +1653,13 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   }
   for (int k = 0; k < 7; ++k)
     if ((rc = dalloc(c, &c->io[k], n))) return fail(rc);
+  if (P->itraj == 1) {
+    if ((rc = dalloc(c, &c->vp_p, n)) || (rc = dalloc(c, &c->vp_ff[0], n)) || (rc = dalloc(c, &c->vp_ff[1], n)) ||
+        (rc = dalloc(c, &c->vp_ff[2], n)) || (rc = dalloc(c, &c->vp_ffd, 3)))
+      return fail(rc);
+    if (hipMemcpy(c->vp_ffd, c->vp_ff, 3 * sizeof(double *), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(set_err(-2, "memcpy"));
+  }
   if ((rc = dalloc(c, &c->io[7], 8))) return fail(rc);  // device array of field pointers
   size_t npart = (size_t)std::max(c->gather_grid, c->vec_grid) * (NFT_MAXD + 1);
   if ((rc = dalloc(c, &c->partials, npart)) || (rc = dalloc(c, &c->counter, 4)) || (rc = dalloc(c, &c->scal, SC_TOTAL)))
@@ -1805,6 +1855,7 @@ int nft_state_init(nft_ctx *c) {
   StepArgs a = step_args(c);
   hipLaunchKernelGGL(k_init_post, g, b, 0, c->stream, a);
   HIPCHK(hipGetLastError());
+  if (c->P.itraj == 1 && (rc = velprd_dev(c, nullptr)) < 0) return rc;
   return sync(c);
 }
 

@@ -253,7 +253,8 @@ enum GradMode {
   GRAD_RAW = 0,     // out0..2 = vol-integrated gradient (derivative idx 1..3)
   GRAD_PRFR = 1,    // out = str - 0.5*dt*g/vol (w: *fz), then bcv(...,1.)       (prfr)
   GRAD_LAPLC = 2,   // out = rh0 * bcv0(0.5*(dt*g/vol), w: *fz) -> (oxy, o2)        (laplc 1st half)
-  GRAD_ZERO_PRFR = 3  // pressure identically zero: out = bcv(str, 1.)
+  GRAD_ZERO_PRFR = 3,  // pressure identically zero: out = bcv(str, 1.)
+  GRAD_ADV = 4      // out0 = f - dt*((s0*gx + s1*gy) + s2*gz), g = gradient/vol      (velprd)
 };
 
 struct GradArgs {
@@ -320,6 +321,11 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S
   if (!L.valid) return;
   if (MODE == GRAD_RAW) {
     a.o0[i] = gx; a.o1[i] = gy; a.o2[i] = gz;
+  } else if (MODE == GRAD_ADV) {
+    // velprd (lippsLFroude_implgwopks3D_unstr3.f:4469-4481): advection of f by s
+    double vl = a.nc.vol[i];
+    gx = gx / vl; gy = gy / vl; gz = gz / vl;
+    a.o0[i] = a.f[i] - a.dt * (a.s0[i] * gx + a.s1[i] * gy + a.s2[i] * gz);
   } else if (MODE == GRAD_PRFR || MODE == GRAD_ZERO_PRFR) {
     double u, v, w;
     if (MODE == GRAD_PRFR) {
@@ -1393,6 +1399,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, EdgeS<T> S,
 // ------------------------------------------------------------------------------------
 struct StepArgs {
   int n;
+  int itraj;       // 1: the advector comes from velprd, so no extrapolation and no uuu0
   double *unk[4], *frc[4], *uuu[3], *uuu0[3], *ud[3];
   const double *alpha, *ue, *ve, *th0, *coefw, *rh0, *dthedz;
   double dt, g;
@@ -1403,7 +1410,8 @@ struct StepArgs {
 __global__ void k_step_pre(StepArgs a) {
   int i = blockIdx.x * NFT_BLOCK + threadIdx.x;
   if (i >= a.n) return;
-  for (int k = 0; k < 3; ++k) a.uuu0[k][i] = a.uuu[k][i];
+  if (!a.itraj)
+    for (int k = 0; k < 3; ++k) a.uuu0[k][i] = a.uuu[k][i];
   for (int k = 0; k < 4; ++k) a.unk[k][i] = a.unk[k][i] + 0.5 * a.frc[k][i] * a.dt;
 }
 
@@ -1450,6 +1458,7 @@ __global__ void k_step_end(StepArgs a) {
   a.frc[1][i] = a.frc[1][i] + (0. - al * (u1 - a.ve[i]));
   a.frc[2][i] = a.frc[2][i] + (0. - al * u2);
   a.frc[3][i] = a.frc[3][i] + (-(al * u3));
+  if (a.itraj) return;
   double rh = a.rh0[i];
   double rhoi = 1. / rh;
   for (int k = 0; k < 3; ++k) {
@@ -1470,12 +1479,37 @@ __global__ void k_init_post(StepArgs a) {
   double rh = a.rh0[i];
   for (int k = 0; k < 3; ++k) {
     double u = a.unk[k][i];
+    if (a.itraj) {   // velprd follows with uuu = u (:229-231, :266)
+      a.uuu[k][i] = u;
+      continue;
+    }
     double m = u * rh;
     a.uuu[k][i] = m;
     if (a.uuuf[0]) a.uuuf[k][i] = (float)m;
     a.ud[k][i] = u;
   }
   for (int k = 0; k < 4; ++k) a.frc[k][i] = 0.;
+}
+
+// velprd (:4464-4467): unkdash = unk + dt*frc, the start of the prediction
+__global__ void k_vp_start(StepArgs a) {
+  int i = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (i >= a.n) return;
+  for (int k = 0; k < 3; ++k) a.ud[k][i] = a.unk[k][i] + a.dt * a.frc[k][i];
+}
+
+// velprd (:4520-4529): advector at n+1/2 and the prediction; p = 0 for the next call
+__global__ void k_vp_end(StepArgs a, const double *const *ff) {
+  int i = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (i >= a.n) return;
+  double rh = a.rh0[i];
+  for (int k = 0; k < 3; ++k) {
+    double f = ff[k][i];
+    double m = (a.uuu[k][i] + f) * 0.5 * rh;
+    a.uuu[k][i] = m;
+    if (a.uuuf[0]) a.uuuf[k][i] = (float)m;
+    a.ud[k][i] = f;
+  }
 }
 
 template <typename T>

@@ -36,6 +36,7 @@ module nft_hip
     integer(c_int) :: irl
     real(c_double) :: dom(6)
     integer(c_int) :: precision
+    integer(c_int) :: itraj
   end type
 
   type, bind(C) :: nft_gcr_stats_t
@@ -289,6 +290,7 @@ contains
     p%irl = irl
     p%dom = [0.0d0, 15000.0d0, -6000.0d0, 6000.0d0, 0.0d0, 6000.0d0]
     p%precision = precision
+    p%itraj = 0   ! mpsolver hard-codes itraj=0 (:111); nft_init accepts 1 for velprd
     nft_dt = delt
     call nft_check(nft_init(m, p, nft_context), 'nft_init')
     if (mg_flag == 1) call nft_attach_multigrid(itrp2, itrp_c1, itrp_c2, itrp_cc)

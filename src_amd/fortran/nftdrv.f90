@@ -9,16 +9,17 @@
 ! meshes mesh3d_c.d / mesh3d_cc.d and the node maps match1.d / match2.d and runs precon's
 ! multigrid branch.
 !
-! Usage: nftdrv <nlev> <precon> <precision> <lord> <strat> [exit_mode]
+! Usage: nftdrv <nlev> <precon> <precision> <lord> <strat> [exit_mode] [itraj]
 !   nlev      levels per column when nodes are numbered column by column (0: no columns)
 !   precon    1 point relaxation (reference), 2 vertical line
 !   precision 0 FP64, 1 mixed
 !   lord      GCR order; strat: N^2/g (the reference hard-codes these in mpsolver)
+!   itraj     0 extrapolated advector (reference), 1 velprd
 program nftdrv
   use iso_c_binding
   use nft_hip
   implicit none
-  integer :: npoin, nedge, nbface, nbpoin, nboun, i, k, ii, nlev, pk, prec, lord, em
+  integer :: npoin, nedge, nbface, nbpoin, nboun, i, k, ii, nlev, pk, prec, lord, em, itj
   integer :: ntime, nrkstage, istst, mrestart, isfera, iper, ielast, itr, lordin, itrp, irl
   integer :: mg_flag, itrp1, itrp2, itrp_c1, itrp_c2, itrp_cc, match_flag
   real(8) :: cfl, delt, adapt, relax, p0inf, u0inf, v0inf, w0inf, re, eps0, betap, strat
@@ -51,6 +52,10 @@ program nftdrv
   em = 0
   if (command_argument_count() >= 6) then
     call get_command_argument(6, arg); read(arg,*) em
+  end if
+  itj = 0
+  if (command_argument_count() >= 7) then
+    call get_command_argument(7, arg); read(arg,*) itj
   end if
 
   open(8, file='input_mg_v3.d', status='old')
@@ -102,6 +107,7 @@ program nftdrv
   p%irl = irl
   p%dom = [0.0d0, 15000.0d0, -6000.0d0, 6000.0d0, 0.0d0, 6000.0d0]
   p%precision = prec
+  p%itraj = itj
 
   call nft_check(nft_init(m, p, nft_context), 'nft_init')
   if (mg_flag == 1) then

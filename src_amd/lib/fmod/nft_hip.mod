@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:80e054baf3325f9f
+﻿!mod$ v1 sum:4f1e5ecd77cdb00c
 !need$ 0bde2ac47243ead2 i iso_c_binding
 module nft_hip
 use,intrinsic::iso_c_binding,only:c_associated
@@ -129,6 +129,7 @@ real(8)::towzu
 integer(4)::irl
 real(8)::dom(1_8:6_8)
 integer(4)::precision
+integer(4)::itraj
 end type
 type,bind(c)::nft_gcr_stats_t
 integer(4)::niter

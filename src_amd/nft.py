@@ -39,7 +39,8 @@ class NftParams(ctypes.Structure):
                 ("dyabu", ctypes.c_double), ("dzabd", ctypes.c_double), ("dzabu", ctypes.c_double),
                 ("towxl", ctypes.c_double), ("towxr", ctypes.c_double), ("towyd", ctypes.c_double),
                 ("towyu", ctypes.c_double), ("towzd", ctypes.c_double), ("towzu", ctypes.c_double),
-                ("irl", ctypes.c_int), ("dom", ctypes.c_double * 6), ("precision", ctypes.c_int)]
+                ("irl", ctypes.c_int), ("dom", ctypes.c_double * 6), ("precision", ctypes.c_int),
+                ("itraj", ctypes.c_int)]
 
 
 class NftDiag(ctypes.Structure):
@@ -187,7 +188,8 @@ PRECISION = {"fp64": 0, "mixed": 1}
 
 
 def make_params(cfg, precon=None, sm_flag=1, itrp1=None, omega=None, eps0=None, exit_mode=None,
-                itr=None, lord=None, precision="fp64", dom=(0.0, 15000.0, -6000.0, 6000.0, 0.0, 6000.0)):
+                itr=None, lord=None, precision="fp64", dom=(0.0, 15000.0, -6000.0, 6000.0, 0.0, 6000.0),
+                itraj=0):
     p = NftParams()
     p.dt = cfg.dt; p.u0 = cfg.u0; p.v0 = 0.0; p.w0 = 0.0
     p.strat = cfg.strat; p.theta0 = cfg.theta0
@@ -205,6 +207,7 @@ def make_params(cfg, precon=None, sm_flag=1, itrp1=None, omega=None, eps0=None, 
     for i, v in enumerate(dom):
         p.dom[i] = v
     p.precision = PRECISION[precision] if isinstance(precision, str) else int(precision)
+    p.itraj = itraj
     return p
 
 

@@ -11,6 +11,7 @@ generated meshes, runs
     step) and for 1, 10 and 100 steps on the 40,000-node mesh (small_run.npz),
   * the multigrid branch of precon (mg_flag=1) on the tiny mesh's generated nested coarse
     meshes: its pieces in `mgops` mode and 1 and 10 steps of the program (tiny_mg.npz),
+  * 1 and 10 steps with the velprd advector (itraj=1) on the tiny mesh (tiny_velprd.npz),
 and stores inputs and outputs. Only numbers are stored.
 """
 from __future__ import annotations
@@ -56,9 +57,9 @@ def strat_literal(cfg):
     return repr(cfg.strat)
 
 
-def build_ref(cfg):
+def build_ref(cfg, itraj=0):
     exe = subprocess.run([os.path.join(ROOT, "oracle", "build_ref.sh"), str(cfg.lord),
-                          strat_literal(cfg)], check=True, capture_output=True, text=True)
+                          strat_literal(cfg), str(itraj)], check=True, capture_output=True, text=True)
     return exe.stdout.strip().splitlines()[-1]
 
 
@@ -225,6 +226,14 @@ def main():
                             match1=levels[0][1], match2=levels[1][1],
                             sweeps=np.array([MG_SWEEPS[k] for k in ("itrp2", "itrp_c1", "itrp_c2", "itrp_cc")]),
                             **mg)
+        # the velprd advector (itraj=1), from a reference build with that switch
+        exe1 = build_ref(tcfg, itraj=1)
+        vp = {}
+        for nt in (1, 10):
+            st = run_steps(exe1, m, tcfg, d, nt)
+            for k in ("unk", "uuu", "frc", "unkdash", "prpot", "niter"):
+                vp[f"s{nt}_{k}"] = st[k]
+        np.savez_compressed(os.path.join(OUT, "tiny_velprd.npz"), mesh_checksum=mesh_checksum(m), **vp)
     print("tiny fixtures written")
     if only_tiny:
         return

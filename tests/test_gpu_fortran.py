@@ -97,3 +97,26 @@ def test_fortran_driver_multigrid(tmp_path):
     assert np.array_equal(st[0:4], o.state("unk"))
     assert np.array_equal(st[4:7], o.state("uuu"))
     assert np.array_equal(st[7], o.state("prpot"))
+
+
+@pytest.mark.gpu
+def test_fortran_driver_velprd(tmp_path):
+    """itraj=1 through nftdrv: equals the oracle with velprd bit for bit."""
+    if not os.path.exists(DRV):
+        pytest.skip("nftdrv not built")
+    m = mesh_for("tiny")
+    cfg = tiny_cfg()
+    nsteps = 3
+    d = str(tmp_path)
+    write_reference_inputs(d, m, cfg, ntime=nsteps, sm_flag=1, itrp1=cfg.itrp1, omega=cfg.omega)
+    r = subprocess.run([DRV, str(m.nlev), "1", "0", str(cfg.lord), repr(cfg.strat), "0", "1"], cwd=d,
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    st = np.fromfile(os.path.join(d, "nftdrv_state.bin"), dtype=np.float64).reshape(8, m.npoin)
+    o = Oracle(m, cfg, dot_order=1, itraj=1)
+    o.init()
+    o.step(nsteps)
+    assert np.array_equal(st[0:4], o.state("unk"))
+    assert np.array_equal(st[4:7], o.state("uuu"))
+    assert np.array_equal(st[7], o.state("prpot"))

@@ -366,3 +366,46 @@ def test_nan_seeded_state_is_reported_not_hung():
     assert st.gcr_solves == 2 and st.gcr_unconverged == 2
     assert not np.isfinite(s.state("unk")).all()
     s.close()
+
+
+@pytest.mark.parametrize("key,kw,nsteps", [("tiny", dict(), 10), ("small", LINE, 3)])
+def test_velprd_steps_bitwise_device_order(key, kw, nsteps):
+    """itraj=1: the advector from velprd (gradient + advection fused in one gather, a GCR
+    solve with impfl=0, prfr) instead of the extrapolation; the GPU equals the oracle
+    (whose restatement equals the reference built with itraj=1) bit for bit."""
+    print(f"case {key} {kw} steps={nsteps}", flush=True)
+    m = mesh_for(key)
+    cfg = (tiny_cfg if key == "tiny" else small_cfg)(**kw)
+    s = solver(m, cfg, itraj=1)
+    s.state_init()
+    o = Oracle(m, cfg, dot_order=1, itraj=1)
+    o.init()
+    its, ito = [s.gcr_stats().niter], [o.stats().niter]
+    st_all = []
+    for _ in range(nsteps):
+        st_all.append(s.run_steps(1))
+        o.step()
+        its.append(s.gcr_stats().niter)
+        ito.append(o.stats().niter)
+    print("iterations", its)
+    assert its == ito
+    assert all(st.gcr_solves == 3 for st in st_all)
+    for name in ("unk", "frc", "uuu", "unkdash", "prpot"):
+        assert np.array_equal(s.state(name), o.state(name)), name
+    s.close()
+
+
+def test_velprd_mixed_precision_close_to_fp64():
+    m = mesh_for("small")
+    cfg = small_cfg(**LINE)
+    out = []
+    for prec in ("fp64", "mixed"):
+        s = solver(m, cfg, itraj=1, precision=prec)
+        s.state_init()
+        s.run_steps(3)
+        out.append(s.state("unk"))
+        s.close()
+    for k in range(4):
+        r = rel_l2(out[1][k], out[0][k])
+        print(f"unk[{k}] mixed vs fp64 {r:.2e}")
+        assert r <= 1e-5

@@ -86,6 +86,29 @@ def test_time_steps_bitwise():
         assert its == g[f"s{nt}_niter"].tolist()
 
 
+def test_velprd_steps_bitwise():
+    """itraj=1: the advector from velprd (F:4448-4531, a third GCR solve per step) instead
+    of the extrapolation; against the reference built with that switch."""
+    g = load("tiny_velprd.npz")
+    m = mesh_for("tiny")
+    assert np.array_equal(g["mesh_checksum"], mesh_checksum(m))
+    for nt in (1, 10):
+        o = Oracle(m, tiny_cfg(), itraj=1)
+        o.init()
+        its = [o.stats().niter]
+        for _ in range(nt):
+            o.step()
+            its.append(o.stats().niter)
+        for k in ("unk", "frc", "uuu", "unkdash", "prpot"):
+            assert np.array_equal(o.state(k), g[f"s{nt}_{k}"]), (nt, k)
+        assert its == g[f"s{nt}_niter"].tolist()
+    # the extrapolated advector gives a different state: the fixture pins the switch
+    o = Oracle(m, tiny_cfg(), itraj=0)
+    o.init()
+    o.step()
+    assert not np.array_equal(o.state("uuu"), g["s1_uuu"])
+
+
 def _printed(x, ref):
     """x agrees with the reference's e11.4 print `ref` to within 0.6 of its last digit."""
     if ref == 0.0:
