@@ -151,6 +151,27 @@ def multigrid_run(mesh, cfg, steps, kind):
                 settings=dict(precon="point Jacobi" if kind == "reference" else "vertical line", **mg))
 
 
+def stream_copy_gbps(nbytes=2 << 30, reps=10):
+    """Device copy of a 2 GiB array (read + write) with torch, timed with events: the
+    practical HBM streaming rate of this GPU, a reference point beside the published peak."""
+    import torch
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (profiles/production_fp64_traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
@@ -295,6 +316,11 @@ def main():
                                  if kname and args.mesh == "production" and measured_traffic(kname) else None),
                      "launches": nl, "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": int(kbytes)},
     }
+    if not args.no_roofline:
+        cp = stream_copy_gbps()
+        line["roofline"]["stream_copy_GBps"] = round(cp, 1)
+        line["roofline"]["frac_of_stream_copy"] = round(achieved / cp, 4) if cp > 0 else None
+        log(f"2 GiB device copy: {cp:.0f} GB/s; {kname} algorithmic rate = {achieved / cp:.2f} of it")
     if not args.no_variants:
         rm = timed_run(mesh, cfg, "mixed", max(K // 2, 2), 1, profile=False)
         sm = rm["st"]
