@@ -26,7 +26,10 @@ def mesh_for(spec_key):
     spec = {"tiny": M.tiny_spec(), "small": M.small_spec(), "tiny2": M.tiny_spec(nx=9, ny=7, nlev=6, seed=3),
             # columns longer than one wavefront: the line solve's coarsened scans (2 and 3 rows per lane)
             "tall": M.tiny_spec(nx=6, ny=5, nlev=100, seed=5),
-            "tall3": M.tiny_spec(nx=5, ny=4, nlev=130, seed=6)}[spec_key]
+            "tall3": M.tiny_spec(nx=5, ny=4, nlev=130, seed=6),
+            # columns of exactly one and two wavefronts
+            "wave64": M.tiny_spec(nx=9, ny=7, nlev=64, seed=8),
+            "wave128": M.tiny_spec(nx=5, ny=4, nlev=128, seed=9)}[spec_key]
     return M.generate(spec)
 
 

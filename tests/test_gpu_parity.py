@@ -409,3 +409,33 @@ def test_velprd_mixed_precision_close_to_fp64():
         r = rel_l2(out[1][k], out[0][k])
         print(f"unk[{k}] mixed vs fp64 {r:.2e}")
         assert r <= 1e-5
+
+
+@pytest.mark.parametrize("key", ["wave64", "wave128"])
+def test_wave_columns_bitwise(key):
+    """Columns of 64 and 128 levels (each wavefront inside one column): laplc, a GCR solve
+    and three steps equal the oracle (device summation order) bit for bit, with both
+    incidence forms."""
+    print(f"case {key}", flush=True)
+    m = mesh_for(key)
+    cfg = tiny_cfg(**LINE)
+    f = seeded_fields(m)
+    o = Oracle(m, cfg, dot_order=1)
+    lap = o.laplc(f["p"], 1)
+    po, uo = o.gcrk(np.zeros(m.npoin), f["u"], f["v"], 0.1 * f["w"], 1)
+    nito = o.stats().niter
+    o2 = Oracle(m, cfg, dot_order=1)
+    o2.init()
+    o2.step(3)
+    for columnar in (1, 0):
+        s = solver(m, cfg)
+        assert s.set_incidence(columnar)
+        assert np.array_equal(s.laplc(f["p"], 1), lap)
+        assert np.array_equal(s.derivative(f["p"], 2), o.derivative(f["p"], 2))
+        pg, ug, st, rc = s.gcrk(np.zeros(m.npoin), f["u"], f["v"], 0.1 * f["w"], 1)
+        assert st.niter == nito and np.array_equal(pg, po)
+        s.state_init()
+        s.run_steps(3)
+        for name in ("unk", "uuu", "prpot"):
+            assert np.array_equal(s.state(name), o2.state(name)), (columnar, name)
+        s.close()
