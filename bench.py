@@ -111,13 +111,21 @@ def learned_run(mesh, cfg, steps, train_steps, log_prefix="[learned]"):
     t_train = time.perf_counter() - t0
     log(f"{log_prefix} {npairs} pairs, {train_steps} Adam steps in {t_train:.1f} s, loss {l0:.4f} -> {l1:.4f}")
     s.set_learned(1, stall=0.99)
+    s.kernel_timer(11)   # the network's forward pass: six bf16 MFMA GEMMs over all columns
     st = s.run_steps(steps)
+    nl, ms, flops = s.kernel_timer_read()
+    s.kernel_timer(0)
     fb, li = s.set_learned(-1)
     s.close()
+    avg = ms / max(nl, 1)
+    tf = flops / (avg * 1e-3) / 1e12 if nl and avg > 0 else 0.0
     return dict(value=round(1000.0 / (st.ms_total / steps), 4), unit="timesteps/s",
                 iters_per_solve=round(st.gcr_iters / max(st.gcr_solves, 1), 2),
                 solve_ms=round(st.ms_gcr / max(st.gcr_solves, 1), 3), steps=steps,
                 unconverged=st.gcr_unconverged, fallbacks=fb, learned_iterations=li,
+                mfma={"bound": "mfma", "achieved": round(tf, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                      "frac": round(tf / 2500.0, 5), "launches": nl, "avg_us": round(avg * 1e3, 2),
+                      "flops_per_launch": int(flops), "dtype": "bf16 operands, fp32 accumulation"},
                 training={"pairs": npairs, "adam_steps": train_steps, "batch": 2, "seconds": round(t_train, 2),
                           "loss_first": round(l0, 5), "loss_last": round(l1, 5),
                           "objective": "|L z - r|^2/|r|^2, z = M_line^-1 r + s N(r/s)"})
@@ -151,24 +159,10 @@ def multigrid_run(mesh, cfg, steps, kind):
                 settings=dict(precon="point Jacobi" if kind == "reference" else "vertical line", **mg))
 
 
-def stream_copy_gbps(nbytes=2 << 30, reps=10):
-    """Device copy of a 2 GiB array (read + write) with torch, timed with events: the
+def stream_copy_gbps(solver, nbytes=2 << 30, reps=10):
+    """Copy of a 2 GiB buffer (read + write) on the device, timed with HIP events: the
     practical HBM streaming rate of this GPU, a reference point beside the published peak."""
-    import torch
-    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
-    b = torch.empty_like(a)
-    for _ in range(2):
-        b.copy_(a)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(reps):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    del a, b
-    torch.cuda.empty_cache()
+    ms = solver.stream_calibration(nbytes, reps)
     return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
@@ -283,6 +277,7 @@ def main():
     unk = r["solver"].state("unk")
     import numpy as np
     finite = bool(np.isfinite(unk).all())
+    cp = stream_copy_gbps(r["solver"]) if not args.no_roofline else 0.0
     r["solver"].close()
     log(f"fp64: {value:.2f} steps/s, {ms_step:.1f} ms/step, {iters:.1f} GCR iterations/solve, "
         f"{st.ms_gcr / max(st.gcr_solves, 1):.2f} ms/solve, unconverged {st.gcr_unconverged}")
@@ -317,7 +312,6 @@ def main():
                      "launches": nl, "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": int(kbytes)},
     }
     if not args.no_roofline:
-        cp = stream_copy_gbps()
         line["roofline"]["stream_copy_GBps"] = round(cp, 1)
         line["roofline"]["frac_of_stream_copy"] = round(achieved / cp, 4) if cp > 0 else None
         log(f"2 GiB device copy: {cp:.0f} GB/s; {kname} algorithmic rate = {achieved / cp:.2f} of it")

@@ -212,7 +212,8 @@ int nft_mesh_release(nft_mesh *mesh);
 /* ---- measurement (no reference counterpart: the reference only times the whole run) -- */
 /* Kernel ids for the timer: 1 laplc gradient half, 2 laplc divergence half, 3 line solve,
  * 4 GCR update, 5 GCR direction, 6..9 MPDATA donor/gradient/FCT/limit, 10 the coarse-level
- * part of the multigrid V-cycle (a chain of kernels; bytes_per_launch 0). While a kernel id
+ * part of the multigrid V-cycle (a chain of kernels; bytes_per_launch 0), 11 the learned
+ * correction's network forward pass (bytes_per_launch holds its MFMA FLOPs). While a kernel id
  * is selected, every launch of that kernel is bracketed by HIP events on the context's
  * stream. bytes_per_launch is the kernel's algorithmic traffic: every array it reads or
  * writes counted once at its stored width (incidence slots actually used, edge metrics,
@@ -222,6 +223,10 @@ int nft_kernel_timer_read(nft_ctx *ctx, int *launches, double *total_ms, double 
 /* plain device copy of one node array (8*npoin bytes read + written) `reps` times, mean ms
  * per copy: the reference point for hardware byte counters */
 int nft_copy_calibration(nft_ctx *ctx, int reps, double *ms);
+/* copy of `bytes` between two device buffers allocated for the call (2 GiB is far larger
+ * than the last-level cache), mean ms per copy: the practical HBM streaming rate beside the
+ * published peak */
+int nft_stream_calibration(nft_ctx *ctx, int64_t bytes, int reps, double *ms);
 
 /* ---- learned preconditioner (no reference counterpart) ------------------------------
  * A column MLP with two horizontal graph convolutions (bf16 MFMA, FP32 accumulation)

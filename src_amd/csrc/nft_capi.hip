@@ -377,7 +377,7 @@ void setup_fields(nft_ctx *c, const nft_mesh *m, std::vector<double> &rh0, std::
 }
 
 // ---- kernel timer ------------------------------------------------------------------------
-enum { KT_LGRAD = 1, KT_LDIV = 2, KT_LINE = 3, KT_UPD = 4, KT_DIR = 5, KT_MP0 = 6, KT_MG = 10 };
+enum { KT_LGRAD = 1, KT_LDIV = 2, KT_LINE = 3, KT_UPD = 4, KT_DIR = 5, KT_MP0 = 6, KT_MG = 10, KT_NET = 11 };
 
 inline bool timing(nft_ctx *c, int k) { return c->timer_kernel == k && c->tev_used + 2 <= (int)c->tev.size(); }
 inline void t_begin(nft_ctx *c, int k) {
@@ -708,8 +708,14 @@ constexpr int NET_SPLIT_ROWS = 1024;
 
 int net_gemm(nft_ctx *c, int M, int N, int K, const float *A, int lda, const uint16_t *Bt, const float *bias,
              int relu, const float *mask, int ldm, float *C, int ldc) {
-  dim3 grid((M + 31) / 32, (N + 31) / 32);
-  hipLaunchKernelGGL(k_net_gemm, grid, dim3(64), 0, c->stream, M, N, K, A, lda, Bt, bias, relu, mask, ldm, C, ldc);
+  if (K % NET_TK == 0) {
+    dim3 grid((M + 63) / 64, (N + 63) / 64);
+    hipLaunchKernelGGL(k_net_gemm_t, grid, dim3(256), 0, c->stream, M, N, K, A, lda, Bt, bias, relu, mask, ldm, C,
+                       ldc);
+  } else {
+    dim3 grid((M + 31) / 32, (N + 31) / 32);
+    hipLaunchKernelGGL(k_net_gemm, grid, dim3(64), 0, c->stream, M, N, K, A, lda, Bt, bias, relu, mask, ldm, C, ldc);
+  }
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -857,7 +863,9 @@ int net_apply_precon(nft_ctx *c, const double *r, double *qr, int sidx) {
   const int M = n.ncol;
   hipLaunchKernelGGL(k_net_input, dim3((M * n.Kp + 255) / 256), dim3(256), 0, c->stream, n.ncol, n.L, n.Kp, r,
                      c->scal, sidx, n.X);
+  t_begin(c, KT_NET);
   int rc = net_forward(c, M);
+  t_end(c, KT_NET);
   if (rc) return rc;
   hipLaunchKernelGGL(k_net_output, dim3((M * n.L + 255) / 256), dim3(256), 0, c->stream, n.ncol, n.L, n.Kp, n.Y,
                      c->scal, sidx, qr);
@@ -1933,7 +1941,7 @@ int nft_get_gcr_stats(nft_ctx *c, nft_gcr_stats *s) {
 }
 
 int nft_kernel_timer(nft_ctx *c, int kernel) {
-  if (!c || kernel < 0 || kernel > 10) return set_err(-1, "nft_kernel_timer: bad argument");
+  if (!c || kernel < 0 || kernel > 11) return set_err(-1, "nft_kernel_timer: bad argument");
   HIPCHK(hipStreamSynchronize(c->stream));
   t_harvest(c);
   if (c->tev.empty()) {
@@ -1968,6 +1976,12 @@ static double kernel_bytes(nft_ctx *c, int k) {
     case KT_MP0 + 1: return (mixed ? geomf : geom) + 4 * 8 * N + 12 * ts * N;
     case KT_MP0 + 2: return (mixed ? geomf : geom) + 3 * ts * N + 4 * 8 * N + 12 * ts * N + 2 * ts * N + 8 * ts * N;
     case KT_MP0 + 3: return (mixed ? geomf : geom) + 3 * ts * N + 4 * 8 * N + 12 * ts * N + 2 * ts * N + 8 * ts * N + 4 * 8 * N;
+    case KT_NET: {  // FLOPs of one forward pass over every column (the six MFMA GEMMs)
+      const auto &nt = c->net;
+      double fl = 0;
+      for (int l = 0; l < 6; ++l) fl += 2.0 * nt.ncol * nt.K[l] * nt.N[l];
+      return fl;
+    }
   }
   return 0;
 }
@@ -2276,6 +2290,39 @@ int nft_copy_calibration(nft_ctx *c, int reps, double *ms) {
   (void)hipEventDestroy(e1);
   if (ms) *ms = t / reps;
   return 0;
+}
+
+// streaming reference: a copy of `bytes` between two fresh device buffers, 16 bytes per
+// lane per load, grid-stride; mean ms per copy over `reps` after one untimed copy
+int nft_stream_calibration(nft_ctx *c, int64_t bytes, int reps, double *ms) {
+  if (!c || reps < 1 || bytes < 16) return set_err(-1, "nft_stream_calibration: bad argument");
+  const size_t n2 = (size_t)bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n2 * 16) != hipSuccess) return set_err(-2, "nft_stream_calibration: alloc");
+  if (hipMalloc(&b, n2 * 16) != hipSuccess) { (void)hipFree(a); return set_err(-2, "nft_stream_calibration: alloc"); }
+  int rc = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float t = 0;
+  const int grid = 8 * 256 * 4;   // a few wavefronts per SIMD on every CU, grid-stride
+  if (hipMemsetAsync(a, 0, n2 * 16, c->stream) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    rc = set_err(-2, "nft_stream_calibration: setup");
+  } else {
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(NFT_BLOCK), 0, c->stream, (const double2 *)a, (double2 *)b, n2);
+    (void)hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(NFT_BLOCK), 0, c->stream, (const double2 *)a, (double2 *)b, n2);
+    (void)hipEventRecord(e1, c->stream);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+      rc = set_err(-2, "nft_stream_calibration: timing");
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (!rc && ms) *ms = t / reps;
+  return rc;
 }
 
 // training objective (0 residual, 1 angle) and activation (linear = 1: identity activations

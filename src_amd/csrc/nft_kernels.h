@@ -1081,6 +1081,11 @@ __global__ void k_sub(int n, const double *a, const double *b, double *c, const 
   if (k < n) c[k] = a[k] - b[k];
 }
 
+// plain streaming copy (bandwidth reference)
+__global__ void __launch_bounds__(NFT_BLOCK) k_stream_copy(const double2 *a, double2 *b, size_t n) {
+  for (size_t k = (size_t)blockIdx.x * NFT_BLOCK + threadIdx.x; k < n; k += (size_t)gridDim.x * NFT_BLOCK) b[k] = a[k];
+}
+
 __global__ void k_copy(int n, const double *a, double *b, const double *skip) {
   if (skipped(skip)) return;
   int k = blockIdx.x * NFT_BLOCK + threadIdx.x;

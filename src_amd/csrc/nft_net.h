@@ -67,6 +67,76 @@ __global__ void __launch_bounds__(64) k_net_gemm(int M, int N, int K, const floa
   }
 }
 
+// The same product with a 64x64 C tile per 256-thread block (four wavefronts, one 32x32
+// MFMA tile each): A (converted to bf16) and Bt are staged through LDS 32 k at a time with
+// coalesced loads, so each A row is read once per 64 output columns instead of once per
+// 32. Each tile's MFMA chain runs over k in the same ascending order as k_net_gemm, so the
+// two kernels give identical bits. K must be a multiple of 32.
+constexpr int NET_TK = 32;        // k per LDS stage
+constexpr int NET_LDS_ROW = 36;   // bf16 per staged row (32 + 4 padding)
+
+__global__ void __launch_bounds__(256) k_net_gemm_t(int M, int N, int K, const float *A, int lda,
+                                                    const uint16_t *Bt, const float *bias, int relu,
+                                                    const float *mask, int ldm, float *C, int ldc) {
+  __shared__ uint16_t As[64 * NET_LDS_ROW];
+  __shared__ uint16_t Bs[64 * NET_LDS_ROW];
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const int row0 = blockIdx.x * 64, col0 = blockIdx.y * 64;
+  const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
+  const int lr = t >> 2, lk = (t & 3) * 8;   // staging: 4 threads per row, 8 k each
+  float16v acc = {};
+  for (int kc = 0; kc < K; kc += NET_TK) {
+    {
+      const int gr = row0 + lr;
+      uint16_t *dst = As + lr * NET_LDS_ROW + lk;
+      if (gr < M) {
+        const float4 *src = (const float4 *)(A + (size_t)gr * lda + kc + lk);
+        const float4 x0 = src[0], x1 = src[1];
+        dst[0] = f2bf(x0.x); dst[1] = f2bf(x0.y); dst[2] = f2bf(x0.z); dst[3] = f2bf(x0.w);
+        dst[4] = f2bf(x1.x); dst[5] = f2bf(x1.y); dst[6] = f2bf(x1.z); dst[7] = f2bf(x1.w);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) dst[j] = 0;
+      }
+      const int gc = col0 + lr;
+      uint16_t *bd = Bs + lr * NET_LDS_ROW + lk;
+      if (gc < N) {
+        const uint16_t *bsrc = Bt + (size_t)gc * K + kc + lk;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bd[j] = bsrc[j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bd[j] = 0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k8 = 0; k8 < NET_TK; k8 += 8) {
+      const int kq = k8 + 4 * (lane >> 5);
+      const uint16_t *ap = As + (wr + (lane & 31)) * NET_LDS_ROW + kq;
+      const uint16_t *bp = Bs + (wc + (lane & 31)) * NET_LDS_ROW + kq;
+      short4v a, b;
+      a[0] = (short)ap[0]; a[1] = (short)ap[1]; a[2] = (short)ap[2]; a[3] = (short)ap[3];
+      b[0] = (short)bp[0]; b[1] = (short)bp[1]; b[2] = (short)bp[2]; b[3] = (short)bp[3];
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const int cc = col0 + wc + (lane & 31);
+  if (cc >= N) return;
+  const float bb = bias ? bias[cc] : 0.0f;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int row = row0 + wr + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+    if (row < M) {
+      float x = acc[v] + bb;
+      if (relu && x < 0.0f) x = 0.0f;
+      if (mask && mask[(size_t)row * ldm + cc] <= 0.0f) x = 0.0f;
+      C[(size_t)row * ldc + cc] = x;
+    }
+  }
+}
+
 // Partial weight gradients over a slice of rows: P[s][k][n] = sum_{m in slice s} A[m][k]*D[m][n]
 // (bf16 operands, FP32 accumulation), same MFMA tile layout with the roles k <-> m.
 __global__ void __launch_bounds__(64) k_net_wgrad(int M, int K, int N, const float *A, int lda,

@@ -115,6 +115,8 @@ _SIGS.update({
                                        ctypes.POINTER(ctypes.c_int)]),
     "nft_net_options": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "nft_set_incidence": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "nft_stream_calibration": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_double)]),
     "nft_set_multigrid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
                                          ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]),
     "nft_multigrid_field": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, f64p,
@@ -483,6 +485,12 @@ class Solver:
         _chk(lib().nft_multigrid_op(self.h, code, level, _p(np.ascontiguousarray(x, np.float64)), _p(out), impfl),
              "nft_multigrid_op")
         return out
+
+    def stream_calibration(self, nbytes=2 << 30, reps=10):
+        """Mean ms of a device copy of nbytes (HBM streaming reference)."""
+        ms = ctypes.c_double()
+        _chk(lib().nft_stream_calibration(self.h, int(nbytes), reps, ctypes.byref(ms)), "nft_stream_calibration")
+        return ms.value
 
     def set_incidence(self, columnar):
         """1: columnar gathers (when available), 0: SELL-64, -1: no change. Returns availability."""
