@@ -23,6 +23,11 @@ struct oc_ctx {
   /* line preconditioner: Thomas factors and operator rows */
   double *la, *linv, *lc;
   double *llo, *ldg, *lup;
+  /* precon 3: in-column band of laplc (impfl 1) by rows, bm2[i] = L[i][i-2] ... bp2[i] =
+   * L[i][i+2], and its LU factors (unit lower l1, l2; upper 1/d, u1, u2); band_L = column
+   * length (0: not built) */
+  double *bm2, *bm1, *b0, *bp1, *bp2, *fl1, *fl2, *fu1, *fu2, *fdinv;
+  int band_L, band_colors;
   int line_L;   /* uniform column length <= 64 (wavefront scans), else 0 */
   int line_R;   /* rows per lane of the coarsened scans for uniform 64 < L <= 512, else 0 (Thomas) */
   /* scratch */
@@ -56,6 +61,7 @@ static double amin1(double a, double b) { return a < b ? a : b; }
 static double *zalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
 
 static void build_line(oc_ctx *c);
+static void build_band(oc_ctx *c);
 
 /* compute_d_unstr (F:1112-1284) over the context's own mesh with density rh0_u */
 static void compute_d(oc_ctx *c, const double *rh0u, double *d) {
@@ -211,6 +217,7 @@ oc_ctx *oc_create(const oc_mesh *m, const oc_params *p) {
     c->coefw[i] = 1. / cw;
   }
   build_line(c);
+  if (p->precon == 3) build_band(c);
   /* launch geometry of the HIP path (src_amd/csrc/nft_capi.hip, nft_init) */
   c->nchunk = (n + 63) / 64;
   c->nblocks = (c->nchunk + 3) / 4;
@@ -303,6 +310,10 @@ void oc_destroy(oc_ctx *c) {
   for (int i = 0; i < 4; ++i) free(c->gb[i]);
   free(c->nb_ptr); free(c->nb_idx);
   for (int i = 0; i < 7; ++i) free(c->vp[i]);
+  {
+    double *band[] = {c->bm2, c->bm1, c->b0, c->bp1, c->bp2, c->fl1, c->fl2, c->fu1, c->fu2, c->fdinv};
+    for (size_t i = 0; i < sizeof(band) / sizeof(band[0]); ++i) free(band[i]);
+  }
   for (int l = 0; l < 2; ++l) { oc_destroy(c->lv[l]); free(c->lv_match[l]); free(c->lv_imatch[l]); }
   for (int i = 0; i < 12; ++i) free(c->mg_w[i]);
   free(c);
@@ -324,11 +335,17 @@ double *oc_field(oc_ctx *c, const char *name) {
   if (!strcmp(name, "line_lo")) return c->llo;
   if (!strcmp(name, "line_dg")) return c->ldg;
   if (!strcmp(name, "line_up")) return c->lup;
+  if (!strcmp(name, "band_m2")) return c->bm2;
+  if (!strcmp(name, "band_m1")) return c->bm1;
+  if (!strcmp(name, "band_0")) return c->b0;
+  if (!strcmp(name, "band_p1")) return c->bp1;
+  if (!strcmp(name, "band_p2")) return c->bp2;
   return NULL;
 }
 
 double oc_beta(oc_ctx *c) { return c->beta1; }
 void oc_get_stats(oc_ctx *c, oc_stats *s) { *s = c->st; }
+void oc_set_itr(oc_ctx *c, int itr) { c->p.itr = itr; }
 
 /* derivative(unk,dpdxa,idx), F:3943-4022, Cartesian branch */
 void oc_derivative(oc_ctx *c, const double *unk, double *dpdxa, int idx) {
@@ -532,6 +549,124 @@ static void build_line(oc_ctx *c) {
  *            beta = beta + gamma*beta[l+s]; gamma = gamma*gamma[l+s]
  * A column's result does not depend on its lane offset in a wavefront: alpha = 0 at its
  * first row and gamma = 0 at its last row turn every cross-column term into an exact 0. */
+/* precon 3: the in-column band of laplc(., impfl=1) probed column-color by column-color.
+ * laplc couples nodes at most two edges apart, so with probe nodes on columns that are
+ * pairwise at least three column hops apart (greedy colouring in column order) and on
+ * levels five apart, entry (row, probe) of laplc(indicator) is the only nonzero term of
+ * each row within two levels of a probe. The band is factorised per column (LU without
+ * pivoting, band width 2). Uniform columns only (band_L = 0 otherwise). */
+static int cmp_i32(const void *a, const void *b) {
+  int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+  return (x > y) - (x < y);
+}
+
+static void build_band(oc_ctx *c) {
+  const oc_mesh *m = &c->m;
+  int n = c->n, nc = m->ncol;
+  c->band_L = 0;
+  if (nc <= 0 || n % nc) return;
+  int L = n / nc;
+  for (int k = 0; k <= nc; ++k)
+    if (m->col_start[k] != k * L) return;
+  /* column adjacency (CSR, unique neighbours) */
+  int32_t *deg = (int32_t *)calloc((size_t)nc + 1, sizeof(int32_t));
+  for (int pass = 0; pass < 2; ++pass) {
+    int ne = pass == 0 ? m->nedge : m->nbface;
+    const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+    for (int e = 0; e < ne; ++e) {
+      int a = (ie[e] - 1) / L, b = (ie[ne + e] - 1) / L;
+      if (a != b) { deg[a + 1]++; deg[b + 1]++; }
+    }
+  }
+  for (int k = 0; k < nc; ++k) deg[k + 1] += deg[k];
+  int32_t *adj = (int32_t *)malloc(sizeof(int32_t) * ((size_t)deg[nc] + 1));
+  int32_t *fill = (int32_t *)calloc((size_t)nc, sizeof(int32_t));
+  for (int pass = 0; pass < 2; ++pass) {
+    int ne = pass == 0 ? m->nedge : m->nbface;
+    const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+    for (int e = 0; e < ne; ++e) {
+      int a = (ie[e] - 1) / L, b = (ie[ne + e] - 1) / L;
+      if (a != b) { adj[deg[a] + fill[a]++] = b; adj[deg[b] + fill[b]++] = a; }
+    }
+  }
+  /* unique neighbours per column (each column pair is joined by many edges) */
+  {
+    int32_t *uptr = (int32_t *)calloc((size_t)nc + 1, sizeof(int32_t));
+    int32_t *uadj = (int32_t *)malloc(sizeof(int32_t) * ((size_t)deg[nc] + 1));
+    int32_t w = 0;
+    for (int k = 0; k < nc; ++k) {
+      int32_t a0 = deg[k], a1 = deg[k + 1];
+      /* sort the row, then copy without repeats */
+      qsort(adj + a0, (size_t)(a1 - a0), sizeof(int32_t), cmp_i32);
+      for (int32_t q = a0; q < a1; ++q)
+        if (q == a0 || adj[q] != adj[q - 1]) uadj[w++] = adj[q];
+      uptr[k + 1] = w;
+    }
+    free(deg); free(adj);
+    deg = uptr; adj = uadj;
+  }
+  /* greedy colouring: a column takes the smallest colour absent within two column hops */
+  int *color = (int *)malloc(sizeof(int) * (size_t)nc);
+  int *mark = (int *)calloc(4096, sizeof(int));
+  int ncolors = 0;
+  for (int k = 0; k < nc; ++k) color[k] = -1;
+  for (int k = 0; k < nc; ++k) {
+    for (int q = deg[k]; q < deg[k + 1]; ++q) {
+      int a = adj[q];
+      if (color[a] >= 0 && color[a] < 4096) mark[color[a]] = k + 1;
+      for (int q2 = deg[a]; q2 < deg[a + 1]; ++q2) {
+        int b = adj[q2];
+        if (color[b] >= 0 && color[b] < 4096) mark[color[b]] = k + 1;
+      }
+    }
+    int col = 0;
+    while (col < 4095 && mark[col] == k + 1) col++;
+    color[k] = col;
+    if (col + 1 > ncolors) ncolors = col + 1;
+  }
+  double **bands[] = {&c->bm2, &c->bm1, &c->b0, &c->bp1, &c->bp2, &c->fl1, &c->fl2, &c->fu1, &c->fu2, &c->fdinv};
+  for (size_t q = 0; q < sizeof(bands) / sizeof(bands[0]); ++q) *bands[q] = zalloc(n);
+  double *v = c->s[9], *Lv = c->s[10];
+  for (int col = 0; col < ncolors; ++col) {
+    for (int r = 0; r < 5; ++r) {
+      for (int i = 0; i < n; ++i) v[i] = 0.;
+      for (int k = 0; k < nc; ++k)
+        if (color[k] == col)
+          for (int l = r; l < L; l += 5) v[k * L + l] = 1.;
+      oc_laplc(c, v, Lv, 1);
+      for (int k = 0; k < nc; ++k) {
+        if (color[k] != col) continue;
+        for (int l = r; l < L; l += 5) {
+          int i = k * L + l;
+          c->b0[i] = Lv[i];
+          if (l + 1 < L) c->bm1[i + 1] = Lv[i + 1];
+          if (l + 2 < L) c->bm2[i + 2] = Lv[i + 2];
+          if (l >= 1) c->bp1[i - 1] = Lv[i - 1];
+          if (l >= 2) c->bp2[i - 2] = Lv[i - 2];
+        }
+      }
+    }
+  }
+  /* LU per column: row k = (l2, l1, 1) x U, U row k = (d, u1, u2) */
+  for (int k = 0; k < nc; ++k) {
+    for (int l = 0; l < L; ++l) {
+      int i = k * L + l;
+      double l2 = 0., l1 = 0., t1 = c->bm1[i], d = c->b0[i];
+      if (l >= 2) { l2 = c->bm2[i] * c->fdinv[i - 2]; t1 = t1 - l2 * c->fu1[i - 2]; d = d - l2 * c->fu2[i - 2]; }
+      if (l >= 1) { l1 = t1 * c->fdinv[i - 1]; d = d - l1 * c->fu1[i - 1]; }
+      double u1 = l + 1 < L ? c->bp1[i] : 0.;
+      if (l >= 1 && l + 1 < L) u1 = u1 - l1 * c->fu2[i - 1];
+      c->fl1[i] = l1; c->fl2[i] = l2;
+      c->fu1[i] = u1;
+      c->fu2[i] = l + 2 < L ? c->bp2[i] : 0.;
+      c->fdinv[i] = 1. / d;
+    }
+  }
+  c->band_L = L;
+  c->band_colors = ncolors;
+  free(deg); free(adj); free(fill); free(color); free(mark);
+}
+
 static void scan_column(const double *la, const double *linv, const double *lc, const double *rhs, int L,
                         double *z) {
   double al[64], be[64], ga[64], na[64], nb[64];
@@ -611,8 +746,37 @@ static void scan_column_r(const double *la, const double *linv, const double *lc
 /* z = M^-1 rhs, column by column: wavefront scans for uniform columns of at most 64
  * levels, coarsened scans up to 512 levels, otherwise a forward then backward Thomas
  * sweep */
+/* precon 3, one column of L rows: forward y_k = r_k - l1_k y_(k-1) - l2_k y_(k-2), then
+ * z_k = (y_k - u1_k z_(k+1) - u2_k z_(k+2)) * (1/d_k), terms past the column's ends
+ * dropped; the HIP path's k_line_penta runs the same sequence per column. */
+static void band_column(const oc_ctx *c, int s0, int L, const double *rhs, double *z) {
+  double y1 = 0., y2 = 0.;
+  for (int k = 0; k < L; ++k) {
+    int i = s0 + k;
+    double y = rhs[i];
+    if (k >= 1) y = y - c->fl1[i] * y1;
+    if (k >= 2) y = y - c->fl2[i] * y2;
+    z[i] = y;
+    y2 = y1; y1 = y;
+  }
+  double z1 = 0., z2 = 0.;
+  for (int k = L - 1; k >= 0; --k) {
+    int i = s0 + k;
+    double t = z[i];
+    if (k + 1 < L) t = t - c->fu1[i] * z1;
+    if (k + 2 < L) t = t - c->fu2[i] * z2;
+    t = t * c->fdinv[i];
+    z[i] = t;
+    z2 = z1; z1 = t;
+  }
+}
+
 void oc_line_solve(oc_ctx *c, const double *rhs, double *z) {
   const oc_mesh *m = &c->m;
+  if (c->band_L > 0) {
+    for (int k = 0; k < m->ncol; ++k) band_column(c, k * c->band_L, c->band_L, rhs, z);
+    return;
+  }
   if (c->line_L > 0) {
     int L = c->line_L;
     for (int k = 0; k < m->ncol; ++k) {

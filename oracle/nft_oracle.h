@@ -15,7 +15,9 @@
  *   V-cycle)                -> golden vectors of the reference run with mg_flag=1 on
  *                              generated nested meshes (tests/golden)
  *   line preconditioner     -> no reference counterpart: pinned by a dense solve of the
- *                              tridiagonal system and by the closed form of the coefficients
+ *                              tridiagonal system and by the closed form of the coefficients;
+ *                              the probed band (precon 3) by the dense matrix of laplc
+ *                              (unit-vector columns) and a dense solve of the band
  */
 #ifndef NFT_ORACLE_H
 #define NFT_ORACLE_H
@@ -48,7 +50,9 @@ typedef struct {
   double strat;           /* N^2/g */
   double theta0;          /* tetbar */
   int lord, itr, exit_mode; double eps0;
-  int precon;             /* 0 none (iprc=0), 1 point relaxation, 2 vertical line */
+  int precon;             /* 0 none (iprc=0), 1 point relaxation, 2 vertical line
+                           * (two-point-flux tridiagonal), 3 vertical line with the
+                           * in-column band of laplc itself (pentadiagonal, probed) */
   int sm_flag;            /* point relaxation: 1 Jacobi, 2 Richardson, 3 mixed */
   int itrp1; double omega;
   /* sponge (irl=1 when any tow is nonzero) */
@@ -75,6 +79,8 @@ void oc_destroy(oc_ctx *c);
 double *oc_field(oc_ctx *c, const char *name);
 double oc_beta(oc_ctx *c);
 void oc_get_stats(oc_ctx *c, oc_stats *s);
+/* maximum outer GCR cycles of later oc_gcrk calls */
+void oc_set_itr(oc_ctx *c, int itr);
 
 /* reference subroutines */
 void oc_derivative(oc_ctx *c, const double *unk, double *dpdxa, int idx);

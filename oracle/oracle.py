@@ -64,6 +64,7 @@ def lib():
         L.oc_beta.restype = ctypes.c_double
         L.oc_beta.argtypes = [ctypes.c_void_p]
         L.oc_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(OcStats)]
+        L.oc_set_itr.argtypes = [ctypes.c_void_p, ctypes.c_int]
         vp = ctypes.c_void_p
         L.oc_derivative.argtypes = [vp, f64p, f64p, ctypes.c_int]
         L.oc_laplc.argtypes = [vp, f64p, f64p, ctypes.c_int]
@@ -158,6 +159,9 @@ class Oracle:
     @property
     def beta(self):
         return lib().oc_beta(self.h)
+
+    def set_itr(self, itr):
+        lib().oc_set_itr(self.h, itr)
 
     def stats(self):
         s = OcStats()

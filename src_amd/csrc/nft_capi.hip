@@ -76,6 +76,10 @@ struct nft_ctx {
   double *la, *linv, *lc;
   float *laf, *linvf, *lcf;
   int line_L = 0;             // uniform column length <= 64 (wavefront scans), else 0
+  // precon 3: in-column band of laplc (host rows bm2..bp2, factors interleaved on the device)
+  std::vector<double> h_band[5];
+  BandFactors band{};
+  int band_L = 0, band_cpb = 0, band_colors = 0;
   int line_R = 0;             // rows per lane of the coarsened scan for 64 < L <= 512, else 0 (Thomas)
   int col_L = 0;              // uniform column length with nodes numbered column by column (0: not)
   int col_on = 0;             // gathers use the columnar incidence (inc.col_L > 0)
@@ -582,7 +586,12 @@ int fine_sweep(nft_ctx *c, const double *rhs, double *p, int impfl, bool first) 
     }
     t_begin(c, KT_LINE);
     const double *pin = first ? nullptr : p;
-    if (c->line_L > 0) {
+    if (P.precon == 3) {
+      const int nb = (c->ncol + c->band_cpb - 1) / c->band_cpb;
+      const size_t shm = (size_t)c->band_cpb * (c->band_L + 1) * sizeof(double);
+      hipLaunchKernelGGL(k_line_penta, dim3(nb), dim3(NFT_WAVE), shm, c->stream, c->ncol, c->band_L, c->band_cpb,
+                         c->band, res, pin, p, P.omega, c->skip);
+    } else if (c->line_L > 0) {
       int cpw = 64 / c->line_L;
       long waves = (c->ncol + cpw - 1) / cpw;
       int cg = (int)((waves + NFT_WPB - 1) / NFT_WPB);
@@ -1329,6 +1338,130 @@ int sync(nft_ctx *c) {
   return 0;
 }
 
+// precon 3: the in-column band of laplc(., impfl=1), probed column-colour by colour.
+// laplc couples nodes at most two edges apart, so with probe nodes on columns pairwise at
+// least three column hops apart (greedy colouring in column order) and on levels five
+// apart, entry (row, probe) of laplc(indicator) is the only nonzero term of each row within
+// two levels of a probe. Each column's band is factorised (LU, no pivoting, band width 2)
+// on the host in the oracle's operation order, and the factors are stored interleaved for
+// k_line_penta.
+int build_band(nft_ctx *c, const nft_mesh *m) {
+  const int n = c->n, nc = m->ncol;
+  if (nc <= 0 || n % nc) return set_err(-1, "nft_init: precon 3 needs columns of one length");
+  const int L = n / nc;
+  for (int k = 0; k <= nc; ++k)
+    if (m->col_start[k] != k * L) return set_err(-1, "nft_init: precon 3 needs columns of one length");
+  const int cpb = L <= 96 ? 64 : L <= 200 ? 32 : L <= 400 ? 16 : 0;
+  if (!cpb) return set_err(-1, "nft_init: precon 3 supports columns of at most 400 levels");
+  // column adjacency (interior edges, then boundary-face edges)
+  std::vector<int32_t> deg(nc + 1, 0);
+  for (int pass = 0; pass < 2; ++pass) {
+    const int ne = pass == 0 ? m->nedge : m->nbface, ld = pass == 0 ? m->ld_edge : m->ld_face;
+    const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+    for (int e = 0; e < ne; ++e) {
+      int a = (ie[e] - 1) / L, b = (ie[ld + e] - 1) / L;
+      if (a != b) { deg[a + 1]++; deg[b + 1]++; }
+    }
+  }
+  for (int k = 0; k < nc; ++k) deg[k + 1] += deg[k];
+  std::vector<int32_t> adj(deg[nc] + 1), fill(nc, 0);
+  for (int pass = 0; pass < 2; ++pass) {
+    const int ne = pass == 0 ? m->nedge : m->nbface, ld = pass == 0 ? m->ld_edge : m->ld_face;
+    const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+    for (int e = 0; e < ne; ++e) {
+      int a = (ie[e] - 1) / L, b = (ie[ld + e] - 1) / L;
+      if (a != b) { adj[deg[a] + fill[a]++] = b; adj[deg[b] + fill[b]++] = a; }
+    }
+  }
+  // unique neighbours per column (each column pair is joined by many edges)
+  {
+    std::vector<int32_t> uptr(nc + 1, 0), uadj;
+    uadj.reserve(adj.size() / 8 + 1);
+    for (int k = 0; k < nc; ++k) {
+      std::vector<int32_t> row(adj.begin() + deg[k], adj.begin() + deg[k + 1]);
+      std::sort(row.begin(), row.end());
+      row.erase(std::unique(row.begin(), row.end()), row.end());
+      uadj.insert(uadj.end(), row.begin(), row.end());
+      uptr[k + 1] = (int32_t)uadj.size();
+    }
+    deg.swap(uptr);
+    adj.swap(uadj);
+  }
+  std::vector<int> color(nc, -1), mark(4096, 0);
+  int ncolors = 0;
+  for (int k = 0; k < nc; ++k) {
+    for (int q = deg[k]; q < deg[k + 1]; ++q) {
+      const int a = adj[q];
+      if (color[a] >= 0 && color[a] < 4096) mark[color[a]] = k + 1;
+      for (int q2 = deg[a]; q2 < deg[a + 1]; ++q2) {
+        const int b = adj[q2];
+        if (color[b] >= 0 && color[b] < 4096) mark[color[b]] = k + 1;
+      }
+    }
+    int col = 0;
+    while (col < 4095 && mark[col] == k + 1) col++;
+    color[k] = col;
+    ncolors = std::max(ncolors, col + 1);
+  }
+  std::vector<double> &bm2 = c->h_band[0], &bm1 = c->h_band[1], &b0 = c->h_band[2], &bp1 = c->h_band[3],
+                      &bp2 = c->h_band[4];
+  for (auto &b : c->h_band) b.assign(n, 0.);
+  {
+    // probes built, applied and read out on the device; the rows come back once
+    int32_t *d_color;
+    int rc = upload(c, &d_color, color.data(), nc);
+    if (rc) return rc;
+    double *band[5] = {c->io[2], c->io[3], c->io[4], c->io[5], c->io[6]};
+    const dim3 g(c->nodes_grid), b(NFT_BLOCK);
+    for (int q = 0; q < 5; ++q) hipLaunchKernelGGL(k_fill, g, b, 0, c->stream, n, band[q], 0.0);
+    for (int col = 0; col < ncolors; ++col) {
+      for (int r = 0; r < 5; ++r) {
+        hipLaunchKernelGGL(k_band_probe, g, b, 0, c->stream, n, L, d_color, col, r, c->io[0]);
+        if ((rc = laplc_dev(c, c->io[0], c->io[1], 1, nullptr))) return rc;
+        hipLaunchKernelGGL(k_band_extract, g, b, 0, c->stream, n, L, d_color, col, r, c->io[1], band[0], band[1],
+                           band[2], band[3], band[4]);
+      }
+    }
+    HIPCHK(hipGetLastError());
+    for (int q = 0; q < 5; ++q)
+      HIPCHK(hipMemcpyAsync(c->h_band[q].data(), band[q], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  // LU per column: row k = (l2, l1, 1) x U, U row k = (d, u1, u2)
+  std::vector<double> fl1(n), fl2(n), fu1(n), fu2(n), fdinv(n);
+  for (int k = 0; k < nc; ++k) {
+    for (int l = 0; l < L; ++l) {
+      const size_t i = (size_t)k * L + l;
+      double l2 = 0., l1 = 0., t1 = bm1[i], d = b0[i];
+      if (l >= 2) { l2 = bm2[i] * fdinv[i - 2]; t1 = t1 - l2 * fu1[i - 2]; d = d - l2 * fu2[i - 2]; }
+      if (l >= 1) { l1 = t1 * fdinv[i - 1]; d = d - l1 * fu1[i - 1]; }
+      double u1 = l + 1 < L ? bp1[i] : 0.;
+      if (l >= 1 && l + 1 < L) u1 = u1 - l1 * fu2[i - 1];
+      fl1[i] = l1; fl2[i] = l2;
+      fu1[i] = u1;
+      fu2[i] = l + 2 < L ? bp2[i] : 0.;
+      fdinv[i] = 1. / d;
+    }
+  }
+  // interleave: block cb of cpb columns, entry (cb*L + l)*cpb + lane
+  const int nb = (nc + cpb - 1) / cpb;
+  const size_t ni = (size_t)nb * L * cpb;
+  std::vector<double> *src[5] = {&fl1, &fl2, &fu1, &fu2, &fdinv};
+  double *dst[5];
+  for (int q = 0; q < 5; ++q) {
+    std::vector<double> il(ni, 0.);
+    for (int k = 0; k < nc; ++k) {
+      const int cb = k / cpb, lane = k % cpb;
+      for (int l = 0; l < L; ++l) il[((size_t)cb * L + l) * cpb + lane] = (*src[q])[(size_t)k * L + l];
+    }
+    int rc = upload(c, &dst[q], il.data(), ni);
+    if (rc) return rc;
+  }
+  c->band = BandFactors{dst[0], dst[1], dst[2], dst[3], dst[4]};
+  c->band_L = L; c->band_cpb = cpb; c->band_colors = ncolors;
+  return 0;
+}
+
 // connectivity and volume checks of a mesh before it reaches any kernel
 int validate_mesh(const nft_mesh *m, const char *who) {
   const std::string w(who);
@@ -1549,8 +1682,8 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   if (!m || !P || !out) return set_err(-1, "nft_init: null argument");
   *out = nullptr;
   if (P->lord < 1 || P->lord > NFT_MAXD) return set_err(-1, "nft_init: lord out of range 1..18");
-  if (P->precon == 2 && m->ncol <= 0) return set_err(-1, "nft_init: line preconditioner needs columns");
-  if (P->precon < 0 || P->precon > 2) return set_err(-1, "nft_init: unknown preconditioner");
+  if (P->precon >= 2 && m->ncol <= 0) return set_err(-1, "nft_init: line preconditioner needs columns");
+  if (P->precon < 0 || P->precon > 3) return set_err(-1, "nft_init: unknown preconditioner");
   if (!(P->dt > 0)) return set_err(-1, "nft_init: dt must be positive");
   if (P->itraj != 0 && P->itraj != 1) return set_err(-1, "nft_init: itraj must be 0 or 1");
   int ndev = 0;
@@ -1685,6 +1818,7 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
     return fail(set_err(-2, "event"));
   for (int k = 0; k < 4; ++k)
     if (hipMemset(c->frc[k], 0, n * sizeof(double)) != hipSuccess) return fail(set_err(-2, "memset"));
+  if (P->precon == 3 && (rc = build_band(c, m))) return fail(rc);
   if (hipDeviceSynchronize() != hipSuccess) return fail(set_err(-2, "sync"));
   *out = c;
   return 0;
@@ -1717,6 +1851,12 @@ int nft_get_field(nft_ctx *c, const char *name, double *host) {
   else if (s == "line_a") v = &c->h_la;
   else if (s == "line_inv") v = &c->h_linv;
   else if (s == "line_c") v = &c->h_lc;
+  else if (s == "band_m2" || s == "band_m1" || s == "band_0" || s == "band_p1" || s == "band_p2") {
+    if (!c->band_L) return set_err(-1, "nft_get_field: no band (precon 3 only)");
+    const char *names[5] = {"band_m2", "band_m1", "band_0", "band_p1", "band_p2"};
+    for (int q = 0; q < 5; ++q)
+      if (s == names[q]) v = &c->h_band[q];
+  }
   if (v) { std::memcpy(host, v->data(), c->n * sizeof(double)); return 0; }
   double *d = nullptr;
   if (s == "rh0") d = c->rh0;
@@ -1968,7 +2108,9 @@ static double kernel_bytes(nft_ctx *c, int k) {
   switch (k) {
     case KT_LGRAD: return geomg + 8 * N /*p*/ + 4 * 8 * N /*vol rh0 coefw ue*/ + 3 * 8 * N /*q*/;
     case KT_LDIV: return geomg + 3 * 8 * N /*q*/ + 2 * 8 * N /*vol rh0*/ + 8 * N /*out*/;
-    case KT_LINE: return c->line_L > 0 ? 3 * ts * N /*rows*/ + 8 * N /*rhs*/ + 8 * N /*p*/
+    case KT_LINE:
+      if (c->P.precon == 3) return 5 * 8 * N /*factors*/ + 8 * N /*rhs*/ + 8 * N /*pin*/ + 8 * N /*p*/;
+      return c->line_L > 0 ? 3 * ts * N /*rows*/ + 8 * N /*rhs*/ + 8 * N /*p*/
                                        : 4 * (c->ncol + 1) + 3 * ts * N /*factors*/ + 8 * N + 8 * N + 2 * ts * N /*z*/;
     case KT_UPD: return 6 * 8 * N;  /* p,r read+write; x, ax read */
     case KT_DIR: return (3 + 2.0 * c->P.lord) * 8 * N;  /* qr, ar, r + up to lord x/ax; writes 2 */
@@ -2089,7 +2231,7 @@ int nft_net_eval(nft_ctx *c, const float *X, const float *T, int nsamples, float
 // store every stride-th (r, M_line^-1 r) pair of the GCR solves that follow, up to max_pairs
 int nft_dump_pairs(nft_ctx *c, int max_pairs, int stride) {
   if (!c || !c->net.ready || max_pairs < 0 || stride < 1) return set_err(-1, "nft_dump_pairs: bad argument");
-  if (max_pairs > 0 && c->P.precon != 2) return set_err(-1, "nft_dump_pairs: needs the line preconditioner");
+  if (max_pairs > 0 && c->P.precon < 2) return set_err(-1, "nft_dump_pairs: needs a line preconditioner");
   auto &n = c->net;
   const size_t N = c->n;
   int rc;
@@ -2401,7 +2543,7 @@ int nft_set_incidence(nft_ctx *c, int columnar, int *available) {
 int nft_set_learned(nft_ctx *c, int kind, double stall_ratio, int *fallbacks, int *learned_iters) {
   if (!c) return set_err(-1, "nft_set_learned: null context");
   if (kind == 1 && !c->net.ready) return set_err(-1, "nft_set_learned: network not initialised");
-  if (kind == 1 && c->P.precon != 2) return set_err(-1, "nft_set_learned: needs the line preconditioner");
+  if (kind == 1 && c->P.precon < 2) return set_err(-1, "nft_set_learned: needs a line preconditioner");
   if (kind >= 0) { c->net.kind = kind; c->net.stall = stall_ratio; }
   if (fallbacks) *fallbacks = c->net.fallbacks;
   if (learned_iters) *learned_iters = c->net.learned_iters;

@@ -1081,6 +1081,113 @@ __global__ void k_sub(int n, const double *a, const double *b, double *c, const 
   if (k < n) c[k] = a[k] - b[k];
 }
 
+// ------------------------------------------------------------------------------------
+// Vertical line preconditioner with the in-column band of laplc itself (precon 3): per
+// column the banded LU factors (unit lower l1, l2; upper 1/d, u1, u2), stored interleaved
+// as f[(block*L + k)*cpb + c] so lane c of a block reads them coalesced level by level.
+// One wavefront per block of cpb consecutive columns: the residual block (contiguous in
+// memory) is staged in LDS with coalesced loads, each lane runs its column's forward and
+// backward substitution sequentially from LDS (rows padded to L+1 doubles), and the result
+// is written back coalesced as p = pin + omega*z. Same operation sequence as the oracle.
+// ------------------------------------------------------------------------------------
+// band probing (set-up of precon 3): v = indicator of the probe nodes (columns of colour
+// col, levels = r mod 5), then each probe's column rows within two levels of it take
+// their entry of laplc(v)
+__global__ void k_band_probe(int n, int L, const int32_t *color, int col, int r, double *v) {
+  const int i = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  v[i] = (color[i / L] == col && (i % L) % 5 == r) ? 1.0 : 0.0;
+}
+
+__global__ void k_band_extract(int n, int L, const int32_t *color, int col, int r, const double *Lv, double *bm2,
+                               double *bm1, double *b0, double *bp1, double *bp2) {
+  const int i = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const int l = i % L;
+  if (color[i / L] != col || l % 5 != r) return;
+  b0[i] = Lv[i];
+  if (l + 1 < L) bm1[i + 1] = Lv[i + 1];
+  if (l + 2 < L) bm2[i + 2] = Lv[i + 2];
+  if (l >= 1) bp1[i - 1] = Lv[i - 1];
+  if (l >= 2) bp2[i - 2] = Lv[i - 2];
+}
+
+#ifndef NFT_PENTA_CHUNK
+#define NFT_PENTA_CHUNK 16
+#endif
+
+struct BandFactors {
+  const double *l1, *l2, *u1, *u2, *dinv;
+};
+
+__global__ void __launch_bounds__(NFT_WAVE) k_line_penta(int ncol, int L, int cpb, BandFactors f,
+                                                         const double *res, const double *pin, double *p,
+                                                         double omega, const double *skip) {
+  if (skipped(skip)) return;
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x, cb = blockIdx.x;
+  const int c0 = cb * cpb, ncb = min(cpb, ncol - c0);
+  const int RL = L + 1;
+  const size_t node0 = (size_t)c0 * L;
+  for (int idx = lane; idx < ncb * L; idx += NFT_WAVE) lds[(idx / L) * RL + idx % L] = res[node0 + idx];
+  __syncthreads();
+  if (lane < ncb) {
+    // the factors of NFT_PENTA_CHUNK levels are loaded together ahead of their steps, so
+    // the substitution chain waits on memory once per chunk, not once per level
+    double *z = lds + lane * RL;
+    const size_t fb = (size_t)cb * L * cpb + lane;
+    double y1 = 0., y2 = 0.;
+    for (int k0 = 0; k0 < L; k0 += NFT_PENTA_CHUNK) {
+      double a1[NFT_PENTA_CHUNK], a2[NFT_PENTA_CHUNK];
+#pragma unroll
+      for (int j = 0; j < NFT_PENTA_CHUNK; ++j) {
+        const int k = min(k0 + j, L - 1);
+        a1[j] = f.l1[fb + (size_t)k * cpb];
+        a2[j] = f.l2[fb + (size_t)k * cpb];
+      }
+#pragma unroll
+      for (int j = 0; j < NFT_PENTA_CHUNK; ++j) {
+        const int k = k0 + j;
+        if (k < L) {
+          double y = z[k];
+          if (k >= 1) y = y - a1[j] * y1;
+          if (k >= 2) y = y - a2[j] * y2;
+          z[k] = y;
+          y2 = y1; y1 = y;
+        }
+      }
+    }
+    double z1 = 0., z2 = 0.;
+    for (int k0 = L - 1; k0 >= 0; k0 -= NFT_PENTA_CHUNK) {
+      double b1[NFT_PENTA_CHUNK], b2[NFT_PENTA_CHUNK], bd[NFT_PENTA_CHUNK];
+#pragma unroll
+      for (int j = 0; j < NFT_PENTA_CHUNK; ++j) {
+        const int k = max(k0 - j, 0);
+        b1[j] = f.u1[fb + (size_t)k * cpb];
+        b2[j] = f.u2[fb + (size_t)k * cpb];
+        bd[j] = f.dinv[fb + (size_t)k * cpb];
+      }
+#pragma unroll
+      for (int j = 0; j < NFT_PENTA_CHUNK; ++j) {
+        const int k = k0 - j;
+        if (k >= 0) {
+          double t = z[k];
+          if (k + 1 < L) t = t - b1[j] * z1;
+          if (k + 2 < L) t = t - b2[j] * z2;
+          t = t * bd[j];
+          z[k] = t;
+          z2 = z1; z1 = t;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int idx = lane; idx < ncb * L; idx += NFT_WAVE) {
+    const double p0 = pin ? pin[node0 + idx] : 0.;
+    p[node0 + idx] = p0 + omega * lds[(idx / L) * RL + idx % L];
+  }
+}
+
 // plain streaming copy (bandwidth reference)
 __global__ void __launch_bounds__(NFT_BLOCK) k_stream_copy(const double2 *a, double2 *b, size_t n) {
   for (size_t k = (size_t)blockIdx.x * NFT_BLOCK + threadIdx.x; k < n; k += (size_t)gridDim.x * NFT_BLOCK) b[k] = a[k];

@@ -298,3 +298,28 @@ def test_summation_order_effect():
         o.init()
         o.step(1)
     assert max(rel_l2(b.state("unk")[k], a.state("unk")[k]) for k in range(4)) > 1e-8
+
+
+@pytest.mark.parametrize("key", ["tiny2", "tall"])
+def test_probed_band_is_laplc_and_its_solve_is_exact(key):
+    """precon 3: the probed in-column band equals the entries of laplc's dense matrix (built
+    from unit vectors) bit for bit, and the banded LU solve equals a dense solve of it."""
+    m = mesh_for(key)
+    o = Oracle(m, tiny_cfg(precon=3, itrp1=1, omega=1.0))
+    n, L = m.npoin, m.nlev
+    A = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        A[:, j] = o.laplc(e, 1)
+    B = np.zeros((n, n))
+    for off, name in ((-2, "m2"), (-1, "m1"), (0, "0"), (1, "p1"), (2, "p2")):
+        b = o.field("band_" + name)
+        for i in range(n):
+            if 0 <= i % L + off < L:
+                assert b[i] == A[i, i + off], (name, i)
+                B[i, i + off] = b[i]
+    r = np.random.default_rng(1).normal(size=n)
+    z = o.line_solve(r)
+    zd = np.linalg.solve(B, r)
+    assert np.abs(z - zd).max() <= 1e-12 * np.abs(zd).max()
