@@ -56,7 +56,10 @@ typedef struct {
   int itr;                   /* max outer GCR cycles */
   int exit_mode;             /* 0: max|r| <= eps0/dt (reference); 1: max|r| <= eps0*max|b| */
   double eps0;
-  int precon;                /* 0 none (iprc=0), 1 point relaxation (reference), 2 vertical line */
+  int precon;                /* 0 none (iprc=0), 1 point relaxation (reference), 2 vertical line
+                                (two-point-flux tridiagonal), 3 vertical line with the in-column
+                                band of laplc (pentadiagonal, probed at nft_init; columns of one
+                                length, at most 400 levels) */
   int sm_flag;               /* point relaxation: 1 Jacobi, 2 Richardson, 3 mixed */
   int itrp1;                 /* preconditioner sweeps */
   double omega;
