@@ -1,11 +1,12 @@
-"""Writes tests/golden/small_line_oracle.npz: 100 NFT steps on the 40,000-node mesh with
-the vertical line preconditioner and a 1e-8 relative GCR(3) exit, computed by the CPU
+"""Writes tests/golden/small_line_oracle.npz and small_band_oracle.npz: 100 NFT steps on
+the 40,000-node mesh with the vertical line preconditioner (two-point-flux tridiagonal,
+and the in-column band of laplc) and a 1e-8 relative GCR(3) exit, computed by the CPU
 oracle in the reference's own (sequential) summation order. The reference itself has no
 line preconditioner and does not converge with its point relaxation in reasonable time,
 so this converging run uses the oracle, which equals the reference bit for bit on every
 operator (tiny_ops.npz, tiny_run.npz, small_run.npz).
 
-    python tests/golden/make_oracle_golden.py
+    python tests/golden/make_oracle_golden.py [line|band]
 """
 import os
 import sys
@@ -20,11 +21,13 @@ from common import mesh_checksum, mesh_for, small_cfg  # noqa: E402
 from oracle import Oracle  # noqa: E402
 
 LINE = dict(precon=2, itrp1=1, omega=1.0, exit_mode=1, eps0=1e-8, itr=200)
+BAND = dict(LINE, precon=3)
 
 
 def main():
+    kind = sys.argv[1] if len(sys.argv) > 1 else "line"
     m = mesh_for("small")
-    o = Oracle(m, small_cfg(**LINE))
+    o = Oracle(m, small_cfg(**(BAND if kind == "band" else LINE)))
     o.init()
     its = [o.stats().niter]
     out = {"mesh_checksum": mesh_checksum(m)}
@@ -41,7 +44,7 @@ def main():
             out[f"s{step}_stride"] = np.array([s])
             out[f"s{step}_niter"] = np.array(its, dtype=np.int32)
             print(step, its[-5:], flush=True)
-    np.savez_compressed(os.path.join(ROOT, "tests", "golden", "small_line_oracle.npz"), **out)
+    np.savez_compressed(os.path.join(ROOT, "tests", "golden", f"small_{kind}_oracle.npz"), **out)
 
 
 if __name__ == "__main__":

@@ -205,19 +205,21 @@ def test_steps_bitwise_device_order(key, kw, nsteps):
     s.close()
 
 
-def test_steps_small_100_line_vs_sequential_oracle():
-    """100 steps, line-preconditioned GCR(3) at 1e-8: the GPU against the oracle in the
-    reference's sequential summation order (fixture small_line_oracle.npz). Only the
-    inner-product summation order differs; tolerance: prognostic fields u, v, w, theta'
-    and the advector within 1e-12 relative L2 (sampled field and full-field norm), GCR
-    iterations within one per solve."""
-    path = os.path.join(GOLD, "small_line_oracle.npz")
+@pytest.mark.parametrize("fixture,precon", [("small_line_oracle.npz", 2), ("small_band_oracle.npz", 3)])
+def test_steps_small_100_line_vs_sequential_oracle(fixture, precon):
+    """100 steps, line-preconditioned GCR(3) at 1e-8 (two-point-flux line, and the band of
+    laplc): the GPU against the oracle in the reference's sequential summation order
+    (fixtures small_line_oracle.npz, small_band_oracle.npz). Only the inner-product
+    summation order differs; tolerance: prognostic fields u, v, w, theta' and the advector
+    within 1e-12 relative L2 (sampled field and full-field norm), GCR iterations within one
+    per solve."""
+    path = os.path.join(GOLD, fixture)
     if not os.path.exists(path):
         pytest.skip("fixture not generated")
     g = np.load(path, allow_pickle=False)
     m = mesh_for("small")
     assert np.array_equal(g["mesh_checksum"], mesh_checksum(m))
-    s = solver(m, small_cfg(**LINE))
+    s = solver(m, small_cfg(**dict(LINE, precon=precon)))
     s.state_init()
     its = [s.gcr_stats().niter]
     done = 0
