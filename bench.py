@@ -96,10 +96,11 @@ def timed_run(mesh, cfg, precision, steps, warmup, profile=True):
     return out
 
 
-def learned_run(mesh, cfg, steps, train_steps, log_prefix="[learned]"):
+def learned_run(mesh, cfg, steps, train_steps, warmup=1, log_prefix="[learned]"):
     """Plain FP64 solver with the learned correction: pairs (r, M_line^-1 r) are stored
-    during one plain step, the network is trained on them (residual objective), then
-    `steps` steps run with the correction (and its stall fallback) in every GCR solve."""
+    during the first plain step, the network is trained on them (residual objective), the
+    rest of the `warmup` steps run plain, then `steps` steps run with the correction (and
+    its stall fallback) in every GCR solve."""
     from src_amd import nft
     s = nft.Solver(mesh, nft.make_params(cfg))
     s.net_init(hidden=128, seed=1, max_batch=2)
@@ -112,6 +113,8 @@ def learned_run(mesh, cfg, steps, train_steps, log_prefix="[learned]"):
     l0, l1 = s.net_train(steps=train_steps, batch=2, lr=1e-3, seed=3)
     t_train = time.perf_counter() - t0
     log(f"{log_prefix} {npairs} pairs, {train_steps} Adam steps in {t_train:.1f} s, loss {l0:.4f} -> {l1:.4f}")
+    if warmup > 1:
+        s.run_steps(warmup - 1)
     s.set_learned(1, stall=0.99)
     s.kernel_timer(11)   # the network's forward pass: six bf16 MFMA GEMMs over all columns
     st = s.run_steps(steps)
@@ -321,16 +324,18 @@ def main():
         line["roofline"]["frac_of_stream_copy"] = round(achieved / cp, 4) if cp > 0 else None
         log(f"2 GiB device copy: {cp:.0f} GB/s; {kname} algorithmic rate = {achieved / cp:.2f} of it")
     if not args.no_variants:
-        rm = timed_run(mesh, cfg, "mixed", max(K // 2, 2), 1, profile=False)
+        # the variants run the same warm-up and step count as the FP64 line, so their
+        # iteration counts cover the same part of the run
+        mk = K
+        rm = timed_run(mesh, cfg, "mixed", mk, args.warmup, profile=False)
         sm = rm["st"]
         rm["solver"].close()
-        mk = max(K // 2, 2)
         line["variants"] = {"mixed": {
             "value": round(1000.0 / (sm.ms_total / mk), 4), "unit": "timesteps/s",
             "iters_per_solve": round(sm.gcr_iters / max(sm.gcr_solves, 1), 2),
             "solve_ms": round(sm.ms_gcr / max(sm.gcr_solves, 1), 3),
             "mpdata_ms_per_step": round(sm.ms_mpdata / mk, 3), "steps": mk}}
-        line["variants"]["learned"] = learned_run(mesh, cfg, mk, args.learned_train_steps)
+        line["variants"]["learned"] = learned_run(mesh, cfg, mk, args.learned_train_steps, max(args.warmup, 1))
         if args.mesh == "production":
             for kind in ("reference", "line"):
                 t0 = time.perf_counter()

@@ -78,7 +78,8 @@ struct nft_ctx {
   int line_L = 0;             // uniform column length <= 64 (wavefront scans), else 0
   // precon 3: in-column band of laplc (host rows bm2..bp2, factors interleaved on the device)
   std::vector<double> h_band[5];
-  BandFactors band{};
+  BandFactors<double> band{};
+  BandFactors<float> bandf{};   // FP32 copies (precision = 1)
   int band_L = 0, band_cpb = 0, band_colors = 0;
   int line_R = 0;             // rows per lane of the coarsened scan for 64 < L <= 512, else 0 (Thomas)
   int col_L = 0;              // uniform column length with nodes numbered column by column (0: not)
@@ -588,9 +589,15 @@ int fine_sweep(nft_ctx *c, const double *rhs, double *p, int impfl, bool first) 
     const double *pin = first ? nullptr : p;
     if (P.precon == 3) {
       const int nb = (c->ncol + c->band_cpb - 1) / c->band_cpb;
-      const size_t shm = (size_t)c->band_cpb * (c->band_L + 1) * sizeof(double);
-      hipLaunchKernelGGL(k_line_penta, dim3(nb), dim3(NFT_WAVE), shm, c->stream, c->ncol, c->band_L, c->band_cpb,
-                         c->band, res, pin, p, P.omega, c->skip);
+      if (P.precision == 1) {
+        const size_t shm = (size_t)c->band_cpb * (c->band_L + 1) * sizeof(float);
+        hipLaunchKernelGGL(k_line_penta<float>, dim3(nb), dim3(NFT_WAVE), shm, c->stream, c->ncol, c->band_L,
+                           c->band_cpb, c->bandf, res, pin, p, P.omega, c->skip);
+      } else {
+        const size_t shm = (size_t)c->band_cpb * (c->band_L + 1) * sizeof(double);
+        hipLaunchKernelGGL(k_line_penta<double>, dim3(nb), dim3(NFT_WAVE), shm, c->stream, c->ncol, c->band_L,
+                           c->band_cpb, c->band, res, pin, p, P.omega, c->skip);
+      }
     } else if (c->line_L > 0) {
       int cpw = 64 / c->line_L;
       long waves = (c->ncol + cpw - 1) / cpw;
@@ -1448,6 +1455,7 @@ int build_band(nft_ctx *c, const nft_mesh *m) {
   const size_t ni = (size_t)nb * L * cpb;
   std::vector<double> *src[5] = {&fl1, &fl2, &fu1, &fu2, &fdinv};
   double *dst[5];
+  float *dstf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   for (int q = 0; q < 5; ++q) {
     std::vector<double> il(ni, 0.);
     for (int k = 0; k < nc; ++k) {
@@ -1456,8 +1464,14 @@ int build_band(nft_ctx *c, const nft_mesh *m) {
     }
     int rc = upload(c, &dst[q], il.data(), ni);
     if (rc) return rc;
+    if (c->P.precision == 1) {
+      std::vector<float> ilf(ni);
+      for (size_t t = 0; t < ni; ++t) ilf[t] = (float)il[t];
+      if ((rc = upload(c, &dstf[q], ilf.data(), ni))) return rc;
+    }
   }
-  c->band = BandFactors{dst[0], dst[1], dst[2], dst[3], dst[4]};
+  c->band = BandFactors<double>{dst[0], dst[1], dst[2], dst[3], dst[4]};
+  c->bandf = BandFactors<float>{dstf[0], dstf[1], dstf[2], dstf[3], dstf[4]};
   c->band_L = L; c->band_cpb = cpb; c->band_colors = ncolors;
   return 0;
 }
@@ -2109,7 +2123,7 @@ static double kernel_bytes(nft_ctx *c, int k) {
     case KT_LGRAD: return geomg + 8 * N /*p*/ + 4 * 8 * N /*vol rh0 coefw ue*/ + 3 * 8 * N /*q*/;
     case KT_LDIV: return geomg + 3 * 8 * N /*q*/ + 2 * 8 * N /*vol rh0*/ + 8 * N /*out*/;
     case KT_LINE:
-      if (c->P.precon == 3) return 5 * 8 * N /*factors*/ + 8 * N /*rhs*/ + 8 * N /*pin*/ + 8 * N /*p*/;
+      if (c->P.precon == 3) return 5 * ts * N /*factors*/ + 8 * N /*rhs*/ + 8 * N /*pin*/ + 8 * N /*p*/;
       return c->line_L > 0 ? 3 * ts * N /*rows*/ + 8 * N /*rhs*/ + 8 * N /*p*/
                                        : 4 * (c->ncol + 1) + 3 * ts * N /*factors*/ + 8 * N + 8 * N + 2 * ts * N /*z*/;
     case KT_UPD: return 6 * 8 * N;  /* p,r read+write; x, ax read */

@@ -1116,29 +1116,34 @@ __global__ void k_band_extract(int n, int L, const int32_t *color, int col, int 
 #define NFT_PENTA_CHUNK 16
 #endif
 
+template <typename T>
 struct BandFactors {
-  const double *l1, *l2, *u1, *u2, *dinv;
+  const T *l1, *l2, *u1, *u2, *dinv;
 };
 
-__global__ void __launch_bounds__(NFT_WAVE) k_line_penta(int ncol, int L, int cpb, BandFactors f,
+// T = double (FP64 path, same operation sequence as the oracle) or float (mixed precision:
+// factors, staged residual and substitution in FP32, the update of p in FP64)
+template <typename T>
+__global__ void __launch_bounds__(NFT_WAVE) k_line_penta(int ncol, int L, int cpb, BandFactors<T> f,
                                                          const double *res, const double *pin, double *p,
                                                          double omega, const double *skip) {
   if (skipped(skip)) return;
-  extern __shared__ double lds[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *lds = (T *)lds_raw;
   const int lane = threadIdx.x, cb = blockIdx.x;
   const int c0 = cb * cpb, ncb = min(cpb, ncol - c0);
   const int RL = L + 1;
   const size_t node0 = (size_t)c0 * L;
-  for (int idx = lane; idx < ncb * L; idx += NFT_WAVE) lds[(idx / L) * RL + idx % L] = res[node0 + idx];
+  for (int idx = lane; idx < ncb * L; idx += NFT_WAVE) lds[(idx / L) * RL + idx % L] = (T)res[node0 + idx];
   __syncthreads();
   if (lane < ncb) {
     // the factors of NFT_PENTA_CHUNK levels are loaded together ahead of their steps, so
     // the substitution chain waits on memory once per chunk, not once per level
-    double *z = lds + lane * RL;
+    T *z = lds + lane * RL;
     const size_t fb = (size_t)cb * L * cpb + lane;
-    double y1 = 0., y2 = 0.;
+    T y1 = 0, y2 = 0;
     for (int k0 = 0; k0 < L; k0 += NFT_PENTA_CHUNK) {
-      double a1[NFT_PENTA_CHUNK], a2[NFT_PENTA_CHUNK];
+      T a1[NFT_PENTA_CHUNK], a2[NFT_PENTA_CHUNK];
 #pragma unroll
       for (int j = 0; j < NFT_PENTA_CHUNK; ++j) {
         const int k = min(k0 + j, L - 1);
@@ -1149,7 +1154,7 @@ __global__ void __launch_bounds__(NFT_WAVE) k_line_penta(int ncol, int L, int cp
       for (int j = 0; j < NFT_PENTA_CHUNK; ++j) {
         const int k = k0 + j;
         if (k < L) {
-          double y = z[k];
+          T y = z[k];
           if (k >= 1) y = y - a1[j] * y1;
           if (k >= 2) y = y - a2[j] * y2;
           z[k] = y;
@@ -1157,9 +1162,9 @@ __global__ void __launch_bounds__(NFT_WAVE) k_line_penta(int ncol, int L, int cp
         }
       }
     }
-    double z1 = 0., z2 = 0.;
+    T z1 = 0, z2 = 0;
     for (int k0 = L - 1; k0 >= 0; k0 -= NFT_PENTA_CHUNK) {
-      double b1[NFT_PENTA_CHUNK], b2[NFT_PENTA_CHUNK], bd[NFT_PENTA_CHUNK];
+      T b1[NFT_PENTA_CHUNK], b2[NFT_PENTA_CHUNK], bd[NFT_PENTA_CHUNK];
 #pragma unroll
       for (int j = 0; j < NFT_PENTA_CHUNK; ++j) {
         const int k = max(k0 - j, 0);
@@ -1171,7 +1176,7 @@ __global__ void __launch_bounds__(NFT_WAVE) k_line_penta(int ncol, int L, int cp
       for (int j = 0; j < NFT_PENTA_CHUNK; ++j) {
         const int k = k0 - j;
         if (k >= 0) {
-          double t = z[k];
+          T t = z[k];
           if (k + 1 < L) t = t - b1[j] * z1;
           if (k + 2 < L) t = t - b2[j] * z2;
           t = t * bd[j];
@@ -1184,7 +1189,7 @@ __global__ void __launch_bounds__(NFT_WAVE) k_line_penta(int ncol, int L, int cp
   __syncthreads();
   for (int idx = lane; idx < ncb * L; idx += NFT_WAVE) {
     const double p0 = pin ? pin[node0 + idx] : 0.;
-    p[node0 + idx] = p0 + omega * lds[(idx / L) * RL + idx % L];
+    p[node0 + idx] = p0 + omega * (double)lds[(idx / L) * RL + idx % L];
   }
 }
 

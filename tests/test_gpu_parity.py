@@ -263,11 +263,13 @@ def test_steps_small_100_reference_parity_mode():
     s.close()
 
 
-def test_mixed_precision_close_to_fp64():
-    """FP32 MPDATA and line preconditioner, FP64 GCR: per-field deviation from the FP64
-    form after 10 steps below 1e-5 relative L2, and the solves still converge."""
+@pytest.mark.parametrize("precon", [2, 3])
+def test_mixed_precision_close_to_fp64(precon):
+    """FP32 MPDATA and line preconditioner (either line), FP64 GCR: per-field deviation
+    from the FP64 form after 10 steps below 1e-5 relative L2, and the solves still
+    converge."""
     m = mesh_for("small")
-    cfg = small_cfg(precon=2, itrp1=1, omega=1.0, exit_mode=1, eps0=1e-8, itr=200)
+    cfg = small_cfg(precon=precon, itrp1=1, omega=1.0, exit_mode=1, eps0=1e-8, itr=200)
     a = solver(m, cfg)
     b = solver(m, cfg, precision="mixed")
     for s in (a, b):
