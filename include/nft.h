@@ -166,7 +166,8 @@ typedef struct {
                       d2/d3 diagonal estimates diverges for omega >= 0.7) */
 } nft_multigrid;
 /* Coarse levels and V-cycle settings. With m1 == NULL only mg_flag and the sweep counts
- * change (the levels of an earlier call stay). */
+ * change (the levels of an earlier call stay). A call with meshes allocates the levels'
+ * device storage anew; storage of earlier levels is released by nft_finalize. */
 int nft_set_multigrid(nft_ctx *ctx, const nft_mesh *m1, const int32_t *match1,
                       const nft_mesh *m2, const int32_t *match2, const nft_multigrid *mg);
 /* per-level set-up values: "d" (d2 / d3), "beta" (1 value), "rh0", "pstep"; *count = length */

@@ -60,8 +60,9 @@ def test_pieces_bitwise(key, transfer):
 
 @pytest.mark.parametrize("key", ["tiny", "small"])
 @pytest.mark.parametrize("kw,transfer,sweeps", [(PARITY, 0, MG_SWEEPS), (PARITY, 1, HEAVY),
-                                                (LINE_MG, 1, HEAVY_C), (LINE_MG, 0, MG_SWEEPS)],
-                         ids=["point-ref", "point-nbr", "line-nbr-omega_c", "line-ref"])
+                                                (LINE_MG, 1, HEAVY_C), (LINE_MG, 0, MG_SWEEPS),
+                                                (dict(LINE_MG, precon=3), 1, HEAVY_C)],
+                         ids=["point-ref", "point-nbr", "line-nbr-omega_c", "line-ref", "band-nbr-omega_c"])
 def test_precon_bitwise(key, kw, transfer, sweeps):
     print(f"case {key} {kw} transfer={transfer}", flush=True)
     m, levels, s, o = pair(key, kw, transfer, sweeps)
