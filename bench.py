@@ -62,15 +62,25 @@ WORKLOADS = {
 }
 
 
-def timed_run(mesh, cfg, precision, steps, warmup, profile=True):
+PRECISION_TEXT = {
+    "mixed": "FP32 MPDATA (fluxes, antidiffusive velocities, limiter, state increments) and FP32 line "
+             "preconditioner; FP64 laplc, GCR dots, orthogonalisation, residual and pressure updates",
+    "mixed_op": "as mixed, and laplc inside the GCR iterations with FP32 edge metrics and FP32 gradient "
+                "temporaries (FP64 sums); every solve ends on the FP64 residual of its pressure",
+}
+
+
+def timed_run(mesh, cfg, precision, steps, warmup, profile=True, pre_steps=0):
+    """`warmup` untimed steps (plus `pre_steps` more, so that a run without the per-kernel
+    profiling pass ends at the same step as one with it), then `steps` timed steps."""
     from src_amd import nft
     s = nft.Solver(mesh, nft.make_params(cfg, precision=precision))
     t0 = time.perf_counter()
     s.state_init()
     init = s.gcr_stats()
     log(f"[{precision}] initial projection: {init.niter} GCR iterations, {time.perf_counter() - t0:.2f} s")
-    if warmup:
-        s.run_steps(warmup)
+    if warmup + pre_steps:
+        s.run_steps(warmup + pre_steps)
     dominant = None
     if profile:
         # one step per kernel id: pick the kernel with the largest total device time
@@ -245,6 +255,9 @@ def main():
                     help="workload: the production mesh (headline metric) or the 16.8M-node stress mesh")
     ap.add_argument("--column-order", default="grid", choices=("grid", "hilbert"),
                     help="numbering of the production mesh's columns")
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed", "mixed_op"),
+                    help="precision of the main line (the headline metric is fp64; the others are for "
+                         "profiling the variants)")
     ap.add_argument("--no-roofline", action="store_true",
                     help="skip the per-kernel timing pass (for runs under a counter profiler)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -270,7 +283,7 @@ def main():
         log(f"copy calibration: {mesh.npoin * 8} bytes read + written per copy, {ms * 1e3:.1f} us, "
             f"{2 * mesh.npoin * 8 / (ms * 1e-3) / 1e9:.0f} GB/s")
         s0.close()
-    r = timed_run(mesh, cfg, "fp64", args.steps, args.warmup, profile=not args.no_roofline)
+    r = timed_run(mesh, cfg, args.precision, args.steps, args.warmup, profile=not args.no_roofline)
     st = r["st"]
     K = args.steps
     ms_step = st.ms_total / max(K, 1)
@@ -279,12 +292,13 @@ def main():
     kname, nl, kms, kbytes = r.get("kernel", (None, 0, 0.0, 0.0))
     avg_ms = kms / max(nl, 1)
     achieved = kbytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    unk = r["solver"].state("unk")
+    unk = r["solver"].state("unk").copy()
     import numpy as np
     finite = bool(np.isfinite(unk).all())
+    profile_steps = 0 if args.no_roofline else len(nft.KERNELS)
     cp = stream_copy_gbps(r["solver"]) if not args.no_roofline else 0.0
     r["solver"].close()
-    log(f"fp64: {value:.2f} steps/s, {ms_step:.1f} ms/step, {iters:.1f} GCR iterations/solve, "
+    log(f"{args.precision}: {value:.2f} steps/s, {ms_step:.1f} ms/step, {iters:.1f} GCR iterations/solve, "
         f"{st.ms_gcr / max(st.gcr_solves, 1):.2f} ms/solve, unconverged {st.gcr_unconverged}")
 
     line = {
@@ -298,7 +312,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f64+f32",
         "data": "synthetic",
         "config": {
             "workload": WORKLOADS[args.mesh].format(nodes=mesh.npoin, edges=mesh.nedge + mesh.nbface),
@@ -307,7 +321,7 @@ def main():
             "preconditioner": ("vertical line: the in-column pentadiagonal band of laplc (probed on the GPU), "
                                "banded LU per column" if cfg.precon == 3 else
                                "vertical line (two-point-flux tridiagonal, wavefront scans of the Thomas factors)"),
-            "precision": "fp64",
+            "precision": args.precision if args.precision == "fp64" else PRECISION_TEXT[args.precision],
         },
         "gcr": {"iters_per_solve": round(iters, 2), "solve_ms": round(st.ms_gcr / max(st.gcr_solves, 1), 3),
                 "solves": st.gcr_solves, "unconverged": st.gcr_unconverged},
@@ -323,18 +337,31 @@ def main():
         line["roofline"]["stream_copy_GBps"] = round(cp, 1)
         line["roofline"]["frac_of_stream_copy"] = round(achieved / cp, 4) if cp > 0 else None
         log(f"2 GiB device copy: {cp:.0f} GB/s; {kname} algorithmic rate = {achieved / cp:.2f} of it")
-    if not args.no_variants:
+    if not args.no_variants and args.precision == "fp64":
         # the variants run the same warm-up and step count as the FP64 line, so their
         # iteration counts cover the same part of the run
         mk = K
-        rm = timed_run(mesh, cfg, "mixed", mk, args.warmup, profile=False)
-        sm = rm["st"]
-        rm["solver"].close()
-        line["variants"] = {"mixed": {
-            "value": round(1000.0 / (sm.ms_total / mk), 4), "unit": "timesteps/s",
-            "iters_per_solve": round(sm.gcr_iters / max(sm.gcr_solves, 1), 2),
-            "solve_ms": round(sm.ms_gcr / max(sm.gcr_solves, 1), 3),
-            "mpdata_ms_per_step": round(sm.ms_mpdata / mk, 3), "steps": mk}}
+        line["variants"] = {}
+        for prec in ("mixed", "mixed_op"):
+            rm = timed_run(mesh, cfg, prec, mk, args.warmup, profile=False, pre_steps=profile_steps)
+            sm = rm["st"]
+            v = {"value": round(1000.0 / (sm.ms_total / mk), 4), "unit": "timesteps/s",
+                 "iters_per_solve": round(sm.gcr_iters / max(sm.gcr_solves, 1), 2),
+                 "solve_ms": round(sm.ms_gcr / max(sm.gcr_solves, 1), 3),
+                 "mpdata_ms_per_step": round(sm.ms_mpdata / mk, 3), "steps": mk,
+                 "unconverged": sm.gcr_unconverged,
+                 "precision": PRECISION_TEXT[prec]}
+            # per-field deviation from the FP64 run after the same number of steps
+            um = rm["solver"].state("unk")
+            v["field_error_vs_fp64"] = {
+                f"step_{args.warmup + profile_steps + mk}": {
+                    name: {"rel_l2": float(np.linalg.norm(um[k] - unk[k]) / np.linalg.norm(unk[k])),
+                           "rel_linf": float(np.abs(um[k] - unk[k]).max() / np.abs(unk[k]).max())}
+                    for k, name in enumerate(("u", "v", "w", "theta"))}}
+            rm["solver"].close()
+            line["variants"][prec] = v
+            log(f"[{prec}] {v['value']} steps/s, {v['iters_per_solve']} iterations/solve, "
+                f"{v['solve_ms']} ms/solve, field error {v['field_error_vs_fp64']}")
         line["variants"]["learned"] = learned_run(mesh, cfg, mk, args.learned_train_steps, max(args.warmup, 1))
         if args.mesh == "production":
             for kind in ("reference", "line"):

@@ -79,7 +79,7 @@ struct nft_ctx {
   // precon 3: in-column band of laplc (host rows bm2..bp2, factors interleaved on the device)
   std::vector<double> h_band[5];
   BandFactors<double> band{};
-  BandFactors<float> bandf{};   // FP32 copies (precision = 1)
+  BandFactors<float> bandf{};   // FP32 copies (precision >= 1)
   int band_L = 0, band_cpb = 0, band_colors = 0;
   int line_R = 0;             // rows per lane of the coarsened scan for 64 < L <= 512, else 0 (Thomas)
   int col_L = 0;              // uniform column length with nodes numbered column by column (0: not)
@@ -106,6 +106,8 @@ struct nft_ctx {
   // work
   double *t[8];          // prfr/div/laplc temporaries
   double2 *txy;          // laplc: (x, y) of the scaled gradient (z in t[2])
+  float2 *txyf = nullptr;   // the FP32 operator's (x, y) and z (precision 2)
+  float *t2f = nullptr;
   double *gx[NFT_MAXD], *gax[NFT_MAXD];
   double *r, *qr, *ar, *pw, *rw;
   double *mp_psi1[4];
@@ -444,23 +446,23 @@ DivArgs div_args(nft_ctx *c) {
   return a;
 }
 
-template <int MODE, int ND, int NMAX>
+template <int MODE, int ND, int NMAX, typename TM = double>
 void div_go(nft_ctx *c, const DivArgs &a) {
   if (c->col_on)
-    hipLaunchKernelGGL((k_div<MODE, ND, NMAX, true>), dim3(c->gather_grid), dim3(NFT_BLOCK), 0, c->stream, c->inc,
-                       c->S, a);
+    hipLaunchKernelGGL((k_div<MODE, ND, NMAX, true, TM>), dim3(c->gather_grid), dim3(NFT_BLOCK), 0, c->stream,
+                       c->inc, c->S, a);
   else
-    hipLaunchKernelGGL((k_div<MODE, ND, NMAX, false>), dim3(c->gather_grid), dim3(NFT_BLOCK), 0, c->stream, c->inc,
-                       c->S, a);
+    hipLaunchKernelGGL((k_div<MODE, ND, NMAX, false, TM>), dim3(c->gather_grid), dim3(NFT_BLOCK), 0, c->stream,
+                       c->inc, c->S, a);
 }
 
-template <int ND>
+template <int ND, typename TM>
 bool div_laplc_nd(nft_ctx *c, const DivArgs &a, int nd) {
   if constexpr (ND > NFT_MAXD) {
     return false;
   } else {
-    if (nd == ND) { div_go<DIV_LAPLC, ND, 0>(c, a); return true; }
-    return div_laplc_nd<ND + 1>(c, a, nd);
+    if (nd == ND) { div_go<DIV_LAPLC, ND, 0, TM>(c, a); return true; }
+    return div_laplc_nd<ND + 1, TM>(c, a, nd);
   }
 }
 
@@ -486,12 +488,13 @@ int final_launch(nft_ctx *c, int nblk, const FinalOp &op0) {
   return 0;
 }
 
-int div_launch(nft_ctx *c, int mode, const DivArgs &a) {
+// f32: the FP32 operator's divergence (FP32 metrics and inputs, DIV_LAPLC only)
+int div_launch(nft_ctx *c, int mode, const DivArgs &a, bool f32 = false) {
   const int nd = a.dots.nd, nmax = a.dots.nmax ? 1 : 0;
   if (mode == DIV_LAPLC) {
     if (nmax) return set_err(-1, "laplc: max reduction not supported");
     t_begin(c, KT_LDIV);
-    bool ok = div_laplc_nd<0>(c, a, nd);
+    bool ok = f32 ? div_laplc_nd<0, float>(c, a, nd) : div_laplc_nd<0, double>(c, a, nd);
     t_end(c, KT_LDIV);
     if (!ok) return set_err(-1, "laplc: too many fused dots");
   } else {
@@ -519,6 +522,31 @@ int laplc_dev(nft_ctx *c, const double *press, double *alap, int impfl, const Do
   d.qxy = c->txy; d.q2 = c->t[2]; d.out = alap;
   if (dots) d.dots = *dots;
   return div_launch(c, DIV_LAPLC, d);
+}
+
+// laplc with FP32 edge metrics and FP32 gradient temporaries, FP64 sums and output: the
+// operator of the Krylov iterations with precision 2
+int laplc32_dev(nft_ctx *c, const double *press, double *alap, int impfl, const DotList *dots) {
+  if (!c->txyf || !c->inc.sxyf) return set_err(-1, "laplc32: FP32 operator not set up (precision 2 only)");
+  GradArgs g = grad_args(c);
+  g.f = press; g.oxyf = c->txyf; g.o2f = c->t2f; g.impfl = impfl;
+  dim3 gg(c->gather_grid), b(NFT_BLOCK);
+  t_begin(c, KT_LGRAD);
+  if (c->col_on)
+    hipLaunchKernelGGL((k_grad<GRAD_LAPLC, true, float>), gg, b, 0, c->stream, c->inc, c->S, g);
+  else
+    hipLaunchKernelGGL((k_grad<GRAD_LAPLC, false, float>), gg, b, 0, c->stream, c->inc, c->S, g);
+  t_end(c, KT_LGRAD);
+  HIPCHK(hipGetLastError());
+  DivArgs d = div_args(c);
+  d.qxyf = c->txyf; d.q2f = c->t2f; d.out = alap;
+  if (dots) d.dots = *dots;
+  return div_launch(c, DIV_LAPLC, d, true);
+}
+
+// the matrix-vector product of the GCR iterations
+int krylov_laplc(nft_ctx *c, const double *press, double *alap, int impfl, const DotList *dots) {
+  return c->P.precision == 2 ? laplc32_dev(c, press, alap, impfl, dots) : laplc_dev(c, press, alap, impfl, dots);
 }
 
 // out = L^T y (transpose of laplc_dev, two gathers; uses t[0..2])
@@ -589,7 +617,7 @@ int fine_sweep(nft_ctx *c, const double *rhs, double *p, int impfl, bool first) 
     const double *pin = first ? nullptr : p;
     if (P.precon == 3) {
       const int nb = (c->ncol + c->band_cpb - 1) / c->band_cpb;
-      if (P.precision == 1) {
+      if (P.precision >= 1) {
         const size_t shm = (size_t)c->band_cpb * (c->band_L + 1) * sizeof(float);
         hipLaunchKernelGGL(k_line_penta<float>, dim3(nb), dim3(NFT_WAVE), shm, c->stream, c->ncol, c->band_L,
                            c->band_cpb, c->bandf, res, pin, p, P.omega, c->skip);
@@ -602,7 +630,7 @@ int fine_sweep(nft_ctx *c, const double *rhs, double *p, int impfl, bool first) 
       int cpw = 64 / c->line_L;
       long waves = (c->ncol + cpw - 1) / cpw;
       int cg = (int)((waves + NFT_WPB - 1) / NFT_WPB);
-      if (P.precision == 1)
+      if (P.precision >= 1)
         hipLaunchKernelGGL(k_line_scan<float>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->line_L,
                            c->laf, c->linvf, c->lcf, res, pin, p, P.omega, c->skip);
       else
@@ -610,11 +638,11 @@ int fine_sweep(nft_ctx *c, const double *rhs, double *p, int impfl, bool first) 
                            c->la, c->linv, c->lc, res, pin, p, P.omega, c->skip);
     } else if (c->line_R > 0) {
       int cg = (int)((c->ncol + NFT_WPB - 1) / NFT_WPB);
-      if (P.precision == 1) line_scan_r_go<float>(c, cg, c->laf, c->linvf, c->lcf, res, pin, p);
+      if (P.precision >= 1) line_scan_r_go<float>(c, cg, c->laf, c->linvf, c->lcf, res, pin, p);
       else line_scan_r_go<double>(c, cg, c->la, c->linv, c->lc, res, pin, p);
     } else {
       int cg = blocks_for(c->ncol);
-      if (P.precision == 1)
+      if (P.precision >= 1)
         hipLaunchKernelGGL(k_line<float>, dim3(cg), dim3(NFT_BLOCK), 0, c->stream, c->ncol, c->col_start,
                            c->laf, c->linvf, c->lcf, res, pin, p, P.omega, c->zbuff, c->skip);
       else
@@ -942,7 +970,7 @@ static int gcr_iteration_rest(nft_ctx *c, int i, int impfl, double epa) {
     dl.nd = i;
     for (int ii = 0; ii < i; ++ii) dl.a[ii] = c->gax[ii];
     dl.fin = FIN_DEL;
-    if ((rc = laplc_dev(c, c->qr, c->ar, impfl, &dl))) return rc;
+    if ((rc = krylov_laplc(c, c->qr, c->ar, impfl, &dl))) return rc;
   }
   DirArgs d{};
   d.n = c->n; d.nx = i;
@@ -964,13 +992,12 @@ static int gcr_iteration_rest(nft_ctx *c, int i, int impfl, double epa) {
 // host round trip; the host reads the mapped exit flag one iteration behind (the GPU
 // always has the next iteration queued) and stops enqueueing once it is set. Iterations
 // enqueued past the exit return at once. Same arithmetic and exit rule as the host loop.
-static int gcr_loop_device(nft_ctx *c, double *p, int impfl, double eps, double rl20, const double *x0, int *nlc,
+static int gcr_loop_device(nft_ctx *c, double *p, int impfl, double eps, int itmn, double rl20, int *nlc,
                            int *converged, double *dvmx) {
   const nft_params &P = c->P;
-  const int lord = P.lord, itmn = 3;
+  const int lord = P.lord;
   const double epa = 1.e-38;
   int rc = 0;
-  (void)x0;
   *c->h_flag = 0.0;
   hipLaunchKernelGGL(k_gcr_ctl_begin, dim3(1), dim3(64), 0, c->stream, c->scal, rl20);
   HIPCHK(hipGetLastError());
@@ -1022,49 +1049,14 @@ static int gcr_loop_device(nft_ctx *c, double *p, int impfl, double eps, double 
   return 0;
 }
 
-// gcrk (reference :1776-1972) on device arrays. Returns 0 when converged, 1 otherwise.
-int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const double *w1, double *u2,
-             double *v2, double *w2, int impfl) {
+// The host-controlled inner loop (learned correction, pair dumps): the exit test after
+// every update is taken on the host, which may drop the learned correction.
+static int gcr_loop_host(nft_ctx *c, double *p, int impfl, double eps, int itmn, double rl20, int *nlc,
+                         int *converged, double *dvmx, const double **qr_last) {
   const nft_params &P = c->P;
-  const int lord = P.lord, itmn = 3;
+  const int lord = P.lord;
   const double epa = 1.e-38;
   int rc;
-  double eps = P.eps0 / P.dt;
-  if (P.exit_mode == 1) {
-    // b = residual of the zero initial guess
-    if ((rc = prfr_dev(c, nullptr, u1, v1, w1, c->t[4], c->t[5], c->t[6], impfl))) return rc;
-    if ((rc = div_dev(c, c->t[4], c->t[5], c->t[6], c->t[7], -1, nullptr))) return rc;
-    if ((rc = norms_dev(c, c->t[7], SC_TMP, SC_BMX))) return rc;
-  }
-  if ((rc = prfr_dev(c, p, u1, v1, w1, u2, v2, w2, impfl))) return rc;
-  {
-    DotList dl{};
-    dl.nd = 1; dl.a[0] = nullptr; dl.nmax = 1; dl.fin = FIN_NORMS;
-    if ((rc = div_dev(c, u2, v2, w2, c->r, -1, &dl))) return rc;
-  }
-  double *x0 = c->gx[0], *ax0 = c->gax[0];
-  c->net.active = c->net.kind == 1 && c->net.ready;
-  if ((rc = precon_dev(c, c->r, x0, impfl))) return rc;
-  if (c->net.active && (rc = net_apply_precon(c, c->r, x0, SC_RMX))) return rc;
-  if ((rc = norms_dev(c, x0, SC_EER, SC_EEM))) return rc;
-  {
-    DotList dl{};
-    dl.nd = 2; dl.a[0] = c->r; dl.a[1] = nullptr; dl.fin = FIN_BETA; dl.slot = 0;
-    if ((rc = laplc_dev(c, x0, ax0, impfl, &dl))) return rc;
-  }
-  if ((rc = read_scal(c, 0, SC_TOTAL))) return rc;
-  if (P.exit_mode == 1) eps = P.eps0 * c->h_scal[SC_BMX];
-  double eer0 = std::max(c->h_scal[SC_EER], epa);
-  double eem0 = std::max(c->h_scal[SC_EEM], epa);
-  double rl20 = std::max(c->h_scal[SC_R2], epa);
-  const double *qr_last = x0;
-  int nlc = 0, converged = 0;
-  double dvmx = 0;
-  if (!c->net.active && !c->net.dumping) {
-    if ((rc = gcr_loop_device(c, p, impfl, eps, rl20, x0, &nlc, &converged, &dvmx))) return rc;
-    if (nlc > 0) qr_last = c->qr;
-    goto done;
-  }
   for (int it = 1; it <= P.itr; ++it) {
     for (int i = 1; i <= lord; ++i) {
       t_begin(c, KT_UPD);
@@ -1078,14 +1070,14 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
       }
       HIPCHK(hipGetLastError());
       if ((rc = read_scal(c, SC_DVMX, 2))) return rc;
-      dvmx = c->h_scal[SC_DVMX];
+      *dvmx = c->h_scal[SC_DVMX];
       double rl2 = c->h_scal[SC_RL2];
-      if (std::isnan(rl2)) goto done;   // NaN residual: stop, reported as not converged
-      if (dvmx <= eps && it >= itmn) { converged = 1; goto done; }
+      if (std::isnan(rl2)) return 0;   // NaN residual: stop, reported as not converged
+      if (*dvmx <= eps && it >= itmn) { *converged = 1; return 0; }
       if (rl2 >= rl20) {
         // stagnation: with the learned correction active, drop it and continue with the
         // line preconditioner alone instead of ending the solve
-        if (!c->net.active) goto done;
+        if (!c->net.active) return 0;
         c->net.active = 0;
         c->net.fallbacks++;
       } else if (c->net.active && rl2 > c->net.stall * rl20) {
@@ -1094,20 +1086,20 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
       }
       rl20 = std::max(rl2, epa);
       if ((rc = precon_dev(c, c->r, c->qr, impfl))) return rc;
-      if (c->net.dumping && (rc = net_store_pair(c, dvmx, rl2, impfl))) return rc;
+      if (c->net.dumping && (rc = net_store_pair(c, *dvmx, rl2, impfl))) return rc;
       if (c->net.active) {
         if ((rc = net_apply_precon(c, c->r, c->qr, SC_DVMX))) return rc;
         c->net.learned_iters++;
       }
-      qr_last = c->qr;
+      *qr_last = c->qr;
       {
         DotList dl{};
         dl.nd = i;
         for (int ii = 0; ii < i; ++ii) dl.a[ii] = c->gax[ii];
         dl.fin = FIN_DEL;
-        if ((rc = laplc_dev(c, c->qr, c->ar, impfl, &dl))) return rc;
+        if ((rc = krylov_laplc(c, c->qr, c->ar, impfl, &dl))) return rc;
       }
-      nlc = nlc + 1;
+      *nlc = *nlc + 1;
       {
         DirArgs d{};
         d.n = c->n; d.nx = i;
@@ -1126,7 +1118,89 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
       }
     }
   }
-done:
+  return 0;
+}
+
+// gcrk (reference :1776-1972) on device arrays. Returns 0 when converged, 1 otherwise.
+//
+// With the FP32 operator (precision 2) the iterations apply laplc with FP32 metrics and
+// temporaries while the residual, its updates, the dots and the orthogonalisation stay
+// FP64. The recursive residual then drifts from the true one by the operator's rounding,
+// so a solve runs in passes: each pass starts from the true FP64 residual of the current
+// pressure (prfr + div, as the reference's start) and iterates until the recursive
+// residual has dropped by NFT_OP32_DROP or passed the tolerance; the next pass checks the
+// tolerance on the true residual and stops there, or restarts GCR from it.
+#ifndef NFT_OP32_DROP
+#define NFT_OP32_DROP 1.0e-3
+#endif
+#ifndef NFT_OP32_PASSES
+#define NFT_OP32_PASSES 24
+#endif
+int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const double *w1, double *u2,
+             double *v2, double *w2, int impfl) {
+  const nft_params &P = c->P;
+  const double epa = 1.e-38;
+  const bool op32 = P.precision == 2;
+  int rc;
+  double eps = P.eps0 / P.dt;
+  if (P.exit_mode == 1) {
+    // b = residual of the zero initial guess
+    if ((rc = prfr_dev(c, nullptr, u1, v1, w1, c->t[4], c->t[5], c->t[6], impfl))) return rc;
+    if ((rc = div_dev(c, c->t[4], c->t[5], c->t[6], c->t[7], -1, nullptr))) return rc;
+    if ((rc = norms_dev(c, c->t[7], SC_TMP, SC_BMX))) return rc;
+  }
+  double *x0 = c->gx[0], *ax0 = c->gax[0];
+  const double *qr_last = x0;
+  int nlc = 0, converged = 0;
+  double dvmx = 0, eer0 = epa, eem0 = epa;
+  c->net.active = c->net.kind == 1 && c->net.ready;
+  for (int pass = 0;; ++pass) {
+    // u2, v2, w2 are prfr of the incoming pressure (outputs of gcrk); later passes use
+    // scratch arrays for theirs
+    double *pu = pass ? c->t[4] : u2, *pv = pass ? c->t[5] : v2, *pw = pass ? c->t[6] : w2;
+    if ((rc = prfr_dev(c, p, u1, v1, w1, pu, pv, pw, impfl))) return rc;
+    {
+      DotList dl{};
+      dl.nd = 1; dl.a[0] = nullptr; dl.nmax = 1; dl.fin = FIN_NORMS;
+      if ((rc = div_dev(c, pu, pv, pw, c->r, -1, &dl))) return rc;
+    }
+    if (pass > 0) {
+      // the true residual of the pressure the previous pass reached
+      if ((rc = read_scal(c, SC_R2, 2))) return rc;
+      dvmx = c->h_scal[SC_RMX];
+      if (dvmx <= eps) { converged = 1; break; }
+      if (!(dvmx == dvmx) || pass >= NFT_OP32_PASSES) break;
+    }
+    if ((rc = precon_dev(c, c->r, x0, impfl))) return rc;
+    if (c->net.active && (rc = net_apply_precon(c, c->r, x0, SC_RMX))) return rc;
+    if ((rc = norms_dev(c, x0, SC_EER, SC_EEM))) return rc;
+    {
+      DotList dl{};
+      dl.nd = 2; dl.a[0] = c->r; dl.a[1] = nullptr; dl.fin = FIN_BETA; dl.slot = 0;
+      if ((rc = krylov_laplc(c, x0, ax0, impfl, &dl))) return rc;
+    }
+    if ((rc = read_scal(c, 0, SC_TOTAL))) return rc;
+    if (pass == 0) {
+      if (P.exit_mode == 1) eps = P.eps0 * c->h_scal[SC_BMX];
+      eer0 = std::max(c->h_scal[SC_EER], epa);
+      eem0 = std::max(c->h_scal[SC_EEM], epa);
+    }
+    const double rl20 = std::max(c->h_scal[SC_R2], epa);
+    // a pass of the FP32 operator ends early, at a fixed drop of the recursive residual
+    const double eps_pass = op32 ? std::max(eps, NFT_OP32_DROP * c->h_scal[SC_RMX]) : eps;
+    const int itmn = pass == 0 ? 3 : 1;
+    int nlc_pass = 0, conv_pass = 0;
+    qr_last = x0;
+    if (!c->net.active && !c->net.dumping) {
+      if ((rc = gcr_loop_device(c, p, impfl, eps_pass, itmn, rl20, &nlc_pass, &conv_pass, &dvmx))) return rc;
+      if (nlc_pass > 0) qr_last = c->qr;
+    } else {
+      if ((rc = gcr_loop_host(c, p, impfl, eps_pass, itmn, rl20, &nlc_pass, &conv_pass, &dvmx, &qr_last))) return rc;
+    }
+    nlc += nlc_pass;
+    if (!op32) { converged = conv_pass; break; }
+    if (!conv_pass) break;   // stagnation, NaN or the iteration cap: not converged
+  }
   if ((rc = norms_dev(c, qr_last, SC_EER, SC_EEM))) return rc;
   if ((rc = read_scal(c, SC_EER, 2))) return rc;
   c->st.eer = c->h_scal[SC_EER] / eer0;
@@ -1192,7 +1266,7 @@ StepArgs step_args(nft_ctx *c) {
   a.rh0 = c->rh0; a.dthedz = c->dthedz;
   a.dt = c->P.dt; a.g = kG;
   a.itraj = c->P.itraj;
-  for (int k = 0; k < 3; ++k) a.uuuf[k] = c->P.precision == 1 ? c->uuuf[k] : nullptr;
+  for (int k = 0; k < 3; ++k) a.uuuf[k] = c->P.precision >= 1 ? c->uuuf[k] : nullptr;
   return a;
 }
 
@@ -1239,7 +1313,7 @@ int mpdata_step(nft_ctx *c) {
   int fl[4] = {1, 2, 3, 4};
   int rc = set_out_fields(c, c->unk, 4);
   if (rc) return rc;
-  if (c->P.precision == 1) return mpdata_dev<float, 4>(c, c->uuuf[0], c->uuuf[1], c->uuuf[2], c->unk, fl);
+  if (c->P.precision >= 1) return mpdata_dev<float, 4>(c, c->uuuf[0], c->uuuf[1], c->uuuf[2], c->unk, fl);
   return mpdata_dev<double, 4>(c, c->uuu[0], c->uuu[1], c->uuu[2], c->unk, fl);
 }
 
@@ -1464,7 +1538,7 @@ int build_band(nft_ctx *c, const nft_mesh *m) {
     }
     int rc = upload(c, &dst[q], il.data(), ni);
     if (rc) return rc;
-    if (c->P.precision == 1) {
+    if (c->P.precision >= 1) {
       std::vector<float> ilf(ni);
       for (size_t t = 0; t < ni; ++t) ilf[t] = (float)il[t];
       if ((rc = upload(c, &dstf[q], ilf.data(), ni))) return rc;
@@ -1700,6 +1774,7 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   if (P->precon < 0 || P->precon > 3) return set_err(-1, "nft_init: unknown preconditioner");
   if (!(P->dt > 0)) return set_err(-1, "nft_init: dt must be positive");
   if (P->itraj != 0 && P->itraj != 1) return set_err(-1, "nft_init: itraj must be 0 or 1");
+  if (P->precision < 0 || P->precision > 2) return set_err(-1, "nft_init: precision must be 0, 1 or 2");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(-3, "nft_init: no HIP device");
   const int n = m->npoin;
@@ -1762,7 +1837,7 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
       std::all_of(c->h_coefw.begin(), c->h_coefw.end(), [&](double v) { return v == c->nc.coefw0; }) ? 1 : 0;
   if ((rc = dalloc(c, &c->zbuf, n))) return fail(rc);
   // FP32 copies for the mixed-precision form
-  if (P->precision == 1) {
+  if (P->precision >= 1) {
     auto tof = [](const std::vector<double> &v) { std::vector<float> f(v.size()); for (size_t i = 0; i < v.size(); ++i) f[i] = (float)v[i]; return f; };
     std::vector<float> f;
     float *fx, *fy, *fz;
@@ -1776,6 +1851,17 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
       return fail(rc);
     for (int k = 0; k < 3; ++k)
       if ((rc = dalloc(c, &c->uuuf[k], n))) return fail(rc);
+    if (P->precision == 2) {
+      // the FP32 operator: (Sx, Sy) pairs rounded to FP32; Sz is the FP32 copy above
+      std::vector<float2> sxyf(etot);
+      for (long e = 0; e < etot; ++e) sxyf[e] = make_float2((float)sx[e], (float)sy[e]);
+      float2 *d_sxyf;
+      if ((rc = upload(c, &d_sxyf, sxyf.data(), etot)) || (rc = dalloc(c, &c->txyf, n)) ||
+          (rc = dalloc(c, &c->t2f, n)))
+        return fail(rc);
+      c->inc.sxyf = d_sxyf;
+      c->inc.szzf = fz;
+    }
   }
 
   // ---- state and work arrays (allocated once)
@@ -1793,7 +1879,7 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   if ((rc = dalloc(c, &c->r, n)) || (rc = dalloc(c, &c->qr, n)) || (rc = dalloc(c, &c->ar, n)) ||
       (rc = dalloc(c, &c->pw, n)) || (rc = dalloc(c, &c->rw, n)))
     return fail(rc);
-  const size_t tsz = P->precision == 1 ? sizeof(float) : sizeof(double);
+  const size_t tsz = P->precision >= 1 ? sizeof(float) : sizeof(double);
   for (int k = 0; k < 4; ++k)
     if ((rc = dalloc(c, &c->mp_psi1[k], n))) return fail(rc);
   for (int k = 0; k < 12; ++k) {
@@ -1980,7 +2066,7 @@ int nft_mpdatam(nft_ctx *c, const double *uuu, int ld, double *unk, int iflag) {
   double *fields[1] = {c->io[3]};
   int fl[1] = {iflag};
   if ((rc = set_out_fields(c, fields, 1))) return rc;
-  if (c->P.precision == 1) {
+  if (c->P.precision >= 1) {
     float *uf[3] = {(float *)c->mp_g[9], (float *)c->mp_g[10], (float *)c->mp_g[11]};
     // the NF=1 call uses only the first 3 gradient slots, so slots 9..11 are free for uuu in FP32
     for (int k = 0; k < 3; ++k)
@@ -2082,7 +2168,7 @@ int nft_set_state(nft_ctx *c, const char *name, const double *host, int ld) {
   if (rc) return rc;
   for (int i = 0; i < k; ++i)
     if ((rc = copy_in(c, p[i], host + (size_t)i * ld))) return rc;
-  if (std::string(name) == "uuu" && c->P.precision == 1)
+  if (std::string(name) == "uuu" && c->P.precision >= 1)
     for (int i = 0; i < 3; ++i)
       hipLaunchKernelGGL(k_to_t<float>, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, c->uuu[i], c->uuuf[i]);
   return sync(c);
@@ -2117,11 +2203,12 @@ static double kernel_bytes(nft_ctx *c, int k) {
   // gradient / divergence sweeps read the columnar tables instead of nbr/eid when in use
   const double geomg = c->col_on ? geom0 + c->col_bytes : geom;
   const double geomf = 2 * E * 8 + 12 * E + 4 * (c->inc.nchunk + 1) + 4 * N + 4 * (N + 1) + NB * (4 + 48);
-  const bool mixed = c->P.precision == 1;
+  const bool mixed = c->P.precision >= 1, op32 = c->P.precision == 2;
   const double ts = mixed ? 4 : 8;
   switch (k) {
-    case KT_LGRAD: return geomg + 8 * N /*p*/ + 4 * 8 * N /*vol rh0 coefw ue*/ + 3 * 8 * N /*q*/;
-    case KT_LDIV: return geomg + 3 * 8 * N /*q*/ + 2 * 8 * N /*vol rh0*/ + 8 * N /*out*/;
+    // the FP32 operator (precision 2) reads 12-byte metrics and writes / gathers FP32 q
+    case KT_LGRAD: return (op32 ? geomg - 12 * E : geomg) + 8 * N /*p*/ + 4 * 8 * N /*vol rh0 coefw ue*/ + 3 * (op32 ? 4 : 8) * N /*q*/;
+    case KT_LDIV: return (op32 ? geomg - 12 * E : geomg) + 3 * (op32 ? 4 : 8) * N /*q*/ + 2 * 8 * N /*vol rh0*/ + 8 * N /*out*/;
     case KT_LINE:
       if (c->P.precon == 3) return 5 * ts * N /*factors*/ + 8 * N /*rhs*/ + 8 * N /*pin*/ + 8 * N /*p*/;
       return c->line_L > 0 ? 3 * ts * N /*rows*/ + 8 * N /*rhs*/ + 8 * N /*p*/

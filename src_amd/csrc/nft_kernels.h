@@ -61,7 +61,23 @@ struct Incidence {
   // two aligned loads per slot instead of three
   const double2 *sxy;
   const double *szz;
+  // FP32 roundings of the same metrics for the FP32 operator of the mixed form (null
+  // otherwise)
+  const float2 *sxyf;
+  const float *szzf;
 };
+
+// One slot's edge metric, widened to FP64 (TM = float reads the FP32 copies)
+template <typename TM>
+__device__ __forceinline__ void load_metric(const Incidence &g, int e, double &sx, double &sy, double &sz) {
+  if constexpr (sizeof(TM) == 4) {
+    const float2 s = g.sxyf[e];
+    sx = s.x; sy = s.y; sz = g.szzf[e];
+  } else {
+    const double2 s = g.sxy[e];
+    sx = s.x; sy = s.y; sz = g.szz[e];
+  }
+}
 
 // Per-lane view of the columnar form.
 struct ColLane {
@@ -261,6 +277,8 @@ struct GradArgs {
   const double *f;
   double *o0, *o1, *o2;
   double2 *oxy;                // GRAD_LAPLC: (x, y) components; z in o2
+  float2 *oxyf;                // GRAD_LAPLC with FP32 metrics: the same rounded to FP32
+  float *o2f;
   const double *s0, *s1, *s2;  // ustr, vstr, wstr (PRFR)
   NodeCoef nc;
   double dt;
@@ -268,8 +286,11 @@ struct GradArgs {
   const double *skip;
 };
 
-template <int MODE, bool COL>
+// TM = float (GRAD_LAPLC only): FP32 metrics, FP64 sums, FP32 outputs (the mixed form's
+// FP32 operator)
+template <int MODE, bool COL, typename TM = double>
 __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S, GradArgs a) {
+  static_assert(sizeof(TM) == 8 || MODE == GRAD_LAPLC, "FP32 metrics: laplc only");
   if (skipped(a.skip)) return;
   Lane L;
   if (!lane_setup(g, L)) return;
@@ -296,8 +317,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S
       for (int u = 0; u < NFT_BATCH_G; ++u) {
         int e = ee[u] >= 0 ? ee[u] : ~ee[u];
         fj[u] = a.f[jj[u]];
-        const double2 sxy = g.sxy[e];
-        sx[u] = sxy.x; sy[u] = sxy.y; sz[u] = g.szz[e];
+        load_metric<TM>(g, e, sx[u], sy[u], sz[u]);
       }
 #pragma unroll
       for (int u = 0; u < NFT_BATCH_G; ++u) {
@@ -351,8 +371,13 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S
       double r = a.nc.rh0[i];
       gx = gx * r; gy = gy * r; gz = gz * r;
     }
-    a.oxy[i] = make_double2(gx, gy);
-    a.o2[i] = gz;
+    if constexpr (sizeof(TM) == 4) {
+      a.oxyf[i] = make_float2((float)gx, (float)gy);
+      a.o2f[i] = (float)gz;
+    } else {
+      a.oxy[i] = make_double2(gx, gy);
+      a.o2[i] = gz;
+    }
   }
 }
 
@@ -390,6 +415,8 @@ enum Scal {
 struct DivArgs {
   const double *q0, *q1, *q2;
   const double2 *qxy;             // DIV_LAPLC: (x, y) components; z in q2
+  const float2 *qxyf;             // DIV_LAPLC with FP32 metrics: FP32 inputs
+  const float *q2f;
   double *out;
   NodeCoef nc;
   int iflg;
@@ -418,8 +445,11 @@ __device__ __forceinline__ void gcr_finalize(const DotList &d, const double *red
   }
 }
 
-template <int MODE, int ND, int NMAX, bool COL>
+// TM = float (DIV_LAPLC only): FP32 metrics and inputs, FP64 sums and output
+template <int MODE, int ND, int NMAX, bool COL, typename TM = double>
 __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S, DivArgs a) {
+  static_assert(sizeof(TM) == 8 || MODE == DIV_LAPLC, "FP32 metrics: laplc only");
+  constexpr bool F32 = sizeof(TM) == 4;
   constexpr int K = ND + NMAX;
   __shared__ double sh[(K > 0 ? K : 1) * NFT_WPB];
   if (skipped(a.skip)) return;
@@ -436,9 +466,12 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
     double ax = 0., ay = 0., az = 0.;
     double qi0 = 0., qi1 = 0., qi2 = 0.;
     if (L.valid) {
-      if (MODE == DIV_LAPLC) { const double2 v = a.qxy[i]; qi0 = v.x; qi1 = v.y; }
-      else { qi0 = a.q0[i]; qi1 = a.q1[i]; }
-      qi2 = a.q2[i];
+      if (F32) { const float2 v = a.qxyf[i]; qi0 = v.x; qi1 = v.y; qi2 = a.q2f[i]; }
+      else {
+        if (MODE == DIV_LAPLC) { const double2 v = a.qxy[i]; qi0 = v.x; qi1 = v.y; }
+        else { qi0 = a.q0[i]; qi1 = a.q1[i]; }
+        qi2 = a.q2[i];
+      }
       if (scale) { double r0 = a.nc.rh0[i]; qi0 = qi0 * r0; qi1 = qi1 * r0; qi2 = qi2 * r0; }
     }
     int2 rn[NFT_BATCH_D];
@@ -457,11 +490,13 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
       for (int u = 0; u < NFT_BATCH_D; ++u) {
         int e = ee[u] >= 0 ? ee[u] : ~ee[u];
         int j = jj[u];
-        if (MODE == DIV_LAPLC) { const double2 v = a.qxy[j]; q0[u] = v.x; q1[u] = v.y; }
-        else { q0[u] = a.q0[j]; q1[u] = a.q1[j]; }
-        q2[u] = a.q2[j];
-        const double2 sxy = g.sxy[e];
-        sx[u] = sxy.x; sy[u] = sxy.y; sz[u] = g.szz[e];
+        if (F32) { const float2 v = a.qxyf[j]; q0[u] = v.x; q1[u] = v.y; q2[u] = a.q2f[j]; }
+        else {
+          if (MODE == DIV_LAPLC) { const double2 v = a.qxy[j]; q0[u] = v.x; q1[u] = v.y; }
+          else { q0[u] = a.q0[j]; q1[u] = a.q1[j]; }
+          q2[u] = a.q2[j];
+        }
+        load_metric<TM>(g, e, sx[u], sy[u], sz[u]);
         if (scale) { double rj = a.nc.rh0[j]; q0[u] = q0[u] * rj; q1[u] = q1[u] * rj; q2[u] = q2[u] * rj; }
       }
 #pragma unroll

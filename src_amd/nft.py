@@ -186,7 +186,9 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
-PRECISION = {"fp64": 0, "mixed": 1}
+# fp64: FP64 everywhere; mixed: FP32 MPDATA and line preconditioner; mixed_op: mixed plus the
+# FP32 laplc inside the GCR iterations, checked against the FP64 residual
+PRECISION = {"fp64": 0, "mixed": 1, "mixed_op": 2}
 
 
 def make_params(cfg, precon=None, sm_flag=1, itrp1=None, omega=None, eps0=None, exit_mode=None,

@@ -38,3 +38,17 @@ def test_init_rejects_bad_arguments_without_touching_a_gpu():
     h = ctypes.c_void_p()
     rc = L.nft_init(None, None, ctypes.byref(h))
     assert rc < 0 and b"null" in L.nft_last_error()
+
+
+@pytest.mark.parametrize("field,value,words", [("precision", 3, b"precision"), ("precision", -1, b"precision"),
+                                               ("itraj", 2, b"itraj"), ("precon", 4, b"preconditioner")])
+def test_init_validates_parameters_without_touching_a_gpu(field, value, words):
+    """nft_init checks its parameters before it looks for a device."""
+    from tests.common import small_cfg
+    L = nft.lib()
+    p = nft.make_params(small_cfg(precon=1))
+    setattr(p, field, value)
+    mesh = nft.NftMesh()
+    h = ctypes.c_void_p()
+    rc = L.nft_init(ctypes.byref(mesh), ctypes.byref(p), ctypes.byref(h))
+    assert rc < 0 and words in L.nft_last_error()

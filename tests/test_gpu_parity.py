@@ -263,15 +263,16 @@ def test_steps_small_100_reference_parity_mode():
     s.close()
 
 
+@pytest.mark.parametrize("precision", ["mixed", "mixed_op"])
 @pytest.mark.parametrize("precon", [2, 3])
-def test_mixed_precision_close_to_fp64(precon):
-    """FP32 MPDATA and line preconditioner (either line), FP64 GCR: per-field deviation
-    from the FP64 form after 10 steps below 1e-5 relative L2, and the solves still
-    converge."""
+def test_mixed_precision_close_to_fp64(precon, precision):
+    """FP32 MPDATA and line preconditioner (either line), FP64 GCR (mixed_op: with the FP32
+    laplc inside the iterations): per-field deviation from the FP64 form after 10 steps
+    below 1e-5 relative L2, and the solves still converge."""
     m = mesh_for("small")
     cfg = small_cfg(precon=precon, itrp1=1, omega=1.0, exit_mode=1, eps0=1e-8, itr=200)
     a = solver(m, cfg)
-    b = solver(m, cfg, precision="mixed")
+    b = solver(m, cfg, precision=precision)
     for s in (a, b):
         s.state_init()
         st = s.run_steps(10)
@@ -283,6 +284,43 @@ def test_mixed_precision_close_to_fp64(precon):
             print(f"mixed vs fp64 after 10 steps: {name}[{k}] rel L2 {err:.3e}")
             assert err <= 1e-5, (name, k, err)
     a.close(); b.close()
+
+
+@pytest.mark.parametrize("precon", [2, 3])
+def test_fp32_operator_solve_meets_fp64_residual(precon):
+    """precision 2 (FP32 laplc inside GCR): the solve ends only when the true residual of
+    the pressure it returns, recomputed here in FP64 by the oracle, is within the
+    tolerance; gcrk's velocity outputs (prfr of the incoming pressure) are those of the FP64
+    solve; the velocities prfr makes from the returned pressure agree with the FP64 solve's
+    to 1e-6 relative L2; the iteration count stays within 35% of the FP64 solve's."""
+    print(f"case precon={precon}", flush=True)
+    m = mesh_for("small")
+    cfg = small_cfg(precon=precon, itrp1=1, omega=1.0, exit_mode=1, eps0=1e-8, itr=200)
+    f = seeded_fields(m)
+    u, v, w = f["u"], f["v"], 0.1 * f["w"]
+    out = {}
+    for prec in ("fp64", "mixed_op"):
+        s = solver(m, cfg, precision=prec)
+        p, vel, st, rc = s.gcrk(np.zeros(m.npoin), u, v, w, 1)
+        out[prec] = (p, vel, st.niter, st.tol, rc)
+        s.close()
+    p32, vel32, nit32, tol, rc = out["mixed_op"]
+    _, vel64, nit64, tol64, rc64 = out["fp64"]
+    assert rc == 0 and rc64 == 0 and tol == tol64
+    o = Oracle(m, cfg)
+    us, vs, ws = o.prfr(p32, u, v, w, 1)
+    r = o.div(us, vs, ws, -1)
+    print(f"FP32 operator: {nit32} iterations (FP64 {nit64}); true max|r| {np.abs(r).max():.3e}, tol {tol:.3e}")
+    assert np.abs(r).max() <= tol * (1 + 1e-9)
+    for k in range(3):
+        assert np.array_equal(vel32[k], vel64[k])
+    p64 = out["fp64"][0]
+    corr32, corr64 = o.prfr(p32, u, v, w, 1), o.prfr(p64, u, v, w, 1)
+    for k in range(3):
+        err = rel_l2(corr32[k], corr64[k])
+        print(f"corrected velocity {k}: rel L2 vs FP64 {err:.2e}")
+        assert err <= 1e-6
+    assert nit32 <= 1.35 * nit64
 
 
 def test_gcr_reports_nonconvergence():
