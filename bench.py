@@ -330,7 +330,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": (round(measured_traffic(kname))
-                                 if kname and args.mesh == "production" and measured_traffic(kname) else None),
+                                 if kname and args.mesh == "production" and args.precision == "fp64"
+                                 and measured_traffic(kname) else None),
                      "launches": nl, "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": int(kbytes)},
     }
     if not args.no_roofline:

@@ -67,7 +67,12 @@ typedef struct {
   double towxl, towxr, towyd, towyu, towzd, towzu;  /* absorber time scales */
   int irl;
   double dom[6];             /* box used by beta_timestep_unstr (reference: input :2255-2260) */
-  int precision;             /* 0: FP64 everywhere; 1: mixed (FP32 MPDATA + preconditioner) */
+  int precision;             /* 0: FP64 everywhere; 1: mixed (FP32 MPDATA + preconditioner);
+                                2: as 1, and laplc inside the GCR iterations with FP32 edge
+                                metrics and FP32 gradient temporaries (FP64 sums, residual,
+                                dots and updates); each solve runs in passes that restart
+                                from the FP64 residual of its pressure and ends only when
+                                that residual meets the tolerance */
   int itraj;                 /* advector at n+1/2: 0 extrapolation (reference default, :427-438);
                                 1 velprd's first-order prediction (:4448-4531, one more GCR
                                 solve per step) */

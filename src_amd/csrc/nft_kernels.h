@@ -261,6 +261,13 @@ __device__ __forceinline__ bool skipped(const double *flag) { return flag && *fl
 #ifndef NFT_BATCH_D
 #define NFT_BATCH_D 2   // divergence sweeps
 #endif
+// the same for the FP32 operator's sweeps (half-size metric and gradient loads)
+#ifndef NFT_BATCH_G32
+#define NFT_BATCH_G32 2
+#endif
+#ifndef NFT_BATCH_D32
+#define NFT_BATCH_D32 2
+#endif
 
 // ------------------------------------------------------------------------------------
 // Gradient of one field (derivative x3) with per-mode epilogues.
@@ -291,6 +298,7 @@ struct GradArgs {
 template <int MODE, bool COL, typename TM = double>
 __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S, GradArgs a) {
   static_assert(sizeof(TM) == 8 || MODE == GRAD_LAPLC, "FP32 metrics: laplc only");
+  constexpr int BG = sizeof(TM) == 4 ? NFT_BATCH_G32 : NFT_BATCH_G;
   if (skipped(a.skip)) return;
   Lane L;
   if (!lane_setup(g, L)) return;
@@ -301,26 +309,26 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S
   if (MODE != GRAD_ZERO_PRFR) {
     double fi = L.valid ? a.f[i] : 0.;
     // the next batch's slot entries are loaded while this batch's gathers are in flight
-    int2 rn[NFT_BATCH_G];
+    int2 rn[BG];
 #pragma unroll
-    for (int u = 0; u < NFT_BATCH_G; ++u) rn[u] = slot_raw<COL>(g, L, C, u);
-    for (int m0 = 0; m0 < L.width; m0 += NFT_BATCH_G) {
-      int jj[NFT_BATCH_G], ee[NFT_BATCH_G];
+    for (int u = 0; u < BG; ++u) rn[u] = slot_raw<COL>(g, L, C, u);
+    for (int m0 = 0; m0 < L.width; m0 += BG) {
+      int jj[BG], ee[BG];
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH_G; ++u) slot_decode<COL>(C, rn[u], jj[u], ee[u]);
-      if (m0 + NFT_BATCH_G < L.width) {
+      for (int u = 0; u < BG; ++u) slot_decode<COL>(C, rn[u], jj[u], ee[u]);
+      if (m0 + BG < L.width) {
 #pragma unroll
-        for (int u = 0; u < NFT_BATCH_G; ++u) rn[u] = slot_raw<COL>(g, L, C, m0 + NFT_BATCH_G + u);
+        for (int u = 0; u < BG; ++u) rn[u] = slot_raw<COL>(g, L, C, m0 + BG + u);
       }
-      double fj[NFT_BATCH_G], sx[NFT_BATCH_G], sy[NFT_BATCH_G], sz[NFT_BATCH_G];
+      double fj[BG], sx[BG], sy[BG], sz[BG];
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH_G; ++u) {
+      for (int u = 0; u < BG; ++u) {
         int e = ee[u] >= 0 ? ee[u] : ~ee[u];
         fj[u] = a.f[jj[u]];
         load_metric<TM>(g, e, sx[u], sy[u], sz[u]);
       }
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH_G; ++u) {
+      for (int u = 0; u < BG; ++u) {
         // selects, not branches: slots past the degree leave the sums exactly as they are
         const bool ok = m0 + u < L.deg;
         const double v = (fi + fj[u]) * 0.5;
@@ -450,6 +458,7 @@ template <int MODE, int ND, int NMAX, bool COL, typename TM = double>
 __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S, DivArgs a) {
   static_assert(sizeof(TM) == 8 || MODE == DIV_LAPLC, "FP32 metrics: laplc only");
   constexpr bool F32 = sizeof(TM) == 4;
+  constexpr int BD = F32 ? NFT_BATCH_D32 : NFT_BATCH_D;
   constexpr int K = ND + NMAX;
   __shared__ double sh[(K > 0 ? K : 1) * NFT_WPB];
   if (skipped(a.skip)) return;
@@ -474,20 +483,20 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
       }
       if (scale) { double r0 = a.nc.rh0[i]; qi0 = qi0 * r0; qi1 = qi1 * r0; qi2 = qi2 * r0; }
     }
-    int2 rn[NFT_BATCH_D];
+    int2 rn[BD];
 #pragma unroll
-    for (int u = 0; u < NFT_BATCH_D; ++u) rn[u] = slot_raw<COL>(g, L, C, u);
-    for (int m0 = 0; m0 < L.width; m0 += NFT_BATCH_D) {
-      int jj[NFT_BATCH_D], ee[NFT_BATCH_D];
+    for (int u = 0; u < BD; ++u) rn[u] = slot_raw<COL>(g, L, C, u);
+    for (int m0 = 0; m0 < L.width; m0 += BD) {
+      int jj[BD], ee[BD];
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH_D; ++u) slot_decode<COL>(C, rn[u], jj[u], ee[u]);
-      if (m0 + NFT_BATCH_D < L.width) {
+      for (int u = 0; u < BD; ++u) slot_decode<COL>(C, rn[u], jj[u], ee[u]);
+      if (m0 + BD < L.width) {
 #pragma unroll
-        for (int u = 0; u < NFT_BATCH_D; ++u) rn[u] = slot_raw<COL>(g, L, C, m0 + NFT_BATCH_D + u);
+        for (int u = 0; u < BD; ++u) rn[u] = slot_raw<COL>(g, L, C, m0 + BD + u);
       }
-      double q0[NFT_BATCH_D], q1[NFT_BATCH_D], q2[NFT_BATCH_D], sx[NFT_BATCH_D], sy[NFT_BATCH_D], sz[NFT_BATCH_D];
+      double q0[BD], q1[BD], q2[BD], sx[BD], sy[BD], sz[BD];
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH_D; ++u) {
+      for (int u = 0; u < BD; ++u) {
         int e = ee[u] >= 0 ? ee[u] : ~ee[u];
         int j = jj[u];
         if (F32) { const float2 v = a.qxyf[j]; q0[u] = v.x; q1[u] = v.y; q2[u] = a.q2f[j]; }
@@ -500,7 +509,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
         if (scale) { double rj = a.nc.rh0[j]; q0[u] = q0[u] * rj; q1[u] = q1[u] * rj; q2[u] = q2[u] * rj; }
       }
 #pragma unroll
-      for (int u = 0; u < NFT_BATCH_D; ++u) {
+      for (int u = 0; u < BD; ++u) {
         const bool ok = m0 + u < L.deg;
         const double u0 = (qi0 + q0[u]) * 0.5, u1 = (qi1 + q1[u]) * 0.5, u2 = (qi2 + q2[u]) * 0.5;
         const double tx = sx[u] * u0, ty = sy[u] * u1, tz = sz[u] * u2;

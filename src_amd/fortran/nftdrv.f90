@@ -11,8 +11,9 @@
 !
 ! Usage: nftdrv <nlev> <precon> <precision> <lord> <strat> [exit_mode] [itraj]
 !   nlev      levels per column when nodes are numbered column by column (0: no columns)
-!   precon    1 point relaxation (reference), 2 vertical line
-!   precision 0 FP64, 1 mixed
+!   precon    1 point relaxation (reference), 2 vertical line (two-point flux), 3 vertical line
+!             (in-column band of laplc)
+!   precision 0 FP64, 1 mixed, 2 mixed with the FP32 operator inside GCR
 !   lord      GCR order; strat: N^2/g (the reference hard-codes these in mpsolver)
 !   itraj     0 extrapolated advector (reference), 1 velprd
 program nftdrv
