@@ -251,7 +251,9 @@ int nft_net_params(nft_ctx *ctx, float *host, int64_t count, int set);
 int nft_net_eval(nft_ctx *ctx, const float *X, const float *T, int nsamples, float *Y, float *grad,
                  double *loss);
 /* store every stride-th pair (r, q = M_line^-1 r) of the GCR iterations that follow, with
- * s = max|r| and |r|^2, until max_pairs are held (device memory: 16*npoin bytes per pair) */
+ * s = max|r| and |r|^2, until max_pairs are held, and for the error objective the iteration's
+ * pressure, turned into x_k - x_final when its solve converges (pairs of a solve that does not
+ * converge are dropped); device memory: 24*npoin bytes per pair */
 int nft_dump_pairs(nft_ctx *ctx, int max_pairs, int stride);
 /* pair store: count, and copies of pairs [first, first+n_get): r and q as n_get x npoin */
 int nft_pairs(nft_ctx *ctx, int *count, int first, int n_get, double *r, double *q, double *s,
@@ -263,7 +265,8 @@ int nft_net_pair_loss(nft_ctx *ctx, int first, int count, double *loss, float *g
 /* Adam over pairs [0, use_pairs) (all when use_pairs <= 0) */
 int nft_net_train(nft_ctx *ctx, int steps, int batch, double lr, uint64_t seed, int use_pairs,
                   double *loss_first, double *loss_last);
-/* training objective: 0 residual |Lz - r|^2/|r|^2, 1 angle 1 - <r,Lz>^2/(|r|^2 |Lz|^2);
+/* training objective: 0 residual |Lz - r|^2/|r|^2, 1 angle 1 - <r,Lz>^2/(|r|^2 |Lz|^2),
+ * 2 error |z - d|^2/|d|^2 with d = x_k - x_final (= L^-1 r to the solve's tolerance);
  * linear = 1: identity activations and biases held at zero (the correction is linear in r) */
 int nft_net_options(nft_ctx *ctx, int objective, int linear);
 /* kind 0: line preconditioner only; 1: learned correction + line fallback. kind < 0 only

@@ -159,15 +159,17 @@ struct nft_ctx {
     float *dC2 = nullptr, *dZ2 = nullptr, *dZ1 = nullptr, *P = nullptr, *lpart = nullptr;
     size_t P_count = 0;
     int step = 0;
-    // training pairs: residual r and its line-preconditioned q, with s = max|r| and |r|^2
-    double *pR = nullptr, *pQ = nullptr;
+    // training pairs: residual r and its line-preconditioned q, with s = max|r| and |r|^2;
+    // pD: the pressure x_k of the iteration, replaced once its solve has converged by
+    // x_k - x_final (= L^-1 r up to the final residual: the error objective's target)
+    double *pR = nullptr, *pQ = nullptr, *pD = nullptr;
     std::vector<double> ps, pr2;
     std::vector<int> pimp;   // impfl of the solve each pair came from
     int npairs = 0, max_pairs = 0, stride = 1, seen = 0;
     // training work arrays (node length) and loss partials
     double *tz = nullptr, *tLz = nullptr, *ty = nullptr, *tg = nullptr, *tpart = nullptr;
     int dumping = 0;
-    int objective = 0;     // training objective: 0 residual, 1 angle
+    int objective = 0;     // training objective: 0 residual, 1 angle, 2 error
     int linear = 0;        // identity activations and zero biases: a fixed linear map
     int learned_iters = 0; // GCR iterations that used the learned correction
     int kind = 0;          // 0 off, 1 learned correction on top of the line preconditioner
@@ -918,12 +920,13 @@ int net_apply_precon(nft_ctx *c, const double *r, double *qr, int sidx) {
 }
 
 // keep every stride-th (r, line-preconditioned q) of the solves while the store has room
-int net_store_pair(nft_ctx *c, double s, double r2, int impfl) {
+int net_store_pair(nft_ctx *c, double s, double r2, int impfl, const double *p) {
   auto &n = c->net;
   if (n.seen++ % n.stride != 0 || n.npairs >= n.max_pairs || !(s > 0) || !(r2 > 0)) return 0;
   const size_t N = c->n;
   HIPCHK(hipMemcpyAsync(n.pR + N * n.npairs, c->r, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(n.pQ + N * n.npairs, c->qr, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(n.pD + N * n.npairs, p, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   n.ps.push_back(s);
   n.pr2.push_back(r2);
   n.pimp.push_back(impfl);
@@ -1086,7 +1089,7 @@ static int gcr_loop_host(nft_ctx *c, double *p, int impfl, double eps, int itmn,
       }
       rl20 = std::max(rl2, epa);
       if ((rc = precon_dev(c, c->r, c->qr, impfl))) return rc;
-      if (c->net.dumping && (rc = net_store_pair(c, *dvmx, rl2, impfl))) return rc;
+      if (c->net.dumping && (rc = net_store_pair(c, *dvmx, rl2, impfl, p))) return rc;
       if (c->net.active) {
         if ((rc = net_apply_precon(c, c->r, c->qr, SC_DVMX))) return rc;
         c->net.learned_iters++;
@@ -1153,6 +1156,7 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
   const double *qr_last = x0;
   int nlc = 0, converged = 0;
   double dvmx = 0, eer0 = epa, eem0 = epa;
+  const int pair0 = c->net.npairs;   // pairs stored from here on belong to this solve
   c->net.active = c->net.kind == 1 && c->net.ready;
   for (int pass = 0;; ++pass) {
     // u2, v2, w2 are prfr of the incoming pressure (outputs of gcrk); later passes use
@@ -1200,6 +1204,22 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
     nlc += nlc_pass;
     if (!op32) { converged = conv_pass; break; }
     if (!conv_pass) break;   // stagnation, NaN or the iteration cap: not converged
+  }
+  if (c->net.npairs > pair0) {
+    // error targets of this solve's pairs: x_k - x_final; a solve that did not converge
+    // gives no targets, and its pairs are dropped
+    auto &n = c->net;
+    if (converged) {
+      for (int k = pair0; k < n.npairs; ++k) {
+        double *d = n.pD + (size_t)c->n * k;
+        hipLaunchKernelGGL(k_sub, dim3(c->nodes_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, d, p, d,
+                           (const double *)nullptr);
+      }
+      HIPCHK(hipGetLastError());
+    } else {
+      n.npairs = pair0;
+      n.ps.resize(pair0); n.pr2.resize(pair0); n.pimp.resize(pair0);
+    }
   }
   if ((rc = norms_dev(c, qr_last, SC_EER, SC_EEM))) return rc;
   if ((rc = read_scal(c, SC_EER, 2))) return rc;
@@ -2337,7 +2357,9 @@ int nft_dump_pairs(nft_ctx *c, int max_pairs, int stride) {
   const size_t N = c->n;
   int rc;
   if (max_pairs > n.max_pairs) {
-    if ((rc = dalloc(c, &n.pR, N * max_pairs)) || (rc = dalloc(c, &n.pQ, N * max_pairs))) return rc;
+    if ((rc = dalloc(c, &n.pR, N * max_pairs)) || (rc = dalloc(c, &n.pQ, N * max_pairs)) ||
+        (rc = dalloc(c, &n.pD, N * max_pairs)))
+      return rc;
     n.max_pairs = max_pairs;
     n.npairs = 0;
     n.ps.clear();
@@ -2385,6 +2407,27 @@ static int net_physics_grad(nft_ctx *c, const int *idx, int B, double *loss) {
     const double s = n.ps[k], a = n.pr2[k];
     const double *r = n.pR + (size_t)N * k, *q = n.pQ + (size_t)N * k;
     hipLaunchKernelGGL(k_net_z, dim3(nb), dim3(256), 0, c->stream, n.ncol, n.L, n.Kp, q, n.Y + per * bi, s, n.tz);
+    if (n.objective == 2) {
+      // error objective: l = |z - d|^2/|d|^2 with d = x_k - x_final, dl/dz = 2(z - d)/|d|^2
+      const double *d = n.pD + (size_t)N * k;
+      hipLaunchKernelGGL(k_net_axpby, dim3(nb), dim3(256), 0, c->stream, N, 1.0, n.tz, -1.0, d, n.tLz);
+      hipLaunchKernelGGL(k_net_dots, dim3(nb), dim3(256), 0, c->stream, N, n.tLz, d, n.tpart);
+      HIPCHK(hipMemcpyAsync(part.data(), n.tpart, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      hipLaunchKernelGGL(k_net_dots, dim3(nb), dim3(256), 0, c->stream, N, d, d, n.tpart + 2 * nb);
+      std::vector<double> part2(2 * (size_t)nb);
+      HIPCHK(hipMemcpyAsync(part2.data(), n.tpart + 2 * nb, 2 * nb * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      double ee = 0, dd = 0;
+      for (int i = 0; i < nb; ++i) { ee += part[2 * i]; dd += part2[2 * i]; }
+      if (!(dd > 0)) return set_err(-1, "net training: a pair with a zero error target");
+      total += ee / dd;
+      hipLaunchKernelGGL(k_net_axpby, dim3(nb), dim3(256), 0, c->stream, N, 2.0 / (dd * B), n.tLz, 0.0, d, n.ty);
+      hipLaunchKernelGGL(k_net_dY, dim3((per + 255) / 256), dim3(256), 0, c->stream, n.ncol, n.L, n.Kp, n.ty, s,
+                         n.dY + per * bi);
+      HIPCHK(hipGetLastError());
+      continue;
+    }
     if ((rc = laplc_dev(c, n.tz, n.tLz, n.pimp[k], nullptr))) return rc;
     hipLaunchKernelGGL(k_net_dots, dim3(nb), dim3(256), 0, c->stream, N, n.tLz, r, n.tpart);
     HIPCHK(hipMemcpyAsync(part.data(), n.tpart, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -2418,7 +2461,7 @@ static int net_train_alloc(nft_ctx *c) {
   const size_t N = c->n;
   int rc;
   if ((rc = dalloc(c, &n.tz, N)) || (rc = dalloc(c, &n.tLz, N)) || (rc = dalloc(c, &n.ty, N)) ||
-      (rc = dalloc(c, &n.tg, N)) || (rc = dalloc(c, &n.tpart, 2 * ((N + 255) / 256))))
+      (rc = dalloc(c, &n.tg, N)) || (rc = dalloc(c, &n.tpart, 4 * ((N + 255) / 256))))
     return rc;
   return 0;
 }
@@ -2568,10 +2611,10 @@ int nft_stream_calibration(nft_ctx *c, int64_t bytes, int reps, double *ms) {
   return rc;
 }
 
-// training objective (0 residual, 1 angle) and activation (linear = 1: identity activations
+// training objective (0 residual, 1 angle, 2 error) and activation (linear = 1: identity activations
 // with the biases held at zero, so the correction is a fixed linear map of r)
 int nft_net_options(nft_ctx *c, int objective, int linear) {
-  if (!c || !c->net.ready || objective < 0 || objective > 1 || linear < 0 || linear > 1)
+  if (!c || !c->net.ready || objective < 0 || objective > 2 || linear < 0 || linear > 1)
     return set_err(-1, "nft_net_options: bad argument");
   auto &n = c->net;
   n.objective = objective;

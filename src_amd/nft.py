@@ -446,7 +446,8 @@ class Solver:
         return fb.value, li.value
 
     def net_options(self, objective=0, linear=0):
-        """Training objective (0 residual, 1 angle); linear=1 makes the correction linear in r."""
+        """Training objective (0 residual, 1 angle, 2 error against x_k - x_final); linear=1 makes
+        the correction linear in r."""
         _chk(lib().nft_net_options(self.h, objective, linear), "nft_net_options")
 
     def copy_calibration(self, reps=10):
