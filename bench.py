@@ -50,6 +50,57 @@ def stress_config(column_order="grid"):
                        sponge_depth=3000.0, sponge_tau=300.0)
 
 
+def small_config():
+    """The 40,000-node mesh and flow (2,000 jittered columns x 20 levels, dt = 10 s, U = 10 m/s,
+    N = 0.01 1/s) with GCR(3) to 1e-8 relative residual and the band line preconditioner."""
+    from src_amd import mesh as M
+    c = M.CONFIGS["small"]
+    return M.RunConfig(c.name, c.mesh, u0=c.u0, bvf=c.bvf, dt=c.dt, steps=c.steps, lord=3,
+                       eps0=1e-8, exit_mode=1, itr=400, precon=3, itrp1=1, omega=1.0,
+                       sponge_depth=3000.0, sponge_tau=300.0)
+
+
+def mixed_error_run(steps, every):
+    """The mixed forms on the 40,000-node mesh and flow, stepped beside the FP64 form from
+    the same initial state: per-field relative L2 and max-norm deviation of u, v, w, theta'
+    from the FP64 state every `every` steps, and each form's timesteps/s."""
+    import numpy as np
+    from src_amd import mesh as M
+    from src_amd import nft
+    cfg = small_config()
+    mesh = M.generate(cfg.mesh)
+    precs = ("fp64", "mixed", "mixed_op")
+    sol = {p: nft.Solver(mesh, nft.make_params(cfg, precision=p)) for p in precs}
+    for sv in sol.values():
+        sv.state_init()
+    ms = dict.fromkeys(precs, 0.0)
+    its = dict.fromkeys(precs, 0)
+    solves = dict.fromkeys(precs, 0)
+    unconv = dict.fromkeys(precs, 0)
+    rows = []
+    for k in range(every, steps + 1, every):
+        for p, sv in sol.items():
+            st = sv.run_steps(every)
+            ms[p] += st.ms_total; its[p] += st.gcr_iters; solves[p] += st.gcr_solves
+            unconv[p] += st.gcr_unconverged
+        ref = sol["fp64"].state("unk")
+        row = {"step": k}
+        for p in precs[1:]:
+            u = sol[p].state("unk")
+            row[p] = {name: [float(np.linalg.norm(u[f] - ref[f]) / np.linalg.norm(ref[f])),
+                             float(np.abs(u[f] - ref[f]).max() / np.abs(ref[f]).max())]
+                      for f, name in enumerate(("u", "v", "w", "theta"))}
+        rows.append(row)
+    for sv in sol.values():
+        sv.close()
+    return {"mesh": f"{mesh.npoin} nodes (2,000 columns x 20 levels), dt=10 s, GCR(3) to 1e-8 relative, band line",
+            "steps": steps,
+            "forms": {p: {"timesteps_per_s": round(1000.0 * steps / ms[p], 2),
+                          "iters_per_solve": round(its[p] / max(solves[p], 1), 2), "unconverged": unconv[p]}
+                      for p in precs},
+            "error_vs_fp64": {"fields": "u, v, w, theta' as [rel L2, rel max-norm]", "every": every, "rows": rows}}
+
+
 WORKLOADS = {
     "production": ("production mesh: 65,536 jittered columns x 64 levels over 200 km x 200 km x 20 km "
                    "({nodes} nodes, {edges} edges), hill h0=1 km, U=10 m/s, N=0.01 1/s (Fr=1), dt=20 s; MPDATA + "
@@ -251,6 +302,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--learned-train-steps", type=int, default=100)
+    ap.add_argument("--mixed-error-steps", type=int, default=1000,
+                    help="steps of the mixed-precision error report on the 40,000-node mesh (0: skip)")
     ap.add_argument("--mesh", default="production", choices=("production", "stress"),
                     help="workload: the production mesh (headline metric) or the 16.8M-node stress mesh")
     ap.add_argument("--column-order", default="grid", choices=("grid", "hilbert"),
@@ -364,6 +417,12 @@ def main():
             log(f"[{prec}] {v['value']} steps/s, {v['iters_per_solve']} iterations/solve, "
                 f"{v['solve_ms']} ms/solve, field error {v['field_error_vs_fp64']}")
         line["variants"]["learned"] = learned_run(mesh, cfg, mk, args.learned_train_steps, max(args.warmup, 1))
+        if args.mesh == "production" and args.mixed_error_steps > 0:
+            t0 = time.perf_counter()
+            line["variants"]["mixed_error_small"] = mixed_error_run(args.mixed_error_steps, 100)
+            me = line["variants"]["mixed_error_small"]
+            log(f"[mixed error, 40,000-node mesh] {me['forms']}; last row {me['error_vs_fp64']['rows'][-1]} "
+                f"({time.perf_counter() - t0:.1f} s)")
         if args.mesh == "production":
             for kind in ("reference", "line"):
                 t0 = time.perf_counter()
