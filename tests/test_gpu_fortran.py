@@ -120,3 +120,32 @@ def test_fortran_driver_velprd(tmp_path):
     assert np.array_equal(st[0:4], o.state("unk"))
     assert np.array_equal(st[4:7], o.state("uuu"))
     assert np.array_equal(st[7], o.state("prpot"))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_fortran_driver_band_line_and_precision(tmp_path, precision):
+    """precon=3 (the band line) with each precision through nftdrv, relative exit: the
+    state equals the ctypes path with the same settings bit for bit (same library, same
+    inputs)."""
+    if not os.path.exists(DRV):
+        pytest.skip("nftdrv not built")
+    print(f"case precision={precision}", flush=True)
+    m = mesh_for("tiny")
+    cfg = tiny_cfg(precon=3, itrp1=1, omega=1.0, exit_mode=1, eps0=1e-8, itr=200)
+    nsteps = 3
+    d = str(tmp_path)
+    write_reference_inputs(d, m, cfg, ntime=nsteps, sm_flag=1, itrp1=cfg.itrp1, omega=cfg.omega)
+    r = subprocess.run([DRV, str(m.nlev), "3", str(precision), str(cfg.lord), repr(cfg.strat), "1"], cwd=d,
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0
+    st = np.fromfile(os.path.join(d, "nftdrv_state.bin"), dtype=np.float64).reshape(8, m.npoin)
+    s = nft.Solver(m, nft.make_params(cfg, precision=precision))
+    s.state_init()
+    s.run_steps(nsteps)
+    unk, uuu, prpot = s.state("unk"), s.state("uuu"), s.state("prpot")
+    s.close()
+    assert np.array_equal(st[0:4], unk)
+    assert np.array_equal(st[4:7], uuu)
+    assert np.array_equal(st[7], prpot)
