@@ -1160,6 +1160,10 @@ __global__ void k_band_extract(int n, int L, const int32_t *color, int col, int 
 #define NFT_PENTA_CHUNK 16
 #endif
 
+#ifndef NFT_PENTA_STAGE
+#define NFT_PENTA_STAGE 16  // loads in flight per lane while staging the band solve's columns (8 and 16 measured)
+#endif
+
 template <typename T>
 struct BandFactors {
   const T *l1, *l2, *u1, *u2, *dinv;
@@ -1178,7 +1182,21 @@ __global__ void __launch_bounds__(NFT_WAVE) k_line_penta(int ncol, int L, int cp
   const int c0 = cb * cpb, ncb = min(cpb, ncol - c0);
   const int RL = L + 1;
   const size_t node0 = (size_t)c0 * L;
-  for (int idx = lane; idx < ncb * L; idx += NFT_WAVE) lds[(idx / L) * RL + idx % L] = (T)res[node0 + idx];
+  const int tot = ncb * L;
+  // staging: NFT_PENTA_STAGE loads in flight per lane before their LDS stores (one wavefront
+  // per block leaves little else to hide the latency)
+  int idx = lane;
+  for (; idx + (NFT_PENTA_STAGE - 1) * NFT_WAVE < tot; idx += NFT_PENTA_STAGE * NFT_WAVE) {
+    double v[NFT_PENTA_STAGE];
+#pragma unroll
+    for (int u = 0; u < NFT_PENTA_STAGE; ++u) v[u] = res[node0 + idx + u * NFT_WAVE];
+#pragma unroll
+    for (int u = 0; u < NFT_PENTA_STAGE; ++u) {
+      const int q = idx + u * NFT_WAVE;
+      lds[(q / L) * RL + q % L] = (T)v[u];
+    }
+  }
+  for (; idx < tot; idx += NFT_WAVE) lds[(idx / L) * RL + idx % L] = (T)res[node0 + idx];
   __syncthreads();
   if (lane < ncb) {
     // the factors of NFT_PENTA_CHUNK levels are loaded together ahead of their steps, so
@@ -1231,7 +1249,18 @@ __global__ void __launch_bounds__(NFT_WAVE) k_line_penta(int ncol, int L, int cp
     }
   }
   __syncthreads();
-  for (int idx = lane; idx < ncb * L; idx += NFT_WAVE) {
+  idx = lane;
+  for (; idx + (NFT_PENTA_STAGE - 1) * NFT_WAVE < tot; idx += NFT_PENTA_STAGE * NFT_WAVE) {
+    double p0[NFT_PENTA_STAGE];
+#pragma unroll
+    for (int u = 0; u < NFT_PENTA_STAGE; ++u) p0[u] = pin ? pin[node0 + idx + u * NFT_WAVE] : 0.;
+#pragma unroll
+    for (int u = 0; u < NFT_PENTA_STAGE; ++u) {
+      const int q = idx + u * NFT_WAVE;
+      p[node0 + q] = p0[u] + omega * (double)lds[(q / L) * RL + q % L];
+    }
+  }
+  for (; idx < tot; idx += NFT_WAVE) {
     const double p0 = pin ? pin[node0 + idx] : 0.;
     p[node0 + idx] = p0 + omega * (double)lds[(idx / L) * RL + idx % L];
   }
