@@ -1260,18 +1260,22 @@ int mpdata_dev(nft_ctx *c, const T *U, const T *V, const T *W, double *const *fi
   else S = *(EdgeS<T> *)&c->Sf;
   dim3 g(c->gather_grid), b(NFT_BLOCK);
   t_begin(c, KT_MP0);
-  hipLaunchKernelGGL((k_mp_donor<T, NF>), g, b, 0, c->stream, c->inc, S, a);
+  if (c->col_on) hipLaunchKernelGGL((k_mp_donor<T, NF, true>), g, b, 0, c->stream, c->inc, S, a);
+  else hipLaunchKernelGGL((k_mp_donor<T, NF, false>), g, b, 0, c->stream, c->inc, S, a);
   t_end(c, KT_MP0);
   // pass 2 reads the donor-cell fields
   t_begin(c, KT_MP0 + 1);
-  hipLaunchKernelGGL((k_mp_grad<T, NF>), g, b, 0, c->stream, c->inc, S, a);
+  if (c->col_on) hipLaunchKernelGGL((k_mp_grad<T, NF, true>), g, b, 0, c->stream, c->inc, S, a);
+  else hipLaunchKernelGGL((k_mp_grad<T, NF, false>), g, b, 0, c->stream, c->inc, S, a);
   t_end(c, KT_MP0 + 1);
   t_begin(c, KT_MP0 + 2);
-  hipLaunchKernelGGL((k_mp_fct<T, NF>), g, b, 0, c->stream, c->inc, S, a);
+  if (c->col_on) hipLaunchKernelGGL((k_mp_fct<T, NF, true>), g, b, 0, c->stream, c->inc, S, a);
+  else hipLaunchKernelGGL((k_mp_fct<T, NF, false>), g, b, 0, c->stream, c->inc, S, a);
   t_end(c, KT_MP0 + 2);
   double *const *out = (double *const *)c->io[7];
   t_begin(c, KT_MP0 + 3);
-  hipLaunchKernelGGL((k_mp_limit<T, NF>), g, b, 0, c->stream, c->inc, S, a, out);
+  if (c->col_on) hipLaunchKernelGGL((k_mp_limit<T, NF, true>), g, b, 0, c->stream, c->inc, S, a, out);
+  else hipLaunchKernelGGL((k_mp_limit<T, NF, false>), g, b, 0, c->stream, c->inc, S, a, out);
   t_end(c, KT_MP0 + 3);
   HIPCHK(hipGetLastError());
   return 0;
@@ -2218,11 +2222,12 @@ int nft_kernel_timer(nft_ctx *c, int kernel) {
 // algorithmic bytes of one launch: each array read or written counted once at its width
 static double kernel_bytes(nft_ctx *c, int k) {
   const double N = c->n, E = c->etot, NB = c->inc.nbp;
-  const double geom0 = 24 * E + 4 * (c->inc.nchunk + 1) + 4 * N + 4 * (N + 1) + NB * (4 + 48);
-  const double geom = geom0 + 2 * E * 8;
-  // gradient / divergence sweeps read the columnar tables instead of nbr/eid when in use
-  const double geomg = c->col_on ? geom0 + c->col_bytes : geom;
-  const double geomf = 2 * E * 8 + 12 * E + 4 * (c->inc.nchunk + 1) + 4 * N + 4 * (N + 1) + NB * (4 + 48);
+  const double rest = 4 * (c->inc.nchunk + 1) + 4 * N + 4 * (N + 1) + NB * (4 + 48);
+  // the sweeps read the columnar tables instead of the nbr/eid slot streams when in use
+  const double slots = c->col_on ? c->col_bytes : 2 * E * 8;
+  const double geomg = 24 * E + slots + rest;   // FP64 metrics
+  const double geomf = 12 * E + slots + rest;   // FP32 metrics
+  const double geom = geomg;
   const bool mixed = c->P.precision >= 1, op32 = c->P.precision == 2;
   const double ts = mixed ? 4 : 8;
   switch (k) {

@@ -1401,11 +1401,39 @@ __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
 template <typename T>
 __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
 
+// Slot m of node i for the MPDATA sweeps: neighbour j and signed edge e (SELL-64 or
+// columnar form, which hold the same slots in the same order)
+template <bool COL>
+__device__ __forceinline__ void mp_slot(const Incidence &g, const Lane &L, const ColLane &C, int m, int &j, int &e) {
+  slot_decode<COL>(C, slot_raw<COL>(g, L, C, m), j, e);
+}
+
+// Edge metric of the MPDATA sweeps: FP64 from the (Sx, Sy) pair copy, FP32 from its own
+// arrays (the same values either way)
+template <typename T>
+__device__ __forceinline__ void mp_metric(const Incidence &g, const EdgeS<T> &S, int e, T &sx, T &sy, T &sz) {
+  if constexpr (sizeof(T) == 8) {
+    const double2 v = g.sxy[e];
+    sx = v.x; sy = v.y; sz = g.szz[e];
+  } else {
+    sx = S.sx[e]; sy = S.sy[e]; sz = S.sz[e];
+  }
+}
+
+// The MPDATA sweeps gather per node over its slots in the reference's edge order. The
+// edge quantities are written with the edge's (i1, i2) roles where they are not symmetric
+// (p2 - p1, the limiter's ratios); sums and products of the two ends are symmetric in IEEE
+// arithmetic, so both end nodes of an edge obtain the reference's bits. Each node's own
+// values are read once, before its slot loop.
+
 // pass 1: donor-cell fluxes, boundary closure and update (reference :3094-3174)
-template <typename T, int NF>
+template <typename T, int NF, bool COL>
 __global__ void __launch_bounds__(NFT_BLOCK) k_mp_donor(Incidence g, EdgeS<T> S, MpArgs<T, NF> a) {
   Lane L;
-  if (!lane_setup(g, L) || !L.valid) return;
+  if (!lane_setup(g, L)) return;
+  ColLane C{};
+  if (COL) col_setup(g, L.valid ? L.i : g.n - 1, C);
+  if (!L.valid) return;
   const int i = L.i;
   T Ui = a.U[i], Vi = a.V[i], Wi = a.W[i];
   T pi[NF], arh2[NF];
@@ -1413,13 +1441,14 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_donor(Incidence g, EdgeS<T> S,
   for (int f = 0; f < NF; ++f) { pi[f] = (T)a.psi[f][i]; arh2[f] = (T)0; }
   T divv = (T)0;
   for (int m = 0; m < L.deg; ++m) {
-    int slot = L.base + m * NFT_WAVE + L.lane;
-    int j = g.nbr[slot], e = g.eid[slot];
-    bool first = e >= 0;
-    int ee = first ? e : ~e;
+    int j, e;
+    mp_slot<COL>(g, L, C, m, j, e);
+    const bool first = e >= 0;
+    const int ee = first ? e : ~e;
     T Uj = a.U[j], Vj = a.V[j], Wj = a.W[j];
-    T sx = S.sx[ee], sy = S.sy[ee], sz = S.sz[ee];
-    T aun = first ? edge_aun<T>(Ui, Uj, Vi, Vj, Wi, Wj, sx, sy, sz) : edge_aun<T>(Uj, Ui, Vj, Vi, Wj, Wi, sx, sy, sz);
+    T sx, sy, sz;
+    mp_metric<T>(g, S, ee, sx, sy, sz);
+    T aun = edge_aun<T>(Ui, Uj, Vi, Vj, Wi, Wj, sx, sy, sz);
     T ap = tmax<T>((T)0, aun), an = tmin<T>((T)0, aun);
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
@@ -1449,25 +1478,31 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_donor(Incidence g, EdgeS<T> S,
 }
 
 // gradients of the donor-cell fields (derivative x3 per field, reference :3182-3184)
-template <typename T, int NF>
+template <typename T, int NF, bool COL>
 __global__ void __launch_bounds__(NFT_BLOCK) k_mp_grad(Incidence g, EdgeS<T> S, MpArgs<T, NF> a) {
   Lane L;
-  if (!lane_setup(g, L) || !L.valid) return;
+  if (!lane_setup(g, L)) return;
+  ColLane C{};
+  if (COL) col_setup(g, L.valid ? L.i : g.n - 1, C);
+  if (!L.valid) return;
   const int i = L.i;
   T pi[NF], gx[NF], gy[NF], gz[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) { pi[f] = (T)a.psi1[f][i]; gx[f] = gy[f] = gz[f] = (T)0; }
   for (int m = 0; m < L.deg; ++m) {
-    int slot = L.base + m * NFT_WAVE + L.lane;
-    int j = g.nbr[slot], e = g.eid[slot];
-    bool first = e >= 0;
-    int ee = first ? e : ~e;
-    T sx = S.sx[ee], sy = S.sy[ee], sz = S.sz[ee];
+    int j, e;
+    mp_slot<COL>(g, L, C, m, j, e);
+    const bool first = e >= 0;
+    const int ee = first ? e : ~e;
+    T sx, sy, sz;
+    mp_metric<T>(g, S, ee, sx, sy, sz);
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       T u = (pi[f] + (T)a.psi1[f][j]) * (T)0.5;
-      if (first) { gx[f] = gx[f] + sx * u; gy[f] = gy[f] + sy * u; gz[f] = gz[f] + sz * u; }
-      else { gx[f] = gx[f] - sx * u; gy[f] = gy[f] - sy * u; gz[f] = gz[f] - sz * u; }
+      T tx = sx * u, ty = sy * u, tz = sz * u;
+      gx[f] = first ? gx[f] + tx : gx[f] - tx;
+      gy[f] = first ? gy[f] + ty : gy[f] - ty;
+      gz[f] = first ? gz[f] + tz : gz[f] - tz;
     }
   }
   for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
@@ -1479,55 +1514,58 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_grad(Incidence g, EdgeS<T> S, 
   for (int f = 0; f < NF; ++f) { a.gx[f][i] = gx[f]; a.gy[f][i] = gy[f]; a.gz[f][i] = gz[f]; }
 }
 
-// antidiffusive velocity of one edge for field f, evaluated with the edge's own (i1,i2)
-// orientation so both end nodes obtain identical bits (reference :3187-3204)
-template <typename T, int NF>
-__device__ __forceinline__ T anti_flux(const MpArgs<T, NF> &a, int f, int i1, int i2, T aun, T volt,
-                                       T U12, T V12, T W12) {
-  T p1 = (T)a.psi1[f][i1], p2 = (T)a.psi1[f][i2];
+// antidiffusive velocity of one edge for one field (reference :3187-3204) from the two end
+// nodes' donor-cell values p1, p2 (edge roles i1, i2) and gradient sums gsx = gx_i1 + gx_i2, ...
+template <typename T>
+__device__ __forceinline__ T anti_flux(T p1, T p2, T gsx, T gsy, T gsz, T aun, T volt, T U12, T V12, T W12, T dt) {
   T un = fabs(aun) * (p2 - p1) * (T)0.5;
-  T dpdx = (T)0.5 * (a.gx[f][i1] + a.gx[f][i2]) / volt;
-  T dpdy = (T)0.5 * (a.gy[f][i1] + a.gy[f][i2]) / volt;
-  T dpdz = (T)0.5 * (a.gz[f][i1] + a.gz[f][i2]) / volt;
+  T dpdx = (T)0.5 * gsx / volt;
+  T dpdy = (T)0.5 * gsy / volt;
+  T dpdz = (T)0.5 * gsz / volt;
   T term = U12 * (T)0.5 * dpdx + V12 * (T)0.5 * dpdy + W12 * (T)0.5 * dpdz;
-  return un - aun * term * a.dt * (T)0.5;
+  return un - aun * term * dt * (T)0.5;
 }
 
 // FCT bounds from the donor-cell solution, in/out fluxes and ratios (reference :3228-3280)
-template <typename T, int NF>
+template <typename T, int NF, bool COL>
 __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, EdgeS<T> S, MpArgs<T, NF> a) {
   Lane L;
-  if (!lane_setup(g, L) || !L.valid) return;
+  if (!lane_setup(g, L)) return;
+  ColLane C{};
+  if (COL) col_setup(g, L.valid ? L.i : g.n - 1, C);
+  if (!L.valid) return;
   const int i = L.i;
   T Ui = a.U[i], Vi = a.V[i], Wi = a.W[i];
   T vri = a.vol[i] * a.rh0[i];
-  T mx[NF], mn[NF], rin[NF], rout[NF], pi[NF];
+  T mx[NF], mn[NF], rin[NF], rout[NF], pi[NF], gxi[NF], gyi[NF], gzi[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     mx[f] = (T)-1.e15; mn[f] = (T)1.e15; rin[f] = (T)0; rout[f] = (T)0;
     pi[f] = (T)a.psi1[f][i];
+    gxi[f] = a.gx[f][i]; gyi[f] = a.gy[f][i]; gzi[f] = a.gz[f][i];
   }
   for (int m = 0; m < L.deg; ++m) {
-    int slot = L.base + m * NFT_WAVE + L.lane;
-    int j = g.nbr[slot], e = g.eid[slot];
-    bool first = e >= 0;
-    int ee = first ? e : ~e;
-    int i1 = first ? i : j, i2 = first ? j : i;
+    int j, e;
+    mp_slot<COL>(g, L, C, m, j, e);
+    const bool first = e >= 0;
+    const int ee = first ? e : ~e;
     T Uj = a.U[j], Vj = a.V[j], Wj = a.W[j];
-    T sx = S.sx[ee], sy = S.sy[ee], sz = S.sz[ee];
-    T aun = first ? edge_aun<T>(Ui, Uj, Vi, Vj, Wi, Wj, sx, sy, sz) : edge_aun<T>(Uj, Ui, Vj, Vi, Wj, Wi, sx, sy, sz);
+    T sx, sy, sz;
+    mp_metric<T>(g, S, ee, sx, sy, sz);
+    T aun = edge_aun<T>(Ui, Uj, Vi, Vj, Wi, Wj, sx, sy, sz);
     T vrj = a.vol[j] * a.rh0[j];
-    T volt = (T)0.5 * (first ? vri + vrj : vrj + vri);
+    T volt = (T)0.5 * (vri + vrj);
     T U12 = Ui + Uj, V12 = Vi + Vj, W12 = Wi + Wj;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      T A = anti_flux<T, NF>(a, f, i1, i2, aun, volt, U12, V12, W12);
       T pj = (T)a.psi1[f][j];
+      T A = anti_flux<T>(first ? pi[f] : pj, first ? pj : pi[f], gxi[f] + a.gx[f][j], gyi[f] + a.gy[f][j],
+                         gzi[f] + a.gz[f][j], aun, volt, U12, V12, W12, a.dt);
       mx[f] = tmax<T>(tmax<T>(mx[f], pi[f]), pj);
       mn[f] = tmin<T>(tmin<T>(mn[f], pi[f]), pj);
       T pp = tmax<T>((T)0, A), pn = -tmin<T>((T)0, A);
-      if (first) { rin[f] = rin[f] + pn; rout[f] = rout[f] + pp; }
-      else { rin[f] = rin[f] + pp; rout[f] = rout[f] + pn; }
+      rin[f] = rin[f] + (first ? pn : pp);
+      rout[f] = rout[f] + (first ? pp : pn);
     }
   }
   T vl = a.vol[i], rh = a.rh0[i];
@@ -1540,34 +1578,47 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, EdgeS<T> S, M
 }
 
 // limited antidiffusive fluxes and the second update (reference :3284-3302, :3098-3174)
-template <typename T, int NF>
+template <typename T, int NF, bool COL>
 __global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, EdgeS<T> S, MpArgs<T, NF> a,
                                                         double *const *out) {
   Lane L;
-  if (!lane_setup(g, L) || !L.valid) return;
+  if (!lane_setup(g, L)) return;
+  ColLane C{};
+  if (COL) col_setup(g, L.valid ? L.i : g.n - 1, C);
+  if (!L.valid) return;
   const int i = L.i;
   T Ui = a.U[i], Vi = a.V[i], Wi = a.W[i];
   T vri = a.vol[i] * a.rh0[i];
-  T arh2[NF];
+  T arh2[NF], pi[NF], gxi[NF], gyi[NF], gzi[NF], cpi[NF], cni[NF];
 #pragma unroll
-  for (int f = 0; f < NF; ++f) arh2[f] = (T)0;
+  for (int f = 0; f < NF; ++f) {
+    arh2[f] = (T)0;
+    pi[f] = (T)a.psi1[f][i];
+    gxi[f] = a.gx[f][i]; gyi[f] = a.gy[f][i]; gzi[f] = a.gz[f][i];
+    cpi[f] = a.cp[f][i]; cni[f] = a.cn[f][i];
+  }
   for (int m = 0; m < L.deg; ++m) {
-    int slot = L.base + m * NFT_WAVE + L.lane;
-    int j = g.nbr[slot], e = g.eid[slot];
-    bool first = e >= 0;
-    int ee = first ? e : ~e;
-    int i1 = first ? i : j, i2 = first ? j : i;
+    int j, e;
+    mp_slot<COL>(g, L, C, m, j, e);
+    const bool first = e >= 0;
+    const int ee = first ? e : ~e;
     T Uj = a.U[j], Vj = a.V[j], Wj = a.W[j];
-    T sx = S.sx[ee], sy = S.sy[ee], sz = S.sz[ee];
-    T aun = first ? edge_aun<T>(Ui, Uj, Vi, Vj, Wi, Wj, sx, sy, sz) : edge_aun<T>(Uj, Ui, Vj, Vi, Wj, Wi, sx, sy, sz);
+    T sx, sy, sz;
+    mp_metric<T>(g, S, ee, sx, sy, sz);
+    T aun = edge_aun<T>(Ui, Uj, Vi, Vj, Wi, Wj, sx, sy, sz);
     T vrj = a.vol[j] * a.rh0[j];
-    T volt = (T)0.5 * (first ? vri + vrj : vrj + vri);
+    T volt = (T)0.5 * (vri + vrj);
     T U12 = Ui + Uj, V12 = Vi + Vj, W12 = Wi + Wj;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      T A = anti_flux<T, NF>(a, f, i1, i2, aun, volt, U12, V12, W12);
-      T lim = A > (T)0 ? tmin<T>(tmin<T>((T)1, a.cp[f][i2]), a.cn[f][i1])
-                       : tmin<T>(tmin<T>((T)1, a.cn[f][i2]), a.cp[f][i1]);
+      T pj = (T)a.psi1[f][j];
+      T A = anti_flux<T>(first ? pi[f] : pj, first ? pj : pi[f], gxi[f] + a.gx[f][j], gyi[f] + a.gy[f][j],
+                         gzi[f] + a.gz[f][j], aun, volt, U12, V12, W12, a.dt);
+      const T cpj = a.cp[f][j], cnj = a.cn[f][j];
+      // edge roles: cp/cn of i2 and i1
+      const T cp2 = first ? cpj : cpi[f], cn1 = first ? cni[f] : cnj;
+      const T cn2 = first ? cnj : cni[f], cp1 = first ? cpi[f] : cpj;
+      T lim = A > (T)0 ? tmin<T>(tmin<T>((T)1, cp2), cn1) : tmin<T>(tmin<T>((T)1, cn2), cp1);
       T fl = A * lim;
       arh2[f] = first ? arh2[f] + fl : arh2[f] - fl;
     }
