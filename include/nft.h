@@ -226,7 +226,8 @@ int nft_mesh_release(nft_mesh *mesh);
  * is selected, every launch of that kernel is bracketed by HIP events on the context's
  * stream. bytes_per_launch is the kernel's algorithmic traffic: every array it reads or
  * writes counted once at its stored width (incidence slots actually used, edge metrics,
- * node arrays). */
+ * node arrays; for laplc's divergence also the vectors its fused dots read, averaged over
+ * the timed launches). */
 int nft_kernel_timer(nft_ctx *ctx, int kernel);
 int nft_kernel_timer_read(nft_ctx *ctx, int *launches, double *total_ms, double *bytes_per_launch);
 /* plain device copy of one node array (8*npoin bytes read + written) `reps` times, mean ms

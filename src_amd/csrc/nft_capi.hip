@@ -184,6 +184,8 @@ struct nft_ctx {
   int tev_used = 0;
   int t_launches = 0;
   double t_ms = 0;
+  std::vector<double> tex;   // per recorded pair: bytes of the launch beyond kernel_bytes()
+  double t_extra = 0;        // their sum over the folded pairs
 };
 
 namespace {
@@ -392,8 +394,13 @@ inline bool timing(nft_ctx *c, int k) { return c->timer_kernel == k && c->tev_us
 inline void t_begin(nft_ctx *c, int k) {
   if (timing(c, k)) (void)hipEventRecord(c->tev[c->tev_used], c->stream);
 }
-inline void t_end(nft_ctx *c, int k) {
-  if (timing(c, k)) { (void)hipEventRecord(c->tev[c->tev_used + 1], c->stream); c->tev_used += 2; }
+// extra: algorithmic bytes of this launch beyond kernel_bytes() (the fused dots' inputs)
+inline void t_end(nft_ctx *c, int k, double extra = 0.0) {
+  if (timing(c, k)) {
+    (void)hipEventRecord(c->tev[c->tev_used + 1], c->stream);
+    c->tex[c->tev_used / 2] = extra;
+    c->tev_used += 2;
+  }
 }
 // fold the first `upto` recorded events (finished pairs) into the totals and move the rest
 // to the front
@@ -401,9 +408,14 @@ void t_harvest_prefix(nft_ctx *c, int upto) {
   upto = std::min(upto, c->tev_used) & ~1;
   for (int i = 0; i + 1 < upto; i += 2) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, c->tev[i], c->tev[i + 1]) == hipSuccess) { c->t_ms += ms; c->t_launches++; }
+    if (hipEventElapsedTime(&ms, c->tev[i], c->tev[i + 1]) == hipSuccess) {
+      c->t_ms += ms;
+      c->t_launches++;
+      c->t_extra += c->tex[i / 2];
+    }
   }
   std::rotate(c->tev.begin(), c->tev.begin() + upto, c->tev.begin() + c->tev_used);
+  std::rotate(c->tex.begin(), c->tex.begin() + upto / 2, c->tex.begin() + c->tev_used / 2);
   c->tev_used -= upto;
 }
 // after a stream synchronisation: fold finished event pairs into the totals
@@ -497,7 +509,10 @@ int div_launch(nft_ctx *c, int mode, const DivArgs &a, bool f32 = false) {
     if (nmax) return set_err(-1, "laplc: max reduction not supported");
     t_begin(c, KT_LDIV);
     bool ok = f32 ? div_laplc_nd<0, float>(c, a, nd) : div_laplc_nd<0, double>(c, a, nd);
-    t_end(c, KT_LDIV);
+    // the fused dots read one node array each, except a dot of the output with itself
+    int reads = 0;
+    for (int k = 0; k < nd; ++k) reads += a.dots.a[k] ? 1 : 0;
+    t_end(c, KT_LDIV, 8.0 * c->n * reads);
     if (!ok) return set_err(-1, "laplc: too many fused dots");
   } else {
     if (nd == 0 && nmax == 0) div_go<DIV_DIV, 0, 0>(c, a);
@@ -2211,11 +2226,13 @@ int nft_kernel_timer(nft_ctx *c, int kernel) {
   if (c->tev.empty()) {
     c->tev.resize(1024);
     for (auto &e : c->tev) HIPCHK(hipEventCreate(&e));
+    c->tex.assign(c->tev.size() / 2, 0.0);
   }
   c->timer_kernel = kernel;
   c->tev_used = 0;
   c->t_launches = 0;
   c->t_ms = 0;
+  c->t_extra = 0;
   return 0;
 }
 
@@ -2260,7 +2277,8 @@ int nft_kernel_timer_read(nft_ctx *c, int *launches, double *total_ms, double *b
   t_harvest(c);
   if (launches) *launches = c->t_launches;
   if (total_ms) *total_ms = c->t_ms;
-  if (bytes_per_launch) *bytes_per_launch = kernel_bytes(c, c->timer_kernel);
+  if (bytes_per_launch)
+    *bytes_per_launch = kernel_bytes(c, c->timer_kernel) + (c->t_launches ? c->t_extra / c->t_launches : 0.0);
   return 0;
 }
 
