@@ -237,6 +237,11 @@ int nft_copy_calibration(nft_ctx *ctx, int reps, double *ms);
  * than the last-level cache), mean ms per copy: the practical HBM streaming rate beside the
  * published peak */
 int nft_stream_calibration(nft_ctx *ctx, int64_t bytes, int reps, double *ms);
+/* precision 2 only (no reference counterpart): a pass of a solve ends when the recursive
+ * residual's max-norm has dropped by `drop` (0 < drop < 1, default 1e-3) from the FP64
+ * residual the pass started from; smaller values mean fewer, longer passes. drop <= 0
+ * only reads the current value into *current (may be null otherwise). */
+int nft_set_mixed_drop(nft_ctx *ctx, double drop, double *current);
 
 /* ---- learned preconditioner (no reference counterpart) ------------------------------
  * A column MLP with two horizontal graph convolutions (bf16 MFMA, FP32 accumulation)

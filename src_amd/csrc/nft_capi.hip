@@ -108,6 +108,7 @@ struct nft_ctx {
   double2 *txy;          // laplc: (x, y) of the scaled gradient (z in t[2])
   float2 *txyf = nullptr;   // the FP32 operator's (x, y) and z (precision 2)
   float *t2f = nullptr;
+  double op32_drop = 1.0e-3;   // recursive-residual drop that ends a pass (precision 2)
   double *gx[NFT_MAXD], *gax[NFT_MAXD];
   double *r, *qr, *ar, *pw, *rw;
   double *mp_psi1[4];
@@ -1146,11 +1147,8 @@ static int gcr_loop_host(nft_ctx *c, double *p, int impfl, double eps, int itmn,
 // FP64. The recursive residual then drifts from the true one by the operator's rounding,
 // so a solve runs in passes: each pass starts from the true FP64 residual of the current
 // pressure (prfr + div, as the reference's start) and iterates until the recursive
-// residual has dropped by NFT_OP32_DROP or passed the tolerance; the next pass checks the
+// residual has dropped by op32_drop (nft_set_mixed_drop) or passed the tolerance; the next pass checks the
 // tolerance on the true residual and stops there, or restarts GCR from it.
-#ifndef NFT_OP32_DROP
-#define NFT_OP32_DROP 1.0e-3
-#endif
 #ifndef NFT_OP32_PASSES
 #define NFT_OP32_PASSES 24
 #endif
@@ -1206,7 +1204,7 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
     }
     const double rl20 = std::max(c->h_scal[SC_R2], epa);
     // a pass of the FP32 operator ends early, at a fixed drop of the recursive residual
-    const double eps_pass = op32 ? std::max(eps, NFT_OP32_DROP * c->h_scal[SC_RMX]) : eps;
+    const double eps_pass = op32 ? std::max(eps, c->op32_drop * c->h_scal[SC_RMX]) : eps;
     const int itmn = pass == 0 ? 3 : 1;
     int nlc_pass = 0, conv_pass = 0;
     qr_last = x0;
@@ -2632,6 +2630,16 @@ int nft_stream_calibration(nft_ctx *c, int64_t bytes, int reps, double *ms) {
   (void)hipFree(b);
   if (!rc && ms) *ms = t / reps;
   return rc;
+}
+
+int nft_set_mixed_drop(nft_ctx *c, double drop, double *current) {
+  if (!c) return set_err(-1, "nft_set_mixed_drop: null context");
+  if (drop > 0) {
+    if (!(drop < 1)) return set_err(-1, "nft_set_mixed_drop: drop must lie in (0, 1)");
+    c->op32_drop = drop;
+  }
+  if (current) *current = c->op32_drop;
+  return 0;
 }
 
 // training objective (0 residual, 1 angle, 2 error) and activation (linear = 1: identity activations

@@ -117,6 +117,7 @@ _SIGS.update({
     "nft_set_incidence": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "nft_stream_calibration": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_double)]),
+    "nft_set_mixed_drop": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
     "nft_set_multigrid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
                                          ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]),
     "nft_multigrid_field": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, f64p,
@@ -494,6 +495,13 @@ class Solver:
         ms = ctypes.c_double()
         _chk(lib().nft_stream_calibration(self.h, int(nbytes), reps, ctypes.byref(ms)), "nft_stream_calibration")
         return ms.value
+
+    def set_mixed_drop(self, drop=0.0):
+        """precision 2: recursive-residual drop that ends a pass of a solve (0 < drop < 1);
+        drop <= 0 only reads it. Returns the value in force."""
+        cur = ctypes.c_double()
+        _chk(lib().nft_set_mixed_drop(self.h, float(drop), ctypes.byref(cur)), "nft_set_mixed_drop")
+        return cur.value
 
     def set_incidence(self, columnar):
         """1: columnar gathers (when available), 0: SELL-64, -1: no change. Returns availability."""

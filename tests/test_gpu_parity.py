@@ -286,14 +286,14 @@ def test_mixed_precision_close_to_fp64(precon, precision):
     a.close(); b.close()
 
 
-@pytest.mark.parametrize("precon", [2, 3])
-def test_fp32_operator_solve_meets_fp64_residual(precon):
+@pytest.mark.parametrize("precon,drop", [(2, None), (3, None), (3, 1e-4), (3, 1e-2)])
+def test_fp32_operator_solve_meets_fp64_residual(precon, drop):
     """precision 2 (FP32 laplc inside GCR): the solve ends only when the true residual of
     the pressure it returns, recomputed here in FP64 by the oracle, is within the
     tolerance; gcrk's velocity outputs (prfr of the incoming pressure) are those of the FP64
     solve; the velocities prfr makes from the returned pressure agree with the FP64 solve's
     to 1e-6 relative L2; the iteration count stays within 35% of the FP64 solve's."""
-    print(f"case precon={precon}", flush=True)
+    print(f"case precon={precon} drop={drop}", flush=True)
     m = mesh_for("small")
     cfg = small_cfg(precon=precon, itrp1=1, omega=1.0, exit_mode=1, eps0=1e-8, itr=200)
     f = seeded_fields(m)
@@ -301,6 +301,12 @@ def test_fp32_operator_solve_meets_fp64_residual(precon):
     out = {}
     for prec in ("fp64", "mixed_op"):
         s = solver(m, cfg, precision=prec)
+        if prec == "mixed_op":
+            assert s.set_mixed_drop() == 1e-3   # the default
+            if drop is not None:
+                assert s.set_mixed_drop(drop) == drop and s.set_mixed_drop(0.0) == drop
+            with pytest.raises(nft.NftError):
+                s.set_mixed_drop(1.5)
         p, vel, st, rc = s.gcrk(np.zeros(m.npoin), u, v, w, 1)
         out[prec] = (p, vel, st.niter, st.tol, rc)
         s.close()
