@@ -160,13 +160,13 @@ def timed_run(mesh, cfg, precision, steps, warmup, profile=True, pre_steps=0, mi
     return out
 
 
-def learned_run(mesh, cfg, steps, train_steps, warmup=1, log_prefix="[learned]"):
-    """Plain FP64 solver with the learned correction: pairs (r, M_line^-1 r) are stored
-    during the first plain step, the network is trained on them (residual objective), the
-    rest of the `warmup` steps run plain, then `steps` steps run with the correction (and
-    its stall fallback) in every GCR solve."""
+def learned_run(mesh, cfg, steps, train_steps, warmup=1, log_prefix="[learned]", precision="fp64"):
+    """Solver of the given precision with the learned correction: pairs (r, M_line^-1 r)
+    are stored during the first plain step, the network is trained on them (residual
+    objective), the rest of the `warmup` steps run plain, then `steps` steps run with the
+    correction (and its stall fallback) in every GCR solve."""
     from src_amd import nft
-    s = nft.Solver(mesh, nft.make_params(cfg))
+    s = nft.Solver(mesh, nft.make_params(cfg, precision=precision))
     s.net_init(hidden=128, seed=1, max_batch=2)
     s.state_init()
     s.dump_pairs(max_pairs=64, stride=4)
@@ -188,7 +188,7 @@ def learned_run(mesh, cfg, steps, train_steps, warmup=1, log_prefix="[learned]")
     s.close()
     avg = ms / max(nl, 1)
     tf = flops / (avg * 1e-3) / 1e12 if nl and avg > 0 else 0.0
-    return dict(value=round(1000.0 / (st.ms_total / steps), 4), unit="timesteps/s",
+    return dict(value=round(1000.0 / (st.ms_total / steps), 4), unit="timesteps/s", precision=precision,
                 iters_per_solve=round(st.gcr_iters / max(st.gcr_solves, 1), 2),
                 solve_ms=round(st.ms_gcr / max(st.gcr_solves, 1), 3), steps=steps,
                 unconverged=st.gcr_unconverged, fallbacks=fb, learned_iterations=li,
@@ -429,6 +429,9 @@ def main():
             log(f"[{name}] {v['value']} steps/s, {v['iters_per_solve']} iterations/solve, "
                 f"{v['solve_ms']} ms/solve, field error {v['field_error_vs_fp64']}")
         line["variants"]["learned"] = learned_run(mesh, cfg, mk, args.learned_train_steps, max(args.warmup, 1))
+        line["variants"]["learned_mixed_op"] = learned_run(mesh, cfg, mk, args.learned_train_steps,
+                                                           max(args.warmup, 1), "[learned, mixed_op]",
+                                                           precision="mixed_op")
         if args.mesh == "production" and args.mixed_error_steps > 0:
             t0 = time.perf_counter()
             line["variants"]["mixed_error_small"] = mixed_error_run(args.mixed_error_steps, 100)
