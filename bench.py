@@ -60,16 +60,18 @@ def small_config():
                        sponge_depth=3000.0, sponge_tau=300.0)
 
 
-def mixed_error_run(steps, every):
-    """The mixed forms on the 40,000-node mesh and flow, stepped beside the FP64 form from
-    the same initial state: per-field relative L2 and max-norm deviation of u, v, w, theta'
-    from the FP64 state every `every` steps, and each form's timesteps/s."""
+def mixed_error_run(steps, every, cfg=None, mesh=None, precs=("fp64", "mixed", "mixed_op"), mesh_text=None):
+    """The mixed forms stepped beside the FP64 form from the same initial state (by default
+    on the 40,000-node mesh and flow): per-field relative L2 and max-norm deviation of u, v,
+    w, theta' from the FP64 state every `every` steps, and each form's timesteps/s."""
     import numpy as np
     from src_amd import mesh as M
     from src_amd import nft
-    cfg = small_config()
-    mesh = M.generate(cfg.mesh)
-    precs = ("fp64", "mixed", "mixed_op")
+    if cfg is None:
+        cfg = small_config()
+        mesh_text = "2,000 columns x 20 levels, dt=10 s, GCR(3) to 1e-8 relative, band line"
+    if mesh is None:
+        mesh = M.generate(cfg.mesh)
     sol = {p: nft.Solver(mesh, nft.make_params(cfg, precision=p)) for p in precs}
     for sv in sol.values():
         sv.state_init()
@@ -90,10 +92,13 @@ def mixed_error_run(steps, every):
             row[p] = {name: [float(np.linalg.norm(u[f] - ref[f]) / np.linalg.norm(ref[f])),
                              float(np.abs(u[f] - ref[f]).max() / np.abs(ref[f]).max())]
                       for f, name in enumerate(("u", "v", "w", "theta"))}
+        row["finite"] = bool(np.isfinite(ref).all())
         rows.append(row)
+        log(f"[error report] step {k}: " + "; ".join(
+            f"{p} max rel L2 {max(v[0] for v in row[p].values()):.2e}" for p in precs[1:]))
     for sv in sol.values():
         sv.close()
-    return {"mesh": f"{mesh.npoin} nodes (2,000 columns x 20 levels), dt=10 s, GCR(3) to 1e-8 relative, band line",
+    return {"mesh": f"{mesh.npoin} nodes ({mesh_text})",
             "steps": steps,
             "forms": {p: {"timesteps_per_s": round(1000.0 * steps / ms[p], 2),
                           "iters_per_solve": round(its[p] / max(solves[p], 1), 2), "unconverged": unconv[p]}
@@ -307,6 +312,9 @@ def main():
     ap.add_argument("--learned-train-steps", type=int, default=100)
     ap.add_argument("--mixed-error-steps", type=int, default=1000,
                     help="steps of the mixed-precision error report on the 40,000-node mesh (0: skip)")
+    ap.add_argument("--long-run", type=int, default=0,
+                    help="instead of the benchmark: N steps of the chosen mesh's full configuration, FP64 "
+                         "and the FP32-operator mixed form side by side, deviation reported every 100 steps")
     ap.add_argument("--mesh", default="production", choices=("production", "stress"),
                     help="workload: the production mesh (headline metric) or the 16.8M-node stress mesh")
     ap.add_argument("--column-order", default="grid", choices=("grid", "hilbert"),
@@ -329,6 +337,11 @@ def main():
     cfg = (production_config if args.mesh == "production" else stress_config)(args.column_order)
     t0 = time.perf_counter()
     mesh = M.generate(cfg.mesh)
+    if args.long_run > 0:
+        r = mixed_error_run(args.long_run, 100, cfg=cfg, mesh=mesh, precs=("fp64", "mixed_op"),
+                            mesh_text=WORKLOADS[args.mesh].format(nodes=mesh.npoin, edges=mesh.nedge + mesh.nbface))
+        print(json.dumps({"long_run": r}), flush=True)
+        return
     log(f"mesh: {mesh.npoin} nodes, {mesh.nedge} edges + {mesh.nbface} semi-edges, "
         f"{mesh.nboun} boundary points ({time.perf_counter() - t0:.1f} s)")
     log("device:", nft.device_name())
