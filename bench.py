@@ -73,8 +73,9 @@ def mixed_error_run(steps, every, cfg=None, mesh=None, precs=("fp64", "mixed", "
     if mesh is None:
         mesh = M.generate(cfg.mesh)
     sol = {p: nft.Solver(mesh, nft.make_params(cfg, precision=p)) for p in precs}
-    for sv in sol.values():
+    for p, sv in sol.items():
         sv.state_init()
+        log(f"[error report] {p}: initialised")
     ms = dict.fromkeys(precs, 0.0)
     its = dict.fromkeys(precs, 0)
     solves = dict.fromkeys(precs, 0)
@@ -85,6 +86,7 @@ def mixed_error_run(steps, every, cfg=None, mesh=None, precs=("fp64", "mixed", "
             st = sv.run_steps(every)
             ms[p] += st.ms_total; its[p] += st.gcr_iters; solves[p] += st.gcr_solves
             unconv[p] += st.gcr_unconverged
+            log(f"[error report] {p}: step {k}, {st.ms_total / 1000.0:.1f} s for the last {every} steps")
         ref = sol["fp64"].state("unk")
         row = {"step": k}
         for p in precs[1:]:
@@ -337,6 +339,7 @@ def main():
     cfg = (production_config if args.mesh == "production" else stress_config)(args.column_order)
     t0 = time.perf_counter()
     mesh = M.generate(cfg.mesh)
+    log(f"mesh: {mesh.npoin} nodes ({time.perf_counter() - t0:.1f} s)")
     if args.long_run > 0:
         r = mixed_error_run(args.long_run, 100, cfg=cfg, mesh=mesh, precs=("fp64", "mixed_op"),
                             mesh_text=WORKLOADS[args.mesh].format(nodes=mesh.npoin, edges=mesh.nedge + mesh.nbface))
