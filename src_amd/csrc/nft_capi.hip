@@ -91,6 +91,7 @@ struct nft_ctx {
   std::vector<nft_diag> diag_meta;   // eer, eem, niter, nitav, step of each pending record
   std::vector<int> diag_slot;
   double col_bytes = 0;       // bytes of the columnar tables
+  int n_flag78 = 0;           // boundary entries with flag 7 or 8 (bcv reads ue there)
   double *zbuf;
   float *zbuff;
 
@@ -1624,6 +1625,7 @@ struct Geometry {
   int gather_grid = 0;
   int col_on = 0;
   double col_bytes = 0;
+  int n_flag78 = 0;   // boundary entries with flag 7 or 8 (where bcv reads ue)
   std::vector<int32_t> i1, i2;
   std::vector<double> sx, sy, sz;
 };
@@ -1732,6 +1734,7 @@ int build_geometry(nft_ctx *c, const nft_mesh *m, Geometry &G) {
     int i = m->ibpoin[b] - 1;
     int q = bfill[i]++;
     bflag[q] = m->ibpoin[m->ld_face + b];
+    if (bflag[q] == 7 || bflag[q] == 8) G.n_flag78++;
     for (int d = 0; d < 3; ++d) {
       bsbb[(size_t)d * nbp + q] = m->sbb[(size_t)d * m->ld_face + b];
       banp[(size_t)d * nbp + q] = m->anp[(size_t)d * m->ld_face + b];
@@ -1834,7 +1837,7 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   Geometry geo;
   if ((rc = build_geometry(c, m, geo))) return fail(rc);
   c->inc = geo.inc; c->S = geo.S; c->gather_grid = geo.gather_grid;
-  c->col_on = geo.col_on; c->col_bytes = geo.col_bytes;
+  c->col_on = geo.col_on; c->col_bytes = geo.col_bytes; c->n_flag78 = geo.n_flag78;
   c->vec_grid = std::min(blocks_for(n), 4096);
   c->nodes_grid = blocks_for(n);
   const std::vector<int32_t> &i1 = geo.i1, &i2 = geo.i2;
@@ -2244,11 +2247,17 @@ static double kernel_bytes(nft_ctx *c, int k) {
   const double geomf = 12 * E + slots + rest;   // FP32 metrics
   const double geom = geomg;
   const bool mixed = c->P.precision >= 1, op32 = c->P.precision == 2;
+  const double rho_b = c->nc.rh0_one ? 0.0 : 8 * N, cw_b = c->nc.coefw_uniform ? 0.0 : 8 * N;
   const double ts = mixed ? 4 : 8;
   switch (k) {
-    // the FP32 operator (precision 2) reads 12-byte metrics and writes / gathers FP32 q
-    case KT_LGRAD: return (op32 ? geomg - 12 * E : geomg) + 8 * N /*p*/ + 4 * 8 * N /*vol rh0 coefw ue*/ + 3 * (op32 ? 4 : 8) * N /*q*/;
-    case KT_LDIV: return (op32 ? geomg - 12 * E : geomg) + 3 * (op32 ? 4 : 8) * N /*q*/ + 2 * 8 * N /*vol rh0*/ + 8 * N /*out*/;
+    // the FP32 operator (precision 2) reads 12-byte metrics and writes / gathers FP32 q; rh0
+    // and coefw are read only when they are not uniform (1.0 / one value), ue only at the
+    // boundary entries of flags 7 and 8
+    case KT_LGRAD: return (op32 ? geomg - 12 * E : geomg) + 8 * N /*p*/ + 8 * N /*vol*/ + rho_b + cw_b +
+                          8.0 * c->n_flag78 /*ue*/ + 3 * (op32 ? 4 : 8) * N /*q*/;
+    // the divergence reads only sbb of the boundary entries (not their flags or anp)
+    case KT_LDIV: return (op32 ? geomg - 12 * E : geomg) - NB * (4 + 24) + 3 * (op32 ? 4 : 8) * N /*q*/ +
+                         8 * N /*vol*/ + rho_b + 8 * N /*out*/;
     case KT_LINE:
       if (c->P.precon == 3) return 5 * ts * N /*factors*/ + 8 * N /*rhs*/ + 8 * N /*pin*/ + 8 * N /*p*/;
       return c->line_L > 0 ? 3 * ts * N /*rows*/ + 8 * N /*rhs*/ + 8 * N /*p*/
