@@ -5,7 +5,7 @@
  * (isfera=0, iper=0, lipps=0, itraj=0 or 1, implgw=1, icorio=0). Each function keeps the
  * reference's loop order and operation order so that, compiled without FP contraction,
  * it reproduces the reference's rounding. Only tests, the smoke check and the CPU leg
- * of the benchmark use it, as the checker of the HIP path.
+ * of the benchmark use it, as what the HIP path's results are compared with.
  *
  * Parity pinning (who checks each function):
  *   derivative, laplc, prfr, div, bcv, precon(point), gcrk, mpdatam, compute_d,

@@ -1,7 +1,8 @@
 """ctypes front end of the CPU oracle (TEST INFRASTRUCTURE ONLY).
 
-Imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg, always as
-the checker of the HIP path. Wraps oracle/libnft_oracle.so (nft_oracle.c).
+Imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg only, always
+as the reference the HIP path's results are compared with, never on the product path.
+Wraps oracle/libnft_oracle.so (nft_oracle.c).
 """
 from __future__ import annotations
 
