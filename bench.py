@@ -330,8 +330,8 @@ def main():
                     help="also run N plain device copies of one node array (byte-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the "
-                                   "column partitions is a design note only (--gpus must be 1)"}))
+        print(json.dumps({"error": "the solver runs on one GPU; sharding the mesh by column partitions "
+                                   "over several GPUs is not implemented (--gpus must be 1)"}))
         sys.exit(2)
     from src_amd import mesh as M
     from src_amd import nft
