@@ -116,6 +116,10 @@ int nft_laplc(nft_ctx *ctx, const double *press, double *alap, int impfl);
 /* out = L^T y, the transpose of laplc (no reference counterpart: trains the learned
  * preconditioner) */
 int nft_laplc_t(nft_ctx *ctx, const double *y, double *out, int impfl);
+/* no reference counterpart: the FP32 operator that precision 2 applies inside the GCR
+ * iterations (edge metrics and scaled gradient in FP32, FP64 sums), on host arrays; for
+ * checking it against its CPU restatement. Precision 2 contexts only. */
+int nft_laplc32(nft_ctx *ctx, const double *press, double *alap, int impfl);
 /* prfr(press,ustr,vstr,wstr,uout,vout,wout,impfl) reference :1536-1580 */
 int nft_prfr(nft_ctx *ctx, const double *press, const double *ustr, const double *vstr,
              const double *wstr, double *uout, double *vout, double *wout, int impfl);

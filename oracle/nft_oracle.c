@@ -347,14 +347,16 @@ double oc_beta(oc_ctx *c) { return c->beta1; }
 void oc_get_stats(oc_ctx *c, oc_stats *s) { *s = c->st; }
 void oc_set_itr(oc_ctx *c, int itr) { c->p.itr = itr; }
 
-/* derivative(unk,dpdxa,idx), F:3943-4022, Cartesian branch */
-void oc_derivative(oc_ctx *c, const double *unk, double *dpdxa, int idx) {
+/* derivative(unk,dpdxa,idx), F:3943-4022, Cartesian branch; f32: the edge metrics sn, sb
+ * rounded to FP32 first (the FP32 operator of the mixed form; sbb stays FP64) */
+static void derivative_m(oc_ctx *c, const double *unk, double *dpdxa, int idx, int f32) {
   const oc_mesh *m = &c->m;
   int n = c->n, d = idx - 1;
   for (int i = 0; i < n; ++i) dpdxa[i] = 0.;
   for (int e = 0; e < m->nedge; ++e) {
     int i1 = m->iedge[e] - 1, i2 = m->iedge[m->nedge + e] - 1;
     double sx = m->sn[(size_t)d * m->nedge + e];
+    if (f32) sx = (double)(float)sx;
     double u = (unk[i1] + unk[i2]) * 0.5;
     dpdxa[i1] = dpdxa[i1] + sx * u;
     dpdxa[i2] = dpdxa[i2] - sx * u;
@@ -362,6 +364,7 @@ void oc_derivative(oc_ctx *c, const double *unk, double *dpdxa, int idx) {
   for (int e = 0; e < m->nbface; ++e) {
     int i1 = m->kbface[e] - 1, i2 = m->kbface[m->nbface + e] - 1;
     double sx = m->sb[(size_t)d * m->nbface + e];
+    if (f32) sx = (double)(float)sx;
     double u = (unk[i1] + unk[i2]) * 0.5;
     dpdxa[i1] = dpdxa[i1] + sx * u;
     dpdxa[i2] = dpdxa[i2] - sx * u;
@@ -373,6 +376,8 @@ void oc_derivative(oc_ctx *c, const double *unk, double *dpdxa, int idx) {
     dpdxa[i1] = dpdxa[i1] + sx * u;
   }
 }
+
+void oc_derivative(oc_ctx *c, const double *unk, double *dpdxa, int idx) { derivative_m(c, unk, dpdxa, idx, 0); }
 
 /* bcv(uout,vout,wout,flag), F:1583-1643, Cartesian branch */
 void oc_bcv(oc_ctx *c, double *u, double *v, double *w, double flag) {
@@ -391,15 +396,16 @@ void oc_bcv(oc_ctx *c, double *u, double *v, double *w, double flag) {
   }
 }
 
-/* laplc(press,alap,impfl), F:1288-1353 */
-void oc_laplc(oc_ctx *c, const double *press, double *alap, int impfl) {
+/* laplc(press,alap,impfl), F:1288-1353; f32: the mixed form's FP32 operator (edge metrics
+ * rounded to FP32, the scaled gradient rounded to FP32 before the divergence, FP64 sums) */
+static void laplc_m(oc_ctx *c, const double *press, double *alap, int impfl, int f32) {
   int n = c->n;
   double *gx = c->s[3], *gy = c->s[4], *gz = c->s[5];
   double dt = c->p.dt;
   for (int i = 0; i < n; ++i) alap[i] = 0.;
-  oc_derivative(c, press, gx, 1);
-  oc_derivative(c, press, gy, 2);
-  oc_derivative(c, press, gz, 3);
+  derivative_m(c, press, gx, 1, f32);
+  derivative_m(c, press, gy, 2, f32);
+  derivative_m(c, press, gz, 3, f32);
   for (int i = 0; i < n; ++i) {
     double v = c->m.vol[i];
     double fz = impfl ? c->coefw[i] : 1.0; /* coefw*impfl+(1-impfl) */
@@ -415,13 +421,17 @@ void oc_laplc(oc_ctx *c, const double *press, double *alap, int impfl) {
     gx[i] = gx[i] * c->rh0[i];
     gy[i] = gy[i] * c->rh0[i];
     gz[i] = gz[i] * c->rh0[i];
+    if (f32) { gx[i] = (double)(float)gx[i]; gy[i] = (double)(float)gy[i]; gz[i] = (double)(float)gz[i]; }
   }
   double *ax = c->s[6], *ay = c->s[7], *az = c->s[8];
-  oc_derivative(c, gx, ax, 1);
-  oc_derivative(c, gy, ay, 2);
-  oc_derivative(c, gz, az, 3);
+  derivative_m(c, gx, ax, 1, f32);
+  derivative_m(c, gy, ay, 2, f32);
+  derivative_m(c, gz, az, 3, f32);
   for (int i = 0; i < n; ++i) alap[i] = (ax[i] + ay[i] + az[i]) / (c->m.vol[i] * c->rh0[i]);
 }
+
+void oc_laplc(oc_ctx *c, const double *press, double *alap, int impfl) { laplc_m(c, press, alap, impfl, 0); }
+void oc_laplc32(oc_ctx *c, const double *press, double *alap, int impfl) { laplc_m(c, press, alap, impfl, 1); }
 
 /* prfr(press,ustr,vstr,wstr,uout,vout,wout,impfl), F:1536-1580 */
 void oc_prfr(oc_ctx *c, const double *press, const double *ustr, const double *vstr,

@@ -85,6 +85,10 @@ void oc_set_itr(oc_ctx *c, int itr);
 /* reference subroutines */
 void oc_derivative(oc_ctx *c, const double *unk, double *dpdxa, int idx);
 void oc_laplc(oc_ctx *c, const double *press, double *alap, int impfl);
+/* the FP32 operator of the mixed form (precision 2): laplc with the edge metrics rounded to
+ * FP32 and the scaled gradient rounded to FP32 before the divergence (no reference
+ * counterpart) */
+void oc_laplc32(oc_ctx *c, const double *press, double *alap, int impfl);
 void oc_prfr(oc_ctx *c, const double *press, const double *ustr, const double *vstr,
              const double *wstr, double *uout, double *vout, double *wout, int impfl);
 void oc_bcv(oc_ctx *c, double *u, double *v, double *w, double flag);

@@ -69,6 +69,7 @@ def lib():
         vp = ctypes.c_void_p
         L.oc_derivative.argtypes = [vp, f64p, f64p, ctypes.c_int]
         L.oc_laplc.argtypes = [vp, f64p, f64p, ctypes.c_int]
+        L.oc_laplc32.argtypes = [vp, f64p, f64p, ctypes.c_int]
         L.oc_prfr.argtypes = [vp, f64p, f64p, f64p, f64p, f64p, f64p, f64p, ctypes.c_int]
         L.oc_bcv.argtypes = [vp, f64p, f64p, f64p, ctypes.c_double]
         L.oc_div.argtypes = [vp, f64p, f64p, f64p, f64p, ctypes.c_int]
@@ -177,6 +178,12 @@ class Oracle:
     def laplc(self, p, impfl):
         out = np.zeros(self.n)
         lib().oc_laplc(self.h, _p(np.ascontiguousarray(p, np.float64)), _p(out), impfl)
+        return out
+
+    def laplc32(self, p, impfl):
+        """The mixed form's FP32 operator (metrics and scaled gradient rounded to FP32)."""
+        out = np.zeros(self.n)
+        lib().oc_laplc32(self.h, _p(np.ascontiguousarray(p, np.float64)), _p(out), impfl)
         return out
 
     def bcv(self, u, v, w, flag):

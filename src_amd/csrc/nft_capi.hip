@@ -2029,6 +2029,16 @@ int nft_laplc(nft_ctx *c, const double *press, double *alap, int impfl) {
   return sync(c);
 }
 
+int nft_laplc32(nft_ctx *c, const double *press, double *alap, int impfl) {
+  if (!c || !press || !alap) return set_err(-1, "nft_laplc32: null argument");
+  if (c->P.precision != 2) return set_err(-1, "nft_laplc32: the FP32 operator exists with precision 2 only");
+  int rc;
+  if ((rc = copy_in(c, c->io[0], press))) return rc;
+  if ((rc = laplc32_dev(c, c->io[0], c->io[1], impfl, nullptr))) return rc;
+  if ((rc = copy_out(c, alap, c->io[1]))) return rc;
+  return sync(c);
+}
+
 int nft_laplc_t(nft_ctx *c, const double *y, double *out, int impfl) {
   if (!c || !y || !out) return set_err(-1, "nft_laplc_t: null argument");
   int rc;

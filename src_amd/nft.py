@@ -78,6 +78,7 @@ _SIGS = {
     "nft_derivative": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, ctypes.c_int]),
     "nft_laplc": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, ctypes.c_int]),
     "nft_laplc_t": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, ctypes.c_int]),
+    "nft_laplc32": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, ctypes.c_int]),
     "nft_prfr": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p, f64p, ctypes.c_int]),
     "nft_bcv": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, f64p, ctypes.c_double]),
     "nft_div": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, f64p, f64p, ctypes.c_int]),
@@ -303,6 +304,12 @@ class Solver:
     def laplc(self, p, impfl):
         out = np.zeros(self.n)
         _chk(lib().nft_laplc(self.h, _p(_f64(p)), _p(out), impfl), "nft_laplc")
+        return out
+
+    def laplc32(self, p, impfl):
+        """The FP32 operator of precision 2 (no reference counterpart)."""
+        out = np.zeros(self.n)
+        _chk(lib().nft_laplc32(self.h, _p(_f64(p)), _p(out), impfl), "nft_laplc32")
         return out
 
     def laplc_t(self, y, impfl):

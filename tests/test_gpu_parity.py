@@ -524,3 +524,25 @@ def test_band_preconditioner_steps_bitwise():
     for name in ("unk", "frc", "uuu", "unkdash", "prpot"):
         assert np.array_equal(s.state(name), o.state(name)), name
     s.close()
+
+
+@pytest.mark.parametrize("key", ["tiny", "small", "wave64"])
+@pytest.mark.parametrize("columnar", [1, 0])
+def test_fp32_operator_bitwise(key, columnar):
+    """The FP32 operator that precision 2 applies inside GCR equals the oracle's
+    restatement (oc_laplc32: FP32 edge metrics, scaled gradient rounded to FP32, FP64 sums)
+    bit for bit, with both incidence forms and both impfl; other precisions refuse it."""
+    print(f"case {key} columnar={columnar}", flush=True)
+    m = mesh_for(key)
+    cfg = small_cfg() if key == "small" else tiny_cfg()
+    s = solver(m, cfg, precision="mixed_op")
+    assert s.set_incidence(columnar)
+    o = Oracle(m, cfg)
+    p = seeded_fields(m)["p"]
+    for impfl in (0, 1):
+        assert np.array_equal(s.laplc32(p, impfl), o.laplc32(p, impfl)), impfl
+    s.close()
+    s64 = solver(m, cfg)
+    with pytest.raises(nft.NftError):
+        s64.laplc32(p, 1)
+    s64.close()

@@ -323,3 +323,36 @@ def test_probed_band_is_laplc_and_its_solve_is_exact(key):
     z = o.line_solve(r)
     zd = np.linalg.solve(B, r)
     assert np.abs(z - zd).max() <= 1e-12 * np.abs(zd).max()
+
+
+@pytest.mark.parametrize("key,impfl", [("tiny2", 1), ("tiny2", 0), ("small", 1)])
+def test_fp32_operator_restatement(key, impfl):
+    """The mixed form's FP32 operator in the oracle (oc_laplc32) against its definition
+    composed here from independent pieces: the FP64 derivative of a mesh copy whose edge
+    metrics sn, sb are rounded to FP32 (sbb kept), laplc's scaling, bcv and rho written out
+    in numpy, the scaled gradient rounded to FP32, and the divergence of that mesh copy.
+    Equal bit for bit; and within FP32 rounding of laplc, but not equal to it."""
+    import dataclasses
+    m = mesh_for(key)
+    cfg = tiny_cfg() if key.startswith("tiny") else small_cfg()
+    o = Oracle(m, cfg)
+    f32 = lambda a: np.asarray(a, np.float64).astype(np.float32).astype(np.float64)
+    mr = dataclasses.replace(m, sn=f32(m.sn), sb=f32(m.sb), _handle=None)   # the copy owns no native mesh
+    orr = Oracle(mr, cfg)
+    p = np.random.default_rng(5).standard_normal(m.npoin)
+    g = [orr.derivative(p, k) for k in (1, 2, 3)]
+    dt, v = cfg.dt, m.vol
+    g = [dt * gk / v for gk in g]
+    fz = o.field("coefw") if impfl else np.ones(m.npoin)
+    g = [0.5 * g[0], 0.5 * g[1], 0.5 * g[2] * fz]
+    g = list(o.bcv(g[0], g[1], g[2], 0.0))
+    rho = o.field("rh0")
+    g = [f32(gk * rho) for gk in g]
+    a = [orr.derivative(g[k], k + 1) for k in range(3)]
+    want = (a[0] + a[1] + a[2]) / (v * rho)
+    got = o.laplc32(p, impfl)
+    assert np.array_equal(got, want)
+    ref = o.laplc(p, impfl)
+    err = rel_l2(got, ref)
+    print(f"FP32 operator vs laplc: rel L2 {err:.2e}")
+    assert 0 < err < 1e-5
