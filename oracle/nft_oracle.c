@@ -781,6 +781,40 @@ static void band_column(const oc_ctx *c, int s0, int L, const double *rhs, doubl
   }
 }
 
+/* the same substitution in FP32 (the mixed forms): residual and LU factors rounded to FP32,
+ * every operation in FP32 */
+static void band_column_f32(const oc_ctx *c, int s0, int L, const double *rhs, double *z) {
+  float *w = (float *)malloc(sizeof(float) * (size_t)L);
+  float y1 = 0.f, y2 = 0.f;
+  for (int k = 0; k < L; ++k) {
+    int i = s0 + k;
+    float y = (float)rhs[i];
+    if (k >= 1) y = y - (float)c->fl1[i] * y1;
+    if (k >= 2) y = y - (float)c->fl2[i] * y2;
+    w[k] = y;
+    y2 = y1; y1 = y;
+  }
+  float z1 = 0.f, z2 = 0.f;
+  for (int k = L - 1; k >= 0; --k) {
+    int i = s0 + k;
+    float t = w[k];
+    if (k + 1 < L) t = t - (float)c->fu1[i] * z1;
+    if (k + 2 < L) t = t - (float)c->fu2[i] * z2;
+    t = t * (float)c->fdinv[i];
+    w[k] = t;
+    z2 = z1; z1 = t;
+  }
+  for (int k = 0; k < L; ++k) z[s0 + k] = (double)w[k];
+  free(w);
+}
+
+/* band line solve of the mixed forms (precon 3 only); returns -1 without a band */
+int oc_line_solve32(oc_ctx *c, const double *rhs, double *z) {
+  if (c->band_L <= 0) return -1;
+  for (int k = 0; k < c->m.ncol; ++k) band_column_f32(c, k * c->band_L, c->band_L, rhs, z);
+  return 0;
+}
+
 void oc_line_solve(oc_ctx *c, const double *rhs, double *z) {
   const oc_mesh *m = &c->m;
   if (c->band_L > 0) {

@@ -89,6 +89,9 @@ void oc_laplc(oc_ctx *c, const double *press, double *alap, int impfl);
  * FP32 and the scaled gradient rounded to FP32 before the divergence (no reference
  * counterpart) */
 void oc_laplc32(oc_ctx *c, const double *press, double *alap, int impfl);
+/* the band line solve of the mixed forms (precon 3): residual, LU factors and substitution
+ * in FP32; -1 when there is no band */
+int oc_line_solve32(oc_ctx *c, const double *rhs, double *z);
 void oc_prfr(oc_ctx *c, const double *press, const double *ustr, const double *vstr,
              const double *wstr, double *uout, double *vout, double *wout, int impfl);
 void oc_bcv(oc_ctx *c, double *u, double *v, double *w, double flag);

@@ -70,6 +70,8 @@ def lib():
         L.oc_derivative.argtypes = [vp, f64p, f64p, ctypes.c_int]
         L.oc_laplc.argtypes = [vp, f64p, f64p, ctypes.c_int]
         L.oc_laplc32.argtypes = [vp, f64p, f64p, ctypes.c_int]
+        L.oc_line_solve32.argtypes = [vp, f64p, f64p]
+        L.oc_line_solve32.restype = ctypes.c_int
         L.oc_prfr.argtypes = [vp, f64p, f64p, f64p, f64p, f64p, f64p, f64p, ctypes.c_int]
         L.oc_bcv.argtypes = [vp, f64p, f64p, f64p, ctypes.c_double]
         L.oc_div.argtypes = [vp, f64p, f64p, f64p, f64p, ctypes.c_int]
@@ -207,6 +209,13 @@ class Oracle:
         p = np.zeros(self.n); r = np.zeros(self.n)
         lib().oc_precon(self.h, _p(np.ascontiguousarray(rhs, np.float64)), _p(p), _p(r), impfl)
         return p
+
+    def line_solve32(self, rhs):
+        """The band line solve of the mixed forms (FP32 residual, factors and substitution)."""
+        z = np.zeros(self.n)
+        if lib().oc_line_solve32(self.h, _p(np.ascontiguousarray(rhs, np.float64)), _p(z)) != 0:
+            raise ValueError("line_solve32 needs the band preconditioner (precon 3)")
+        return z
 
     def line_solve(self, rhs):
         z = np.zeros(self.n)

@@ -514,6 +514,25 @@ def test_band_preconditioner_bitwise(key):
     s.close()
 
 
+@pytest.mark.parametrize("key", ["tiny2", "tall", "wave64", "small"])
+def test_band_preconditioner_fp32_bitwise(key):
+    """The mixed forms' band solve (FP32 residual, factors and substitution; k_line_penta
+    <float>) equals the oracle's FP32 restatement bit for bit, and the FP64 solve to FP32
+    rounding."""
+    print(f"case {key}", flush=True)
+    m = mesh_for(key)
+    cfg = (small_cfg if key == "small" else tiny_cfg)(**BAND)
+    s, o = solver(m, cfg, precision="mixed"), Oracle(m, cfg)
+    r = seeded_fields(m)["r"]
+    got = s.precon(r, 1)                      # one sweep from p = 0 with omega = 1: p = M^-1 r
+    want = o.line_solve32(r)
+    assert np.array_equal(got, want)
+    err = rel_l2(got, o.line_solve(r))
+    print(f"FP32 band solve vs FP64: rel L2 {err:.2e}")
+    assert err < 1e-4
+    s.close()
+
+
 def test_band_preconditioner_steps_bitwise():
     m = mesh_for("small")
     cfg = small_cfg(**BAND)

@@ -323,6 +323,12 @@ def test_probed_band_is_laplc_and_its_solve_is_exact(key):
     z = o.line_solve(r)
     zd = np.linalg.solve(B, r)
     assert np.abs(z - zd).max() <= 1e-12 * np.abs(zd).max()
+    # the mixed forms' FP32 substitution: a solve of the same band to FP32 accuracy, and
+    # not the FP64 one
+    z32 = o.line_solve32(r)
+    assert np.linalg.norm(B @ z32 - r) <= 1e-5 * np.linalg.norm(r)
+    assert 0 < np.abs(z32 - zd).max() <= 1e-4 * np.abs(zd).max()
+    assert np.array_equal(z32, z32.astype(np.float32).astype(np.float64))   # FP32 values
 
 
 @pytest.mark.parametrize("key,impfl", [("tiny2", 1), ("tiny2", 0), ("small", 1)])
