@@ -1284,6 +1284,138 @@ void oc_mpdatam(oc_ctx *c, const double *uuu, double *unk, int iflag) {
   free(aun); free(aunb); free(aunbb1);
 }
 
+/* mpdatam of the mixed forms: the loops above in FP32 with the HIP path's operands — the
+ * advector, edge metrics, boundary closures, vol and rh0 rounded to FP32, the FP64 field
+ * rounded to FP32 where it is read, every product and sum in FP32, and each pass's FP32
+ * increment subtracted from the FP64 field (no reference counterpart). */
+static float f32min(float a, float b) { return a < b ? a : b; }
+static float f32max(float a, float b) { return a > b ? a : b; }
+
+void oc_mpdatam32(oc_ctx *c, const double *uuu, double *unk, int iflag) {
+  const oc_mesh *m = &c->m;
+  int n = c->n, E = m->nedge, B = m->nbface, Bp = m->nboun;
+  float *U = (float *)malloc(sizeof(float) * 3 * (size_t)n), *V = U + n, *W = U + 2 * (size_t)n;
+  for (size_t k = 0; k < 3 * (size_t)n; ++k) U[k] = (float)uuu[k];
+  float *vr = (float *)malloc(sizeof(float) * 12 * (size_t)n);
+  float *vl = vr + n, *rh = vr + 2 * (size_t)n, *divv = vr + 3 * (size_t)n, *arh2 = vr + 4 * (size_t)n;
+  float *gx = vr + 5 * (size_t)n, *gy = vr + 6 * (size_t)n, *gz = vr + 7 * (size_t)n;
+  float *mx = vr + 8 * (size_t)n, *mn = vr + 9 * (size_t)n, *rin = vr + 10 * (size_t)n, *rout = vr + 11 * (size_t)n;
+  float *cp = (float *)malloc(sizeof(float) * 2 * (size_t)n), *cn = cp + n;
+  float *aun = (float *)malloc(sizeof(float) * ((size_t)E + B)), *aunb = aun + E;
+  const float dt = (float)c->p.dt, tol = (float)1.e-10;
+  const double afree[5] = {c->p.u0, c->p.v0, c->p.w0, 0., 1.};
+  for (int i = 0; i < n; ++i) {
+    vl[i] = (float)m->vol[i]; rh[i] = (float)c->rh0[i]; vr[i] = vl[i] * rh[i];
+    divv[i] = 0.f; mx[i] = (float)-1.e15; mn[i] = (float)1.e15;
+  }
+  for (int ipass = 1; ipass <= 2; ++ipass) {
+    for (int i = 0; i < n; ++i) arh2[i] = 0.f;
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? E : B;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      const double *S = pass == 0 ? m->sn : m->sb;
+      float *a = pass == 0 ? aun : aunb;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+        float sx = (float)S[e], sy = (float)S[ne + e], sz = (float)S[2 * (size_t)ne + e];
+        float fl;
+        if (ipass == 1) {
+          a[e] = (U[i1] + U[i2]) * 0.5f * sx + (V[i1] + V[i2]) * 0.5f * sy + (W[i1] + W[i2]) * 0.5f * sz;
+          fl = (float)unk[i1] * f32max(0.f, a[e]) + (float)unk[i2] * f32min(0.f, a[e]);
+          divv[i1] = divv[i1] + a[e];
+          divv[i2] = divv[i2] - a[e];
+        } else {
+          fl = a[e];
+        }
+        arh2[i1] = arh2[i1] + fl;
+        arh2[i2] = arh2[i2] - fl;
+      }
+    }
+    if (ipass == 1) {
+      for (int b = 0; b < Bp; ++b) {
+        int i1 = m->ibpoin[b] - 1, f = m->ibpoin[Bp + b];
+        float aunbb1 = -divv[i1];
+        float af = iflag == 1 ? (float)c->ue[i1] : (float)afree[iflag - 1];
+        float fl1 = (f == 11 || f == 5 || f == 6) ? 0.f
+                                                  : af * f32min(0.f, aunbb1) + (float)unk[i1] * f32max(0.f, aunbb1);
+        arh2[i1] = arh2[i1] + fl1 * 1.f;
+      }
+    } else {
+      for (int b = 0; b < Bp; ++b) { int i1 = m->ibpoin[b] - 1; arh2[i1] = arh2[i1] + 0.f * 1.f; }
+    }
+    for (int i = 0; i < n; ++i) {
+      float inc = arh2[i] * dt / vl[i] / rh[i];
+      unk[i] = unk[i] - (double)inc;
+    }
+    if (ipass == 2) break;
+
+    /* gradients of the donor-cell field */
+    for (int i = 0; i < n; ++i) { gx[i] = 0.f; gy[i] = 0.f; gz[i] = 0.f; }
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? E : B;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      const double *S = pass == 0 ? m->sn : m->sb;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+        float sx = (float)S[e], sy = (float)S[ne + e], sz = (float)S[2 * (size_t)ne + e];
+        float u = ((float)unk[i1] + (float)unk[i2]) * 0.5f;
+        float tx = sx * u, ty = sy * u, tz = sz * u;
+        gx[i1] = gx[i1] + tx; gy[i1] = gy[i1] + ty; gz[i1] = gz[i1] + tz;
+        gx[i2] = gx[i2] - tx; gy[i2] = gy[i2] - ty; gz[i2] = gz[i2] - tz;
+      }
+    }
+    for (int b = 0; b < Bp; ++b) {
+      int i1 = m->ibpoin[b] - 1;
+      float pi = (float)unk[i1];
+      gx[i1] = gx[i1] + (float)m->sbb[b] * pi;
+      gy[i1] = gy[i1] + (float)m->sbb[Bp + b] * pi;
+      gz[i1] = gz[i1] + (float)m->sbb[2 * (size_t)Bp + b] * pi;
+    }
+    /* antidiffusive velocities, FCT bounds, in/out sums */
+    for (int i = 0; i < n; ++i) { rin[i] = 0.f; rout[i] = 0.f; }
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? E : B;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      float *a = pass == 0 ? aun : aunb;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+        float p1 = (float)unk[i1], p2 = (float)unk[i2];
+        float un = fabsf(a[e]) * (p2 - p1) * 0.5f;
+        float volt = 0.5f * (vr[i1] + vr[i2]);
+        float dpdx = 0.5f * (gx[i1] + gx[i2]) / volt;
+        float dpdy = 0.5f * (gy[i1] + gy[i2]) / volt;
+        float dpdz = 0.5f * (gz[i1] + gz[i2]) / volt;
+        float term = (U[i1] + U[i2]) * 0.5f * dpdx + (V[i1] + V[i2]) * 0.5f * dpdy + (W[i1] + W[i2]) * 0.5f * dpdz;
+        float A = un - a[e] * term * dt * 0.5f;
+        a[e] = A;
+        mx[i1] = f32max(f32max(mx[i1], p1), p2);
+        mn[i1] = f32min(f32min(mn[i1], p1), p2);
+        mx[i2] = f32max(f32max(mx[i2], p2), p1);
+        mn[i2] = f32min(f32min(mn[i2], p2), p1);
+        float pp = f32max(0.f, A), pn = -f32min(0.f, A);
+        rin[i1] = rin[i1] + pn; rout[i1] = rout[i1] + pp;
+        rin[i2] = rin[i2] + pp; rout[i2] = rout[i2] + pn;
+      }
+    }
+    for (int i = 0; i < n; ++i) {
+      float pi = (float)unk[i];
+      cp[i] = (mx[i] - pi) * vl[i] * rh[i] / (rin[i] * dt + tol);
+      cn[i] = (pi - mn[i]) * vl[i] * rh[i] / (rout[i] * dt + tol);
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? E : B;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      float *a = pass == 0 ? aun : aunb;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+        if (a[e] > 0.f) a[e] = a[e] * f32min(f32min(1.f, cp[i2]), cn[i1]);
+        else a[e] = a[e] * f32min(f32min(1.f, cn[i2]), cp[i1]);
+      }
+    }
+  }
+  free(U); free(vr); free(cp); free(aun);
+}
+
 /* ---- per-step diagnostics (F:456-459), in the reference's loop order -------------------
  * out[25]: div(uuu,rhs,1) statistics of rhs*timestp (F:2148-2164: dvmx, dvmn, dvav, dvsd),
  * courant(uuu,cour,flag) (F:1974-2094: crnmx, crnmn, crnav, crnsd, ic), stats(unk)

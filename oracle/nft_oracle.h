@@ -92,6 +92,9 @@ void oc_laplc32(oc_ctx *c, const double *press, double *alap, int impfl);
 /* the band line solve of the mixed forms (precon 3): residual, LU factors and substitution
  * in FP32; -1 when there is no band */
 int oc_line_solve32(oc_ctx *c, const double *rhs, double *z);
+/* mpdatam of the mixed forms: FP32 arithmetic on FP32 roundings of the advector, metrics,
+ * closures, vol and rh0, FP32 increments subtracted from the FP64 field */
+void oc_mpdatam32(oc_ctx *c, const double *uuu, double *unk, int iflag);
 void oc_prfr(oc_ctx *c, const double *press, const double *ustr, const double *vstr,
              const double *wstr, double *uout, double *vout, double *wout, int impfl);
 void oc_bcv(oc_ctx *c, double *u, double *v, double *w, double flag);

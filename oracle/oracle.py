@@ -78,6 +78,7 @@ def lib():
         L.oc_precon.argtypes = [vp, f64p, f64p, f64p, ctypes.c_int]
         L.oc_gcrk.argtypes = [vp, f64p, f64p, f64p, f64p, f64p, f64p, f64p, ctypes.c_int]
         L.oc_mpdatam.argtypes = [vp, f64p, f64p, ctypes.c_int]
+        L.oc_mpdatam32.argtypes = [vp, f64p, f64p, ctypes.c_int]
         L.oc_line_solve.argtypes = [vp, f64p, f64p]
         L.oc_nft_init.argtypes = [vp]
         L.oc_nft_step.argtypes = [vp]
@@ -228,6 +229,13 @@ class Oracle:
         ins = [np.ascontiguousarray(a, np.float64) for a in (u1, v1, w1)]
         lib().oc_gcrk(self.h, _p(p), *[_p(a) for a in ins], *[_p(a) for a in outs], impfl)
         return p, outs
+
+    def mpdatam32(self, uuu, unk, iflag):
+        """mpdatam of the mixed forms (FP32 arithmetic, FP64 field)."""
+        unk = np.array(unk, dtype=np.float64)
+        uuu = np.ascontiguousarray(uuu, np.float64)
+        lib().oc_mpdatam32(self.h, _p(uuu), _p(unk), iflag)
+        return unk
 
     def mpdatam(self, uuu, unk, iflag):
         unk = np.array(unk, dtype=np.float64)

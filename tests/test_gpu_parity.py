@@ -565,3 +565,21 @@ def test_fp32_operator_bitwise(key, columnar):
     with pytest.raises(nft.NftError):
         s64.laplc32(p, 1)
     s64.close()
+
+
+@pytest.mark.parametrize("key", ["tiny", "small"])
+@pytest.mark.parametrize("columnar", [1, 0])
+def test_fp32_mpdatam_bitwise(key, columnar):
+    """The mixed forms' MPDATA (k_mp_*<float>: FP32 advector, metrics, vol, rho and
+    arithmetic, FP64 fields) equals the oracle's FP32 restatement bit for bit, every
+    iflag, with both incidence forms."""
+    print(f"case {key} columnar={columnar}", flush=True)
+    m = mesh_for(key)
+    cfg = small_cfg() if key == "small" else tiny_cfg()
+    s = solver(m, cfg, precision="mixed")
+    assert s.set_incidence(columnar)
+    o = Oracle(m, cfg)
+    f = seeded_fields(m)
+    for iflag, name in ((1, "u"), (2, "v"), (3, "w"), (4, "p")):
+        assert np.array_equal(s.mpdatam(f["uuu"], f[name], iflag), o.mpdatam32(f["uuu"], f[name], iflag)), iflag
+    s.close()

@@ -5,7 +5,7 @@ import os
 import numpy as np
 import pytest
 
-from common import mesh_checksum, mesh_for, small_cfg, tiny_cfg, rel_l2
+from common import mesh_checksum, mesh_for, seeded_fields, small_cfg, tiny_cfg, rel_l2
 from oracle import Oracle
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -362,3 +362,21 @@ def test_fp32_operator_restatement(key, impfl):
     err = rel_l2(got, ref)
     print(f"FP32 operator vs laplc: rel L2 {err:.2e}")
     assert 0 < err < 1e-5
+
+
+@pytest.mark.parametrize("key", ["tiny2", "small"])
+def test_fp32_mpdatam_restatement_tracks_fp64(key):
+    """The mixed forms' MPDATA restatement (oc_mpdatam32, FP32 arithmetic) stays within FP32
+    rounding of the FP64 mpdatam, which is the reference's bit for bit: a dropped term, a
+    wrong sign or a transposed operand moves the result by orders of magnitude more. Every
+    inflow treatment (iflag 1-4)."""
+    m = mesh_for(key)
+    o = Oracle(m, small_cfg() if key == "small" else tiny_cfg())
+    f = seeded_fields(m)
+    for iflag, name in ((1, "u"), (2, "v"), (3, "w"), (4, "p")):
+        a = o.mpdatam(f["uuu"], f[name], iflag)
+        b = o.mpdatam32(f["uuu"], f[name], iflag)
+        moved = rel_l2(a, f[name])
+        err = rel_l2(b, a)
+        print(f"iflag {iflag}: FP32 vs FP64 {err:.2e} (the step moves the field by {moved:.2e})")
+        assert 0 < err <= 2e-7 and moved > 1e3 * err
