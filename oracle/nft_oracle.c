@@ -872,7 +872,8 @@ static void fine_sweep(oc_ctx *c, const double *rhs, double *p, double *r, int i
   } else {
     double *res = c->s[15], *z = c->s[2];
     for (int i = 0; i < n; ++i) res[i] = rhs[i] - r[i];
-    oc_line_solve(c, res, z);
+    if (c->p.precision >= 1 && c->band_L > 0) oc_line_solve32(c, res, z);
+    else oc_line_solve(c, res, z);
     for (int i = 0; i < n; ++i) p[i] = p[i] + om * z[i];
   }
 }
@@ -1597,7 +1598,10 @@ void oc_nft_step(oc_ctx *c) {
     if (!traj) { uuu0[i] = uuu[i]; uuu0[n + i] = uuu[n + i]; uuu0[2 * n + i] = uuu[2 * n + i]; }
     for (int k = 0; k < 4; ++k) unk[k * n + i] = unk[k * n + i] + 0.5 * frc[k * n + i] * dt;
   }
-  for (int k = 0; k < 4; ++k) oc_mpdatam(c, uuu, unk + (size_t)k * n, k + 1);
+  for (int k = 0; k < 4; ++k) {
+    if (c->p.precision >= 1) oc_mpdatam32(c, uuu, unk + (size_t)k * n, k + 1);
+    else oc_mpdatam(c, uuu, unk + (size_t)k * n, k + 1);
+  }
   for (int mtri = 1; mtri <= 2; ++mtri) {
     for (int i = 0; i < n; ++i) {
       double frm1 = 0. - al[i] * (ud[i] - ue[i]);

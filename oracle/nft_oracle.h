@@ -68,6 +68,9 @@ typedef struct {
   /* advector at n+1/2: 0 extrapolation 1.5 rho u^n - 0.5 u^(n-1) (F:427-438); 1 the
    * first-order prediction velprd (F:4448-4531), which adds a GCR solve per step */
   int itraj;
+  /* 0 FP64; 1 the mixed form: MPDATA as oc_mpdatam32 and, with the band line (precon 3),
+   * the line solve as oc_line_solve32; everything else FP64 */
+  int precision;
 } oc_params;
 
 typedef struct { int niter, nitsm, icount; double eer, eem; } oc_stats;

@@ -181,8 +181,8 @@ def _steps(m, cfg, nsteps, **kw):
     return s, its
 
 
-def _oracle_steps(m, cfg, nsteps, dot_order=1):
-    o = Oracle(m, cfg, dot_order=dot_order)
+def _oracle_steps(m, cfg, nsteps, dot_order=1, precision=0):
+    o = Oracle(m, cfg, dot_order=dot_order, precision=precision)
     o.init()
     its = [o.stats().niter]
     for _ in range(nsteps):
@@ -582,4 +582,21 @@ def test_fp32_mpdatam_bitwise(key, columnar):
     f = seeded_fields(m)
     for iflag, name in ((1, "u"), (2, "v"), (3, "w"), (4, "p")):
         assert np.array_equal(s.mpdatam(f["uuu"], f[name], iflag), o.mpdatam32(f["uuu"], f[name], iflag)), iflag
+    s.close()
+
+
+@pytest.mark.parametrize("key,nsteps", [("tiny", 10), ("small", 3)])
+def test_mixed_steps_bitwise(key, nsteps):
+    """The mixed form (precision 1: FP32 MPDATA and FP32 band solve, FP64 GCR and laplc):
+    the initial projection and full time steps equal the oracle's mixed form (oc_mpdatam32,
+    oc_line_solve32, the GPU's summation order) bit for bit, with equal GCR iteration counts."""
+    print(f"case {key}", flush=True)
+    m = mesh_for(key)
+    cfg = (small_cfg if key == "small" else tiny_cfg)(**BAND)
+    s, its = _steps(m, cfg, nsteps, precision="mixed")
+    o, ito = _oracle_steps(m, cfg, nsteps, precision=1)
+    print("iterations", its, ito)
+    assert its == ito
+    for name in ("unk", "frc", "uuu", "unkdash", "prpot"):
+        assert np.array_equal(s.state(name), o.state(name)), name
     s.close()
