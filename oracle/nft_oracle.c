@@ -1058,8 +1058,14 @@ void oc_gcrk(oc_ctx *c, double *p, const double *u1, const double *v1, const dou
   double *r = c->gr, *qr = c->gqr, *ar = c->gar, *x = c->gx, *ax = c->gax;
   double ax2[64], axar[64], del[64];
   double epa = 1.e-38;
-  int nlc = 0, itmn = 3;
+  int nlc = 0;
   double eps = c->p.eps0 / c->p.dt;
+  /* precision 2: the iterations apply the FP32 operator and the solve runs in passes,
+   * each from the FP64 residual of the current pressure, ending when the recursive
+   * residual has dropped by mixed_drop (or met the tolerance); a pass after the first
+   * stops the solve when that FP64 residual meets the tolerance */
+  const int op32 = c->p.precision == 2;
+  const double drop = c->p.mixed_drop > 0 ? c->p.mixed_drop : 1.e-3;
   for (int k = 0; k < n; ++k) { r[k] = 0.; ar[k] = 0.; qr[k] = 0.; }
   for (size_t k = 0; k < (size_t)lord * n; ++k) { x[k] = 0.; ax[k] = 0.; }
   double bmx = 0.;
@@ -1071,94 +1077,108 @@ void oc_gcrk(oc_ctx *c, double *p, const double *u1, const double *v1, const dou
     for (int k = 0; k < n; ++k) bmx = amax1(bmx, fabs(z[k]));
     eps = c->p.eps0 * bmx;
   }
-  oc_prfr(c, p, u1, v1, w1, u2, v2, w2, impfl);
-  oc_div(c, u2, v2, w2, r, -1);
-  oc_precon(c, r, qr, ar, impfl);
-  double eer0 = 0., eem0 = -1.e15, rl20 = 0.;
   const int dev = c->p.dot_order == 1;
-  for (int k = 0; k < n; ++k) {
-    eer0 = eer0 + qr[k] * qr[k];
-    eem0 = amax1(eem0, fabs(qr[k]));
-    rl20 = rl20 + r[k] * r[k];
-  }
-  if (dev) {
-    eer0 = dsum_stride(c, qr, qr);
-    rl20 = dsum_gather(c, NULL, r);
-  }
-  eer0 = amax1(eer0, epa);
-  eem0 = amax1(eem0, epa);
-  rl20 = amax1(rl20, epa);
-  for (int k = 0; k < n; ++k) x[k] = qr[k];
-  oc_laplc(c, x, ax, impfl);
-  for (int it = 1; it <= c->p.itr; ++it) {
-    for (int i = 1; i <= lord; ++i) {
-      double *xi = x + (size_t)(i - 1) * n, *axi = ax + (size_t)(i - 1) * n;
-      double rax = 0.;
-      ax2[i - 1] = 0.;
-      if (!dev) {
-        for (int k = 0; k < n; ++k) {
-          rax = rax + r[k] * axi[k];
-          ax2[i - 1] = ax2[i - 1] + axi[k] * axi[k];
-        }
-      } else if (it == 1 && i == 1) {   /* fused into the divergence kernel of laplc */
-        rax = dsum_gather(c, r, axi);
-        ax2[i - 1] = dsum_gather(c, NULL, axi);
-      } else {                          /* fused into the direction kernel */
-        rax = dsum_stride(c, r, axi);
-        ax2[i - 1] = dsum_stride(c, axi, axi);
-      }
-      ax2[i - 1] = amax1(epa, ax2[i - 1]);
-      double beta = -rax / ax2[i - 1];
-      double dvmx = -1.e15, rl2 = 0.;
+  double eer0 = 0., eem0 = -1.e15;
+  for (int pass = 0;; ++pass) {
+    /* u2, v2, w2: prfr of the incoming pressure; later passes use scratch arrays */
+    double *pu = pass ? c->gb[0] : u2, *pv = pass ? c->gb[1] : v2, *pw = pass ? c->gb[2] : w2;
+    oc_prfr(c, p, u1, v1, w1, pu, pv, pw, impfl);
+    oc_div(c, pu, pv, pw, r, -1);
+    double rmx = 0.;
+    for (int k = 0; k < n; ++k) rmx = amax1(rmx, fabs(r[k]));
+    if (pass > 0 && (rmx <= eps || !(rmx == rmx) || pass >= 24)) break;
+    oc_precon(c, r, qr, ar, impfl);
+    double rl20 = 0.;
+    if (pass == 0) {
       for (int k = 0; k < n; ++k) {
-        p[k] = p[k] + beta * xi[k];
-        r[k] = r[k] + beta * axi[k];
-        dvmx = amax1(dvmx, fabs(r[k]));
-        rl2 = rl2 + r[k] * r[k];
+        eer0 = eer0 + qr[k] * qr[k];
+        eem0 = amax1(eem0, fabs(qr[k]));
       }
-      if (dev) rl2 = dsum_stride(c, r, r);
-      if (dvmx <= eps && it >= itmn) goto done;
-      if (rl2 >= rl20) goto done;
-      rl20 = amax1(rl2, epa);
-      oc_precon(c, r, qr, ar, impfl);
-      oc_laplc(c, qr, ar, impfl);
-      nlc = nlc + 1;
-      for (int ii = 1; ii <= i; ++ii) {
-        double *axii = ax + (size_t)(ii - 1) * n;
-        axar[ii - 1] = 0.;
-        if (dev) axar[ii - 1] = dsum_gather(c, axii, ar);
-        else
-          for (int k = 0; k < n; ++k) axar[ii - 1] = axar[ii - 1] + axii[k] * ar[k];
-        del[ii - 1] = -axar[ii - 1] / ax2[ii - 1];
-      }
-      if (i < lord) {
-        double *xn = x + (size_t)i * n, *axn = ax + (size_t)i * n;
-        for (int k = 0; k < n; ++k) { xn[k] = qr[k]; axn[k] = ar[k]; }
-        for (int ii = 1; ii <= i; ++ii) {
-          double *xii = x + (size_t)(ii - 1) * n, *axii = ax + (size_t)(ii - 1) * n;
+      if (dev) eer0 = dsum_stride(c, qr, qr);
+      eer0 = amax1(eer0, epa);
+      eem0 = amax1(eem0, epa);
+    }
+    for (int k = 0; k < n; ++k) rl20 = rl20 + r[k] * r[k];
+    if (dev) rl20 = dsum_gather(c, NULL, r);
+    rl20 = amax1(rl20, epa);
+    for (int k = 0; k < n; ++k) x[k] = qr[k];
+    if (op32) oc_laplc32(c, x, ax, impfl);
+    else oc_laplc(c, x, ax, impfl);
+    const double eps_pass = op32 ? amax1(eps, drop * rmx) : eps;
+    const int itmn = pass == 0 ? 3 : 1;
+    int conv = 0;
+    for (int it = 1; it <= c->p.itr; ++it) {
+      for (int i = 1; i <= lord; ++i) {
+        double *xi = x + (size_t)(i - 1) * n, *axi = ax + (size_t)(i - 1) * n;
+        double rax = 0.;
+        ax2[i - 1] = 0.;
+        if (!dev) {
           for (int k = 0; k < n; ++k) {
-            xn[k] = xn[k] + del[ii - 1] * xii[k];
-            axn[k] = axn[k] + del[ii - 1] * axii[k];
+            rax = rax + r[k] * axi[k];
+            ax2[i - 1] = ax2[i - 1] + axi[k] * axi[k];
           }
+        } else if (it == 1 && i == 1) {   /* fused into the divergence kernel of laplc */
+          rax = dsum_gather(c, r, axi);
+          ax2[i - 1] = dsum_gather(c, NULL, axi);
+        } else {                          /* fused into the direction kernel */
+          rax = dsum_stride(c, r, axi);
+          ax2[i - 1] = dsum_stride(c, axi, axi);
         }
-      } else {
+        ax2[i - 1] = amax1(epa, ax2[i - 1]);
+        double beta = -rax / ax2[i - 1];
+        double dvmx = -1.e15, rl2 = 0.;
         for (int k = 0; k < n; ++k) {
-          x[k] = qr[k] + del[0] * x[k];
-          ax[k] = ar[k] + del[0] * ax[k];
+          p[k] = p[k] + beta * xi[k];
+          r[k] = r[k] + beta * axi[k];
+          dvmx = amax1(dvmx, fabs(r[k]));
+          rl2 = rl2 + r[k] * r[k];
         }
-        for (int ii = 2; ii <= i; ++ii) {
-          double *xii = x + (size_t)(ii - 1) * n, *axii = ax + (size_t)(ii - 1) * n;
+        if (dev) rl2 = dsum_stride(c, r, r);
+        if (dvmx <= eps_pass && it >= itmn) { conv = 1; goto pass_end; }
+        if (rl2 >= rl20) goto pass_end;
+        rl20 = amax1(rl2, epa);
+        oc_precon(c, r, qr, ar, impfl);
+        if (op32) oc_laplc32(c, qr, ar, impfl);
+        else oc_laplc(c, qr, ar, impfl);
+        nlc = nlc + 1;
+        for (int ii = 1; ii <= i; ++ii) {
+          double *axii = ax + (size_t)(ii - 1) * n;
+          axar[ii - 1] = 0.;
+          if (dev) axar[ii - 1] = dsum_gather(c, axii, ar);
+          else
+            for (int k = 0; k < n; ++k) axar[ii - 1] = axar[ii - 1] + axii[k] * ar[k];
+          del[ii - 1] = -axar[ii - 1] / ax2[ii - 1];
+        }
+        if (i < lord) {
+          double *xn = x + (size_t)i * n, *axn = ax + (size_t)i * n;
+          for (int k = 0; k < n; ++k) { xn[k] = qr[k]; axn[k] = ar[k]; }
+          for (int ii = 1; ii <= i; ++ii) {
+            double *xii = x + (size_t)(ii - 1) * n, *axii = ax + (size_t)(ii - 1) * n;
+            for (int k = 0; k < n; ++k) {
+              xn[k] = xn[k] + del[ii - 1] * xii[k];
+              axn[k] = axn[k] + del[ii - 1] * axii[k];
+            }
+          }
+        } else {
           for (int k = 0; k < n; ++k) {
-            x[k] = x[k] + del[ii - 1] * xii[k];
-            xii[k] = 0.;
-            ax[k] = ax[k] + del[ii - 1] * axii[k];
-            axii[k] = 0.;
+            x[k] = qr[k] + del[0] * x[k];
+            ax[k] = ar[k] + del[0] * ax[k];
+          }
+          for (int ii = 2; ii <= i; ++ii) {
+            double *xii = x + (size_t)(ii - 1) * n, *axii = ax + (size_t)(ii - 1) * n;
+            for (int k = 0; k < n; ++k) {
+              x[k] = x[k] + del[ii - 1] * xii[k];
+              xii[k] = 0.;
+              ax[k] = ax[k] + del[ii - 1] * axii[k];
+              axii[k] = 0.;
+            }
           }
         }
       }
     }
+  pass_end:
+    if (!op32 || !conv) break;
   }
-done:;
   double eer = 0., eem = -1.e15;
   for (int k = 0; k < n; ++k) { eer = eer + qr[k] * qr[k]; eem = amax1(eem, fabs(qr[k])); }
   if (dev) eer = dsum_stride(c, qr, qr);

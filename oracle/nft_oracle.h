@@ -69,8 +69,10 @@ typedef struct {
    * first-order prediction velprd (F:4448-4531), which adds a GCR solve per step */
   int itraj;
   /* 0 FP64; 1 the mixed form: MPDATA as oc_mpdatam32 and, with the band line (precon 3),
-   * the line solve as oc_line_solve32; everything else FP64 */
+   * the line solve as oc_line_solve32; everything else FP64; 2 as 1, with the FP32
+   * operator (oc_laplc32) inside gcrk and the solve in passes from the FP64 residual */
   int precision;
+  double mixed_drop;   /* precision 2: recursive-residual drop that ends a pass (0: 1e-3) */
 } oc_params;
 
 typedef struct { int niter, nitsm, icount; double eer, eem; } oc_stats;

@@ -36,7 +36,7 @@ class OcParams(ctypes.Structure):
                 ("towxl", ctypes.c_double), ("towxr", ctypes.c_double), ("towyd", ctypes.c_double),
                 ("towyu", ctypes.c_double), ("towzd", ctypes.c_double), ("towzu", ctypes.c_double),
                 ("irl", ctypes.c_int), ("dom", ctypes.c_double * 6), ("dot_order", ctypes.c_int),
-                ("itraj", ctypes.c_int), ("precision", ctypes.c_int)]
+                ("itraj", ctypes.c_int), ("precision", ctypes.c_int), ("mixed_drop", ctypes.c_double)]
 
 
 class OcStats(ctypes.Structure):
@@ -108,7 +108,8 @@ class Oracle:
     """FP64 CPU restatement bound to one mesh and run configuration."""
 
     def __init__(self, mesh, cfg, precon=None, sm_flag=1, itrp1=None, omega=None, eps0=None,
-                 exit_mode=None, itr=None, lord=None, use_columns=True, dot_order=0, itraj=0, precision=0):
+                 exit_mode=None, itr=None, lord=None, use_columns=True, dot_order=0, itraj=0, precision=0,
+                 mixed_drop=0.0):
         L = lib()
         self.mesh = mesh
         self.n = mesh.npoin
@@ -138,6 +139,7 @@ class Oracle:
         p.dot_order = dot_order
         p.itraj = itraj
         p.precision = precision
+        p.mixed_drop = mixed_drop
         self._m, self._params = m, p
         self.h = L.oc_create(ctypes.byref(m), ctypes.byref(p))
 

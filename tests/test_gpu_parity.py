@@ -585,16 +585,31 @@ def test_fp32_mpdatam_bitwise(key, columnar):
     s.close()
 
 
-@pytest.mark.parametrize("key,nsteps", [("tiny", 10), ("small", 3)])
-def test_mixed_steps_bitwise(key, nsteps):
-    """The mixed form (precision 1: FP32 MPDATA and FP32 band solve, FP64 GCR and laplc):
-    the initial projection and full time steps equal the oracle's mixed form (oc_mpdatam32,
-    oc_line_solve32, the GPU's summation order) bit for bit, with equal GCR iteration counts."""
-    print(f"case {key}", flush=True)
+@pytest.mark.parametrize("key,nsteps,precision,drop", [("tiny", 10, 1, 0.0), ("small", 3, 1, 0.0),
+                                                       ("small", 3, 2, 0.0), ("small", 3, 2, 1e-4)])
+def test_mixed_steps_bitwise(key, nsteps, precision, drop):
+    """The mixed forms — precision 1: FP32 MPDATA and FP32 band solve, FP64 GCR and laplc;
+    precision 2: also the FP32 operator inside GCR and the solves in passes from the FP64
+    residual (default pass threshold and 1e-4): the initial projection and full time steps
+    equal the oracle's mixed forms (oc_mpdatam32, oc_line_solve32, oc_laplc32, the same
+    passes, the GPU's summation order) bit for bit, with equal GCR iteration counts."""
+    print(f"case {key} precision={precision} drop={drop}", flush=True)
     m = mesh_for(key)
     cfg = (small_cfg if key == "small" else tiny_cfg)(**BAND)
-    s, its = _steps(m, cfg, nsteps, precision="mixed")
-    o, ito = _oracle_steps(m, cfg, nsteps, precision=1)
+    s = solver(m, cfg, precision=precision)
+    if drop:
+        s.set_mixed_drop(drop)
+    s.state_init()
+    its = [s.gcr_stats().niter]
+    for _ in range(nsteps):
+        s.run_steps(1)
+        its.append(s.gcr_stats().niter)
+    o = Oracle(m, cfg, dot_order=1, precision=precision, mixed_drop=drop)
+    o.init()
+    ito = [o.stats().niter]
+    for _ in range(nsteps):
+        o.step()
+        ito.append(o.stats().niter)
     print("iterations", its, ito)
     assert its == ito
     for name in ("unk", "frc", "uuu", "unkdash", "prpot"):
