@@ -72,6 +72,8 @@ struct nft_ctx {
   double *vol, *rh0, *th0, *ue, *ve, *coefw, *alpha, *d1, *dthedz, *pstep;
   NodeCoef nc{};
   float *volf, *rh0f;
+  double *vr = nullptr;       // vol*rh0 (MPDATA)
+  float *vrf = nullptr;       // the same product of the FP32 copies
   int32_t *col_start;
   double *la, *linv, *lc;
   float *laf, *linvf, *lcf;
@@ -1255,18 +1257,16 @@ int mpdata_dev(nft_ctx *c, const T *U, const T *V, const T *W, double *const *fi
   for (int f = 0; f < NF; ++f) {
     a.psi[f] = fields[f];
     a.psi1[f] = c->mp_psi1[f];
-    a.gx[f] = (T *)c->mp_g[3 * f];
-    a.gy[f] = (T *)c->mp_g[3 * f + 1];
+    a.gxy[f] = (typename MpArgs<T, NF>::T2 *)c->mp_g[3 * f];   // 2n values
     a.gz[f] = (T *)c->mp_g[3 * f + 2];
-    a.cp[f] = (T *)c->mp_c[2 * f];
-    a.cn[f] = (T *)c->mp_c[2 * f + 1];
+    a.cpn[f] = (typename MpArgs<T, NF>::T2 *)c->mp_c[2 * f];   // 2n values
     int fl = iflags[f];
     double afree[5] = {c->P.u0, c->P.v0, c->P.w0, 0., 1.};
     a.use_ue[f] = fl == 1;
     a.afree[f] = (T)afree[fl - 1];
   }
-  if (sizeof(T) == 8) { a.vol = (const T *)c->vol; a.rh0 = (const T *)c->rh0; }
-  else { a.vol = (const T *)c->volf; a.rh0 = (const T *)c->rh0f; }
+  if (sizeof(T) == 8) { a.vol = (const T *)c->vol; a.rh0 = (const T *)c->rh0; a.vr = (const T *)c->vr; }
+  else { a.vol = (const T *)c->volf; a.rh0 = (const T *)c->rh0f; a.vr = (const T *)c->vrf; }
   a.ue = c->ue;
   a.dt = (T)c->P.dt;
   EdgeS<T> S;
@@ -1876,6 +1876,11 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   c->nc.coefw_uniform =
       std::all_of(c->h_coefw.begin(), c->h_coefw.end(), [&](double v) { return v == c->nc.coefw0; }) ? 1 : 0;
   if ((rc = dalloc(c, &c->zbuf, n))) return fail(rc);
+  {
+    std::vector<double> vr(n);
+    for (int i = 0; i < n; ++i) vr[i] = volv[i] * rh0[i];
+    if ((rc = upload(c, &c->vr, vr.data(), n))) return fail(rc);
+  }
   // FP32 copies for the mixed-precision form
   if (P->precision >= 1) {
     auto tof = [](const std::vector<double> &v) { std::vector<float> f(v.size()); for (size_t i = 0; i < v.size(); ++i) f[i] = (float)v[i]; return f; };
@@ -1885,6 +1890,14 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
         (rc = upload(c, &fz, (f = tof(sz)).data(), etot)))
       return fail(rc);
     c->Sf = EdgeS<float>{fx, fy, fz};
+    {
+      std::vector<float> vrf(n);
+      for (int i = 0; i < n; ++i) {
+        const float a = (float)volv[i], b = (float)rh0[i];
+        vrf[i] = a * b;
+      }
+      if ((rc = upload(c, &c->vrf, vrf.data(), n))) return fail(rc);
+    }
     if ((rc = upload(c, &c->volf, (f = tof(volv)).data(), n)) || (rc = upload(c, &c->rh0f, (f = tof(rh0)).data(), n)) ||
         (rc = upload(c, &c->laf, (f = tof(c->h_la)).data(), n)) || (rc = upload(c, &c->linvf, (f = tof(c->h_linv)).data(), n)) ||
         (rc = upload(c, &c->lcf, (f = tof(c->h_lc)).data(), n)) || (rc = dalloc(c, &c->zbuff, n)))
@@ -1922,14 +1935,17 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   const size_t tsz = P->precision >= 1 ? sizeof(float) : sizeof(double);
   for (int k = 0; k < 4; ++k)
     if ((rc = dalloc(c, &c->mp_psi1[k], n))) return fail(rc);
+  // MPDATA work arrays per field: gradient (x, y) pairs (2n) and z (n), the (cp, cn) pairs
+  // (2n); slots 3f+1 and 2f+1 are not used by the sweeps (the single-field mpdatam entry
+  // keeps its FP32 advector in slots 9..11)
   for (int k = 0; k < 12; ++k) {
     void *q;
-    if (hipMalloc(&q, n * tsz) != hipSuccess) return fail(set_err(-2, "alloc"));
+    if (hipMalloc(&q, (k % 3 == 0 ? 2 : 1) * n * tsz) != hipSuccess) return fail(set_err(-2, "alloc"));
     c->allocs.push_back(q); c->mp_g[k] = q;
   }
   for (int k = 0; k < 8; ++k) {
     void *q;
-    if (hipMalloc(&q, n * tsz) != hipSuccess) return fail(set_err(-2, "alloc"));
+    if (hipMalloc(&q, (k % 2 == 0 ? 2 : 1) * n * tsz) != hipSuccess) return fail(set_err(-2, "alloc"));
     c->allocs.push_back(q); c->mp_c[k] = q;
   }
   for (int k = 0; k < 7; ++k)
@@ -2276,8 +2292,9 @@ static double kernel_bytes(nft_ctx *c, int k) {
     case KT_DIR: return (3 + 2.0 * c->P.lord) * 8 * N;  /* qr, ar, r + up to lord x/ax; writes 2 */
     case KT_MP0: return (mixed ? geomf : geom) + 3 * ts * N + 4 * 8 * N + 4 * 8 * N + 2 * ts * N + 8 * N;
     case KT_MP0 + 1: return (mixed ? geomf : geom) + 4 * 8 * N + 12 * ts * N;
-    case KT_MP0 + 2: return (mixed ? geomf : geom) + 3 * ts * N + 4 * 8 * N + 12 * ts * N + 2 * ts * N + 8 * ts * N;
-    case KT_MP0 + 3: return (mixed ? geomf : geom) + 3 * ts * N + 4 * 8 * N + 12 * ts * N + 2 * ts * N + 8 * ts * N + 4 * 8 * N;
+    // fct and limit read vol, rh0 and their product vr (3 arrays)
+    case KT_MP0 + 2: return (mixed ? geomf : geom) + 3 * ts * N + 4 * 8 * N + 12 * ts * N + 3 * ts * N + 8 * ts * N;
+    case KT_MP0 + 3: return (mixed ? geomf : geom) + 3 * ts * N + 4 * 8 * N + 12 * ts * N + 3 * ts * N + 8 * ts * N + 4 * 8 * N;
     case KT_NET: {  // FLOPs of one forward pass over every column (the six MFMA GEMMs)
       const auto &nt = c->net;
       double fl = 0;

@@ -1377,14 +1377,22 @@ __global__ void k_bcv(Incidence g, double *u, double *v, double *w, const double
 // T is the arithmetic type of fluxes, antidiffusive velocities and limiter; the fields
 // themselves stay FP64 and are updated by increments computed in T.
 // ------------------------------------------------------------------------------------
+// (x, y) pair of T (double2 / float2)
+template <typename T> struct Pair;
+template <> struct Pair<double> { typedef double2 type; };
+template <> struct Pair<float> { typedef float2 type; };
+
 template <typename T, int NF>
 struct MpArgs {
+  typedef typename Pair<T>::type T2;
   const T *U, *V, *W;              // advecting momentum uuu
   const double *psi[NF];           // fields in (pass 1) / donor-cell fields (pass 2)
   double *psi1[NF];                // donor-cell result / final result
-  T *gx[NF], *gy[NF], *gz[NF];     // gradients of the donor-cell fields
-  T *cp[NF], *cn[NF];              // limiter ratios
+  T2 *gxy[NF];                     // gradients of the donor-cell fields: (x, y) pairs ...
+  T *gz[NF];                       // ... and z
+  T2 *cpn[NF];                     // limiter ratios (cp, cn) per node
   const T *vol, *rh0;              // T copies
+  const T *vr;                     // vol*rh0, the same product formed once per node
   const double *ue;
   T afree[NF];                     // inflow value per field (0 => use ue)
   int use_ue[NF];
@@ -1511,7 +1519,12 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_grad(Incidence g, EdgeS<T> S, 
     for (int f = 0; f < NF; ++f) { gx[f] = gx[f] + bx * pi[f]; gy[f] = gy[f] + by * pi[f]; gz[f] = gz[f] + bz * pi[f]; }
   }
 #pragma unroll
-  for (int f = 0; f < NF; ++f) { a.gx[f][i] = gx[f]; a.gy[f][i] = gy[f]; a.gz[f][i] = gz[f]; }
+  for (int f = 0; f < NF; ++f) {
+    typename MpArgs<T, NF>::T2 v;
+    v.x = gx[f]; v.y = gy[f];
+    a.gxy[f][i] = v;
+    a.gz[f][i] = gz[f];
+  }
 }
 
 // antidiffusive velocity of one edge for one field (reference :3187-3204) from the two end
@@ -1536,13 +1549,14 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, EdgeS<T> S, M
   if (!L.valid) return;
   const int i = L.i;
   T Ui = a.U[i], Vi = a.V[i], Wi = a.W[i];
-  T vri = a.vol[i] * a.rh0[i];
+  T vri = a.vr[i];
   T mx[NF], mn[NF], rin[NF], rout[NF], pi[NF], gxi[NF], gyi[NF], gzi[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     mx[f] = (T)-1.e15; mn[f] = (T)1.e15; rin[f] = (T)0; rout[f] = (T)0;
     pi[f] = (T)a.psi1[f][i];
-    gxi[f] = a.gx[f][i]; gyi[f] = a.gy[f][i]; gzi[f] = a.gz[f][i];
+    const typename MpArgs<T, NF>::T2 g2 = a.gxy[f][i];
+    gxi[f] = g2.x; gyi[f] = g2.y; gzi[f] = a.gz[f][i];
   }
   for (int m = 0; m < L.deg; ++m) {
     int j, e;
@@ -1553,13 +1567,14 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, EdgeS<T> S, M
     T sx, sy, sz;
     mp_metric<T>(g, S, ee, sx, sy, sz);
     T aun = edge_aun<T>(Ui, Uj, Vi, Vj, Wi, Wj, sx, sy, sz);
-    T vrj = a.vol[j] * a.rh0[j];
+    T vrj = a.vr[j];
     T volt = (T)0.5 * (vri + vrj);
     T U12 = Ui + Uj, V12 = Vi + Vj, W12 = Wi + Wj;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       T pj = (T)a.psi1[f][j];
-      T A = anti_flux<T>(first ? pi[f] : pj, first ? pj : pi[f], gxi[f] + a.gx[f][j], gyi[f] + a.gy[f][j],
+      const typename MpArgs<T, NF>::T2 gj = a.gxy[f][j];
+      T A = anti_flux<T>(first ? pi[f] : pj, first ? pj : pi[f], gxi[f] + gj.x, gyi[f] + gj.y,
                          gzi[f] + a.gz[f][j], aun, volt, U12, V12, W12, a.dt);
       mx[f] = tmax<T>(tmax<T>(mx[f], pi[f]), pj);
       mn[f] = tmin<T>(tmin<T>(mn[f], pi[f]), pj);
@@ -1572,8 +1587,10 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, EdgeS<T> S, M
   const T tol = (T)1.e-10;
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    a.cp[f][i] = (mx[f] - pi[f]) * vl * rh / (rin[f] * a.dt + tol);
-    a.cn[f][i] = (pi[f] - mn[f]) * vl * rh / (rout[f] * a.dt + tol);
+    typename MpArgs<T, NF>::T2 r;
+    r.x = (mx[f] - pi[f]) * vl * rh / (rin[f] * a.dt + tol);
+    r.y = (pi[f] - mn[f]) * vl * rh / (rout[f] * a.dt + tol);
+    a.cpn[f][i] = r;
   }
 }
 
@@ -1588,14 +1605,15 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, EdgeS<T> S,
   if (!L.valid) return;
   const int i = L.i;
   T Ui = a.U[i], Vi = a.V[i], Wi = a.W[i];
-  T vri = a.vol[i] * a.rh0[i];
+  T vri = a.vr[i];
   T arh2[NF], pi[NF], gxi[NF], gyi[NF], gzi[NF], cpi[NF], cni[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     arh2[f] = (T)0;
     pi[f] = (T)a.psi1[f][i];
-    gxi[f] = a.gx[f][i]; gyi[f] = a.gy[f][i]; gzi[f] = a.gz[f][i];
-    cpi[f] = a.cp[f][i]; cni[f] = a.cn[f][i];
+    const typename MpArgs<T, NF>::T2 g2 = a.gxy[f][i], c2 = a.cpn[f][i];
+    gxi[f] = g2.x; gyi[f] = g2.y; gzi[f] = a.gz[f][i];
+    cpi[f] = c2.x; cni[f] = c2.y;
   }
   for (int m = 0; m < L.deg; ++m) {
     int j, e;
@@ -1606,15 +1624,16 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, EdgeS<T> S,
     T sx, sy, sz;
     mp_metric<T>(g, S, ee, sx, sy, sz);
     T aun = edge_aun<T>(Ui, Uj, Vi, Vj, Wi, Wj, sx, sy, sz);
-    T vrj = a.vol[j] * a.rh0[j];
+    T vrj = a.vr[j];
     T volt = (T)0.5 * (vri + vrj);
     T U12 = Ui + Uj, V12 = Vi + Vj, W12 = Wi + Wj;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       T pj = (T)a.psi1[f][j];
-      T A = anti_flux<T>(first ? pi[f] : pj, first ? pj : pi[f], gxi[f] + a.gx[f][j], gyi[f] + a.gy[f][j],
+      const typename MpArgs<T, NF>::T2 gj = a.gxy[f][j], cj = a.cpn[f][j];
+      T A = anti_flux<T>(first ? pi[f] : pj, first ? pj : pi[f], gxi[f] + gj.x, gyi[f] + gj.y,
                          gzi[f] + a.gz[f][j], aun, volt, U12, V12, W12, a.dt);
-      const T cpj = a.cp[f][j], cnj = a.cn[f][j];
+      const T cpj = cj.x, cnj = cj.y;
       // edge roles: cp/cn of i2 and i1
       const T cp2 = first ? cpj : cpi[f], cn1 = first ? cni[f] : cnj;
       const T cn2 = first ? cnj : cni[f], cp1 = first ? cpi[f] : cpj;
