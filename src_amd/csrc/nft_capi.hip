@@ -1150,8 +1150,9 @@ static int gcr_loop_host(nft_ctx *c, double *p, int impfl, double eps, int itmn,
 // FP64. The recursive residual then drifts from the true one by the operator's rounding,
 // so a solve runs in passes: each pass starts from the true FP64 residual of the current
 // pressure (prfr + div, as the reference's start) and iterates until the recursive
-// residual has dropped by op32_drop (nft_set_mixed_drop) or passed the tolerance; the next pass checks the
-// tolerance on the true residual and stops there, or restarts GCR from it.
+// residual has dropped by op32_drop (nft_set_mixed_drop) or passed the tolerance; the
+// next pass checks the tolerance on the true residual and stops there, or restarts GCR
+// from it.
 #ifndef NFT_OP32_PASSES
 #define NFT_OP32_PASSES 24
 #endif
