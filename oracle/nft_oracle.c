@@ -231,8 +231,9 @@ oc_ctx *oc_create(const oc_mesh *m, const oc_params *p) {
 /* ---- the HIP path's reduction tree (dot_order = 1) --------------------------------
  * wave: lane l += lane l+off for off = 32,16,...,1 (shfl_down), result in lane 0;
  * block: ((w0 + w1) + w2) + w3 over its four wavefronts;
- * grid: one block of 1024 threads; thread t sums partials t, t+1024, ... in order, then
- * the wavefront trees and the 16 wavefront results in order. Inactive lanes hold 0. */
+ * grid: 1024 threads (16 one-wavefront workgroups); thread t sums partials t, t+1024, ...
+ * in order, then the wavefront trees and the 16 wavefront results in order. Inactive
+ * lanes hold 0. */
 static double tree64(double *v) {
   for (int off = 32; off > 0; off >>= 1)
     for (int l = 0; l < off; ++l) v[l] = v[l] + v[l + off];

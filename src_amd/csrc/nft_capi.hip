@@ -117,7 +117,8 @@ struct nft_ctx {
   double *mp_psi1[4];
   void *mp_g[12], *mp_c[8];
   double *partials;
-  unsigned *counter;
+  unsigned *counter;   // [0]: arrivals at the end of k_reduce_final (zero between launches)
+  double *fin_wres;    // k_reduce_final's wavefront results: NFT_FIN_WG x (NFT_MAXD + 1)
   double *scal;
   double *h_scal;  // pinned
   // device-controlled GCR loop: exit flag handed to the loop kernels (null outside it) and
@@ -490,8 +491,8 @@ bool final_go(nft_ctx *c, int nblk, const FinalOp &op) {
     return false;
   } else {
     if (op.K == K) {
-      hipLaunchKernelGGL(k_reduce_final<K>, dim3(1), dim3(NFT_FIN_THREADS), 0, c->stream, c->partials, nblk, op,
-                         c->scal);
+      hipLaunchKernelGGL(k_reduce_final<K>, dim3(NFT_FIN_WG), dim3(64), 0, c->stream, c->partials, nblk, op,
+                         c->scal, c->fin_wres, c->counter);
       return true;
     }
     return final_go<K + 1>(c, nblk, op);
@@ -1960,7 +1961,8 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   }
   if ((rc = dalloc(c, &c->io[7], 8))) return fail(rc);  // device array of field pointers
   size_t npart = (size_t)std::max(c->gather_grid, c->vec_grid) * (NFT_MAXD + 1);
-  if ((rc = dalloc(c, &c->partials, npart)) || (rc = dalloc(c, &c->counter, 4)) || (rc = dalloc(c, &c->scal, SC_TOTAL)))
+  if ((rc = dalloc(c, &c->partials, npart)) || (rc = dalloc(c, &c->counter, 4)) || (rc = dalloc(c, &c->scal, SC_TOTAL)) ||
+      (rc = dalloc(c, &c->fin_wres, (size_t)NFT_FIN_WG * (NFT_MAXD + 1))))
     return fail(rc);
   if (hipMemset(c->counter, 0, 4 * sizeof(unsigned)) != hipSuccess) return fail(set_err(-2, "memset"));
   if (hipMemset(c->scal, 0, SC_TOTAL * sizeof(double)) != hipSuccess) return fail(set_err(-2, "memset"));

@@ -436,20 +436,31 @@ struct DivArgs {
   const double *skip;
 };
 
-__device__ __forceinline__ void gcr_finalize(const DotList &d, const double *red, double *scal,
+// a[idx] for a run-time idx as a select chain, so a stays in registers (an indexed access
+// would place the array in scratch memory)
+template <int K>
+__device__ __forceinline__ double pick(const double (&a)[K], int idx) {
+  double r = 0.;
+#pragma unroll
+  for (int k = 0; k < K; ++k) r = k == idx ? a[k] : r;
+  return r;
+}
+
+template <int K>
+__device__ __forceinline__ void gcr_finalize(const DotList &d, const double (&red)[K], double *scal,
                                              double epa) {
   if (d.fin == FIN_BETA) {
     // red[0] = <r, ax_s>, red[1] = <ax_s, ax_s>  (reference :1878-1885)
-    double ax2 = fmax(epa, red[1]);
+    double ax2 = fmax(epa, pick(red, 1));
     scal[SC_AX2 + d.slot] = ax2;
     scal[SC_RAX] = red[0];
     scal[SC_BETA] = -red[0] / ax2;
   } else if (d.fin == FIN_DEL) {
     // red[ii] = <ax_ii, ar>, del_ii = -axar/ax2_ii  (reference :1906-1912)
-    for (int ii = 0; ii < d.nd; ++ii) scal[SC_DEL + ii] = -red[ii] / scal[SC_AX2 + ii];
+    for (int ii = 0; ii < d.nd; ++ii) scal[SC_DEL + ii] = -pick(red, ii) / scal[SC_AX2 + ii];
   } else if (d.fin == FIN_NORMS) {
     scal[SC_R2] = red[0];
-    if (d.nmax) scal[SC_RMX] = red[d.nd];
+    if (d.nmax) scal[SC_RMX] = pick(red, d.nd);
   }
 }
 
@@ -746,20 +757,23 @@ struct FinalOp {
   DiagSpec diag;   // FK_DIAG_MOM / FK_DIAG_VAR
 };
 
+// The grid-level sum runs as NFT_FIN_WG one-wavefront workgroups, so the partials are
+// fetched by 16 CUs at once (one CU alone is bound by the misses it can keep in flight).
+// Summation order: virtual thread t = 64*w + l (workgroup w, lane l) sums blocks t, t+1024,
+// ... in that order; wavefront tree; then the 16 wavefront results in order w = 0..15,
+// summed by the workgroup that arrives last at the counter.
 #define NFT_FIN_THREADS 1024
+#define NFT_FIN_WG (NFT_FIN_THREADS / 64)
 template <int K>
-__global__ void __launch_bounds__(NFT_FIN_THREADS) k_reduce_final(const double *partials, int nblk, FinalOp op,
-                                                                  double *scal) {
-  constexpr int NW = NFT_FIN_THREADS / 64;
-  constexpr int U = K <= 4 ? 8 : (K <= 8 ? 4 : (K <= 16 ? 2 : 1));
-  __shared__ double sh[K * NW];
+__global__ void __launch_bounds__(64) k_reduce_final(const double *partials, int nblk, FinalOp op, double *scal,
+                                                     double *wres, unsigned *arrived) {
+  constexpr int U = K <= 4 ? 16 : (64 / K > 0 ? 64 / K : 1);
   if (skipped(op.skip)) return;
   double acc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) acc[k] = ((op.ismax >> k) & 1) ? -1.0e300 : 0.0;
-  // thread t: partials of blocks t, t+1024, ... in that order; U blocks' K values are
-  // loaded before they are added
-  for (int b0 = threadIdx.x; b0 < nblk; b0 += U * NFT_FIN_THREADS) {
+  // U blocks' K values are loaded before they are added
+  for (int b0 = blockIdx.x * 64 + threadIdx.x; b0 < nblk; b0 += U * NFT_FIN_THREADS) {
     double pv[U][K];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -775,30 +789,39 @@ __global__ void __launch_bounds__(NFT_FIN_THREADS) k_reduce_final(const double *
       }
     }
   }
-  // wavefront trees, then the 16 wavefront results in order
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < K; ++k) acc[k] = ((op.ismax >> k) & 1) ? wave_max(acc[k]) : wave_sum(acc[k]);
-  if (l == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) sh[k * NW + w] = acc[k];
-  }
-  __syncthreads();
   if (threadIdx.x != 0) return;
+  // publish this wavefront's result, then count the arrival (release); the last arrival
+  // (acquire) reads all 16 results and resets the counter for the next launch
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    __hip_atomic_store(reinterpret_cast<unsigned long long *>(wres) + blockIdx.x * K + k,
+                       (unsigned long long)__double_as_longlong(acc[k]), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned prev = __hip_atomic_fetch_add(arrived, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  if (prev != NFT_FIN_WG - 1) return;
+  const unsigned long long *wr = reinterpret_cast<const unsigned long long *>(wres);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    double a = sh[k * NW];
-    for (int j = 1; j < NW; ++j) a = ((op.ismax >> k) & 1) ? fmax(a, sh[k * NW + j]) : a + sh[k * NW + j];
+    double a = __longlong_as_double(
+        (long long)__hip_atomic_load(wr + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    for (int j = 1; j < NFT_FIN_WG; ++j) {
+      const double x = __longlong_as_double(
+          (long long)__hip_atomic_load(wr + j * K + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      a = ((op.ismax >> k) & 1) ? fmax(a, x) : a + x;
+    }
     acc[k] = a;
   }
+  __hip_atomic_store(arrived, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (op.kind == FK_DIAG_MOM) {
     // acc: per field (max, min, sum of w*x), then the sum of weights
     const DiagSpec &d = op.diag;
-    const double W = d.w ? acc[3 * d.nf] : d.n;
+    const double W = d.w ? pick(acc, 3 * d.nf) : d.n;
     for (int f = 0; f < d.nf; ++f) {
-      d.out[4 * f + 0] = fmax(d.init_mx, acc[3 * f + 0]);
-      d.out[4 * f + 1] = fmin(d.init_mn, -acc[3 * f + 1]);
-      d.out[4 * f + 2] = acc[3 * f + 2] / (d.mean_by_w[f] ? W : d.n);
+      d.out[4 * f + 0] = fmax(d.init_mx, pick(acc, 3 * f + 0));
+      d.out[4 * f + 1] = fmin(d.init_mn, -pick(acc, 3 * f + 1));
+      d.out[4 * f + 2] = pick(acc, 3 * f + 2) / (d.mean_by_w[f] ? W : d.n);
       d.out[4 * f + 3] = W;   // kept for the second pass (overwritten by the sd)
     }
     return;
@@ -808,9 +831,9 @@ __global__ void __launch_bounds__(NFT_FIN_THREADS) k_reduce_final(const double *
     const DiagSpec &d = op.diag;
     for (int f = 0; f < d.nf; ++f) {
       const double W = d.out[4 * f + 3];
-      d.out[4 * f + 3] = sqrt(acc[f] / (d.sd_by_w[f] ? W : d.n));
+      d.out[4 * f + 3] = sqrt(pick(acc, f) / (d.sd_by_w[f] ? W : d.n));
     }
-    if (d.argmax) d.out[4 * d.nf] = acc[d.nf];
+    if (d.argmax) d.out[4 * d.nf] = pick(acc, d.nf);
     return;
   }
   if (op.kind == FK_DIV) {
