@@ -258,9 +258,12 @@ def measured_traffic(kernel):
 
 
 def cpu_baseline(mesh, cfg, iters_per_solve, mesh_name="production"):
-    """Serial FP64 oracle on the bench mesh: one MPDATA call per field, one prfr,
-    div, line preconditioner and laplc, and one GCR cycle are timed and scaled to a step
-    with the GPU run's mean iteration count."""
+    """C oracle, band-line method, sampled: the serial FP64 oracle running this bench's own
+    method (GCR with the band line preconditioner) on the bench mesh. One MPDATA call per
+    field, one prfr, div, line preconditioner and laplc, and one GCR cycle are timed and
+    scaled to a step with the GPU run's mean iteration count. Beside it, the cost of one GCR
+    iteration of the reference's own method (point relaxation, two sweeps: precon
+    :543-612 with itrp1 = 2, then laplc), timed the same way."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
@@ -297,8 +300,24 @@ def cpu_baseline(mesh, cfg, iters_per_solve, mesh_name="production"):
     t_iter = max((t_cycle - t_setup) / lord, t_pre + t_lap)
     step = t_mp + 2 * (t_setup + iters_per_solve * t_iter + t_prfr)
     o.close()
+    # the reference's own preconditioner: two point-Jacobi sweeps (each a laplc) per application
+    import dataclasses
+    op = Oracle(mesh, dataclasses.replace(cfg, precon=1, itrp1=2, omega=0.5))
+    t = time.perf_counter()
+    qp = op.precon(r, 1)
+    t_pre_pt = time.perf_counter() - t
+    t = time.perf_counter()
+    op.laplc(qp, 1)
+    t_lap_pt = time.perf_counter() - t
+    op.close()
     return dict(value=1.0 / step, unit="timesteps/s", cores=1, kind="port",
-                sample=(f"FP64 C oracle, 1 thread, {mesh_name} mesh ({n} nodes): mpdatam x4 {t_mp:.2f} s, "
+                label="C oracle, band-line method, sampled",
+                reference_method_iteration_s=round(t_pre_pt + t_lap_pt, 4),
+                reference_method_note=("one GCR iteration of the reference's own method (point-Jacobi "
+                                       "precon with two sweeps + laplc) in the same oracle, seconds; that "
+                                       "method needs many more iterations per solve than the band line"),
+                sample=(f"C oracle, band-line method, sampled: FP64, 1 thread, {mesh_name} mesh ({n} nodes): "
+                        f"mpdatam x4 {t_mp:.2f} s, "
                         f"GCR iteration {t_iter:.2f} s (precon {t_pre:.2f} s + laplc {t_lap:.2f} s + "
                         f"vector work), scaled to one step with {iters_per_solve:.1f} iterations per "
                         f"solve (2 solves) -> {step:.1f} s/step; on identical work the oracle takes 0.80x "

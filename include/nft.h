@@ -199,6 +199,13 @@ typedef struct {
 } nft_diag;
 /* the three diagnostics of the current state (courant with iflag 1, 2 or 3) */
 int nft_diagnostics(nft_ctx *ctx, int courant_flag, nft_diag *out);
+/* courant(uuu,cour,iflag) (:1974-2094) of the advector uuu(ld,3) given on the host: the
+ * per-node Courant numbers into cour (may be null) and their statistics into out->cour,
+ * out->crnic; the device-resident time-step state is not touched */
+int nft_courant(nft_ctx *ctx, const double *uuu, int ld, double *cour, int iflag, nft_diag *out);
+/* stats(unk) (:2847-2912) of unk(ld,4) given on the host: out->u, v, w, t; the
+ * device-resident time-step state is not touched */
+int nft_stats(nft_ctx *ctx, const double *unk, int ld, nft_diag *out);
 /* record them on the GPU every `every` steps of nft_run_steps (0: off), keeping the last
  * `capacity` records; nft_get_diagnostics copies the records made since the previous call
  * (oldest first, at most max) and sets *count to their number */
@@ -252,7 +259,8 @@ int nft_set_mixed_drop(nft_ctx *ctx, double drop, double *current);
  * adds a learned correction to the vertical line preconditioner inside the flexible GCR;
  * when an iteration stalls (rl2 > stall_ratio * rl2_prev, or no decrease) the solve drops
  * the correction and continues with the line preconditioner alone. Needs uniform columns
- * and precon = 2. Parameters: per layer W (K x N, row-major) then b (N); layers
+ * and a vertical line preconditioner (precon >= 2: the two-point-flux tridiagonal or the
+ * in-column band of laplc), with any precision (0, 1 or 2). Parameters: per layer W (K x N, row-major) then b (N); layers
  * Kp->H, H->H, 2H->H, 2H->H, H->H, H->Kp with Kp = levels rounded up to a multiple of 8. */
 int nft_net_init(nft_ctx *ctx, int hidden, uint64_t seed, int max_batch);
 int nft_net_info(nft_ctx *ctx, int *kp, int *levels, int *ncol, int64_t *nparams);

@@ -67,6 +67,14 @@ def build_fortran():
         return None
     exe = os.path.join(LIB, "nftdrv")
     _run([fc, "-O2", "-J", bdir, *srcs, "-o", exe, "-L" + LIB, "-lnft_hip", "-Wl,-rpath,$ORIGIN"])
+    # test program of the drop-in subroutines (tests/test_gpu_dropin.py runs it)
+    tsrc = os.path.join(ROOT, "tests", "fortran", "nftdrop.f90")
+    if os.path.exists(tsrc):
+        tdir = os.path.join(bdir, "tests")
+        os.makedirs(tdir, exist_ok=True)
+        _run([fc, "-O2", "-J", tdir, os.path.join(fdir, "nft_hip.f90"), tsrc,
+              "-o", os.path.join(ROOT, "tests", "fortran", "nftdrop"), "-L" + LIB, "-lnft_hip",
+              "-Wl,-rpath,$ORIGIN/../../src_amd/lib"])
     return exe
 
 

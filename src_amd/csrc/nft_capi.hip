@@ -1374,35 +1374,47 @@ int diag_stats(nft_ctx *c, const DiagSpec &d) {
   return final_launch(c, c->vec_grid, op2);
 }
 
-// the reference's per-step diagnostics of the current state into rec[kDiagLen] (device)
-int diag_dev(nft_ctx *c, int flag, double *rec) {
+// the reference's per-step diagnostics into rec[kDiagLen] (device), of the advector uuu
+// and the fields unk given as device arrays; parts: 1 div, 2 courant (numbers in t[5]),
+// 4 stats (sums of these)
+int diag_dev_of(nft_ctx *c, int flag, double *rec, double *const *uuu, double *const *unk, int parts) {
   int rc;
-  if ((rc = div_dev(c, c->uuu[0], c->uuu[1], c->uuu[2], c->t[4], 1, nullptr))) return rc;
-  DiagSpec d{};
-  d.nf = 1; d.x[0] = c->t[4]; d.scale = c->P.dt; d.n = (double)c->n;
-  d.init_mx = -1.e15; d.init_mn = 1.e15; d.out = rec;
-  if ((rc = diag_stats<1>(c, d))) return rc;
-  dim3 g(c->gather_grid), b(NFT_BLOCK);
-#define NFT_COUR(F, COL)                                                                                        \
-  hipLaunchKernelGGL((k_courant<F, COL>), g, b, 0, c->stream, c->inc, c->S, c->uuu[0], c->uuu[1], c->uuu[2], c->nc, \
-                     c->P.dt, c->t[5])
-  if (c->col_on) {
-    if (flag == 2) NFT_COUR(2, true); else if (flag == 3) NFT_COUR(3, true); else NFT_COUR(1, true);
-  } else {
-    if (flag == 2) NFT_COUR(2, false); else if (flag == 3) NFT_COUR(3, false); else NFT_COUR(1, false);
+  if (parts & 1) {
+    if ((rc = div_dev(c, uuu[0], uuu[1], uuu[2], c->t[4], 1, nullptr))) return rc;
+    DiagSpec d{};
+    d.nf = 1; d.x[0] = c->t[4]; d.scale = c->P.dt; d.n = (double)c->n;
+    d.init_mx = -1.e15; d.init_mn = 1.e15; d.out = rec;
+    if ((rc = diag_stats<1>(c, d))) return rc;
   }
+  if (parts & 2) {
+    dim3 g(c->gather_grid), b(NFT_BLOCK);
+#define NFT_COUR(F, COL)                                                                                        \
+    hipLaunchKernelGGL((k_courant<F, COL>), g, b, 0, c->stream, c->inc, c->S, uuu[0], uuu[1], uuu[2], c->nc, \
+                       c->P.dt, c->t[5])
+    if (c->col_on) {
+      if (flag == 2) NFT_COUR(2, true); else if (flag == 3) NFT_COUR(3, true); else NFT_COUR(1, true);
+    } else {
+      if (flag == 2) NFT_COUR(2, false); else if (flag == 3) NFT_COUR(3, false); else NFT_COUR(1, false);
+    }
 #undef NFT_COUR
-  HIPCHK(hipGetLastError());
-  DiagSpec cd{};
-  cd.nf = 1; cd.x[0] = c->t[5]; cd.scale = 1.0; cd.n = (double)c->n;
-  cd.init_mx = -1.e9; cd.init_mn = 1.e9; cd.argmax = 1; cd.out = rec + 4;
-  if ((rc = diag_stats<1>(c, cd))) return rc;
-  DiagSpec sd{};
-  sd.nf = 4; sd.w = c->rh0; sd.scale = 1.0; sd.n = (double)c->n;
-  sd.init_mx = -1.e10; sd.init_mn = 1.e10; sd.out = rec + 9;
-  for (int f = 0; f < 4; ++f) { sd.x[f] = c->unk[f]; sd.mean_by_w[f] = 1; sd.sd_by_w[f] = f < 3 ? 1 : 0; }
-  return diag_stats<4>(c, sd);
+    HIPCHK(hipGetLastError());
+    DiagSpec cd{};
+    cd.nf = 1; cd.x[0] = c->t[5]; cd.scale = 1.0; cd.n = (double)c->n;
+    cd.init_mx = -1.e9; cd.init_mn = 1.e9; cd.argmax = 1; cd.out = rec + 4;
+    if ((rc = diag_stats<1>(c, cd))) return rc;
+  }
+  if (parts & 4) {
+    DiagSpec sd{};
+    sd.nf = 4; sd.w = c->rh0; sd.scale = 1.0; sd.n = (double)c->n;
+    sd.init_mx = -1.e10; sd.init_mn = 1.e10; sd.out = rec + 9;
+    for (int f = 0; f < 4; ++f) { sd.x[f] = unk[f]; sd.mean_by_w[f] = 1; sd.sd_by_w[f] = f < 3 ? 1 : 0; }
+    if ((rc = diag_stats<4>(c, sd))) return rc;
+  }
+  return 0;
 }
+
+// the three diagnostics of the device-resident state
+int diag_dev(nft_ctx *c, int flag, double *rec) { return diag_dev_of(c, flag, rec, c->uuu, c->unk, 7); }
 
 void diag_unpack(const double *r, nft_diag &d) {
   for (int k = 0; k < 4; ++k) { d.div[k] = r[k]; d.cour[k] = r[4 + k]; }
@@ -2707,6 +2719,38 @@ int nft_diagnostics(nft_ctx *c, int courant_flag, nft_diag *out) {
   diag_unpack(r, *out);
   diag_meta_fill(c, *out);
   return 0;
+}
+
+// courant(uuu,cour,iflag) and stats(unk) on host arrays: the arguments are staged in the
+// io buffers, so the device-resident time-step state is left as it is
+static int diag_host(nft_ctx *c, int flag, const double *const *cols, int ncols, int parts, double *cour,
+                     nft_diag *out) {
+  int rc;
+  if (!c->diag_now && (rc = dalloc(c, &c->diag_now, kDiagLen))) return rc;
+  for (int k = 0; k < ncols; ++k)
+    if ((rc = copy_in(c, c->io[k], cols[k]))) return rc;
+  HIPCHK(hipMemsetAsync(c->diag_now, 0, kDiagLen * sizeof(double), c->stream));
+  if ((rc = diag_dev_of(c, flag, c->diag_now, c->io, c->io, parts))) return rc;
+  double r[kDiagLen];
+  HIPCHK(hipMemcpyAsync(r, c->diag_now, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  if (cour && (rc = copy_out(c, cour, c->t[5]))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *out = nft_diag{};
+  diag_unpack(r, *out);
+  diag_meta_fill(c, *out);
+  return 0;
+}
+
+int nft_courant(nft_ctx *c, const double *uuu, int ld, double *cour, int iflag, nft_diag *out) {
+  if (!c || !uuu || !out || ld < c->n || iflag < 1 || iflag > 3) return set_err(-1, "nft_courant: bad argument");
+  const double *cols[3] = {uuu, uuu + (size_t)ld, uuu + 2 * (size_t)ld};
+  return diag_host(c, iflag, cols, 3, 2, cour, out);
+}
+
+int nft_stats(nft_ctx *c, const double *unk, int ld, nft_diag *out) {
+  if (!c || !unk || !out || ld < c->n) return set_err(-1, "nft_stats: bad argument");
+  const double *cols[4] = {unk, unk + (size_t)ld, unk + 2 * (size_t)ld, unk + 3 * (size_t)ld};
+  return diag_host(c, 1, cols, 4, 4, nullptr, out);
 }
 
 int nft_set_diagnostics(nft_ctx *c, int every, int capacity, int courant_flag) {

@@ -190,6 +190,22 @@ module nft_hip
       integer(c_int), value :: courant_flag
       type(nft_diag_t), intent(out) :: d
     end function
+    integer(c_int) function nft_courant(ctx, uuu, ld, cour, iflag, d) bind(C, name="nft_courant")
+      import :: c_ptr, c_int, c_double, nft_diag_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: uuu(*)
+      integer(c_int), value :: ld
+      real(c_double), intent(out) :: cour(*)
+      integer(c_int), value :: iflag
+      type(nft_diag_t), intent(out) :: d
+    end function
+    integer(c_int) function nft_stats(ctx, unk, ld, d) bind(C, name="nft_stats")
+      import :: c_ptr, c_int, c_double, nft_diag_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: unk(*)
+      integer(c_int), value :: ld
+      type(nft_diag_t), intent(out) :: d
+    end function
     integer(c_int) function nft_get_field(ctx, name, host) bind(C, name="nft_get_field")
       import :: c_ptr, c_int, c_double, c_char
       type(c_ptr), value :: ctx
@@ -241,8 +257,9 @@ contains
 
   ! Hands the reference's COMMON-block mesh and control data to nft_init. Call after the
   ! reference's `input`. ncol/col_start: vertical columns for the line preconditioner
-  ! (ncol = 0: none); precon_kind: 1 point relaxation, 2 vertical line; precision: 0 FP64,
-  ! 1 mixed; strat/theta0: the values mpsolver hard-codes (:134-135).
+  ! (ncol = 0: none); precon_kind: 1 point relaxation, 2 vertical line (two-point-flux
+  ! tridiagonal), 3 vertical line (in-column band of laplc); precision: 0 FP64, 1 mixed
+  ! (FP32 MPDATA and line solve), 2 mixed with the FP32 operator inside GCR; strat/theta0: the values mpsolver hard-codes (:134-135).
   subroutine nft_attach(ncol, col_start, precon_kind, precision, lord, strat, theta0)
     integer, intent(in) :: ncol, precon_kind, precision, lord
     integer(c_int), intent(in), target :: col_start(*)
@@ -453,27 +470,26 @@ subroutine mpdatam(uuu, unk, iflag)
   call nft_check(nft_mpdatam(nft_context, uuu, npoinx, unk, iflag), 'mpdatam')
 end subroutine
 
-! courant(uuu,cour,iflag) (reference :1974-2094) on the GPU: uploads uuu, computes the
-! Courant numbers and their statistics, returns cour and prints the reference's lines.
+! courant(uuu,cour,iflag) (reference :1974-2094) on the GPU: the Courant numbers of the
+! uuu given and their statistics; returns cour and prints the reference's lines. The
+! device-resident time-step state is left as it is.
 subroutine courant(uuu, cour, iflag)
   use nft_hip
   implicit none
   real(8) :: uuu(npoinx,3), cour(*)
   integer :: iflag
   type(nft_diag_t) :: d
-  call nft_check(nft_set_state(nft_context, 'uuu'//c_null_char, uuu, npoinx), 'courant')
-  call nft_check(nft_diagnostics(nft_context, iflag, d), 'courant')
-  call nft_check(nft_get_field(nft_context, 'courant'//c_null_char, cour), 'courant')
+  call nft_check(nft_courant(nft_context, uuu, npoinx, cour, iflag, d), 'courant')
   call nft_print_diag(d, 2)
 end subroutine
 
-! stats(unk) (reference :2847-2912) on the GPU: uploads unk and prints the reference's lines.
+! stats(unk) (reference :2847-2912) on the GPU: prints the reference's lines for the unk
+! given; the device-resident time-step state is left as it is.
 subroutine stats(unk)
   use nft_hip
   implicit none
   real(8) :: unk(npoinx,4)
   type(nft_diag_t) :: d
-  call nft_check(nft_set_state(nft_context, 'unk'//c_null_char, unk, npoinx), 'stats')
-  call nft_check(nft_diagnostics(nft_context, 1, d), 'stats')
+  call nft_check(nft_stats(nft_context, unk, npoinx, d), 'stats')
   call nft_print_diag(d, 4)
 end subroutine

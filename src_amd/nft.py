@@ -125,6 +125,8 @@ _SIGS.update({
                                            ctypes.POINTER(ctypes.c_int)]),
     "nft_multigrid_op": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, f64p, f64p, ctypes.c_int]),
     "nft_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(NftDiag)]),
+    "nft_courant": (ctypes.c_int, [ctypes.c_void_p, f64p, ctypes.c_int, f64p, ctypes.c_int, ctypes.POINTER(NftDiag)]),
+    "nft_stats": (ctypes.c_int, [ctypes.c_void_p, f64p, ctypes.c_int, ctypes.POINTER(NftDiag)]),
     "nft_set_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "nft_get_diagnostics": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(NftDiag),
                                            ctypes.c_int]),
@@ -186,6 +188,9 @@ def _p(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+STATE_BLOCKS = {"unk": 4, "frc": 4, "uuu": 3, "unkdash": 3, "prpot": 1}
 
 
 # fp64: FP64 everywhere; mixed: FP32 MPDATA and line preconditioner; mixed_op: mixed plus the
@@ -295,63 +300,73 @@ class Solver:
         except Exception:
             pass
 
+    def _vec(self, a, what="array"):
+        """A contiguous float64 node array of length npoin (NftError otherwise: the library
+        reads exactly npoin values from the pointer it is given)."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.ndim != 1 or a.shape[0] != self.n:
+            raise NftError(f"{what}: expected a node array of shape ({self.n},), got {a.shape}")
+        return a
+
     # ---- reference subroutines -------------------------------------------------------
     def derivative(self, unk, idx):
         out = np.zeros(self.n)
-        _chk(lib().nft_derivative(self.h, _p(_f64(unk)), _p(out), idx), "nft_derivative")
+        _chk(lib().nft_derivative(self.h, _p(self._vec(unk, "unk")), _p(out), idx), "nft_derivative")
         return out
 
     def laplc(self, p, impfl):
         out = np.zeros(self.n)
-        _chk(lib().nft_laplc(self.h, _p(_f64(p)), _p(out), impfl), "nft_laplc")
+        _chk(lib().nft_laplc(self.h, _p(self._vec(p, "press")), _p(out), impfl), "nft_laplc")
         return out
 
     def laplc32(self, p, impfl):
         """The FP32 operator of precision 2 (no reference counterpart)."""
         out = np.zeros(self.n)
-        _chk(lib().nft_laplc32(self.h, _p(_f64(p)), _p(out), impfl), "nft_laplc32")
+        _chk(lib().nft_laplc32(self.h, _p(self._vec(p, "press")), _p(out), impfl), "nft_laplc32")
         return out
 
     def laplc_t(self, y, impfl):
         """Transpose of laplc (no reference counterpart)."""
         out = np.zeros(self.n)
-        _chk(lib().nft_laplc_t(self.h, _p(_f64(y)), _p(out), impfl), "nft_laplc_t")
+        _chk(lib().nft_laplc_t(self.h, _p(self._vec(y, "y")), _p(out), impfl), "nft_laplc_t")
         return out
 
     def prfr(self, p, us, vs, ws, impfl):
         outs = [np.zeros(self.n) for _ in range(3)]
-        ins = [_f64(a) for a in (p, us, vs, ws)]
+        ins = [self._vec(a, "prfr argument") for a in (p, us, vs, ws)]
         _chk(lib().nft_prfr(self.h, *[_p(a) for a in ins], *[_p(a) for a in outs], impfl), "nft_prfr")
         return outs
 
     def bcv(self, u, v, w, flag):
-        u, v, w = (np.array(a, dtype=np.float64) for a in (u, v, w))
+        u, v, w = (np.array(self._vec(a, "bcv argument")) for a in (u, v, w))
         _chk(lib().nft_bcv(self.h, _p(u), _p(v), _p(w), float(flag)), "nft_bcv")
         return u, v, w
 
     def div(self, u, v, w, iflg):
         out = np.zeros(self.n)
-        ins = [_f64(a) for a in (u, v, w)]
+        ins = [self._vec(a, "div argument") for a in (u, v, w)]
         _chk(lib().nft_div(self.h, *[_p(a) for a in ins], _p(out), iflg), "nft_div")
         return out
 
     def precon(self, rhs, impfl):
         p = np.zeros(self.n)
-        _chk(lib().nft_precon(self.h, _p(_f64(rhs)), _p(p), impfl), "nft_precon")
+        _chk(lib().nft_precon(self.h, _p(self._vec(rhs, "rhs")), _p(p), impfl), "nft_precon")
         return p
 
     def gcrk(self, p, u1, v1, w1, impfl):
-        p = np.array(p, dtype=np.float64)
+        p = np.array(self._vec(p, "p"))
         outs = [np.zeros(self.n) for _ in range(3)]
-        ins = [_f64(a) for a in (u1, v1, w1)]
+        ins = [self._vec(a, "gcrk argument") for a in (u1, v1, w1)]
         st = GcrStats()
         rc = _chk(lib().nft_gcrk(self.h, _p(p), *[_p(a) for a in ins], *[_p(a) for a in outs], impfl,
                                  ctypes.byref(st)), "nft_gcrk")
         return p, outs, st, rc
 
     def mpdatam(self, uuu, unk, iflag):
-        unk = np.array(unk, dtype=np.float64)
+        unk = np.array(self._vec(unk, "unk"))
         uuu = _f64(uuu)
+        if uuu.ndim != 2 or uuu.shape[0] != 3 or uuu.shape[1] < self.n:
+            raise NftError(f"uuu: expected shape (3, ld) with ld >= {self.n}, got {uuu.shape}")
         _chk(lib().nft_mpdatam(self.h, _p(uuu), uuu.shape[1], _p(unk), iflag), "nft_mpdatam")
         return unk
 
@@ -365,13 +380,19 @@ class Solver:
         return st
 
     def state(self, name):
-        k = {"unk": 4, "frc": 4, "uuu": 3, "unkdash": 3, "prpot": 1}[name]
+        if name not in STATE_BLOCKS:
+            raise NftError(f"unknown state block {name!r}")
+        k = STATE_BLOCKS[name]
         a = np.zeros((k, self.n))
         _chk(lib().nft_get_state(self.h, name.encode(), _p(a), self.n), "nft_get_state")
         return a if k > 1 else a[0]
 
     def set_state(self, name, arr):
+        if name not in STATE_BLOCKS:
+            raise NftError(f"unknown state block {name!r}")
         a = np.ascontiguousarray(np.atleast_2d(arr), dtype=np.float64)
+        if a.shape != (STATE_BLOCKS[name], self.n):
+            raise NftError(f"{name}: expected shape ({STATE_BLOCKS[name]}, {self.n}), got {a.shape}")
         _chk(lib().nft_set_state(self.h, name.encode(), _p(a), self.n), "nft_set_state")
 
     def field(self, name):
@@ -520,6 +541,26 @@ class Solver:
         """The reference's per-step diagnostics of the current state (NftDiag)."""
         d = NftDiag()
         _chk(lib().nft_diagnostics(self.h, courant_flag, ctypes.byref(d)), "nft_diagnostics")
+        return d
+
+    def courant(self, uuu, iflag=1):
+        """courant(uuu,cour,iflag) of a host advector (3, ld): (per-node numbers, NftDiag with
+        the statistics); the device-resident state is not touched."""
+        uuu = _f64(uuu)
+        if uuu.ndim != 2 or uuu.shape[0] != 3 or uuu.shape[1] < self.n:
+            raise NftError(f"uuu: expected shape (3, ld) with ld >= {self.n}, got {uuu.shape}")
+        cour = np.zeros(self.n)
+        d = NftDiag()
+        _chk(lib().nft_courant(self.h, _p(uuu), uuu.shape[1], _p(cour), iflag, ctypes.byref(d)), "nft_courant")
+        return cour, d
+
+    def stats(self, unk):
+        """stats(unk) of host fields (4, ld): NftDiag with u, v, w, t; the state is not touched."""
+        unk = _f64(unk)
+        if unk.ndim != 2 or unk.shape[0] != 4 or unk.shape[1] < self.n:
+            raise NftError(f"unk: expected shape (4, ld) with ld >= {self.n}, got {unk.shape}")
+        d = NftDiag()
+        _chk(lib().nft_stats(self.h, _p(unk), unk.shape[1], ctypes.byref(d)), "nft_stats")
         return d
 
     def set_diagnostics(self, every, capacity=1024, courant_flag=1):

@@ -101,3 +101,52 @@ def mg_levels_for(spec_key, as_read=False):
     if as_read:
         m2 = M.as_reference_reads(m2, m1)
     return [(m1, t1), (m2, t2)]
+
+
+def production_cfg(**kw):
+    """Production workload: 65,536 columns x 64 levels (4,194,304 nodes), dt = 20 s, U = 10 m/s,
+    N = 0.01 1/s, GCR(4) to 1e-8 relative residual with the band line preconditioner."""
+    c = M.CONFIGS["production"]
+    base = dict(u0=c.u0, bvf=c.bvf, dt=c.dt, steps=c.steps, lord=4, eps0=1e-8, exit_mode=1, itr=400, precon=3,
+                itrp1=1, omega=1.0, sponge_depth=3000.0, sponge_tau=300.0)
+    base.update(kw)
+    return M.RunConfig("production", c.mesh, **base)
+
+
+def stress_cfg(**kw):
+    """Stress workload: 131,072 columns x 128 stretched levels (16,777,216 nodes), dt = 15 s,
+    U = 5 m/s, N = 0.025 1/s (Fr = 0.1), GCR(4) to 1e-8 with the band line preconditioner."""
+    c = M.CONFIGS["stiff"]
+    base = dict(u0=c.u0, bvf=c.bvf, dt=c.dt, steps=c.steps, lord=4, eps0=1e-8, exit_mode=1, itr=400, precon=3,
+                itrp1=1, omega=1.0, sponge_depth=3000.0, sponge_tau=300.0)
+    base.update(kw)
+    return M.RunConfig("stiff", c.mesh, **base)
+
+
+def interior_blob(m, width=0.15):
+    """A Gaussian blob in the middle of the domain, exactly zero at every boundary node."""
+    x, y, z = m.coord
+    cx, cy = 0.5 * (x.min() + x.max()), 0.5 * (y.min() + y.max())
+    lx, ly = x.max() - x.min(), y.max() - y.min()
+    zc = 0.5 * (z.min() + z.max())
+    hz = z.max() - z.min()
+    psi = np.exp(-(((x - cx) / (width * lx)) ** 2 + ((y - cy) / (width * ly)) ** 2 + ((z - zc) / (0.2 * hz)) ** 2))
+    psi[m.ibpoin[0] - 1] = 0.0
+    return psi
+
+
+def diag_close(mine, ref):
+    """Diagnostics rows [div 4, cour 4, crnic, u 4, v 4, w 4, t 4]: extrema and indices exact;
+    means and deviations (tree vs sequential sums) to 1e-10 of the field's magnitude."""
+    mine, ref = np.asarray(mine), np.asarray(ref)
+    groups = [(0, 4), (4, 8)] + [(9 + 4 * k, 13 + 4 * k) for k in range(4)]
+    for a, b in groups:
+        assert mine[a] == ref[a] and mine[a + 1] == ref[a + 1], (a, mine[a:b], ref[a:b])
+        scale = max(abs(ref[a]), abs(ref[a + 1]), 1e-300)
+        assert abs(mine[a + 2] - ref[a + 2]) <= 1e-10 * scale, (a, mine[a + 2], ref[a + 2])
+        assert abs(mine[a + 3] - ref[a + 3]) <= 1e-10 * scale, (a, mine[a + 3], ref[a + 3])
+    assert mine[8] == ref[8]
+
+
+# GCR(k) to 1e-8 relative residual with the band line preconditioner (the benchmark's solver)
+BAND_KW = dict(precon=3, itrp1=1, omega=1.0, exit_mode=1, eps0=1e-8, itr=200)

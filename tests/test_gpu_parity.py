@@ -10,7 +10,7 @@ import os
 import numpy as np
 import pytest
 
-from common import mesh_checksum, mesh_for, rel_l2, seeded_fields, small_cfg, tiny_cfg
+from common import diag_close as _diag_close, mesh_checksum, mesh_for, rel_l2, seeded_fields, small_cfg, tiny_cfg
 from oracle import Oracle
 from src_amd import nft
 
@@ -350,19 +350,6 @@ def test_init_rejects_corrupt_mesh():
     m2.iedge = bad
     with pytest.raises(nft.NftError, match="out of range"):
         nft.Solver(m2, nft.make_params(tiny_cfg()))
-
-
-def _diag_close(mine, ref):
-    """Extrema and indices exact; means and deviations (tree vs sequential sums) to 1e-10
-    of the field's magnitude."""
-    mine, ref = np.asarray(mine), np.asarray(ref)
-    groups = [(0, 4), (4, 8)] + [(9 + 4 * k, 13 + 4 * k) for k in range(4)]
-    for a, b in groups:
-        assert mine[a] == ref[a] and mine[a + 1] == ref[a + 1], (a, mine[a:b], ref[a:b])
-        scale = max(abs(ref[a]), abs(ref[a + 1]), 1e-300)
-        assert abs(mine[a + 2] - ref[a + 2]) <= 1e-10 * scale, (a, mine[a + 2], ref[a + 2])
-        assert abs(mine[a + 3] - ref[a + 3]) <= 1e-10 * scale, (a, mine[a + 3], ref[a + 3])
-    assert mine[8] == ref[8]
 
 
 @pytest.mark.parametrize("key,columnar", [("tiny", 1), ("tiny", 0), ("small", 1)])
