@@ -116,6 +116,17 @@ int nft_laplc(nft_ctx *ctx, const double *press, double *alap, int impfl);
 /* out = L^T y, the transpose of laplc (no reference counterpart: trains the learned
  * preconditioner) */
 int nft_laplc_t(nft_ctx *ctx, const double *y, double *out, int impfl);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): laplc
+ * (:1288-1353) on DEVICE arrays press and alap (npoin doubles each, on the context's GPU,
+ * ready when the call is made), with laplc's intermediate held in a fixed device array of
+ * the library instead of the context's work arrays. The call allocates nothing; it
+ * enqueues two kernels on the context's stream and waits for them. Limit: meshes of at most
+ * 1,048,576 nodes (the array holds 24 MiB on each GPU the library runs on, until the
+ * process ends); larger meshes are rejected with an error. Every call on one GPU shares that
+ * array, so calls that overlap on one GPU (from several contexts or threads) race. Results
+ * equal nft_laplc's bit for bit. */
+int nft_laplc_noalloc(nft_ctx *ctx, const double *press, double *alap, int impfl);
 /* no reference counterpart: the FP32 operator that precision 2 applies inside the GCR
  * iterations (edge metrics and scaled gradient in FP32, FP64 sums), on host arrays; for
  * checking it against its CPU restatement. Precision 2 contexts only. */

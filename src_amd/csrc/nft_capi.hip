@@ -45,6 +45,39 @@ int set_err(int code, const std::string &m) {
 
 static const double kG = 9.80616;  // reference :94
 
+}  // namespace
+
+// ---- fixed scratch of the allocation-free entry point (nft_laplc_noalloc) ---------------
+// laplc's intermediate, the scaled gradient ((x, y) pairs for every node, then z), for
+// meshes of at most kScratchNodes nodes: 3 x 8 B x 1,048,576 = 24 MiB, held by every GPU
+// the library runs on until the process ends. In host code `scratch` names a host-side
+// stand-in, not device memory: kernels get the per-device address that
+// hipGetSymbolAddress returns (g_scratch_addr), looked up once per device by nft_init.
+// The array is shared by every call on one GPU and keeps the last call's contents.
+static constexpr int kScratchNodes = 1 << 20;
+static constexpr int kScratchDevices = 64;
+static __device__ __align__(256) double scratch[3 * (size_t)kScratchNodes];
+static double *g_scratch_addr[kScratchDevices] = {};
+
+namespace {
+
+// the device address of `scratch` on the current device (looked up on first use per device)
+int scratch_address(double **out) {
+  *out = nullptr;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices) return set_err(-2, "scratch: device ordinal out of range");
+  if (!g_scratch_addr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_err(-2, std::string("scratch: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+    g_scratch_addr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_addr[dev];
+  return 0;
+}
+
 template <typename T>
 struct DevBuf {
   T *p = nullptr;
@@ -128,6 +161,10 @@ struct nft_ctx {
   hipEvent_t poll_ev[2] = {nullptr, nullptr};
   // io staging for the per-subroutine entry points
   double *io[8];
+  // this device's address of the fixed scratch array (nft_laplc_noalloc); null if the
+  // lookup failed at nft_init (the entry point then reports it)
+  double *scratch_dev = nullptr;
+  std::string scratch_err;
 
   nft_gcr_stats st{};
 
@@ -1844,6 +1881,7 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   int rc = 0;
   auto fail = [&](int code) { nft_finalize(c); return code; };
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(set_err(-2, "stream"));
+  if (scratch_address(&c->scratch_dev)) c->scratch_err = g_err;
   for (auto &e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) return fail(set_err(-2, "event"));
 
@@ -2058,6 +2096,31 @@ int nft_laplc(nft_ctx *c, const double *press, double *alap, int impfl) {
   if ((rc = laplc_dev(c, c->io[0], c->io[1], impfl, nullptr))) return rc;
   if ((rc = copy_out(c, alap, c->io[1]))) return rc;
   return sync(c);
+}
+
+int nft_laplc_noalloc(nft_ctx *c, const double *press, double *alap, int impfl) {
+  if (!c || !press || !alap || (impfl != 0 && impfl != 1)) return set_err(-1, "nft_laplc_noalloc: bad argument");
+  if (c->n > kScratchNodes)
+    return set_err(-1, "nft_laplc_noalloc: the mesh has " + std::to_string(c->n) + " nodes, more than the " +
+                           std::to_string(kScratchNodes) + " its fixed scratch array holds");
+  if (!c->scratch_dev) return set_err(-2, "nft_laplc_noalloc: " + c->scratch_err);
+  // k_grad writes the scaled gradient of every node before k_div reads any of it, so the
+  // scratch needs no clearing; the kernels are launched directly (no timer events)
+  GradArgs g = grad_args(c);
+  g.f = press; g.oxy = reinterpret_cast<double2 *>(c->scratch_dev); g.o2 = c->scratch_dev + 2 * (size_t)c->n;
+  g.impfl = impfl;
+  dim3 gg(c->gather_grid), b(NFT_BLOCK);
+  if (c->col_on)
+    hipLaunchKernelGGL((k_grad<GRAD_LAPLC, true>), gg, b, 0, c->stream, c->inc, c->S, g);
+  else
+    hipLaunchKernelGGL((k_grad<GRAD_LAPLC, false>), gg, b, 0, c->stream, c->inc, c->S, g);
+  HIPCHK(hipGetLastError());
+  DivArgs d = div_args(c);
+  d.qxy = g.oxy; d.q2 = g.o2; d.out = alap;
+  div_go<DIV_LAPLC, 0, 0>(c, d);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 int nft_laplc32(nft_ctx *c, const double *press, double *alap, int impfl) {

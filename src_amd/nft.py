@@ -79,6 +79,7 @@ _SIGS = {
     "nft_laplc": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, ctypes.c_int]),
     "nft_laplc_t": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, ctypes.c_int]),
     "nft_laplc32": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, ctypes.c_int]),
+    "nft_laplc_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "nft_prfr": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p, f64p, ctypes.c_int]),
     "nft_bcv": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, f64p, ctypes.c_double]),
     "nft_div": (ctypes.c_int, [ctypes.c_void_p, f64p, f64p, f64p, f64p, ctypes.c_int]),
@@ -318,6 +319,13 @@ class Solver:
         out = np.zeros(self.n)
         _chk(lib().nft_laplc(self.h, _p(self._vec(p, "press")), _p(out), impfl), "nft_laplc")
         return out
+
+    def laplc_noalloc(self, press_ptr, alap_ptr, impfl):
+        """laplc on device arrays given by address (npoin float64 each), with the
+        intermediate in the library's fixed scratch array: allocates nothing (no reference
+        counterpart). Meshes of more than 1,048,576 nodes are rejected."""
+        _chk(lib().nft_laplc_noalloc(self.h, ctypes.c_void_p(int(press_ptr)), ctypes.c_void_p(int(alap_ptr)),
+                                     int(impfl)), "nft_laplc_noalloc")
 
     def laplc32(self, p, impfl):
         """The FP32 operator of precision 2 (no reference counterpart)."""
