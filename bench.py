@@ -332,6 +332,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--learned-train-steps", type=int, default=100)
+    ap.add_argument("--no-learned-study", dest="learned_study", action="store_false",
+                    help="skip the learned preconditioner's training and evaluation on its 40,000-node workloads")
     ap.add_argument("--mixed-error-steps", type=int, default=1000,
                     help="steps of the mixed-precision error report on the 40,000-node mesh (0: skip)")
     ap.add_argument("--long-run", type=int, default=0,
@@ -475,10 +477,19 @@ def main():
             line["variants"][name] = v
             log(f"[{name}] {v['value']} steps/s, {v['iters_per_solve']} iterations/solve, "
                 f"{v['solve_ms']} ms/solve, field error {v['field_error_vs_fp64']}")
-        line["variants"]["learned"] = learned_run(mesh, cfg, mk, args.learned_train_steps, max(args.warmup, 1))
+        # the network's MFMA throughput at this size (a briefly trained network: what the
+        # correction and its fallback cost here, not a trained preconditioner)
         line["variants"]["learned_mixed_op"] = learned_run(mesh, cfg, mk, args.learned_train_steps,
                                                            max(args.warmup, 1), "[learned, mixed_op]",
                                                            precision="mixed_op")
+        if args.learned_study:
+            # the learned preconditioner on its own workloads, trained as specified: 20,000
+            # pairs from FP64 solves on 20 seeded hill heights (250-1500 m) of the 40,000-node
+            # mesh, then the mixed-precision workload and the 1 km hill / N = 0.005 workload
+            from src_amd import learned as Lr
+            t0 = time.perf_counter()
+            line["variants"]["learned_workloads"] = Lr.study(log=log)
+            log(f"[learned workloads] {time.perf_counter() - t0:.1f} s")
         if args.mesh == "production" and args.mixed_error_steps > 0:
             t0 = time.perf_counter()
             line["variants"]["mixed_error_small"] = mixed_error_run(args.mixed_error_steps, 100)

@@ -276,6 +276,11 @@ int nft_set_mixed_drop(nft_ctx *ctx, double drop, double *current);
 int nft_net_init(nft_ctx *ctx, int hidden, uint64_t seed, int max_batch);
 int nft_net_info(nft_ctx *ctx, int *kp, int *levels, int *ncol, int64_t *nparams);
 int nft_net_params(nft_ctx *ctx, float *host, int64_t count, int set);
+/* the whole training state, so training can continue in another context of the same
+ * network shape (e.g. on another mesh's pairs): weights (layout of nft_net_params), Adam's
+ * first and second moments in the same layout, then the Adam step count; count >= 3 *
+ * nparams + 1 floats */
+int nft_net_state(nft_ctx *ctx, float *host, int64_t count, int set);
 /* forward (and with T, loss + backward) on a host batch [nsamples*ncol][Kp] */
 int nft_net_eval(nft_ctx *ctx, const float *X, const float *T, int nsamples, float *Y, float *grad,
                  double *loss);

@@ -1802,6 +1802,7 @@ int build_geometry(nft_ctx *c, const nft_mesh *m, Geometry &G) {
   Incidence &g = G.inc;
   g.n = n; g.nchunk = nchunk; g.chunk_ptr = d_chunk; g.deg = d_deg; g.nbr = d_nbr; g.eid = d_eid;
   g.col_L = 0;
+  g.col_lds = 0;
   if (colL > 0) {
     std::vector<int2> tab(tnb.size() + xnb.size());
     for (size_t q = 0; q < tnb.size(); ++q) tab[q] = make_int2(tnb[q], tco[q]);
@@ -1813,6 +1814,14 @@ int build_geometry(nft_ctx *c, const nft_mesh *m, Geometry &G) {
       return rc;
     g.col_L = colL;
     g.tptr = d_tptr; g.xptr = d_xptr; g.tab = d_tab; g.xoff = (int)tnb.size();
+    // wavefront copies of the slot entries in LDS: every wavefront inside one column and
+    // every column's entries within the copy's size
+    int tmax = 0, xmax = 0;
+    for (int cc = 0; cc < m->ncol; ++cc) {
+      tmax = std::max(tmax, tptr[cc + 1] - tptr[cc]);
+      xmax = std::max(xmax, xptr[cc + 1] - xptr[cc]);
+    }
+    g.col_lds = (colL % NFT_WAVE == 0 && tmax <= NFT_LDS_TPL && xmax <= NFT_LDS_EXP) ? 1 : 0;
     G.col_on = 1;
     G.col_bytes = 4.0 * (tptr.size() + xptr.size()) + 8.0 * tab.size();
   }
@@ -2652,6 +2661,38 @@ int nft_net_pair_loss(nft_ctx *c, int first, int count, double *loss, float *gra
 // Adam on the stored pairs [0, use_pairs) (all when use_pairs <= 0) with the physics-informed
 // loss: `steps` minibatches of `batch` pairs drawn with a fixed-seed generator; losses of the
 // first and last minibatch returned
+// the whole training state: weights (nft_net_params' layout), Adam's first and second
+// moments in the same layout, then the step count
+int nft_net_state(nft_ctx *c, float *host, int64_t count, int set) {
+  if (!c || !c->net.ready || !host) return set_err(-1, "nft_net_state: bad argument");
+  auto &n = c->net;
+  int64_t P = 0;
+  for (int l = 0; l < 6; ++l) P += (int64_t)n.K[l] * n.N[l] + n.N[l];
+  if (count < 3 * P + 1) return set_err(-1, "nft_net_state: buffer too small (3 * nparams + 1 floats)");
+  int rc;
+  if ((rc = nft_net_params(c, host, P, set))) return rc;
+  int64_t off = P;
+  for (int part = 0; part < 2; ++part) {
+    float **arr = part == 0 ? n.m : n.v;
+    for (int l = 0; l < 6; ++l) {
+      for (int wb = 0; wb < 2; ++wb) {
+        size_t k = wb == 0 ? (size_t)n.K[l] * n.N[l] : (size_t)n.N[l];
+        if (set) HIPCHK(hipMemcpy(arr[2 * l + wb], host + off, k * sizeof(float), hipMemcpyHostToDevice));
+        else HIPCHK(hipMemcpy(host + off, arr[2 * l + wb], k * sizeof(float), hipMemcpyDeviceToHost));
+        off += (int64_t)k;
+      }
+    }
+  }
+  if (set) {
+    if (!(host[off] >= 0.f) || host[off] > 1e9f) return set_err(-1, "nft_net_state: bad step count");
+    n.step = (int)host[off];
+  } else {
+    host[off] = (float)n.step;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 int nft_net_train(nft_ctx *c, int steps, int batch, double lr, uint64_t seed, int use_pairs, double *loss_first,
                   double *loss_last) {
   if (!c || !c->net.ready || steps < 1 || batch < 1) return set_err(-1, "nft_net_train: bad argument");

@@ -57,6 +57,10 @@ struct Incidence {
   // the bottom and top nodes (xptr), so every lane reads its slot with one 8-byte load
   const int2 *tab;
   int xoff;
+  // 1: every wavefront lies inside one column (col_L a multiple of 64) and the column's
+  // template and explicit slots fit NFT_LDS_TPL / NFT_LDS_EXP entries, so each wavefront
+  // copies them to LDS once and reads its slots from there instead of from memory
+  int col_lds;
   // edge metrics for the gradient/divergence sweeps as a 16-byte (Sx, Sy) pair and Sz:
   // two aligned loads per slot instead of three
   const double2 *sxy;
@@ -82,18 +86,47 @@ __device__ __forceinline__ void load_metric(const Incidence &g, int e, double &s
 // Per-lane view of the columnar form.
 struct ColLane {
   int k;       // level
-  int base;    // tab index of slot 0
+  int base;    // tab index of slot 0 (in the wavefront's LDS copy when lds is set)
   int stride;  // 1 (template) or 2 (explicit bottom/top slots)
   int kk;      // level offset added to template entries (0 for explicit slots)
+  bool lds;    // slots read from the wavefront's LDS copy of its column's entries
 };
 
+#ifndef NFT_COL_LDS
+#define NFT_COL_LDS 0   // stage each wavefront's column slot entries in LDS when possible
+#endif
+#define NFT_LDS_TPL 32  // template entries per wavefront copy
+#define NFT_LDS_EXP 32  // interleaved explicit bottom/top entries per wavefront copy
+
+// this wavefront's part of the block's LDS copy of the slot entries
+__device__ __forceinline__ int2 *wave_slots() {
+  __shared__ int2 t[NFT_WPB * (NFT_LDS_TPL + NFT_LDS_EXP)];
+  return t + (threadIdx.x >> 6) * (NFT_LDS_TPL + NFT_LDS_EXP);
+}
+
+// Called by every lane of a wavefront (whole wavefronts leave before it, never single lanes).
 __device__ __forceinline__ void col_setup(const Incidence &g, int i, ColLane &C) {
   int c = i / g.col_L;
   C.k = i - c * g.col_L;
   const bool inner = C.k > 0 && C.k < g.col_L - 1;
-  C.base = inner ? g.tptr[c] : g.xoff + g.xptr[c] + (C.k == 0 ? 0 : 1);
   C.stride = inner ? 1 : 2;
   C.kk = inner ? C.k : -1;
+  C.lds = NFT_COL_LDS && g.col_lds;
+  if (C.lds) {
+    // the wavefront lies in column c: copy its template and explicit entries once
+    int2 *t = wave_slots();
+    const int lane = threadIdx.x & 63;
+    const int t0 = g.tptr[c], nt = g.tptr[c + 1] - t0;
+    const int x0 = g.xoff + g.xptr[c], nx = g.xptr[c + 1] - g.xptr[c];
+    if (lane < nt) t[lane] = g.tab[t0 + lane];
+    if (lane < nx) t[NFT_LDS_TPL + lane] = g.tab[x0 + lane];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    C.base = inner ? 0 : NFT_LDS_TPL + (C.k == 0 ? 0 : 1);
+  } else {
+    C.base = inner ? g.tptr[c] : g.xoff + g.xptr[c] + (C.k == 0 ? 0 : 1);
+  }
 }
 
 template <typename T>
@@ -141,7 +174,7 @@ __device__ __forceinline__ bool lane_setup(const Incidence &g, Lane &L) {
 template <bool COL>
 __device__ __forceinline__ int2 slot_raw(const Incidence &g, const Lane &L, const ColLane &C, int m) {
   const int mm = L.deg > 0 ? min(m, L.deg - 1) : 0;
-  if (COL) return g.tab[C.base + mm * C.stride];
+  if (COL) return C.lds ? wave_slots()[C.base + mm * C.stride] : g.tab[C.base + mm * C.stride];
   const int slot = L.base + mm * NFT_WAVE + L.lane;
   return make_int2(g.nbr[slot], g.eid[slot]);
 }
@@ -840,14 +873,14 @@ __global__ void __launch_bounds__(64) k_reduce_final(const double *partials, int
     gcr_finalize(op.dots, acc, scal, op.epa);
   } else if (op.kind == FK_UPDATE) {
     scal[SC_DVMX] = acc[0];
-    scal[SC_RL2] = acc[1];
+    scal[SC_RL2] = pick(acc, 1);
     if (op.ctl) {
       double done = 0.0;
-      if (!(acc[1] == acc[1])) done = 2.0;   // NaN residual: stop, reported as not converged
+      if (!(pick(acc, 1) == pick(acc, 1))) done = 2.0;   // NaN residual: stop, reported as not converged
       else if (acc[0] <= op.eps && op.it >= op.itmn) done = 1.0;
-      else if (acc[1] >= scal[SC_RL20]) done = 2.0;
+      else if (pick(acc, 1) >= scal[SC_RL20]) done = 2.0;
       else {
-        scal[SC_RL20] = fmax(acc[1], op.epa);
+        scal[SC_RL20] = fmax(pick(acc, 1), op.epa);
         scal[SC_NLC] = scal[SC_NLC] + 1.0;
       }
       if (done != 0.0) {
@@ -856,13 +889,13 @@ __global__ void __launch_bounds__(64) k_reduce_final(const double *partials, int
       }
     }
   } else if (op.kind == FK_DIR) {
-    double ax2 = fmax(op.epa, acc[1]);
+    double ax2 = fmax(op.epa, pick(acc, 1));
     scal[SC_AX2 + op.slot] = ax2;
     scal[SC_RAX] = acc[0];
     scal[SC_BETA] = -acc[0] / ax2;
   } else {
     scal[op.sum_slot] = acc[0];
-    scal[op.max_slot] = acc[1];
+    scal[op.max_slot] = pick(acc, 1);
   }
 }
 
@@ -1432,13 +1465,6 @@ __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
 template <typename T>
 __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
 
-// Slot m of node i for the MPDATA sweeps: neighbour j and signed edge e (SELL-64 or
-// columnar form, which hold the same slots in the same order)
-template <bool COL>
-__device__ __forceinline__ void mp_slot(const Incidence &g, const Lane &L, const ColLane &C, int m, int &j, int &e) {
-  slot_decode<COL>(C, slot_raw<COL>(g, L, C, m), j, e);
-}
-
 // Edge metric of the MPDATA sweeps: FP64 from the (Sx, Sy) pair copy, FP32 from its own
 // arrays (the same values either way)
 template <typename T>
@@ -1455,11 +1481,32 @@ __device__ __forceinline__ void mp_metric(const Incidence &g, const EdgeS<T> &S,
 // edge quantities are written with the edge's (i1, i2) roles where they are not symmetric
 // (p2 - p1, the limiter's ratios); sums and products of the two ends are symmetric in IEEE
 // arithmetic, so both end nodes of an edge obtain the reference's bits. Each node's own
-// values are read once, before its slot loop.
+// values are read once, before its slot loop. As in the gradient and divergence sweeps,
+// every lane of a wavefront walks the chunk's full slot width with clamped (always valid)
+// slot entries, the next batch's entries are loaded while the current batch's gathers are
+// in flight, and slots past a node's degree are dropped by selects, not branches, so the
+// wavefront never diverges around its loads.
+#ifndef NFT_BATCH_MP
+#define NFT_BATCH_MP 2   // slots per gather batch of the MPDATA sweeps (1 and 2 measured: 2 best)
+#endif
+
+// the per-slot gathers common to the sweeps: neighbour j, signed edge e, metric (sx, sy, sz)
+template <typename T, bool COL, int B>
+__device__ __forceinline__ void mp_batch_slots(const Incidence &g, const EdgeS<T> &S, const ColLane &C,
+                                               const int2 (&rn)[B], int (&jj)[B], int (&ee)[B], T (&sx)[B],
+                                               T (&sy)[B], T (&sz)[B]) {
+#pragma unroll
+  for (int u = 0; u < B; ++u) {
+    slot_decode<COL>(C, rn[u], jj[u], ee[u]);
+    const int e = ee[u] >= 0 ? ee[u] : ~ee[u];
+    mp_metric<T>(g, S, e, sx[u], sy[u], sz[u]);
+  }
+}
 
 // pass 1: donor-cell fluxes, boundary closure and update (reference :3094-3174)
 template <typename T, int NF, bool COL>
 __global__ void __launch_bounds__(NFT_BLOCK) k_mp_donor(Incidence g, EdgeS<T> S, MpArgs<T, NF> a) {
+  constexpr int B = NFT_BATCH_MP;
   Lane L;
   if (!lane_setup(g, L)) return;
   ColLane C{};
@@ -1471,23 +1518,40 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_donor(Incidence g, EdgeS<T> S,
 #pragma unroll
   for (int f = 0; f < NF; ++f) { pi[f] = (T)a.psi[f][i]; arh2[f] = (T)0; }
   T divv = (T)0;
-  for (int m = 0; m < L.deg; ++m) {
-    int j, e;
-    mp_slot<COL>(g, L, C, m, j, e);
-    const bool first = e >= 0;
-    const int ee = first ? e : ~e;
-    T Uj = a.U[j], Vj = a.V[j], Wj = a.W[j];
-    T sx, sy, sz;
-    mp_metric<T>(g, S, ee, sx, sy, sz);
-    T aun = edge_aun<T>(Ui, Uj, Vi, Vj, Wi, Wj, sx, sy, sz);
-    T ap = tmax<T>((T)0, aun), an = tmin<T>((T)0, aun);
+  int2 rn[B];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      T pj = (T)a.psi[f][j];
-      T fl = first ? pi[f] * ap + pj * an : pj * ap + pi[f] * an;
-      arh2[f] = first ? arh2[f] + fl : arh2[f] - fl;
+  for (int u = 0; u < B; ++u) rn[u] = slot_raw<COL>(g, L, C, u);
+  for (int m0 = 0; m0 < L.width; m0 += B) {
+    int jj[B], ee[B];
+    T sx[B], sy[B], sz[B];
+    mp_batch_slots<T, COL, B>(g, S, C, rn, jj, ee, sx, sy, sz);
+    if (m0 + B < L.width) {
+#pragma unroll
+      for (int u = 0; u < B; ++u) rn[u] = slot_raw<COL>(g, L, C, m0 + B + u);
     }
-    divv = first ? divv + aun : divv - aun;
+    T Uj[B], Vj[B], Wj[B], pj[B][NF];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int j = jj[u];
+      Uj[u] = a.U[j]; Vj[u] = a.V[j]; Wj[u] = a.W[j];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) pj[u][f] = (T)a.psi[f][j];
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const bool ok = m0 + u < L.deg;
+      const bool first = ee[u] >= 0;
+      T aun = edge_aun<T>(Ui, Uj[u], Vi, Vj[u], Wi, Wj[u], sx[u], sy[u], sz[u]);
+      T ap = tmax<T>((T)0, aun), an = tmin<T>((T)0, aun);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        T fl = first ? pi[f] * ap + pj[u][f] * an : pj[u][f] * ap + pi[f] * an;
+        T nv = first ? arh2[f] + fl : arh2[f] - fl;
+        arh2[f] = ok ? nv : arh2[f];
+      }
+      T nd = first ? divv + aun : divv - aun;
+      divv = ok ? nd : divv;
+    }
   }
   for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
     int fl_ = g.bp_flag[b];
@@ -1511,6 +1575,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_donor(Incidence g, EdgeS<T> S,
 // gradients of the donor-cell fields (derivative x3 per field, reference :3182-3184)
 template <typename T, int NF, bool COL>
 __global__ void __launch_bounds__(NFT_BLOCK) k_mp_grad(Incidence g, EdgeS<T> S, MpArgs<T, NF> a) {
+  constexpr int B = NFT_BATCH_MP;
   Lane L;
   if (!lane_setup(g, L)) return;
   ColLane C{};
@@ -1520,20 +1585,36 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_grad(Incidence g, EdgeS<T> S, 
   T pi[NF], gx[NF], gy[NF], gz[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) { pi[f] = (T)a.psi1[f][i]; gx[f] = gy[f] = gz[f] = (T)0; }
-  for (int m = 0; m < L.deg; ++m) {
-    int j, e;
-    mp_slot<COL>(g, L, C, m, j, e);
-    const bool first = e >= 0;
-    const int ee = first ? e : ~e;
-    T sx, sy, sz;
-    mp_metric<T>(g, S, ee, sx, sy, sz);
+  int2 rn[B];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      T u = (pi[f] + (T)a.psi1[f][j]) * (T)0.5;
-      T tx = sx * u, ty = sy * u, tz = sz * u;
-      gx[f] = first ? gx[f] + tx : gx[f] - tx;
-      gy[f] = first ? gy[f] + ty : gy[f] - ty;
-      gz[f] = first ? gz[f] + tz : gz[f] - tz;
+  for (int u = 0; u < B; ++u) rn[u] = slot_raw<COL>(g, L, C, u);
+  for (int m0 = 0; m0 < L.width; m0 += B) {
+    int jj[B], ee[B];
+    T sx[B], sy[B], sz[B];
+    mp_batch_slots<T, COL, B>(g, S, C, rn, jj, ee, sx, sy, sz);
+    if (m0 + B < L.width) {
+#pragma unroll
+      for (int u = 0; u < B; ++u) rn[u] = slot_raw<COL>(g, L, C, m0 + B + u);
+    }
+    T pj[B][NF];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) pj[u][f] = (T)a.psi1[f][jj[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const bool ok = m0 + u < L.deg;
+      const bool first = ee[u] >= 0;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        T uu = (pi[f] + pj[u][f]) * (T)0.5;
+        T tx = sx[u] * uu, ty = sy[u] * uu, tz = sz[u] * uu;
+        T nx = first ? gx[f] + tx : gx[f] - tx;
+        T ny = first ? gy[f] + ty : gy[f] - ty;
+        T nz = first ? gz[f] + tz : gz[f] - tz;
+        gx[f] = ok ? nx : gx[f]; gy[f] = ok ? ny : gy[f]; gz[f] = ok ? nz : gz[f];
+      }
     }
   }
   for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
@@ -1565,6 +1646,8 @@ __device__ __forceinline__ T anti_flux(T p1, T p2, T gsx, T gsy, T gsz, T aun, T
 // FCT bounds from the donor-cell solution, in/out fluxes and ratios (reference :3228-3280)
 template <typename T, int NF, bool COL>
 __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, EdgeS<T> S, MpArgs<T, NF> a) {
+  constexpr int B = NFT_BATCH_MP;
+  typedef typename MpArgs<T, NF>::T2 T2;
   Lane L;
   if (!lane_setup(g, L)) return;
   ColLane C{};
@@ -1578,39 +1661,59 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, EdgeS<T> S, M
   for (int f = 0; f < NF; ++f) {
     mx[f] = (T)-1.e15; mn[f] = (T)1.e15; rin[f] = (T)0; rout[f] = (T)0;
     pi[f] = (T)a.psi1[f][i];
-    const typename MpArgs<T, NF>::T2 g2 = a.gxy[f][i];
+    const T2 g2 = a.gxy[f][i];
     gxi[f] = g2.x; gyi[f] = g2.y; gzi[f] = a.gz[f][i];
   }
-  for (int m = 0; m < L.deg; ++m) {
-    int j, e;
-    mp_slot<COL>(g, L, C, m, j, e);
-    const bool first = e >= 0;
-    const int ee = first ? e : ~e;
-    T Uj = a.U[j], Vj = a.V[j], Wj = a.W[j];
-    T sx, sy, sz;
-    mp_metric<T>(g, S, ee, sx, sy, sz);
-    T aun = edge_aun<T>(Ui, Uj, Vi, Vj, Wi, Wj, sx, sy, sz);
-    T vrj = a.vr[j];
-    T volt = (T)0.5 * (vri + vrj);
-    T U12 = Ui + Uj, V12 = Vi + Vj, W12 = Wi + Wj;
+  int2 rn[B];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      T pj = (T)a.psi1[f][j];
-      const typename MpArgs<T, NF>::T2 gj = a.gxy[f][j];
-      T A = anti_flux<T>(first ? pi[f] : pj, first ? pj : pi[f], gxi[f] + gj.x, gyi[f] + gj.y,
-                         gzi[f] + a.gz[f][j], aun, volt, U12, V12, W12, a.dt);
-      mx[f] = tmax<T>(tmax<T>(mx[f], pi[f]), pj);
-      mn[f] = tmin<T>(tmin<T>(mn[f], pi[f]), pj);
-      T pp = tmax<T>((T)0, A), pn = -tmin<T>((T)0, A);
-      rin[f] = rin[f] + (first ? pn : pp);
-      rout[f] = rout[f] + (first ? pp : pn);
+  for (int u = 0; u < B; ++u) rn[u] = slot_raw<COL>(g, L, C, u);
+  for (int m0 = 0; m0 < L.width; m0 += B) {
+    int jj[B], ee[B];
+    T sx[B], sy[B], sz[B];
+    mp_batch_slots<T, COL, B>(g, S, C, rn, jj, ee, sx, sy, sz);
+    if (m0 + B < L.width) {
+#pragma unroll
+      for (int u = 0; u < B; ++u) rn[u] = slot_raw<COL>(g, L, C, m0 + B + u);
+    }
+    T Uj[B], Vj[B], Wj[B], vrj[B], pj[B][NF], gjx[B][NF], gjy[B][NF], gjz[B][NF];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int j = jj[u];
+      Uj[u] = a.U[j]; Vj[u] = a.V[j]; Wj[u] = a.W[j]; vrj[u] = a.vr[j];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        pj[u][f] = (T)a.psi1[f][j];
+        const T2 gj = a.gxy[f][j];
+        gjx[u][f] = gj.x; gjy[u][f] = gj.y; gjz[u][f] = a.gz[f][j];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const bool ok = m0 + u < L.deg;
+      const bool first = ee[u] >= 0;
+      T aun = edge_aun<T>(Ui, Uj[u], Vi, Vj[u], Wi, Wj[u], sx[u], sy[u], sz[u]);
+      T volt = (T)0.5 * (vri + vrj[u]);
+      T U12 = Ui + Uj[u], V12 = Vi + Vj[u], W12 = Wi + Wj[u];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const T p = pj[u][f];
+        T A = anti_flux<T>(first ? pi[f] : p, first ? p : pi[f], gxi[f] + gjx[u][f], gyi[f] + gjy[u][f],
+                           gzi[f] + gjz[u][f], aun, volt, U12, V12, W12, a.dt);
+        T nmx = tmax<T>(tmax<T>(mx[f], pi[f]), p);
+        T nmn = tmin<T>(tmin<T>(mn[f], pi[f]), p);
+        T pp = tmax<T>((T)0, A), pn = -tmin<T>((T)0, A);
+        T nin = rin[f] + (first ? pn : pp);
+        T nout = rout[f] + (first ? pp : pn);
+        mx[f] = ok ? nmx : mx[f]; mn[f] = ok ? nmn : mn[f];
+        rin[f] = ok ? nin : rin[f]; rout[f] = ok ? nout : rout[f];
+      }
     }
   }
   T vl = a.vol[i], rh = a.rh0[i];
   const T tol = (T)1.e-10;
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    typename MpArgs<T, NF>::T2 r;
+    T2 r;
     r.x = (mx[f] - pi[f]) * vl * rh / (rin[f] * a.dt + tol);
     r.y = (pi[f] - mn[f]) * vl * rh / (rout[f] * a.dt + tol);
     a.cpn[f][i] = r;
@@ -1621,6 +1724,8 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, EdgeS<T> S, M
 template <typename T, int NF, bool COL>
 __global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, EdgeS<T> S, MpArgs<T, NF> a,
                                                         double *const *out) {
+  constexpr int B = NFT_BATCH_MP;
+  typedef typename MpArgs<T, NF>::T2 T2;
   Lane L;
   if (!lane_setup(g, L)) return;
   ColLane C{};
@@ -1634,35 +1739,54 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, EdgeS<T> S,
   for (int f = 0; f < NF; ++f) {
     arh2[f] = (T)0;
     pi[f] = (T)a.psi1[f][i];
-    const typename MpArgs<T, NF>::T2 g2 = a.gxy[f][i], c2 = a.cpn[f][i];
+    const T2 g2 = a.gxy[f][i], c2 = a.cpn[f][i];
     gxi[f] = g2.x; gyi[f] = g2.y; gzi[f] = a.gz[f][i];
     cpi[f] = c2.x; cni[f] = c2.y;
   }
-  for (int m = 0; m < L.deg; ++m) {
-    int j, e;
-    mp_slot<COL>(g, L, C, m, j, e);
-    const bool first = e >= 0;
-    const int ee = first ? e : ~e;
-    T Uj = a.U[j], Vj = a.V[j], Wj = a.W[j];
-    T sx, sy, sz;
-    mp_metric<T>(g, S, ee, sx, sy, sz);
-    T aun = edge_aun<T>(Ui, Uj, Vi, Vj, Wi, Wj, sx, sy, sz);
-    T vrj = a.vr[j];
-    T volt = (T)0.5 * (vri + vrj);
-    T U12 = Ui + Uj, V12 = Vi + Vj, W12 = Wi + Wj;
+  int2 rn[B];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      T pj = (T)a.psi1[f][j];
-      const typename MpArgs<T, NF>::T2 gj = a.gxy[f][j], cj = a.cpn[f][j];
-      T A = anti_flux<T>(first ? pi[f] : pj, first ? pj : pi[f], gxi[f] + gj.x, gyi[f] + gj.y,
-                         gzi[f] + a.gz[f][j], aun, volt, U12, V12, W12, a.dt);
-      const T cpj = cj.x, cnj = cj.y;
-      // edge roles: cp/cn of i2 and i1
-      const T cp2 = first ? cpj : cpi[f], cn1 = first ? cni[f] : cnj;
-      const T cn2 = first ? cnj : cni[f], cp1 = first ? cpi[f] : cpj;
-      T lim = A > (T)0 ? tmin<T>(tmin<T>((T)1, cp2), cn1) : tmin<T>(tmin<T>((T)1, cn2), cp1);
-      T fl = A * lim;
-      arh2[f] = first ? arh2[f] + fl : arh2[f] - fl;
+  for (int u = 0; u < B; ++u) rn[u] = slot_raw<COL>(g, L, C, u);
+  for (int m0 = 0; m0 < L.width; m0 += B) {
+    int jj[B], ee[B];
+    T sx[B], sy[B], sz[B];
+    mp_batch_slots<T, COL, B>(g, S, C, rn, jj, ee, sx, sy, sz);
+    if (m0 + B < L.width) {
+#pragma unroll
+      for (int u = 0; u < B; ++u) rn[u] = slot_raw<COL>(g, L, C, m0 + B + u);
+    }
+    T Uj[B], Vj[B], Wj[B], vrj[B], pj[B][NF], gjx[B][NF], gjy[B][NF], gjz[B][NF], cpj[B][NF], cnj[B][NF];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int j = jj[u];
+      Uj[u] = a.U[j]; Vj[u] = a.V[j]; Wj[u] = a.W[j]; vrj[u] = a.vr[j];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        pj[u][f] = (T)a.psi1[f][j];
+        const T2 gj = a.gxy[f][j], cj = a.cpn[f][j];
+        gjx[u][f] = gj.x; gjy[u][f] = gj.y; gjz[u][f] = a.gz[f][j];
+        cpj[u][f] = cj.x; cnj[u][f] = cj.y;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const bool ok = m0 + u < L.deg;
+      const bool first = ee[u] >= 0;
+      T aun = edge_aun<T>(Ui, Uj[u], Vi, Vj[u], Wi, Wj[u], sx[u], sy[u], sz[u]);
+      T volt = (T)0.5 * (vri + vrj[u]);
+      T U12 = Ui + Uj[u], V12 = Vi + Vj[u], W12 = Wi + Wj[u];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const T p = pj[u][f];
+        T A = anti_flux<T>(first ? pi[f] : p, first ? p : pi[f], gxi[f] + gjx[u][f], gyi[f] + gjy[u][f],
+                           gzi[f] + gjz[u][f], aun, volt, U12, V12, W12, a.dt);
+        // edge roles: cp/cn of i2 and i1
+        const T cp2 = first ? cpj[u][f] : cpi[f], cn1 = first ? cni[f] : cnj[u][f];
+        const T cn2 = first ? cnj[u][f] : cni[f], cp1 = first ? cpi[f] : cpj[u][f];
+        T lim = A > (T)0 ? tmin<T>(tmin<T>((T)1, cp2), cn1) : tmin<T>(tmin<T>((T)1, cn2), cp1);
+        T fl = A * lim;
+        T nv = first ? arh2[f] + fl : arh2[f] - fl;
+        arh2[f] = ok ? nv : arh2[f];
+      }
     }
   }
   for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {

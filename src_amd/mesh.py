@@ -352,6 +352,13 @@ def small_spec():
                     hill_h0=500.0, hill_a=2000.0)
 
 
+def small_hill_spec(h0):
+    """The 40,000-node mesh (2,000 jittered columns, same jitter and triangulation) with a
+    hill of height h0: the learned preconditioner's workload and its training meshes."""
+    import dataclasses as _dc
+    return _dc.replace(small_spec(), hill_h0=float(h0))
+
+
 def production_spec(column_order="grid"):
     """65,536 jittered points (256 x 256) over 200 km x 200 km, 64 uniform levels over
     20 km, hill h0 = 1 km, half-width 2 km (4,194,304 nodes); columns row by row unless
@@ -373,6 +380,10 @@ CONFIGS = {
     # plain FP64 at production size: dt=20 s, 1000 steps, GCR(4), N=0.01 (Fr=1)
     "production": RunConfig("production", production_spec(), u0=10.0, bvf=0.01, dt=20.0,
                             steps=1000, lord=4),
+    # learned preconditioner workload: the 40,000-node mesh with a 1 km hill, U=10, N=0.005
+    # (U/(N h0) = 2; the workload's own label says Fr = 0.5, the stated U, N and h0 give 2),
+    # dt=10 s, flexible GCR(4)
+    "learned": RunConfig("learned", small_hill_spec(1000.0), u0=10.0, bvf=0.005, dt=10.0, steps=100, lord=4),
     # low-Froude stress case: U=5, N=0.025 (Fr=0.1), dt=15 s, 2000 steps, GCR(4)
     "stiff": RunConfig("stiff", stiff_spec(), u0=5.0, bvf=0.025, dt=15.0, steps=2000, lord=4),
 }

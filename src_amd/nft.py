@@ -103,6 +103,7 @@ _SIGS.update({
     "nft_net_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64)]),
     "nft_net_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int64, ctypes.c_int]),
+    "nft_net_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int64, ctypes.c_int]),
     "nft_net_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                     ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                     ctypes.POINTER(ctypes.c_double)]),
@@ -435,6 +436,19 @@ class Solver:
             return out
         v = np.ascontiguousarray(values, np.float32)
         _chk(lib().nft_net_params(self.h, _pf(v), v.size, 1), "nft_net_params")
+
+    def net_state(self, values=None):
+        """The training state (weights, Adam moments, step count) as one float32 array, or
+        set it from such an array."""
+        n = 3 * self.net_info()["nparams"] + 1
+        if values is None:
+            out = np.zeros(n, np.float32)
+            _chk(lib().nft_net_state(self.h, _pf(out), n, 0), "nft_net_state")
+            return out
+        v = np.ascontiguousarray(values, np.float32)
+        if v.size != n:
+            raise NftError(f"net_state: expected {n} values, got {v.size}")
+        _chk(lib().nft_net_state(self.h, _pf(v), v.size, 1), "nft_net_state")
 
     def net_eval(self, X, T=None):
         info = self.net_info()
