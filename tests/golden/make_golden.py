@@ -12,6 +12,8 @@ generated meshes, runs
   * the multigrid branch of precon (mg_flag=1) on the tiny mesh's generated nested coarse
     meshes: its pieces in `mgops` mode and 1 and 10 steps of the program (tiny_mg.npz),
   * 1 and 10 steps with the velprd advector (itraj=1) on the tiny mesh (tiny_velprd.npz),
+  * with --converged only: 1, 10 and 100 steps on the 40,000-node mesh of a reference build
+    whose GCR converges (no preconditioner, relative exit test; small_conv.npz),
 and stores inputs and outputs. Only numbers are stored.
 """
 from __future__ import annotations
@@ -57,10 +59,45 @@ def strat_literal(cfg):
     return repr(cfg.strat)
 
 
-def build_ref(cfg, itraj=0):
+def build_ref(cfg, itraj=0, iprc=1, relexit=0):
     exe = subprocess.run([os.path.join(ROOT, "oracle", "build_ref.sh"), str(cfg.lord),
-                          strat_literal(cfg), str(itraj)], check=True, capture_output=True, text=True)
+                          strat_literal(cfg), str(itraj), str(iprc), str(relexit)], check=True,
+                         capture_output=True, text=True)
     return exe.stdout.strip().splitlines()[-1]
+
+
+def converged_config():
+    """The 40,000-node mesh and flow with a GCR(3) that converges: no preconditioner (gcrk's
+    iprc=0, precon's p = rhs branch) and the relative exit max|r| <= 1e-8 max|b|."""
+    c = M.CONFIGS["small"]
+    return M.RunConfig("small", c.mesh, u0=c.u0, bvf=c.bvf, dt=c.dt, steps=100, lord=3,
+                       eps0=1e-8, exit_mode=1, itr=400, precon=0, itrp1=2, omega=0.5,
+                       sponge_depth=3000.0, sponge_tau=300.0)
+
+
+def write_converged():
+    """1, 10 and 100 steps of the reference itself in a converging regime (small_conv.npz):
+    built with iprc=0 and the relative exit test, the other settings as small_run.npz."""
+    ccfg = converged_config()
+    exe = build_ref(ccfg, iprc=0, relexit=1)
+    m = M.generate(ccfg.mesh)
+    stride = {1: 40, 10: 40, 100: 10}
+    with tempfile.TemporaryDirectory() as d:
+        runs = {}
+        for nt in (1, 10, 100):
+            st = run_steps(exe, m, ccfg, d, nt)
+            s = stride[nt]
+            for k in ("unk", "uuu", "frc", "unkdash"):
+                a = st[k]
+                runs[f"s{nt}_{k}_sample"] = a[:, ::s].copy()
+                runs[f"s{nt}_{k}_l2"] = np.sqrt((a * a).sum(axis=1))
+            runs[f"s{nt}_prpot_sample"] = st["prpot"][::s].copy()
+            runs[f"s{nt}_prpot_l2"] = np.array([np.sqrt((st["prpot"] ** 2).sum())])
+            runs[f"s{nt}_stride"] = np.array([s])
+            runs[f"s{nt}_niter"] = st["niter"]
+            print(f"converged run: {nt} steps done, niter {st['niter'].tolist()[:12]}...")
+        np.savez_compressed(os.path.join(OUT, "small_conv.npz"), mesh_checksum=mesh_checksum(m), **runs)
+    print("converged fixtures written")
 
 
 def mesh_checksum(m):
@@ -200,6 +237,9 @@ def run_steps(exe, m, cfg, d, ntime, mg=None):
 
 
 def main():
+    if "--converged" in sys.argv:
+        write_converged()
+        return
     only_tiny = "--tiny-only" in sys.argv
     tcfg = tiny_config()
     exe = build_ref(tcfg)

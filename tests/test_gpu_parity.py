@@ -242,6 +242,45 @@ def test_steps_small_100_line_vs_sequential_oracle(fixture, precon):
     s.close()
 
 
+def test_steps_small_100_converged_vs_reference():
+    """100 steps on the 40,000-node mesh in a regime where the reference's own GCR converges:
+    the reference built with no preconditioner (gcrk's iprc=0, precon's p = rhs branch) and a
+    relative exit test (max|r| <= 1e-8 max|b|), GCR(3); fixture small_conv.npz written by
+    the reference itself. The GPU FP64 path with the same settings, against it directly (only
+    the inner-product summation order differs): u, v, w, theta' and the advector within 1e-12
+    relative L2 (sampled field and full-field norm) after 1, 10 and 100 steps, GCR iterations
+    within one per solve."""
+    path = os.path.join(GOLD, "small_conv.npz")
+    g = np.load(path, allow_pickle=False)
+    m = mesh_for("small")
+    assert np.array_equal(g["mesh_checksum"], mesh_checksum(m))
+    s = solver(m, small_cfg(precon=0, exit_mode=1, eps0=1e-8, itr=400))
+    s.state_init()
+    its = [s.gcr_stats().niter]
+    done = 0
+    for nt in (1, 10, 100):
+        while done < nt:
+            st = s.run_steps(1)
+            assert st.gcr_unconverged == 0
+            its.append(s.gcr_stats().niter)
+            done += 1
+        st_ = int(g[f"s{nt}_stride"][0])
+        ref_its = g[f"s{nt}_niter"].tolist()
+        assert len(ref_its) == len(its)
+        assert all(abs(a - b) <= 1 for a, b in zip(its, ref_its)), (its, ref_its)
+        for name in ("unk", "uuu"):
+            a = s.state(name)
+            for k in range(a.shape[0]):
+                err = rel_l2(a[k, ::st_], g[f"s{nt}_{name}_sample"][k])
+                l2 = np.sqrt((a[k] * a[k]).sum())
+                ref = g[f"s{nt}_{name}_l2"][k]
+                print(f"step {nt} {name}[{k}] sampled rel L2 {err:.3e}, norm rel {abs(l2 - ref) / ref:.1e}")
+                assert err <= 1e-12, (nt, name, k, err)
+                assert abs(l2 - ref) <= 1e-12 * ref
+    print("iterations per solve (GPU)", its[:12], "...")
+    s.close()
+
+
 def test_steps_small_100_reference_parity_mode():
     """100 steps in the reference's own settings (point Jacobi, 120 GCR iterations per
     solve, never reaching eps0) against the flang reference (small_run.npz). In this

@@ -163,6 +163,32 @@ def test_small_mesh_steps_against_reference():
         assert its == g[f"s{nt}_niter"].tolist()
 
 
+def test_small_mesh_converged_steps_against_reference():
+    """40,000-node mesh, GCR(3) that converges: the reference built with no preconditioner
+    (gcrk's iprc=0) and the relative exit test max|r| <= 1e-8 max|b| (small_conv.npz). The
+    oracle with the same settings, in the reference's sequential summation order, equals it
+    bit for bit after 1 and 10 steps, iteration counts included; every solve converged
+    (counts below the 1200-iteration cap)."""
+    g = load("small_conv.npz")
+    m = mesh_for("small")
+    assert np.array_equal(g["mesh_checksum"], mesh_checksum(m))
+    o = Oracle(m, small_cfg(precon=0, exit_mode=1, eps0=1e-8, itr=400))
+    o.init()
+    its = [o.stats().niter]
+    done = 0
+    for nt in (1, 10):
+        while done < nt:
+            o.step()
+            its.append(o.stats().niter)
+            done += 1
+        s = int(g[f"s{nt}_stride"][0])
+        for k in ("unk", "uuu", "frc", "unkdash"):
+            a = o.state(k)
+            assert np.array_equal(a[:, ::s], g[f"s{nt}_{k}_sample"]), (nt, k)
+        assert its == g[f"s{nt}_niter"].tolist()
+    assert max(g["s100_niter"]) < 1200
+
+
 # ---- closed forms ------------------------------------------------------------------
 @pytest.mark.parametrize("key", ["tiny2", "tall", "tall3"])
 def test_line_solve_matches_dense(key):
