@@ -1802,7 +1802,6 @@ int build_geometry(nft_ctx *c, const nft_mesh *m, Geometry &G) {
   Incidence &g = G.inc;
   g.n = n; g.nchunk = nchunk; g.chunk_ptr = d_chunk; g.deg = d_deg; g.nbr = d_nbr; g.eid = d_eid;
   g.col_L = 0;
-  g.col_lds = 0;
   if (colL > 0) {
     std::vector<int2> tab(tnb.size() + xnb.size());
     for (size_t q = 0; q < tnb.size(); ++q) tab[q] = make_int2(tnb[q], tco[q]);
@@ -1814,14 +1813,6 @@ int build_geometry(nft_ctx *c, const nft_mesh *m, Geometry &G) {
       return rc;
     g.col_L = colL;
     g.tptr = d_tptr; g.xptr = d_xptr; g.tab = d_tab; g.xoff = (int)tnb.size();
-    // wavefront copies of the slot entries in LDS: every wavefront inside one column and
-    // every column's entries within the copy's size
-    int tmax = 0, xmax = 0;
-    for (int cc = 0; cc < m->ncol; ++cc) {
-      tmax = std::max(tmax, tptr[cc + 1] - tptr[cc]);
-      xmax = std::max(xmax, xptr[cc + 1] - xptr[cc]);
-    }
-    g.col_lds = (colL % NFT_WAVE == 0 && tmax <= NFT_LDS_TPL && xmax <= NFT_LDS_EXP) ? 1 : 0;
     G.col_on = 1;
     G.col_bytes = 4.0 * (tptr.size() + xptr.size()) + 8.0 * tab.size();
   }

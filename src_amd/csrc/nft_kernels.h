@@ -57,10 +57,6 @@ struct Incidence {
   // the bottom and top nodes (xptr), so every lane reads its slot with one 8-byte load
   const int2 *tab;
   int xoff;
-  // 1: every wavefront lies inside one column (col_L a multiple of 64) and the column's
-  // template and explicit slots fit NFT_LDS_TPL / NFT_LDS_EXP entries, so each wavefront
-  // copies them to LDS once and reads its slots from there instead of from memory
-  int col_lds;
   // edge metrics for the gradient/divergence sweeps as a 16-byte (Sx, Sy) pair and Sz:
   // two aligned loads per slot instead of three
   const double2 *sxy;
@@ -86,47 +82,18 @@ __device__ __forceinline__ void load_metric(const Incidence &g, int e, double &s
 // Per-lane view of the columnar form.
 struct ColLane {
   int k;       // level
-  int base;    // tab index of slot 0 (in the wavefront's LDS copy when lds is set)
+  int base;    // tab index of slot 0
   int stride;  // 1 (template) or 2 (explicit bottom/top slots)
   int kk;      // level offset added to template entries (0 for explicit slots)
-  bool lds;    // slots read from the wavefront's LDS copy of its column's entries
 };
 
-#ifndef NFT_COL_LDS
-#define NFT_COL_LDS 0   // stage each wavefront's column slot entries in LDS when possible
-#endif
-#define NFT_LDS_TPL 32  // template entries per wavefront copy
-#define NFT_LDS_EXP 32  // interleaved explicit bottom/top entries per wavefront copy
-
-// this wavefront's part of the block's LDS copy of the slot entries
-__device__ __forceinline__ int2 *wave_slots() {
-  __shared__ int2 t[NFT_WPB * (NFT_LDS_TPL + NFT_LDS_EXP)];
-  return t + (threadIdx.x >> 6) * (NFT_LDS_TPL + NFT_LDS_EXP);
-}
-
-// Called by every lane of a wavefront (whole wavefronts leave before it, never single lanes).
 __device__ __forceinline__ void col_setup(const Incidence &g, int i, ColLane &C) {
   int c = i / g.col_L;
   C.k = i - c * g.col_L;
   const bool inner = C.k > 0 && C.k < g.col_L - 1;
+  C.base = inner ? g.tptr[c] : g.xoff + g.xptr[c] + (C.k == 0 ? 0 : 1);
   C.stride = inner ? 1 : 2;
   C.kk = inner ? C.k : -1;
-  C.lds = NFT_COL_LDS && g.col_lds;
-  if (C.lds) {
-    // the wavefront lies in column c: copy its template and explicit entries once
-    int2 *t = wave_slots();
-    const int lane = threadIdx.x & 63;
-    const int t0 = g.tptr[c], nt = g.tptr[c + 1] - t0;
-    const int x0 = g.xoff + g.xptr[c], nx = g.xptr[c + 1] - g.xptr[c];
-    if (lane < nt) t[lane] = g.tab[t0 + lane];
-    if (lane < nx) t[NFT_LDS_TPL + lane] = g.tab[x0 + lane];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    C.base = inner ? 0 : NFT_LDS_TPL + (C.k == 0 ? 0 : 1);
-  } else {
-    C.base = inner ? g.tptr[c] : g.xoff + g.xptr[c] + (C.k == 0 ? 0 : 1);
-  }
 }
 
 template <typename T>
@@ -174,7 +141,7 @@ __device__ __forceinline__ bool lane_setup(const Incidence &g, Lane &L) {
 template <bool COL>
 __device__ __forceinline__ int2 slot_raw(const Incidence &g, const Lane &L, const ColLane &C, int m) {
   const int mm = L.deg > 0 ? min(m, L.deg - 1) : 0;
-  if (COL) return C.lds ? wave_slots()[C.base + mm * C.stride] : g.tab[C.base + mm * C.stride];
+  if (COL) return g.tab[C.base + mm * C.stride];
   const int slot = L.base + mm * NFT_WAVE + L.lane;
   return make_int2(g.nbr[slot], g.eid[slot]);
 }
