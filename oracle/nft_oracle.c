@@ -53,12 +53,34 @@ struct oc_ctx {
   double *mg_w[12];
   /* velprd work (itraj = 1): uuu1[3], ff[3], pprpot */
   double *vp[7];
+  /* periodic box: vol summed over the coincident sets (c->m.vol points here), and the
+   * set sums of one closure */
+  double *volc, *cuc;
 };
 
 static double amax1(double a, double b) { return a > b ? a : b; }
 static double amin1(double a, double b) { return a < b ? a : b; }
 
 static double *zalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
+
+/* cyclicr, cyclicr2, cyclicr4 (F:4377-4446), called in that order: every node of a
+ * coincident set takes the sum of the set's values, each list's sums formed before any is
+ * stored */
+static void cyclic_all(oc_ctx *c, double *u) {
+  const oc_mesh *m = &c->m;
+  double *uc = c->cuc;
+  for (int ic = 0; ic < m->ncoin; ++ic) uc[ic] = u[m->icoin[ic] - 1] + u[m->icoin[m->ncoin + ic] - 1];
+  for (int ic = 0; ic < m->ncoin; ++ic) { u[m->icoin[ic] - 1] = uc[ic]; u[m->icoin[m->ncoin + ic] - 1] = uc[ic]; }
+  for (int ic = 0; ic < m->ncoin2; ++ic) uc[ic] = u[m->icoin2[ic] - 1] + u[m->icoin2[m->ncoin2 + ic] - 1];
+  for (int ic = 0; ic < m->ncoin2; ++ic) { u[m->icoin2[ic] - 1] = uc[ic]; u[m->icoin2[m->ncoin2 + ic] - 1] = uc[ic]; }
+  const int q = m->ncoin4;
+  for (int ic = 0; ic < q; ++ic) {
+    const int32_t *s4 = m->icoin4;
+    uc[ic] = u[s4[ic] - 1] + u[s4[q + ic] - 1] + u[s4[2 * q + ic] - 1] + u[s4[3 * q + ic] - 1];
+  }
+  for (int ic = 0; ic < q; ++ic)
+    for (int k = 0; k < 4; ++k) u[m->icoin4[k * q + ic] - 1] = uc[ic];
+}
 
 static void build_line(oc_ctx *c);
 static void build_band(oc_ctx *c);
@@ -94,6 +116,22 @@ oc_ctx *oc_create(const oc_mesh *m, const oc_params *p) {
   c->p = *p;
   int n = m->npoin;
   c->n = n;
+  if (p->iper == 2) {
+    /* the periodic branches exist in FP64 only, and need the coincident sets */
+    if (p->precision != 0 || (m->ncoin && !m->icoin) || (m->ncoin2 && !m->icoin2) || (m->ncoin4 && !m->icoin4)) {
+      free(c);
+      return NULL;
+    }
+    int mc = m->ncoin > m->ncoin2 ? m->ncoin : m->ncoin2;
+    c->cuc = zalloc((size_t)(mc > m->ncoin4 ? mc : m->ncoin4));
+    /* mpsolver F:99-103: vol summed over the coincident sets before anything reads it */
+    c->volc = zalloc(n);
+    memcpy(c->volc, m->vol, sizeof(double) * (size_t)n);
+    cyclic_all(c, c->volc);
+    c->m.vol = c->volc;
+  } else {
+    c->m.ncoin = c->m.ncoin2 = c->m.ncoin4 = 0;
+  }
   c->rh0 = zalloc(n); c->th0 = zalloc(n); c->the = zalloc(n); c->ue = zalloc(n);
   c->ve = zalloc(n); c->coefw = zalloc(n); c->alpha = zalloc(n); c->d1 = zalloc(n);
   c->dthedz = zalloc(n);
@@ -181,7 +219,7 @@ oc_ctx *oc_create(const oc_mesh *m, const oc_params *p) {
           Z[i2] != d[4] && Z[i2] != d[5] && Z[i1] > hill && Z[i2] > hill) {
         double hx = fabs(m->sn[e]), hy = fabs(m->sn[m->nedge + e]), hz = fabs(m->sn[2 * (size_t)m->nedge + e]);
         double h = amax1(amax1(hx, hy), hz);
-        ddx = amin1(amin1(ddx, m->vol[i1] / h), m->vol[i2] / h);
+        ddx = amin1(amin1(ddx, c->m.vol[i1] / h), c->m.vol[i2] / h);
       }
     }
     c->beta1 = 0.5 * ddx * ddx * 1.0;
@@ -317,10 +355,12 @@ void oc_destroy(oc_ctx *c) {
   }
   for (int l = 0; l < 2; ++l) { oc_destroy(c->lv[l]); free(c->lv_match[l]); free(c->lv_imatch[l]); }
   for (int i = 0; i < 12; ++i) free(c->mg_w[i]);
+  free(c->volc); free(c->cuc);
   free(c);
 }
 
 double *oc_field(oc_ctx *c, const char *name) {
+  if (!strcmp(name, "vol")) return (double *)c->m.vol;   /* closed over the coincident sets in the periodic box */
   if (!strcmp(name, "rh0")) return c->rh0;
   if (!strcmp(name, "th0")) return c->th0;
   if (!strcmp(name, "the")) return c->the;
@@ -348,7 +388,7 @@ double oc_beta(oc_ctx *c) { return c->beta1; }
 void oc_get_stats(oc_ctx *c, oc_stats *s) { *s = c->st; }
 void oc_set_itr(oc_ctx *c, int itr) { c->p.itr = itr; }
 
-/* derivative(unk,dpdxa,idx), F:3943-4022, Cartesian branch; f32: the edge metrics sn, sb
+/* derivative(unk,dpdxa,idx), F:3943-4022, Cartesian branches (closed and periodic); f32: the edge metrics sn, sb
  * rounded to FP32 first (the FP32 operator of the mixed form; sbb stays FP64) */
 static void derivative_m(oc_ctx *c, const double *unk, double *dpdxa, int idx, int f32) {
   const oc_mesh *m = &c->m;
@@ -370,6 +410,11 @@ static void derivative_m(oc_ctx *c, const double *unk, double *dpdxa, int idx, i
     dpdxa[i1] = dpdxa[i1] + sx * u;
     dpdxa[i2] = dpdxa[i2] - sx * u;
   }
+  if (c->p.iper == 2) {
+    /* periodic branch (F:4015-4019): no boundary-point terms, sums over the sets */
+    cyclic_all(c, dpdxa);
+    return;
+  }
   for (int b = 0; b < m->nboun; ++b) {
     int i1 = m->ibpoin[b] - 1;
     double sx = m->sbb[(size_t)d * m->nboun + b];
@@ -380,13 +425,24 @@ static void derivative_m(oc_ctx *c, const double *unk, double *dpdxa, int idx, i
 
 void oc_derivative(oc_ctx *c, const double *unk, double *dpdxa, int idx) { derivative_m(c, unk, dpdxa, idx, 0); }
 
-/* bcv(uout,vout,wout,flag), F:1583-1643, Cartesian branch */
+/* bcv(uout,vout,wout,flag), F:1583-1643, Cartesian branches: the periodic box
+ * (F:1603-1612) keeps only the flag-11 slip projection */
 void oc_bcv(oc_ctx *c, double *u, double *v, double *w, double flag) {
   const oc_mesh *m = &c->m;
   int nb = m->nboun;
+  const int per = c->p.iper == 2;
   for (int b = 0; b < nb; ++b) {
     int ii = m->ibpoin[b] - 1, f = m->ibpoin[nb + b];
     double ax = m->anp[b], ay = m->anp[nb + b], az = m->anp[2 * (size_t)nb + b];
+    if (per) {
+      if (f == 11) {
+        double vn = u[ii] * ax + v[ii] * ay + w[ii] * az;
+        u[ii] = u[ii] - ax * vn;
+        v[ii] = v[ii] - ay * vn;
+        w[ii] = w[ii] - az * vn;
+      }
+      continue;
+    }
     if (f == 11 || f == 5 || f == 6) {
       double vn = u[ii] * ax + v[ii] * ay + w[ii] * az;
       u[ii] = u[ii] - ax * vn;
@@ -616,18 +672,61 @@ static void build_band(oc_ctx *c) {
     free(deg); free(adj);
     deg = uptr; adj = uadj;
   }
-  /* greedy colouring: a column takes the smallest colour absent within two column hops */
+  /* periodic box: twin columns (those holding the nodes of one coincident set) */
+  int32_t *tptr = NULL, *tadj = NULL;
+  if (c->p.iper == 2) {
+    tptr = (int32_t *)calloc((size_t)nc + 1, sizeof(int32_t));
+    const int32_t *lists[3] = {m->icoin, m->icoin2, m->icoin4};
+    const int cnt[3] = {m->ncoin, m->ncoin2, m->ncoin4}, w[3] = {2, 2, 4};
+    for (int pass = 0; pass < 2; ++pass) {
+      int32_t *tf = pass ? (int32_t *)calloc((size_t)nc, sizeof(int32_t)) : NULL;
+      for (int t = 0; t < 3; ++t)
+        for (int ic = 0; ic < cnt[t]; ++ic)
+          for (int a = 0; a < w[t]; ++a)
+            for (int b = 0; b < w[t]; ++b) {
+              int ca = (lists[t][a * cnt[t] + ic] - 1) / L, cb = (lists[t][b * cnt[t] + ic] - 1) / L;
+              if (ca == cb) continue;
+              if (pass == 0) tptr[ca + 1]++;
+              else tadj[tptr[ca] + tf[ca]++] = cb;
+            }
+      if (pass == 0) {
+        for (int k = 0; k < nc; ++k) tptr[k + 1] += tptr[k];
+        tadj = (int32_t *)malloc(sizeof(int32_t) * ((size_t)tptr[nc] + 1));
+      }
+      free(tf);
+    }
+  }
+  /* greedy colouring: a column takes the smallest colour absent among the columns its
+   * probe reaches, within two column hops; in the periodic box the closures join twin
+   * columns as well, so the reach is twin, hop, twin, hop, twin from the column */
   int *color = (int *)malloc(sizeof(int) * (size_t)nc);
   int *mark = (int *)calloc(4096, sizeof(int));
+  int32_t *reach = tptr ? (int32_t *)malloc(sizeof(int32_t) * (size_t)nc) : NULL;
+  int *seen = tptr ? (int *)calloc((size_t)nc, sizeof(int)) : NULL;
   int ncolors = 0;
   for (int k = 0; k < nc; ++k) color[k] = -1;
   for (int k = 0; k < nc; ++k) {
-    for (int q = deg[k]; q < deg[k + 1]; ++q) {
-      int a = adj[q];
-      if (color[a] >= 0 && color[a] < 4096) mark[color[a]] = k + 1;
-      for (int q2 = deg[a]; q2 < deg[a + 1]; ++q2) {
-        int b = adj[q2];
-        if (color[b] >= 0 && color[b] < 4096) mark[color[b]] = k + 1;
+    if (tptr) {
+      int cnt = 1;
+      reach[0] = k;
+      seen[k] = k + 1;
+      for (int step = 0; step < 5; ++step) {
+        const int32_t *ptr = step % 2 == 0 ? tptr : deg, *ad = step % 2 == 0 ? tadj : adj;
+        int c0 = cnt;
+        for (int q = 0; q < c0; ++q)
+          for (int e = ptr[reach[q]]; e < ptr[reach[q] + 1]; ++e)
+            if (seen[ad[e]] != k + 1) { seen[ad[e]] = k + 1; reach[cnt++] = ad[e]; }
+      }
+      for (int q = 1; q < cnt; ++q)
+        if (color[reach[q]] >= 0 && color[reach[q]] < 4096) mark[color[reach[q]]] = k + 1;
+    } else {
+      for (int q = deg[k]; q < deg[k + 1]; ++q) {
+        int a = adj[q];
+        if (color[a] >= 0 && color[a] < 4096) mark[color[a]] = k + 1;
+        for (int q2 = deg[a]; q2 < deg[a + 1]; ++q2) {
+          int b = adj[q2];
+          if (color[b] >= 0 && color[b] < 4096) mark[color[b]] = k + 1;
+        }
       }
     }
     int col = 0;
@@ -676,6 +775,7 @@ static void build_band(oc_ctx *c) {
   c->band_L = L;
   c->band_colors = ncolors;
   free(deg); free(adj); free(fill); free(color); free(mark);
+  free(tptr); free(tadj); free(reach); free(seen);
 }
 
 static void scan_column(const double *la, const double *linv, const double *lc, const double *rhs, int L,
@@ -983,6 +1083,7 @@ int oc_set_multigrid(oc_ctx *c, const oc_mesh *m1, const int32_t *match1, const 
                      const int32_t *match2, int itrp2, int itrp_c1, int itrp_c2, int itrp_cc,
                      int transfer) {
   if (!m1) { c->mg_on = 0; return 0; }
+  if (c->p.iper == 2) return -1;   /* the coarse levels' periodic closures are not restated */
   const oc_mesh *ms[2] = {m1, m2};
   const int32_t *mt[2] = {match1, match2};
   int nf[2] = {c->n, m1->npoin};
@@ -1192,7 +1293,13 @@ void oc_gcrk(oc_ctx *c, double *p, const double *u1, const double *v1, const dou
 
 /* mpdatam(uuu,unk,iflag), F:3029-3311: donor cell + one antidiffusive pass (iord=2)
  * with the FCT limiter; bounds from the donor-cell solution only. */
+static void mpdatamper(oc_ctx *c, const double *uuu, double *unk);
+
 void oc_mpdatam(oc_ctx *c, const double *uuu, double *unk, int iflag) {
+  if (c->p.iper == 2) {   /* mpsolver F:314-319 */
+    mpdatamper(c, uuu, unk);
+    return;
+  }
   const oc_mesh *m = &c->m;
   int n = c->n, E = m->nedge, B = m->nbface, Bp = m->nboun;
   const double *U = uuu, *V = uuu + n, *W = uuu + 2 * (size_t)n;
@@ -1304,6 +1411,145 @@ void oc_mpdatam(oc_ctx *c, const double *uuu, double *unk, int iflag) {
     }
   }
   free(aun); free(aunb); free(aunbb1);
+}
+
+/* mpdatamper(uuu,unk,iflag), F:3617-3941: the two passes of mpdatam with the periodic
+ * closures. No boundary-point fluxes (the boundary points add 0, F:3733-3739); the
+ * increments are summed over the coincident sets before the update (F:3741-3743); the
+ * FCT bounds of each set are merged (F:3831-3879: a pair takes the extremes of both
+ * nodes; a quadruple takes the maxima of all four, and its minima are formed, as the
+ * reference writes them, from the four nodes' maxima and values); the in/out flux sums
+ * are summed over the sets (F:3897-3902). iflag selects nothing here. */
+static double max8(const double *v) { double r = v[0]; for (int k = 1; k < 8; ++k) r = amax1(r, v[k]); return r; }
+static double min8(const double *v) { double r = v[0]; for (int k = 1; k < 8; ++k) r = amin1(r, v[k]); return r; }
+
+static void mpdatamper(oc_ctx *c, const double *uuu, double *unk) {
+  const oc_mesh *m = &c->m;
+  int n = c->n, E = m->nedge, B = m->nbface, Bp = m->nboun;
+  const double *U = uuu, *V = uuu + n, *W = uuu + 2 * (size_t)n;
+  double dt = c->p.dt;
+  double *aun = zalloc(E), *aunb = zalloc(B);
+  double *divv = c->s[0], *arh2 = c->s[1], *gx = c->s[2], *gy = c->s[3], *gz = c->s[4];
+  double *mx = c->s[5], *mn = c->s[6], *rin = c->s[7], *rout = c->s[8], *cp = c->s[9], *cn = c->s[10];
+  double tol = 1.e-10;
+  for (int i = 0; i < n; ++i) { divv[i] = 0.; mx[i] = -1.e15; mn[i] = 1.e15; }
+  for (int ipass = 1; ipass <= 2; ++ipass) {
+    for (int i = 0; i < n; ++i) arh2[i] = 0.;
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? E : B;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      const double *S = pass == 0 ? m->sn : m->sb;
+      double *a = pass == 0 ? aun : aunb;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+        double sx = S[e], sy = S[ne + e], sz = S[2 * (size_t)ne + e];
+        double fl;
+        if (ipass == 1) {
+          a[e] = (U[i1] + U[i2]) * 0.5 * sx + (V[i1] + V[i2]) * 0.5 * sy + (W[i1] + W[i2]) * 0.5 * sz;
+          fl = unk[i1] * amax1(0., a[e]) + unk[i2] * amin1(0., a[e]);
+        } else {
+          fl = a[e];
+        }
+        arh2[i1] = arh2[i1] + fl * (1. + 1.) * 0.5;
+        arh2[i2] = arh2[i2] - fl * (1. + 1.) * 0.5;
+        divv[i1] = divv[i1] + a[e];
+        divv[i2] = divv[i2] - a[e];
+      }
+    }
+    for (int b = 0; b < Bp; ++b) {
+      int i1 = m->ibpoin[b] - 1;
+      arh2[i1] = arh2[i1] + 0. * 1.;
+    }
+    cyclic_all(c, arh2);
+    for (int i = 0; i < n; ++i) unk[i] = unk[i] - arh2[i] * dt / m->vol[i] / c->rh0[i];
+    if (ipass == 2) break;
+
+    oc_derivative(c, unk, gx, 1);
+    oc_derivative(c, unk, gy, 2);
+    oc_derivative(c, unk, gz, 3);
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? E : B;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      double *a = pass == 0 ? aun : aunb;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+        double un = fabs(a[e]) * (unk[i2] - unk[i1]) * 0.5;
+        double volt = 0.5 * (m->vol[i1] * c->rh0[i1] + m->vol[i2] * c->rh0[i2]);
+        double dpdx = 0.5 * (gx[i1] + gx[i2]) / volt;
+        double dpdy = 0.5 * (gy[i1] + gy[i2]) / volt;
+        double dpdz = 0.5 * (gz[i1] + gz[i2]) / volt;
+        double term = (U[i1] + U[i2]) * 0.5 * dpdx + (V[i1] + V[i2]) * 0.5 * dpdy + (W[i1] + W[i2]) * 0.5 * dpdz;
+        a[e] = un - a[e] * term * dt * 0.5;
+      }
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? E : B;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+        double p1 = unk[i1], p2 = unk[i2];
+        mx[i1] = amax1(amax1(mx[i1], p1), p2);
+        mn[i1] = amin1(amin1(mn[i1], p1), p2);
+        mx[i2] = amax1(amax1(mx[i2], p2), p1);
+        mn[i2] = amin1(amin1(mn[i2], p2), p1);
+      }
+    }
+    for (int list = 0; list < 2; ++list) {
+      int nl = list == 0 ? m->ncoin : m->ncoin2;
+      const int32_t *pr = list == 0 ? m->icoin : m->icoin2;
+      for (int ic = 0; ic < nl; ++ic) {
+        int i1 = pr[ic] - 1, i2 = pr[nl + ic] - 1;
+        double p1 = unk[i1], p2 = unk[i2];
+        mx[i1] = amax1(amax1(amax1(mx[i1], p1), p2), mx[i2]);
+        mn[i1] = amin1(amin1(amin1(mn[i1], p1), p2), mn[i2]);
+        mx[i2] = amax1(amax1(amax1(mx[i2], p1), p2), mx[i1]);
+        mn[i2] = amin1(amin1(amin1(mn[i2], p1), p2), mn[i1]);
+      }
+    }
+    for (int ic = 0; ic < m->ncoin4; ++ic) {
+      int q = m->ncoin4, ii[4];
+      for (int k = 0; k < 4; ++k) ii[k] = m->icoin4[k * q + ic] - 1;
+      double v[8];
+      for (int k = 0; k < 4; ++k) v[4 + k] = unk[ii[k]];
+      for (int k = 0; k < 4; ++k) {
+        for (int j = 0; j < 4; ++j) v[j] = mx[ii[j]];
+        mx[ii[k]] = max8(v);
+        v[k] = mx[ii[k]];
+        mn[ii[k]] = min8(v);
+      }
+    }
+    for (int i = 0; i < n; ++i) { rin[i] = 0.; rout[i] = 0.; cp[i] = 0.; cn[i] = 0.; }
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? E : B;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      double *a = pass == 0 ? aun : aunb;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+        double pp = amax1(0., a[e]), pn = -amin1(0., a[e]);
+        rin[i1] = rin[i1] + pn;
+        rout[i1] = rout[i1] + pp;
+        rin[i2] = rin[i2] + pp;
+        rout[i2] = rout[i2] + pn;
+      }
+    }
+    cyclic_all(c, rin);
+    cyclic_all(c, rout);
+    for (int i = 0; i < n; ++i) {
+      cp[i] = (mx[i] - unk[i]) * m->vol[i] * c->rh0[i] / (rin[i] * dt + tol);
+      cn[i] = (unk[i] - mn[i]) * m->vol[i] * c->rh0[i] / (rout[i] * dt + tol);
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+      int ne = pass == 0 ? E : B;
+      const int32_t *ie = pass == 0 ? m->iedge : m->kbface;
+      double *a = pass == 0 ? aun : aunb;
+      for (int e = 0; e < ne; ++e) {
+        int i1 = ie[e] - 1, i2 = ie[ne + e] - 1;
+        if (a[e] > 0.) a[e] = a[e] * amin1(amin1(1., cp[i2]), cn[i1]);
+        else a[e] = a[e] * amin1(amin1(1., cn[i2]), cp[i1]);
+      }
+    }
+  }
+  free(aun); free(aunb);
 }
 
 /* mpdatam of the mixed forms: the loops above in FP32 with the HIP path's operands — the
@@ -1477,7 +1723,8 @@ void oc_diagnostics(oc_ctx *c, int flag, double *out) {
       if (flag == 3) { cour[i1] = cour[i1] + fmax(0., aun); cour[i2] = cour[i2] - fmin(0., aun); }
     }
   }
-  for (int b = 0; b < m->nboun; ++b) {
+  /* the periodic box skips the boundary points (F:2049) */
+  for (int b = 0; b < (c->p.iper == 2 ? 0 : m->nboun); ++b) {
     int i1 = m->ibpoin[b] - 1;
     double sx = m->sbb[b], sy = m->sbb[m->nboun + b], sz = m->sbb[2 * (size_t)m->nboun + b];
     double aun = u[i1] * sx + v[i1] * sy + w[i1] * sz;

@@ -1,16 +1,24 @@
 /* CPU oracle for the edge-based NFT step: TEST INFRASTRUCTURE ONLY.
  *
  * A plain-C FP64 restatement of the reference Fortran
- * (lippsLFroude_implgwopks3D_unstr3.f) for the Cartesian, non-periodic path
- * (isfera=0, iper=0, lipps=0, itraj=0 or 1, implgw=1, icorio=0). Each function keeps the
+ * (lippsLFroude_implgwopks3D_unstr3.f) for the Cartesian path, closed or doubly periodic
+ * (isfera=0, iper=0 or 2, lipps=0, itraj=0 or 1, implgw=1, icorio=0). Each function keeps the
  * reference's loop order and operation order so that, compiled without FP contraction,
  * it reproduces the reference's rounding. Only tests, the smoke check and the CPU leg
  * of the benchmark use it, as what the HIP path's results are compared with.
+ *
+ * The doubly periodic variant (iper = 2) restates the reference's periodic branches:
+ * cyclicr/cyclicr2/cyclicr4, mpdatamper and the periodic forms of derivative, bcv, courant
+ * and mpsolver's set-up.
  *
  * Parity pinning (who checks each function):
  *   derivative, laplc, prfr, div, bcv, precon(point), gcrk, mpdatam, compute_d,
  *   absorbers, step driver  -> golden vectors written by the flang-compiled reference
  *                              (tests/golden) plus closed forms in tests/test_oracle.py
+ *   periodic variant (closures, mpdatamper, the periodic derivative/bcv/courant, the
+ *   closed vol in d1 and beta)
+ *                           -> golden vectors of the reference built from the same
+ *                              source and run with iper=2 on a generated periodic mesh
  *   multigrid branch of precon (restriction, prolongation, coarse laplc, d2/d3, the
  *   V-cycle)                -> golden vectors of the reference run with mg_flag=1 on
  *                              generated nested meshes (tests/golden)
@@ -42,6 +50,13 @@ typedef struct {
   const double *vol;      /* [npoin] */
   int ncol;               /* vertical columns (0: none) */
   const int32_t *col_start; /* [ncol+1], 0-based node ranges */
+  /* doubly periodic box (iper = 2): the coincident node sets read after the mesh
+   * (F:2569-2583), 1-based: pairs of the x sides, pairs of the y sides, quadruples of the
+   * corner columns */
+  int ncoin, ncoin2, ncoin4;
+  const int32_t *icoin;   /* [2][ncoin] */
+  const int32_t *icoin2;  /* [2][ncoin2] */
+  const int32_t *icoin4;  /* [4][ncoin4] */
 } oc_mesh;
 
 typedef struct {
@@ -73,6 +88,12 @@ typedef struct {
    * operator (oc_laplc32) inside gcrk and the solve in passes from the FP64 residual */
   int precision;
   double mixed_drop;   /* precision 2: recursive-residual drop that ends a pass (0: 1e-3) */
+  /* 0 the closed box; 2 the doubly periodic box (the reference's iper = 2, FP64 only): vol
+   * summed over the coincident sets at set-up (F:99-103), derivative without the
+   * boundary-point terms and summed over the sets (F:4007-4019), bcv on flag 11 only
+   * (F:1603-1612), courant without its boundary-point loop (F:2049), mpdatamper in place
+   * of mpdatam (F:314-319, F:3617-3941) */
+  int iper;
 } oc_params;
 
 typedef struct { int niter, nitsm, icount; double eer, eem; } oc_stats;

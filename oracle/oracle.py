@@ -22,7 +22,9 @@ class OcMesh(ctypes.Structure):
     _fields_ = [("npoin", ctypes.c_int), ("nedge", ctypes.c_int), ("nbface", ctypes.c_int),
                 ("nboun", ctypes.c_int), ("iedge", i32p), ("sn", f64p), ("kbface", i32p),
                 ("sb", f64p), ("ibpoin", i32p), ("sbb", f64p), ("anp", f64p), ("coord", f64p),
-                ("vol", f64p), ("ncol", ctypes.c_int), ("col_start", i32p)]
+                ("vol", f64p), ("ncol", ctypes.c_int), ("col_start", i32p),
+                ("ncoin", ctypes.c_int), ("ncoin2", ctypes.c_int), ("ncoin4", ctypes.c_int),
+                ("icoin", i32p), ("icoin2", i32p), ("icoin4", i32p)]
 
 
 class OcParams(ctypes.Structure):
@@ -36,7 +38,8 @@ class OcParams(ctypes.Structure):
                 ("towxl", ctypes.c_double), ("towxr", ctypes.c_double), ("towyd", ctypes.c_double),
                 ("towyu", ctypes.c_double), ("towzd", ctypes.c_double), ("towzu", ctypes.c_double),
                 ("irl", ctypes.c_int), ("dom", ctypes.c_double * 6), ("dot_order", ctypes.c_int),
-                ("itraj", ctypes.c_int), ("precision", ctypes.c_int), ("mixed_drop", ctypes.c_double)]
+                ("itraj", ctypes.c_int), ("precision", ctypes.c_int), ("mixed_drop", ctypes.c_double),
+                ("iper", ctypes.c_int)]
 
 
 class OcStats(ctypes.Structure):
@@ -120,6 +123,11 @@ class Oracle:
         m = OcMesh(mesh.npoin, mesh.nedge, mesh.nbface, mesh.nboun, _p(k[0]), _p(k[1]), _p(k[2]),
                    _p(k[3]), _p(k[4]), _p(k[5]), _p(k[6]), _p(k[7]), _p(k[8]),
                    mesh.nh if use_columns else 0, _p(k[9]))
+        if mesh.coin is not None:
+            # the doubly periodic box: coincident sets as the reference reads them
+            self._keep += [np.ascontiguousarray(a, dtype=np.int32) for a in (mesh.coin, mesh.coin2, mesh.coin4)]
+            m.ncoin, m.ncoin2, m.ncoin4 = mesh.coin.shape[1], mesh.coin2.shape[1], mesh.coin4.shape[1]
+            m.icoin, m.icoin2, m.icoin4 = _p(self._keep[-3]), _p(self._keep[-2]), _p(self._keep[-1])
         p = OcParams()
         p.dt = cfg.dt; p.u0 = cfg.u0; p.v0 = 0.0; p.w0 = 0.0
         p.strat = cfg.strat; p.theta0 = cfg.theta0
@@ -140,8 +148,11 @@ class Oracle:
         p.itraj = itraj
         p.precision = precision
         p.mixed_drop = mixed_drop
+        p.iper = 2 if mesh.coin is not None else 0
         self._m, self._params = m, p
         self.h = L.oc_create(ctypes.byref(m), ctypes.byref(p))
+        if not self.h:
+            raise ValueError("oracle: unsupported configuration (the periodic box is FP64 only)")
 
     def close(self):
         if self.h:

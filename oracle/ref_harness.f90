@@ -7,6 +7,9 @@
 !                 of mpsolver (lippsLFroude_implgwopks3D_unstr3.f:108-170, 250-252), then
 !                 call the reference's own derivative/laplc/bcv/prfr/div/precon/mpdatam/gcrk
 !                 on the vectors of vec_in.bin and write every result to vec_out.bin.
+!                 With iper=2 (the periodic box) the set-up closes vol over the coincident
+!                 nodes first, as mpsolver does (:99-103), and mpdatamper takes the place of
+!                 mpdatam, as in mpsolver's step (:314-319).
 !   nft_ref mgops : the same set-up including the coarse meshes (beta_timestep_unstr and
 !                 compute_d_unstr for levels 2 and 3), then the multigrid pieces:
 !                 restriction_unstr, prolongation_unstr, laplc_c, laplc_cc, and precon and
@@ -32,6 +35,8 @@ program nft_ref_harness
   common/elast/ cfels,iflels
   common/n_c/npoin_c,nedge_c,nbface_c,nbpoin_c,nboun_c
   common/n_cc/npoin_cc,nedge_cc,nbface_cc,nbpoin_cc,nboun_cc
+  common/per/iper
+  common/volume/vol(npoinx)
   real, save :: alpha(npoinx), v(npoinx,8), o1(npoinx), o2(npoinx), o3(npoinx), o4(npoinx)
   real, save :: uuu(npoinx,3), wk(npoinx)
   character(len=16) :: mode
@@ -54,6 +59,11 @@ program nft_ref_harness
   open(21,file='harness.d',access='sequential',status='old')
   call input
   read(21,*) strat_in
+  if (iper == 2) then
+    call cyclicr(vol)
+    call cyclicr2(vol)
+    call cyclicr4(vol)
+  end if
 
   ! mpsolver set-up (Cartesian, lipps=0, implgw=1)
   iflels=0
@@ -166,7 +176,11 @@ program nft_ref_harness
   uuu(1:npoin,3)=v(1:npoin,8)
   do k=1,4,3
     o1(1:npoin)=v(1:npoin,1)
-    call mpdatam(uuu,o1,k)
+    if (iper == 2) then
+      call mpdatamper(uuu,o1,k)
+    else
+      call mpdatam(uuu,o1,k)
+    end if
     write(23) (o1(ip),ip=1,npoin)
   end do
   o4(1:npoin)=0.

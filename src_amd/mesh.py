@@ -74,6 +74,10 @@ class MeshSpec:
     # grid cells, so columns that are neighbours in the plane get nearby ids and the
     # metrics of their shared edges are read close together in time)
     column_order: str = "grid"
+    # doubly periodic box (the reference's iper=2): the boundary points of opposite sides
+    # mirror each other and the nodes on them are listed as coincident pairs (x and y
+    # sides) and quadruples (the four corner columns)
+    periodic: bool = False
 
 
 @dataclasses.dataclass
@@ -92,6 +96,11 @@ class Mesh:
     anp: np.ndarray
     tri: np.ndarray
     _handle: Optional[int] = None
+    # periodic meshes: 1-based coincident node sets, as the reference reads them after the
+    # mesh (icoin pairs [2, ncoin], icoin2 pairs [2, ncoin2], icoin4 quadruples [4, ncoin4])
+    coin: Optional[np.ndarray] = None
+    coin2: Optional[np.ndarray] = None
+    coin4: Optional[np.ndarray] = None
 
     @property
     def npoin(self):
@@ -120,6 +129,13 @@ class Mesh:
         rc = lib.nft_meshgen_write_text(ctypes.c_void_p(self._handle), path.encode())
         if rc != 0:
             raise OSError(f"writing {path} failed ({rc})")
+        if self.coin is not None:
+            # the coincident sets follow the mesh (the reference's input :2569-2583)
+            with open(path, "a") as f:
+                for arr in (self.coin, self.coin2, self.coin4):
+                    f.write(f"{arr.shape[1]}\n")
+                    for q in range(arr.shape[1]):
+                        f.write(" ".join(str(int(v)) for v in arr[:, q]) + "\n")
 
     def release(self):
         if self._handle is not None:
@@ -164,7 +180,32 @@ def jittered_points(spec: MeshSpec) -> np.ndarray:
     xy = np.zeros((2, nh), dtype=np.float64)
     _meshlib().nft_meshgen_points(spec.nx, spec.ny, spec.lx, spec.ly, spec.jitter,
                                   ctypes.c_uint64(spec.seed), _dp(xy))
+    if spec.periodic:
+        # opposite sides mirror each other: the east side takes the west side's y, the
+        # north side the south side's x (boundary points move only along their side)
+        g = xy.reshape(2, spec.ny, spec.nx)
+        g[1, :, spec.nx - 1] = g[1, :, 0]
+        g[0, spec.ny - 1, :] = g[0, 0, :]
     return xy
+
+
+def coincident_sets(spec: MeshSpec):
+    """1-based coincident node sets of a periodic grid-ordered mesh (node = column * nlev +
+    level + 1, column = iy * nx + ix): west/east pairs and south/north pairs without the
+    corner columns, and the four corner columns as quadruples, level by level."""
+    nx, ny, L = spec.nx, spec.ny, spec.nlev
+    k = np.arange(L)
+
+    def nodes(ix, iy):
+        return (np.asarray(iy)[..., None] * nx + np.asarray(ix)[..., None]) * L + k + 1
+
+    iy = np.arange(1, ny - 1)
+    ix = np.arange(1, nx - 1)
+    coin = np.stack([nodes(np.zeros_like(iy), iy).ravel(), nodes(np.full_like(iy, nx - 1), iy).ravel()])
+    coin2 = np.stack([nodes(ix, np.zeros_like(ix)).ravel(), nodes(ix, np.full_like(ix, ny - 1)).ravel()])
+    coin4 = np.stack([nodes(0, 0).ravel(), nodes(nx - 1, 0).ravel(), nodes(0, ny - 1).ravel(),
+                      nodes(nx - 1, ny - 1).ravel()])
+    return tuple(np.ascontiguousarray(a.astype(np.int32)) for a in (coin, coin2, coin4))
 
 
 def hilbert_index(ix: np.ndarray, iy: np.ndarray, order: int) -> np.ndarray:
@@ -231,7 +272,12 @@ def generate(spec: MeshSpec) -> Mesh:
         xy = np.ascontiguousarray(xy[:, perm])
         rank = np.ascontiguousarray(perm.astype(np.int32))
     zeta = zeta_levels(spec.nlev, spec.htop, spec.dz_bottom, spec.dz_top)
-    return _build(spec, xy, zeta, rank)
+    m = _build(spec, xy, zeta, rank)
+    if spec.periodic:
+        if perm is not None:
+            raise ValueError("periodic meshes are built for the grid column order")
+        m.coin, m.coin2, m.coin4 = coincident_sets(spec)
+    return m
 
 
 def _coarse_index(n: int) -> np.ndarray:
@@ -357,6 +403,12 @@ def small_hill_spec(h0):
     hill of height h0: the learned preconditioner's workload and its training meshes."""
     import dataclasses as _dc
     return _dc.replace(small_spec(), hill_h0=float(h0))
+
+
+def periodic_spec(nx=8, ny=7, nlev=5, seed=2):
+    """A small doubly periodic box (the reference's iper=2) with a centred hill."""
+    return MeshSpec(nx=nx, ny=ny, nlev=nlev, lx=3500.0, ly=3000.0, htop=2000.0, seed=seed,
+                    hill_h0=200.0, hill_a=700.0, periodic=True)
 
 
 def production_spec(column_order="grid"):

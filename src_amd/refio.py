@@ -50,7 +50,8 @@ def write_reference_inputs(d: str, mesh: Mesh, cfg: RunConfig, ntime: int,
     sp = cfg.mesh
     with open(os.path.join(d, "inputinc.d"), "w") as f:
         f.write("ntime nrkstage istst mrestart isfera iper\n")
-        f.write(f"{ntime} 2 2 0 0 0\n")
+        # iper = 2: the doubly periodic box, whose coincident node sets follow the mesh
+        f.write(f"{ntime} 2 2 0 0 {2 if mesh.coin is not None else 0}\n")
         f.write("cfl delt adapt relax\n")
         f.write(f"0.5 {fmt(cfg.dt)} 0. 0.\n")
         f.write("p0inf u0inf v0inf w0inf Re\n")

@@ -21,6 +21,34 @@ def small_cfg(**kw):
     return M.RunConfig("small", c.mesh, **base)
 
 
+def periodic_cfg(converged=False, **kw):
+    """The doubly periodic box (8 x 7 columns x 5 levels, iper=2) with the tiny mesh's flow;
+    converged: no preconditioner and the relative exit test to 1e-8 (the reference build that
+    converges)."""
+    base = dict(u0=10.0, bvf=0.01, dt=10.0, steps=10, lord=3, eps0=1e-7, exit_mode=0, itr=30, precon=1,
+                itrp1=2, omega=0.5, sponge_depth=500.0, sponge_tau=300.0)
+    if converged:
+        base.update(eps0=1e-8, exit_mode=1, itr=400, precon=0)
+    base.update(kw)
+    return M.RunConfig("periodic", M.periodic_spec(), **base)
+
+
+def periodic_field(m, f):
+    """f with every coincident set holding its first node's value (a field of the periodic box)."""
+    g = np.array(f, dtype=np.float64, copy=True)
+    for s in (m.coin, m.coin2, m.coin4):
+        g[s - 1] = g[s[0] - 1]
+    return g
+
+
+def unique_nodes(m):
+    """Mask of one node per coincident set (and every node outside the sets)."""
+    keep = np.ones(m.npoin, dtype=bool)
+    for s in (m.coin, m.coin2, m.coin4):
+        keep[s[1:].ravel() - 1] = False
+    return keep
+
+
 @functools.lru_cache(maxsize=None)
 def mesh_for(spec_key):
     spec = {"tiny": M.tiny_spec(), "small": M.small_spec(), "tiny2": M.tiny_spec(nx=9, ny=7, nlev=6, seed=3),
@@ -29,7 +57,9 @@ def mesh_for(spec_key):
             "tall3": M.tiny_spec(nx=5, ny=4, nlev=130, seed=6),
             # columns of exactly one and two wavefronts
             "wave64": M.tiny_spec(nx=9, ny=7, nlev=64, seed=8),
-            "wave128": M.tiny_spec(nx=5, ny=4, nlev=128, seed=9)}[spec_key]
+            "wave128": M.tiny_spec(nx=5, ny=4, nlev=128, seed=9),
+            # the doubly periodic box (iper=2) and a larger one (several wavefronts of sets)
+            "periodic": M.periodic_spec(), "periodic2": M.periodic_spec(nx=40, ny=33, nlev=12, seed=5)}[spec_key]
     return M.generate(spec)
 
 

@@ -14,6 +14,10 @@ generated meshes, runs
   * 1 and 10 steps with the velprd advector (itraj=1) on the tiny mesh (tiny_velprd.npz),
   * with --converged only: 1, 10 and 100 steps on the 40,000-node mesh of a reference build
     whose GCR converges (no preconditioner, relative exit test; small_conv.npz),
+  * with --periodic only: the doubly periodic box (iper=2) on a generated 280-node periodic
+    mesh: the `ops` outputs (mpdatamper in place of mpdatam) and 1 and 10 steps of the
+    program, both with the reference build of tiny_ops.npz and with the converging build of
+    small_conv.npz (tiny_per.npz),
 and stores inputs and outputs. Only numbers are stored.
 """
 from __future__ import annotations
@@ -98,6 +102,39 @@ def write_converged():
             print(f"converged run: {nt} steps done, niter {st['niter'].tolist()[:12]}...")
         np.savez_compressed(os.path.join(OUT, "small_conv.npz"), mesh_checksum=mesh_checksum(m), **runs)
     print("converged fixtures written")
+
+
+def periodic_config(converged=False):
+    """The periodic box (src_amd.mesh.periodic_spec, 8 x 7 columns x 5 levels) with the
+    tiny mesh's flow; converged: no preconditioner and the relative exit test to 1e-8."""
+    if converged:
+        return M.RunConfig("periodic", M.periodic_spec(), u0=10.0, bvf=0.01, dt=10.0, steps=10, lord=3,
+                           eps0=1e-8, exit_mode=1, itr=400, precon=0, itrp1=2, omega=0.5,
+                           sponge_depth=500.0, sponge_tau=300.0)
+    return M.RunConfig("periodic", M.periodic_spec(), u0=10.0, bvf=0.01, dt=10.0, steps=10, lord=3,
+                       eps0=1e-7, exit_mode=0, itr=30, precon=1, itrp1=2, omega=0.5,
+                       sponge_depth=500.0, sponge_tau=300.0)
+
+
+def write_periodic():
+    """tiny_per.npz: the reference with iper=2 on the periodic box."""
+    cfg, ccfg = periodic_config(), periodic_config(converged=True)
+    m = M.generate(cfg.mesh)
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        exe = build_ref(cfg)
+        out.update(run_ops(exe, m, cfg, d))
+        exe_c = build_ref(ccfg, iprc=0, relexit=1)
+        for tag, e, c in (("", exe, cfg), ("conv_", exe_c, ccfg)):
+            for nt in (1, 10):
+                st = run_steps(e, m, c, d, nt)
+                for k in ("unk", "uuu", "frc", "unkdash", "prpot", "niter"):
+                    out[f"{tag}s{nt}_{k}"] = st[k]
+                print(f"periodic {tag or 'parity'} run: {nt} steps, niter {st['niter'].tolist()}")
+    np.savez_compressed(os.path.join(OUT, "tiny_per.npz"), mesh_checksum=mesh_checksum(m),
+                        coin_checksum=np.array([a.sum() for a in (m.coin, m.coin2, m.coin4)], dtype=np.int64),
+                        strat=np.array([cfg.strat]), **out)
+    print("periodic fixtures written")
 
 
 def mesh_checksum(m):
@@ -239,6 +276,9 @@ def run_steps(exe, m, cfg, d, ntime, mg=None):
 def main():
     if "--converged" in sys.argv:
         write_converged()
+        return
+    if "--periodic" in sys.argv:
+        write_periodic()
         return
     only_tiny = "--tiny-only" in sys.argv
     tcfg = tiny_config()
