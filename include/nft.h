@@ -45,6 +45,20 @@ typedef struct {
   const int32_t *col_start; /* [ncol+1] */
 } nft_mesh;
 
+/* Doubly periodic box (the reference's iper = 2): the coincident node sets that `input`
+ * reads after the mesh (:2569-2583; COMMON /coin/ icoin(nfacesx*2,2), /per2/
+ * icoin2(nfacesx,2), /per4/ icoin4(nfacesx,4)), 1-based, with their leading dimensions.
+ * Every node may belong to one set at most. */
+typedef struct {
+  int ncoin, ncoin2, ncoin4;
+  int ld_coin, ld_coin2, ld_coin4;
+  const int32_t *icoin;   /* icoin(ld_coin,2): pairs of the x sides */
+  const int32_t *icoin2;  /* icoin2(ld_coin2,2): pairs of the y sides */
+  const int32_t *icoin4;  /* icoin4(ld_coin4,4): quadruples of the corner columns */
+  int vol_closed;         /* nonzero: vol already holds the sums over the sets (the reference's
+                             vol after mpsolver :99-103); 0: nft_init_periodic forms them */
+} nft_periodic;
+
 /* Run parameters: the reference's inputinc.d / input_mg_v3.d knobs and the constants
  * hard-coded in its mpsolver (Cartesian, lipps=0, implgw=1, icorio=0; itraj below). */
 typedef struct {
@@ -104,6 +118,14 @@ typedef struct {
  * beta_timestep_unstr(1), compute_d_unstr(1), coefw; :140-170, :250-252): uploads the
  * mesh, builds the node-sorted incidence and the column factors. */
 int nft_init(const nft_mesh *mesh, const nft_params *params, nft_ctx **out);
+/* nft_init for the doubly periodic box (iper = 2): mpsolver's set-up with vol summed over
+ * the coincident sets first (:99-103, unless per->vol_closed says it is); afterwards every entry point runs the reference's
+ * periodic branches: derivative without the boundary-point terms and summed over the sets
+ * (cyclicr/cyclicr2/cyclicr4, :4015-4019, :4377-4446), bcv with the flag-11 slip only
+ * (:1603-1612), courant without its boundary points (:2049), and nft_mpdatam and the step
+ * with mpdatamper (:314-319, :3617-3941). FP64 only (precision 0); the multigrid branch, the
+ * learned preconditioner and nft_laplc_t are not available. */
+int nft_init_periodic(const nft_mesh *mesh, const nft_params *params, const nft_periodic *per, nft_ctx **out);
 int nft_finalize(nft_ctx *ctx);
 const char *nft_last_error(void);
 int nft_device_name(char *buf, int len);
@@ -121,7 +143,8 @@ int nft_laplc_t(nft_ctx *ctx, const double *y, double *out, int impfl);
  * (:1288-1353) on DEVICE arrays press and alap (npoin doubles each, on the context's GPU,
  * ready when the call is made), with laplc's intermediate held in a fixed device array of
  * the library instead of the context's work arrays. The call allocates nothing; it
- * enqueues two kernels on the context's stream and waits for them. Limit: meshes of at most
+ * enqueues two kernels (four in the periodic box, whose set sums live in the context) on
+ * the context's stream and waits for them. Limit: meshes of at most
  * 1,048,576 nodes (the array holds 24 MiB on each GPU the library runs on, until the
  * process ends); larger meshes are rejected with an error. Every call on one GPU shares that
  * array, so calls that overlap on one GPU (from several contexts or threads) race. Results

@@ -168,6 +168,12 @@ struct nft_ctx {
 
   nft_gcr_stats st{};
 
+  // doubly periodic box (iper = 2): the coincident sets on the device (inc.per, inc.cslot
+  // and inc.cval point into them) and, on the host, the sets as 0-based member lists
+  int per = 0;
+  PerSets ps{};
+  std::vector<int4> h_sets;
+
   // multigrid branch of precon (mg_flag=1): coarse levels 1 and 2 on meshes of their own
   struct MgLevel {
     int n = 0, nodes_grid = 0, gather_grid = 0, col_on = 0;
@@ -464,8 +470,17 @@ void t_harvest_prefix(nft_ctx *c, int upto) {
 void t_harvest(nft_ctx *c) { t_harvest_prefix(c, c->tev_used); }
 
 // ---- device operators -------------------------------------------------------------------
+// periodic box: the set kernel of the three gathered sums that the next k_grad / k_div
+// launch takes at the coincident nodes
+void per_sums3(nft_ctx *c, const double *q0, const double *q1, const double *q2, const double2 *qxy,
+               const double *scale) {
+  PerSums3 a{q0, q1, q2, qxy, scale, c->skip};
+  hipLaunchKernelGGL(k_per_sums3, dim3(blocks_for(c->ps.nset)), dim3(NFT_BLOCK), 0, c->stream, c->inc, c->ps, a);
+}
+
 int grad_launch(nft_ctx *c, int mode, const GradArgs &a) {
   dim3 g(c->gather_grid), b(NFT_BLOCK);
+  if (c->per && mode != GRAD_ZERO_PRFR) per_sums3(c, a.f, a.f, a.f, nullptr, nullptr);
   if (mode == GRAD_LAPLC) t_begin(c, KT_LGRAD);
   if (c->col_on) {
     switch (mode) {
@@ -547,6 +562,11 @@ int final_launch(nft_ctx *c, int nblk, const FinalOp &op0) {
 // f32: the FP32 operator's divergence (FP32 metrics and inputs, DIV_LAPLC only)
 int div_launch(nft_ctx *c, int mode, const DivArgs &a, bool f32 = false) {
   const int nd = a.dots.nd, nmax = a.dots.nmax ? 1 : 0;
+  if (c->per) {
+    if (f32) return set_err(-1, "periodic box: the FP32 operator is not available");
+    if (mode == DIV_LAPLC) per_sums3(c, nullptr, nullptr, a.q2, a.qxy, nullptr);
+    else per_sums3(c, a.q0, a.q1, a.q2, nullptr, a.iflg == -1 && !c->nc.rh0_one ? c->rh0 : nullptr);
+  }
   if (mode == DIV_LAPLC) {
     if (nmax) return set_err(-1, "laplc: max reduction not supported");
     t_begin(c, KT_LDIV);
@@ -1312,20 +1332,35 @@ int mpdata_dev(nft_ctx *c, const T *U, const T *V, const T *W, double *const *fi
   if (sizeof(T) == 8) S = *(EdgeS<T> *)&c->S;
   else S = *(EdgeS<T> *)&c->Sf;
   dim3 g(c->gather_grid), b(NFT_BLOCK);
+  // periodic box (mpdatamper): each sweep's set kernel first
+  const dim3 gs(blocks_for(c->ps.nset));
+  const bool per = c->per != 0;
+  if constexpr (sizeof(T) == 8) {
+    if (per) hipLaunchKernelGGL(k_per_mp_donor<NF>, gs, b, 0, c->stream, c->inc, c->ps, a);
+  }
   t_begin(c, KT_MP0);
   if (c->col_on) hipLaunchKernelGGL((k_mp_donor<T, NF, true>), g, b, 0, c->stream, c->inc, S, a);
   else hipLaunchKernelGGL((k_mp_donor<T, NF, false>), g, b, 0, c->stream, c->inc, S, a);
   t_end(c, KT_MP0);
   // pass 2 reads the donor-cell fields
+  if constexpr (sizeof(T) == 8) {
+    if (per) hipLaunchKernelGGL(k_per_mp_grad<NF>, gs, b, 0, c->stream, c->inc, c->ps, a);
+  }
   t_begin(c, KT_MP0 + 1);
   if (c->col_on) hipLaunchKernelGGL((k_mp_grad<T, NF, true>), g, b, 0, c->stream, c->inc, S, a);
   else hipLaunchKernelGGL((k_mp_grad<T, NF, false>), g, b, 0, c->stream, c->inc, S, a);
   t_end(c, KT_MP0 + 1);
+  if constexpr (sizeof(T) == 8) {
+    if (per) hipLaunchKernelGGL(k_per_mp_fct<NF>, gs, b, 0, c->stream, c->inc, c->ps, a);
+  }
   t_begin(c, KT_MP0 + 2);
   if (c->col_on) hipLaunchKernelGGL((k_mp_fct<T, NF, true>), g, b, 0, c->stream, c->inc, S, a);
   else hipLaunchKernelGGL((k_mp_fct<T, NF, false>), g, b, 0, c->stream, c->inc, S, a);
   t_end(c, KT_MP0 + 2);
   double *const *out = (double *const *)c->io[7];
+  if constexpr (sizeof(T) == 8) {
+    if (per) hipLaunchKernelGGL(k_per_mp_limit<NF>, gs, b, 0, c->stream, c->inc, c->ps, a);
+  }
   t_begin(c, KT_MP0 + 3);
   if (c->col_on) hipLaunchKernelGGL((k_mp_limit<T, NF, true>), g, b, 0, c->stream, c->inc, S, a, out);
   else hipLaunchKernelGGL((k_mp_limit<T, NF, false>), g, b, 0, c->stream, c->inc, S, a, out);
@@ -1557,15 +1592,53 @@ int build_band(nft_ctx *c, const nft_mesh *m) {
     deg.swap(uptr);
     adj.swap(uadj);
   }
+  // periodic box: twin columns (those holding the nodes of one coincident set)
+  std::vector<int32_t> tptr, tadj;
+  if (c->per) {
+    std::vector<std::vector<int32_t>> tw(nc);
+    for (const int4 &q : c->h_sets) {
+      const int cnt = q.z >= 0 ? 4 : 2;
+      const int mem[4] = {q.x, q.y, q.z, q.w};
+      for (int a = 0; a < cnt; ++a)
+        for (int b = 0; b < cnt; ++b)
+          if (mem[a] / L != mem[b] / L) tw[mem[a] / L].push_back(mem[b] / L);
+    }
+    tptr.assign(nc + 1, 0);
+    for (int k = 0; k < nc; ++k) {
+      std::sort(tw[k].begin(), tw[k].end());
+      tw[k].erase(std::unique(tw[k].begin(), tw[k].end()), tw[k].end());
+      tadj.insert(tadj.end(), tw[k].begin(), tw[k].end());
+      tptr[k + 1] = (int32_t)tadj.size();
+    }
+  }
+  // greedy colouring: a column takes the smallest colour absent among the columns its
+  // probe reaches, within two column hops; in the periodic box the closures join twin
+  // columns as well, so the reach is twin, hop, twin, hop, twin from the column
   std::vector<int> color(nc, -1), mark(4096, 0);
+  std::vector<int32_t> reach;
+  std::vector<int> seen(c->per ? nc : 0, 0);
   int ncolors = 0;
   for (int k = 0; k < nc; ++k) {
-    for (int q = deg[k]; q < deg[k + 1]; ++q) {
-      const int a = adj[q];
-      if (color[a] >= 0 && color[a] < 4096) mark[color[a]] = k + 1;
-      for (int q2 = deg[a]; q2 < deg[a + 1]; ++q2) {
-        const int b = adj[q2];
-        if (color[b] >= 0 && color[b] < 4096) mark[color[b]] = k + 1;
+    if (c->per) {
+      reach.assign(1, k);
+      seen[k] = k + 1;
+      for (int step = 0; step < 5; ++step) {
+        const std::vector<int32_t> &ptr = step % 2 == 0 ? tptr : deg, &ad = step % 2 == 0 ? tadj : adj;
+        const size_t c0 = reach.size();
+        for (size_t q = 0; q < c0; ++q)
+          for (int e = ptr[reach[q]]; e < ptr[reach[q] + 1]; ++e)
+            if (seen[ad[e]] != k + 1) { seen[ad[e]] = k + 1; reach.push_back(ad[e]); }
+      }
+      for (size_t q = 1; q < reach.size(); ++q)
+        if (color[reach[q]] >= 0 && color[reach[q]] < 4096) mark[color[reach[q]]] = k + 1;
+    } else {
+      for (int q = deg[k]; q < deg[k + 1]; ++q) {
+        const int a = adj[q];
+        if (color[a] >= 0 && color[a] < 4096) mark[color[a]] = k + 1;
+        for (int q2 = deg[a]; q2 < deg[a + 1]; ++q2) {
+          const int b = adj[q2];
+          if (color[b] >= 0 && color[b] < 4096) mark[color[b]] = k + 1;
+        }
       }
     }
     int col = 0;
@@ -1857,7 +1930,40 @@ int nft_device_name(char *buf, int len) {
   return 0;
 }
 
-int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
+}  // extern "C"
+
+namespace {
+
+// the coincident sets of a periodic box: checked (1-based nodes in range, every node in at
+// most one set) and flattened into 0-based member lists in the reference's order (icoin,
+// icoin2, icoin4)
+int periodic_sets(const nft_periodic *pp, int n, std::vector<int4> &sets) {
+  if (pp->ncoin < 0 || pp->ncoin2 < 0 || pp->ncoin4 < 0 || pp->ld_coin < pp->ncoin || pp->ld_coin2 < pp->ncoin2 ||
+      pp->ld_coin4 < pp->ncoin4)
+    return set_err(-1, "nft_init_periodic: bad coincident-set counts or leading dimensions");
+  if ((pp->ncoin && !pp->icoin) || (pp->ncoin2 && !pp->icoin2) || (pp->ncoin4 && !pp->icoin4))
+    return set_err(-1, "nft_init_periodic: null coincident-set array");
+  std::vector<char> seen(n, 0);
+  sets.clear();
+  const int32_t *lists[3] = {pp->icoin, pp->icoin2, pp->icoin4};
+  const int cnt[3] = {pp->ncoin, pp->ncoin2, pp->ncoin4}, ld[3] = {pp->ld_coin, pp->ld_coin2, pp->ld_coin4};
+  for (int t = 0; t < 3; ++t)
+    for (int q = 0; q < cnt[t]; ++q) {
+      int mem[4] = {-1, -1, -1, -1};
+      for (int k = 0; k < (t == 2 ? 4 : 2); ++k) {
+        const int32_t v = lists[t][(size_t)k * ld[t] + q];
+        if (v < 1 || v > n) return set_err(-1, "nft_init_periodic: coincident node out of range");
+        if (seen[v - 1]) return set_err(-1, "nft_init_periodic: node " + std::to_string(v) + " is in more than one coincident set");
+        seen[v - 1] = 1;
+        mem[k] = v - 1;
+      }
+      sets.push_back(make_int4(mem[0], mem[1], mem[2], mem[3]));
+    }
+  if (sets.empty()) return set_err(-1, "nft_init_periodic: no coincident sets");
+  return 0;
+}
+
+int init_impl(const nft_mesh *m, const nft_params *P, const nft_periodic *pp, nft_ctx **out) {
   if (!m || !P || !out) return set_err(-1, "nft_init: null argument");
   *out = nullptr;
   if (P->lord < 1 || P->lord > NFT_MAXD) return set_err(-1, "nft_init: lord out of range 1..18");
@@ -1874,6 +1980,28 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
     int rc0 = validate_mesh(m, "nft_init");
     if (rc0) return rc0;
   }
+  // periodic box: vol summed over the coincident sets before anything reads it (mpsolver
+  // :99-103, cyclicr then cyclicr2 then cyclicr4; the sets are disjoint)
+  std::vector<int4> psets;
+  std::vector<double> volc;
+  nft_mesh mper{};
+  if (pp) {
+    if (P->precision != 0) return set_err(-1, "nft_init_periodic: the periodic box runs in FP64 (precision 0)");
+    int rc0 = periodic_sets(pp, n, psets);
+    if (rc0) return rc0;
+    volc.assign(m->vol, m->vol + n);
+    if (!pp->vol_closed) {
+      for (const int4 &q : psets) {
+        double u = volc[q.x] + volc[q.y];
+        if (q.z >= 0) u = (u + volc[q.z]) + volc[q.w];
+        volc[q.x] = u; volc[q.y] = u;
+        if (q.z >= 0) { volc[q.z] = u; volc[q.w] = u; }
+      }
+    }
+    mper = *m;
+    mper.vol = volc.data();
+    m = &mper;
+  }
   nft_ctx *c = new nft_ctx();
   c->P = *P;
   c->n = n; c->nedge = m->nedge; c->nbface = m->nbface; c->nboun = m->nboun; c->etot = (int)etot;
@@ -1889,6 +2017,24 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   Geometry geo;
   if ((rc = build_geometry(c, m, geo))) return fail(rc);
   c->inc = geo.inc; c->S = geo.S; c->gather_grid = geo.gather_grid;
+  if (pp) {
+    // node -> slot of the set sums (16 values per node: the most a sweep keeps, four
+    // fields' MPDATA bounds and in/out sums)
+    std::vector<int32_t> cslot(n, -1);
+    int ncn = 0;
+    for (const int4 &q : psets)
+      for (int k = 0; k < (q.z >= 0 ? 4 : 2); ++k) cslot[k == 0 ? q.x : k == 1 ? q.y : k == 2 ? q.z : q.w] = ncn++;
+    int4 *d_sets;
+    int32_t *d_cslot;
+    double *d_cval;
+    if ((rc = upload(c, &d_sets, psets.data(), psets.size())) || (rc = upload(c, &d_cslot, cslot.data(), n)) ||
+        (rc = dalloc(c, &d_cval, (size_t)16 * ncn)))
+      return fail(rc);
+    c->per = 1;
+    c->h_sets = psets;
+    c->ps = PerSets{(int)psets.size(), d_sets, d_cslot, d_cval};
+    c->inc.per = 1; c->inc.cslot = d_cslot; c->inc.cval = d_cval;
+  }
   c->col_on = geo.col_on; c->col_bytes = geo.col_bytes; c->n_flag78 = geo.n_flag78;
   c->vec_grid = std::min(blocks_for(n), 4096);
   c->nodes_grid = blocks_for(n);
@@ -2033,6 +2179,17 @@ int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) {
   return 0;
 }
 
+}  // namespace
+
+extern "C" {
+
+int nft_init(const nft_mesh *m, const nft_params *P, nft_ctx **out) { return init_impl(m, P, nullptr, out); }
+
+int nft_init_periodic(const nft_mesh *m, const nft_params *P, const nft_periodic *per, nft_ctx **out) {
+  if (!per) return set_err(-1, "nft_init_periodic: null argument");
+  return init_impl(m, P, per, out);
+}
+
 int nft_finalize(nft_ctx *c) {
   if (!c) return 0;
   if (c->stream) hipStreamSynchronize(c->stream);
@@ -2110,6 +2267,7 @@ int nft_laplc_noalloc(nft_ctx *c, const double *press, double *alap, int impfl) 
   g.f = press; g.oxy = reinterpret_cast<double2 *>(c->scratch_dev); g.o2 = c->scratch_dev + 2 * (size_t)c->n;
   g.impfl = impfl;
   dim3 gg(c->gather_grid), b(NFT_BLOCK);
+  if (c->per) per_sums3(c, press, press, press, nullptr, nullptr);
   if (c->col_on)
     hipLaunchKernelGGL((k_grad<GRAD_LAPLC, true>), gg, b, 0, c->stream, c->inc, c->S, g);
   else
@@ -2117,6 +2275,7 @@ int nft_laplc_noalloc(nft_ctx *c, const double *press, double *alap, int impfl) 
   HIPCHK(hipGetLastError());
   DivArgs d = div_args(c);
   d.qxy = g.oxy; d.q2 = g.o2; d.out = alap;
+  if (c->per) per_sums3(c, nullptr, nullptr, g.o2, g.oxy, nullptr);
   div_go<DIV_LAPLC, 0, 0>(c, d);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -2135,6 +2294,7 @@ int nft_laplc32(nft_ctx *c, const double *press, double *alap, int impfl) {
 
 int nft_laplc_t(nft_ctx *c, const double *y, double *out, int impfl) {
   if (!c || !y || !out) return set_err(-1, "nft_laplc_t: null argument");
+  if (c->per) return set_err(-1, "nft_laplc_t: not available for the periodic box");
   int rc;
   if ((rc = copy_in(c, c->io[0], y))) return rc;
   if ((rc = laplt_dev(c, c->io[0], c->io[1], impfl))) return rc;
@@ -2399,6 +2559,7 @@ int nft_kernel_timer_read(nft_ctx *c, int *launches, double *total_ms, double *b
 int nft_net_init(nft_ctx *c, int hidden, uint64_t seed, int max_batch) {
   if (!c || hidden < 32 || hidden % 32 || max_batch < 1) return set_err(-1, "nft_net_init: bad argument");
   if (c->ncol <= 0 || c->col_L == 0) return set_err(-1, "nft_net_init: needs uniform columns numbered column by column");
+  if (c->per) return set_err(-1, "nft_net_init: the learned preconditioner is not available for the periodic box");
   if (c->net.ready) return set_err(-1, "nft_net_init: already initialised");
   // horizontal column graph from the edges joining different columns at one level
   std::vector<std::vector<int>> nb(c->ncol);
@@ -2913,6 +3074,7 @@ int nft_set_learned(nft_ctx *c, int kind, double stall_ratio, int *fallbacks, in
 int nft_set_multigrid(nft_ctx *c, const nft_mesh *m1, const int32_t *match1, const nft_mesh *m2,
                       const int32_t *match2, const nft_multigrid *mgp) {
   if (!c || !mgp) return set_err(-1, "nft_set_multigrid: null argument");
+  if (c->per) return set_err(-1, "nft_set_multigrid: the coarse levels have no periodic closures here");
   if (mgp->itrp2 < 0 || mgp->itrp_c1 < 0 || mgp->itrp_c2 < 0 || mgp->itrp_cc < 0)
     return set_err(-1, "nft_set_multigrid: negative sweep count");
   if (mgp->transfer != MG_AS_REFERENCE && mgp->transfer != MG_NEIGHBOURS)

@@ -65,6 +65,13 @@ struct Incidence {
   // otherwise)
   const float2 *sxyf;
   const float *szzf;
+  // Doubly periodic box (the reference's iper = 2): per = 1 drops the boundary-point terms
+  // of the gathers and keeps only bcv's flag-11 slip. A node with cslot[i] >= 0 belongs to
+  // a coincident set; its gathered sums are replaced by cval[cslot[i]*K + k], the sums over
+  // its set (cyclicr/cyclicr2/cyclicr4) that a set kernel (k_per_*) left there just before.
+  int per;
+  const int32_t *cslot;
+  const double *cval;
 };
 
 // One slot's edge metric, widened to FP64 (TM = float reads the FP32 copies)
@@ -224,6 +231,7 @@ __device__ __forceinline__ void bcv_node(const Incidence &g, int i, T &u, T &v, 
   int b0 = g.bp_ptr[i], b1 = g.bp_ptr[i + 1];
   for (int b = b0; b < b1; ++b) {
     int f = g.bp_flag[b];
+    if (g.per && f != 11) continue;   // periodic box (:1603-1612): the flag-11 slip only
     if (f == 11 || f == 5 || f == 6) {
       T ax = (T)g.bp_anp[b], ay = (T)g.bp_anp[g.nbp + b], az = (T)g.bp_anp[2 * g.nbp + b];
       T vn = u * ax + v * ay + w * az;
@@ -338,12 +346,16 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S
         gx = ok ? nx : gx; gy = ok ? ny : gy; gz = ok ? nz : gz;
       }
     }
-    if (L.valid) {
+    if (L.valid && !g.per) {
       for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
         gx = gx + g.bp_sbb[b] * fi;
         gy = gy + g.bp_sbb[g.nbp + b] * fi;
         gz = gz + g.bp_sbb[2 * g.nbp + b] * fi;
       }
+    }
+    if (L.valid && g.per) {
+      const int cs = g.cslot[i];
+      if (cs >= 0) { gx = g.cval[3 * cs]; gy = g.cval[3 * cs + 1]; gz = g.cval[3 * cs + 2]; }
     }
   }
   if (!L.valid) return;
@@ -530,10 +542,14 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
       }
     }
     if (L.valid) {
-      for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
+      for (int b = g.bp_ptr[i]; b < (g.per ? 0 : g.bp_ptr[i + 1]); ++b) {
         ax = ax + g.bp_sbb[b] * qi0;
         ay = ay + g.bp_sbb[g.nbp + b] * qi1;
         az = az + g.bp_sbb[2 * g.nbp + b] * qi2;
+      }
+      if (g.per) {
+        const int cs = g.cslot[i];
+        if (cs >= 0) { ax = g.cval[3 * cs]; ay = g.cval[3 * cs + 1]; az = g.cval[3 * cs + 2]; }
       }
       if (MODE == DIV_LAPLC) {
         out = a.nc.rh0_one ? (ax + ay + az) / a.nc.vol[i] : (ax + ay + az) / (a.nc.vol[i] * a.nc.rh0[i]);
@@ -963,7 +979,8 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_courant(Incidence g, EdgeS<double
     else if (FLAG == 2) acc = e >= 0 ? acc + a : acc - a;
     else acc = e >= 0 ? acc + fmax(0., a) : acc - fmin(0., a);
   }
-  for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
+  // the periodic box skips the boundary points (:2049)
+  for (int b = g.bp_ptr[i]; b < (g.per ? 0 : g.bp_ptr[i + 1]); ++b) {
     double a = ui * g.bp_sbb[b] + vi * g.bp_sbb[g.nbp + b] + wi * g.bp_sbb[2 * g.nbp + b];
     if (!nc.rh0_one) a = a / ri;
     if (FLAG == 1) acc = acc + fabs(a);
@@ -1523,13 +1540,19 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_donor(Incidence g, EdgeS<T> S,
   for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
     int fl_ = g.bp_flag[b];
     T aunbb1 = -divv;
-    bool slip = (fl_ == 11 || fl_ == 5 || fl_ == 6);
+    // the periodic box (mpdatamper :3733-3739) has no boundary-point flux
+    bool slip = g.per || (fl_ == 11 || fl_ == 5 || fl_ == 6);
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       T af = a.use_ue[f] ? (T)a.ue[i] : a.afree[f];
       T fl1 = slip ? (T)0 : af * tmin<T>((T)0, aunbb1) + pi[f] * tmax<T>((T)0, aunbb1);
       arh2[f] = arh2[f] + fl1 * (T)1;
     }
+  }
+  if (g.per) {
+    const int cs = g.cslot[i];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) arh2[f] = cs >= 0 ? (T)g.cval[NF * cs + f] : arh2[f];
   }
   T vl = a.vol[i], rh = a.rh0[i];
 #pragma unroll
@@ -1584,10 +1607,20 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_grad(Incidence g, EdgeS<T> S, 
       }
     }
   }
-  for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
+  for (int b = g.bp_ptr[i]; b < (g.per ? 0 : g.bp_ptr[i + 1]); ++b) {
     T bx = (T)g.bp_sbb[b], by = (T)g.bp_sbb[g.nbp + b], bz = (T)g.bp_sbb[2 * g.nbp + b];
 #pragma unroll
     for (int f = 0; f < NF; ++f) { gx[f] = gx[f] + bx * pi[f]; gy[f] = gy[f] + by * pi[f]; gz[f] = gz[f] + bz * pi[f]; }
+  }
+  if (g.per) {
+    const int cs = g.cslot[i];
+    if (cs >= 0) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        gx[f] = (T)g.cval[3 * NF * cs + 3 * f]; gy[f] = (T)g.cval[3 * NF * cs + 3 * f + 1];
+        gz[f] = (T)g.cval[3 * NF * cs + 3 * f + 2];
+      }
+    }
   }
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
@@ -1676,6 +1709,17 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, EdgeS<T> S, M
       }
     }
   }
+  if (g.per) {
+    // bounds merged and in/out sums closed over the coincident set (mpdatamper :3831-3902)
+    const int cs = g.cslot[i];
+    if (cs >= 0) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const double *q = g.cval + 4 * NF * (size_t)cs + 4 * f;
+        mx[f] = (T)q[0]; mn[f] = (T)q[1]; rin[f] = (T)q[2]; rout[f] = (T)q[3];
+      }
+    }
+  }
   T vl = a.vol[i], rh = a.rh0[i];
   const T tol = (T)1.e-10;
 #pragma unroll
@@ -1760,12 +1804,321 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, EdgeS<T> S,
 #pragma unroll
     for (int f = 0; f < NF; ++f) arh2[f] = arh2[f] + (T)0 * (T)1;
   }
+  if (g.per) {
+    const int cs = g.cslot[i];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) arh2[f] = cs >= 0 ? (T)g.cval[NF * cs + f] : arh2[f];
+  }
   T vl = a.vol[i], rh = a.rh0[i];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     T inc = arh2[f] * a.dt / vl / rh;
     out[f][i] = a.psi1[f][i] - (double)inc;
   }
+}
+
+// ------------------------------------------------------------------------------------
+// Periodic box (the reference's iper = 2): set kernels. One thread per coincident set
+// (cyclicr pairs, cyclicr2 pairs, cyclicr4 quadruples; the sets are disjoint) forms each
+// member's gathered sums exactly as the gather kernel that follows would (the member's
+// SELL slots in the reference's edge order, the same operations), sums them over the set
+// in the set's order (a + b; ((a + b) + c) + d) and stores the result for every member in
+// cval; the gather kernel then takes these sums at the set's nodes. The sets hold the two
+// or four copies of a node on opposite sides of the box, so the work is a thin layer of
+// side columns.
+// ------------------------------------------------------------------------------------
+struct PerSets {
+  int nset;
+  const int4 *sets;       // 0-based member nodes in the set's order; z = w = -1 for pairs
+  const int32_t *cslot;   // node -> slot in cval, -1 outside the sets
+  double *cval;
+};
+
+__device__ __forceinline__ int set_size(const int4 &s) { return s.z >= 0 ? 4 : 2; }
+__device__ __forceinline__ int set_member(const int4 &s, int q) {
+  return q == 0 ? s.x : q == 1 ? s.y : q == 2 ? s.z : s.w;
+}
+
+// slot m of node j in the SELL form
+__device__ __forceinline__ void sell_slot(const Incidence &g, int j, int m, int &nb, int &e) {
+  const int slot = g.chunk_ptr[j >> 6] + m * NFT_WAVE + (j & 63);
+  nb = g.nbr[slot];
+  e = g.eid[slot];
+}
+
+// v[q][k] of the set's members -> the set's sums, stored for every member
+template <int K>
+__device__ __forceinline__ void per_close_store(const PerSets &p, const int4 &s, double (&v)[4][K]) {
+  const int cnt = set_size(s);
+  double c[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    c[k] = v[0][k] + v[1][k];
+    if (cnt == 4) c[k] = (c[k] + v[2][k]) + v[3][k];
+  }
+  for (int q = 0; q < cnt; ++q) {
+    double *d = p.cval + (size_t)K * p.cslot[set_member(s, q)];
+#pragma unroll
+    for (int k = 0; k < K; ++k) d[k] = c[k];
+  }
+}
+
+// Three gathered sums per node, sum over slots of +-S_c * ((q_c,i + q_c,j) * 0.5): the
+// gradient of one field (q0 = q1 = q2, k_grad) or the divergence of three (k_div; packed
+// (x, y) inputs when qxy is set, each input times rh0 when scale is set).
+struct PerSums3 {
+  const double *q0, *q1, *q2;
+  const double2 *qxy;
+  const double *scale;
+  const double *skip;
+};
+
+__global__ void __launch_bounds__(NFT_BLOCK) k_per_sums3(Incidence g, PerSets p, PerSums3 a) {
+  if (skipped(a.skip)) return;
+  const int t = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (t >= p.nset) return;
+  const int4 s = p.sets[t];
+  double v[4][3];
+  for (int q = 0; q < set_size(s); ++q) {
+    const int i = set_member(s, q);
+    double qi0, qi1, qi2;
+    if (a.qxy) { const double2 x = a.qxy[i]; qi0 = x.x; qi1 = x.y; }
+    else { qi0 = a.q0[i]; qi1 = a.q1[i]; }
+    qi2 = a.q2[i];
+    if (a.scale) { const double r0 = a.scale[i]; qi0 = qi0 * r0; qi1 = qi1 * r0; qi2 = qi2 * r0; }
+    double ax = 0., ay = 0., az = 0.;
+    const int deg = g.deg[i];
+    for (int m = 0; m < deg; ++m) {
+      int j, e;
+      sell_slot(g, i, m, j, e);
+      const int ea = e >= 0 ? e : ~e;
+      double q0, q1, q2;
+      if (a.qxy) { const double2 x = a.qxy[j]; q0 = x.x; q1 = x.y; }
+      else { q0 = a.q0[j]; q1 = a.q1[j]; }
+      q2 = a.q2[j];
+      if (a.scale) { const double rj = a.scale[j]; q0 = q0 * rj; q1 = q1 * rj; q2 = q2 * rj; }
+      const double2 sxy = g.sxy[ea];
+      const double sz = g.szz[ea];
+      const double u0 = (qi0 + q0) * 0.5, u1 = (qi1 + q1) * 0.5, u2 = (qi2 + q2) * 0.5;
+      const double tx = sxy.x * u0, ty = sxy.y * u1, tz = sz * u2;
+      ax = e >= 0 ? ax + tx : ax - tx;
+      ay = e >= 0 ? ay + ty : ay - ty;
+      az = e >= 0 ? az + tz : az - tz;
+    }
+    v[q][0] = ax; v[q][1] = ay; v[q][2] = az;
+  }
+  per_close_store<3>(p, s, v);
+}
+
+// mpdatamper pass 1 (:3666-3743): donor-cell increments of NF fields, the boundary points'
+// zero fluxes added as the reference adds them
+template <int NF>
+__global__ void __launch_bounds__(NFT_BLOCK) k_per_mp_donor(Incidence g, PerSets p, MpArgs<double, NF> a) {
+  const int t = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (t >= p.nset) return;
+  const int4 s = p.sets[t];
+  double v[4][NF];
+  for (int q = 0; q < set_size(s); ++q) {
+    const int i = set_member(s, q);
+    const double Ui = a.U[i], Vi = a.V[i], Wi = a.W[i];
+    double pi[NF], arh2[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) { pi[f] = a.psi[f][i]; arh2[f] = 0.; }
+    for (int m = 0; m < g.deg[i]; ++m) {
+      int j, e;
+      sell_slot(g, i, m, j, e);
+      const int ea = e >= 0 ? e : ~e;
+      const double2 sxy = g.sxy[ea];
+      const bool first = e >= 0;
+      const double aun = edge_aun<double>(Ui, a.U[j], Vi, a.V[j], Wi, a.W[j], sxy.x, sxy.y, g.szz[ea]);
+      const double ap = tmax<double>(0., aun), an = tmin<double>(0., aun);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const double pj = a.psi[f][j];
+        const double fl = first ? pi[f] * ap + pj * an : pj * ap + pi[f] * an;
+        arh2[f] = first ? arh2[f] + fl : arh2[f] - fl;
+      }
+    }
+    for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) arh2[f] = arh2[f] + 0. * 1.;
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) v[q][f] = arh2[f];
+  }
+  per_close_store<NF>(p, s, v);
+}
+
+// the periodic derivative x3 of the NF donor-cell fields (:3749-3751)
+template <int NF>
+__global__ void __launch_bounds__(NFT_BLOCK) k_per_mp_grad(Incidence g, PerSets p, MpArgs<double, NF> a) {
+  const int t = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (t >= p.nset) return;
+  const int4 s = p.sets[t];
+  double v[4][3 * NF];
+  for (int q = 0; q < set_size(s); ++q) {
+    const int i = set_member(s, q);
+    double pi[NF], gx[NF], gy[NF], gz[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) { pi[f] = a.psi1[f][i]; gx[f] = gy[f] = gz[f] = 0.; }
+    for (int m = 0; m < g.deg[i]; ++m) {
+      int j, e;
+      sell_slot(g, i, m, j, e);
+      const int ea = e >= 0 ? e : ~e;
+      const double2 sxy = g.sxy[ea];
+      const double sz = g.szz[ea];
+      const bool first = e >= 0;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const double uu = (pi[f] + a.psi1[f][j]) * 0.5;
+        const double tx = sxy.x * uu, ty = sxy.y * uu, tz = sz * uu;
+        gx[f] = first ? gx[f] + tx : gx[f] - tx;
+        gy[f] = first ? gy[f] + ty : gy[f] - ty;
+        gz[f] = first ? gz[f] + tz : gz[f] - tz;
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) { v[q][3 * f] = gx[f]; v[q][3 * f + 1] = gy[f]; v[q][3 * f + 2] = gz[f]; }
+  }
+  per_close_store<3 * NF>(p, s, v);
+}
+
+// FCT bounds and in/out sums of the set's members (:3806-3902): each member's bounds from
+// its edges, merged over the set (a pair: the extremes of both nodes and both values; a
+// quadruple: node by node the maximum of the four maxima and values, and as minimum the
+// least of the four maxima and values, as the reference writes it); the in/out sums summed
+// over the set.
+template <int NF>
+__global__ void __launch_bounds__(NFT_BLOCK) k_per_mp_fct(Incidence g, PerSets p, MpArgs<double, NF> a) {
+  const int t = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (t >= p.nset) return;
+  const int4 s = p.sets[t];
+  const int cnt = set_size(s);
+  double mx[4][NF], mn[4][NF], rio[4][2 * NF], pv[4][NF];
+  for (int q = 0; q < cnt; ++q) {
+    const int i = set_member(s, q);
+    const double Ui = a.U[i], Vi = a.V[i], Wi = a.W[i], vri = a.vr[i];
+    double pi[NF], gxi[NF], gyi[NF], gzi[NF], rin[NF], rout[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      pi[f] = a.psi1[f][i];
+      pv[q][f] = pi[f];
+      const double2 g2 = a.gxy[f][i];
+      gxi[f] = g2.x; gyi[f] = g2.y; gzi[f] = a.gz[f][i];
+      mx[q][f] = -1.e15; mn[q][f] = 1.e15; rin[f] = 0.; rout[f] = 0.;
+    }
+    for (int m = 0; m < g.deg[i]; ++m) {
+      int j, e;
+      sell_slot(g, i, m, j, e);
+      const int ea = e >= 0 ? e : ~e;
+      const double2 sxy = g.sxy[ea];
+      const bool first = e >= 0;
+      const double Uj = a.U[j], Vj = a.V[j], Wj = a.W[j];
+      const double aun = edge_aun<double>(Ui, Uj, Vi, Vj, Wi, Wj, sxy.x, sxy.y, g.szz[ea]);
+      const double volt = 0.5 * (vri + a.vr[j]);
+      const double U12 = Ui + Uj, V12 = Vi + Vj, W12 = Wi + Wj;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const double pj = a.psi1[f][j];
+        const double2 gj = a.gxy[f][j];
+        const double A = anti_flux<double>(first ? pi[f] : pj, first ? pj : pi[f], gxi[f] + gj.x, gyi[f] + gj.y,
+                                           gzi[f] + a.gz[f][j], aun, volt, U12, V12, W12, a.dt);
+        mx[q][f] = tmax<double>(tmax<double>(mx[q][f], pi[f]), pj);
+        mn[q][f] = tmin<double>(tmin<double>(mn[q][f], pi[f]), pj);
+        const double pp = tmax<double>(0., A), pn = -tmin<double>(0., A);
+        rin[f] = rin[f] + (first ? pn : pp);
+        rout[f] = rout[f] + (first ? pp : pn);
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) { rio[q][2 * f] = rin[f]; rio[q][2 * f + 1] = rout[f]; }
+  }
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    if (cnt == 2) {
+      const double p1 = pv[0][f], p2 = pv[1][f];
+      mx[0][f] = tmax<double>(tmax<double>(tmax<double>(mx[0][f], p1), p2), mx[1][f]);
+      mn[0][f] = tmin<double>(tmin<double>(tmin<double>(mn[0][f], p1), p2), mn[1][f]);
+      mx[1][f] = tmax<double>(tmax<double>(tmax<double>(mx[1][f], p1), p2), mx[0][f]);
+      mn[1][f] = tmin<double>(tmin<double>(tmin<double>(mn[1][f], p1), p2), mn[0][f]);
+    } else {
+      for (int k = 0; k < 4; ++k) {
+        double hi = mx[0][f], lo;
+        for (int j = 1; j < 4; ++j) hi = tmax<double>(hi, mx[j][f]);
+        for (int j = 0; j < 4; ++j) hi = tmax<double>(hi, pv[j][f]);
+        mx[k][f] = hi;
+        lo = mx[0][f];
+        for (int j = 1; j < 4; ++j) lo = tmin<double>(lo, mx[j][f]);
+        for (int j = 0; j < 4; ++j) lo = tmin<double>(lo, pv[j][f]);
+        mn[k][f] = lo;
+      }
+    }
+  }
+  double c[2 * NF];
+#pragma unroll
+  for (int k = 0; k < 2 * NF; ++k) {
+    c[k] = rio[0][k] + rio[1][k];
+    if (cnt == 4) c[k] = (c[k] + rio[2][k]) + rio[3][k];
+  }
+  for (int q = 0; q < cnt; ++q) {
+    double *d = p.cval + (size_t)4 * NF * p.cslot[set_member(s, q)];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      d[4 * f] = mx[q][f]; d[4 * f + 1] = mn[q][f]; d[4 * f + 2] = c[2 * f]; d[4 * f + 3] = c[2 * f + 1];
+    }
+  }
+}
+
+// mpdatamper pass 2 (:3666-3743 with ipass = 2): limited antidiffusive increments
+template <int NF>
+__global__ void __launch_bounds__(NFT_BLOCK) k_per_mp_limit(Incidence g, PerSets p, MpArgs<double, NF> a) {
+  const int t = blockIdx.x * NFT_BLOCK + threadIdx.x;
+  if (t >= p.nset) return;
+  const int4 s = p.sets[t];
+  double v[4][NF];
+  for (int q = 0; q < set_size(s); ++q) {
+    const int i = set_member(s, q);
+    const double Ui = a.U[i], Vi = a.V[i], Wi = a.W[i], vri = a.vr[i];
+    double pi[NF], gxi[NF], gyi[NF], gzi[NF], cpi[NF], cni[NF], arh2[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      pi[f] = a.psi1[f][i];
+      const double2 g2 = a.gxy[f][i], c2 = a.cpn[f][i];
+      gxi[f] = g2.x; gyi[f] = g2.y; gzi[f] = a.gz[f][i];
+      cpi[f] = c2.x; cni[f] = c2.y;
+      arh2[f] = 0.;
+    }
+    for (int m = 0; m < g.deg[i]; ++m) {
+      int j, e;
+      sell_slot(g, i, m, j, e);
+      const int ea = e >= 0 ? e : ~e;
+      const double2 sxy = g.sxy[ea];
+      const bool first = e >= 0;
+      const double Uj = a.U[j], Vj = a.V[j], Wj = a.W[j];
+      const double aun = edge_aun<double>(Ui, Uj, Vi, Vj, Wi, Wj, sxy.x, sxy.y, g.szz[ea]);
+      const double volt = 0.5 * (vri + a.vr[j]);
+      const double U12 = Ui + Uj, V12 = Vi + Vj, W12 = Wi + Wj;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const double pj = a.psi1[f][j];
+        const double2 gj = a.gxy[f][j], cj = a.cpn[f][j];
+        const double A = anti_flux<double>(first ? pi[f] : pj, first ? pj : pi[f], gxi[f] + gj.x, gyi[f] + gj.y,
+                                           gzi[f] + a.gz[f][j], aun, volt, U12, V12, W12, a.dt);
+        const double cp2 = first ? cj.x : cpi[f], cn1 = first ? cni[f] : cj.y;
+        const double cn2 = first ? cj.y : cni[f], cp1 = first ? cpi[f] : cj.x;
+        const double lim = A > 0. ? tmin<double>(tmin<double>(1., cp2), cn1) : tmin<double>(tmin<double>(1., cn2), cp1);
+        const double fl = A * lim;
+        arh2[f] = first ? arh2[f] + fl : arh2[f] - fl;
+      }
+    }
+    for (int b = g.bp_ptr[i]; b < g.bp_ptr[i + 1]; ++b) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) arh2[f] = arh2[f] + 0. * 1.;
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) v[q][f] = arh2[f];
+  }
+  per_close_store<NF>(p, s, v);
 }
 
 // ------------------------------------------------------------------------------------

@@ -3,10 +3,10 @@
 ! Module nft_hip: bind(C) types and interfaces of every nft_* entry point, plus
 ! nft_attach, which hands the reference's COMMON-block mesh and control data to nft_init.
 !
-! External subroutines derivative, laplc, prfr, bcv, div, precon, gcrk and mpdatam keep
-! the names and argument lists of the reference's subroutines
+! External subroutines derivative, laplc, prfr, bcv, div, precon, gcrk, mpdatam and
+! mpdatamper keep the names and argument lists of the reference's subroutines
 ! (lippsLFroude_implgwopks3D_unstr3.f :3943, :1288, :1536, :1583, :2096, :482, :1776,
-! :3029) so callers with implicit interfaces (mpsolver) link to them unchanged; they move
+! :3029, :3617) so callers with implicit interfaces (mpsolver) link to them unchanged; they move
 ! their arguments to the GPU, run the HIP kernels and copy the results back. gcrk fills
 ! COMMON /itero/ as the reference does (:1953-1963); div with iflg=1 prints the same
 ! diagnostics line (:2148-2168).
@@ -65,6 +65,13 @@ module nft_hip
     real(c_double) :: omega_c
   end type
 
+  ! doubly periodic box (nft_periodic in nft.h)
+  type, bind(C) :: nft_periodic_t
+    integer(c_int) :: ncoin, ncoin2, ncoin4, ld_coin, ld_coin2, ld_coin4
+    type(c_ptr) :: icoin, icoin2, icoin4
+    integer(c_int) :: vol_closed
+  end type
+
   type(c_ptr), save :: nft_context = c_null_ptr
   real(c_double), save :: nft_dt = 0.0d0
 
@@ -81,6 +88,13 @@ module nft_hip
       type(nft_mesh_t), intent(in) :: m1, m2
       type(c_ptr), value :: match1, match2
       type(nft_multigrid_t), intent(in) :: mg
+    end function
+    integer(c_int) function nft_init_periodic(mesh, params, per, ctx) bind(C, name="nft_init_periodic")
+      import :: nft_mesh_t, nft_params_t, nft_periodic_t, c_ptr, c_int
+      type(nft_mesh_t), intent(in) :: mesh
+      type(nft_params_t), intent(in) :: params
+      type(nft_periodic_t), intent(in) :: per
+      type(c_ptr), intent(out) :: ctx
     end function
     integer(c_int) function nft_finalize(ctx) bind(C, name="nft_finalize")
       import :: c_ptr, c_int
@@ -260,10 +274,22 @@ contains
   ! (ncol = 0: none); precon_kind: 1 point relaxation, 2 vertical line (two-point-flux
   ! tridiagonal), 3 vertical line (in-column band of laplc); precision: 0 FP64, 1 mixed
   ! (FP32 MPDATA and line solve), 2 mixed with the FP32 operator inside GCR; strat/theta0: the values mpsolver hard-codes (:134-135).
-  subroutine nft_attach(ncol, col_start, precon_kind, precision, lord, strat, theta0)
+  ! With iper = 2 in COMMON /per/ (the periodic box) the coincident sets of /coin/, /per2/
+  ! and /per4/ go to nft_init_periodic; vol_closed = 1 says that COMMON vol already holds
+  ! mpsolver's sums over the sets (:99-103: attach after them), 0 or absent that it does not
+  ! (attach before them, e.g. right after `input`).
+  subroutine nft_attach(ncol, col_start, precon_kind, precision, lord, strat, theta0, vol_closed)
     integer, intent(in) :: ncol, precon_kind, precision, lord
     integer(c_int), intent(in), target :: col_start(*)
     real(8), intent(in) :: strat, theta0
+    integer, intent(in), optional :: vol_closed
+    integer :: iper, ncoin, ncoin2, ncoin4, icoin, icoin2, icoin4, irecog1
+    common/per/iper
+    common/coin/ncoin,icoin(nfacesx*2,2),irecog1(nfacesx)
+    common/per2/icoin2(nfacesx,2),ncoin2
+    common/per4/icoin4(nfacesx,4),ncoin4
+    target :: icoin, icoin2, icoin4
+    type(nft_periodic_t) :: per
     integer :: npoin, nedge, nbface, nbpoin, nboun
     common/n/npoin,nedge,nbface,nbpoin,nboun
     real(8) :: coord, vol, sn, sb, sbb, anp
@@ -309,7 +335,16 @@ contains
     p%precision = precision
     p%itraj = 0   ! mpsolver hard-codes itraj=0 (:111); nft_init accepts 1 for velprd
     nft_dt = delt
-    call nft_check(nft_init(m, p, nft_context), 'nft_init')
+    if (iper == 2) then
+      per%ncoin = ncoin; per%ncoin2 = ncoin2; per%ncoin4 = ncoin4
+      per%ld_coin = 2 * nfacesx; per%ld_coin2 = nfacesx; per%ld_coin4 = nfacesx
+      per%icoin = c_loc(icoin); per%icoin2 = c_loc(icoin2); per%icoin4 = c_loc(icoin4)
+      per%vol_closed = 0
+      if (present(vol_closed)) per%vol_closed = vol_closed
+      call nft_check(nft_init_periodic(m, p, per, nft_context), 'nft_init_periodic')
+    else
+      call nft_check(nft_init(m, p, nft_context), 'nft_init')
+    end if
     if (mg_flag == 1) call nft_attach_multigrid(itrp2, itrp_c1, itrp_c2, itrp_cc)
   end subroutine
 
@@ -468,6 +503,16 @@ subroutine mpdatam(uuu, unk, iflag)
   real(8) :: uuu(npoinx,3), unk(*)
   integer :: iflag
   call nft_check(nft_mpdatam(nft_context, uuu, npoinx, unk, iflag), 'mpdatam')
+end subroutine
+
+! mpdatamper (reference :3617-3941), which mpsolver calls in the periodic box (:314-319):
+! nft_mpdatam on the periodic context that nft_attach made from iper = 2
+subroutine mpdatamper(uuu, unk, iflag)
+  use nft_hip
+  implicit none
+  real(8) :: uuu(npoinx,3), unk(*)
+  integer :: iflag
+  call nft_check(nft_mpdatam(nft_context, uuu, npoinx, unk, iflag), 'mpdatamper')
 end subroutine
 
 ! courant(uuu,cour,iflag) (reference :1974-2094) on the GPU: the Courant numbers of the

@@ -7,7 +7,8 @@
 ! reference's per-step iteration line and writes the final state (unk, uuu, prpot) as
 ! raw float64 to nftdrv_state.bin. With mg_flag=1 in input_mg_v3.d it also reads the coarse
 ! meshes mesh3d_c.d / mesh3d_cc.d and the node maps match1.d / match2.d and runs precon's
-! multigrid branch.
+! multigrid branch. With iper=2 in inputinc.d (the periodic box) it reads the coincident
+! sets that follow the mesh (:2569-2583) and runs the periodic scheme (nft_init_periodic).
 !
 ! Usage: nftdrv <nlev> <precon> <precision> <lord> <strat> [exit_mode] [itraj]
 !   nlev      levels per column when nodes are numbered column by column (0: no columns)
@@ -35,6 +36,9 @@ program nftdrv
   integer(c_int), allocatable, target :: cols(:), match1(:), match2(:)
   type(nft_mesh_t) :: m1, m2
   type(nft_multigrid_t) :: mg
+  type(nft_periodic_t) :: per
+  integer(c_int), allocatable, target :: coin(:,:), coin2(:,:), coin4(:,:)
+  integer :: ncs(3) = 0
   real(8), allocatable :: state(:,:)
   type(nft_mesh_t) :: m
   type(nft_params_t) :: p
@@ -77,8 +81,8 @@ program nftdrv
     read(1,*); read(1,*) tw
   end if
   close(1)
-  if (isfera /= 0 .or. iper /= 0) then
-    write(*,*) 'nftdrv: only the Cartesian, non-periodic form is supported'
+  if (isfera /= 0 .or. (iper /= 0 .and. iper /= 2)) then
+    write(*,*) 'nftdrv: only the Cartesian box, closed (iper=0) or periodic (iper=2), is supported'
     error stop 2
   end if
 
@@ -110,7 +114,15 @@ program nftdrv
   p%precision = prec
   p%itraj = itj
 
-  call nft_check(nft_init(m, p, nft_context), 'nft_init')
+  if (iper == 2) then
+    per%ncoin = ncs(1); per%ncoin2 = ncs(2); per%ncoin4 = ncs(3)
+    per%ld_coin = size(coin, 1); per%ld_coin2 = size(coin2, 1); per%ld_coin4 = size(coin4, 1)
+    per%icoin = c_loc(coin); per%icoin2 = c_loc(coin2); per%icoin4 = c_loc(coin4)
+    per%vol_closed = 0
+    call nft_check(nft_init_periodic(m, p, per, nft_context), 'nft_init_periodic')
+  else
+    call nft_check(nft_init(m, p, nft_context), 'nft_init')
+  end if
   if (mg_flag == 1) then
     ! precon's multigrid branch: coarse meshes and node maps as `input` reads them
     ! (:2415-2672), including its level-2 metrics quirk (sn_cc y/z from sn_c, :2467-2468)
@@ -192,6 +204,27 @@ contains
     do i = 1, mt%nboun
       read(5,*) ii, (mt%anp(i,k), k=1,3)
     end do
+    if (iper == 2 .and. fname == 'mesh3d.d') then
+      ! the coincident sets of the periodic box (:2569-2583)
+      read(5,*) k
+      ncs(1) = k
+      allocate(coin(max(k,1),2))
+      do i = 1, k
+        read(5,*) coin(i,1), coin(i,2)
+      end do
+      read(5,*) k
+      ncs(2) = k
+      allocate(coin2(max(k,1),2))
+      do i = 1, k
+        read(5,*) coin2(i,1), coin2(i,2)
+      end do
+      read(5,*) k
+      ncs(3) = k
+      allocate(coin4(max(k,1),4))
+      do i = 1, k
+        read(5,*) coin4(i,1), coin4(i,2), coin4(i,3), coin4(i,4)
+      end do
+    end if
     close(5)
   end subroutine
 

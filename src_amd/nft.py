@@ -29,6 +29,12 @@ class NftMesh(ctypes.Structure):
                 ("vol", f64p), ("ncol", ctypes.c_int), ("col_start", i32p)]
 
 
+class NftPeriodic(ctypes.Structure):
+    _fields_ = [("ncoin", ctypes.c_int), ("ncoin2", ctypes.c_int), ("ncoin4", ctypes.c_int),
+                ("ld_coin", ctypes.c_int), ("ld_coin2", ctypes.c_int), ("ld_coin4", ctypes.c_int),
+                ("icoin", i32p), ("icoin2", i32p), ("icoin4", i32p), ("vol_closed", ctypes.c_int)]
+
+
 class NftParams(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_double), ("u0", ctypes.c_double), ("v0", ctypes.c_double),
                 ("w0", ctypes.c_double), ("strat", ctypes.c_double), ("theta0", ctypes.c_double),
@@ -72,6 +78,8 @@ class RunStats(ctypes.Structure):
 _SIGS = {
     "nft_init": (ctypes.c_int, [ctypes.POINTER(NftMesh), ctypes.POINTER(NftParams),
                                 ctypes.POINTER(ctypes.c_void_p)]),
+    "nft_init_periodic": (ctypes.c_int, [ctypes.POINTER(NftMesh), ctypes.POINTER(NftParams),
+                                         ctypes.POINTER(NftPeriodic), ctypes.POINTER(ctypes.c_void_p)]),
     "nft_finalize": (ctypes.c_int, [ctypes.c_void_p]),
     "nft_last_error": (ctypes.c_char_p, []),
     "nft_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
@@ -288,7 +296,17 @@ class Solver:
         m, self._keep = _mesh_struct(mesh, use_columns)
         self.params = params
         h = ctypes.c_void_p()
-        _chk(L.nft_init(ctypes.byref(m), ctypes.byref(params), ctypes.byref(h)), "nft_init")
+        self.periodic = getattr(mesh, "coin", None) is not None
+        if self.periodic:
+            # the doubly periodic box: the coincident sets as the reference reads them
+            sets = [np.ascontiguousarray(a, dtype=np.int32) for a in (mesh.coin, mesh.coin2, mesh.coin4)]
+            self._keep = list(self._keep) + sets
+            per = NftPeriodic(*(a.shape[1] for a in sets), *(a.shape[1] for a in sets),
+                              *(a.ctypes.data_as(i32p) for a in sets), 0)
+            _chk(L.nft_init_periodic(ctypes.byref(m), ctypes.byref(params), ctypes.byref(per), ctypes.byref(h)),
+                 "nft_init_periodic")
+        else:
+            _chk(L.nft_init(ctypes.byref(m), ctypes.byref(params), ctypes.byref(h)), "nft_init")
         self.h = h
 
     def close(self):

@@ -8,7 +8,9 @@
 !   prfr(press,u*,v*,w*,u,v,w,impfl)   :1536     bcv(uout,vout,wout,flag)       :1583
 !   div(ua,va,wa,rhs,iflg)             :2096     precon(rhs,p,r,iprc,elas,impfl) :482
 !   mpdatam(uuu,unk(1,k),k)            as at :309-312 (columns of unk(npoinx,4) by
-!                                      sequence association)
+!                                      sequence association); mpdatamper as at :315-318
+!                                      when inputinc.d sets iper = 2 (the periodic box,
+!                                      whose coincident sets follow the mesh, :2569-2583)
 !   gcrk(p,unk1(1,1),unk1(1,2),unk1(1,3),uuu(1,1),uuu(1,2),uuu(1,3),1)  as at :362-365
 !   courant(uuu,cour,iflag)            :1974     stats(unk)                     :2847
 ! Inputs: vec_in.bin, 8 float64 node arrays (p, u, v, w, r, U, V, W). Outputs: every result
@@ -44,12 +46,17 @@ program nftdrop
   real(8) :: omega, sm_flag1, sm_flag2
   common/vcycle/mg_flag,omega,itrp1,itrp2,itrp_c1,itrp_c2,itrp_cc
   common/smooth/sm_flag1,sm_flag2
+  integer :: iper, ncoin, ncoin2, ncoin4, icoin, icoin2, icoin4, irecog1
+  common/per/iper
+  common/coin/ncoin,icoin(nfacesx*2,2),irecog1(nfacesx)
+  common/per2/icoin2(nfacesx,2),ncoin2
+  common/per4/icoin4(nfacesx,4),ncoin4
 
   ! mpsolver's work arrays, dimensioned with npoinx as the reference's
   real(8), allocatable :: unk(:,:), uuu(:,:), unk1(:,:), vin(:,:)
   real(8), allocatable :: a1(:), a2(:), a3(:), pp(:), rr(:), cour(:)
   integer(c_int), allocatable, target :: cols(:)
-  integer :: nlev, pk, prec, lord, i, k, ii, ntime, nrkstage, istst, mrestart, isfera, iper
+  integer :: nlev, pk, prec, lord, i, k, ii, ntime, nrkstage, istst, mrestart, isfera
   integer :: ielast, lordin, itrp, match_flag, ncol
   real(8) :: strat, cfl, relax, re, betap, elas
   character(len=64) :: arg
@@ -107,6 +114,21 @@ program nftdrop
   do i = 1, nboun
     read(5,*) ii, (anp(i,k), k=1,3)
   end do
+  ncoin = 0; ncoin2 = 0; ncoin4 = 0
+  if (iper == 2) then
+    read(5,*) ncoin
+    do i = 1, ncoin
+      read(5,*) icoin(i,1), icoin(i,2)
+    end do
+    read(5,*) ncoin2
+    do i = 1, ncoin2
+      read(5,*) icoin2(i,1), icoin2(i,2)
+    end do
+    read(5,*) ncoin4
+    do i = 1, ncoin4
+      read(5,*) icoin4(i,1), icoin4(i,2), icoin4(i,3), icoin4(i,4)
+    end do
+  end if
   close(5)
 
   ncol = 0
@@ -159,7 +181,11 @@ program nftdrop
   unk(1:npoin,1) = vin(:,2); unk(1:npoin,2) = vin(:,3); unk(1:npoin,3) = vin(:,4)
   unk(1:npoin,4) = vin(:,1)
   do k = 1, 4
-    call mpdatam(uuu, unk(1,k), k)
+    if (iper == 2) then
+      call mpdatamper(uuu, unk(1,k), k)
+    else
+      call mpdatam(uuu, unk(1,k), k)
+    end if
   end do
   do k = 1, 4
     write(30) unk(1:npoin,k)

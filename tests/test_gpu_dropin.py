@@ -6,7 +6,9 @@ calls nft_attach, then derivative, laplc, prfr, bcv, div, precon, mpdatam(uuu,un
 and gcrk(p,unk1(1,1),...,uuu(1,1),...,1) with npoinx leading dimensions, and courant/stats;
 every output and /itero/ (lippsLFroude_implgwopks3D_unstr3.f :1953-1963) must equal the
 oracle bit for bit. courant and stats work on the arrays they are given and leave the
-device-resident time-step state alone.
+device-resident time-step state alone. In the periodic box (iper = 2 in inputinc.d) the
+program reads the coincident sets after the mesh into /coin/, /per2/, /per4/, nft_attach
+builds the periodic context from them, and mpdatamper replaces mpdatam.
 """
 import os
 import subprocess
@@ -14,7 +16,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from common import BAND_KW, diag_close, mesh_for, seeded_fields, small_cfg, tiny_cfg
+from common import BAND_KW, diag_close, mesh_for, periodic_cfg, seeded_fields, small_cfg, tiny_cfg
 from oracle import Oracle
 from src_amd import nft
 from src_amd.refio import write_reference_inputs
@@ -24,14 +26,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DROP = os.path.join(ROOT, "tests", "fortran", "nftdrop")
 
 
-@pytest.mark.parametrize("key,kw", [("tiny", dict()), ("small", dict(BAND_KW, exit_mode=0, itr=5))],
-                         ids=["tiny-point-relaxation", "small-band"])
+@pytest.mark.parametrize("key,kw", [("tiny", dict()), ("small", dict(BAND_KW, exit_mode=0, itr=5)),
+                                    ("periodic2", dict())],
+                         ids=["tiny-point-relaxation", "small-band", "periodic-box"])
 def test_dropin_subroutines_match_oracle(tmp_path, key, kw):
     if not os.path.exists(DROP):
         pytest.skip("nftdrop not built (flang missing)")
     print(f"case {key} {kw}", flush=True)
     m = mesh_for(key)
-    cfg = (tiny_cfg if key == "tiny" else small_cfg)(**kw)
+    cfg = {"tiny": tiny_cfg, "small": small_cfg, "periodic2": periodic_cfg}[key](**kw)
     d = str(tmp_path)
     write_reference_inputs(d, m, cfg, ntime=1, sm_flag=1, itrp1=cfg.itrp1, omega=cfg.omega)
     f = seeded_fields(m)

@@ -7,7 +7,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from common import mesh_for, parse_diagnostics, tiny_cfg, rel_l2
+from common import mesh_for, parse_diagnostics, periodic_cfg, tiny_cfg, rel_l2
 from oracle import Oracle
 from src_amd import nft
 from src_amd.refio import write_reference_inputs
@@ -149,3 +149,34 @@ def test_fortran_driver_band_line_and_precision(tmp_path, precision):
     assert np.array_equal(st[0:4], unk)
     assert np.array_equal(st[4:7], uuu)
     assert np.array_equal(st[7], prpot)
+
+
+@pytest.mark.gpu
+def test_fortran_driver_periodic_box(tmp_path):
+    """iper=2: nftdrv reads the coincident sets after the mesh and runs the periodic scheme
+    through nft_init_periodic; 3 steps equal the ctypes path and the oracle (tree order)
+    bit for bit."""
+    if not os.path.exists(DRV):
+        pytest.skip("nftdrv not built")
+    m = mesh_for("periodic2")
+    cfg = periodic_cfg(converged=True)
+    nsteps = 3
+    d = str(tmp_path)
+    write_reference_inputs(d, m, cfg, ntime=nsteps, sm_flag=1, itrp1=cfg.itrp1, omega=cfg.omega)
+    r = subprocess.run([DRV, str(m.nlev), str(cfg.precon), "0", str(cfg.lord), repr(cfg.strat), str(cfg.exit_mode)],
+                       cwd=d, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    st = np.fromfile(os.path.join(d, "nftdrv_state.bin"), dtype=np.float64).reshape(8, m.npoin)
+    s = nft.Solver(m, nft.make_params(cfg))
+    s.state_init()
+    s.run_steps(nsteps)
+    assert np.array_equal(st[0:4], s.state("unk"))
+    assert np.array_equal(st[4:7], s.state("uuu"))
+    assert np.array_equal(st[7], s.state("prpot"))
+    s.close()
+    o = Oracle(m, cfg, dot_order=1)
+    o.init()
+    o.step(nsteps)
+    assert np.array_equal(st[0:4], o.state("unk"))
+    assert np.array_equal(st[7], o.state("prpot"))

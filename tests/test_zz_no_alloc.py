@@ -9,7 +9,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from common import BAND_KW, mesh_for, seeded_fields, small_cfg, tiny_cfg
+from common import BAND_KW, mesh_for, periodic_cfg, seeded_fields, small_cfg, tiny_cfg
 from src_amd import mesh as M
 from src_amd import nft
 
@@ -47,11 +47,12 @@ class _Hip:
 
 
 def test_laplc_noalloc_equals_laplc():
-    """Calls in a row on the 40,000-node mesh, then on the 120-node mesh (a smaller input
-    while the shared scratch still holds the larger intermediate), both impfl: every result
-    equals nft_laplc's."""
+    """Calls in a row on the 40,000-node mesh, then on the 15,840-node periodic box (its
+    set sums in the context), then on the 120-node mesh (each a smaller input while the
+    shared scratch still holds the larger intermediate), both impfl: every result equals
+    nft_laplc's."""
     hip = _Hip()
-    for key, cfg in (("small", small_cfg(**BAND_KW)), ("tiny", tiny_cfg())):
+    for key, cfg in (("small", small_cfg(**BAND_KW)), ("periodic2", periodic_cfg()), ("tiny", tiny_cfg())):
         print(f"case {key}", flush=True)
         m = mesh_for(key)
         s = nft.Solver(m, nft.make_params(cfg))
