@@ -475,7 +475,8 @@ void t_harvest(nft_ctx *c) { t_harvest_prefix(c, c->tev_used); }
 void per_sums3(nft_ctx *c, const double *q0, const double *q1, const double *q2, const double2 *qxy,
                const double *scale) {
   PerSums3 a{q0, q1, q2, qxy, scale, c->skip};
-  hipLaunchKernelGGL(k_per_sums3, dim3(blocks_for(c->ps.nset)), dim3(NFT_BLOCK), 0, c->stream, c->inc, c->ps, a);
+  hipLaunchKernelGGL(k_per_sums3, dim3((c->ps.nset + NFT_SETS_PER_BLOCK - 1) / NFT_SETS_PER_BLOCK), dim3(NFT_BLOCK), 0,
+                     c->stream, c->inc, c->ps, a);
 }
 
 int grad_launch(nft_ctx *c, int mode, const GradArgs &a) {

@@ -1818,9 +1818,9 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, EdgeS<T> S,
 }
 
 // ------------------------------------------------------------------------------------
-// Periodic box (the reference's iper = 2): set kernels. One thread per coincident set
-// (cyclicr pairs, cyclicr2 pairs, cyclicr4 quadruples; the sets are disjoint) forms each
-// member's gathered sums exactly as the gather kernel that follows would (the member's
+// Periodic box (the reference's iper = 2): set kernels. Per coincident set (cyclicr pairs,
+// cyclicr2 pairs, cyclicr4 quadruples; the sets are disjoint) one thread, or one per member
+// in k_per_sums3, the kernel of every laplc/div/prfr, forms each member's gathered sums exactly as the gather kernel that follows would (the member's
 // SELL slots in the reference's edge order, the same operations), sums them over the set
 // in the set's order (a + b; ((a + b) + c) + d) and stores the result for every member in
 // cval; the gather kernel then takes these sums at the set's nodes. The sets hold the two
@@ -1873,20 +1873,24 @@ struct PerSums3 {
   const double *skip;
 };
 
+// four threads per set, one per member (pairs leave two idle): each gathers its member's
+// sums, then every member thread closes the set's sums from LDS and stores its own copy
+#define NFT_SETS_PER_BLOCK (NFT_BLOCK / 4)
 __global__ void __launch_bounds__(NFT_BLOCK) k_per_sums3(Incidence g, PerSets p, PerSums3 a) {
-  if (skipped(a.skip)) return;
-  const int t = blockIdx.x * NFT_BLOCK + threadIdx.x;
-  if (t >= p.nset) return;
-  const int4 s = p.sets[t];
-  double v[4][3];
-  for (int q = 0; q < set_size(s); ++q) {
+  __shared__ double sh[3][NFT_BLOCK];
+  if (skipped(a.skip)) return;   // uniform: every thread leaves before the barrier
+  const int t = blockIdx.x * NFT_SETS_PER_BLOCK + (threadIdx.x >> 2), q = threadIdx.x & 3;
+  const bool on = t < p.nset;
+  const int4 s = on ? p.sets[t] : make_int4(-1, -1, -1, -1);
+  const int cnt = on ? set_size(s) : 0;
+  double ax = 0., ay = 0., az = 0.;
+  if (q < cnt) {
     const int i = set_member(s, q);
     double qi0, qi1, qi2;
     if (a.qxy) { const double2 x = a.qxy[i]; qi0 = x.x; qi1 = x.y; }
     else { qi0 = a.q0[i]; qi1 = a.q1[i]; }
     qi2 = a.q2[i];
     if (a.scale) { const double r0 = a.scale[i]; qi0 = qi0 * r0; qi1 = qi1 * r0; qi2 = qi2 * r0; }
-    double ax = 0., ay = 0., az = 0.;
     const int deg = g.deg[i];
     for (int m = 0; m < deg; ++m) {
       int j, e;
@@ -1905,9 +1909,19 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_per_sums3(Incidence g, PerSets p,
       ay = e >= 0 ? ay + ty : ay - ty;
       az = e >= 0 ? az + tz : az - tz;
     }
-    v[q][0] = ax; v[q][1] = ay; v[q][2] = az;
   }
-  per_close_store<3>(p, s, v);
+  sh[0][threadIdx.x] = ax; sh[1][threadIdx.x] = ay; sh[2][threadIdx.x] = az;
+  __syncthreads();
+  if (q < cnt) {
+    const int b = threadIdx.x & ~3;
+    double* d = p.cval + (size_t)3 * p.cslot[set_member(s, q)];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double c = sh[k][b] + sh[k][b + 1];
+      if (cnt == 4) c = (c + sh[k][b + 2]) + sh[k][b + 3];
+      d[k] = c;
+    }
+  }
 }
 
 // mpdatamper pass 1 (:3666-3743): donor-cell increments of NF fields, the boundary points'
