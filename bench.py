@@ -50,6 +50,16 @@ def stress_config(column_order="grid"):
                        sponge_depth=3000.0, sponge_tau=300.0)
 
 
+def periodic_config(column_order="grid", precon=0):
+    """The production mesh and flow as a doubly periodic box (the reference's iper = 2:
+    coincident side nodes, no boundary-point terms, mpdatamper), GCR(4) to 1e-8 without a
+    preconditioner (0; measured fastest there) or with the two-point-flux line (2); a
+    secondary workload."""
+    import dataclasses
+    c = production_config("grid")
+    return dataclasses.replace(c, name="periodic", mesh=dataclasses.replace(c.mesh, periodic=True), precon=precon)
+
+
 def small_config():
     """The 40,000-node mesh and flow (2,000 jittered columns x 20 levels, dt = 10 s, U = 10 m/s,
     N = 0.01 1/s) with GCR(3) to 1e-8 relative residual and the band line preconditioner."""
@@ -113,6 +123,9 @@ WORKLOADS = {
                    "({nodes} nodes, {edges} edges), hill h0=1 km, U=10 m/s, N=0.01 1/s (Fr=1), dt=20 s; MPDATA + "
                    "2 x GCR(4) to 1e-8 relative residual with the vertical line preconditioner (in-column band of the "
                    "pressure operator), FP64"),
+    "periodic": ("production mesh as a doubly periodic box (iper=2): 65,536 jittered columns x 64 levels "
+                 "({nodes} nodes, {edges} edges), hill h0=1 km, U=10 m/s, N=0.01 1/s, dt=20 s; mpdatamper + "
+                 "2 x GCR(4) to 1e-8 relative residual without preconditioner, FP64"),
     "stress": ("stress mesh: 131,072 jittered columns x 128 levels stretched 50-400 m over 300 km x 300 km x 25 km "
                "({nodes} nodes, {edges} edges), hill h0=2 km, U=5 m/s, N=0.025 1/s (Fr=0.1), dt=15 s; MPDATA + "
                "2 x GCR(4) to 1e-8 relative residual with the vertical line preconditioner (in-column band of the "
@@ -339,8 +352,9 @@ def main():
     ap.add_argument("--long-run", type=int, default=0,
                     help="instead of the benchmark: N steps of the chosen mesh's full configuration, FP64 "
                          "and the FP32-operator mixed form side by side, deviation reported every 100 steps")
-    ap.add_argument("--mesh", default="production", choices=("production", "stress"),
-                    help="workload: the production mesh (headline metric) or the 16.8M-node stress mesh")
+    ap.add_argument("--mesh", default="production", choices=("production", "stress", "periodic"),
+                    help="workload: the production mesh (headline metric), the 16.8M-node stress mesh or the "
+                         "production mesh as a periodic box")
     ap.add_argument("--column-order", default="grid", choices=("grid", "hilbert"),
                     help="numbering of the production mesh's columns")
     ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed", "mixed_op"),
@@ -358,7 +372,8 @@ def main():
     from src_amd import mesh as M
     from src_amd import nft
 
-    cfg = (production_config if args.mesh == "production" else stress_config)(args.column_order)
+    cfg = {"production": production_config, "stress": stress_config, "periodic": periodic_config}[args.mesh](
+        args.column_order)
     t0 = time.perf_counter()
     mesh = M.generate(cfg.mesh)
     log(f"mesh: {mesh.npoin} nodes ({time.perf_counter() - t0:.1f} s)")
@@ -412,9 +427,10 @@ def main():
             "workload": WORKLOADS[args.mesh].format(nodes=mesh.npoin, edges=mesh.nedge + mesh.nbface),
             "nodes": mesh.npoin, "edges": mesh.nedge + mesh.nbface, "gcr_order": cfg.lord,
             "tolerance": cfg.eps0,
-            "preconditioner": ("vertical line: the in-column pentadiagonal band of laplc (probed on the GPU), "
-                               "banded LU per column" if cfg.precon == 3 else
-                               "vertical line (two-point-flux tridiagonal, wavefront scans of the Thomas factors)"),
+            "preconditioner": {3: "vertical line: the in-column pentadiagonal band of laplc (probed on the GPU), "
+                                  "banded LU per column",
+                               2: "vertical line (two-point-flux tridiagonal, wavefront scans of the Thomas factors)",
+                               0: "none"}[cfg.precon],
             "precision": args.precision if args.precision == "fp64" else PRECISION_TEXT[args.precision],
         },
         "gcr": {"iters_per_solve": round(iters, 2), "solve_ms": round(st.ms_gcr / max(st.gcr_solves, 1), 3),
@@ -497,12 +513,33 @@ def main():
             log(f"[mixed error, 40,000-node mesh] {me['forms']}; last row {me['error_vs_fp64']['rows'][-1]} "
                 f"({time.perf_counter() - t0:.1f} s)")
         if args.mesh == "production":
+            # the production mesh as a periodic box (iper = 2): the set kernels' cost, with
+            # the two-point-flux line and with no preconditioner
+            for pk in (2, 0):
+                t0 = time.perf_counter()
+                pc = periodic_config(precon=pk)
+                pm = M.generate(pc.mesh)
+                rp = timed_run(pm, pc, "fp64", mk, args.warmup, profile=False)
+                sp = rp["st"]
+                line["variants"][f"periodic_precon{pk}"] = {
+                    "value": round(1000.0 / (sp.ms_total / mk), 4), "unit": "timesteps/s",
+                    "iters_per_solve": round(sp.gcr_iters / max(sp.gcr_solves, 1), 2),
+                    "ms_per_gcr_iteration": round(sp.ms_gcr / max(sp.gcr_iters, 1), 4),
+                    "mpdata_ms_per_step": round(sp.ms_mpdata / mk, 3), "steps": mk,
+                    "unconverged": sp.gcr_unconverged,
+                    "coincident_sets": int(pm.coin.shape[1] + pm.coin2.shape[1] + pm.coin4.shape[1]),
+                    "preconditioner": "two-point-flux vertical line" if pk == 2 else "none"}
+                rp["solver"].close()
+                log(f"[periodic precon {pk}] {line['variants'][f'periodic_precon{pk}']} "
+                    f"({time.perf_counter() - t0:.1f} s)")
             for kind in ("reference", "line"):
                 t0 = time.perf_counter()
                 line["variants"][f"multigrid_{kind}"] = multigrid_run(mesh, cfg, 2, kind)
                 log(f"[multigrid {kind}] {line['variants'][f'multigrid_{kind}']} "
                     f"({time.perf_counter() - t0:.1f} s)")
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.mesh == "periodic":
+        log("cpu baseline: sampled for the production and stress meshes only")
+    elif not args.no_cpu_baseline:
         t0 = time.perf_counter()
         line["cpu_baseline"] = cpu_baseline(mesh, cfg, iters, args.mesh)
         log(f"cpu baseline sample took {time.perf_counter() - t0:.1f} s")
