@@ -17,6 +17,7 @@ import pytest
 from common import (diag_close, mesh_checksum, mesh_for, periodic_cfg, periodic_field, seeded_fields,
                     unique_nodes)
 from oracle import Oracle
+from src_amd import mesh as M
 from src_amd import nft
 
 pytestmark = pytest.mark.gpu
@@ -232,3 +233,17 @@ def test_periodic_refuses_what_it_does_not_restate():
     with pytest.raises(nft.NftError, match="periodic"):
         s.laplc_t(np.zeros(m.npoin), 1)
     s.close()
+
+
+def test_periodic_init_rejects_bad_sets():
+    """nft_init_periodic checks its sets before anything reaches a kernel: a node in two
+    sets, a node id out of range."""
+    bad = M.generate(M.periodic_spec())     # a mesh of its own (the cached one stays intact)
+    coin = bad.coin.copy()
+    bad.coin[1, 0] = bad.coin2[0, 0]        # also in a y-side pair
+    with pytest.raises(nft.NftError, match="more than one"):
+        solver(bad, periodic_cfg())
+    bad.coin = coin.copy()
+    bad.coin[0, 3] = bad.npoin + 1
+    with pytest.raises(nft.NftError, match="out of range"):
+        solver(bad, periodic_cfg())
