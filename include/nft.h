@@ -267,7 +267,8 @@ int nft_mesh_release(nft_mesh *mesh);
 /* Kernel ids for the timer: 1 laplc gradient half, 2 laplc divergence half, 3 line solve,
  * 4 GCR update, 5 GCR direction, 6..9 MPDATA donor/gradient/FCT/limit, 10 the coarse-level
  * part of the multigrid V-cycle (a chain of kernels; bytes_per_launch 0), 11 the learned
- * correction's network forward pass (bytes_per_launch holds its MFMA FLOPs). While a kernel id
+ * correction's network forward pass (bytes_per_launch holds its MFMA FLOPs), 12 MPDATA's
+ * antidiffusive-flux sweep (each edge once, between the gradient and FCT sweeps). While a kernel id
  * is selected, every launch of that kernel is bracketed by HIP events on the context's
  * stream. bytes_per_launch is the kernel's algorithmic traffic: every array it reads or
  * writes counted once at its stored width (incidence slots actually used, edge metrics,

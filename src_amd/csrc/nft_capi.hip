@@ -95,6 +95,10 @@ struct nft_ctx {
   std::vector<void *> allocs;
 
   Incidence inc;
+  // every edge once, in the role of its first node i1 (MPDATA's antidiffusive-flux sweep);
+  // same chunks and launch shape as inc
+  Incidence inc_own{};
+  double own_col_bytes = 0;
   int gather_grid = 0;   // launched blocks of gather kernels
   int vec_grid = 0;      // fixed grid of grid-stride vector kernels
   int nodes_grid = 0;    // one thread per node
@@ -149,6 +153,7 @@ struct nft_ctx {
   double *r, *qr, *ar, *pw, *rw;
   double *mp_psi1[4];
   void *mp_g[12], *mp_c[8];
+  void *mp_flux = nullptr;   // antidiffusive velocity per edge: two (field, field) pair arrays
   double *partials;
   unsigned *counter;   // [0]: arrivals at the end of k_reduce_final (zero between launches)
   double *fin_wres;    // k_reduce_final's wavefront results: NFT_FIN_WG x (NFT_MAXD + 1)
@@ -436,7 +441,7 @@ void setup_fields(nft_ctx *c, const nft_mesh *m, std::vector<double> &rh0, std::
 }
 
 // ---- kernel timer ------------------------------------------------------------------------
-enum { KT_LGRAD = 1, KT_LDIV = 2, KT_LINE = 3, KT_UPD = 4, KT_DIR = 5, KT_MP0 = 6, KT_MG = 10, KT_NET = 11 };
+enum { KT_LGRAD = 1, KT_LDIV = 2, KT_LINE = 3, KT_UPD = 4, KT_DIR = 5, KT_MP0 = 6, KT_MG = 10, KT_NET = 11, KT_MPF = 12 };
 
 inline bool timing(nft_ctx *c, int k) { return c->timer_kernel == k && c->tev_used + 2 <= (int)c->tev.size(); }
 inline void t_begin(nft_ctx *c, int k) {
@@ -1320,6 +1325,8 @@ int mpdata_dev(nft_ctx *c, const T *U, const T *V, const T *W, double *const *fi
     a.gxy[f] = (typename MpArgs<T, NF>::T2 *)c->mp_g[3 * f];   // 2n values
     a.gz[f] = (T *)c->mp_g[3 * f + 2];
     a.cpn[f] = (typename MpArgs<T, NF>::T2 *)c->mp_c[2 * f];   // 2n values
+    if (f % 2 == 0)
+      a.aflux[f / 2] = (typename MpArgs<T, NF>::T2 *)c->mp_flux + (size_t)(f / 2) * c->etot;
     int fl = iflags[f];
     double afree[5] = {c->P.u0, c->P.v0, c->P.w0, 0., 1.};
     a.use_ue[f] = fl == 1;
@@ -1354,17 +1361,24 @@ int mpdata_dev(nft_ctx *c, const T *U, const T *V, const T *W, double *const *fi
   if constexpr (sizeof(T) == 8) {
     if (per) hipLaunchKernelGGL(k_per_mp_fct<NF>, gs, b, 0, c->stream, c->inc, c->ps, a);
   }
+  // every edge's antidiffusive velocity once, over the owned incidence (same chunks and grid)
+  Incidence go = c->inc_own;
+  go.sxy = c->inc.sxy; go.szz = c->inc.szz;
+  t_begin(c, KT_MPF);
+  if (c->col_on && go.col_L > 0) hipLaunchKernelGGL((k_mp_flux<T, NF, true>), g, b, 0, c->stream, go, S, a);
+  else hipLaunchKernelGGL((k_mp_flux<T, NF, false>), g, b, 0, c->stream, go, S, a);
+  t_end(c, KT_MPF);
   t_begin(c, KT_MP0 + 2);
-  if (c->col_on) hipLaunchKernelGGL((k_mp_fct<T, NF, true>), g, b, 0, c->stream, c->inc, S, a);
-  else hipLaunchKernelGGL((k_mp_fct<T, NF, false>), g, b, 0, c->stream, c->inc, S, a);
+  if (c->col_on) hipLaunchKernelGGL((k_mp_fct<T, NF, true>), g, b, 0, c->stream, c->inc, a);
+  else hipLaunchKernelGGL((k_mp_fct<T, NF, false>), g, b, 0, c->stream, c->inc, a);
   t_end(c, KT_MP0 + 2);
   double *const *out = (double *const *)c->io[7];
   if constexpr (sizeof(T) == 8) {
     if (per) hipLaunchKernelGGL(k_per_mp_limit<NF>, gs, b, 0, c->stream, c->inc, c->ps, a);
   }
   t_begin(c, KT_MP0 + 3);
-  if (c->col_on) hipLaunchKernelGGL((k_mp_limit<T, NF, true>), g, b, 0, c->stream, c->inc, S, a, out);
-  else hipLaunchKernelGGL((k_mp_limit<T, NF, false>), g, b, 0, c->stream, c->inc, S, a, out);
+  if (c->col_on) hipLaunchKernelGGL((k_mp_limit<T, NF, true>), g, b, 0, c->stream, c->inc, a, out);
+  else hipLaunchKernelGGL((k_mp_limit<T, NF, false>), g, b, 0, c->stream, c->inc, a, out);
   t_end(c, KT_MP0 + 3);
   HIPCHK(hipGetLastError());
   return 0;
@@ -1753,44 +1767,37 @@ int validate_mesh(const nft_mesh *m, const char *who) {
   return 0;
 }
 
-// Device geometry of one mesh: the node-sorted SELL-64 incidence (each node's slots in the
-// global edge order: interior edges, then boundary-face edges), its columnar form when the
-// mesh is extruded with identical interior levels, the per-node boundary-point tables and
-// the edge metrics. Host copies of the edge lists and metrics stay in G for the callers.
-struct Geometry {
-  Incidence inc{};
-  EdgeS<double> S{};
-  int gather_grid = 0;
-  int col_on = 0;
+// Node-sorted slots of one incidence on the host: SELL-64 (each node's slots in the
+// global edge order, interior edges then boundary-face edges) and, when the mesh is extruded
+// with identical interior levels, the columnar form of the same slots. both: every edge in
+// the roles of both its nodes (the gathers); otherwise only in the role of its first node
+// i1 (the owned incidence: each edge appears once). Every chunk gets at least one slot and
+// every column at least one bottom/top slot pair, so a lane whose node has no slots still
+// reads a valid entry (which it then drops).
+struct HostSlots {
+  int nchunk = 0;
+  std::vector<int32_t> deg, chunk_ptr, nbr, eid;
+  int colL = 0;
+  std::vector<int32_t> tptr, xptr;
+  std::vector<int2> tab;
+  int xoff = 0;
   double col_bytes = 0;
-  int n_flag78 = 0;   // boundary entries with flag 7 or 8 (where bcv reads ue)
-  std::vector<int32_t> i1, i2;
-  std::vector<double> sx, sy, sz;
 };
 
-int build_geometry(nft_ctx *c, const nft_mesh *m, Geometry &G) {
-  const int n = m->npoin;
-  const long etot = (long)m->nedge + m->nbface;
-  int rc = 0;
-  G.i1.assign(etot, 0); G.i2.assign(etot, 0);
-  G.sx.assign(etot, 0.); G.sy.assign(etot, 0.); G.sz.assign(etot, 0.);
-  std::vector<int32_t> &i1 = G.i1, &i2 = G.i2;
-  std::vector<double> &sx = G.sx, &sy = G.sy, &sz = G.sz;
-  for (int e = 0; e < m->nedge; ++e) {
-    i1[e] = m->iedge[e] - 1; i2[e] = m->iedge[m->ld_edge + e] - 1;
-    sx[e] = m->sn[e]; sy[e] = m->sn[m->ld_edge + e]; sz[e] = m->sn[2 * (size_t)m->ld_edge + e];
+int host_slots(int n, int ncol, const int32_t *col_start, const std::vector<int32_t> &i1,
+               const std::vector<int32_t> &i2, bool both, HostSlots &H) {
+  const long etot = (long)i1.size();
+  std::vector<int32_t> &deg = H.deg, &chunk_ptr = H.chunk_ptr, &nbr = H.nbr, &eid = H.eid;
+  deg.assign(n, 0);
+  for (long e = 0; e < etot; ++e) {
+    deg[i1[e]]++;
+    if (both) deg[i2[e]]++;
   }
-  for (int e = 0; e < m->nbface; ++e) {
-    long k = m->nedge + e;
-    i1[k] = m->kbface[e] - 1; i2[k] = m->kbface[m->ld_face + e] - 1;
-    sx[k] = m->sb[e]; sy[k] = m->sb[m->ld_face + e]; sz[k] = m->sb[2 * (size_t)m->ld_face + e];
-  }
-  std::vector<int32_t> deg(n, 0);
-  for (long e = 0; e < etot; ++e) { deg[i1[e]]++; deg[i2[e]]++; }
   const int nchunk = (n + NFT_WAVE - 1) / NFT_WAVE;
-  std::vector<int32_t> chunk_ptr(nchunk + 1, 0);
+  H.nchunk = nchunk;
+  chunk_ptr.assign(nchunk + 1, 0);
   for (int ch = 0; ch < nchunk; ++ch) {
-    int w = 0;
+    int w = 1;
     for (int r = 0; r < NFT_WAVE; ++r) {
       int i = ch * NFT_WAVE + r;
       if (i < n) w = std::max(w, deg[i]);
@@ -1799,7 +1806,8 @@ int build_geometry(nft_ctx *c, const nft_mesh *m, Geometry &G) {
     chunk_ptr[ch + 1] = chunk_ptr[ch] + NFT_WAVE * w;
   }
   const long nslot = chunk_ptr[nchunk];
-  std::vector<int32_t> nbr(nslot), eid(nslot), fill(n, 0);
+  nbr.assign(nslot, 0); eid.assign(nslot, 0);
+  std::vector<int32_t> fill(n, 0);
   for (int i = 0; i < n; ++i) {
     int ch = i / NFT_WAVE, r = i % NFT_WAVE;
     int w = (chunk_ptr[ch + 1] - chunk_ptr[ch]) / NFT_WAVE;
@@ -1812,19 +1820,20 @@ int build_geometry(nft_ctx *c, const nft_mesh *m, Geometry &G) {
   };
   for (long e = 0; e < etot; ++e) {
     put(i1[e], i2[e], (int32_t)e);
-    put(i2[e], i1[e], ~(int32_t)e);
+    if (both) put(i2[e], i1[e], ~(int32_t)e);
   }
   // columnar form: template of level 1 per column, checked against every interior level
-  std::vector<int32_t> tptr, tnb, tco, xptr, xnb, xco;
-  int colL = 0;
-  if (m->ncol > 0 && n % m->ncol == 0 && n / m->ncol >= 3) {
-    const int Lc = n / m->ncol;
+  std::vector<int32_t> &tptr = H.tptr, &xptr = H.xptr;
+  std::vector<int32_t> tnb, tco, xnb, xco;
+  H.colL = 0;
+  if (ncol > 0 && n % ncol == 0 && n / ncol >= 3) {
+    const int Lc = n / ncol;
     bool ok = true;
-    for (int k = 0; k <= m->ncol && ok; ++k) ok = m->col_start[k] == k * Lc;
+    for (int k = 0; k <= ncol && ok; ++k) ok = col_start[k] == k * Lc;
     auto sl = [&](int i, int q) { int ch = i / NFT_WAVE, r = i % NFT_WAVE; return chunk_ptr[ch] + (long)q * NFT_WAVE + r; };
-    tptr.assign(m->ncol + 1, 0);
-    xptr.assign(m->ncol + 1, 0);
-    for (int cc = 0; cc < m->ncol && ok; ++cc) {
+    tptr.assign(ncol + 1, 0);
+    xptr.assign(ncol + 1, 0);
+    for (int cc = 0; cc < ncol && ok; ++cc) {
       const int i1n = cc * Lc + 1, w = deg[i1n];
       for (int q = 0; q < w; ++q) {
         long s1 = sl(i1n, q);
@@ -1848,7 +1857,7 @@ int build_geometry(nft_ctx *c, const nft_mesh *m, Geometry &G) {
         }
       }
       const int ib = cc * Lc, it = cc * Lc + Lc - 1;
-      const int xw = std::max(deg[ib], deg[it]);
+      const int xw = std::max(1, std::max(deg[ib], deg[it]));
       for (int q = 0; q < xw; ++q)
         for (int t = 0; t < 2; ++t) {
           int i = t ? it : ib;
@@ -1858,8 +1867,81 @@ int build_geometry(nft_ctx *c, const nft_mesh *m, Geometry &G) {
         }
       xptr[cc + 1] = xptr[cc] + 2 * xw;
     }
-    if (ok) colL = Lc;
+    if (ok) {
+      H.colL = Lc;
+      H.tab.resize(tnb.size() + xnb.size());
+      for (size_t q = 0; q < tnb.size(); ++q) H.tab[q] = make_int2(tnb[q], tco[q]);
+      for (size_t q = 0; q < xnb.size(); ++q) H.tab[tnb.size() + q] = make_int2(xnb[q], xco[q]);
+      H.xoff = (int)tnb.size();
+      H.col_bytes = 4.0 * (tptr.size() + xptr.size()) + 8.0 * H.tab.size();
+    }
   }
+  return 0;
+}
+
+// the slot arrays of H on the device, into g (sizes, SELL-64 arrays, columnar tables and the
+// launch shape; boundary tables and metrics are the caller's)
+int upload_slots(nft_ctx *c, int n, const HostSlots &H, Incidence &g) {
+  int rc;
+  int32_t *d_chunk, *d_deg, *d_nbr, *d_eid;
+  if ((rc = upload(c, &d_chunk, H.chunk_ptr.data(), H.chunk_ptr.size())) ||
+      (rc = upload(c, &d_deg, H.deg.data(), H.deg.size())) || (rc = upload(c, &d_nbr, H.nbr.data(), H.nbr.size())) ||
+      (rc = upload(c, &d_eid, H.eid.data(), H.eid.size())))
+    return rc;
+  g.n = n; g.nchunk = H.nchunk; g.chunk_ptr = d_chunk; g.deg = d_deg; g.nbr = d_nbr; g.eid = d_eid;
+  g.col_L = 0;
+  if (H.colL > 0) {
+    int32_t *d_tptr, *d_xptr;
+    int2 *d_tab;
+    if ((rc = upload(c, &d_tptr, H.tptr.data(), H.tptr.size())) ||
+        (rc = upload(c, &d_xptr, H.xptr.data(), H.xptr.size())) || (rc = upload(c, &d_tab, H.tab.data(), H.tab.size())))
+      return rc;
+    g.col_L = H.colL;
+    g.tptr = d_tptr; g.xptr = d_xptr; g.tab = d_tab; g.xoff = H.xoff;
+  }
+  g.nblocks = (H.nchunk + NFT_WPB - 1) / NFT_WPB;
+  g.xcd_per = g.nblocks >= 64 ? (g.nblocks + 7) / 8 : 0;
+  return 0;
+}
+
+// Device geometry of one mesh: the node-sorted SELL-64 incidence (each node's slots in the
+// global edge order: interior edges, then boundary-face edges), its columnar form when the
+// mesh is extruded with identical interior levels, the per-node boundary-point tables and
+// the edge metrics. Host copies of the edge lists and metrics stay in G for the callers.
+struct Geometry {
+  Incidence inc{};
+  EdgeS<double> S{};
+  int gather_grid = 0;
+  int col_on = 0;
+  double col_bytes = 0;
+  int n_flag78 = 0;   // boundary entries with flag 7 or 8 (where bcv reads ue)
+  // the owned incidence (every edge once, at its first node), built when want_own is set
+  int want_own = 0;
+  Incidence own{};
+  double own_col_bytes = 0;
+  std::vector<int32_t> i1, i2;
+  std::vector<double> sx, sy, sz;
+};
+
+int build_geometry(nft_ctx *c, const nft_mesh *m, Geometry &G) {
+  const int n = m->npoin;
+  const long etot = (long)m->nedge + m->nbface;
+  int rc = 0;
+  G.i1.assign(etot, 0); G.i2.assign(etot, 0);
+  G.sx.assign(etot, 0.); G.sy.assign(etot, 0.); G.sz.assign(etot, 0.);
+  std::vector<int32_t> &i1 = G.i1, &i2 = G.i2;
+  std::vector<double> &sx = G.sx, &sy = G.sy, &sz = G.sz;
+  for (int e = 0; e < m->nedge; ++e) {
+    i1[e] = m->iedge[e] - 1; i2[e] = m->iedge[m->ld_edge + e] - 1;
+    sx[e] = m->sn[e]; sy[e] = m->sn[m->ld_edge + e]; sz[e] = m->sn[2 * (size_t)m->ld_edge + e];
+  }
+  for (int e = 0; e < m->nbface; ++e) {
+    long k = m->nedge + e;
+    i1[k] = m->kbface[e] - 1; i2[k] = m->kbface[m->ld_face + e] - 1;
+    sx[k] = m->sb[e]; sy[k] = m->sb[m->ld_face + e]; sz[k] = m->sb[2 * (size_t)m->ld_face + e];
+  }
+  HostSlots H;
+  if ((rc = host_slots(n, m->ncol, m->col_start, i1, i2, true, H))) return rc;
 
   // boundary points grouped per node, in entry order
   std::vector<int32_t> bp_ptr(n + 1, 0);
@@ -1878,39 +1960,27 @@ int build_geometry(nft_ctx *c, const nft_mesh *m, Geometry &G) {
       banp[(size_t)d * nbp + q] = m->anp[(size_t)d * m->ld_face + b];
     }
   }
-  int32_t *d_chunk, *d_deg, *d_nbr, *d_eid, *d_bp_ptr, *d_bflag;
+  int32_t *d_bp_ptr, *d_bflag;
   double *d_bsbb, *d_banp;
-  if ((rc = upload(c, &d_chunk, chunk_ptr.data(), chunk_ptr.size())) ||
-      (rc = upload(c, &d_deg, deg.data(), deg.size())) || (rc = upload(c, &d_nbr, nbr.data(), nbr.size())) ||
-      (rc = upload(c, &d_eid, eid.data(), eid.size())) || (rc = upload(c, &d_bp_ptr, bp_ptr.data(), bp_ptr.size())) ||
+  if ((rc = upload(c, &d_bp_ptr, bp_ptr.data(), bp_ptr.size())) ||
       (rc = upload(c, &d_bflag, bflag.data(), bflag.size())) || (rc = upload(c, &d_bsbb, bsbb.data(), bsbb.size())) ||
       (rc = upload(c, &d_banp, banp.data(), banp.size())))
     return rc;
   Incidence &g = G.inc;
-  g.n = n; g.nchunk = nchunk; g.chunk_ptr = d_chunk; g.deg = d_deg; g.nbr = d_nbr; g.eid = d_eid;
-  g.col_L = 0;
-  if (colL > 0) {
-    std::vector<int2> tab(tnb.size() + xnb.size());
-    for (size_t q = 0; q < tnb.size(); ++q) tab[q] = make_int2(tnb[q], tco[q]);
-    for (size_t q = 0; q < xnb.size(); ++q) tab[tnb.size() + q] = make_int2(xnb[q], xco[q]);
-    int32_t *d_tptr, *d_xptr;
-    int2 *d_tab;
-    if ((rc = upload(c, &d_tptr, tptr.data(), tptr.size())) || (rc = upload(c, &d_xptr, xptr.data(), xptr.size())) ||
-        (rc = upload(c, &d_tab, tab.data(), tab.size())))
-      return rc;
-    g.col_L = colL;
-    g.tptr = d_tptr; g.xptr = d_xptr; g.tab = d_tab; g.xoff = (int)tnb.size();
+  if ((rc = upload_slots(c, n, H, g))) return rc;
+  if (g.col_L > 0) {
     G.col_on = 1;
-    G.col_bytes = 4.0 * (tptr.size() + xptr.size()) + 8.0 * tab.size();
+    G.col_bytes = H.col_bytes;
   }
   g.bp_ptr = d_bp_ptr; g.bp_flag = d_bflag; g.bp_sbb = d_bsbb; g.bp_anp = d_banp; g.nbp = nbp;
-  g.nblocks = (nchunk + NFT_WPB - 1) / NFT_WPB;
-  if (g.nblocks >= 64) {
-    g.xcd_per = (g.nblocks + 7) / 8;
-    G.gather_grid = 8 * g.xcd_per;
-  } else {
-    g.xcd_per = 0;
-    G.gather_grid = g.nblocks;
+  G.gather_grid = g.xcd_per > 0 ? 8 * g.xcd_per : g.nblocks;
+  if (G.want_own) {
+    // each edge once, in the role of its first node: MPDATA's antidiffusive-flux sweep
+    HostSlots Ho;
+    if ((rc = host_slots(n, m->ncol, m->col_start, i1, i2, false, Ho)) || (rc = upload_slots(c, n, Ho, G.own)))
+      return rc;
+    G.own.nbp = 0;
+    G.own_col_bytes = Ho.col_bytes;
   }
   double *dsx, *dsy, *dsz;
   if ((rc = upload(c, &dsx, sx.data(), etot)) || (rc = upload(c, &dsy, sy.data(), etot)) ||
@@ -2029,8 +2099,10 @@ int init_impl(const nft_mesh *m, const nft_params *P, const nft_periodic *pp, nf
 
   // ---- geometry: incidence, columnar form, boundary tables, edge metrics
   Geometry geo;
+  geo.want_own = 1;
   if ((rc = build_geometry(c, m, geo))) return fail(rc);
   c->inc = geo.inc; c->S = geo.S; c->gather_grid = geo.gather_grid;
+  c->inc_own = geo.own; c->own_col_bytes = geo.own_col_bytes;
   if (pp) {
     // node -> slot of the set sums (16 values per node: the most a sweep keeps, four
     // fields' MPDATA bounds and in/out sums)
@@ -2159,6 +2231,13 @@ int init_impl(const nft_mesh *m, const nft_params *P, const nft_periodic *pp, nf
     void *q;
     if (hipMalloc(&q, (k % 2 == 0 ? 2 : 1) * n * tsz) != hipSuccess) return fail(set_err(-2, "alloc"));
     c->allocs.push_back(q); c->mp_c[k] = q;
+  }
+  {
+    // two arrays of (field, field) pairs over all edges, sized for FP64
+    void *q;
+    if (hipMalloc(&q, (size_t)std::max(c->etot, 1) * 2 * sizeof(double2)) != hipSuccess)
+      return fail(set_err(-2, "alloc"));
+    c->allocs.push_back(q); c->mp_flux = q;
   }
   for (int k = 0; k < 7; ++k)
     if ((rc = dalloc(c, &c->io[k], n))) return fail(rc);
@@ -2499,7 +2578,7 @@ int nft_get_gcr_stats(nft_ctx *c, nft_gcr_stats *s) {
 }
 
 int nft_kernel_timer(nft_ctx *c, int kernel) {
-  if (!c || kernel < 0 || kernel > 11) return set_err(-1, "nft_kernel_timer: bad argument");
+  if (!c || kernel < 0 || kernel > 12) return set_err(-1, "nft_kernel_timer: bad argument");
   HIPCHK(hipStreamSynchronize(c->stream));
   t_harvest(c);
   if (c->tev.empty()) {
@@ -2518,7 +2597,8 @@ int nft_kernel_timer(nft_ctx *c, int kernel) {
 // algorithmic bytes of one launch: each array read or written counted once at its width
 static double kernel_bytes(nft_ctx *c, int k) {
   const double N = c->n, E = c->etot, NB = c->inc.nbp;
-  const double rest = 4 * (c->inc.nchunk + 1) + 4 * N + 4 * (N + 1) + NB * (4 + 48);
+  const double chunks = 4 * (c->inc.nchunk + 1) + 4 * N;   // chunk pointers and degrees
+  const double rest = chunks + 4 * (N + 1) + NB * (4 + 48);
   // the sweeps read the columnar tables instead of the nbr/eid slot streams when in use
   const double slots = c->col_on ? c->col_bytes : 2 * E * 8;
   const double geomg = 24 * E + slots + rest;   // FP64 metrics
@@ -2544,9 +2624,15 @@ static double kernel_bytes(nft_ctx *c, int k) {
     case KT_DIR: return (3 + 2.0 * c->P.lord) * 8 * N;  /* qr, ar, r + up to lord x/ax; writes 2 */
     case KT_MP0: return (mixed ? geomf : geom) + 3 * ts * N + 4 * 8 * N + 4 * 8 * N + 2 * ts * N + 8 * N;
     case KT_MP0 + 1: return (mixed ? geomf : geom) + 4 * 8 * N + 12 * ts * N;
-    // fct and limit read vol, rh0 and their product vr (3 arrays)
-    case KT_MP0 + 2: return (mixed ? geomf : geom) + 3 * ts * N + 4 * 8 * N + 12 * ts * N + 3 * ts * N + 8 * ts * N;
-    case KT_MP0 + 3: return (mixed ? geomf : geom) + 3 * ts * N + 4 * 8 * N + 12 * ts * N + 3 * ts * N + 8 * ts * N + 4 * 8 * N;
+    // the flux sweep walks the owned incidence (each edge once) and reads the metrics, the
+    // advector, vr, the donor-cell fields and their gradients; it writes 4 values per edge
+    case KT_MPF: return (c->col_on && c->inc_own.col_L > 0 ? c->own_col_bytes : 8 * E) + chunks + (mixed ? 12 : 24) * E +
+                        3 * ts * N + ts * N + 4 * 8 * N + 12 * ts * N + 4 * ts * E;
+    // bounds and ratios: slots, donor-cell fields, the edge fluxes, vol and rh0; (cp, cn) out
+    case KT_MP0 + 2: return slots + chunks + 4 * 8 * N + 4 * ts * E + 2 * ts * N + 8 * ts * N;
+    // limited update: slots, boundary-entry ranges, the edge fluxes, (cp, cn), donor-cell
+    // fields, vol and rh0; the fields out
+    case KT_MP0 + 3: return slots + chunks + 4 * (N + 1) + 4 * ts * E + 8 * ts * N + 4 * 8 * N + 2 * ts * N + 4 * 8 * N;
     case KT_NET: {  // FLOPs of one forward pass over every column (the six MFMA GEMMs)
       const auto &nt = c->net;
       double fl = 0;

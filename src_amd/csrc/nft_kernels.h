@@ -1431,6 +1431,7 @@ struct MpArgs {
   T2 *gxy[NF];                     // gradients of the donor-cell fields: (x, y) pairs ...
   T *gz[NF];                       // ... and z
   T2 *cpn[NF];                     // limiter ratios (cp, cn) per node
+  T2 *aflux[(NF + 1) / 2];         // antidiffusive velocity per edge, fields in pairs
   const T *vol, *rh0;              // T copies
   const T *vr;                     // vol*rh0, the same product formed once per node
   const double *ue;
@@ -1464,7 +1465,9 @@ __device__ __forceinline__ void mp_metric(const Incidence &g, const EdgeS<T> &S,
 // The MPDATA sweeps gather per node over its slots in the reference's edge order. The
 // edge quantities are written with the edge's (i1, i2) roles where they are not symmetric
 // (p2 - p1, the limiter's ratios); sums and products of the two ends are symmetric in IEEE
-// arithmetic, so both end nodes of an edge obtain the reference's bits. Each node's own
+// arithmetic, so both end nodes of an edge obtain the reference's bits. The antidiffusive
+// velocity, the costly one (three divisions per field), is evaluated once per edge by its
+// first node and stored; the bounds and limiter sweeps read it. Each node's own
 // values are read once, before its slot loop. As in the gradient and divergence sweeps,
 // every lane of a wavefront walks the chunk's full slot width with clamped (always valid)
 // slot entries, the next batch's entries are loaded while the current batch's gathers are
@@ -1643,9 +1646,13 @@ __device__ __forceinline__ T anti_flux(T p1, T p2, T gsx, T gsy, T gsz, T aun, T
   return un - aun * term * dt * (T)0.5;
 }
 
-// FCT bounds from the donor-cell solution, in/out fluxes and ratios (reference :3228-3280)
+// Antidiffusive velocities of the edges (reference :3187-3221), each evaluated once: a node
+// walks the owned incidence (only the slots where it is the edge's first node i1), so every
+// edge is taken exactly once, and stores the edge's value of every field at the edge index:
+// aflux[0][e] = (field 0, field 1), aflux[1][e] = (field 2, field 3). The bounds and limiter
+// sweeps read it from there in both end nodes' slot loops.
 template <typename T, int NF, bool COL>
-__global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, EdgeS<T> S, MpArgs<T, NF> a) {
+__global__ void __launch_bounds__(NFT_BLOCK) k_mp_flux(Incidence g, EdgeS<T> S, MpArgs<T, NF> a) {
   constexpr int B = NFT_BATCH_MP;
   typedef typename MpArgs<T, NF>::T2 T2;
   Lane L;
@@ -1656,10 +1663,9 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, EdgeS<T> S, M
   const int i = L.i;
   T Ui = a.U[i], Vi = a.V[i], Wi = a.W[i];
   T vri = a.vr[i];
-  T mx[NF], mn[NF], rin[NF], rout[NF], pi[NF], gxi[NF], gyi[NF], gzi[NF];
+  T pi[NF], gxi[NF], gyi[NF], gzi[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    mx[f] = (T)-1.e15; mn[f] = (T)1.e15; rin[f] = (T)0; rout[f] = (T)0;
     pi[f] = (T)a.psi1[f][i];
     const T2 g2 = a.gxy[f][i];
     gxi[f] = g2.x; gyi[f] = g2.y; gzi[f] = a.gz[f][i];
@@ -1689,16 +1695,81 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, EdgeS<T> S, M
     }
 #pragma unroll
     for (int u = 0; u < B; ++u) {
-      const bool ok = m0 + u < L.deg;
-      const bool first = ee[u] >= 0;
+      // every slot of the owned incidence has this node as i1
       T aun = edge_aun<T>(Ui, Uj[u], Vi, Vj[u], Wi, Wj[u], sx[u], sy[u], sz[u]);
       T volt = (T)0.5 * (vri + vrj[u]);
       T U12 = Ui + Uj[u], V12 = Vi + Vj[u], W12 = Wi + Wj[u];
+      T A[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        A[f] = anti_flux<T>(pi[f], pj[u][f], gxi[f] + gjx[u][f], gyi[f] + gjy[u][f], gzi[f] + gjz[u][f], aun, volt,
+                            U12, V12, W12, a.dt);
+      if (m0 + u < L.deg) {
+#pragma unroll
+        for (int q = 0; q < (NF + 1) / 2; ++q) {
+          T2 v;
+          v.x = A[2 * q];
+          v.y = 2 * q + 1 < NF ? A[2 * q + 1] : (T)0;
+          a.aflux[q][ee[u]] = v;
+        }
+      }
+    }
+  }
+}
+
+// the antidiffusive velocities of one slot's edge, all fields, from the flux sweep's store
+template <typename T, int NF>
+__device__ __forceinline__ void mp_load_flux(const MpArgs<T, NF> &a, int e, T (&A)[NF]) {
+#pragma unroll
+  for (int q = 0; q < (NF + 1) / 2; ++q) {
+    const typename MpArgs<T, NF>::T2 v = a.aflux[q][e];
+    A[2 * q] = v.x;
+    if (2 * q + 1 < NF) A[2 * q + 1] = v.y;
+  }
+}
+
+// FCT bounds from the donor-cell solution, in/out fluxes and ratios (reference :3228-3280)
+template <typename T, int NF, bool COL>
+__global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, MpArgs<T, NF> a) {
+  constexpr int B = NFT_BATCH_MP;
+  typedef typename MpArgs<T, NF>::T2 T2;
+  Lane L;
+  if (!lane_setup(g, L)) return;
+  ColLane C{};
+  if (COL) col_setup(g, L.valid ? L.i : g.n - 1, C);
+  if (!L.valid) return;
+  const int i = L.i;
+  T mx[NF], mn[NF], rin[NF], rout[NF], pi[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    mx[f] = (T)-1.e15; mn[f] = (T)1.e15; rin[f] = (T)0; rout[f] = (T)0;
+    pi[f] = (T)a.psi1[f][i];
+  }
+  int2 rn[B];
+#pragma unroll
+  for (int u = 0; u < B; ++u) rn[u] = slot_raw<COL>(g, L, C, u);
+  for (int m0 = 0; m0 < L.width; m0 += B) {
+    int jj[B], ee[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) slot_decode<COL>(C, rn[u], jj[u], ee[u]);
+    if (m0 + B < L.width) {
+#pragma unroll
+      for (int u = 0; u < B; ++u) rn[u] = slot_raw<COL>(g, L, C, m0 + B + u);
+    }
+    T pj[B][NF], Af[B][NF];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      mp_load_flux<T, NF>(a, ee[u] >= 0 ? ee[u] : ~ee[u], Af[u]);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) pj[u][f] = (T)a.psi1[f][jj[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const bool ok = m0 + u < L.deg;
+      const bool first = ee[u] >= 0;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        const T p = pj[u][f];
-        T A = anti_flux<T>(first ? pi[f] : p, first ? p : pi[f], gxi[f] + gjx[u][f], gyi[f] + gjy[u][f],
-                           gzi[f] + gjz[u][f], aun, volt, U12, V12, W12, a.dt);
+        const T p = pj[u][f], A = Af[u][f];
         T nmx = tmax<T>(tmax<T>(mx[f], pi[f]), p);
         T nmn = tmin<T>(tmin<T>(mn[f], pi[f]), p);
         T pp = tmax<T>((T)0, A), pn = -tmin<T>((T)0, A);
@@ -1733,8 +1804,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, EdgeS<T> S, M
 
 // limited antidiffusive fluxes and the second update (reference :3284-3302, :3098-3174)
 template <typename T, int NF, bool COL>
-__global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, EdgeS<T> S, MpArgs<T, NF> a,
-                                                        double *const *out) {
+__global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, MpArgs<T, NF> a, double *const *out) {
   constexpr int B = NFT_BATCH_MP;
   typedef typename MpArgs<T, NF>::T2 T2;
   Lane L;
@@ -1743,15 +1813,11 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, EdgeS<T> S,
   if (COL) col_setup(g, L.valid ? L.i : g.n - 1, C);
   if (!L.valid) return;
   const int i = L.i;
-  T Ui = a.U[i], Vi = a.V[i], Wi = a.W[i];
-  T vri = a.vr[i];
-  T arh2[NF], pi[NF], gxi[NF], gyi[NF], gzi[NF], cpi[NF], cni[NF];
+  T arh2[NF], cpi[NF], cni[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     arh2[f] = (T)0;
-    pi[f] = (T)a.psi1[f][i];
-    const T2 g2 = a.gxy[f][i], c2 = a.cpn[f][i];
-    gxi[f] = g2.x; gyi[f] = g2.y; gzi[f] = a.gz[f][i];
+    const T2 c2 = a.cpn[f][i];
     cpi[f] = c2.x; cni[f] = c2.y;
   }
   int2 rn[B];
@@ -1759,22 +1825,19 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, EdgeS<T> S,
   for (int u = 0; u < B; ++u) rn[u] = slot_raw<COL>(g, L, C, u);
   for (int m0 = 0; m0 < L.width; m0 += B) {
     int jj[B], ee[B];
-    T sx[B], sy[B], sz[B];
-    mp_batch_slots<T, COL, B>(g, S, C, rn, jj, ee, sx, sy, sz);
+#pragma unroll
+    for (int u = 0; u < B; ++u) slot_decode<COL>(C, rn[u], jj[u], ee[u]);
     if (m0 + B < L.width) {
 #pragma unroll
       for (int u = 0; u < B; ++u) rn[u] = slot_raw<COL>(g, L, C, m0 + B + u);
     }
-    T Uj[B], Vj[B], Wj[B], vrj[B], pj[B][NF], gjx[B][NF], gjy[B][NF], gjz[B][NF], cpj[B][NF], cnj[B][NF];
+    T Af[B][NF], cpj[B][NF], cnj[B][NF];
 #pragma unroll
     for (int u = 0; u < B; ++u) {
-      const int j = jj[u];
-      Uj[u] = a.U[j]; Vj[u] = a.V[j]; Wj[u] = a.W[j]; vrj[u] = a.vr[j];
+      mp_load_flux<T, NF>(a, ee[u] >= 0 ? ee[u] : ~ee[u], Af[u]);
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        pj[u][f] = (T)a.psi1[f][j];
-        const T2 gj = a.gxy[f][j], cj = a.cpn[f][j];
-        gjx[u][f] = gj.x; gjy[u][f] = gj.y; gjz[u][f] = a.gz[f][j];
+        const T2 cj = a.cpn[f][jj[u]];
         cpj[u][f] = cj.x; cnj[u][f] = cj.y;
       }
     }
@@ -1782,14 +1845,9 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, EdgeS<T> S,
     for (int u = 0; u < B; ++u) {
       const bool ok = m0 + u < L.deg;
       const bool first = ee[u] >= 0;
-      T aun = edge_aun<T>(Ui, Uj[u], Vi, Vj[u], Wi, Wj[u], sx[u], sy[u], sz[u]);
-      T volt = (T)0.5 * (vri + vrj[u]);
-      T U12 = Ui + Uj[u], V12 = Vi + Vj[u], W12 = Wi + Wj[u];
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        const T p = pj[u][f];
-        T A = anti_flux<T>(first ? pi[f] : p, first ? p : pi[f], gxi[f] + gjx[u][f], gyi[f] + gjy[u][f],
-                           gzi[f] + gjz[u][f], aun, volt, U12, V12, W12, a.dt);
+        const T A = Af[u][f];
         // edge roles: cp/cn of i2 and i1
         const T cp2 = first ? cpj[u][f] : cpi[f], cn1 = first ? cni[f] : cnj[u][f];
         const T cn2 = first ? cnj[u][f] : cni[f], cp1 = first ? cpi[f] : cpj[u][f];

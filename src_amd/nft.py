@@ -160,7 +160,7 @@ def _pf(a):
 
 
 KERNELS = {1: "laplc_grad", 2: "laplc_div", 3: "line_solve", 4: "gcr_update", 5: "gcr_dir",
-           6: "mp_donor", 7: "mp_grad", 8: "mp_fct", 9: "mp_limit"}
+           6: "mp_donor", 7: "mp_grad", 12: "mp_flux", 8: "mp_fct", 9: "mp_limit"}
 
 
 def lib():
