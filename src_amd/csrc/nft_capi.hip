@@ -155,6 +155,10 @@ struct nft_ctx {
   void *mp_g[12], *mp_c[8];
   void *mp_flux = nullptr;   // antidiffusive velocity per edge: two (field, field) pair arrays
   double *partials;
+  // second partials buffer: with fold, the direction kernel writes here while it reads the
+  // divergence's partials, and the next update reads them here while it writes its own
+  double *partials2 = nullptr;
+  int fold = 0;        // finalize folded into the consuming kernels (small grids, lord <= NFT_FOLD_MAXD)
   unsigned *counter;   // [0]: arrivals at the end of k_reduce_final (zero between launches)
   double *fin_wres;    // k_reduce_final's wavefront results: NFT_FIN_WG x (NFT_MAXD + 1)
   double *scal;
@@ -549,8 +553,12 @@ bool final_go(nft_ctx *c, int nblk, const FinalOp &op) {
     return false;
   } else {
     if (op.K == K) {
-      hipLaunchKernelGGL(k_reduce_final<K>, dim3(NFT_FIN_WG), dim3(64), 0, c->stream, c->partials, nblk, op,
-                         c->scal, c->fin_wres, c->counter);
+      // small grids: one wavefront replays the 16 wavefronts' sums (same result, no counter)
+      if (nblk <= NFT_FOLD_BLOCKS)
+        hipLaunchKernelGGL(k_reduce_small<K>, dim3(1), dim3(64), 0, c->stream, c->partials, nblk, op, c->scal);
+      else
+        hipLaunchKernelGGL(k_reduce_final<K>, dim3(NFT_FIN_WG), dim3(64), 0, c->stream, c->partials, nblk, op,
+                           c->scal, c->fin_wres, c->counter);
       return true;
     }
     return final_go<K + 1>(c, nblk, op);
@@ -589,7 +597,7 @@ int div_launch(nft_ctx *c, int mode, const DivArgs &a, bool f32 = false) {
     else return set_err(-1, "div: unsupported fused reduction");
   }
   HIPCHK(hipGetLastError());
-  if (nd + nmax > 0) {
+  if (nd + nmax > 0 && !a.dots.fold) {
     FinalOp op{};
     op.kind = FK_DIV; op.K = nd + nmax; op.ismax = nmax ? (1u << nd) : 0u; op.dots = a.dots; op.epa = a.epa;
     return final_launch(c, c->gather_grid, op);
@@ -1038,17 +1046,26 @@ int norms_dev(nft_ctx *c, const double *a, int sum_slot, int max_slot) {
 // its reductions, preconditioner, laplc of the preconditioned residual with the fused
 // <ax_ii, ar>, new direction with <r, ax>, <ax, ax>. `ctl` is the finalize of the update
 // (device exit test when ctl.ctl is set).
-static int gcr_iteration_update(nft_ctx *c, double *p, int i, const FinalOp &ctl) {
+// fold_slot >= 0: the previous direction's finalize is folded into this update (small grids)
+static int gcr_iteration_update(nft_ctx *c, double *p, int i, const FinalOp &ctl, int fold_slot) {
+  UpdFold fo{};
+  if (fold_slot >= 0) {
+    fo.part = c->partials2; fo.nblk = c->vec_grid; fo.slot = fold_slot; fo.epa = ctl.epa;
+  }
   t_begin(c, KT_UPD);
   hipLaunchKernelGGL(k_gcr_update, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, p, c->r, c->gx[i - 1],
-                     c->gax[i - 1], c->scal, c->partials, c->skip);
+                     c->gax[i - 1], c->scal, c->partials, c->skip, fo);
   t_end(c, KT_UPD);
   HIPCHK(hipGetLastError());
   return final_launch(c, c->vec_grid, ctl);
 }
 
+// With c->fold (grids of at most NFT_FOLD_BLOCKS blocks) the finalize of the divergence's
+// dots runs inside the direction kernel and the direction's inside the next update, so an
+// iteration launches two grid-level sums fewer; the sums and their order are unchanged.
 static int gcr_iteration_rest(nft_ctx *c, int i, int impfl, double epa) {
   const int lord = c->P.lord;
+  const bool fold = c->fold != 0;
   int rc;
   if ((rc = precon_dev(c, c->r, c->qr, impfl))) return rc;
   {
@@ -1056,6 +1073,7 @@ static int gcr_iteration_rest(nft_ctx *c, int i, int impfl, double epa) {
     dl.nd = i;
     for (int ii = 0; ii < i; ++ii) dl.a[ii] = c->gax[ii];
     dl.fin = FIN_DEL;
+    dl.fold = fold ? 1 : 0;
     if ((rc = krylov_laplc(c, c->qr, c->ar, impfl, &dl))) return rc;
   }
   DirArgs d{};
@@ -1064,11 +1082,14 @@ static int gcr_iteration_rest(nft_ctx *c, int i, int impfl, double epa) {
   d.xn = c->gx[slot]; d.axn = c->gax[slot]; d.slot = slot;
   for (int ii = 0; ii < i; ++ii) { d.x[ii] = c->gx[ii]; d.ax[ii] = c->gax[ii]; }
   d.qr = c->qr; d.ar = c->ar; d.r = c->r;
-  d.scal = c->scal; d.partials = c->partials; d.counter = c->counter; d.epa = epa; d.skip = c->skip;
+  d.scal = c->scal; d.partials = fold ? c->partials2 : c->partials; d.counter = c->counter; d.epa = epa;
+  d.skip = c->skip;
+  if (fold) { d.fold_part = c->partials; d.fold_nblk = c->gather_grid; }
   t_begin(c, KT_DIR);
   hipLaunchKernelGGL(k_gcr_dir, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, d);
   t_end(c, KT_DIR);
   HIPCHK(hipGetLastError());
+  if (fold) return 0;   // the next update folds this finalize in
   FinalOp op{};
   op.kind = FK_DIR; op.K = 2; op.ismax = 0u; op.slot = slot; op.epa = epa;
   return final_launch(c, c->vec_grid, op);
@@ -1094,14 +1115,16 @@ static int gcr_loop_device(nft_ctx *c, double *p, int impfl, double eps, int itm
   long ev_off = 0;   // events already folded into the totals
   int enq = 0;
   bool stop = false;
+  int prev_slot = -1;   // slot of the direction whose finalize the next update folds in
   for (int it = 1; it <= P.itr && !stop && !rc; ++it) {
     for (int i = 1; i <= lord && !stop; ++i) {
       ev_mark.push_back(ev_off + c->tev_used);
       FinalOp op{};
       op.kind = FK_UPDATE; op.K = 2; op.ismax = 1u;
       op.ctl = 1; op.it = it; op.itmn = itmn; op.eps = eps; op.epa = epa; op.hflag = c->d_hflag;
-      if ((rc = gcr_iteration_update(c, p, i, op))) break;
+      if ((rc = gcr_iteration_update(c, p, i, op, c->fold ? prev_slot : -1))) break;
       if ((rc = gcr_iteration_rest(c, i, impfl, epa))) break;
+      prev_slot = i < lord ? i : 0;
       if (hipEventRecord(ev[enq & 1], c->stream) != hipSuccess) { rc = set_err(-2, "hipEventRecord"); break; }
       ++enq;
       if (enq > 1) {
@@ -1147,7 +1170,7 @@ static int gcr_loop_host(nft_ctx *c, double *p, int impfl, double eps, int itmn,
     for (int i = 1; i <= lord; ++i) {
       t_begin(c, KT_UPD);
       hipLaunchKernelGGL(k_gcr_update, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, p, c->r,
-                         c->gx[i - 1], c->gax[i - 1], c->scal, c->partials, (const double *)nullptr);
+                         c->gx[i - 1], c->gax[i - 1], c->scal, c->partials, (const double *)nullptr, UpdFold{});
       t_end(c, KT_UPD);
       {
         FinalOp op{};
@@ -2250,7 +2273,10 @@ int init_impl(const nft_mesh *m, const nft_params *P, const nft_periodic *pp, nf
   }
   if ((rc = dalloc(c, &c->io[7], 8))) return fail(rc);  // device array of field pointers
   size_t npart = (size_t)std::max(c->gather_grid, c->vec_grid) * (NFT_MAXD + 1);
-  if ((rc = dalloc(c, &c->partials, npart)) || (rc = dalloc(c, &c->counter, 4)) || (rc = dalloc(c, &c->scal, SC_TOTAL)) ||
+  // small grids fold the GCR finalizes into the consuming kernels (same sums, same order)
+  c->fold = c->gather_grid <= NFT_FOLD_BLOCKS && c->vec_grid <= NFT_FOLD_BLOCKS && P->lord <= NFT_FOLD_MAXD;
+  if ((rc = dalloc(c, &c->partials, npart)) || (rc = dalloc(c, &c->partials2, npart)) ||
+      (rc = dalloc(c, &c->counter, 4)) || (rc = dalloc(c, &c->scal, SC_TOTAL)) ||
       (rc = dalloc(c, &c->fin_wres, (size_t)NFT_FIN_WG * (NFT_MAXD + 1))))
     return fail(rc);
   if (hipMemset(c->counter, 0, 4 * sizeof(unsigned)) != hipSuccess) return fail(set_err(-2, "memset"));

@@ -29,6 +29,15 @@
 #define NFT_BLOCK 256
 #define NFT_WPB (NFT_BLOCK / NFT_WAVE)
 #define NFT_MAXD 18
+// the grid-level finalize of the reductions: 1024 virtual threads in 16 wavefronts
+#define NFT_FIN_THREADS 1024
+#define NFT_FIN_WG (NFT_FIN_THREADS / 64)
+// GCR orders up to this one, on grids of at most NFT_FOLD_BLOCKS blocks, have their
+// finalize folded into the consuming kernel (at most 1,024: one partial per virtual thread)
+#define NFT_FOLD_MAXD 4
+#ifndef NFT_FOLD_BLOCKS
+#define NFT_FOLD_BLOCKS 256
+#endif
 
 namespace nft {
 
@@ -219,6 +228,70 @@ __device__ __forceinline__ void block_partial(double (&v)[K], unsigned ismax, do
 #pragma unroll
     for (int k = 0; k < K; ++k) partials[(size_t)blockIdx.x * K + k] = v[k];
   }
+}
+
+// The grid-level sum of k_reduce_final, evaluated inside the consuming kernel for grids of
+// at most 1,024 blocks, where every virtual thread t = 64*w + l of the finalize holds one
+// block's K partials: wavefront 0 of each block replays the 16 virtual wavefronts (the same
+// 64-lane trees over the same values, init where t >= nblk) and adds their results in order
+// w = 0..15, so the result equals k_reduce_final's bit for bit; every thread of the block
+// reads it from sh. Sums only (the GCR dots). K <= KMAX values per block.
+// virtual-wavefront replay of the finalize's sum over nblk <= NFT_FOLD_BLOCKS blocks, in
+// wavefront 0 (all 64 lanes); value k is a max where bit k of ismax is set; result in lane 0
+template <int KMAX>
+__device__ __forceinline__ void fold_wave(const double *partials, int nblk, int K, unsigned ismax,
+                                          double (&tot)[KMAX]) {
+  // virtual wavefronts that can hold blocks; their loads are all issued before the first tree
+  constexpr int NW = (NFT_FOLD_BLOCKS + 63) / 64;
+  static_assert(NW <= NFT_FIN_WG, "fold: at most one partial per virtual thread");
+  const int l = threadIdx.x & 63;
+  double pv[NW][KMAX];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const int t = 64 * w + l;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) pv[w][k] = k < K && t < nblk ? partials[(size_t)t * K + k] : 0.0;
+  }
+#pragma unroll
+  for (int w = 0; w < NFT_FIN_WG; ++w) {
+    double acc[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) acc[k] = ((ismax >> k) & 1) ? -1.0e300 : 0.0;
+    if (w < NW && 64 * w < nblk) {
+      // wave-uniform branch: all 64 lanes take part in the trees; lanes past nblk add
+      // nothing, as the finalize's threads past nblk
+      const int t = 64 * w + l;
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k) {
+        const double v = pv[w < NW ? w : 0][k];
+        const bool mx = (ismax >> k) & 1;
+        acc[k] = t < nblk ? (mx ? fmax(acc[k], v) : acc[k] + v) : acc[k];
+        acc[k] = mx ? wave_max(acc[k]) : wave_sum(acc[k]);
+      }
+    }
+    // a virtual wavefront without blocks reduces 64 initial values: +0.0 or -1e300
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k)
+      tot[k] = w == 0 ? acc[k] : (((ismax >> k) & 1) ? fmax(tot[k], acc[k]) : tot[k] + acc[k]);
+  }
+}
+
+// the same inside a consuming block: every thread reads the sums from sh (sums only)
+template <int KMAX>
+__device__ __forceinline__ void fold_reduce(const double *partials, int nblk, int K, double (&res)[KMAX],
+                                            double *sh) {
+  if (threadIdx.x < 64) {
+    double tot[KMAX];
+    fold_wave<KMAX>(partials, nblk, K, 0u, tot);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k) sh[k] = tot[k];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) res[k] = sh[k];
+  __syncthreads();
 }
 
 // ------------------------------------------------------------------------------------
@@ -416,6 +489,7 @@ struct DotList {
   int fin;                        // finalize mode (see gcr_finalize)
   int slot;                       // GCR slot for FIN_BETA
   int nmax;                       // nonzero: also reduce max|out| (entry nd)
+  int fold;                       // nonzero: the consuming kernel does the finalize
 };
 
 enum Finalize { FIN_NONE = 0, FIN_BETA = 1, FIN_DEL = 2, FIN_NORMS = 3 };
@@ -649,12 +723,34 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_laplT_2(Incidence g, EdgeS<double
 // Vector kernels of GCR (grid-stride, fixed grid -> deterministic partials)
 // ------------------------------------------------------------------------------------
 // p += beta*x; r += beta*ax; reduce max|r| and sum r^2   (reference :1886-1893)
+// With fold (grids of at most 1,024 blocks) the direction's finalize (reference :1878-1885)
+// is folded in: beta = -<r,ax>/max(epa, <ax,ax>) from the direction kernel's partials, and
+// block 0 stores the scalars k_reduce_final would have stored.
+struct UpdFold {
+  const double *part;   // the direction kernel's partials (null: beta from scal)
+  int nblk, slot;
+  double epa;
+};
+
 __global__ void __launch_bounds__(NFT_BLOCK) k_gcr_update(int n, double *p, double *r, const double *x,
                                                           const double *ax, double *scal,
-                                                          double *partials, const double *skip) {
+                                                          double *partials, const double *skip, UpdFold fo) {
   __shared__ double sh[2 * NFT_WPB];
   if (skipped(skip)) return;
-  const double beta = scal[SC_BETA];
+  double beta;
+  if (fo.part) {
+    double red[2];
+    fold_reduce<2>(fo.part, fo.nblk, 2, red, sh);
+    const double ax2 = fmax(fo.epa, red[1]);
+    beta = -red[0] / ax2;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      scal[SC_AX2 + fo.slot] = ax2;
+      scal[SC_RAX] = red[0];
+      scal[SC_BETA] = beta;
+    }
+  } else {
+    beta = scal[SC_BETA];
+  }
   double v[2] = {-1.e15, 0.};
   const int stride = gridDim.x * NFT_BLOCK;
   for (int k0 = blockIdx.x * NFT_BLOCK + threadIdx.x; k0 < n; k0 += 4 * stride) {
@@ -690,16 +786,28 @@ struct DirArgs {
   int slot;                  // slot index of xn
   double epa;
   const double *skip;
+  // folded finalize of the divergence's dots <ax_ii, ar> (reference :1906-1912): their
+  // partials over fold_nblk blocks (null: del_ii from scal)
+  const double *fold_part;
+  int fold_nblk;
 };
 
 // x_new = qr + sum del_ii x_ii ; ax_new = ar + sum del_ii ax_ii   (reference :1913-1936)
 // then <r, ax_new>, <ax_new, ax_new> -> beta for the next inner iteration
 __global__ void __launch_bounds__(NFT_BLOCK) k_gcr_dir(DirArgs a) {
-  __shared__ double sh[2 * NFT_WPB];
+  __shared__ double sh[(NFT_FOLD_MAXD > 2 * NFT_WPB ? NFT_FOLD_MAXD : 2 * NFT_WPB)];
   if (skipped(a.skip)) return;
   double del[NFT_MAXD];
+  if (a.fold_part) {
+    double red[NFT_FOLD_MAXD];
+    fold_reduce<NFT_FOLD_MAXD>(a.fold_part, a.fold_nblk, a.nx, red, sh);
 #pragma unroll
-  for (int ii = 0; ii < NFT_MAXD; ++ii) del[ii] = ii < a.nx ? a.scal[SC_DEL + ii] : 0.;
+    for (int ii = 0; ii < NFT_MAXD; ++ii)
+      del[ii] = ii < a.nx && ii < NFT_FOLD_MAXD ? -red[ii < NFT_FOLD_MAXD ? ii : 0] / a.scal[SC_AX2 + ii] : 0.;
+  } else {
+#pragma unroll
+    for (int ii = 0; ii < NFT_MAXD; ++ii) del[ii] = ii < a.nx ? a.scal[SC_DEL + ii] : 0.;
+  }
   double v[2] = {0., 0.};
   for (int k = blockIdx.x * NFT_BLOCK + threadIdx.x; k < a.n; k += gridDim.x * NFT_BLOCK) {
     double xv = a.qr[k], av = a.ar[k];
@@ -773,63 +881,9 @@ struct FinalOp {
   DiagSpec diag;   // FK_DIAG_MOM / FK_DIAG_VAR
 };
 
-// The grid-level sum runs as NFT_FIN_WG one-wavefront workgroups, so the partials are
-// fetched by 16 CUs at once (one CU alone is bound by the misses it can keep in flight).
-// Summation order: virtual thread t = 64*w + l (workgroup w, lane l) sums blocks t, t+1024,
-// ... in that order; wavefront tree; then the 16 wavefront results in order w = 0..15,
-// summed by the workgroup that arrives last at the counter.
-#define NFT_FIN_THREADS 1024
-#define NFT_FIN_WG (NFT_FIN_THREADS / 64)
+// the solver's scalar update from a grid-level result acc (k_reduce_final, k_reduce_small)
 template <int K>
-__global__ void __launch_bounds__(64) k_reduce_final(const double *partials, int nblk, FinalOp op, double *scal,
-                                                     double *wres, unsigned *arrived) {
-  constexpr int U = K <= 4 ? 16 : (64 / K > 0 ? 64 / K : 1);
-  if (skipped(op.skip)) return;
-  double acc[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) acc[k] = ((op.ismax >> k) & 1) ? -1.0e300 : 0.0;
-  // U blocks' K values are loaded before they are added
-  for (int b0 = blockIdx.x * 64 + threadIdx.x; b0 < nblk; b0 += U * NFT_FIN_THREADS) {
-    double pv[U][K];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int b = b0 + u * NFT_FIN_THREADS;
-#pragma unroll
-      for (int k = 0; k < K; ++k) pv[u][k] = b < nblk ? partials[(size_t)b * K + k] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (b0 + u * NFT_FIN_THREADS < nblk) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] = ((op.ismax >> k) & 1) ? fmax(acc[k], pv[u][k]) : acc[k] + pv[u][k];
-      }
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < K; ++k) acc[k] = ((op.ismax >> k) & 1) ? wave_max(acc[k]) : wave_sum(acc[k]);
-  if (threadIdx.x != 0) return;
-  // publish this wavefront's result, then count the arrival (release); the last arrival
-  // (acquire) reads all 16 results and resets the counter for the next launch
-#pragma unroll
-  for (int k = 0; k < K; ++k)
-    __hip_atomic_store(reinterpret_cast<unsigned long long *>(wres) + blockIdx.x * K + k,
-                       (unsigned long long)__double_as_longlong(acc[k]), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  const unsigned prev = __hip_atomic_fetch_add(arrived, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  if (prev != NFT_FIN_WG - 1) return;
-  const unsigned long long *wr = reinterpret_cast<const unsigned long long *>(wres);
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double a = __longlong_as_double(
-        (long long)__hip_atomic_load(wr + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    for (int j = 1; j < NFT_FIN_WG; ++j) {
-      const double x = __longlong_as_double(
-          (long long)__hip_atomic_load(wr + j * K + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      a = ((op.ismax >> k) & 1) ? fmax(a, x) : a + x;
-    }
-    acc[k] = a;
-  }
-  __hip_atomic_store(arrived, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void final_apply(const FinalOp &op, double (&acc)[K], double *scal) {
   if (op.kind == FK_DIAG_MOM) {
     // acc: per field (max, min, sum of w*x), then the sum of weights
     const DiagSpec &d = op.diag;
@@ -880,6 +934,75 @@ __global__ void __launch_bounds__(64) k_reduce_final(const double *partials, int
     scal[op.sum_slot] = acc[0];
     scal[op.max_slot] = pick(acc, 1);
   }
+}
+
+// The grid-level sum runs as NFT_FIN_WG one-wavefront workgroups, so the partials are
+// fetched by 16 CUs at once (one CU alone is bound by the misses it can keep in flight).
+// Summation order: virtual thread t = 64*w + l (workgroup w, lane l) sums blocks t, t+1024,
+// ... in that order; wavefront tree; then the 16 wavefront results in order w = 0..15,
+// summed by the workgroup that arrives last at the counter.
+template <int K>
+__global__ void __launch_bounds__(64) k_reduce_final(const double *partials, int nblk, FinalOp op, double *scal,
+                                                     double *wres, unsigned *arrived) {
+  constexpr int U = K <= 4 ? 16 : (64 / K > 0 ? 64 / K : 1);
+  if (skipped(op.skip)) return;
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = ((op.ismax >> k) & 1) ? -1.0e300 : 0.0;
+  // U blocks' K values are loaded before they are added
+  for (int b0 = blockIdx.x * 64 + threadIdx.x; b0 < nblk; b0 += U * NFT_FIN_THREADS) {
+    double pv[U][K];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u * NFT_FIN_THREADS;
+#pragma unroll
+      for (int k = 0; k < K; ++k) pv[u][k] = b < nblk ? partials[(size_t)b * K + k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (b0 + u * NFT_FIN_THREADS < nblk) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] = ((op.ismax >> k) & 1) ? fmax(acc[k], pv[u][k]) : acc[k] + pv[u][k];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = ((op.ismax >> k) & 1) ? wave_max(acc[k]) : wave_sum(acc[k]);
+  if (threadIdx.x != 0) return;
+  // publish this wavefront's result, then count the arrival (release); the last arrival
+  // (acquire) reads all 16 results and resets the counter for the next launch
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    __hip_atomic_store(reinterpret_cast<unsigned long long *>(wres) + blockIdx.x * K + k,
+                       (unsigned long long)__double_as_longlong(acc[k]), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned prev = __hip_atomic_fetch_add(arrived, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  if (prev != NFT_FIN_WG - 1) return;
+  const unsigned long long *wr = reinterpret_cast<const unsigned long long *>(wres);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double a = __longlong_as_double(
+        (long long)__hip_atomic_load(wr + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    for (int j = 1; j < NFT_FIN_WG; ++j) {
+      const double x = __longlong_as_double(
+          (long long)__hip_atomic_load(wr + j * K + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      a = ((op.ismax >> k) & 1) ? fmax(a, x) : a + x;
+    }
+    acc[k] = a;
+  }
+  __hip_atomic_store(arrived, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  final_apply<K>(op, acc, scal);
+}
+
+// The same grid-level sum for at most NFT_FOLD_BLOCKS blocks as one wavefront that replays
+// the 16 virtual wavefronts (fold_wave): no arrival counter, same result bit for bit.
+template <int K>
+__global__ void __launch_bounds__(64) k_reduce_small(const double *partials, int nblk, FinalOp op, double *scal) {
+  if (skipped(op.skip)) return;
+  double acc[K];
+  fold_wave<K>(partials, nblk, K, op.ismax, acc);
+  if (threadIdx.x != 0) return;
+  final_apply<K>(op, acc, scal);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1476,6 +1599,9 @@ __device__ __forceinline__ void mp_metric(const Incidence &g, const EdgeS<T> &S,
 #ifndef NFT_BATCH_MP
 #define NFT_BATCH_MP 2   // slots per gather batch of the MPDATA sweeps (1 and 2 measured: 2 best)
 #endif
+#ifndef NFT_BATCH_MPL
+#define NFT_BATCH_MPL 2  // the same for the bounds and limiter sweeps (stored fluxes, few registers)
+#endif
 
 // the per-slot gathers common to the sweeps: neighbour j, signed edge e, metric (sx, sy, sz)
 template <typename T, bool COL, int B>
@@ -1731,7 +1857,7 @@ __device__ __forceinline__ void mp_load_flux(const MpArgs<T, NF> &a, int e, T (&
 // FCT bounds from the donor-cell solution, in/out fluxes and ratios (reference :3228-3280)
 template <typename T, int NF, bool COL>
 __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, MpArgs<T, NF> a) {
-  constexpr int B = NFT_BATCH_MP;
+  constexpr int B = NFT_BATCH_MPL;
   typedef typename MpArgs<T, NF>::T2 T2;
   Lane L;
   if (!lane_setup(g, L)) return;
@@ -1805,7 +1931,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, MpArgs<T, NF>
 // limited antidiffusive fluxes and the second update (reference :3284-3302, :3098-3174)
 template <typename T, int NF, bool COL>
 __global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, MpArgs<T, NF> a, double *const *out) {
-  constexpr int B = NFT_BATCH_MP;
+  constexpr int B = NFT_BATCH_MPL;
   typedef typename MpArgs<T, NF>::T2 T2;
   Lane L;
   if (!lane_setup(g, L)) return;
