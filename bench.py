@@ -126,6 +126,9 @@ WORKLOADS = {
     "periodic": ("production mesh as a doubly periodic box (iper=2): 65,536 jittered columns x 64 levels "
                  "({nodes} nodes, {edges} edges), hill h0=1 km, U=10 m/s, N=0.01 1/s, dt=20 s; mpdatamper + "
                  "2 x GCR(4) to 1e-8 relative residual without preconditioner, FP64"),
+    "small": ("40,000-node mesh: 2,000 jittered columns x 20 levels over 20 km x 20 km x 10 km ({nodes} nodes, "
+              "{edges} edges), hill h0=500 m, U=10 m/s, N=0.01 1/s (Fr=2), dt=10 s; MPDATA + 2 x GCR(3) to 1e-8 "
+              "relative residual with the vertical line preconditioner (in-column band of the pressure operator), FP64"),
     "stress": ("stress mesh: 131,072 jittered columns x 128 levels stretched 50-400 m over 300 km x 300 km x 25 km "
                "({nodes} nodes, {edges} edges), hill h0=2 km, U=5 m/s, N=0.025 1/s (Fr=0.1), dt=15 s; MPDATA + "
                "2 x GCR(4) to 1e-8 relative residual with the vertical line preconditioner (in-column band of the "
@@ -352,9 +355,9 @@ def main():
     ap.add_argument("--long-run", type=int, default=0,
                     help="instead of the benchmark: N steps of the chosen mesh's full configuration, FP64 "
                          "and the FP32-operator mixed form side by side, deviation reported every 100 steps")
-    ap.add_argument("--mesh", default="production", choices=("production", "stress", "periodic"),
-                    help="workload: the production mesh (headline metric), the 16.8M-node stress mesh or the "
-                         "production mesh as a periodic box")
+    ap.add_argument("--mesh", default="production", choices=("production", "stress", "periodic", "small"),
+                    help="workload: the production mesh (headline metric), the 16.8M-node stress mesh, the "
+                         "production mesh as a periodic box or the 40,000-node mesh")
     ap.add_argument("--column-order", default="grid", choices=("grid", "hilbert"),
                     help="numbering of the production mesh's columns")
     ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed", "mixed_op"),
@@ -372,8 +375,8 @@ def main():
     from src_amd import mesh as M
     from src_amd import nft
 
-    cfg = {"production": production_config, "stress": stress_config, "periodic": periodic_config}[args.mesh](
-        args.column_order)
+    cfg = {"production": production_config, "stress": stress_config, "periodic": periodic_config,
+           "small": lambda order: small_config()}[args.mesh](args.column_order)
     t0 = time.perf_counter()
     mesh = M.generate(cfg.mesh)
     log(f"mesh: {mesh.npoin} nodes ({time.perf_counter() - t0:.1f} s)")

@@ -1053,8 +1053,12 @@ static int gcr_iteration_update(nft_ctx *c, double *p, int i, const FinalOp &ctl
     fo.part = c->partials2; fo.nblk = c->vec_grid; fo.slot = fold_slot; fo.epa = ctl.epa;
   }
   t_begin(c, KT_UPD);
-  hipLaunchKernelGGL(k_gcr_update, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, p, c->r, c->gx[i - 1],
-                     c->gax[i - 1], c->scal, c->partials, c->skip, fo);
+  if (fo.part)
+    hipLaunchKernelGGL(k_gcr_update<true>, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, p, c->r,
+                       c->gx[i - 1], c->gax[i - 1], c->scal, c->partials, c->skip, fo);
+  else
+    hipLaunchKernelGGL(k_gcr_update<false>, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, p, c->r,
+                       c->gx[i - 1], c->gax[i - 1], c->scal, c->partials, c->skip, fo);
   t_end(c, KT_UPD);
   HIPCHK(hipGetLastError());
   return final_launch(c, c->vec_grid, ctl);
@@ -1086,7 +1090,8 @@ static int gcr_iteration_rest(nft_ctx *c, int i, int impfl, double epa) {
   d.skip = c->skip;
   if (fold) { d.fold_part = c->partials; d.fold_nblk = c->gather_grid; }
   t_begin(c, KT_DIR);
-  hipLaunchKernelGGL(k_gcr_dir, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, d);
+  if (fold) hipLaunchKernelGGL(k_gcr_dir<true>, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, d);
+  else hipLaunchKernelGGL(k_gcr_dir<false>, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, d);
   t_end(c, KT_DIR);
   HIPCHK(hipGetLastError());
   if (fold) return 0;   // the next update folds this finalize in
@@ -1169,7 +1174,7 @@ static int gcr_loop_host(nft_ctx *c, double *p, int impfl, double eps, int itmn,
   for (int it = 1; it <= P.itr; ++it) {
     for (int i = 1; i <= lord; ++i) {
       t_begin(c, KT_UPD);
-      hipLaunchKernelGGL(k_gcr_update, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, p, c->r,
+      hipLaunchKernelGGL(k_gcr_update<false>, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, c->n, p, c->r,
                          c->gx[i - 1], c->gax[i - 1], c->scal, c->partials, (const double *)nullptr, UpdFold{});
       t_end(c, KT_UPD);
       {
@@ -1218,7 +1223,7 @@ static int gcr_loop_host(nft_ctx *c, double *p, int impfl, double eps, int itmn,
         d.qr = c->qr; d.ar = c->ar; d.r = c->r;
         d.scal = c->scal; d.partials = c->partials; d.counter = c->counter; d.epa = epa;
         t_begin(c, KT_DIR);
-        hipLaunchKernelGGL(k_gcr_dir, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, d);
+        hipLaunchKernelGGL(k_gcr_dir<false>, dim3(c->vec_grid), dim3(NFT_BLOCK), 0, c->stream, d);
         t_end(c, KT_DIR);
         HIPCHK(hipGetLastError());
         FinalOp op{};

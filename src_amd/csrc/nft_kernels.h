@@ -732,13 +732,14 @@ struct UpdFold {
   double epa;
 };
 
+template <bool FOLD>
 __global__ void __launch_bounds__(NFT_BLOCK) k_gcr_update(int n, double *p, double *r, const double *x,
                                                           const double *ax, double *scal,
                                                           double *partials, const double *skip, UpdFold fo) {
   __shared__ double sh[2 * NFT_WPB];
   if (skipped(skip)) return;
   double beta;
-  if (fo.part) {
+  if constexpr (FOLD) {
     double red[2];
     fold_reduce<2>(fo.part, fo.nblk, 2, red, sh);
     const double ax2 = fmax(fo.epa, red[1]);
@@ -794,11 +795,12 @@ struct DirArgs {
 
 // x_new = qr + sum del_ii x_ii ; ax_new = ar + sum del_ii ax_ii   (reference :1913-1936)
 // then <r, ax_new>, <ax_new, ax_new> -> beta for the next inner iteration
+template <bool FOLD>
 __global__ void __launch_bounds__(NFT_BLOCK) k_gcr_dir(DirArgs a) {
   __shared__ double sh[(NFT_FOLD_MAXD > 2 * NFT_WPB ? NFT_FOLD_MAXD : 2 * NFT_WPB)];
   if (skipped(a.skip)) return;
   double del[NFT_MAXD];
-  if (a.fold_part) {
+  if constexpr (FOLD) {
     double red[NFT_FOLD_MAXD];
     fold_reduce<NFT_FOLD_MAXD>(a.fold_part, a.fold_nblk, a.nx, red, sh);
 #pragma unroll
