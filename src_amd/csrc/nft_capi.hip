@@ -146,8 +146,7 @@ struct nft_ctx {
   // work
   double *t[8];          // prfr/div/laplc temporaries
   double2 *txy;          // laplc: (x, y) of the scaled gradient (z in t[2])
-  float2 *txyf = nullptr;   // the FP32 operator's (x, y) and z (precision 2)
-  float *t2f = nullptr;
+  F3 *t3f = nullptr;        // the FP32 operator's scaled gradient, packed (precision 2)
   double op32_drop = 1.0e-3;   // recursive-residual drop that ends a pass (precision 2)
   double *gx[NFT_MAXD], *gax[NFT_MAXD];
   double *r, *qr, *ar, *pw, *rw;
@@ -620,9 +619,9 @@ int laplc_dev(nft_ctx *c, const double *press, double *alap, int impfl, const Do
 // laplc with FP32 edge metrics and FP32 gradient temporaries, FP64 sums and output: the
 // operator of the Krylov iterations with precision 2
 int laplc32_dev(nft_ctx *c, const double *press, double *alap, int impfl, const DotList *dots) {
-  if (!c->txyf || !c->inc.sxyf) return set_err(-1, "laplc32: FP32 operator not set up (precision 2 only)");
+  if (!c->t3f || !c->inc.s3f) return set_err(-1, "laplc32: FP32 operator not set up (precision 2 only)");
   GradArgs g = grad_args(c);
-  g.f = press; g.oxyf = c->txyf; g.o2f = c->t2f; g.impfl = impfl;
+  g.f = press; g.o3f = c->t3f; g.impfl = impfl;
   dim3 gg(c->gather_grid), b(NFT_BLOCK);
   t_begin(c, KT_LGRAD);
   if (c->col_on)
@@ -632,7 +631,7 @@ int laplc32_dev(nft_ctx *c, const double *press, double *alap, int impfl, const 
   t_end(c, KT_LGRAD);
   HIPCHK(hipGetLastError());
   DivArgs d = div_args(c);
-  d.qxyf = c->txyf; d.q2f = c->t2f; d.out = alap;
+  d.q3f = c->t3f; d.out = alap;
   if (dots) d.dots = *dots;
   return div_launch(c, DIV_LAPLC, d, true);
 }
@@ -2217,15 +2216,12 @@ int init_impl(const nft_mesh *m, const nft_params *P, const nft_periodic *pp, nf
     for (int k = 0; k < 3; ++k)
       if ((rc = dalloc(c, &c->uuuf[k], n))) return fail(rc);
     if (P->precision == 2) {
-      // the FP32 operator: (Sx, Sy) pairs rounded to FP32; Sz is the FP32 copy above
-      std::vector<float2> sxyf(etot);
-      for (long e = 0; e < etot; ++e) sxyf[e] = make_float2((float)sx[e], (float)sy[e]);
-      float2 *d_sxyf;
-      if ((rc = upload(c, &d_sxyf, sxyf.data(), etot)) || (rc = dalloc(c, &c->txyf, n)) ||
-          (rc = dalloc(c, &c->t2f, n)))
-        return fail(rc);
-      c->inc.sxyf = d_sxyf;
-      c->inc.szzf = fz;
+      // the FP32 operator: (Sx, Sy, Sz) rounded to FP32 and packed per edge
+      std::vector<F3> s3f(etot);
+      for (long e = 0; e < etot; ++e) s3f[e] = F3{(float)sx[e], (float)sy[e], (float)sz[e]};
+      F3 *d_s3f;
+      if ((rc = upload(c, &d_s3f, s3f.data(), etot)) || (rc = dalloc(c, &c->t3f, n))) return fail(rc);
+      c->inc.s3f = d_s3f;
     }
   }
 

@@ -41,6 +41,11 @@
 
 namespace nft {
 
+// three packed FP32 values (12 bytes, one dwordx3 load)
+struct __align__(4) F3 {
+  float x, y, z;
+};
+
 struct Incidence {
   int n, nchunk;
   const int32_t *chunk_ptr;  // [nchunk+1]
@@ -70,10 +75,9 @@ struct Incidence {
   // two aligned loads per slot instead of three
   const double2 *sxy;
   const double *szz;
-  // FP32 roundings of the same metrics for the FP32 operator of the mixed form (null
-  // otherwise)
-  const float2 *sxyf;
-  const float *szzf;
+  // FP32 roundings of the same metrics for the FP32 operator of the mixed form, packed as
+  // (Sx, Sy, Sz) so a slot reads them with one 12-byte load (null otherwise)
+  const F3 *s3f;
   // Doubly periodic box (the reference's iper = 2): per = 1 drops the boundary-point terms
   // of the gathers and keeps only bcv's flag-11 slip. A node with cslot[i] >= 0 belongs to
   // a coincident set; its gathered sums are replaced by cval[cslot[i]*K + k], the sums over
@@ -87,8 +91,8 @@ struct Incidence {
 template <typename TM>
 __device__ __forceinline__ void load_metric(const Incidence &g, int e, double &sx, double &sy, double &sz) {
   if constexpr (sizeof(TM) == 4) {
-    const float2 s = g.sxyf[e];
-    sx = s.x; sy = s.y; sz = g.szzf[e];
+    const F3 s = g.s3f[e];
+    sx = s.x; sy = s.y; sz = s.z;
   } else {
     const double2 s = g.sxy[e];
     sx = s.x; sy = s.y; sz = g.szz[e];
@@ -365,8 +369,7 @@ struct GradArgs {
   const double *f;
   double *o0, *o1, *o2;
   double2 *oxy;                // GRAD_LAPLC: (x, y) components; z in o2
-  float2 *oxyf;                // GRAD_LAPLC with FP32 metrics: the same rounded to FP32
-  float *o2f;
+  F3 *o3f;                     // GRAD_LAPLC with FP32 metrics: the same rounded to FP32, packed
   const double *s0, *s1, *s2;  // ustr, vstr, wstr (PRFR)
   NodeCoef nc;
   double dt;
@@ -465,8 +468,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_grad(Incidence g, EdgeS<double> S
       gx = gx * r; gy = gy * r; gz = gz * r;
     }
     if constexpr (sizeof(TM) == 4) {
-      a.oxyf[i] = make_float2((float)gx, (float)gy);
-      a.o2f[i] = (float)gz;
+      a.o3f[i] = F3{(float)gx, (float)gy, (float)gz};
     } else {
       a.oxy[i] = make_double2(gx, gy);
       a.o2[i] = gz;
@@ -509,8 +511,7 @@ enum Scal {
 struct DivArgs {
   const double *q0, *q1, *q2;
   const double2 *qxy;             // DIV_LAPLC: (x, y) components; z in q2
-  const float2 *qxyf;             // DIV_LAPLC with FP32 metrics: FP32 inputs
-  const float *q2f;
+  const F3 *q3f;                  // DIV_LAPLC with FP32 metrics: FP32 inputs, packed
   double *out;
   NodeCoef nc;
   int iflg;
@@ -572,7 +573,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
     double ax = 0., ay = 0., az = 0.;
     double qi0 = 0., qi1 = 0., qi2 = 0.;
     if (L.valid) {
-      if (F32) { const float2 v = a.qxyf[i]; qi0 = v.x; qi1 = v.y; qi2 = a.q2f[i]; }
+      if (F32) { const F3 v = a.q3f[i]; qi0 = v.x; qi1 = v.y; qi2 = v.z; }
       else {
         if (MODE == DIV_LAPLC) { const double2 v = a.qxy[i]; qi0 = v.x; qi1 = v.y; }
         else { qi0 = a.q0[i]; qi1 = a.q1[i]; }
@@ -596,7 +597,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
       for (int u = 0; u < BD; ++u) {
         int e = ee[u] >= 0 ? ee[u] : ~ee[u];
         int j = jj[u];
-        if (F32) { const float2 v = a.qxyf[j]; q0[u] = v.x; q1[u] = v.y; q2[u] = a.q2f[j]; }
+        if (F32) { const F3 v = a.q3f[j]; q0[u] = v.x; q1[u] = v.y; q2[u] = v.z; }
         else {
           if (MODE == DIV_LAPLC) { const double2 v = a.qxy[j]; q0[u] = v.x; q1[u] = v.y; }
           else { q0[u] = a.q0[j]; q1[u] = a.q1[j]; }
