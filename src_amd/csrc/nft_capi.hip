@@ -158,6 +158,7 @@ struct nft_ctx {
   // divergence's partials, and the next update reads them here while it writes its own
   double *partials2 = nullptr;
   int fold = 0;        // finalize folded into the consuming kernels (small grids, lord <= NFT_FOLD_MAXD)
+  ExitFold xfold{};    // set around a device-loop iteration whose exit test the band solve takes
   unsigned *counter;   // [0]: arrivals at the end of k_reduce_final (zero between launches)
   double *fin_wres;    // k_reduce_final's wavefront results: NFT_FIN_WG x (NFT_MAXD + 1)
   double *scal;
@@ -709,14 +710,24 @@ int fine_sweep(nft_ctx *c, const double *rhs, double *p, int impfl, bool first) 
     const double *pin = first ? nullptr : p;
     if (P.precon == 3) {
       const int nb = (c->ncol + c->band_cpb - 1) / c->band_cpb;
+      // the exit test of the update before it, folded in on small grids (first sweep only)
+      const ExitFold xf = first ? c->xfold : ExitFold{};
       if (P.precision >= 1) {
         const size_t shm = (size_t)c->band_cpb * (c->band_L + 1) * sizeof(float);
-        hipLaunchKernelGGL(k_line_penta<float>, dim3(nb), dim3(NFT_WAVE), shm, c->stream, c->ncol, c->band_L,
-                           c->band_cpb, c->bandf, res, pin, p, P.omega, c->skip);
+        if (xf.part)
+          hipLaunchKernelGGL((k_line_penta<float, true>), dim3(nb), dim3(NFT_WAVE), shm, c->stream, c->ncol,
+                             c->band_L, c->band_cpb, c->bandf, res, pin, p, P.omega, c->skip, xf);
+        else
+          hipLaunchKernelGGL((k_line_penta<float, false>), dim3(nb), dim3(NFT_WAVE), shm, c->stream, c->ncol,
+                             c->band_L, c->band_cpb, c->bandf, res, pin, p, P.omega, c->skip, xf);
       } else {
         const size_t shm = (size_t)c->band_cpb * (c->band_L + 1) * sizeof(double);
-        hipLaunchKernelGGL(k_line_penta<double>, dim3(nb), dim3(NFT_WAVE), shm, c->stream, c->ncol, c->band_L,
-                           c->band_cpb, c->band, res, pin, p, P.omega, c->skip);
+        if (xf.part)
+          hipLaunchKernelGGL((k_line_penta<double, true>), dim3(nb), dim3(NFT_WAVE), shm, c->stream, c->ncol,
+                             c->band_L, c->band_cpb, c->band, res, pin, p, P.omega, c->skip, xf);
+        else
+          hipLaunchKernelGGL((k_line_penta<double, false>), dim3(nb), dim3(NFT_WAVE), shm, c->stream, c->ncol,
+                             c->band_L, c->band_cpb, c->band, res, pin, p, P.omega, c->skip, xf);
       }
     } else if (c->line_L > 0) {
       int cpw = 64 / c->line_L;
@@ -1045,8 +1056,10 @@ int norms_dev(nft_ctx *c, const double *a, int sum_slot, int max_slot) {
 // its reductions, preconditioner, laplc of the preconditioned residual with the fused
 // <ax_ii, ar>, new direction with <r, ax>, <ax, ax>. `ctl` is the finalize of the update
 // (device exit test when ctl.ctl is set).
-// fold_slot >= 0: the previous direction's finalize is folded into this update (small grids)
-static int gcr_iteration_update(nft_ctx *c, double *p, int i, const FinalOp &ctl, int fold_slot) {
+// fold_slot >= 0: the previous direction's finalize is folded into this update (small grids);
+// exit_fold: the exit test is left to the band solve that follows (c->xfold)
+static int gcr_iteration_update(nft_ctx *c, double *p, int i, const FinalOp &ctl, int fold_slot,
+                                bool exit_fold = false) {
   UpdFold fo{};
   if (fold_slot >= 0) {
     fo.part = c->partials2; fo.nblk = c->vec_grid; fo.slot = fold_slot; fo.epa = ctl.epa;
@@ -1060,6 +1073,7 @@ static int gcr_iteration_update(nft_ctx *c, double *p, int i, const FinalOp &ctl
                        c->gx[i - 1], c->gax[i - 1], c->scal, c->partials, c->skip, fo);
   t_end(c, KT_UPD);
   HIPCHK(hipGetLastError());
+  if (exit_fold) return 0;
   return final_launch(c, c->vec_grid, ctl);
 }
 
@@ -1120,14 +1134,22 @@ static int gcr_loop_device(nft_ctx *c, double *p, int impfl, double eps, int itm
   int enq = 0;
   bool stop = false;
   int prev_slot = -1;   // slot of the direction whose finalize the next update folds in
+  // small grids with the band line (one sweep, no multigrid): the exit test runs inside the
+  // band solve of the same iteration; the previous sum r^2 alternates between two slots
+  const bool xfold = c->fold && P.precon == 3 && !c->mg_on;
   for (int it = 1; it <= P.itr && !stop && !rc; ++it) {
     for (int i = 1; i <= lord && !stop; ++i) {
       ev_mark.push_back(ev_off + c->tev_used);
       FinalOp op{};
       op.kind = FK_UPDATE; op.K = 2; op.ismax = 1u;
       op.ctl = 1; op.it = it; op.itmn = itmn; op.eps = eps; op.epa = epa; op.hflag = c->d_hflag;
-      if ((rc = gcr_iteration_update(c, p, i, op, c->fold ? prev_slot : -1))) break;
-      if ((rc = gcr_iteration_rest(c, i, impfl, epa))) break;
+      if ((rc = gcr_iteration_update(c, p, i, op, c->fold ? prev_slot : -1, xfold))) break;
+      if (xfold) {
+        c->xfold = ExitFold{c->partials, c->vec_grid, it, itmn, enq & 1, eps, epa, c->scal, c->d_hflag};
+      }
+      rc = gcr_iteration_rest(c, i, impfl, epa);
+      c->xfold = ExitFold{};
+      if (rc) break;
       prev_slot = i < lord ? i : 0;
       if (hipEventRecord(ev[enq & 1], c->stream) != hipSuccess) { rc = set_err(-2, "hipEventRecord"); break; }
       ++enq;

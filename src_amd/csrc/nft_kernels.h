@@ -502,7 +502,7 @@ enum Scal {
   SC_R2 = 7, SC_RMX = 8, SC_TMP = 9,
   SC_DONE = 10,  // GCR exit decided on the device: 0 running, 1 converged, 2 stagnated
   SC_NLC = 11,   // inner iterations that passed the exit test
-  SC_RL20 = 12,  // previous sum r^2 (stagnation test)
+  SC_RL20 = 12,  // previous sum r^2 (stagnation test); 12 and 13 alternate when the exit test is folded
   SC_AX2 = 16,   // [16, 16+NFT_MAXD)
   SC_DEL = 40,   // [40, 40+NFT_MAXD)
   SC_TOTAL = 64
@@ -1335,13 +1335,47 @@ struct BandFactors {
   const T *l1, *l2, *u1, *u2, *dinv;
 };
 
+// The exit test of GCR's inner loop (reference :1893-1897) folded into the band solve that
+// follows the residual update, for small grids: every block replays the finalize's sum of
+// the update's partials (max|r|, sum r^2; fold_wave) and takes the same decision as
+// final_apply's FK_UPDATE; block 0 stores the scalars, the exit flag and its host copy. The
+// previous sum r^2 alternates between two slots by iteration parity (read par, written
+// par ^ 1), so no block reads a value another block of the same launch has updated.
+struct ExitFold {
+  const double *part;   // the update's partials (null: no fold)
+  int nblk, it, itmn, par;
+  double eps, epa;
+  double *scal, *hflag;
+};
+
 // T = double (FP64 path, same operation sequence as the oracle) or float (mixed precision:
 // factors, staged residual and substitution in FP32, the update of p in FP64)
-template <typename T>
+template <typename T, bool XF = false>
 __global__ void __launch_bounds__(NFT_WAVE) k_line_penta(int ncol, int L, int cpb, BandFactors<T> f,
                                                          const double *res, const double *pin, double *p,
-                                                         double omega, const double *skip) {
+                                                         double omega, const double *skip, ExitFold xf) {
   if (skipped(skip)) return;
+  if constexpr (XF) {
+    double acc[2];
+    fold_wave<2>(xf.part, xf.nblk, 2, 1u, acc);   // the block is one wavefront
+    const double dvmx = __shfl(acc[0], 0, 64), rl2 = __shfl(acc[1], 0, 64);
+    double done = 0.0;
+    if (!(rl2 == rl2)) done = 2.0;   // NaN residual: stop, reported as not converged
+    else if (dvmx <= xf.eps && xf.it >= xf.itmn) done = 1.0;
+    else if (rl2 >= xf.scal[SC_RL20 + xf.par]) done = 2.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      xf.scal[SC_DVMX] = dvmx;
+      xf.scal[SC_RL2] = rl2;
+      if (done != 0.0) {
+        xf.scal[SC_DONE] = done;
+        if (xf.hflag) *xf.hflag = done;
+      } else {
+        xf.scal[SC_RL20 + (xf.par ^ 1)] = fmax(rl2, xf.epa);
+        xf.scal[SC_NLC] = xf.scal[SC_NLC] + 1.0;
+      }
+    }
+    if (done != 0.0) return;
+  }
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *lds = (T *)lds_raw;
   const int lane = threadIdx.x, cb = blockIdx.x;
