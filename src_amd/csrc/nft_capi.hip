@@ -1633,8 +1633,10 @@ int build_band(nft_ctx *c, const nft_mesh *m) {
   const int L = n / nc;
   for (int k = 0; k <= nc; ++k)
     if (m->col_start[k] != k * L) return set_err(-1, "nft_init: precon 3 needs columns of one length");
-  const int cpb = L <= 96 ? 64 : L <= 200 ? 32 : L <= 400 ? 16 : 0;
+  int cpb = L <= 96 ? 64 : L <= 200 ? 32 : L <= 400 ? 16 : 0;
   if (!cpb) return set_err(-1, "nft_init: precon 3 supports columns of at most 400 levels");
+  // few columns: fewer per block, so that the one-wavefront blocks spread over the CUs
+  while (cpb > 1 && (nc + cpb - 1) / cpb < NFT_PENTA_MINBLK) cpb /= 2;
   // column adjacency (interior edges, then boundary-face edges)
   std::vector<int32_t> deg(nc + 1, 0);
   for (int pass = 0; pass < 2; ++pass) {

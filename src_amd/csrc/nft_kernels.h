@@ -1326,6 +1326,13 @@ __global__ void k_band_extract(int n, int L, const int32_t *color, int col, int 
 #define NFT_PENTA_CHUNK 16
 #endif
 
+#ifndef NFT_PENTA_MINBLK
+// band solve: columns per block halved until the grid has at least this many one-wavefront
+// blocks (2,000 columns: 4 per block, 500 blocks; measured 15.0 -> 13.2 us against 64 per
+// block, while 2 and 1 per block were slower)
+#define NFT_PENTA_MINBLK 256
+#endif
+
 #ifndef NFT_PENTA_STAGE
 #define NFT_PENTA_STAGE 16  // loads in flight per lane while staging the band solve's columns (8 and 16 measured)
 #endif
