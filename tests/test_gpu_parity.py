@@ -12,6 +12,7 @@ import pytest
 
 from common import diag_close as _diag_close, mesh_checksum, mesh_for, rel_l2, seeded_fields, small_cfg, tiny_cfg
 from oracle import Oracle
+from src_amd import mesh as M
 from src_amd import nft
 
 pytestmark = pytest.mark.gpu
@@ -557,6 +558,32 @@ def test_band_preconditioner_fp32_bitwise(key):
     print(f"FP32 band solve vs FP64: rel L2 {err:.2e}")
     assert err < 1e-4
     s.close()
+
+
+@pytest.mark.parametrize("nx,cpb", [(128, 64), (96, 32), (64, 16), (48, 8), (24, 2)])
+def test_band_solve_columns_per_block(nx, cpb):
+    """The band solve runs one wavefront per block with one column per lane; on meshes with
+    few columns it takes fewer columns per block (halved from 64 until the grid has at least
+    256 blocks: nx*nx columns give cpb per block here), and its factors are laid out for that
+    width. The band, the FP64 solve and the FP32 solve equal the oracle's at every width."""
+    print(f"case {nx}x{nx} columns, {cpb} per block", flush=True)
+    spec = M.MeshSpec(nx=nx, ny=nx, nlev=6, lx=500.0 * nx, ly=500.0 * nx, htop=1500.0, hill_h0=200.0,
+                      hill_a=2000.0, seed=nx)
+    m = M.generate(spec)
+    want = 64
+    while want > 1 and -(-m.nh // want) < 256:
+        want //= 2
+    assert m.nh == nx * nx and want == cpb   # the case table above
+    cfg = tiny_cfg(**BAND)
+    s, o = solver(m, cfg), Oracle(m, cfg, dot_order=1)
+    for name in ("band_m2", "band_m1", "band_0", "band_p1", "band_p2"):
+        assert np.array_equal(s.field(name), o.field(name)), name
+    r = seeded_fields(m)["r"]
+    assert np.array_equal(s.precon(r, 1), o.precon(r, 1))
+    s.close()
+    s32 = solver(m, cfg, precision="mixed")
+    assert np.array_equal(s32.precon(r, 1), o.line_solve32(r))
+    s32.close()
 
 
 def test_band_preconditioner_steps_bitwise():
