@@ -99,6 +99,12 @@ struct nft_ctx {
   // same chunks and launch shape as inc
   Incidence inc_own{};
   double own_col_bytes = 0;
+  // column patches of the LDS-staged MPDATA bounds and limiter sweeps (columns of 64 levels,
+  // closed box): the incidence with LDS-relative neighbours, the patch tables, and the
+  // largest staged set (columns); patch_on = 0 keeps the sweeps on memory gathers
+  Incidence inc_lds{};
+  MpPatches patches{};
+  int patch_smax = 0, patch_on = 0;
   int gather_grid = 0;   // launched blocks of gather kernels
   int vec_grid = 0;      // fixed grid of grid-stride vector kernels
   int nodes_grid = 0;    // one thread per node
@@ -1364,6 +1370,13 @@ int gcrk_dev(nft_ctx *c, double *p, const double *u1, const double *v1, const do
   return converged ? 0 : 1;
 }
 
+// LDS bytes of the patch sweeps: the staged columns' donor-cell values (bounds) or (cp, cn)
+// pairs (limiter)
+template <typename T, int NF>
+size_t patch_lds_bytes(const nft_ctx *c, bool limit) {
+  return (size_t)c->patch_smax * NFT_WAVE * NF * sizeof(T) * (limit ? 2 : 1);
+}
+
 template <typename T, int NF>
 int mpdata_dev(nft_ctx *c, const T *U, const T *V, const T *W, double *const *fields, const int *iflags) {
   MpArgs<T, NF> a{};
@@ -1417,8 +1430,15 @@ int mpdata_dev(nft_ctx *c, const T *U, const T *V, const T *W, double *const *fi
   if (c->col_on && go.col_L > 0) hipLaunchKernelGGL((k_mp_flux<T, NF, true>), g, b, 0, c->stream, go, S, a);
   else hipLaunchKernelGGL((k_mp_flux<T, NF, false>), g, b, 0, c->stream, go, S, a);
   t_end(c, KT_MPF);
+  // columns of 64 levels in a closed box: bounds and limiter over LDS-staged column patches
+  const bool lds = c->patch_on && c->col_on && !per;
+  const dim3 gp(c->patches.xcd_per > 0 ? 8 * c->patches.xcd_per : c->patches.npatch), bp(NFT_WAVE * NFT_PATCH_COLS);
+  const size_t shf = patch_lds_bytes<T, NF>(c, false), shl = patch_lds_bytes<T, NF>(c, true);
   t_begin(c, KT_MP0 + 2);
-  if (c->col_on) hipLaunchKernelGGL((k_mp_fct<T, NF, true>), g, b, 0, c->stream, c->inc, a);
+  if (lds)
+    hipLaunchKernelGGL((k_mp_fct_lds<T, NF>), gp, bp, shf, c->stream, c->inc_lds,
+                       c->patches, a);
+  else if (c->col_on) hipLaunchKernelGGL((k_mp_fct<T, NF, true>), g, b, 0, c->stream, c->inc, a);
   else hipLaunchKernelGGL((k_mp_fct<T, NF, false>), g, b, 0, c->stream, c->inc, a);
   t_end(c, KT_MP0 + 2);
   double *const *out = (double *const *)c->io[7];
@@ -1426,7 +1446,10 @@ int mpdata_dev(nft_ctx *c, const T *U, const T *V, const T *W, double *const *fi
     if (per) hipLaunchKernelGGL(k_per_mp_limit<NF>, gs, b, 0, c->stream, c->inc, c->ps, a);
   }
   t_begin(c, KT_MP0 + 3);
-  if (c->col_on) hipLaunchKernelGGL((k_mp_limit<T, NF, true>), g, b, 0, c->stream, c->inc, a, out);
+  if (lds)
+    hipLaunchKernelGGL((k_mp_limit_lds<T, NF>), gp, bp, shl, c->stream, c->inc_lds,
+                       c->patches, a, out);
+  else if (c->col_on) hipLaunchKernelGGL((k_mp_limit<T, NF, true>), g, b, 0, c->stream, c->inc, a, out);
   else hipLaunchKernelGGL((k_mp_limit<T, NF, false>), g, b, 0, c->stream, c->inc, a, out);
   t_end(c, KT_MP0 + 3);
   HIPCHK(hipGetLastError());
@@ -1955,6 +1978,87 @@ int upload_slots(nft_ctx *c, int n, const HostSlots &H, Incidence &g) {
   return 0;
 }
 
+// Column patches for the LDS-staged MPDATA bounds and limiter sweeps (columns of L =
+// NFT_WAVE levels, columnar slots H): each patch grows from the lowest unassigned column,
+// adding the unassigned neighbour with most links into the patch (then the lowest id) while
+// the staged set (its columns and every column their slots reach) stays within
+// NFT_PATCH_STAGE, up to NFT_PATCH_COLS columns. Staged lists hold the patch's columns
+// first, then the others in increasing order; tabl is H.tab with each neighbour rewritten
+// as (staged index * L + level), relative to the patch of the column that owns the entry.
+// Returns the largest staged set, or 0 when a column alone exceeds the limit.
+int build_patches(int ncol, int L, const HostSlots &H, std::vector<int32_t> &ptr, std::vector<int32_t> &col,
+                  std::vector<uint8_t> &nown, std::vector<int2> &tabl) {
+  std::vector<std::vector<int32_t>> nb(ncol);
+  for (int c = 0; c < ncol; ++c) {
+    std::vector<int32_t> &v = nb[c];
+    for (int t = H.tptr[c]; t < H.tptr[c + 1]; ++t) v.push_back((H.tab[t].x + 1) / L);
+    for (int t = H.xptr[c]; t < H.xptr[c + 1]; ++t) v.push_back(H.tab[H.xoff + t].x / L);
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    v.erase(std::remove(v.begin(), v.end(), c), v.end());
+  }
+  std::vector<int32_t> owner(ncol, -1), mark(ncol, -1), sidx(ncol, -1);
+  ptr.assign(1, 0);
+  col.clear();
+  nown.clear();
+  int smax = 0;
+  for (int seed = 0; seed < ncol; ++seed) {
+    if (owner[seed] >= 0) continue;
+    const int p = (int)nown.size();
+    std::vector<int32_t> own{seed}, S{seed};
+    mark[seed] = p;
+    for (int z : nb[seed])
+      if (mark[z] != p) { mark[z] = p; S.push_back(z); }
+    if ((int)S.size() > NFT_PATCH_STAGE) return 0;
+    owner[seed] = p;
+    while ((int)own.size() < NFT_PATCH_COLS) {
+      int best = -1, bl = -1;
+      for (int y : S) {
+        if (owner[y] >= 0) continue;
+        int inc = 0, links = 0;
+        for (int z : nb[y]) {
+          inc += mark[z] != p;
+          links += owner[z] == p;
+        }
+        if ((int)S.size() + inc > NFT_PATCH_STAGE) continue;
+        if (links > bl || (links == bl && y < best)) { best = y; bl = links; }
+      }
+      if (best < 0) break;
+      owner[best] = p;
+      own.push_back(best);
+      for (int z : nb[best])
+        if (mark[z] != p) { mark[z] = p; S.push_back(z); }
+    }
+    std::vector<int32_t> rest;
+    for (int y : S)
+      if (owner[y] != p) rest.push_back(y);
+    std::sort(rest.begin(), rest.end());
+    col.insert(col.end(), own.begin(), own.end());
+    col.insert(col.end(), rest.begin(), rest.end());
+    nown.push_back((uint8_t)own.size());
+    ptr.push_back((int32_t)col.size());
+    smax = std::max(smax, (int)S.size());
+  }
+  tabl = H.tab;
+  const int np = (int)nown.size();
+  for (int p = 0; p < np; ++p) {
+    for (int k = ptr[p]; k < ptr[p + 1]; ++k) sidx[col[k]] = k - ptr[p];
+    for (int k = ptr[p]; k < ptr[p] + nown[p]; ++k) {
+      const int c = col[k];
+      for (int t = H.tptr[c]; t < H.tptr[c + 1]; ++t) {
+        const int cj = (H.tab[t].x + 1) / L;
+        tabl[t].x = H.tab[t].x - cj * L + sidx[cj] * L;
+      }
+      for (int t = H.xoff + H.xptr[c]; t < H.xoff + H.xptr[c + 1]; ++t) {
+        const int cj = H.tab[t].x / L;
+        tabl[t].x = sidx[cj] * L + (H.tab[t].x - cj * L);
+      }
+    }
+    for (int k = ptr[p]; k < ptr[p + 1]; ++k) sidx[col[k]] = -1;
+  }
+  return smax;
+}
+
 // Device geometry of one mesh: the node-sorted SELL-64 incidence (each node's slots in the
 // global edge order: interior edges, then boundary-face edges), its columnar form when the
 // mesh is extruded with identical interior levels, the per-node boundary-point tables and
@@ -1970,6 +2074,11 @@ struct Geometry {
   int want_own = 0;
   Incidence own{};
   double own_col_bytes = 0;
+  // column patches (want_patches, columns of NFT_WAVE levels): see build_patches
+  int want_patches = 0;
+  Incidence inc_lds{};
+  MpPatches patches{};
+  int patch_smax = 0;
   std::vector<int32_t> i1, i2;
   std::vector<double> sx, sy, sz;
 };
@@ -2025,6 +2134,25 @@ int build_geometry(nft_ctx *c, const nft_mesh *m, Geometry &G) {
   }
   g.bp_ptr = d_bp_ptr; g.bp_flag = d_bflag; g.bp_sbb = d_bsbb; g.bp_anp = d_banp; g.nbp = nbp;
   G.gather_grid = g.xcd_per > 0 ? 8 * g.xcd_per : g.nblocks;
+  if (G.want_patches && H.colL == NFT_WAVE) {
+    std::vector<int32_t> pptr, pcol;
+    std::vector<uint8_t> pown;
+    std::vector<int2> tabl;
+    const int smax = build_patches(m->ncol, NFT_WAVE, H, pptr, pcol, pown, tabl);
+    if (smax > 0) {
+      int32_t *d_pptr, *d_pcol;
+      uint8_t *d_pown;
+      int2 *d_tabl;
+      if ((rc = upload(c, &d_pptr, pptr.data(), pptr.size())) || (rc = upload(c, &d_pcol, pcol.data(), pcol.size())) ||
+          (rc = upload(c, &d_pown, pown.data(), pown.size())) || (rc = upload(c, &d_tabl, tabl.data(), tabl.size())))
+        return rc;
+      const int np = (int)pown.size();
+      G.patches = MpPatches{np, np >= 64 ? (np + 7) / 8 : 0, smax * NFT_WAVE, d_pptr, d_pcol, d_pown};
+      G.inc_lds = g;
+      G.inc_lds.tab = d_tabl;
+      G.patch_smax = smax;
+    }
+  }
   if (G.want_own) {
     // each edge once, in the role of its first node: MPDATA's antidiffusive-flux sweep
     HostSlots Ho;
@@ -2046,6 +2174,7 @@ int build_geometry(nft_ctx *c, const nft_mesh *m, Geometry &G) {
     g.sxy = d_sxy;
     g.szz = dsz;
   }
+  if (G.patch_smax > 0) { G.inc_lds.sxy = g.sxy; G.inc_lds.szz = g.szz; }
   return 0;
 }
 
@@ -2095,6 +2224,35 @@ int periodic_sets(const nft_periodic *pp, int n, std::vector<int4> &sets) {
       sets.push_back(make_int4(mem[0], mem[1], mem[2], mem[3]));
     }
   if (sets.empty()) return set_err(-1, "nft_init_periodic: no coincident sets");
+  return 0;
+}
+
+// the patch sweeps' launch attribute beyond the 64 KB default LDS
+template <typename T, int NF>
+int patch_lds_attr(const nft_ctx *c, int optin) {
+  const size_t bf = patch_lds_bytes<T, NF>(c, false), bl = patch_lds_bytes<T, NF>(c, true);
+  if ((long)bl > optin) return -1;
+  if (hipFuncSetAttribute((const void *)k_mp_fct_lds<T, NF>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bf) !=
+          hipSuccess ||
+      hipFuncSetAttribute((const void *)k_mp_limit_lds<T, NF>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bl) !=
+          hipSuccess)
+    return -1;
+  return 0;
+}
+
+// 0 when every instance of the patch sweeps can take its LDS on this device
+int patch_lds_setup(const nft_ctx *c) {
+  int dev = 0, optin = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || optin <= 0) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  if (patch_lds_attr<double, 4>(c, optin) || patch_lds_attr<float, 4>(c, optin) || patch_lds_attr<double, 1>(c, optin) ||
+      patch_lds_attr<float, 1>(c, optin)) {
+    (void)hipGetLastError();
+    return -1;
+  }
   return 0;
 }
 
@@ -2151,9 +2309,12 @@ int init_impl(const nft_mesh *m, const nft_params *P, const nft_periodic *pp, nf
   // ---- geometry: incidence, columnar form, boundary tables, edge metrics
   Geometry geo;
   geo.want_own = 1;
+  geo.want_patches = pp ? 0 : 1;
   if ((rc = build_geometry(c, m, geo))) return fail(rc);
   c->inc = geo.inc; c->S = geo.S; c->gather_grid = geo.gather_grid;
   c->inc_own = geo.own; c->own_col_bytes = geo.own_col_bytes;
+  c->inc_lds = geo.inc_lds; c->patches = geo.patches; c->patch_smax = geo.patch_smax;
+  if (c->patch_smax > 0) c->patch_on = patch_lds_setup(c) == 0 ? 1 : 0;
   if (pp) {
     // node -> slot of the set sums (16 values per node: the most a sweep keeps, four
     // fields' MPDATA bounds and in/out sums)

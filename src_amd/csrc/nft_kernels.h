@@ -1899,15 +1899,13 @@ __device__ __forceinline__ void mp_load_flux(const MpArgs<T, NF> &a, int e, T (&
 }
 
 // FCT bounds from the donor-cell solution, in/out fluxes and ratios (reference :3228-3280)
-template <typename T, int NF, bool COL>
-__global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, MpArgs<T, NF> a) {
+// Per-node body of the bounds sweep; rd(j, f) reads the donor-cell value of field f at
+// neighbour j (from memory, or from the patch's LDS copy, where j is the LDS index)
+template <typename T, int NF, bool COL, typename RD>
+__device__ __forceinline__ void mp_fct_node(const Incidence &g, const Lane &L, const ColLane &C, const MpArgs<T, NF> &a,
+                                            RD rd) {
   constexpr int B = NFT_BATCH_MPL;
   typedef typename MpArgs<T, NF>::T2 T2;
-  Lane L;
-  if (!lane_setup(g, L)) return;
-  ColLane C{};
-  if (COL) col_setup(g, L.valid ? L.i : g.n - 1, C);
-  if (!L.valid) return;
   const int i = L.i;
   T mx[NF], mn[NF], rin[NF], rout[NF], pi[NF];
 #pragma unroll
@@ -1931,7 +1929,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, MpArgs<T, NF>
     for (int u = 0; u < B; ++u) {
       mp_load_flux<T, NF>(a, ee[u] >= 0 ? ee[u] : ~ee[u], Af[u]);
 #pragma unroll
-      for (int f = 0; f < NF; ++f) pj[u][f] = (T)a.psi1[f][jj[u]];
+      for (int f = 0; f < NF; ++f) pj[u][f] = rd(jj[u], f);
     }
 #pragma unroll
     for (int u = 0; u < B; ++u) {
@@ -1972,16 +1970,25 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, MpArgs<T, NF>
   }
 }
 
-// limited antidiffusive fluxes and the second update (reference :3284-3302, :3098-3174)
+
 template <typename T, int NF, bool COL>
-__global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, MpArgs<T, NF> a, double *const *out) {
-  constexpr int B = NFT_BATCH_MPL;
-  typedef typename MpArgs<T, NF>::T2 T2;
+__global__ void __launch_bounds__(NFT_BLOCK) k_mp_fct(Incidence g, MpArgs<T, NF> a) {
   Lane L;
   if (!lane_setup(g, L)) return;
   ColLane C{};
   if (COL) col_setup(g, L.valid ? L.i : g.n - 1, C);
   if (!L.valid) return;
+  mp_fct_node<T, NF, COL>(g, L, C, a, [&](int j, int f) { return (T)a.psi1[f][j]; });
+}
+
+// limited antidiffusive fluxes and the second update (reference :3284-3302, :3098-3174)
+// Per-node body of the limiter sweep; rd(j, f) reads the (cp, cn) pair of field f at
+// neighbour j (from memory, or from the patch's LDS copy)
+template <typename T, int NF, bool COL, typename RD>
+__device__ __forceinline__ void mp_limit_node(const Incidence &g, const Lane &L, const ColLane &C,
+                                              const MpArgs<T, NF> &a, double *const *out, RD rd) {
+  constexpr int B = NFT_BATCH_MPL;
+  typedef typename MpArgs<T, NF>::T2 T2;
   const int i = L.i;
   T arh2[NF], cpi[NF], cni[NF];
 #pragma unroll
@@ -2007,7 +2014,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, MpArgs<T, N
       mp_load_flux<T, NF>(a, ee[u] >= 0 ? ee[u] : ~ee[u], Af[u]);
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        const T2 cj = a.cpn[f][jj[u]];
+        const T2 cj = rd(jj[u], f);
         cpj[u][f] = cj.x; cnj[u][f] = cj.y;
       }
     }
@@ -2043,6 +2050,138 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_limit(Incidence g, MpArgs<T, N
     T inc = arh2[f] * a.dt / vl / rh;
     out[f][i] = a.psi1[f][i] - (double)inc;
   }
+}
+
+template <typename T, int NF, bool COL>
+__global__ void __launch_bounds__(NFT_BLOCK, 7) k_mp_limit(Incidence g, MpArgs<T, NF> a, double *const *out) {
+  Lane L;
+  if (!lane_setup(g, L)) return;
+  ColLane C{};
+  if (COL) col_setup(g, L.valid ? L.i : g.n - 1, C);
+  if (!L.valid) return;
+  mp_limit_node<T, NF, COL>(g, L, C, a, out, [&](int j, int f) { return a.cpn[f][j]; });
+}
+
+// ------------------------------------------------------------------------------------
+// LDS-staged bounds and limiter sweeps over column patches (columns of NFT_WAVE levels)
+// ------------------------------------------------------------------------------------
+// A patch is up to NFT_PATCH_COLS columns, one wavefront each, in one workgroup. Its
+// columns and every column their slots reach (col[ptr[p]] .. col[ptr[p+1]-1], the patch's
+// own nown[p] columns first) are copied to LDS with coalesced loads, and the slot loops read
+// neighbour values from that copy: the incidence passed in has tab entries rewritten to
+// LDS-relative neighbours (staged index * NFT_WAVE + level, decoded exactly as the node
+// ids), so each node's slot order and arithmetic are unchanged.
+#ifndef NFT_PATCH_COLS
+#define NFT_PATCH_COLS 8
+#endif
+#ifndef NFT_PATCH_STAGE
+#define NFT_PATCH_STAGE 20   // staged columns per patch at most (80 KB of LDS for four FP64 (cp, cn) pairs: two patches per CU)
+#endif
+// staged nodes per thread at most
+#define NFT_PATCH_SU ((NFT_PATCH_STAGE * NFT_WAVE + NFT_WAVE * NFT_PATCH_COLS - 1) / (NFT_WAVE * NFT_PATCH_COLS))
+
+struct MpPatches {
+  int npatch;
+  int xcd_per;               // patches per XCD slice (block remap), 0 = identity
+  int stride;                // LDS values per field: the largest staged set * NFT_WAVE
+  const int32_t *ptr;        // [npatch+1] into col
+  const int32_t *col;        // staged columns
+  const uint8_t *nown;       // [npatch] columns owned by the patch
+};
+
+__device__ __forceinline__ int patch_of_block(const MpPatches &P) {
+  const int b = blockIdx.x;
+  return P.xcd_per > 0 ? (b & 7) * P.xcd_per + (b >> 3) : b;
+}
+
+// the lane of node (column c, level = lane) in a whole-column wavefront
+__device__ __forceinline__ void patch_lane(const Incidence &g, int c, Lane &L) {
+  L.lane = threadIdx.x & 63;
+  L.i = c * NFT_WAVE + L.lane;
+  L.base = __builtin_amdgcn_readfirstlane(g.chunk_ptr[c]);
+  L.width = __builtin_amdgcn_readfirstlane((g.chunk_ptr[c + 1] - L.base) >> 6);
+  L.valid = true;
+  L.deg = g.deg[L.i];
+}
+
+template <typename T, int NF>
+__global__ void __launch_bounds__(NFT_WAVE *NFT_PATCH_COLS) k_mp_fct_lds(Incidence g, MpPatches P, MpArgs<T, NF> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *sp = (T *)lds_raw;   // [field][staged column * NFT_WAVE + level]: lanes read consecutive words
+  const int p = patch_of_block(P);
+  if (p >= P.npatch) return;
+  const int s0 = P.ptr[p], ns = P.ptr[p + 1] - s0;
+  {
+    // every staging load in flight before the first LDS store (clamped, always valid)
+    T v[NFT_PATCH_SU][NF];
+#pragma unroll
+    for (int r = 0; r < NFT_PATCH_SU; ++r) {
+      const int q = min((int)threadIdx.x + r * NFT_WAVE * NFT_PATCH_COLS, ns * NFT_WAVE - 1);
+      const int node = P.col[s0 + (q >> 6)] * NFT_WAVE + (q & (NFT_WAVE - 1));
+#pragma unroll
+      for (int f = 0; f < NF; ++f) v[r][f] = (T)a.psi1[f][node];
+    }
+#pragma unroll
+    for (int r = 0; r < NFT_PATCH_SU; ++r) {
+      const int q = threadIdx.x + r * NFT_WAVE * NFT_PATCH_COLS;
+      if (q < ns * NFT_WAVE) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) sp[f * P.stride + q] = v[r][f];
+      }
+    }
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6;
+  if (w >= P.nown[p]) return;
+  Lane L;
+  patch_lane(g, P.col[s0 + w], L);
+  ColLane C{};
+  col_setup(g, L.i, C);
+  mp_fct_node<T, NF, true>(g, L, C, a, [&](int j, int f) { return sp[f * P.stride + j]; });
+}
+
+template <typename T, int NF>
+__global__ void __launch_bounds__(NFT_WAVE *NFT_PATCH_COLS) k_mp_limit_lds(Incidence g, MpPatches P, MpArgs<T, NF> a,
+                                                                         double *const *out) {
+  typedef typename MpArgs<T, NF>::T2 T2;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T2 *sc = (T2 *)lds_raw;   // [field][staged column * NFT_WAVE + level]
+  const int p = patch_of_block(P);
+  if (p >= P.npatch) return;
+  const int s0 = P.ptr[p], ns = P.ptr[p + 1] - s0;
+  {
+    T vx[NFT_PATCH_SU][NF], vy[NFT_PATCH_SU][NF];   // (cp, cn) as scalars: kept in registers
+#pragma unroll
+    for (int r = 0; r < NFT_PATCH_SU; ++r) {
+      const int q = min((int)threadIdx.x + r * NFT_WAVE * NFT_PATCH_COLS, ns * NFT_WAVE - 1);
+      const int node = P.col[s0 + (q >> 6)] * NFT_WAVE + (q & (NFT_WAVE - 1));
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const T2 v = a.cpn[f][node];
+        vx[r][f] = v.x; vy[r][f] = v.y;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NFT_PATCH_SU; ++r) {
+      const int q = threadIdx.x + r * NFT_WAVE * NFT_PATCH_COLS;
+      if (q < ns * NFT_WAVE) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          T2 v;
+          v.x = vx[r][f]; v.y = vy[r][f];
+          sc[f * P.stride + q] = v;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6;
+  if (w >= P.nown[p]) return;
+  Lane L;
+  patch_lane(g, P.col[s0 + w], L);
+  ColLane C{};
+  col_setup(g, L.i, C);
+  mp_limit_node<T, NF, true>(g, L, C, a, out, [&](int j, int f) { return sc[f * P.stride + j]; });
 }
 
 // ------------------------------------------------------------------------------------

@@ -111,6 +111,41 @@ def test_mpdatam_bitwise(pair):
         assert np.array_equal(a, b), iflag
 
 
+@pytest.mark.parametrize("key", ["wave64", "patches"])
+def test_mpdatam_column_patches(key):
+    """Columns of 64 levels in a closed box: the bounds and limiter sweeps stage column patches
+    (up to 16 columns, with every column their slots reach) in LDS and gather neighbour
+    values from there. Single-field mpdatam (FP64 and the FP32 form) and two full steps (the
+    four fields of a step in one sweep) equal the oracle bit for bit (63 columns in 5
+    patches; 1,024 columns in 69, dealt to the XCDs in ranges)."""
+    print(f"case {key}", flush=True)
+    if key == "wave64":
+        m, cfg = mesh_for("wave64"), tiny_cfg(**BAND)
+    else:
+        spec = M.MeshSpec(nx=32, ny=32, nlev=64, lx=25000.0, ly=25000.0, htop=20000.0, hill_h0=1000.0,
+                          hill_a=2000.0, seed=4)
+        m = M.generate(spec)
+        cfg = M.RunConfig("patches", spec, u0=10.0, bvf=0.01, dt=20.0, steps=2, lord=4, eps0=1e-8, exit_mode=1,
+                          itr=200, precon=3, itrp1=1, omega=1.0, sponge_depth=3000.0, sponge_tau=300.0)
+    f = seeded_fields(m)
+    o = Oracle(m, cfg, dot_order=1)
+    s = solver(m, cfg)
+    for iflag, name in ((1, "u"), (2, "v"), (3, "w"), (4, "p")):
+        assert np.array_equal(s.mpdatam(f["uuu"], f[name], iflag), o.mpdatam(f["uuu"], f[name], iflag)), iflag
+    s32 = solver(m, cfg, precision="mixed")
+    for iflag, name in ((1, "u"), (4, "p")):
+        assert np.array_equal(s32.mpdatam(f["uuu"], f[name], iflag), o.mpdatam32(f["uuu"], f[name], iflag)), iflag
+    s32.close()
+    s.state_init()
+    o.init()
+    s.run_steps(2)
+    for _ in range(2):
+        o.step()
+    for name in ("unk", "frc", "uuu", "prpot"):
+        assert np.array_equal(s.state(name), o.state(name)), name
+    s.close()
+
+
 def test_mpdatam_against_reference_vectors():
     g = np.load(os.path.join(GOLD, "tiny_ops.npz"), allow_pickle=False)
     m = mesh_for("tiny")
