@@ -533,18 +533,14 @@ DivArgs div_args(nft_ctx *c) {
   return a;
 }
 
-// the divergence sweeps follow a gradient sweep over the same edges: they walk the blocks
-// in the reverse order (Incidence::rev)
 template <int MODE, int ND, int NMAX, typename TM = double>
 void div_go(nft_ctx *c, const DivArgs &a) {
-  Incidence g = c->inc;
-  g.rev = NFT_DIV_REV;
   if (c->col_on)
-    hipLaunchKernelGGL((k_div<MODE, ND, NMAX, true, TM>), dim3(c->gather_grid), dim3(NFT_BLOCK), 0, c->stream, g,
-                       c->S, a);
+    hipLaunchKernelGGL((k_div<MODE, ND, NMAX, true, TM>), dim3(c->gather_grid), dim3(NFT_BLOCK), 0, c->stream,
+                       c->inc, c->S, a);
   else
-    hipLaunchKernelGGL((k_div<MODE, ND, NMAX, false, TM>), dim3(c->gather_grid), dim3(NFT_BLOCK), 0, c->stream, g,
-                       c->S, a);
+    hipLaunchKernelGGL((k_div<MODE, ND, NMAX, false, TM>), dim3(c->gather_grid), dim3(NFT_BLOCK), 0, c->stream,
+                       c->inc, c->S, a);
 }
 
 template <int ND, typename TM>

@@ -59,7 +59,6 @@ struct Incidence {
   int nbp;
   int xcd_per;               // logical blocks per XCD slice (remap), 0 = identity
   int nblocks;               // logical blocks
-  int rev;                   // 1: each XCD walks its slice from the end (see logical_block)
   // Columnar form of the same slots (extruded meshes, node i = c*col_L + k): the interior
   // levels 0 < k < col_L-1 of column c share one slot template t in [tptr[c], tptr[c+1]):
   // neighbour tab[t].x + k, edge (tab[t].y >> 1) - 1 + k, in the i2 role when tab[t].y & 1;
@@ -123,22 +122,11 @@ struct EdgeS {
 };
 
 // Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous range of chunks
-// so neighbouring columns share that XCD's L2. With rev set each XCD walks its range from
-// the end, so a sweep launched after one in the forward order starts on the columns whose
-// data the previous sweep touched last (still in the last-level cache). Which nodes form a
-// logical block, and so every block sum, does not depend on the order.
+// so neighbouring columns share that XCD's L2.
 __device__ __forceinline__ int logical_block(const Incidence &g) {
   int b = blockIdx.x;
-  if (g.xcd_per <= 0) return g.rev ? g.nblocks - 1 - b : b;
-  const int k = b >> 3;
-  return (b & 7) * g.xcd_per + (g.rev ? g.xcd_per - 1 - k : k);
-}
-
-// The block index that holds logical block lb in the forward order: block partials are
-// stored there, so a reverse walk leaves every partial, and every grid sum, unchanged.
-__device__ __forceinline__ int forward_slot(const Incidence &g, int lb) {
-  if (g.xcd_per <= 0) return lb;
-  return ((lb % g.xcd_per) << 3) | (lb / g.xcd_per);
+  if (g.xcd_per <= 0) return b;
+  return (b & 7) * g.xcd_per + (b >> 3);
 }
 
 struct Lane {
@@ -235,17 +223,14 @@ __device__ __forceinline__ double ld_agent(const double *p) {
 }
 
 // Writes this block's K partial results (block tree, result of thread 0) to
-// partials[slot*K + k], slot = blockIdx.x unless given. The grid-level sum is
-// k_reduce_final, launched next on the same stream: the kernel boundary makes every partial
-// visible without a fence per block.
+// partials[blockIdx.x*K + k]. The grid-level sum is k_reduce_final, launched next on the
+// same stream: the kernel boundary makes every partial visible without a fence per block.
 template <int K>
-__device__ __forceinline__ void block_partial(double (&v)[K], unsigned ismax, double *partials, double *sh,
-                                              int slot = -1) {
+__device__ __forceinline__ void block_partial(double (&v)[K], unsigned ismax, double *partials, double *sh) {
   block_reduce<K>(v, ismax, sh);
   if (threadIdx.x == 0) {
-    const size_t b = slot >= 0 ? (size_t)slot : (size_t)blockIdx.x;
 #pragma unroll
-    for (int k = 0; k < K; ++k) partials[b * K + k] = v[k];
+    for (int k = 0; k < K; ++k) partials[(size_t)blockIdx.x * K + k] = v[k];
   }
 }
 
@@ -355,9 +340,6 @@ __device__ __forceinline__ bool skipped(const double *flag) { return flag && *fl
 // Slots are consumed NFT_BATCH_G (gradient) or NFT_BATCH_D (divergence) at a time: the batch's indices, then all its gathers, are
 // issued before any use, so a wavefront keeps several independent loads in flight; the
 // sums still run slot by slot in the reference's edge order.
-#ifndef NFT_DIV_REV
-#define NFT_DIV_REV 1   // divergence sweeps walk the blocks in reverse (Incidence::rev)
-#endif
 #ifndef NFT_BATCH_G
 #define NFT_BATCH_G 2   // gradient sweeps (2 measured best on gfx950, 1..8 tried)
 #endif
@@ -665,7 +647,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_div(Incidence g, EdgeS<double> S,
       for (int k = 0; k < ND; ++k) v[k] = (a.dots.a[k] ? a.dots.a[k][L.i] : out) * out;
       if (NMAX) v[ND] = fabs(out);
     }
-    block_partial<K>(v, NMAX ? (1u << ND) : 0u, a.partials, sh, forward_slot(g, logical_block(g)));
+    block_partial<K>(v, NMAX ? (1u << ND) : 0u, a.partials, sh);
   }
 }
 
