@@ -1376,6 +1376,11 @@ template <typename T, int NF>
 size_t patch_lds_bytes(const nft_ctx *c, bool limit) {
   return (size_t)c->patch_smax * NFT_WAVE * NF * sizeof(T) * (limit ? 2 : 1);
 }
+// the donor-cell sweep's: advector and fields
+template <typename T, int NF>
+size_t patch_lds_bytes_donor(const nft_ctx *c) {
+  return (size_t)c->patch_smax * NFT_WAVE * (3 + NF) * sizeof(T);
+}
 
 template <typename T, int NF>
 int mpdata_dev(nft_ctx *c, const T *U, const T *V, const T *W, double *const *fields, const int *iflags) {
@@ -1408,8 +1413,15 @@ int mpdata_dev(nft_ctx *c, const T *U, const T *V, const T *W, double *const *fi
   if constexpr (sizeof(T) == 8) {
     if (per) hipLaunchKernelGGL(k_per_mp_donor<NF>, gs, b, 0, c->stream, c->inc, c->ps, a);
   }
+  // columns of 64 levels in a closed box: the gathering sweeps over LDS-staged column patches
+  const bool lds = c->patch_on && c->col_on && !per;
+  const dim3 gp(c->patches.xcd_per > 0 ? 8 * c->patches.xcd_per : c->patches.npatch), bp(NFT_WAVE * NFT_PATCH_COLS);
+  const size_t shd = patch_lds_bytes_donor<T, NF>(c);
+  const size_t shf = patch_lds_bytes<T, NF>(c, false), shl = patch_lds_bytes<T, NF>(c, true);
   t_begin(c, KT_MP0);
-  if (c->col_on) hipLaunchKernelGGL((k_mp_donor<T, NF, true>), g, b, 0, c->stream, c->inc, S, a);
+  if (lds && NFT_MP_DONOR_LDS)
+    hipLaunchKernelGGL((k_mp_donor_lds<T, NF>), gp, bp, shd, c->stream, c->inc_lds, S, c->patches, a);
+  else if (c->col_on) hipLaunchKernelGGL((k_mp_donor<T, NF, true>), g, b, 0, c->stream, c->inc, S, a);
   else hipLaunchKernelGGL((k_mp_donor<T, NF, false>), g, b, 0, c->stream, c->inc, S, a);
   t_end(c, KT_MP0);
   // pass 2 reads the donor-cell fields
@@ -1417,7 +1429,9 @@ int mpdata_dev(nft_ctx *c, const T *U, const T *V, const T *W, double *const *fi
     if (per) hipLaunchKernelGGL(k_per_mp_grad<NF>, gs, b, 0, c->stream, c->inc, c->ps, a);
   }
   t_begin(c, KT_MP0 + 1);
-  if (c->col_on) hipLaunchKernelGGL((k_mp_grad<T, NF, true>), g, b, 0, c->stream, c->inc, S, a);
+  if (lds && NFT_MP_GRAD_LDS)
+    hipLaunchKernelGGL((k_mp_grad_lds<T, NF>), gp, bp, shf, c->stream, c->inc_lds, S, c->patches, a);
+  else if (c->col_on) hipLaunchKernelGGL((k_mp_grad<T, NF, true>), g, b, 0, c->stream, c->inc, S, a);
   else hipLaunchKernelGGL((k_mp_grad<T, NF, false>), g, b, 0, c->stream, c->inc, S, a);
   t_end(c, KT_MP0 + 1);
   if constexpr (sizeof(T) == 8) {
@@ -1430,10 +1444,6 @@ int mpdata_dev(nft_ctx *c, const T *U, const T *V, const T *W, double *const *fi
   if (c->col_on && go.col_L > 0) hipLaunchKernelGGL((k_mp_flux<T, NF, true>), g, b, 0, c->stream, go, S, a);
   else hipLaunchKernelGGL((k_mp_flux<T, NF, false>), g, b, 0, c->stream, go, S, a);
   t_end(c, KT_MPF);
-  // columns of 64 levels in a closed box: bounds and limiter over LDS-staged column patches
-  const bool lds = c->patch_on && c->col_on && !per;
-  const dim3 gp(c->patches.xcd_per > 0 ? 8 * c->patches.xcd_per : c->patches.npatch), bp(NFT_WAVE * NFT_PATCH_COLS);
-  const size_t shf = patch_lds_bytes<T, NF>(c, false), shl = patch_lds_bytes<T, NF>(c, true);
   t_begin(c, KT_MP0 + 2);
   if (lds)
     hipLaunchKernelGGL((k_mp_fct_lds<T, NF>), gp, bp, shf, c->stream, c->inc_lds,
@@ -2231,8 +2241,13 @@ int periodic_sets(const nft_periodic *pp, int n, std::vector<int4> &sets) {
 template <typename T, int NF>
 int patch_lds_attr(const nft_ctx *c, int optin) {
   const size_t bf = patch_lds_bytes<T, NF>(c, false), bl = patch_lds_bytes<T, NF>(c, true);
-  if ((long)bl > optin) return -1;
-  if (hipFuncSetAttribute((const void *)k_mp_fct_lds<T, NF>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bf) !=
+  const size_t bd = patch_lds_bytes_donor<T, NF>(c);
+  if ((long)bl > optin || (long)bd > optin) return -1;
+  if (hipFuncSetAttribute((const void *)k_mp_donor_lds<T, NF>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bd) !=
+          hipSuccess ||
+      hipFuncSetAttribute((const void *)k_mp_grad_lds<T, NF>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bf) !=
+          hipSuccess ||
+      hipFuncSetAttribute((const void *)k_mp_fct_lds<T, NF>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bf) !=
           hipSuccess ||
       hipFuncSetAttribute((const void *)k_mp_limit_lds<T, NF>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bl) !=
           hipSuccess)

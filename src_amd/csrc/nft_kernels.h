@@ -340,6 +340,12 @@ __device__ __forceinline__ bool skipped(const double *flag) { return flag && *fl
 // Slots are consumed NFT_BATCH_G (gradient) or NFT_BATCH_D (divergence) at a time: the batch's indices, then all its gathers, are
 // issued before any use, so a wavefront keeps several independent loads in flight; the
 // sums still run slot by slot in the reference's edge order.
+#ifndef NFT_MP_GRAD_LDS
+#define NFT_MP_GRAD_LDS 1   // MPDATA's gradient sweep over LDS-staged column patches too
+#endif
+#ifndef NFT_MP_DONOR_LDS
+#define NFT_MP_DONOR_LDS 1   // MPDATA's donor-cell sweep over LDS-staged column patches too
+#endif
 #ifndef NFT_BATCH_G
 #define NFT_BATCH_G 2   // gradient sweeps (2 measured best on gfx950, 1..8 tried)
 #endif
@@ -1661,14 +1667,12 @@ __device__ __forceinline__ void mp_batch_slots(const Incidence &g, const EdgeS<T
 }
 
 // pass 1: donor-cell fluxes, boundary closure and update (reference :3094-3174)
-template <typename T, int NF, bool COL>
-__global__ void __launch_bounds__(NFT_BLOCK) k_mp_donor(Incidence g, EdgeS<T> S, MpArgs<T, NF> a) {
+// Per-node body of the donor-cell sweep; rd(j, U, V, W, psi) reads the advector and the
+// fields at neighbour j (from memory, or from the patch's LDS copy)
+template <typename T, int NF, bool COL, typename RD>
+__device__ __forceinline__ void mp_donor_node(const Incidence &g, const EdgeS<T> &S, const Lane &L, const ColLane &C,
+                                              const MpArgs<T, NF> &a, RD rd) {
   constexpr int B = NFT_BATCH_MP;
-  Lane L;
-  if (!lane_setup(g, L)) return;
-  ColLane C{};
-  if (COL) col_setup(g, L.valid ? L.i : g.n - 1, C);
-  if (!L.valid) return;
   const int i = L.i;
   T Ui = a.U[i], Vi = a.V[i], Wi = a.W[i];
   T pi[NF], arh2[NF];
@@ -1689,10 +1693,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_donor(Incidence g, EdgeS<T> S,
     T Uj[B], Vj[B], Wj[B], pj[B][NF];
 #pragma unroll
     for (int u = 0; u < B; ++u) {
-      const int j = jj[u];
-      Uj[u] = a.U[j]; Vj[u] = a.V[j]; Wj[u] = a.W[j];
-#pragma unroll
-      for (int f = 0; f < NF; ++f) pj[u][f] = (T)a.psi[f][j];
+      rd(jj[u], Uj[u], Vj[u], Wj[u], pj[u]);
     }
 #pragma unroll
     for (int u = 0; u < B; ++u) {
@@ -1735,15 +1736,27 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_donor(Incidence g, EdgeS<T> S,
   }
 }
 
-// gradients of the donor-cell fields (derivative x3 per field, reference :3182-3184)
 template <typename T, int NF, bool COL>
-__global__ void __launch_bounds__(NFT_BLOCK) k_mp_grad(Incidence g, EdgeS<T> S, MpArgs<T, NF> a) {
-  constexpr int B = NFT_BATCH_MP;
+__global__ void __launch_bounds__(NFT_BLOCK) k_mp_donor(Incidence g, EdgeS<T> S, MpArgs<T, NF> a) {
   Lane L;
   if (!lane_setup(g, L)) return;
   ColLane C{};
   if (COL) col_setup(g, L.valid ? L.i : g.n - 1, C);
   if (!L.valid) return;
+  mp_donor_node<T, NF, COL>(g, S, L, C, a, [&](int j, T &U, T &V, T &W, T (&p)[NF]) {
+    U = a.U[j]; V = a.V[j]; W = a.W[j];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) p[f] = (T)a.psi[f][j];
+  });
+}
+
+// gradients of the donor-cell fields (derivative x3 per field, reference :3182-3184)
+// Per-node body of the gradient sweep of the donor-cell fields; rd(j, f) reads field f at
+// neighbour j (from memory, or from the patch's LDS copy)
+template <typename T, int NF, bool COL, typename RD>
+__device__ __forceinline__ void mp_grad_node(const Incidence &g, const EdgeS<T> &S, const Lane &L, const ColLane &C,
+                                             const MpArgs<T, NF> &a, RD rd) {
+  constexpr int B = NFT_BATCH_MP;
   const int i = L.i;
   T pi[NF], gx[NF], gy[NF], gz[NF];
 #pragma unroll
@@ -1763,7 +1776,7 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_grad(Incidence g, EdgeS<T> S, 
 #pragma unroll
     for (int u = 0; u < B; ++u) {
 #pragma unroll
-      for (int f = 0; f < NF; ++f) pj[u][f] = (T)a.psi1[f][jj[u]];
+      for (int f = 0; f < NF; ++f) pj[u][f] = rd(jj[u], f);
     }
 #pragma unroll
     for (int u = 0; u < B; ++u) {
@@ -1802,6 +1815,16 @@ __global__ void __launch_bounds__(NFT_BLOCK) k_mp_grad(Incidence g, EdgeS<T> S, 
     a.gxy[f][i] = v;
     a.gz[f][i] = gz[f];
   }
+}
+
+template <typename T, int NF, bool COL>
+__global__ void __launch_bounds__(NFT_BLOCK) k_mp_grad(Incidence g, EdgeS<T> S, MpArgs<T, NF> a) {
+  Lane L;
+  if (!lane_setup(g, L)) return;
+  ColLane C{};
+  if (COL) col_setup(g, L.valid ? L.i : g.n - 1, C);
+  if (!L.valid) return;
+  mp_grad_node<T, NF, COL>(g, S, L, C, a, [&](int j, int f) { return (T)a.psi1[f][j]; });
 }
 
 // antidiffusive velocity of one edge for one field (reference :3187-3204) from the two end
@@ -2182,6 +2205,85 @@ __global__ void __launch_bounds__(NFT_WAVE *NFT_PATCH_COLS) k_mp_limit_lds(Incid
   ColLane C{};
   col_setup(g, L.i, C);
   mp_limit_node<T, NF, true>(g, L, C, a, out, [&](int j, int f) { return sc[f * P.stride + j]; });
+}
+
+// the gradient sweep of the donor-cell fields over the same patches
+template <typename T, int NF>
+__global__ void __launch_bounds__(NFT_WAVE *NFT_PATCH_COLS) k_mp_grad_lds(Incidence g, EdgeS<T> S, MpPatches P,
+                                                                        MpArgs<T, NF> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *sp = (T *)lds_raw;   // [field][staged column * NFT_WAVE + level]
+  const int p = patch_of_block(P);
+  if (p >= P.npatch) return;
+  const int s0 = P.ptr[p], ns = P.ptr[p + 1] - s0;
+  {
+    T v[NFT_PATCH_SU][NF];
+#pragma unroll
+    for (int r = 0; r < NFT_PATCH_SU; ++r) {
+      const int q = min((int)threadIdx.x + r * NFT_WAVE * NFT_PATCH_COLS, ns * NFT_WAVE - 1);
+      const int node = P.col[s0 + (q >> 6)] * NFT_WAVE + (q & (NFT_WAVE - 1));
+#pragma unroll
+      for (int f = 0; f < NF; ++f) v[r][f] = (T)a.psi1[f][node];
+    }
+#pragma unroll
+    for (int r = 0; r < NFT_PATCH_SU; ++r) {
+      const int q = threadIdx.x + r * NFT_WAVE * NFT_PATCH_COLS;
+      if (q < ns * NFT_WAVE) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) sp[f * P.stride + q] = v[r][f];
+      }
+    }
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6;
+  if (w >= P.nown[p]) return;
+  Lane L;
+  patch_lane(g, P.col[s0 + w], L);
+  ColLane C{};
+  col_setup(g, L.i, C);
+  mp_grad_node<T, NF, true>(g, S, L, C, a, [&](int j, int f) { return sp[f * P.stride + j]; });
+}
+
+// the donor-cell sweep over the same patches: advector and fields staged in LDS
+template <typename T, int NF>
+__global__ void __launch_bounds__(NFT_WAVE *NFT_PATCH_COLS) k_mp_donor_lds(Incidence g, EdgeS<T> S, MpPatches P,
+                                                                         MpArgs<T, NF> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *sv = (T *)lds_raw;   // [U, V, W, field 0 .. NF-1][staged column * NFT_WAVE + level]
+  const int p = patch_of_block(P);
+  if (p >= P.npatch) return;
+  const int s0 = P.ptr[p], ns = P.ptr[p + 1] - s0;
+  {
+    T v[NFT_PATCH_SU][3 + NF];
+#pragma unroll
+    for (int r = 0; r < NFT_PATCH_SU; ++r) {
+      const int q = min((int)threadIdx.x + r * NFT_WAVE * NFT_PATCH_COLS, ns * NFT_WAVE - 1);
+      const int node = P.col[s0 + (q >> 6)] * NFT_WAVE + (q & (NFT_WAVE - 1));
+      v[r][0] = a.U[node]; v[r][1] = a.V[node]; v[r][2] = a.W[node];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) v[r][3 + f] = (T)a.psi[f][node];
+    }
+#pragma unroll
+    for (int r = 0; r < NFT_PATCH_SU; ++r) {
+      const int q = threadIdx.x + r * NFT_WAVE * NFT_PATCH_COLS;
+      if (q < ns * NFT_WAVE) {
+#pragma unroll
+        for (int k = 0; k < 3 + NF; ++k) sv[k * P.stride + q] = v[r][k];
+      }
+    }
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6;
+  if (w >= P.nown[p]) return;
+  Lane L;
+  patch_lane(g, P.col[s0 + w], L);
+  ColLane C{};
+  col_setup(g, L.i, C);
+  mp_donor_node<T, NF, true>(g, S, L, C, a, [&](int j, T &U, T &V, T &W, T (&pj)[NF]) {
+    U = sv[j]; V = sv[P.stride + j]; W = sv[2 * P.stride + j];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) pj[f] = sv[(3 + f) * P.stride + j];
+  });
 }
 
 // ------------------------------------------------------------------------------------
