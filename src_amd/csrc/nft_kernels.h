@@ -2086,7 +2086,8 @@ __global__ void __launch_bounds__(NFT_BLOCK, 7) k_mp_limit(Incidence g, MpArgs<T
 }
 
 // ------------------------------------------------------------------------------------
-// LDS-staged bounds and limiter sweeps over column patches (columns of NFT_WAVE levels)
+// LDS-staged MPDATA sweeps over column patches (columns of NFT_WAVE levels): donor cell,
+// gradients, bounds and limiter
 // ------------------------------------------------------------------------------------
 // A patch is up to NFT_PATCH_COLS columns, one wavefront each, in one workgroup. Its
 // columns and every column their slots reach (col[ptr[p]] .. col[ptr[p+1]-1], the patch's
@@ -2098,7 +2099,7 @@ __global__ void __launch_bounds__(NFT_BLOCK, 7) k_mp_limit(Incidence g, MpArgs<T
 #define NFT_PATCH_COLS 8
 #endif
 #ifndef NFT_PATCH_STAGE
-#define NFT_PATCH_STAGE 20   // staged columns per patch at most (80 KB of LDS for four FP64 (cp, cn) pairs: two patches per CU)
+#define NFT_PATCH_STAGE 20   // staged columns per patch at most (at most 80 KB of LDS at FP64: two patches per CU)
 #endif
 // staged nodes per thread at most
 #define NFT_PATCH_SU ((NFT_PATCH_STAGE * NFT_WAVE + NFT_WAVE * NFT_PATCH_COLS - 1) / (NFT_WAVE * NFT_PATCH_COLS))
